@@ -1,0 +1,315 @@
+"""hyset_amd: exact threshold set-similarity join on MI355X (gfx950).
+
+Python view of libhyset_amd.so (C ABI in include/hyset_amd.h).  Every join
+runs in the HIP library; this module only moves arrays across the boundary.
+Importing it raises if the library has not been built -- there is no
+fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+
+_HERE = Path(__file__).resolve().parent
+LIB_PATH = _HERE / "libhyset_amd.so"
+
+if not LIB_PATH.exists():
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `python -m hyset_amd.build` "
+        "(hipcc for gfx950); hyset_amd has no non-HIP fallback")
+
+_lib = C.CDLL(str(LIB_PATH))
+
+HS_OK, HS_E_INVALID, HS_E_OOM, HS_E_OVERFLOW, HS_E_DEVICE, HS_E_UNSUPPORTED, HS_E_BUSY = 0, -1, -2, -3, -4, -5, -6
+JACCARD, COSINE, DICE = 0, 1, 2
+BOUNDS_REFERENCE, BOUNDS_EXACT = 0, 1
+FGSS, BITMAP, HYBRID = 0, 1, 2
+SIZE_UNIFORM, SIZE_LOGNORMAL = 0, 1
+ALGOS = {"fgss": FGSS, "bitmap": BITMAP, "hybrid": HYBRID}
+SIMS = {"jaccard": JACCARD, "cosine": COSINE, "dice": DICE}
+
+_u32p = C.POINTER(C.c_uint32)
+
+
+class HostCollection(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("n_tokens", C.c_uint64), ("universe", C.c_uint32),
+                ("tokens", _u32p), ("starts", _u32p), ("sizes", _u32p)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("n", C.c_uint32), ("size_dist", C.c_int32), ("size_median", C.c_double),
+                ("size_sigma", C.c_double), ("size_lo", C.c_uint32), ("size_hi", C.c_uint32),
+                ("universe", C.c_uint32), ("zipf_s", C.c_double), ("seed", C.c_uint64)]
+
+
+class Range(C.Structure):
+    _fields_ = [("first", C.c_uint32), ("last", C.c_uint32)]
+
+
+class Tune(C.Structure):
+    _fields_ = [("bitmap_bits", C.c_uint32), ("probes_per_tile", C.c_uint32),
+                ("verify_scenario", C.c_uint32), ("host_threads", C.c_uint32),
+                ("batch_candidates", C.c_uint64), ("flags", C.c_uint32)]
+
+
+class Result(C.Structure):
+    _fields_ = [("count", C.c_uint64), ("capacity", C.c_uint64), ("r", _u32p), ("s", _u32p),
+                ("overlap", _u32p), ("count_only", C.c_int32)]
+
+
+class Stats(C.Structure):
+    _fields_ = [("index_ms", C.c_double), ("filter_ms", C.c_double), ("transfer_ms", C.c_double),
+                ("total_ms", C.c_double), ("host_filter_ms", C.c_double),
+                ("index_entries", C.c_uint64), ("posting_visits", C.c_uint64),
+                ("candidates", C.c_uint64), ("pairs", C.c_uint64), ("probe_prefix_tokens", C.c_uint64)]
+
+
+def _sig(name, restype, *argtypes):
+    f = getattr(_lib, name)
+    f.restype = restype
+    f.argtypes = list(argtypes)
+    return f
+
+
+_vp = C.c_void_p
+_sig("hs_generate_self", C.c_int, C.POINTER(GenParams), C.POINTER(HostCollection))
+_sig("hs_generate_rs", C.c_int, C.POINTER(GenParams), C.POINTER(GenParams), C.c_double, C.c_double,
+     C.c_double, C.POINTER(HostCollection), C.POINTER(HostCollection))
+_sig("hs_generate_queries", C.c_int, C.POINTER(GenParams), C.POINTER(GenParams), C.POINTER(HostCollection),
+     C.POINTER(HostCollection))
+_sig("hs_host_collection_free", None, C.POINTER(HostCollection))
+_sig("hs_read_collection_text", C.c_int, C.c_char_p, C.POINTER(HostCollection))
+_sig("hs_write_collection_text", C.c_int, C.c_char_p, _u32p, _u32p, _u32p, C.c_uint32)
+_sig("hs_write_pairs_text", C.c_int, C.c_char_p, _u32p, _u32p, _u32p, C.c_uint64)
+_sig("hs_create", C.c_int, C.c_int, C.POINTER(_vp))
+_sig("hs_destroy", None, _vp)
+_sig("hs_last_error", C.c_char_p, _vp)
+_sig("hs_upload_collection", C.c_int, _vp, _u32p, _u32p, _u32p, C.c_uint32, C.c_uint64, C.POINTER(_vp))
+_sig("hs_free_collection", None, _vp)
+_sig("hs_set_threshold", C.c_int, _vp, C.c_int32, C.c_double, C.c_int32)
+_sig("hs_join", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.c_int32, C.POINTER(Tune),
+     C.POINTER(Result))
+_sig("hs_verify_candidates", C.c_int, _vp, _vp, _vp, _u32p, _u32p, _u32p, C.c_uint32, C.c_int32,
+     C.POINTER(Result))
+_sig("hs_copy_last_result", C.c_int, _vp, C.POINTER(Result))
+_sig("hs_get_stats", C.c_int, _vp, C.POINTER(Stats))
+
+
+class HysetError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hyset_amd error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_u32p)
+
+
+def _u32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+# ----------------------------------------------------------------- inputs
+
+@dataclass
+class SetCollection:
+    """Host CSR collection: set i = tokens[starts[i]:starts[i]+sizes[i]]."""
+    tokens: np.ndarray
+    starts: np.ndarray
+    sizes: np.ndarray
+    universe: int
+
+    @property
+    def n(self) -> int:
+        return int(self.sizes.shape[0])
+
+    def set(self, i: int) -> np.ndarray:
+        return self.tokens[self.starts[i]:self.starts[i] + self.sizes[i]]
+
+    @staticmethod
+    def from_sets(sets) -> "SetCollection":
+        sizes = np.array([len(s) for s in sets], dtype=np.uint32)
+        starts = np.zeros(len(sets), dtype=np.uint32)
+        if len(sets):
+            starts[1:] = np.cumsum(sizes[:-1], dtype=np.uint64).astype(np.uint32)
+        toks = np.array([t for s in sets for t in s], dtype=np.uint32)
+        uni = int(toks.max()) + 1 if toks.size else 1
+        return SetCollection(toks, starts, sizes, uni)
+
+
+def _take(hc: HostCollection) -> SetCollection:
+    try:
+        n, nt = hc.n, hc.n_tokens
+        toks = np.ctypeslib.as_array(hc.tokens, shape=(max(nt, 1),))[:nt].copy()
+        starts = np.ctypeslib.as_array(hc.starts, shape=(max(n, 1),))[:n].copy()
+        sizes = np.ctypeslib.as_array(hc.sizes, shape=(max(n, 1),))[:n].copy()
+        return SetCollection(toks, starts, sizes, int(hc.universe))
+    finally:
+        _lib.hs_host_collection_free(C.byref(hc))
+
+
+def gen_params(n, *, size_dist=SIZE_LOGNORMAL, median=10.0, sigma=0.6, lo=2, hi=200, universe=1_000_000,
+               zipf=1.1, seed=1) -> GenParams:
+    return GenParams(n, size_dist, float(median), float(sigma), lo, hi, universe, float(zipf), seed)
+
+
+def generate_self(p: GenParams) -> SetCollection:
+    hc = HostCollection()
+    rc = _lib.hs_generate_self(C.byref(p), C.byref(hc))
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_generate_self")
+    return _take(hc)
+
+
+def generate_rs(pr: GenParams, ps: GenParams, dup_frac=0.2, edit_lo=0.05, edit_hi=0.15):
+    hr, hs_ = HostCollection(), HostCollection()
+    rc = _lib.hs_generate_rs(C.byref(pr), C.byref(ps), dup_frac, edit_lo, edit_hi, C.byref(hr), C.byref(hs_))
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_generate_rs")
+    return _take(hr), _take(hs_)
+
+
+def generate_queries(pi: GenParams, pq: GenParams):
+    hi, hq = HostCollection(), HostCollection()
+    rc = _lib.hs_generate_queries(C.byref(pi), C.byref(pq), C.byref(hi), C.byref(hq))
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_generate_queries")
+    return _take(hi), _take(hq)
+
+
+def read_collection_text(path) -> SetCollection:
+    hc = HostCollection()
+    rc = _lib.hs_read_collection_text(str(path).encode(), C.byref(hc))
+    if rc != HS_OK:
+        raise HysetError(rc, f"cannot read {path}")
+    return _take(hc)
+
+
+def write_collection_text(path, c: SetCollection) -> None:
+    t, st, sz = _u32(c.tokens), _u32(c.starts), _u32(c.sizes)
+    rc = _lib.hs_write_collection_text(str(path).encode(), _ptr(t), _ptr(st), _ptr(sz), c.n)
+    if rc != HS_OK:
+        raise HysetError(rc, f"cannot write {path}")
+
+
+def write_pairs_text(path, r, s, overlap=None) -> None:
+    r, s = _u32(r), _u32(s)
+    ov = _u32(overlap) if overlap is not None else None
+    rc = _lib.hs_write_pairs_text(str(path).encode(), _ptr(r), _ptr(s), _ptr(ov) if ov is not None else None,
+                                  r.shape[0])
+    if rc != HS_OK:
+        raise HysetError(rc, f"cannot write {path}")
+
+
+# ------------------------------------------------------------------ device
+
+class Context:
+    """One HIP stream plus its scratch on one device (hs_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = _vp()
+        rc = _lib.hs_create(device, C.byref(h))
+        if rc != HS_OK:
+            raise HysetError(rc, f"hs_create({device}) failed")
+        self._h = h
+        self._colls = []
+
+    def _check(self, rc: int, what: str):
+        if rc != HS_OK:
+            raise HysetError(rc, f"{what}: {_lib.hs_last_error(self._h).decode(errors='replace')}")
+
+    def upload(self, c: SetCollection) -> "DeviceCollection":
+        t, st, sz = _u32(c.tokens), _u32(c.starts), _u32(c.sizes)
+        h = _vp()
+        self._check(_lib.hs_upload_collection(self._h, _ptr(t), _ptr(st), _ptr(sz), c.n, t.shape[0], C.byref(h)),
+                    "hs_upload_collection")
+        dc = DeviceCollection(h, c)
+        self._colls.append(dc)
+        return dc
+
+    def set_threshold(self, t: float, sim="jaccard", bounds=BOUNDS_REFERENCE) -> None:
+        kind = SIMS[sim] if isinstance(sim, str) else int(sim)
+        self._check(_lib.hs_set_threshold(self._h, kind, float(t), int(bounds)), "hs_set_threshold")
+
+    def join(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, algo="fgss",
+             indexed_range=None, probe_range=None, tune: Tune | None = None, count_only=False,
+             capacity=None):
+        """Returns (r, s, overlap) arrays, or the pair count when count_only."""
+        probe = indexed if probe is None else probe
+        a = ALGOS[algo] if isinstance(algo, str) else int(algo)
+        ir = Range(*indexed_range) if indexed_range is not None else None
+        pr = Range(*probe_range) if probe_range is not None else None
+        res = Result()
+        res.count_only = 1
+        self._check(_lib.hs_join(self._h, indexed._h, probe._h, C.byref(ir) if ir else None,
+                                 C.byref(pr) if pr else None, a, C.byref(tune) if tune else None, C.byref(res)),
+                    "hs_join")
+        if count_only:
+            return int(res.count)
+        return self.last_result(int(res.count))
+
+    def last_result(self, count: int):
+        r = np.empty(max(count, 1), dtype=np.uint32)
+        s = np.empty_like(r)
+        ov = np.empty_like(r)
+        res = Result(0, count, _ptr(r), _ptr(s), _ptr(ov), 0)
+        self._check(_lib.hs_copy_last_result(self._h, C.byref(res)), "hs_copy_last_result")
+        return r[:count], s[:count], ov[:count]
+
+    def verify_candidates(self, indexed, probe, probes, offsets, candidates, scenario=0):
+        pr, of, ca = _u32(probes), _u32(offsets), _u32(candidates)
+        res = Result()
+        res.count_only = 1
+        self._check(_lib.hs_verify_candidates(self._h, indexed._h, probe._h, _ptr(pr), _ptr(of), _ptr(ca),
+                                              pr.shape[0], scenario, C.byref(res)), "hs_verify_candidates")
+        return self.last_result(int(res.count))
+
+    def stats(self) -> dict:
+        st = Stats()
+        _lib.hs_get_stats(self._h, C.byref(st))
+        return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def close(self):
+        for c in self._colls:
+            c.free()
+        self._colls = []
+        if self._h:
+            _lib.hs_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceCollection:
+    def __init__(self, h, host: SetCollection):
+        self._h = h
+        self.host = host
+
+    @property
+    def n(self):
+        return self.host.n
+
+    def free(self):
+        if self._h:
+            _lib.hs_free_collection(self._h)
+            self._h = None
+
+
+def sort_pairs(r, s, ov):
+    """Sorts result triples by (r, s)."""
+    key = (np.asarray(r, dtype=np.uint64) << np.uint64(32)) | np.asarray(s, dtype=np.uint64)
+    order = np.argsort(key, kind="stable")
+    return np.asarray(r)[order], np.asarray(s)[order], np.asarray(ov)[order]

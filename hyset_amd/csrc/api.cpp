@@ -1,0 +1,747 @@
+// C ABI of libhyset_amd.so: contexts, device-resident collections, bound
+// tables and the join drivers for the three techniques.
+//
+// Reference counterparts (paths relative to the reference repository):
+//   hs_create / hs_destroy     bitmap::handler, fgssjoin::handler and
+//                              hybrid::device_handler constructors/destructors
+//                              (include/hyset/device_algorithms.hpp:138-171,
+//                              :322-348; include/hyset/hybrid_algorithms.hpp:41-61)
+//   hs_upload_collection       device_collection (include/hyset/collection.hpp:54-71)
+//   hs_join                    handler::join(block*, block*) and
+//                              join(partition*, partition*) (device_algorithms.hpp:177-247,
+//                              :363-432); the technique switch of src/queue.cu:181-214
+//   hs_verify_candidates       device_handler::addCandidates / join / invoke / flush
+//                              (hybrid_algorithms.hpp:71-200)
+// The reference throws on CUDA errors and downcasts its output sink; here
+// every call returns a code and the sink is a plain struct.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "hs_internal.h"
+
+using hs::DevBuf;
+using hs::Error;
+
+// ---------------------------------------------------------------- DevBuf
+
+void hs::DevBuf::ensure(size_t need) {
+  if (need <= bytes && ptr) return;
+  release();
+  size_t want = need < 256 ? 256 : need + need / 8;
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw Error{HS_E_OOM, std::string("hipMalloc of ") + std::to_string(want) + " bytes failed"};
+  }
+  ptr = p;
+  bytes = want;
+}
+
+void hs::DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+// Page-locked host staging buffer (hipHostMalloc; grow-only).
+struct PinnedBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t need) {
+    if (need <= bytes && ptr) return;
+    release();
+    size_t want = need < 256 ? 256 : need + need / 8;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error{HS_E_OOM, "hipHostMalloc failed"};
+    }
+    ptr = p;
+    bytes = want;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(ptr); }
+  ~PinnedBuf() { release(); }
+};
+
+// ------------------------------------------------------------ structures
+
+struct hs_coll {
+  int device = 0;
+  uint32_t n = 0;
+  uint64_t n_tokens = 0;
+  uint32_t universe = 0;
+  uint32_t maxlen = 0;
+  DevBuf tokens, starts, sizes, size_first, sigs;
+  uint32_t sig_words = 0;
+  std::vector<uint32_t> h_tokens, h_starts, h_sizes, h_size_first;
+
+  hs::DevColl dev() const {
+    return hs::DevColl{tokens.as<uint32_t>(), starts.as<uint32_t>(), sizes.as<uint32_t>(), n};
+  }
+};
+
+// One double-buffered slot of the hybrid candidate pipeline.
+struct HybridSlot {
+  PinnedBuf h_probes, h_offsets, h_cands;
+  DevBuf d_probes, d_offsets, d_cands;
+  hipEvent_t done = nullptr;
+  bool used = false;
+};
+
+struct hs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool have_threshold = false;
+  int kind = HS_JACCARD, mode = HS_BOUNDS_REFERENCE;
+  double t = 0.0;
+  hs::BoundTables tables;
+  bool tables_valid = false;
+  DevBuf d_tables;
+  hs::DevBounds dev_bounds{};
+  hs::IndexBuild ib;
+  DevBuf out_r, out_s, out_ov, counters;
+  uint64_t out_cap = 0;
+  PinnedBuf h_counters;
+  hipEvent_t ev[6] = {};
+  hs_stats stats{};
+  uint64_t last_count = 0;
+  HybridSlot slots[2];
+  std::mutex submit_mu;
+  std::vector<uint32_t> h_offsets, h_set_ids;  // host copy of the index (hybrid)
+};
+
+namespace {
+
+template <typename F>
+int guarded(hs_ctx* ctx, F&& body) {
+  try {
+    if (ctx) {
+      hipError_t e = hipSetDevice(ctx->device);
+      if (e != hipSuccess) throw Error{HS_E_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e)};
+    }
+    body();
+    if (ctx) ctx->err.clear();
+    return HS_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->err = "host allocation failed";
+    return HS_E_OOM;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return HS_E_INVALID;
+  } catch (...) {
+    if (ctx) ctx->err = "unknown error";
+    return HS_E_INVALID;
+  }
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.f;
+  }
+  return ms;
+}
+
+// Bound tables covering set sizes up to L, uploaded to the device.
+void ensure_bounds(hs_ctx* ctx, uint32_t L) {
+  HS_REQUIRE(ctx->have_threshold, HS_E_INVALID, "hs_set_threshold has not been called");
+  if (ctx->tables_valid && ctx->tables.L >= L) return;
+  uint32_t want = std::max<uint32_t>(L, 64);
+  ctx->tables.build(ctx->kind, ctx->mode, ctx->t, want);
+  const auto& T = ctx->tables;
+  const size_t nl = T.L + 2;
+  const size_t total = 4 * nl + T.mo.size();
+  std::vector<uint32_t> flat(total);
+  std::copy(T.minsize.begin(), T.minsize.end(), flat.begin());
+  std::copy(T.maxsize.begin(), T.maxsize.end(), flat.begin() + nl);
+  std::copy(T.maxprefix.begin(), T.maxprefix.end(), flat.begin() + 2 * nl);
+  std::copy(T.midprefix.begin(), T.midprefix.end(), flat.begin() + 3 * nl);
+  std::copy(T.mo.begin(), T.mo.end(), flat.begin() + 4 * nl);
+  ctx->d_tables.ensure(total * sizeof(uint32_t));
+  HS_CHECK_HIP(hipMemcpy(ctx->d_tables.ptr, flat.data(), total * sizeof(uint32_t), hipMemcpyHostToDevice));
+  const uint32_t* base = ctx->d_tables.as<uint32_t>();
+  ctx->dev_bounds = hs::DevBounds{base, base + nl, base + 2 * nl, base + 3 * nl, base + 4 * nl, T.L, T.kind};
+  ctx->tables_valid = true;
+}
+
+void ensure_output(hs_ctx* ctx, uint64_t cap) {
+  if (cap <= ctx->out_cap && ctx->out_r.ptr) return;
+  ctx->out_r.ensure(cap * sizeof(uint32_t));
+  ctx->out_s.ensure(cap * sizeof(uint32_t));
+  ctx->out_ov.ensure(cap * sizeof(uint32_t));
+  ctx->out_cap = cap;
+}
+
+hs::DevOut dev_out(hs_ctx* ctx) {
+  return hs::DevOut{ctx->out_r.as<uint32_t>(), ctx->out_s.as<uint32_t>(), ctx->out_ov.as<uint32_t>(),
+                    ctx->counters.as<unsigned long long>(), ctx->out_cap};
+}
+
+void reset_counters(hs_ctx* ctx) {
+  hs::launch_fill_u64(ctx->stream, ctx->counters.as<unsigned long long>(), hs::CNT_WORDS, 0);
+}
+
+void read_counters(hs_ctx* ctx, unsigned long long* dst) {
+  HS_CHECK_HIP(hipMemcpyAsync(ctx->h_counters.ptr, ctx->counters.ptr, hs::CNT_WORDS * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(dst, ctx->h_counters.ptr, hs::CNT_WORDS * sizeof(unsigned long long));
+}
+
+void copy_pairs(hs_ctx* ctx, uint64_t count, hs_result* res) {
+  if (count == 0) return;
+  HS_REQUIRE(res->r && res->s, HS_E_INVALID, "result arrays are NULL");
+  HS_CHECK_HIP(hipMemcpyAsync(res->r, ctx->out_r.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HS_CHECK_HIP(hipMemcpyAsync(res->s, ctx->out_s.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  if (res->overlap)
+    HS_CHECK_HIP(hipMemcpyAsync(res->overlap, ctx->out_ov.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+hs_range clamp_range(const hs_range* r, uint32_t n) {
+  hs_range out{0, 0};
+  if (n == 0) return hs_range{1, 0};
+  if (!r) return hs_range{0, n - 1};
+  out = *r;
+  if (out.last >= n) out.last = n - 1;
+  return out;
+}
+
+bool range_empty(const hs_range& r) { return r.first > r.last; }
+
+// ------------------------------------------------------- hybrid pipeline
+
+struct HybridShared {
+  hs_ctx* ctx;
+  hs_coll* indexed;
+  hs_coll* probe;
+  bool self;
+  hs_range ir, pr;
+  int scenario;
+  uint64_t batch;
+  std::atomic<uint64_t> candidates{0}, visits{0};
+};
+
+// Copies one candidate batch into a pinned slot and queues its
+// verification on the context stream; the host returns to filtering while
+// the device verifies (the reference double-buffers with a thread per
+// batch, hybrid_algorithms.hpp:107-120).
+void submit_batch(HybridShared& H, const std::vector<uint32_t>& probes, const std::vector<uint32_t>& offsets,
+                  const std::vector<uint32_t>& cands, uint32_t& next_slot) {
+  if (probes.empty()) return;
+  hs_ctx* ctx = H.ctx;
+  std::lock_guard<std::mutex> lock(ctx->submit_mu);
+  HybridSlot& sl = ctx->slots[next_slot];
+  next_slot ^= 1u;
+  if (sl.used) HS_CHECK_HIP(hipEventSynchronize(sl.done));
+  const size_t np = probes.size(), nc = cands.size();
+  sl.h_probes.ensure(np * 4);
+  sl.h_offsets.ensure(np * 4);
+  sl.h_cands.ensure(nc * 4);
+  sl.d_probes.ensure(np * 4);
+  sl.d_offsets.ensure(np * 4);
+  sl.d_cands.ensure(nc * 4);
+  std::memcpy(sl.h_probes.ptr, probes.data(), np * 4);
+  std::memcpy(sl.h_offsets.ptr, offsets.data(), np * 4);
+  std::memcpy(sl.h_cands.ptr, cands.data(), nc * 4);
+  HS_CHECK_HIP(hipMemcpyAsync(sl.d_probes.ptr, sl.h_probes.ptr, np * 4, hipMemcpyHostToDevice, ctx->stream));
+  HS_CHECK_HIP(hipMemcpyAsync(sl.d_offsets.ptr, sl.h_offsets.ptr, np * 4, hipMemcpyHostToDevice, ctx->stream));
+  HS_CHECK_HIP(hipMemcpyAsync(sl.d_cands.ptr, sl.h_cands.ptr, nc * 4, hipMemcpyHostToDevice, ctx->stream));
+  hs::VerifyArgs va{H.probe->dev(), H.indexed->dev(), ctx->dev_bounds, sl.d_probes.as<uint32_t>(),
+                    sl.d_offsets.as<uint32_t>(), sl.d_cands.as<uint32_t>(), (uint32_t)np, (uint64_t)nc,
+                    dev_out(ctx)};
+  hs::launch_verify(ctx->stream, va, H.scenario);
+  HS_CHECK_HIP(hipEventRecord(sl.done, ctx->stream));
+  sl.used = true;
+}
+
+// CPU candidate generation for probes [first, last] (hybrid::allpairs,
+// include/hyset/hybrid_algorithms.hpp:209-291): every posting of a probe
+// prefix token inside the length window, deduplicated per probe, no
+// verification.  Lists are id-sorted, so the window start is found by
+// binary search instead of the reference's linear skip.
+void host_filter(HybridShared& H, uint32_t first, uint32_t last, uint32_t& next_slot) {
+  hs_ctx* ctx = H.ctx;
+  const hs_coll& P = *H.probe;
+  const hs_coll& S = *H.indexed;
+  const auto& T = ctx->tables;
+  std::vector<uint32_t> stamp(S.n, 0xFFFFFFFFu);
+  std::vector<uint32_t> probes, offsets, cands;
+  probes.reserve(1 << 16);
+  offsets.reserve(1 << 16);
+  cands.reserve(std::min<uint64_t>(H.batch, 1ull << 26));
+  uint64_t n_cand = 0, n_visit = 0;
+  const uint32_t last_len = (uint32_t)S.h_size_first.size() - 1;
+  for (uint32_t p = first; p <= last && first <= last; ++p) {
+    const uint32_t l = P.h_sizes[p];
+    const uint32_t* rt = P.h_tokens.data() + P.h_starts[p];
+    const uint32_t mins = std::min(T.minsize[l], last_len);
+    const uint32_t maxs1 = std::min(T.maxsize[l], last_len - 1) + 1;
+    uint32_t lo = std::max(S.h_size_first[mins], H.ir.first);
+    uint32_t hi = std::min(S.h_size_first[maxs1], H.ir.last + 1);
+    if (H.self) hi = std::min(hi, p);
+    if (lo >= hi) continue;
+    const uint32_t pre = T.maxprefix[l];
+    const size_t row_begin = cands.size();
+    for (uint32_t j = 0; j < pre; ++j) {
+      const uint32_t tok = rt[j];
+      if (tok >= S.universe) continue;
+      const uint32_t* a = ctx->h_set_ids.data() + ctx->h_offsets[tok];
+      const uint32_t* e = ctx->h_set_ids.data() + ctx->h_offsets[tok + 1];
+      for (const uint32_t* it = std::lower_bound(a, e, lo); it != e && *it < hi; ++it) {
+        ++n_visit;
+        if (stamp[*it] != p) {
+          stamp[*it] = p;
+          cands.push_back(*it);
+        }
+      }
+    }
+    if (cands.size() > row_begin) {
+      probes.push_back(p);
+      offsets.push_back((uint32_t)cands.size());
+      n_cand += cands.size() - row_begin;
+      if (cands.size() >= H.batch) {
+        submit_batch(H, probes, offsets, cands, next_slot);
+        probes.clear();
+        offsets.clear();
+        cands.clear();
+      }
+    }
+  }
+  submit_batch(H, probes, offsets, cands, next_slot);
+  H.candidates += n_cand;
+  H.visits += n_visit;
+}
+
+// Probe partition boundaries: sqrt-spaced for a self-join, where the work
+// of probe r grows with r (src/cpu_multi.cu:118-139), even otherwise.
+std::vector<uint32_t> probe_splits(const hs_range& pr, uint32_t parts, bool self) {
+  std::vector<uint32_t> pts;
+  const uint64_t n = (uint64_t)pr.last - pr.first + 1;
+  pts.push_back(pr.first);
+  for (uint32_t i = 1; i < parts; ++i) {
+    double f = self ? std::sqrt((double)i / parts) : (double)i / parts;
+    pts.push_back(pr.first + (uint32_t)(n * f));
+  }
+  pts.push_back(pr.last + 1);
+  return pts;
+}
+
+void run_hybrid(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const hs_range& ir, const hs_range& pr,
+                const hs_tune& tune) {
+  // host copy of the device-built index (the reference builds its CPU index
+  // on the device and copies it back, index.hpp:250-282)
+  const uint32_t U = indexed->universe;
+  ctx->h_offsets.resize((size_t)U + 1);
+  HS_CHECK_HIP(hipMemcpyAsync(ctx->h_offsets.data(), ctx->ib.offsets.ptr, ((size_t)U + 1) * 4, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const uint32_t entries = ctx->h_offsets[U];
+  ctx->h_set_ids.resize(std::max<uint32_t>(entries, 1));
+  if (entries)
+    HS_CHECK_HIP(hipMemcpyAsync(ctx->h_set_ids.data(), ctx->ib.set_ids.ptr, (size_t)entries * 4,
+                                hipMemcpyDeviceToHost, ctx->stream));
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->stats.index_entries = entries;
+
+  HybridShared H;
+  H.ctx = ctx;
+  H.indexed = indexed;
+  H.probe = probe;
+  H.self = self;
+  H.ir = ir;
+  H.pr = pr;
+  const double avg = probe->n ? (double)probe->n_tokens / probe->n : 0.0;
+  H.scenario = tune.verify_scenario ? (int)tune.verify_scenario : (avg > 64.0 ? 2 : 1);
+  H.batch = tune.batch_candidates ? tune.batch_candidates : (1ull << 22);
+  for (auto& sl : ctx->slots)
+    if (!sl.done) HS_CHECK_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+
+  const uint32_t threads = std::max<uint32_t>(1, tune.host_threads);
+  auto t0 = std::chrono::steady_clock::now();
+  if (!range_empty(pr)) {
+    if (threads == 1) {
+      uint32_t slot = 0;
+      host_filter(H, pr.first, pr.last, slot);
+    } else {
+      std::vector<uint32_t> pts = probe_splits(pr, threads, self);
+      std::vector<std::thread> pool;
+      std::vector<Error> errs;
+      std::mutex emu;
+      for (uint32_t i = 0; i < threads; ++i) {
+        if (pts[i + 1] <= pts[i]) continue;
+        pool.emplace_back([&, i] {
+          try {
+            uint32_t slot = i & 1u;
+            host_filter(H, pts[i], pts[i + 1] - 1, slot);
+          } catch (const Error& e) {
+            std::lock_guard<std::mutex> g(emu);
+            errs.push_back(e);
+          } catch (...) {
+            std::lock_guard<std::mutex> g(emu);
+            errs.push_back(Error{HS_E_INVALID, "host filter failed"});
+          }
+        });
+      }
+      for (auto& th : pool) th.join();
+      if (!errs.empty()) throw errs.front();
+    }
+  }
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  auto t1 = std::chrono::steady_clock::now();
+  ctx->stats.host_filter_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  ctx->stats.candidates = H.candidates.load();
+  ctx->stats.posting_visits = H.visits.load();
+}
+
+void check_pair(hs_ctx* ctx, hs_coll* a, hs_coll* b) {
+  HS_REQUIRE(ctx && a && b, HS_E_INVALID, "NULL context or collection");
+  HS_REQUIRE(a->device == ctx->device && b->device == ctx->device, HS_E_INVALID,
+             "collection lives on another device");
+}
+
+}  // namespace
+
+// ================================================================ C ABI
+
+extern "C" {
+
+int hs_create(int device, hs_ctx** out) {
+  if (!out) return HS_E_INVALID;
+  *out = nullptr;
+  hs_ctx* ctx = nullptr;
+  try {
+    ctx = new hs_ctx();
+  } catch (...) {
+    return HS_E_OOM;
+  }
+  ctx->device = device;
+  int rc = guarded(ctx, [&] {
+    int count = 0;
+    HS_CHECK_HIP(hipGetDeviceCount(&count));
+    HS_REQUIRE(device >= 0 && device < count, HS_E_INVALID, "no such device");
+    HS_CHECK_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    for (auto& e : ctx->ev) HS_CHECK_HIP(hipEventCreate(&e));
+    ctx->counters.ensure(hs::CNT_WORDS * sizeof(unsigned long long));
+    ctx->h_counters.ensure(hs::CNT_WORDS * sizeof(unsigned long long));
+    ensure_output(ctx, 1u << 20);
+  });
+  if (rc != HS_OK) {
+    hs_destroy(ctx);
+    return rc;
+  }
+  *out = ctx;
+  return HS_OK;
+}
+
+void hs_destroy(hs_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto& sl : ctx->slots)
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* hs_last_error(const hs_ctx* ctx) { return ctx ? ctx->err.c_str() : "NULL context"; }
+
+int hs_upload_collection(hs_ctx* ctx, const uint32_t* tokens, const uint32_t* starts, const uint32_t* sizes,
+                         uint32_t n, uint64_t n_tokens, hs_coll** out) {
+  if (!ctx || !out) return HS_E_INVALID;
+  *out = nullptr;
+  std::unique_ptr<hs_coll> c;
+  int rc = guarded(ctx, [&] {
+    HS_REQUIRE(n == 0 || (sizes && tokens), HS_E_INVALID, "NULL arrays");
+    HS_REQUIRE(n_tokens < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "collections are limited to 2^32-1 tokens");
+    c.reset(new hs_coll());
+    c->device = ctx->device;
+    c->n = n;
+    c->h_sizes.assign(sizes, sizes + n);
+    c->h_starts.resize(n);
+    uint64_t pos = 0;
+    uint32_t maxtok = 0, maxlen = 0, prev = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint32_t l = sizes[i];
+      HS_REQUIRE(l >= 1, HS_E_INVALID, "empty set at id " + std::to_string(i));
+      HS_REQUIRE(l >= prev, HS_E_INVALID, "sets are not sorted by size at id " + std::to_string(i));
+      prev = l;
+      const uint64_t src = starts ? starts[i] : pos;
+      HS_REQUIRE(src + l <= n_tokens, HS_E_INVALID, "set " + std::to_string(i) + " exceeds the token array");
+      const uint32_t* t = tokens + src;
+      for (uint32_t k = 1; k < l; ++k)
+        HS_REQUIRE(t[k - 1] < t[k], HS_E_INVALID, "tokens not strictly ascending in set " + std::to_string(i));
+      maxtok = std::max(maxtok, t[l - 1]);
+      maxlen = std::max(maxlen, l);
+      c->h_starts[i] = (uint32_t)pos;
+      pos += l;
+    }
+    HS_REQUIRE(maxtok < 0xFFFFFFFFu, HS_E_UNSUPPORTED, "token id 2^32-1 is reserved");
+    c->n_tokens = pos;
+    c->maxlen = maxlen;
+    c->universe = n ? maxtok + 1 : 1;
+    c->h_tokens.resize(pos);
+    for (uint32_t i = 0; i < n; ++i) {
+      const uint64_t src = starts ? starts[i] : c->h_starts[i];
+      std::memcpy(c->h_tokens.data() + c->h_starts[i], tokens + src, sizeof(uint32_t) * sizes[i]);
+    }
+    c->h_size_first.assign((size_t)maxlen + 2, n);
+    for (uint32_t l = 0, i = 0; l <= maxlen + 1; ++l) {
+      while (i < n && c->h_sizes[i] < l) ++i;
+      c->h_size_first[l] = i;
+    }
+    c->tokens.ensure(std::max<uint64_t>(pos, 1) * 4);
+    c->starts.ensure(std::max<uint32_t>(n, 1) * 4);
+    c->sizes.ensure(std::max<uint32_t>(n, 1) * 4);
+    c->size_first.ensure(c->h_size_first.size() * 4);
+    if (pos) HS_CHECK_HIP(hipMemcpy(c->tokens.ptr, c->h_tokens.data(), pos * 4, hipMemcpyHostToDevice));
+    if (n) {
+      HS_CHECK_HIP(hipMemcpy(c->starts.ptr, c->h_starts.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+      HS_CHECK_HIP(hipMemcpy(c->sizes.ptr, c->h_sizes.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+    }
+    HS_CHECK_HIP(hipMemcpy(c->size_first.ptr, c->h_size_first.data(), c->h_size_first.size() * 4,
+                           hipMemcpyHostToDevice));
+  });
+  if (rc == HS_OK) *out = c.release();
+  return rc;
+}
+
+void hs_free_collection(hs_coll* coll) {
+  if (!coll) return;
+  (void)hipSetDevice(coll->device);
+  (void)hipDeviceSynchronize();
+  delete coll;
+}
+
+int hs_set_threshold(hs_ctx* ctx, int32_t sim, double threshold, int32_t bounds_mode) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    hs::BoundTables probe_tables;
+    probe_tables.build(sim, bounds_mode, threshold, 4);  // validates the combination
+    ctx->kind = sim;
+    ctx->mode = bounds_mode;
+    ctx->t = threshold;
+    ctx->have_threshold = true;
+    ctx->tables_valid = false;
+  });
+}
+
+int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* indexed_range,
+            const hs_range* probe_range, int32_t algo, const hs_tune* tune_in, hs_result* result) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    check_pair(ctx, indexed, probe);
+    HS_REQUIRE(result, HS_E_INVALID, "NULL result");
+    HS_REQUIRE(algo == HS_ALGO_FGSS || algo == HS_ALGO_BITMAP || algo == HS_ALGO_HYBRID, HS_E_INVALID,
+               "unknown algorithm");
+    hs_tune tune{};
+    if (tune_in) tune = *tune_in;
+    const bool self = indexed == probe;
+    const hs_range ir = clamp_range(indexed_range, indexed->n);
+    const hs_range pr = clamp_range(probe_range, probe->n);
+    ensure_bounds(ctx, std::max(indexed->maxlen, probe->maxlen));
+    ctx->stats = hs_stats{};
+    hipStream_t st = ctx->stream;
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
+
+    // ---- index over the indexed range (not needed by the bitmap technique)
+    const uint32_t* prefix_tab = self ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
+    if (algo != HS_ALGO_BITMAP && !range_empty(ir)) {
+      const uint64_t bound = (uint64_t)indexed->h_starts[ir.last] + indexed->h_sizes[ir.last] -
+                             indexed->h_starts[ir.first];
+      hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->universe, bound, ctx->ib);
+    } else {
+      ctx->ib.universe = 0;
+    }
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
+
+    // ---- candidate generation + verification
+    unsigned long long cnt[hs::CNT_WORDS];
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      reset_counters(ctx);
+      if (range_empty(pr) || range_empty(ir)) {
+        // nothing to join
+      } else if (algo == HS_ALGO_FGSS) {
+        hs::FgssArgs a{};
+        a.probe = probe->dev();
+        a.indexed = indexed->dev();
+        a.index = hs::DevIndex{ctx->ib.offsets.as<uint32_t>(), ctx->ib.set_ids.as<uint32_t>(), indexed->universe};
+        a.bounds = ctx->dev_bounds;
+        a.size_first = indexed->size_first.as<uint32_t>();
+        a.size_first_len = (uint32_t)indexed->h_size_first.size();
+        a.probe_first = pr.first;
+        a.probe_last = pr.last;
+        a.idx_first = ir.first;
+        a.idx_last = ir.last;
+        a.self = self;
+        a.out = dev_out(ctx);
+        hs::launch_fgss(st, a, tune.probes_per_tile);
+      } else if (algo == HS_ALGO_BITMAP) {
+        const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
+        HS_REQUIRE(bits == 64 || bits == 128 || bits == 256, HS_E_INVALID, "bitmap width must be 64, 128 or 256");
+        const uint32_t words = bits / 64;
+        hs_coll* colls[2] = {indexed, probe};
+        for (hs_coll* c : colls) {
+          if (c->sig_words != words) {
+            c->sigs.ensure((size_t)std::max<uint32_t>(c->n, 1) * words * 8);
+            c->sig_words = 0;
+          }
+          hs::launch_signatures(st, c->dev(), words, c->sigs.as<uint64_t>());
+          c->sig_words = words;
+          if (self) break;
+        }
+        hs::BitmapArgs b{};
+        b.probe = probe->dev();
+        b.indexed = indexed->dev();
+        b.bounds = ctx->dev_bounds;
+        b.probe_sigs = probe->sigs.as<uint64_t>();
+        b.indexed_sigs = indexed->sigs.as<uint64_t>();
+        b.size_first = indexed->size_first.as<uint32_t>();
+        b.size_first_len = (uint32_t)indexed->h_size_first.size();
+        b.words = words;
+        b.probe_first = pr.first;
+        b.probe_last = pr.last;
+        b.idx_first = ir.first;
+        b.idx_last = ir.last;
+        b.self = self;
+        b.out = dev_out(ctx);
+        hs::launch_bitmap(st, b);
+      } else {
+        run_hybrid(ctx, indexed, probe, self, ir, pr, tune);
+      }
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
+      read_counters(ctx, cnt);
+      if (cnt[hs::CNT_PAIRS] <= ctx->out_cap) break;
+      HS_REQUIRE(attempt == 0, HS_E_DEVICE, "result count changed between runs");
+      ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
+    }
+    const uint64_t count = cnt[hs::CNT_PAIRS];
+    ctx->last_count = count;
+    result->count = count;
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[3], st));
+    if (!result->count_only && count > 0) {
+      HS_REQUIRE(count <= result->capacity, HS_E_OVERFLOW, "result capacity too small");
+      copy_pairs(ctx, count, result);
+    }
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[4], st));
+    HS_CHECK_HIP(hipEventSynchronize(ctx->ev[4]));
+    ctx->stats.index_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+    ctx->stats.filter_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+    ctx->stats.transfer_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+    ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
+    ctx->stats.pairs = count;
+    if (algo != HS_ALGO_HYBRID) {
+      ctx->stats.posting_visits = cnt[hs::CNT_VISITS];
+      ctx->stats.candidates = cnt[hs::CNT_CANDS];
+    }
+  });
+}
+
+int hs_verify_candidates(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const uint32_t* probes,
+                         const uint32_t* offsets, const uint32_t* candidates, uint32_t n_probes, int32_t scenario,
+                         hs_result* result) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    check_pair(ctx, indexed, probe);
+    HS_REQUIRE(result, HS_E_INVALID, "NULL result");
+    HS_REQUIRE(n_probes == 0 || (probes && offsets), HS_E_INVALID, "NULL batch arrays");
+    const uint64_t nc = n_probes ? offsets[n_probes - 1] : 0;
+    HS_REQUIRE(nc == 0 || candidates, HS_E_INVALID, "NULL candidates");
+    uint32_t prev = 0;
+    for (uint32_t i = 0; i < n_probes; ++i) {
+      HS_REQUIRE(offsets[i] >= prev, HS_E_INVALID, "offsets must be non-decreasing");
+      HS_REQUIRE(probes[i] < probe->n, HS_E_INVALID, "probe id out of range");
+      prev = offsets[i];
+    }
+    for (uint64_t c = 0; c < nc; ++c) HS_REQUIRE(candidates[c] < indexed->n, HS_E_INVALID, "candidate out of range");
+    ensure_bounds(ctx, std::max(indexed->maxlen, probe->maxlen));
+    ctx->stats = hs_stats{};
+    hipStream_t st = ctx->stream;
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
+    HybridSlot& sl = ctx->slots[0];
+    if (!sl.done) HS_CHECK_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+    if (sl.used) HS_CHECK_HIP(hipEventSynchronize(sl.done));
+    sl.d_probes.ensure((size_t)std::max<uint32_t>(n_probes, 1) * 4);
+    sl.d_offsets.ensure((size_t)std::max<uint32_t>(n_probes, 1) * 4);
+    sl.d_cands.ensure((size_t)std::max<uint64_t>(nc, 1) * 4);
+    if (n_probes) {
+      HS_CHECK_HIP(hipMemcpyAsync(sl.d_probes.ptr, probes, (size_t)n_probes * 4, hipMemcpyHostToDevice, st));
+      HS_CHECK_HIP(hipMemcpyAsync(sl.d_offsets.ptr, offsets, (size_t)n_probes * 4, hipMemcpyHostToDevice, st));
+    }
+    if (nc) HS_CHECK_HIP(hipMemcpyAsync(sl.d_cands.ptr, candidates, nc * 4, hipMemcpyHostToDevice, st));
+    const double avg = probe->n ? (double)probe->n_tokens / probe->n : 0.0;
+    const int scen = scenario ? scenario : (avg > 64.0 ? 2 : 1);
+    unsigned long long cnt[hs::CNT_WORDS];
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      reset_counters(ctx);
+      hs::VerifyArgs va{probe->dev(), indexed->dev(), ctx->dev_bounds, sl.d_probes.as<uint32_t>(),
+                        sl.d_offsets.as<uint32_t>(), sl.d_cands.as<uint32_t>(), n_probes, nc, dev_out(ctx)};
+      hs::launch_verify(st, va, scen);
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
+      read_counters(ctx, cnt);
+      if (cnt[hs::CNT_PAIRS] <= ctx->out_cap) break;
+      ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
+    }
+    sl.used = false;
+    const uint64_t count = cnt[hs::CNT_PAIRS];
+    ctx->last_count = count;
+    result->count = count;
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[3], st));
+    if (!result->count_only && count > 0) {
+      HS_REQUIRE(count <= result->capacity, HS_E_OVERFLOW, "result capacity too small");
+      copy_pairs(ctx, count, result);
+    }
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[4], st));
+    HS_CHECK_HIP(hipEventSynchronize(ctx->ev[4]));
+    ctx->stats.filter_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+    ctx->stats.transfer_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+    ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
+    ctx->stats.pairs = count;
+    ctx->stats.candidates = nc;
+  });
+}
+
+int hs_copy_last_result(hs_ctx* ctx, hs_result* result) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    HS_REQUIRE(result, HS_E_INVALID, "NULL result");
+    result->count = ctx->last_count;
+    HS_REQUIRE(ctx->last_count <= result->capacity, HS_E_OVERFLOW, "result capacity too small");
+    copy_pairs(ctx, ctx->last_count, result);
+  });
+}
+
+int hs_get_stats(const hs_ctx* ctx, hs_stats* out) {
+  if (!ctx || !out) return HS_E_INVALID;
+  *out = ctx->stats;
+  return HS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,191 @@
+// Bitmap-signature filter (the bitmap technique) for gfx950.
+//
+// Reference: include/hyset/device_algorithms.hpp:58-118 (generate: one
+// thread per set toggling bit token % b; filter: one thread per right set
+// looping over every left set of the block, writing a dense blockSize^2
+// matrix) and helpers.hpp:138-150 (change_bit, hamming_distance).
+//
+// A set's signature is the XOR-parity of its tokens hashed to b bits
+// (b = 64, 128, 256), so popcount(sig_r ^ sig_s) <= |r xor s| and
+// |r n s| <= (|r| + |s| - popcount) / 2.  The reference keeps a pair when
+// (|r|+|s|-pop)/2.0 >= minoverlap, i.e. exactly when the integer test
+// |r| + |s| - pop >= 2 * minoverlap holds; that integer form is used here.
+// The bit layout inside a signature differs from the reference's
+// (bit token % b sits in word (token % b) / 64), which leaves every popcount
+// unchanged.
+//
+// Filter design: a workgroup takes 256 consecutive probes (one per lane,
+// signature in registers).  Their windows overlap almost entirely (sizes
+// are sorted), so the union window is streamed through LDS in chunks of
+// signatures + sizes and every lane tests its own window slice against the
+// broadcast LDS entries.  Survivors are verified on the spot by a merge.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "hs_internal.h"
+
+namespace hs {
+
+template <int W>
+__global__ void k_signatures(DevColl c, uint64_t* __restrict__ sigs) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.n) return;
+  uint64_t w[W];
+#pragma unroll
+  for (int k = 0; k < W; ++k) w[k] = 0;
+  const uint32_t* t = c.tokens + c.starts[i];
+  const uint32_t l = c.sizes[i];
+  for (uint32_t k = 0; k < l; ++k) {
+    uint32_t bit = t[k] & (64u * W - 1u);
+#pragma unroll
+    for (int q = 0; q < W; ++q)
+      if ((bit >> 6) == (uint32_t)q) w[q] ^= 1ull << (bit & 63u);
+  }
+#pragma unroll
+  for (int k = 0; k < W; ++k) sigs[(size_t)i * W + k] = w[k];
+}
+
+void launch_signatures(hipStream_t st, const DevColl& c, uint32_t words, uint64_t* sigs) {
+  if (c.n == 0) return;
+  dim3 g((c.n + 255) / 256), b(256);
+  switch (words) {
+    case 1: hipLaunchKernelGGL(k_signatures<1>, g, b, 0, st, c, sigs); break;
+    case 2: hipLaunchKernelGGL(k_signatures<2>, g, b, 0, st, c, sigs); break;
+    case 4: hipLaunchKernelGGL(k_signatures<4>, g, b, 0, st, c, sigs); break;
+    default: throw Error{HS_E_INVALID, "bitmap width must be 64, 128 or 256 bits"};
+  }
+  HS_CHECK_HIP(hipGetLastError());
+}
+
+// Full merge with the early-termination bound; exact overlap for passes.
+__device__ __forceinline__ bool verify_merge(const uint32_t* r, uint32_t lr, const uint32_t* s, uint32_t ls,
+                                             uint32_t mo, uint32_t* ov) {
+  uint32_t i = 0, k = 0, found = 0;
+  uint32_t a = r[0], b = s[0];
+  while (i < lr && k < ls) {
+    if (found + min(lr - i, ls - k) < mo) return false;
+    bool eq = a == b;
+    bool adv_a = a <= b;
+    bool adv_b = a >= b;
+    found += eq;
+    i += adv_a;
+    k += adv_b;
+    if (adv_a && i < lr) a = r[i];
+    if (adv_b && k < ls) b = s[k];
+  }
+  *ov = found;
+  return found >= mo;
+}
+
+constexpr int BM_TPB = 256;
+constexpr uint32_t BM_CHUNK = 1024;
+
+template <int W>
+__global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles) {
+  __shared__ uint64_t s_sig[BM_CHUNK * W];
+  __shared__ uint32_t s_size[BM_CHUNK];
+  __shared__ uint32_t s_red[2 * (BM_TPB / WAVE)];
+  __shared__ unsigned long long s_stats[1];
+
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
+  if (tile == 0xFFFFFFFFu) return;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t p = A.probe_first + tile * BM_TPB + tid;
+  const bool valid = p <= A.probe_last;
+  if (tid == 0) s_stats[0] = 0;
+
+  uint64_t sig[W];
+  uint32_t lr = 0, lo = 0xFFFFFFFFu, hi = 0;
+  if (valid) {
+    lr = A.probe.sizes[p];
+    const uint32_t last = A.size_first_len - 1;
+    const uint32_t mins = min(A.bounds.minsize[lr], last);
+    const uint32_t maxs1 = min(A.bounds.maxsize[lr], last - 1) + 1;
+    lo = max(A.size_first[mins], A.idx_first);
+    hi = min(A.size_first[maxs1], A.idx_last + 1);
+    if (A.self) hi = min(hi, p);
+    if (lo >= hi) {
+      lo = 0xFFFFFFFFu;
+      hi = 0;
+    }
+#pragma unroll
+    for (int k = 0; k < W; ++k) sig[k] = A.probe_sigs[(size_t)p * W + k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < W; ++k) sig[k] = 0;
+  }
+  // union window of the tile
+  uint32_t wlo = wave_min(lo), whi = wave_max(hi);
+  if (lane_id() == 0) {
+    s_red[tid / WAVE] = wlo;
+    s_red[BM_TPB / WAVE + tid / WAVE] = whi;
+  }
+  __syncthreads();
+  uint32_t ulo = 0xFFFFFFFFu, uhi = 0;
+#pragma unroll
+  for (int w = 0; w < BM_TPB / WAVE; ++w) {
+    ulo = min(ulo, s_red[w]);
+    uhi = max(uhi, s_red[BM_TPB / WAVE + w]);
+  }
+
+  const uint32_t* rtok = valid ? A.probe.tokens + A.probe.starts[p] : nullptr;
+  uint32_t cur_ls = 0xFFFFFFFFu;
+  int32_t slack = -1;  // |r| + |s| - 2*minoverlap for the current |s|
+  uint32_t mo_cur = 0;
+  uint32_t survivors = 0;
+
+  for (uint32_t c0 = ulo; c0 < uhi; c0 += BM_CHUNK) {
+    const uint32_t cn = min(BM_CHUNK, uhi - c0);
+    __syncthreads();
+    for (uint32_t i = tid; i < cn * W; i += BM_TPB) s_sig[i] = A.indexed_sigs[(size_t)c0 * W + i];
+    for (uint32_t i = tid; i < cn; i += BM_TPB) s_size[i] = A.indexed.sizes[c0 + i];
+    __syncthreads();
+    const uint32_t b = max(lo, c0), e = min(hi, c0 + cn);
+    for (uint32_t sidx = b; sidx < e; ++sidx) {
+      const uint32_t q = sidx - c0;
+      const uint32_t ls = s_size[q];
+      if (ls != cur_ls) {
+        cur_ls = ls;
+        mo_cur = dev_minoverlap(A.bounds, lr, ls);
+        slack = (int32_t)(lr + ls) - 2 * (int32_t)mo_cur;
+      }
+      uint32_t pop = 0;
+#pragma unroll
+      for (int k = 0; k < W; ++k) pop += __popcll(sig[k] ^ s_sig[q * W + k]);
+      if ((int32_t)pop <= slack) {
+        ++survivors;
+        uint32_t ov = 0;
+        const uint32_t* s = A.indexed.tokens + A.indexed.starts[sidx];
+        if (verify_merge(rtok, lr, s, ls, mo_cur, &ov)) {
+          // single-lane append (survivors are rare and lanes diverge here)
+          unsigned long long idx = atomicAdd(&A.out.counters[CNT_PAIRS], 1ull);
+          if (idx < A.out.capacity) {
+            A.out.r[idx] = p;
+            A.out.s[idx] = sidx;
+            A.out.ov[idx] = ov;
+          }
+        }
+      }
+    }
+  }
+  survivors = wave_sum(survivors);
+  if (lane_id() == 0 && survivors) atomicAdd(&s_stats[0], (unsigned long long)survivors);
+  __syncthreads();
+  if (tid == 0 && s_stats[0]) atomicAdd(&A.out.counters[CNT_CANDS], s_stats[0]);
+}
+
+void launch_bitmap(hipStream_t st, const BitmapArgs& a) {
+  if (a.probe.n == 0 || a.probe_first > a.probe_last) return;
+  const uint32_t nprobes = a.probe_last - a.probe_first + 1;
+  const uint32_t ntiles = (nprobes + BM_TPB - 1) / BM_TPB;
+  const uint32_t grid = (ntiles + 7) / 8 * 8;
+  switch (a.words) {
+    case 1: hipLaunchKernelGGL(k_bitmap<1>, dim3(grid), dim3(BM_TPB), 0, st, a, ntiles); break;
+    case 2: hipLaunchKernelGGL(k_bitmap<2>, dim3(grid), dim3(BM_TPB), 0, st, a, ntiles); break;
+    case 4: hipLaunchKernelGGL(k_bitmap<4>, dim3(grid), dim3(BM_TPB), 0, st, a, ntiles); break;
+    default: throw Error{HS_E_INVALID, "bitmap width must be 64, 128 or 256 bits"};
+  }
+  HS_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace hs

@@ -1,0 +1,232 @@
+// Prefix-filter candidate generation fused with verification (the fgss
+// technique) for gfx950.
+//
+// Reference: include/hyset/device_algorithms.hpp:265-303 (prefix_filter:
+// one block per probe, global atomicAdd of shared-prefix counts into a dense
+// blockSize^2 matrix) followed by verification.hpp:60-138 (scan of the whole
+// matrix, sequential merge per cell).  The result set is the reference's:
+// pairs (r, s) inside the length window of r that share a probe-prefix token
+// of r with the index prefix of s and reach minoverlap(|r|, |s|).
+//
+// Design here:
+//   * a workgroup owns a tile of consecutive probes; their tokens are one
+//     contiguous CSR range and are staged in LDS with coalesced loads;
+//   * the length window of a probe is an id interval (sets are sorted by
+//     size), so each (probe, prefix token) "item" binary-searches its slice
+//     of the id-sorted posting list -- no length checks per posting;
+//   * items, then the posting entries of each batch of items ("visits"),
+//     are flattened with workgroup scans so every lane gets one visit;
+//   * a (r, s) pair reached through several prefix tokens is verified once:
+//     only the visit whose token is the first common token of r and s
+//     continues (that token always lies in both prefixes), which replaces
+//     the reference's dense count matrix by a merge that starts anyway;
+//   * the merge keeps the early-termination bound of verify_pair
+//     (verification.hpp:9-31) for failing pairs and runs to the end for
+//     passing ones, which yields the exact overlap;
+//   * passing pairs leave through one atomic per wave.
+#include <hip/hip_runtime.h>
+
+#include "device_common.h"
+#include "hs_internal.h"
+
+namespace hs {
+
+constexpr int FG_TPB = 256;          // threads per workgroup, also max probes per tile
+constexpr uint32_t FG_TOKCAP = 4096;  // staged probe tokens (16 KiB)
+constexpr uint32_t FG_ITEMCAP = 1024; // items per batch
+constexpr uint32_t NO_LDS = 0xFFFFFFFFu;
+
+// Verify (r, s) from the visit reached through r[j].  Returns true when this
+// visit is the canonical one and the pair passes; *ov receives the overlap,
+// *canon whether the visit was canonical.
+__device__ __forceinline__ bool verify_from_first_common(const uint32_t* r, uint32_t lr, const uint32_t* s,
+                                                         uint32_t ls, uint32_t j, uint32_t mo, uint32_t* ov,
+                                                         bool* canon) {
+  uint32_t i = 0, k = 0;
+  uint32_t a = r[0], b = s[0];
+  // r[j] is in s (it indexes s), so a common token is met at or before it.
+  while (a != b) {
+    bool adv_a = a < b;
+    i += adv_a;
+    k += !adv_a;
+    if (i >= lr || k >= ls || i > j) {
+      *canon = false;
+      return false;
+    }
+    if (adv_a) a = r[i];
+    else b = s[k];
+  }
+  if (i != j) {
+    *canon = false;
+    return false;
+  }
+  *canon = true;
+  uint32_t found = 1;
+  ++i;
+  ++k;
+  if (i < lr) a = r[i];
+  if (k < ls) b = s[k];
+  while (i < lr && k < ls) {
+    uint32_t rest = min(lr - i, ls - k);
+    if (found + rest < mo) return false;
+    bool eq = a == b;
+    bool adv_a = a <= b;
+    bool adv_b = a >= b;
+    found += eq;
+    i += adv_a;
+    k += adv_b;
+    if (adv_a && i < lr) a = r[i];
+    if (adv_b && k < ls) b = s[k];
+  }
+  *ov = found;
+  return found >= mo;
+}
+
+__global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint32_t ntiles) {
+  __shared__ uint32_t s_tok[FG_TOKCAP];
+  __shared__ uint32_t s_pstart[FG_TPB];
+  __shared__ uint32_t s_psize[FG_TPB];
+  __shared__ uint32_t s_plds[FG_TPB];
+  __shared__ uint32_t s_lo[FG_TPB];
+  __shared__ uint32_t s_hi[FG_TPB];
+  __shared__ uint32_t s_item_off[FG_TPB + 1];
+  __shared__ uint32_t s_it_a[FG_ITEMCAP];
+  __shared__ uint32_t s_it_pj[FG_ITEMCAP];
+  __shared__ uint32_t s_vis_off[FG_ITEMCAP + 1];
+  __shared__ uint32_t s_tmp[FG_TPB / WAVE + 1];
+  __shared__ unsigned long long s_stats[2];
+
+  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
+  if (tile == 0xFFFFFFFFu) return;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t p0 = A.probe_first + tile * ppt;
+  const uint32_t p1 = min(p0 + ppt, A.probe_last + 1);
+  const uint32_t np = p1 - p0;
+
+  if (tid < 2) s_stats[tid] = 0;
+
+  // ---- phase A: probe metadata, windows, LDS staging of probe tokens
+  const uint32_t base = A.probe.starts[p0];
+  const uint32_t tok_end = A.probe.starts[p1 - 1] + A.probe.sizes[p1 - 1];
+  const uint32_t ntok = tok_end - base;
+  uint32_t n_items = 0;
+  if (tid < np) {
+    const uint32_t p = p0 + tid;
+    const uint32_t l = A.probe.sizes[p];
+    const uint32_t st = A.probe.starts[p];
+    const uint32_t last = A.size_first_len - 1;
+    const uint32_t mins = min(A.bounds.minsize[l], last);
+    const uint32_t maxs1 = min(A.bounds.maxsize[l], last - 1) + 1;
+    uint32_t lo = max(A.size_first[mins], A.idx_first);
+    uint32_t hi = min(A.size_first[maxs1], A.idx_last + 1);
+    if (A.self) hi = min(hi, p);
+    s_pstart[tid] = st;
+    s_psize[tid] = l;
+    s_lo[tid] = lo;
+    s_hi[tid] = hi;
+    const uint32_t off = st - base;
+    s_plds[tid] = (off + l <= FG_TOKCAP) ? off : NO_LDS;
+    n_items = (lo < hi) ? A.bounds.maxprefix[l] : 0;
+  }
+  for (uint32_t i = tid; i < ntok && i < FG_TOKCAP; i += FG_TPB) s_tok[i] = A.probe.tokens[base + i];
+  uint32_t total_items;
+  uint32_t ioff = block_exclusive_scan<FG_TPB>(n_items, s_tmp, &total_items);
+  if (tid < np) s_item_off[tid] = ioff;
+  if (tid == 0) s_item_off[np] = total_items;
+  __syncthreads();
+
+  uint32_t my_cands = 0;
+
+  // ---- phase B: batches of items
+  for (uint32_t ib = 0; ib < total_items; ib += FG_ITEMCAP) {
+    const uint32_t nb = min(FG_ITEMCAP, total_items - ib);
+    // slice of the posting list for every item of the batch
+    for (uint32_t it = tid; it < FG_ITEMCAP; it += FG_TPB) {
+      uint32_t len = 0;
+      if (it < nb) {
+        const uint32_t item = ib + it;
+        const uint32_t pl = segment_of(s_item_off, np, item);
+        const uint32_t j = item - s_item_off[pl];
+        const uint32_t* r = (s_plds[pl] != NO_LDS) ? (s_tok + s_plds[pl]) : (A.probe.tokens + s_pstart[pl]);
+        const uint32_t token = r[j];
+        uint32_t a0 = 0;
+        if (token < A.index.universe) {
+          const uint32_t lb = A.index.offsets[token];
+          const uint32_t le = A.index.offsets[token + 1];
+          a0 = lower_bound_u32(A.index.set_ids, lb, le, s_lo[pl]);
+          const uint32_t b0 = lower_bound_u32(A.index.set_ids, a0, le, s_hi[pl]);
+          len = b0 - a0;
+        }
+        s_it_a[it] = a0;
+        s_it_pj[it] = pl | (j << 8);
+      }
+      s_vis_off[it] = len;
+    }
+    __syncthreads();
+    // exclusive scan of the slice lengths (4 per thread)
+    constexpr uint32_t PER = FG_ITEMCAP / FG_TPB;
+    uint32_t v[PER];
+    uint32_t local = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+      v[k] = s_vis_off[tid * PER + k];
+      local += v[k];
+    }
+    uint32_t total_vis;
+    uint32_t run = block_exclusive_scan<FG_TPB>(local, s_tmp, &total_vis);
+#pragma unroll
+    for (uint32_t k = 0; k < PER; ++k) {
+      s_vis_off[tid * PER + k] = run;
+      run += v[k];
+    }
+    if (tid == 0) s_vis_off[FG_ITEMCAP] = total_vis;
+    __syncthreads();
+    if (tid == 0) s_stats[0] += total_vis;
+
+    // one visit per lane
+    for (uint32_t vb = 0; vb < total_vis; vb += FG_TPB) {
+      const uint32_t vi = vb + tid;
+      bool pass = false, canon = false;
+      uint32_t rid = 0, sid = 0, ov = 0;
+      if (vi < total_vis) {
+        const uint32_t it = segment_of(s_vis_off, nb, vi);
+        const uint32_t k = vi - s_vis_off[it];
+        sid = A.index.set_ids[s_it_a[it] + k];
+        const uint32_t pj = s_it_pj[it];
+        const uint32_t pl = pj & 0xFFu;
+        const uint32_t j = pj >> 8;
+        rid = p0 + pl;
+        const uint32_t lr = s_psize[pl];
+        const uint32_t* r = (s_plds[pl] != NO_LDS) ? (s_tok + s_plds[pl]) : (A.probe.tokens + s_pstart[pl]);
+        const uint32_t ls = A.indexed.sizes[sid];
+        const uint32_t* s = A.indexed.tokens + A.indexed.starts[sid];
+        const uint32_t mo = dev_minoverlap(A.bounds, lr, ls);
+        pass = verify_from_first_common(r, lr, s, ls, j, mo, &ov, &canon);
+      }
+      my_cands += canon;
+      emit_pairs(pass, rid, sid, ov, A.out.r, A.out.s, A.out.ov, &A.out.counters[CNT_PAIRS], A.out.capacity);
+    }
+    __syncthreads();
+  }
+
+  // ---- statistics: one global atomic per workgroup and counter
+  my_cands = wave_sum(my_cands);
+  if (lane_id() == 0 && my_cands) atomicAdd(&s_stats[1], (unsigned long long)my_cands);
+  __syncthreads();
+  if (tid == 0) {
+    if (s_stats[0]) atomicAdd(&A.out.counters[CNT_VISITS], s_stats[0]);
+    if (s_stats[1]) atomicAdd(&A.out.counters[CNT_CANDS], s_stats[1]);
+  }
+}
+
+void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt) {
+  if (a.probe.n == 0 || a.probe_first > a.probe_last) return;
+  if (ppt == 0 || ppt > (uint32_t)FG_TPB) ppt = FG_TPB;
+  const uint32_t nprobes = a.probe_last - a.probe_first + 1;
+  const uint32_t ntiles = (nprobes + ppt - 1) / ppt;
+  const uint32_t grid = (ntiles + 7) / 8 * 8;
+  hipLaunchKernelGGL(k_fgss, dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+  HS_CHECK_HIP(hipGetLastError());
+}
+
+}  // namespace hs

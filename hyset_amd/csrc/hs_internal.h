@@ -1,0 +1,178 @@
+// Internal declarations shared by the host code and the HIP kernels of
+// libhyset_amd.so.  Nothing here crosses the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "hyset_amd.h"
+
+namespace hs {
+
+// ------------------------------------------------------------------ errors
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define HS_CHECK_HIP(expr)                                                                  \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      throw ::hs::Error{HS_E_DEVICE, std::string(#expr " failed: ") + hipGetErrorString(_e)}; \
+  } while (0)
+
+#define HS_REQUIRE(cond, code, text)                 \
+  do {                                               \
+    if (!(cond)) throw ::hs::Error{(code), (text)};  \
+  } while (0)
+
+// ----------------------------------------------------------- device memory
+
+// A growable device allocation.  Memory is only ever grown, never shrunk,
+// so steady-state joins allocate nothing.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t need);
+  void release();
+  template <typename T>
+  T* as() const { return static_cast<T*>(ptr); }
+  ~DevBuf() { release(); }
+};
+
+// ------------------------------------------------------------ bound tables
+
+// Integer bound tables for one (similarity, threshold, mode).  All entries
+// are computed on the host with the same expressions as the CPU join, then
+// used as plain integers on the device, so device and host can never round
+// differently.
+struct BoundTables {
+  int kind = HS_JACCARD;
+  int mode = HS_BOUNDS_REFERENCE;
+  double t = 0.0;
+  uint32_t L = 0;  // largest set size covered
+  std::vector<uint32_t> minsize, maxsize, maxprefix, midprefix;
+  // minoverlap before the min(l1,l2) clamp: keyed by l1+l2 for Jaccard and
+  // Dice, by l1*(L+1)+l2 for cosine (clamped already).
+  std::vector<uint32_t> mo;
+  void build(int kind, int mode, double t, uint32_t L);
+  uint32_t minoverlap(uint32_t l1, uint32_t l2) const;
+};
+
+// Device view of BoundTables (one allocation, sub-arrays).
+struct DevBounds {
+  const uint32_t* minsize;
+  const uint32_t* maxsize;
+  const uint32_t* maxprefix;
+  const uint32_t* midprefix;
+  const uint32_t* mo;
+  uint32_t L;
+  int kind;
+};
+
+__host__ __device__ inline uint32_t dev_minoverlap(const DevBounds& b, uint32_t l1, uint32_t l2) {
+  uint32_t m = l1 < l2 ? l1 : l2;
+  uint32_t v = (b.kind == HS_COSINE) ? b.mo[(size_t)l1 * (b.L + 1) + l2] : b.mo[l1 + l2];
+  return v < m ? v : m;
+}
+
+// Host-side bound evaluation (also used to fill the tables).
+uint32_t bound_minsize(int kind, int mode, double t, uint64_t p, uint64_t q, uint32_t l);
+uint32_t bound_maxsize(int kind, int mode, double t, uint64_t p, uint64_t q, uint32_t l);
+uint32_t bound_minoverlap_raw(int kind, int mode, double t, uint64_t p, uint64_t q, uint32_t l1, uint32_t l2);
+bool threshold_rational(double t, uint64_t* p, uint64_t* q);
+
+// --------------------------------------------------------------- kernels
+
+struct DevColl {
+  const uint32_t* tokens;
+  const uint32_t* starts;
+  const uint32_t* sizes;
+  uint32_t n;
+};
+
+struct DevIndex {
+  const uint32_t* offsets;  // universe + 1
+  const uint32_t* set_ids;  // entries, sorted by set id inside every list
+  uint32_t universe;
+};
+
+// Output cursor block: [0] pairs found, [1] posting visits, [2] canonical
+// candidates, [3] spare.  64-bit counters.
+enum { CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_WORDS = 4 };
+
+struct DevOut {
+  uint32_t* r;
+  uint32_t* s;
+  uint32_t* ov;
+  unsigned long long* counters;
+  uint64_t capacity;
+};
+
+// Launch helpers (defined in the .hip files).
+void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
+void launch_fill_u64(hipStream_t st, unsigned long long* p, size_t n, unsigned long long v);
+void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* out2, size_t n,
+                           DevBuf& scratch);
+
+struct IndexBuild {
+  DevBuf offsets, cursor, set_ids, long_queue, scan_scratch;
+  uint32_t universe = 0;
+  uint64_t entries = 0;
+};
+// Builds the inverted index over the prefix entries of sets [first, last]
+// of `c`; prefix_len is indexed by set size.  `entries` must be the exact
+// number of prefix entries (computed on the host).
+void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
+                 uint32_t universe, uint64_t entries, IndexBuild& ib);
+
+struct FgssArgs {
+  DevColl probe;
+  DevColl indexed;
+  DevIndex index;
+  DevBounds bounds;
+  const uint32_t* size_first;  // indexed collection: first id with size >= l, l in [0, maxlen+1]
+  uint32_t size_first_len;     // maxlen + 2
+  uint32_t probe_first, probe_last;  // inclusive
+  uint32_t idx_first, idx_last;      // inclusive
+  int self;                          // 1: restrict to s < r
+  DevOut out;
+};
+void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile);
+
+struct BitmapArgs {
+  DevColl probe;
+  DevColl indexed;
+  DevBounds bounds;
+  const uint64_t* probe_sigs;
+  const uint64_t* indexed_sigs;
+  const uint32_t* size_first;
+  uint32_t size_first_len;
+  uint32_t words;  // 1, 2 or 4 (64, 128, 256 bits)
+  uint32_t probe_first, probe_last;
+  uint32_t idx_first, idx_last;
+  int self;
+  DevOut out;
+};
+void launch_signatures(hipStream_t st, const DevColl& c, uint32_t words, uint64_t* sigs);
+void launch_bitmap(hipStream_t st, const BitmapArgs& a);
+
+struct VerifyArgs {
+  DevColl probe;
+  DevColl indexed;
+  DevBounds bounds;
+  const uint32_t* probes;      // probe id per batch row
+  const uint32_t* offsets;     // inclusive end of each row's candidates
+  const uint32_t* candidates;  // indexed ids
+  uint32_t n_probes;
+  uint64_t n_candidates;
+  DevOut out;
+};
+void launch_verify(hipStream_t st, const VerifyArgs& a, int scenario);
+
+}  // namespace hs

@@ -1,0 +1,217 @@
+/*
+ * hyset_amd -- exact threshold set-similarity join on MI355X (gfx950).
+ *
+ * C ABI of libhyset_amd.so.  Plain pointers and sizes only; every call
+ * returns HS_OK (0) or a negative HS_E_* code, and no C++ exception crosses
+ * the boundary.  One hs_ctx per host thread; each context owns one HIP
+ * stream.  Device-resident collections may be shared read-only between
+ * contexts on the same device.
+ *
+ * The entry points replace the GPU partition of the reference's hybrid join
+ * (paths relative to the reference repository):
+ *   hs_create / hs_destroy             handler constructors / destructors,
+ *                                      include/hyset/device_algorithms.hpp:138-171,
+ *                                      :322-348, :249-259, :434-444
+ *   hs_upload_collection               hyset::collection::device_collection,
+ *                                      include/hyset/collection.hpp:54-71
+ *   hs_set_threshold                   the `threshold` argument of every handler
+ *                                      and the bounds of similarity.hpp:11-55
+ *   hs_join                            bitmap::handler::join / fgssjoin::handler::join
+ *                                      for block and partition pairs,
+ *                                      device_algorithms.hpp:177-247, :363-432
+ *   hs_verify_candidates               hybrid::device_handler::addCandidates + flush,
+ *                                      include/hyset/hybrid_algorithms.hpp:71-200
+ *   hs_result                          output::count_handler / pairs_handler,
+ *                                      include/hyset/output.hpp:19-37
+ */
+#ifndef HYSET_AMD_H
+#define HYSET_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ----------------------------------------------------------- return codes */
+enum {
+  HS_OK = 0,
+  HS_E_INVALID = -1,     /* bad argument or malformed collection */
+  HS_E_OOM = -2,         /* host or device allocation failed */
+  HS_E_OVERFLOW = -3,    /* result capacity too small; hs_result.count holds the size needed */
+  HS_E_DEVICE = -4,      /* HIP runtime error; see hs_last_error */
+  HS_E_UNSUPPORTED = -5, /* combination not supported (e.g. set size above the bound tables) */
+  HS_E_BUSY = -6         /* admission refused: the request does not fit the memory budget now */
+};
+
+/* ------------------------------------------------------ host collections */
+
+/* A collection in CSR form: set i holds tokens[starts[i] .. starts[i]+sizes[i]),
+ * strictly ascending; sets ordered by size ascending.  Token ids are dense
+ * 0..universe-1 ranked by ascending global frequency.  Arrays are malloc'd by
+ * the generator and released with hs_host_collection_free. */
+typedef struct {
+  uint32_t n;
+  uint64_t n_tokens;
+  uint32_t universe; /* max token + 1 */
+  uint32_t* tokens;
+  uint32_t* starts;
+  uint32_t* sizes;
+} hs_host_collection;
+
+enum { HS_SIZE_UNIFORM = 0, HS_SIZE_LOGNORMAL = 1 };
+
+/* Synthetic-input recipe: n sets, sizes uniform in [size_lo, size_hi] or
+ * round(lognormal(median, sigma)) clipped to [size_lo, size_hi]; tokens drawn
+ * from Zipf(zipf_s) over ranks 1..universe without repetition inside a set;
+ * then re-ranked by ascending frequency (ids from 1), tokens sorted inside a
+ * set and sets sorted by (size, tokens).  xoshiro256** seeded by splitmix64. */
+typedef struct {
+  uint32_t n;
+  int32_t size_dist;
+  double size_median;
+  double size_sigma;
+  uint32_t size_lo;
+  uint32_t size_hi;
+  uint32_t universe;
+  double zipf_s;
+  uint64_t seed;
+} hs_gen_params;
+
+int hs_generate_self(const hs_gen_params* p, hs_host_collection* out);
+
+/* R and S with a shared token ranking; the first round(dup_frac*|S|) raw S sets
+ * are copies of random R sets with round(f*|r|) edits, f ~ U[edit_lo, edit_hi],
+ * each edit a substitution, deletion or insertion with equal probability. */
+int hs_generate_rs(const hs_gen_params* r, const hs_gen_params* s, double dup_frac, double edit_lo,
+                   double edit_hi, hs_host_collection* out_r, hs_host_collection* out_s);
+
+/* An indexed collection and a query collection drawn from the same token
+ * space; tokens are ranked by their frequency in the indexed collection and
+ * query-only tokens get ids above every indexed token. */
+int hs_generate_queries(const hs_gen_params* index_params, const hs_gen_params* query_params,
+                        hs_host_collection* out_index, hs_host_collection* out_queries);
+
+void hs_host_collection_free(hs_host_collection* c);
+
+/* Reads the reference's text format (include/hyset/collection.hpp:92-177):
+ * one set per line, space-separated token ids, empty lines skipped. */
+int hs_read_collection_text(const char* path, hs_host_collection* out);
+int hs_write_collection_text(const char* path, const uint32_t* tokens, const uint32_t* starts,
+                             const uint32_t* sizes, uint32_t n);
+
+/* Writes pairs as "r s\n" lines (include/hyset/output.hpp:39-76), or
+ * "r s overlap\n" when overlap is not NULL. */
+int hs_write_pairs_text(const char* path, const uint32_t* r, const uint32_t* s, const uint32_t* overlap,
+                        uint64_t count);
+
+/* --------------------------------------------------------- join settings */
+
+enum { HS_JACCARD = 0, HS_COSINE = 1, HS_DICE = 2 };
+
+/* HS_BOUNDS_REFERENCE: Jaccard bounds are the reference's IEEE-double
+ * expressions (include/hyset/similarity.hpp:40-50), so pairs come out
+ * exactly as the reference's cpu_single emits them, including at the sums
+ * where the double quotient rounds up.  HS_BOUNDS_EXACT: the threshold is
+ * read as the rational given by its shortest decimal form and every bound is
+ * integer-exact.  Cosine and Dice always use the exact form. */
+enum { HS_BOUNDS_REFERENCE = 0, HS_BOUNDS_EXACT = 1 };
+
+/* Technique selection of the reference's drivers (src/queue.cu:181-214):
+ * bitmap signatures, prefix filter (fgssjoin), or CPU filtering with device
+ * verification (hybrid). */
+enum { HS_ALGO_FGSS = 0, HS_ALGO_BITMAP = 1, HS_ALGO_HYBRID = 2 };
+
+typedef struct hs_ctx hs_ctx;
+typedef struct hs_coll hs_coll;
+
+/* Inclusive set-id interval, like a block's startID/endID
+ * (include/hyset/structs.hpp:10-54).  first > last is an empty range. */
+typedef struct {
+  uint32_t first;
+  uint32_t last;
+} hs_range;
+
+/* Tuning knobs; zero selects the default. */
+typedef struct {
+  uint32_t bitmap_bits;       /* bitmap: signature width 64, 128 or 256 (default 64) */
+  uint32_t probes_per_tile;   /* fgss: probes per workgroup, 1..256 (default 256) */
+  uint32_t verify_scenario;   /* hybrid: 1 lane per candidate, 2 wave merge-path per
+                                 candidate, 0 pick by set size (default 0) */
+  uint32_t host_threads;      /* hybrid: CPU filter threads (default 1) */
+  uint64_t batch_candidates;  /* hybrid: candidates per device batch (default 1<<22) */
+  uint32_t flags;             /* reserved, must be 0 */
+} hs_tune;
+
+/* Result sink.  count is always written.  With count_only == 0 the pairs
+ * (r = probe id, s = indexed id, overlap = |r n s|) are copied into the
+ * caller's arrays when count <= capacity, otherwise HS_E_OVERFLOW is
+ * returned and nothing is copied.  With count_only != 0 the pairs stay on
+ * the device until the next join and can be fetched with
+ * hs_copy_last_result.  Pair order is unspecified (the reference emits in
+ * atomic order as well); sort for comparisons.  overlap may be NULL. */
+typedef struct {
+  uint64_t count;
+  uint64_t capacity;
+  uint32_t* r;
+  uint32_t* s;
+  uint32_t* overlap;
+  int32_t count_only;
+} hs_result;
+
+/* Timings (ms, HIP events on the context stream) and work counters of the
+ * last join on a context. */
+typedef struct {
+  double index_ms;
+  double filter_ms; /* candidate generation + verification (fused for fgss/bitmap) */
+  double transfer_ms;
+  double total_ms;
+  double host_filter_ms; /* hybrid: CPU filtering wall time */
+  uint64_t index_entries;
+  uint64_t posting_visits;
+  uint64_t candidates;
+  uint64_t pairs;
+  uint64_t probe_prefix_tokens;
+} hs_stats;
+
+/* ------------------------------------------------------------- contexts */
+
+int hs_create(int device, hs_ctx** out);
+void hs_destroy(hs_ctx* ctx);
+const char* hs_last_error(const hs_ctx* ctx);
+
+/* Copies a collection to device memory.  sizes[i] >= 1; starts may be NULL
+ * for a dense CSR (starts[i] = sum of earlier sizes).  Sets must be sorted
+ * by size and tokens ascending inside a set (checked). */
+int hs_upload_collection(hs_ctx* ctx, const uint32_t* tokens, const uint32_t* starts, const uint32_t* sizes,
+                         uint32_t n, uint64_t n_tokens, hs_coll** out);
+void hs_free_collection(hs_coll* coll);
+
+int hs_set_threshold(hs_ctx* ctx, int32_t sim, double threshold, int32_t bounds_mode);
+
+/* Joins probe sets of `probe` (within probe_range) against the sets of
+ * `indexed` (within indexed_range).  indexed == probe is a self-join and
+ * keeps pairs with s < r only; otherwise every (r, s) is considered (R-S
+ * join, index prefix = maxprefix).  NULL ranges mean the whole collection. */
+int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* indexed_range,
+            const hs_range* probe_range, int32_t algo, const hs_tune* tune, hs_result* result);
+
+/* Verifies one CSR batch of candidates: row i is probe probes[i] with
+ * candidates[(i ? offsets[i-1] : 0) .. offsets[i]) of `indexed`
+ * (inclusive-end offsets, as hybrid_algorithms.hpp:71-105 builds them).
+ * scenario: 1 lane per candidate, 2 merge-path wave per candidate, 0 auto. */
+int hs_verify_candidates(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const uint32_t* probes,
+                         const uint32_t* offsets, const uint32_t* candidates, uint32_t n_probes,
+                         int32_t scenario, hs_result* result);
+
+/* Copies the pairs of the last join into result (count_only ignored). */
+int hs_copy_last_result(hs_ctx* ctx, hs_result* result);
+
+int hs_get_stats(const hs_ctx* ctx, hs_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HYSET_AMD_H */
