@@ -1,0 +1,105 @@
+"""GPU parity: every technique returns exactly the oracle's sorted (r, s,
+overlap) triples (the reference's cpu_single result plus exact overlaps)."""
+import numpy as np
+import pytest
+
+import hyset_amd as H
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _check(got, want, what):
+    r, s, ov = H.sort_pairs(*got)
+    assert r.shape[0] == want.count, f"{what}: {r.shape[0]} pairs, oracle {want.count}"
+    assert np.array_equal(r, want.r), what
+    assert np.array_equal(s, want.s), what
+    assert np.array_equal(ov, want.overlap), what
+
+
+@pytest.fixture(scope="module")
+def small(ctx):
+    c = H.generate_self(H.gen_params(3000, size_dist=H.SIZE_UNIFORM, lo=2, hi=40, universe=800, zipf=1.0,
+                                     seed=5))
+    return c, ctx.upload(c)
+
+
+@pytest.mark.parametrize("t", [0.6, 0.7, 0.75, 0.8, 0.9])
+@pytest.mark.parametrize("sim", ["jaccard", "cosine", "dice"])
+def test_fgss_small(ctx, small, t, sim):
+    c, d = small
+    ctx.set_threshold(t, sim)
+    want = O.self_join(c, t, sim)
+    _check(ctx.join(d, algo="fgss"), want, f"fgss {sim} {t}")
+
+
+@pytest.mark.parametrize("bits", [64, 128, 256])
+@pytest.mark.parametrize("t", [0.6, 0.8, 0.9])
+def test_bitmap_small(ctx, small, t, bits):
+    c, d = small
+    ctx.set_threshold(t)
+    want = O.self_join(c, t)
+    _check(ctx.join(d, algo="bitmap", tune=H.Tune(bitmap_bits=bits)), want, f"bitmap{bits} {t}")
+
+
+@pytest.mark.parametrize("scenario", [1, 2])
+@pytest.mark.parametrize("threads", [1, 3])
+def test_hybrid_small(ctx, small, scenario, threads):
+    c, d = small
+    ctx.set_threshold(0.7)
+    want = O.self_join(c, 0.7)
+    tune = H.Tune(verify_scenario=scenario, host_threads=threads, batch_candidates=5000)
+    _check(ctx.join(d, algo="hybrid", tune=tune), want, f"hybrid s{scenario} x{threads}")
+
+
+def test_block_pairs(ctx, small):
+    """join(block*, block*): index over one block, probes from a later block."""
+    c, d = small
+    ctx.set_threshold(0.6)
+    for ib, pb in [((0, 999), (1000, 1999)), ((1000, 1999), (1000, 1999)), ((500, 2500), (2000, 2999))]:
+        want = O.self_join_blocks(c, 0.6, ib, pb)
+        for algo in ("fgss", "bitmap", "hybrid"):
+            _check(ctx.join(d, algo=algo, indexed_range=ib, probe_range=pb), want, f"{algo} {ib} {pb}")
+
+
+@pytest.mark.parametrize("t", [0.7, 0.8, 0.9])
+def test_fgss_config2_full(ctx, t):
+    """1,000,000 sets, lognormal sizes (median 10, sigma 0.6, [2,200]),
+    Zipf(1.1) over 1M tokens, seed 1: bit-exact against the oracle."""
+    c = H.generate_self(H.gen_params(1_000_000))
+    d = ctx.upload(c)
+    try:
+        ctx.set_threshold(t)
+        want = O.cpu_multi(c, t, 8)
+        _check(ctx.join(d, algo="fgss"), want, f"fgss 1M t={t}")
+        st = ctx.stats()
+        assert st["candidates"] == want.candidates
+        assert st["posting_visits"] == want.posting_visits
+    finally:
+        d.free()
+
+
+def test_verify_candidates(ctx, small):
+    """hs_verify_candidates on a CSR batch (duplicates, ids on both sides of
+    the probe) keeps exactly the candidates with |r n s| >= minoverlap."""
+    c, d = small
+    t = 0.5
+    ctx.set_threshold(t)
+    rng = np.random.default_rng(0)
+    probes = np.arange(100, 3000, 7, dtype=np.int64)
+    cands, offs = [], []
+    for p in probes:
+        cands.extend(rng.integers(max(0, p - 60), min(c.n, p + 20), rng.integers(0, 40)).tolist())
+        offs.append(len(cands))
+    exp = []
+    for p, row in zip(probes, np.split(np.array(cands, dtype=np.uint32), offs[:-1])):
+        for s in row:
+            ov = O.intersect(c.set(p), c.set(s))
+            if ov >= O.bounds("jaccard", t, int(c.sizes[p]), int(c.sizes[s]))[2]:
+                exp.append((int(p), int(s), ov))
+    exp.sort()
+    assert len(exp) > 50
+    for scen in (1, 2):
+        r, s, ov = H.sort_pairs(*ctx.verify_candidates(d, d, probes, np.array(offs, dtype=np.uint32),
+                                                        np.array(cands, dtype=np.uint32), scen))
+        assert list(zip(r.tolist(), s.tolist(), ov.tolist())) == exp, f"scenario {scen}"
