@@ -1,0 +1,188 @@
+"""Benchmark: exact self-join of 1,000,000 sets at Jaccard >= 0.8 on one MI355X.
+
+Workload (the headline configuration): 1M sets, sizes round(lognormal(median
+10, sigma 0.6)) clipped to [2, 200], tokens Zipf(1.1) over 1M ranks re-ranked
+by ascending frequency, sets sorted by size, seed 1; Jaccard threshold 0.8
+with the reference's double-precision bounds; prefix-filter (fgss)
+technique.  A step is one full join from the device-resident CSR collection:
+index build + candidate generation + verification, result pairs left in HBM
+(their count read back).  Prints one JSON line on stdout; progress goes to
+stderr.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, published
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def fgss_bytes(st: dict, n_probes: int, n_probe_tokens: int) -> int:
+    """Algorithmic HBM bytes of one k_fgss launch:
+    probe CSR (4 B per token + starts and sizes), per probe-prefix item the two
+    list offsets, 4 B per posting visited, per verified candidate its start and
+    size plus 4 B per token, 12 B per result triple."""
+    return (4 * n_probe_tokens + 8 * n_probes + 8 * st["probe_prefix_tokens"] + 4 * st["posting_visits"]
+            + 8 * st["candidates"] + 4 * st["candidate_tokens"] + 12 * st["pairs"])
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--quick", action="store_true", help="headline only: no side measurements, no CPU baseline")
+    ap.add_argument("--threshold", type=float, default=0.8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import numpy as np
+
+    import hyset_amd as H
+
+    t = args.threshold
+    log("generating 1M-set collection")
+    g0 = time.perf_counter()
+    coll = H.generate_self(H.gen_params(1_000_000, median=10, sigma=0.6, lo=2, hi=200, universe=1_000_000,
+                                        zipf=1.1, seed=1))
+    log(f"generated n={coll.n} tokens={coll.tokens.shape[0]} in {time.perf_counter() - g0:.1f}s")
+    ctx = H.Context(0)
+    d = ctx.upload(coll)
+    ctx.set_threshold(t)
+    copy_gbps = ctx.copy_bandwidth(1 << 30, 10)
+    log(f"device copy kernel: {copy_gbps:.0f} GB/s")
+
+    for _ in range(args.warmup):
+        ctx.join(d, algo="fgss", count_only=True)
+    steps = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        pairs = ctx.join(d, algo="fgss", count_only=True)
+        steps.append(ctx.stats())
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    st = steps[-1]
+    kern_ms = statistics.median(s["kernel_ms"] for s in steps)
+    dev_ms = statistics.median(s["total_ms"] for s in steps)
+    idx_ms = statistics.median(s["index_ms"] for s in steps)
+    log(f"fgss t={t}: {pairs} pairs, {ms_per_step:.3f} ms/step wall, device {dev_ms:.3f} ms "
+        f"(index {idx_ms:.3f}, k_fgss {kern_ms:.3f})")
+
+    # with the explicit device-to-host copy of the pairs
+    d2h = []
+    for _ in range(5):
+        a = time.perf_counter()
+        r, s, ov = ctx.join(d, algo="fgss")
+        d2h.append((time.perf_counter() - a) * 1e3)
+    ms_with_d2h = statistics.median(d2h)
+
+    alg = fgss_bytes(st, coll.n, int(coll.tokens.shape[0]))
+    achieved = alg / (kern_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = ROOT / "profiles" / "fgss_1M_t0.8_traffic.json"
+    if tfile.exists():
+        traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+
+    extra = {"join_ms_device": round(dev_ms, 4), "index_ms": round(idx_ms, 4), "k_fgss_ms": round(kern_ms, 4),
+             "join_ms_with_d2h": round(ms_with_d2h, 4), "pairs": int(pairs),
+             "candidates": int(st["candidates"]), "posting_visits": int(st["posting_visits"]),
+             "index_entries": int(st["index_entries"]), "probe_prefix_tokens": int(st["probe_prefix_tokens"]),
+             "hbm_copy_kernel_gbps": round(copy_gbps, 1)}
+
+    cpu = None
+    if not args.quick:
+        import oracle as O
+        # parity of the measured result against the oracle (cpu_single restated)
+        times = []
+        for _ in range(3):
+            a = time.perf_counter()
+            want = O.self_join(coll, t)
+            times.append(time.perf_counter() - a)
+        r, s, ov = H.sort_pairs(r, s, ov)
+        extra["parity_vs_oracle"] = bool(np.array_equal(r, want.r) and np.array_equal(s, want.s)
+                                         and np.array_equal(ov, want.overlap))
+        cpu_s = statistics.median(times)
+        cpu = {"value": round(want.count / cpu_s, 1), "unit": "pairs/s", "cores": 1, "kind": "port",
+               "sample": f"full workload: cpu_single AllPairs restated in C (index build + join), "
+                         f"1M sets, t={t}, median of 3 = {cpu_s * 1e3:.1f} ms"}
+        ncores = len(os.sched_getaffinity(0))
+        a = time.perf_counter()
+        O.cpu_multi(coll, t, ncores)
+        extra["cpu_multi_ms"] = round((time.perf_counter() - a) * 1e3, 1)
+        extra["cpu_multi_threads"] = ncores
+        extra["cpu_single_ms"] = round(cpu_s * 1e3, 1)
+        log(f"oracle: {want.count} pairs in {cpu_s * 1e3:.0f} ms (1 core); parity={extra['parity_vs_oracle']}")
+
+        # the other thresholds and techniques of the configuration
+        for tt in (0.6, 0.7, 0.9):
+            ctx.set_threshold(tt)
+            ctx.join(d, algo="fgss", count_only=True)
+            ms = []
+            for _ in range(3):
+                c = ctx.join(d, algo="fgss", count_only=True)
+                ms.append(ctx.stats()["total_ms"])
+            extra[f"fgss_t{tt}_ms"] = round(statistics.median(ms), 3)
+            extra[f"fgss_t{tt}_pairs"] = c
+            log(f"fgss t={tt}: {c} pairs {statistics.median(ms):.3f} ms")
+        ctx.set_threshold(t)
+        for bits in (64, 128, 256):
+            tune = H.Tune(bitmap_bits=bits)
+            c = ctx.join(d, algo="bitmap", tune=tune, count_only=True)
+            sb = ctx.stats()
+            extra[f"bitmap{bits}_ms"] = round(sb["total_ms"], 3)
+            extra[f"bitmap{bits}_pair_tests"] = int(sb["posting_visits"])
+            extra[f"bitmap{bits}_survivors"] = int(sb["candidates"])
+            assert c == pairs
+            log(f"bitmap{bits}: {sb['total_ms']:.2f} ms, {sb['posting_visits']} tests, {sb['candidates']} survivors")
+        a = time.perf_counter()
+        c = ctx.join(d, algo="hybrid", tune=H.Tune(host_threads=min(16, ncores)), count_only=True)
+        extra["hybrid_ms_wall"] = round((time.perf_counter() - a) * 1e3, 2)
+        assert c == pairs
+        log(f"hybrid ({min(16, ncores)} host threads): {extra['hybrid_ms_wall']} ms")
+
+    out = {
+        "metric": "pairs/s (1M-set Jaccard>=0.8 self-join, fgss)",
+        "value": round(pairs / (ms_per_step * 1e-3), 1),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (Zipf/lognormal generator, seed 1)",
+        "config": {"workload": "self-join, 1,000,000 sets, lognormal sizes median 10 sigma 0.6 in [2,200], "
+                               "Zipf(1.1) over 1M tokens, Jaccard 0.8 (reference double bounds), fgss",
+                   "sets": coll.n, "tokens": int(coll.tokens.shape[0]), "threshold": t, "algorithm": "fgss"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
+                     "kernel": "k_fgss", "algorithmic_bytes": int(alg)},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

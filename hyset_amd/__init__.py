@@ -62,9 +62,10 @@ class Result(C.Structure):
 
 class Stats(C.Structure):
     _fields_ = [("index_ms", C.c_double), ("filter_ms", C.c_double), ("transfer_ms", C.c_double),
-                ("total_ms", C.c_double), ("host_filter_ms", C.c_double),
+                ("total_ms", C.c_double), ("host_filter_ms", C.c_double), ("kernel_ms", C.c_double),
                 ("index_entries", C.c_uint64), ("posting_visits", C.c_uint64),
-                ("candidates", C.c_uint64), ("pairs", C.c_uint64), ("probe_prefix_tokens", C.c_uint64)]
+                ("candidates", C.c_uint64), ("pairs", C.c_uint64), ("probe_prefix_tokens", C.c_uint64),
+                ("candidate_tokens", C.c_uint64)]
 
 
 def _sig(name, restype, *argtypes):
@@ -96,6 +97,19 @@ _sig("hs_verify_candidates", C.c_int, _vp, _vp, _vp, _u32p, _u32p, _u32p, C.c_ui
      C.POINTER(Result))
 _sig("hs_copy_last_result", C.c_int, _vp, C.POINTER(Result))
 _sig("hs_get_stats", C.c_int, _vp, C.POINTER(Stats))
+_sig("hs_copy_bandwidth", C.c_int, _vp, C.c_uint64, C.c_int32, C.POINTER(C.c_double))
+_sig("hs_bounds", C.c_int, C.c_int32, C.c_double, C.c_int32, C.c_uint32, C.c_uint32, _u32p)
+
+
+def bounds(sim, t, l1, l2=None, mode=BOUNDS_REFERENCE):
+    """(minsize, maxsize, minoverlap(l1,l2), maxprefix, midprefix) of l1 as the device tables hold them."""
+    l2 = l1 if l2 is None else l2
+    out = np.zeros(5, dtype=np.uint32)
+    kind = SIMS[sim] if isinstance(sim, str) else int(sim)
+    rc = _lib.hs_bounds(kind, float(t), int(mode), l1, l2, _ptr(out))
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_bounds")
+    return tuple(int(x) for x in out)
 
 
 class HysetError(RuntimeError):
@@ -271,6 +285,12 @@ class Context:
         st = Stats()
         _lib.hs_get_stats(self._h, C.byref(st))
         return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def copy_bandwidth(self, nbytes=1 << 30, reps=10) -> float:
+        """GB/s (read + write) of a plain device copy kernel."""
+        g = C.c_double()
+        self._check(_lib.hs_copy_bandwidth(self._h, nbytes, reps, C.byref(g)), "hs_copy_bandwidth")
+        return g.value
 
     def close(self):
         for c in self._colls:

@@ -585,8 +585,12 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     unsigned long long cnt[hs::CNT_WORDS];
     for (int attempt = 0; attempt < 2; ++attempt) {
       reset_counters(ctx);
+      if (algo == HS_ALGO_FGSS && !range_empty(ir))  // index size, read back with the counters
+        HS_CHECK_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + hs::CNT_ENTRIES,
+                                    ctx->ib.offsets.as<uint32_t>() + indexed->universe, 4,
+                                    hipMemcpyDeviceToDevice, st));
       if (range_empty(pr) || range_empty(ir)) {
-        // nothing to join
+        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
       } else if (algo == HS_ALGO_FGSS) {
         hs::FgssArgs a{};
         a.probe = probe->dev();
@@ -601,6 +605,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         a.idx_last = ir.last;
         a.self = self;
         a.out = dev_out(ctx);
+        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
         hs::launch_fgss(st, a, tune.probes_per_tile);
       } else if (algo == HS_ALGO_BITMAP) {
         const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
@@ -631,8 +636,10 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         b.idx_last = ir.last;
         b.self = self;
         b.out = dev_out(ctx);
+        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
         hs::launch_bitmap(st, b);
       } else {
+        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
         run_hybrid(ctx, indexed, probe, self, ir, pr, tune);
       }
       HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
@@ -655,11 +662,15 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     ctx->stats.filter_ms = elapsed(ctx->ev[1], ctx->ev[2]);
     ctx->stats.transfer_ms = elapsed(ctx->ev[3], ctx->ev[4]);
     ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
+    ctx->stats.kernel_ms = elapsed(ctx->ev[5], ctx->ev[2]);
     ctx->stats.pairs = count;
     if (algo != HS_ALGO_HYBRID) {
       ctx->stats.posting_visits = cnt[hs::CNT_VISITS];
       ctx->stats.candidates = cnt[hs::CNT_CANDS];
+      ctx->stats.candidate_tokens = cnt[hs::CNT_CAND_TOKENS];
+      ctx->stats.probe_prefix_tokens = cnt[hs::CNT_ITEMS];
     }
+    if (algo == HS_ALGO_FGSS) ctx->stats.index_entries = cnt[hs::CNT_ENTRIES];
   });
 }
 
@@ -701,6 +712,7 @@ int hs_verify_candidates(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const ui
     unsigned long long cnt[hs::CNT_WORDS];
     for (int attempt = 0; attempt < 2; ++attempt) {
       reset_counters(ctx);
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
       hs::VerifyArgs va{probe->dev(), indexed->dev(), ctx->dev_bounds, sl.d_probes.as<uint32_t>(),
                         sl.d_offsets.as<uint32_t>(), sl.d_cands.as<uint32_t>(), n_probes, nc, dev_out(ctx)};
       hs::launch_verify(st, va, scen);
@@ -721,6 +733,7 @@ int hs_verify_candidates(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const ui
     HS_CHECK_HIP(hipEventRecord(ctx->ev[4], st));
     HS_CHECK_HIP(hipEventSynchronize(ctx->ev[4]));
     ctx->stats.filter_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+    ctx->stats.kernel_ms = elapsed(ctx->ev[5], ctx->ev[2]);
     ctx->stats.transfer_ms = elapsed(ctx->ev[3], ctx->ev[4]);
     ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
     ctx->stats.pairs = count;
@@ -742,6 +755,38 @@ int hs_get_stats(const hs_ctx* ctx, hs_stats* out) {
   if (!ctx || !out) return HS_E_INVALID;
   *out = ctx->stats;
   return HS_OK;
+}
+
+int hs_bounds(int32_t sim, double threshold, int32_t bounds_mode, uint32_t l1, uint32_t l2, uint32_t out[5]) {
+  return guarded(nullptr, [&] {
+    HS_REQUIRE(out, HS_E_INVALID, "NULL output");
+    hs::BoundTables T;
+    T.build(sim, bounds_mode, threshold, std::max(l1, l2) + 1);
+    out[0] = T.minsize[l1];
+    out[1] = T.maxsize[l1];
+    out[2] = T.minoverlap(l1, l2);
+    out[3] = T.maxprefix[l1];
+    out[4] = T.midprefix[l1];
+  });
+}
+
+int hs_copy_bandwidth(hs_ctx* ctx, uint64_t bytes, int32_t reps, double* gbps) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    HS_REQUIRE(gbps && reps > 0 && bytes >= 16, HS_E_INVALID, "bad copy-bandwidth arguments");
+    bytes &= ~(uint64_t)15;
+    DevBuf a, b;
+    a.ensure(bytes);
+    b.ensure(bytes);
+    hs::launch_fill_u32(ctx->stream, a.as<uint32_t>(), bytes / 4, 1u);
+    hs::launch_copy(ctx->stream, a.ptr, b.ptr, bytes);  // warm-up
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int i = 0; i < reps; ++i) hs::launch_copy(ctx->stream, (i & 1) ? b.ptr : a.ptr, (i & 1) ? a.ptr : b.ptr, bytes);
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    HS_CHECK_HIP(hipEventSynchronize(ctx->ev[1]));
+    const double ms = elapsed(ctx->ev[0], ctx->ev[1]);
+    *gbps = ms > 0 ? 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9 : 0.0;
+  });
 }
 
 }  // extern "C"

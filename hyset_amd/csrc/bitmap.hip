@@ -85,14 +85,14 @@ __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles
   __shared__ uint64_t s_sig[BM_CHUNK * W];
   __shared__ uint32_t s_size[BM_CHUNK];
   __shared__ uint32_t s_red[2 * (BM_TPB / WAVE)];
-  __shared__ unsigned long long s_stats[1];
+  __shared__ unsigned long long s_stats[2];
 
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
   if (tile == 0xFFFFFFFFu) return;
   const uint32_t tid = threadIdx.x;
   const uint32_t p = A.probe_first + tile * BM_TPB + tid;
   const bool valid = p <= A.probe_last;
-  if (tid == 0) s_stats[0] = 0;
+  if (tid < 2) s_stats[tid] = 0;
 
   uint64_t sig[W];
   uint32_t lr = 0, lo = 0xFFFFFFFFu, hi = 0;
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles
   uint32_t cur_ls = 0xFFFFFFFFu;
   int32_t slack = -1;  // |r| + |s| - 2*minoverlap for the current |s|
   uint32_t mo_cur = 0;
-  uint32_t survivors = 0;
+  uint32_t survivors = 0, tests = 0;
 
   for (uint32_t c0 = ulo; c0 < uhi; c0 += BM_CHUNK) {
     const uint32_t cn = min(BM_CHUNK, uhi - c0);
@@ -141,6 +141,7 @@ __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles
     for (uint32_t i = tid; i < cn; i += BM_TPB) s_size[i] = A.indexed.sizes[c0 + i];
     __syncthreads();
     const uint32_t b = max(lo, c0), e = min(hi, c0 + cn);
+    tests += e > b ? e - b : 0u;
     for (uint32_t sidx = b; sidx < e; ++sidx) {
       const uint32_t q = sidx - c0;
       const uint32_t ls = s_size[q];
@@ -169,9 +170,16 @@ __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles
     }
   }
   survivors = wave_sum(survivors);
-  if (lane_id() == 0 && survivors) atomicAdd(&s_stats[0], (unsigned long long)survivors);
+  tests = wave_sum(tests);
+  if (lane_id() == 0) {
+    if (survivors) atomicAdd(&s_stats[0], (unsigned long long)survivors);
+    if (tests) atomicAdd(&s_stats[1], (unsigned long long)tests);
+  }
   __syncthreads();
-  if (tid == 0 && s_stats[0]) atomicAdd(&A.out.counters[CNT_CANDS], s_stats[0]);
+  if (tid == 0) {
+    if (s_stats[0]) atomicAdd(&A.out.counters[CNT_CANDS], s_stats[0]);
+    if (s_stats[1]) atomicAdd(&A.out.counters[CNT_VISITS], s_stats[1]);
+  }
 }
 
 void launch_bitmap(hipStream_t st, const BitmapArgs& a) {

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
   __shared__ uint32_t s_it_pj[FG_ITEMCAP];
   __shared__ uint32_t s_vis_off[FG_ITEMCAP + 1];
   __shared__ uint32_t s_tmp[FG_TPB / WAVE + 1];
-  __shared__ unsigned long long s_stats[2];
+  __shared__ unsigned long long s_stats[4];
 
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
   if (tile == 0xFFFFFFFFu) return;
@@ -103,7 +103,7 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
   const uint32_t p1 = min(p0 + ppt, A.probe_last + 1);
   const uint32_t np = p1 - p0;
 
-  if (tid < 2) s_stats[tid] = 0;
+  if (tid < 4) s_stats[tid] = 0;
 
   // ---- phase A: probe metadata, windows, LDS staging of probe tokens
   const uint32_t base = A.probe.starts[p0];
@@ -135,7 +135,8 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
   if (tid == 0) s_item_off[np] = total_items;
   __syncthreads();
 
-  uint32_t my_cands = 0;
+  uint32_t my_cands = 0, my_cand_tokens = 0;
+  if (tid == 0) s_stats[3] = total_items;
 
   // ---- phase B: batches of items
   for (uint32_t ib = 0; ib < total_items; ib += FG_ITEMCAP) {
@@ -204,6 +205,7 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
         pass = verify_from_first_common(r, lr, s, ls, j, mo, &ov, &canon);
       }
       my_cands += canon;
+      my_cand_tokens += canon ? A.indexed.sizes[sid] : 0u;
       emit_pairs(pass, rid, sid, ov, A.out.r, A.out.s, A.out.ov, &A.out.counters[CNT_PAIRS], A.out.capacity);
     }
     __syncthreads();
@@ -211,11 +213,17 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
 
   // ---- statistics: one global atomic per workgroup and counter
   my_cands = wave_sum(my_cands);
-  if (lane_id() == 0 && my_cands) atomicAdd(&s_stats[1], (unsigned long long)my_cands);
+  my_cand_tokens = wave_sum(my_cand_tokens);
+  if (lane_id() == 0 && my_cands) {
+    atomicAdd(&s_stats[1], (unsigned long long)my_cands);
+    atomicAdd(&s_stats[2], (unsigned long long)my_cand_tokens);
+  }
   __syncthreads();
   if (tid == 0) {
     if (s_stats[0]) atomicAdd(&A.out.counters[CNT_VISITS], s_stats[0]);
     if (s_stats[1]) atomicAdd(&A.out.counters[CNT_CANDS], s_stats[1]);
+    if (s_stats[2]) atomicAdd(&A.out.counters[CNT_CAND_TOKENS], s_stats[2]);
+    if (s_stats[3]) atomicAdd(&A.out.counters[CNT_ITEMS], s_stats[3]);
   }
 }
 

@@ -102,9 +102,10 @@ struct DevIndex {
   uint32_t universe;
 };
 
-// Output cursor block: [0] pairs found, [1] posting visits, [2] canonical
-// candidates, [3] spare.  64-bit counters.
-enum { CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_WORDS = 4 };
+// Output cursor and work counters (64-bit): pairs found, posting visits
+// (bitmap: pair tests), distinct candidates verified (bitmap: filter
+// survivors), tokens of the verified candidate sets, probe-prefix items.
+enum { CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_CAND_TOKENS = 3, CNT_ITEMS = 4, CNT_ENTRIES = 5, CNT_WORDS = 8 };
 
 struct DevOut {
   uint32_t* r;
@@ -117,6 +118,7 @@ struct DevOut {
 // Launch helpers (defined in the .hip files).
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_fill_u64(hipStream_t st, unsigned long long* p, size_t n, unsigned long long v);
+void launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
 void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* out2, size_t n,
                            DevBuf& scratch);
 

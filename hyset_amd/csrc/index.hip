@@ -40,6 +40,20 @@ void launch_fill_u64(hipStream_t st, unsigned long long* p, size_t n, unsigned l
   HS_CHECK_HIP(hipGetLastError());
 }
 
+// Plain device copy in 16-byte vectors: the reference rate for HBM traffic.
+__global__ void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+void launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes) {
+  const size_t n = bytes / sizeof(uint4);
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_copy_u128, dim3(grid_for(n, 256, 16384)), dim3(256), 0, st, (const uint4*)src, (uint4*)dst,
+                     n);
+  HS_CHECK_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------- scan
 // Three phases: per-tile sums, one-workgroup scan of the tile sums, per-tile
 // scan with the tile offset.  A tile is 256 threads x 16 elements.

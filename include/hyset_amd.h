@@ -126,6 +126,12 @@ enum { HS_ALGO_FGSS = 0, HS_ALGO_BITMAP = 1, HS_ALGO_HYBRID = 2 };
 typedef struct hs_ctx hs_ctx;
 typedef struct hs_coll hs_coll;
 
+/* The integer bounds the device tables hold for (sim, threshold, mode):
+ * out[0] = minsize(l1), out[1] = maxsize(l1), out[2] = minoverlap(l1, l2),
+ * out[3] = maxprefix(l1) (probe prefix), out[4] = midprefix(l1) (self-join
+ * index prefix).  Host-only; needs no device. */
+int hs_bounds(int32_t sim, double threshold, int32_t bounds_mode, uint32_t l1, uint32_t l2, uint32_t out[5]);
+
 /* Inclusive set-id interval, like a block's startID/endID
  * (include/hyset/structs.hpp:10-54).  first > last is an empty range. */
 typedef struct {
@@ -168,11 +174,13 @@ typedef struct {
   double transfer_ms;
   double total_ms;
   double host_filter_ms; /* hybrid: CPU filtering wall time */
+  double kernel_ms;      /* the main filter/verify kernel alone */
   uint64_t index_entries;
-  uint64_t posting_visits;
-  uint64_t candidates;
+  uint64_t posting_visits;   /* fgss: postings inside length windows; bitmap: pairs tested */
+  uint64_t candidates;       /* distinct pairs verified; bitmap: filter survivors */
   uint64_t pairs;
   uint64_t probe_prefix_tokens;
+  uint64_t candidate_tokens; /* fgss: sum of |s| over verified candidates */
 } hs_stats;
 
 /* ------------------------------------------------------------- contexts */
@@ -209,6 +217,12 @@ int hs_verify_candidates(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const ui
 int hs_copy_last_result(hs_ctx* ctx, hs_result* result);
 
 int hs_get_stats(const hs_ctx* ctx, hs_stats* out);
+
+/* Device-to-device copy of `bytes` with a plain 16-byte-vector kernel,
+ * repeated `reps` times on the context stream; *gbps receives the rate
+ * (read + write bytes per second / 1e9).  The reference rate for HBM
+ * roofline fractions on the machine at hand. */
+int hs_copy_bandwidth(hs_ctx* ctx, uint64_t bytes, int32_t reps, double* gbps);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,97 @@
+"""Host-side checks of libhyset_amd.so that need no GPU: the C ABI exports
+every declared symbol, the input generator, and the text formats."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+
+import hyset_amd as H
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared_functions():
+    names = set()
+    for hdr in (ROOT / "include").rglob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", hdr.read_text(), flags=re.S)
+        names.update(re.findall(r"\b(hs_[a-z_0-9]+)\s*\(", text))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(str(H.LIB_PATH))
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def check_collection(c, lo, hi):
+    assert c.sizes.min() >= lo and c.sizes.max() <= hi
+    assert np.all(np.diff(c.sizes.astype(np.int64)) >= 0), "sets sorted by size"
+    assert c.starts[0] == 0 and np.all(c.starts[1:] == np.cumsum(c.sizes)[:-1])
+    for i in range(0, c.n, max(1, c.n // 500)):
+        s = c.set(i)
+        assert np.all(np.diff(s.astype(np.int64)) > 0), "tokens strictly ascending"
+    assert c.tokens.min() >= 1
+    assert c.universe == int(c.tokens.max()) + 1
+
+
+def test_generator_properties_and_determinism():
+    p = H.gen_params(20000, median=10, sigma=0.6, lo=2, hi=200, universe=50000, zipf=1.1, seed=1)
+    a, b = H.generate_self(p), H.generate_self(p)
+    assert np.array_equal(a.tokens, b.tokens) and np.array_equal(a.sizes, b.sizes)
+    check_collection(a, 2, 200)
+    # ids ranked by ascending frequency
+    freq = np.bincount(a.tokens, minlength=a.universe)[1:]
+    assert np.all(np.diff(freq) >= 0)
+    assert abs(np.median(a.sizes) - 10) <= 1
+    p2 = H.gen_params(20000, median=10, sigma=0.6, lo=2, hi=200, universe=50000, zipf=1.1, seed=2)
+    assert not np.array_equal(H.generate_self(p2).tokens, a.tokens)
+
+
+def test_generator_uniform_sizes():
+    c = H.generate_self(H.gen_params(5000, size_dist=H.SIZE_UNIFORM, lo=5, hi=50, universe=10000, zipf=1.0))
+    check_collection(c, 5, 50)
+    assert len(np.unique(c.sizes)) == 46
+
+
+def test_generator_rs_near_duplicates():
+    r, s = H.generate_rs(H.gen_params(500, size_dist=H.SIZE_UNIFORM, lo=100, hi=300, universe=5000, zipf=0.8, seed=1),
+                         H.gen_params(500, size_dist=H.SIZE_UNIFORM, lo=100, hi=300, universe=5000, zipf=0.8, seed=2),
+                         dup_frac=0.2)
+    check_collection(r, 100, 300)
+    assert s.sizes.min() >= 80
+    rsets = {tuple(r.set(i).tolist()) for i in range(r.n)}
+    close = 0
+    for j in range(s.n):
+        sj = set(s.set(j).tolist())
+        best = max(len(sj & set(x)) / len(sj | set(x)) for x in list(rsets)[:500])
+        close += best >= 0.7
+    assert 80 <= close <= 120  # ~20% planted, 5-15% edits
+
+
+def test_generator_queries_share_ranking():
+    idx, q = H.generate_queries(H.gen_params(5000, universe=20000, seed=1), H.gen_params(300, universe=20000, seed=2))
+    check_collection(idx, 2, 200)
+    # query tokens unseen in the index get ids above the index universe
+    assert q.tokens.max() >= idx.universe - 1
+    assert np.isin(q.tokens[q.tokens < idx.universe], np.unique(idx.tokens)).all()
+
+
+def test_text_round_trip(tmp_path):
+    c = H.generate_self(H.gen_params(300, size_dist=H.SIZE_UNIFORM, lo=1, hi=9, universe=100, zipf=1.0))
+    path = tmp_path / "c.txt"
+    H.write_collection_text(path, c)
+    d = H.read_collection_text(path)
+    assert np.array_equal(c.tokens, d.tokens) and np.array_equal(c.sizes, d.sizes)
+    # empty lines are skipped, as the reference's loader does
+    path.write_text("1 2 3\n\n4 5\n\n")
+    e = H.read_collection_text(path)
+    assert e.n == 2 and e.sizes.tolist() == [3, 2]
+    pairs = tmp_path / "p.txt"
+    H.write_pairs_text(pairs, [3, 4], [1, 2], None)
+    assert pairs.read_text() == "3 1\n4 2\n"
+    H.write_pairs_text(pairs, [3], [1], [7])
+    assert pairs.read_text() == "3 1 7\n"
