@@ -17,6 +17,7 @@ ROOT = HERE.parent
 CSRC = HERE / "csrc"
 BUILD = HERE / "_build"
 LIB = HERE / "libhyset_amd.so"
+DRIVER = HERE / "hyset_join"
 
 HIP_SOURCES = ["index.hip", "fgss.hip", "bitmap.hip", "verify.hip"]
 CPP_SOURCES = ["api.cpp", "bounds.cpp", "datagen.cpp", "host_tools.cpp"]
@@ -65,8 +66,16 @@ def build(verbose: bool = False) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
     os.replace(tmp, LIB)
+    # C++ command-line driver over the handler layer, linked to the library
+    drv_tmp = DRIVER.with_suffix(".tmp")
+    cmd = ["g++", "-O2", "-std=c++17", f"-I{ROOT / 'include'}", str(HERE / "drivers" / "hyset_join.cpp"),
+           "-o", str(drv_tmp), f"-L{HERE}", "-lhyset_amd", "-Wl,-rpath,$ORIGIN"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    os.replace(drv_tmp, DRIVER)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {DRIVER}")
     return LIB
 
 

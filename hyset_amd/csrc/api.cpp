@@ -120,7 +120,8 @@ struct hs_ctx {
   hs::IndexBuild ib;
   DevBuf out_r, out_s, out_ov, counters;
   uint64_t out_cap = 0;
-  PinnedBuf h_counters;
+  PinnedBuf h_counters, h_cum;
+  DevBuf d_cum;
   hipEvent_t ev[6] = {};
   hs_stats stats{};
   uint64_t last_count = 0;
@@ -573,9 +574,25 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     // ---- index over the indexed range (not needed by the bitmap technique)
     const uint32_t* prefix_tab = self ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
     if (algo != HS_ALGO_BITMAP && !range_empty(ir)) {
-      const uint64_t bound = (uint64_t)indexed->h_starts[ir.last] + indexed->h_sizes[ir.last] -
-                             indexed->h_starts[ir.first];
-      hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->universe, bound, ctx->ib);
+      // entries per size class of the range (sets are sorted by size)
+      const auto& ptab = self ? ctx->tables.midprefix : ctx->tables.maxprefix;
+      const auto& sf = indexed->h_size_first;
+      const uint32_t nl = (uint32_t)sf.size();  // maxlen + 2
+      ctx->h_cum.ensure((size_t)nl * 4);
+      ctx->d_cum.ensure((size_t)nl * 4);
+      uint32_t* cum = ctx->h_cum.as<uint32_t>();
+      uint64_t acc = 0;
+      for (uint32_t l = 0; l + 1 < nl; ++l) {
+        cum[l] = (uint32_t)acc;
+        const uint32_t lo = std::max(ir.first, sf[l]);
+        const uint32_t hi = std::min(ir.last + 1, sf[l + 1]);
+        if (hi > lo) acc += (uint64_t)(hi - lo) * ptab[l];
+        HS_REQUIRE(acc < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
+      }
+      cum[nl - 1] = (uint32_t)acc;
+      HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, cum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
+      hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
+                      ctx->d_cum.as<uint32_t>(), indexed->universe, acc, ctx->ib);
     } else {
       ctx->ib.universe = 0;
     }
@@ -595,7 +612,8 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         hs::FgssArgs a{};
         a.probe = probe->dev();
         a.indexed = indexed->dev();
-        a.index = hs::DevIndex{ctx->ib.offsets.as<uint32_t>(), ctx->ib.set_ids.as<uint32_t>(), indexed->universe};
+        a.index = hs::DevIndex{ctx->ib.offsets.as<uint32_t>(), ctx->ib.set_ids.as<uint32_t>(),
+                               ctx->ib.positions.as<uint16_t>(), indexed->universe};
         a.bounds = ctx->dev_bounds;
         a.size_first = indexed->size_first.as<uint32_t>();
         a.size_first_len = (uint32_t)indexed->h_size_first.size();
@@ -605,8 +623,15 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         a.idx_last = ir.last;
         a.self = self;
         a.out = dev_out(ctx);
+        uint32_t ppt = tune.probes_per_tile;
+        if (ppt == 0) {
+          // as many probes per workgroup as keep their tokens within the LDS stage
+          const uint64_t ntok = (uint64_t)probe->h_starts[pr.last] + probe->h_sizes[pr.last] - probe->h_starts[pr.first];
+          const double avg = (double)ntok / ((double)pr.last - pr.first + 1);
+          ppt = (uint32_t)std::min(256.0, std::max(4.0, 3072.0 / std::max(avg, 1.0)));
+        }
         HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
-        hs::launch_fgss(st, a, tune.probes_per_tile);
+        hs::launch_fgss(st, a, ppt);
       } else if (algo == HS_ALGO_BITMAP) {
         const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
         HS_REQUIRE(bits == 64 || bits == 128 || bits == 256, HS_E_INVALID, "bitmap width must be 64, 128 or 256");

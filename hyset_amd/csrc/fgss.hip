@@ -20,9 +20,12 @@
 //     only the visit whose token is the first common token of r and s
 //     continues (that token always lies in both prefixes), which replaces
 //     the reference's dense count matrix by a merge that starts anyway;
-//   * the merge keeps the early-termination bound of verify_pair
-//     (verification.hpp:9-31) for failing pairs and runs to the end for
-//     passing ones, which yields the exact overlap;
+//   * PPJoin's positional filter (host_algorithms.hpp:196-199) runs first,
+//     from the token positions stored in the index;
+//   * the merge is driven by s, whose tokens arrive 16 at a time in
+//     registers, against r in LDS; it keeps the early-termination bound of
+//     verify_pair (verification.hpp:9-31) for failing pairs and runs to the
+//     end for passing ones, which yields the exact overlap;
 //   * passing pairs leave through one atomic per wave.
 #include <hip/hip_runtime.h>
 
@@ -36,47 +39,42 @@ constexpr uint32_t FG_TOKCAP = 4096;  // staged probe tokens (16 KiB)
 constexpr uint32_t FG_ITEMCAP = 1024; // items per batch
 constexpr uint32_t NO_LDS = 0xFFFFFFFFu;
 
-// Verify (r, s) from the visit reached through r[j].  Returns true when this
-// visit is the canonical one and the pair passes; *ov receives the overlap,
-// *canon whether the visit was canonical.
-__device__ __forceinline__ bool verify_from_first_common(const uint32_t* r, uint32_t lr, const uint32_t* s,
-                                                         uint32_t ls, uint32_t j, uint32_t mo, uint32_t* ov,
-                                                         bool* canon) {
-  uint32_t i = 0, k = 0;
-  uint32_t a = r[0], b = s[0];
-  // r[j] is in s (it indexes s), so a common token is met at or before it.
-  while (a != b) {
-    bool adv_a = a < b;
-    i += adv_a;
-    k += !adv_a;
-    if (i >= lr || k >= ls || i > j) {
-      *canon = false;
-      return false;
+constexpr uint32_t SREG = 16;  // tokens of s held in registers per refill
+
+// Verify (r, s) from the visit reached through r[j]: an s-driven merge.  The
+// tokens of s are loaded SREG at a time into registers with independent
+// loads (one memory latency per chunk instead of one per merge step); r is
+// read from LDS (kRInLds) or global memory.  The first common token must be
+// r[j] -- otherwise another visit of the same pair owns it and this one
+// stops.  The bound found + min(|r| - i, |s| - q) >= minoverlap of
+// verify_pair (verification.hpp:9-31) ends failing pairs early; passing
+// pairs run to the end, which gives the exact overlap.
+template <bool kRInLds>
+__device__ __forceinline__ bool verify_visit(const uint32_t* __restrict__ r, uint32_t lr,
+                                             const uint32_t* __restrict__ s, uint32_t ls, uint32_t j, uint32_t mo,
+                                             uint32_t* ov) {
+  uint32_t i = 0, found = 0;
+  for (uint32_t base = 0; base < ls; base += SREG) {
+    uint32_t sv[SREG];
+#pragma unroll
+    for (uint32_t q = 0; q < SREG; ++q) sv[q] = (base + q < ls) ? s[base + q] : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t q = 0; q < SREG; ++q) {
+      const uint32_t pos = base + q;
+      if (pos >= ls) break;
+      if (found + min(lr - i, ls - pos) < mo) return false;
+      const uint32_t x = sv[q];
+      while (i < lr && r[i] < x) ++i;
+      if (i >= lr) {
+        *ov = found;
+        return found >= mo;
+      }
+      if (r[i] == x) {
+        if (found == 0 && i != j) return false;  // not the first-common visit
+        ++found;
+        ++i;
+      }
     }
-    if (adv_a) a = r[i];
-    else b = s[k];
-  }
-  if (i != j) {
-    *canon = false;
-    return false;
-  }
-  *canon = true;
-  uint32_t found = 1;
-  ++i;
-  ++k;
-  if (i < lr) a = r[i];
-  if (k < ls) b = s[k];
-  while (i < lr && k < ls) {
-    uint32_t rest = min(lr - i, ls - k);
-    if (found + rest < mo) return false;
-    bool eq = a == b;
-    bool adv_a = a <= b;
-    bool adv_b = a >= b;
-    found += eq;
-    i += adv_a;
-    k += adv_b;
-    if (adv_a && i < lr) a = r[i];
-    if (adv_b && k < ls) b = s[k];
   }
   *ov = found;
   return found >= mo;
@@ -187,25 +185,36 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
     // one visit per lane
     for (uint32_t vb = 0; vb < total_vis; vb += FG_TPB) {
       const uint32_t vi = vb + tid;
-      bool pass = false, canon = false;
-      uint32_t rid = 0, sid = 0, ov = 0;
+      bool pass = false;
+      uint32_t rid = 0, sid = 0, ov = 0, checked_len = 0;
       if (vi < total_vis) {
         const uint32_t it = segment_of(s_vis_off, nb, vi);
         const uint32_t k = vi - s_vis_off[it];
         sid = A.index.set_ids[s_it_a[it] + k];
+        const uint32_t kp = A.index.positions[s_it_a[it] + k];
         const uint32_t pj = s_it_pj[it];
         const uint32_t pl = pj & 0xFFu;
         const uint32_t j = pj >> 8;
         rid = p0 + pl;
         const uint32_t lr = s_psize[pl];
-        const uint32_t* r = (s_plds[pl] != NO_LDS) ? (s_tok + s_plds[pl]) : (A.probe.tokens + s_pstart[pl]);
         const uint32_t ls = A.indexed.sizes[sid];
-        const uint32_t* s = A.indexed.tokens + A.indexed.starts[sid];
         const uint32_t mo = dev_minoverlap(A.bounds, lr, ls);
-        pass = verify_from_first_common(r, lr, s, ls, j, mo, &ov, &canon);
+        // Positional filter (helpers.hpp:46-50) at this visit's token: if it is
+        // the first common token, |r n s| <= 1 + min(|r|-1-j, |s|-1-kp).  Only
+        // the first-common visit can emit the pair, so a visit failing the
+        // bound is dropped without reading s.  (starts is loaded beside sizes
+        // to shorten the dependent-load chain.)
+        const uint32_t sst = A.indexed.starts[sid];
+        if (kp == 0xFFFFu || 1u + min(lr - 1u - j, ls - 1u - kp) >= mo) {
+          checked_len = ls;  // a verified candidate
+          const uint32_t* sp = A.indexed.tokens + sst;
+          const uint32_t off = s_plds[pl];
+          if (off != NO_LDS) pass = verify_visit<true>(s_tok + off, lr, sp, ls, j, mo, &ov);
+          else pass = verify_visit<false>(A.probe.tokens + s_pstart[pl], lr, sp, ls, j, mo, &ov);
+        }
       }
-      my_cands += canon;
-      my_cand_tokens += canon ? A.indexed.sizes[sid] : 0u;
+      my_cands += checked_len != 0;
+      my_cand_tokens += checked_len;
       emit_pairs(pass, rid, sid, ov, A.out.r, A.out.s, A.out.ov, &A.out.counters[CNT_PAIRS], A.out.capacity);
     }
     __syncthreads();

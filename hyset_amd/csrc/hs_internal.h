@@ -97,8 +97,9 @@ struct DevColl {
 };
 
 struct DevIndex {
-  const uint32_t* offsets;  // universe + 1
-  const uint32_t* set_ids;  // entries, sorted by set id inside every list
+  const uint32_t* offsets;    // universe + 1
+  const uint32_t* set_ids;    // entries, sorted by set id inside every list
+  const uint16_t* positions;  // position of the token inside its set (0xFFFF: >= 65535)
   uint32_t universe;
 };
 
@@ -123,15 +124,18 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
                            DevBuf& scratch);
 
 struct IndexBuild {
-  DevBuf offsets, cursor, set_ids, long_queue, scan_scratch;
+  DevBuf offsets, set_ids, positions;  // the index
+  DevBuf keys[2], vals, pos, hist, scan_scratch;  // radix-sort work space
   uint32_t universe = 0;
   uint64_t entries = 0;
 };
 // Builds the inverted index over the prefix entries of sets [first, last]
-// of `c`; prefix_len is indexed by set size.  `entries` must be the exact
-// number of prefix entries (computed on the host).
+// of `c` (sets sorted by size).  prefix_len is indexed by set size;
+// size_first[l] is the first set id of size >= l; cum[l] is the number of
+// prefix entries of the range's sets shorter than l; `entries` is the total.
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
-                 uint32_t universe, uint64_t entries, IndexBuild& ib);
+                 const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
+                 IndexBuild& ib);
 
 struct FgssArgs {
   DevColl probe;

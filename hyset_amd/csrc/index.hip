@@ -4,8 +4,10 @@
 // create_index with atomic scatter, thrust::sort by (token, setID)).
 // Here: count -> exclusive scan -> atomic scatter of set ids -> per-list sort
 // by set id, so list(t) = set_ids[offsets[t] .. offsets[t+1]) ascending --
-// the order of the reference's sorted host index (index.hpp:278-279).  Lists
-// hold 4-byte set ids only; positions are not needed by the device filters.
+// the order of the reference's sorted host index (index.hpp:278-279).  A
+// parallel 16-bit array keeps each entry's position inside its set (the
+// `position` of the reference's 12-byte entry, structs.hpp:56-63), used by
+// the positional filter.
 #include <hip/hip_runtime.h>
 
 #include "device_common.h"
@@ -109,9 +111,10 @@ __global__ __launch_bounds__(SC_TPB) void k_scan_sums(uint32_t* sums, uint32_t n
   }
 }
 
-__global__ __launch_bounds__(SC_TPB) void k_scan_apply(const uint32_t* __restrict__ in, size_t n,
-                                                       const uint32_t* __restrict__ sums, uint32_t* __restrict__ out,
-                                                       uint32_t* __restrict__ out2) {
+// in, out and out2 may alias (in-place scans): each tile is read into LDS
+// before any of it is written.
+__global__ __launch_bounds__(SC_TPB) void k_scan_apply(const uint32_t* in, size_t n, const uint32_t* __restrict__ sums,
+                                                       uint32_t* out, uint32_t* out2) {
   __shared__ uint32_t tile[SC_TILE];
   __shared__ uint32_t tmp[SC_TPB / WAVE + 1];
   size_t base = (size_t)blockIdx.x * SC_TILE;
@@ -162,134 +165,162 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 }
 
 // ------------------------------------------------------------ index build
+//
+// 1. emit: one thread per set writes its prefix entries (token, set id,
+//    position) in set-id order.  The entry offset of set i comes from a
+//    per-size table (sets are sorted by size, so all sets of one size are
+//    contiguous): off(i) = cum[l] + (i - first id of size l) * prefix_len[l].
+// 2. a stable LSD radix sort by token, 8 bits per pass, only as many passes
+//    as the universe needs; stability keeps every list in set-id order.
+//    Per pass: per-tile digit histograms (LDS atomics), a device scan of the
+//    digit-major histogram table, and a scatter that ranks each element
+//    inside its tile with wave ballots (no global atomics anywhere).
+// 3. offsets[t] = first entry with token >= t (binary search per token).
 
-// One thread per set: count every prefix token.  Sets are contiguous in the
-// CSR, so a wave reads ~64 neighbouring prefixes.
-__global__ void k_index_count(DevColl c, uint32_t first, uint32_t last, const uint32_t* __restrict__ prefix_len,
-                              uint32_t* __restrict__ counts) {
-  uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > last) return;
-  uint32_t st = c.starts[i];
-  uint32_t pl = prefix_len[c.sizes[i]];
-  for (uint32_t k = 0; k < pl; ++k) atomicAdd(&counts[c.tokens[st + k]], 1u);
-}
+constexpr int RS_TPB = 256;
+constexpr int RS_ROUNDS = 16;
+constexpr uint32_t RS_TILE = RS_TPB * RS_ROUNDS;  // 4096 entries per tile
+constexpr int RS_WAVES = RS_TPB / WAVE;
 
-__global__ void k_index_scatter(DevColl c, uint32_t first, uint32_t last, const uint32_t* __restrict__ prefix_len,
-                                uint32_t* __restrict__ cursor, uint32_t* __restrict__ set_ids) {
-  uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void k_index_emit(DevColl c, uint32_t first, uint32_t last, const uint32_t* __restrict__ prefix_len,
+                             const uint32_t* __restrict__ size_first, const uint32_t* __restrict__ cum,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint16_t* __restrict__ pos) {
+  const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
   if (i > last) return;
-  uint32_t st = c.starts[i];
-  uint32_t pl = prefix_len[c.sizes[i]];
+  const uint32_t l = c.sizes[i];
+  const uint32_t pl = prefix_len[l];
+  const uint32_t o = cum[l] + (i - max(size_first[l], first)) * pl;
+  const uint32_t* t = c.tokens + c.starts[i];
   for (uint32_t k = 0; k < pl; ++k) {
-    uint32_t pos = atomicAdd(&cursor[c.tokens[st + k]], 1u);
-    set_ids[pos] = i;
+    keys[o + k] = t[k];
+    vals[o + k] = i;
+    pos[o + k] = (uint16_t)min(k, 0xFFFFu);
   }
 }
 
-constexpr uint32_t SHORT_LIST = 32;
-
-// One thread per token: insertion-sort lists of up to SHORT_LIST ids in
-// place; queue longer lists for k_sort_long_lists.
-__global__ void k_sort_short_lists(const uint32_t* __restrict__ offsets, uint32_t universe, uint32_t* set_ids,
-                                   uint32_t* long_queue) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= universe) return;
-  uint32_t a = offsets[t], b = offsets[t + 1];
-  uint32_t len = b - a;
-  if (len < 2) return;
-  if (len > SHORT_LIST) {
-    uint32_t q = atomicAdd(&long_queue[0], 1u);
-    long_queue[1 + q] = t;
-    return;
+__global__ __launch_bounds__(RS_TPB) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
+                                                       uint32_t* __restrict__ hist, uint32_t ntiles) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * RS_TILE;
+#pragma unroll 4
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const uint32_t idx = base + r * RS_TPB + threadIdx.x;
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
   }
-  uint32_t* p = set_ids + a;
-  for (uint32_t i = 1; i < len; ++i) {
-    uint32_t x = p[i];
-    uint32_t j = i;
-    while (j > 0 && p[j - 1] > x) {
-      p[j] = p[j - 1];
-      --j;
+  __syncthreads();
+  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                          const uint32_t* __restrict__ vin,
+                                                          const uint16_t* __restrict__ pin, uint32_t n,
+                                                          uint32_t shift, const uint32_t* __restrict__ offs,
+                                                          uint32_t ntiles, uint32_t* __restrict__ kout,
+                                                          uint32_t* __restrict__ vout, uint16_t* __restrict__ pout) {
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_run[256];
+  __shared__ uint32_t s_wcnt[RS_WAVES][256];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid / WAVE;
+  const uint32_t lane = lane_id();
+  s_base[tid] = offs[tid * ntiles + blockIdx.x];
+  s_run[tid] = 0;
+#pragma unroll
+  for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][tid] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * RS_TILE;
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const uint32_t idx = base + r * RS_TPB + tid;
+    const bool valid = idx < n;
+    const uint32_t key = valid ? kin[idx] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
+    // lanes of this wave holding the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
     }
-    p[j] = x;
+    const uint32_t rank = lanes_below(peers);
+    if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t before = s_run[d];
+      for (uint32_t w = 0; w < wave; ++w) before += s_wcnt[w][d];
+      const uint32_t o = s_base[d] + before + rank;
+      kout[o] = key;
+      vout[o] = vin[idx];
+      pout[o] = pin[idx];
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) {
+      add += s_wcnt[w][tid];
+      s_wcnt[w][tid] = 0;
+    }
+    s_run[tid] += add;
+    __syncthreads();
   }
 }
 
-constexpr int LS_TPB = 256;
-constexpr uint32_t LS_LDS = 8192;  // ids sorted in LDS (32 KiB)
-
-// Ascending bitonic network in the "mirror" form: the first step of every
-// stage compares i with i ^ (k-1), the later steps i with i ^ j, and every
-// comparator puts the smaller id at the lower index.  Positions >= len act
-// as +infinity; since no comparator ever moves a real element upward past a
-// larger one, a pair with a virtual partner never swaps and the network can
-// run on exactly `len` stored elements (LDS or global memory).
-__device__ void mirror_bitonic_sort(uint32_t* p, uint32_t len) {
-  uint32_t n2 = 1;
-  while (n2 < len) n2 <<= 1;
-  for (uint32_t k = 2; k <= n2; k <<= 1) {
-    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-      const uint32_t flip = (j == (k >> 1)) ? (k - 1) : j;
-      for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) {
-        uint32_t l = i ^ flip;
-        if (l > i && l < len) {
-          uint32_t x = p[i], y = p[l];
-          if (x > y) {
-            p[i] = y;
-            p[l] = x;
-          }
-        }
-      }
-      __syncthreads();
-    }
-  }
-}
-
-// One workgroup per queued long list (grid-stride over the queue).
-__global__ __launch_bounds__(LS_TPB) void k_sort_long_lists(const uint32_t* __restrict__ offsets, uint32_t* set_ids,
-                                                            const uint32_t* __restrict__ long_queue) {
-  __shared__ uint32_t buf[LS_LDS];
-  const uint32_t nq = long_queue[0];
-  for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
-    uint32_t t = long_queue[1 + qi];
-    uint32_t a = offsets[t], len = offsets[t + 1] - a;
-    uint32_t* p = set_ids + a;
-    if (len <= LS_LDS) {
-      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) buf[i] = p[i];
-      __syncthreads();
-      mirror_bitonic_sort(buf, len);
-      for (uint32_t i = threadIdx.x; i < len; i += blockDim.x) p[i] = buf[i];
-      __syncthreads();
-    } else {
-      mirror_bitonic_sort(p, len);  // rare: very long lists sort in global memory
-    }
-  }
+// offsets[t] = number of entries with token < t, for t in [0, universe].
+__global__ void k_index_offsets(const uint32_t* __restrict__ keys, uint32_t n, uint32_t universe,
+                                uint32_t* __restrict__ offsets) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t > universe) return;
+  offsets[t] = lower_bound_u32(keys, 0, n, t);
 }
 
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
-                 uint32_t universe, uint64_t entries_bound, IndexBuild& ib) {
+                 const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
+                 IndexBuild& ib) {
+  HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
+  const uint32_t n = (uint32_t)entries;
   ib.universe = universe;
+  ib.entries = entries;
+  const size_t cap = entries ? entries : 1;
   ib.offsets.ensure(((size_t)universe + 1) * sizeof(uint32_t));
-  ib.cursor.ensure(((size_t)universe + 1) * sizeof(uint32_t));
-  ib.set_ids.ensure((entries_bound ? entries_bound : 1) * sizeof(uint32_t));
-  ib.long_queue.ensure(((size_t)universe + 1) * sizeof(uint32_t));
-  uint32_t* counts = ib.cursor.as<uint32_t>();
-  launch_fill_u32(st, counts, (size_t)universe + 1, 0);
-  launch_fill_u32(st, ib.long_queue.as<uint32_t>(), 1, 0);
-  if (c.n > 0 && first <= last) {
-    uint32_t nsets = last - first + 1;
-    hipLaunchKernelGGL(k_index_count, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len, counts);
+  ib.set_ids.ensure(cap * sizeof(uint32_t));
+  ib.positions.ensure(cap * sizeof(uint16_t));
+  ib.keys[0].ensure(cap * sizeof(uint32_t));
+  ib.keys[1].ensure(cap * sizeof(uint32_t));
+  ib.vals.ensure(cap * sizeof(uint32_t));
+  ib.pos.ensure(cap * sizeof(uint16_t));
+  if (c.n > 0 && first <= last && n > 0) {
+    // passes: enough 8-bit digits for the largest token
+    uint32_t passes = 1;
+    while (passes < 4 && (uint64_t)(universe - 1) >> (8 * passes)) ++passes;
+    const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+    ib.hist.ensure((size_t)256 * ntiles * sizeof(uint32_t));
+    // ping-pong between (keys[0], vals, pos) and (keys[1], set_ids,
+    // positions); emit into the buffer that makes the last pass land in the
+    // second one
+    uint32_t* kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
+    uint32_t* vbuf[2] = {ib.vals.as<uint32_t>(), ib.set_ids.as<uint32_t>()};
+    uint16_t* pbuf[2] = {ib.pos.as<uint16_t>(), ib.positions.as<uint16_t>()};
+    int cur = (passes % 2 == 1) ? 0 : 1;
+    const uint32_t nsets = last - first + 1;
+    hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
+                       size_first, cum, kbuf[cur], vbuf[cur], pbuf[cur]);
+    for (uint32_t p = 0; p < passes; ++p) {
+      const uint32_t shift = 8 * p;
+      hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift,
+                         ib.hist.as<uint32_t>(), ntiles);
+      launch_exclusive_scan(st, ib.hist.as<uint32_t>(), ib.hist.as<uint32_t>(), nullptr, (size_t)256 * ntiles,
+                            ib.scan_scratch);
+      hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], vbuf[cur], pbuf[cur], n,
+                         shift, ib.hist.as<uint32_t>(), ntiles, kbuf[cur ^ 1], vbuf[cur ^ 1], pbuf[cur ^ 1]);
+      cur ^= 1;
+    }
+    hipLaunchKernelGGL(k_index_offsets, dim3((universe + 1 + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
+                       ib.offsets.as<uint32_t>());
+  } else {
+    launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }
-  // offsets = exclusive scan of counts; the scan also refreshes the cursor.
-  launch_exclusive_scan(st, counts, ib.offsets.as<uint32_t>(), counts, (size_t)universe + 1, ib.scan_scratch);
-  if (c.n > 0 && first <= last) {
-    uint32_t nsets = last - first + 1;
-    hipLaunchKernelGGL(k_index_scatter, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
-                       counts, ib.set_ids.as<uint32_t>());
-  }
-  hipLaunchKernelGGL(k_sort_short_lists, dim3((universe + 255) / 256), dim3(256), 0, st, ib.offsets.as<uint32_t>(),
-                     universe, ib.set_ids.as<uint32_t>(), ib.long_queue.as<uint32_t>());
-  hipLaunchKernelGGL(k_sort_long_lists, dim3(512), dim3(LS_TPB), 0, st, ib.offsets.as<uint32_t>(),
-                     ib.set_ids.as<uint32_t>(), ib.long_queue.as<uint32_t>());
   HS_CHECK_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,195 @@
+// Command-line self-join driver over the handler layer, with the flags of
+// the reference's GPU drivers (src/bitmap.cu, src/fgss.cu, src/hybrid.cu):
+//
+//   hyset_join --input FILE [--threshold 0.95] [--block 10000]
+//              [--algorithm fgss|bitmap|hybrid] [--bitmap 64]
+//              [--scenario 1] [--threads 32] [--device-memory 512M]
+//              [--output FILE] [--overlap]
+//
+// The whole collection is one partition of blocks joined with itself
+// (src/fgss.cu:108-116).  With --output the pairs are written as "r s" lines
+// (output.hpp:39-76; "r s overlap" with --overlap); otherwise the count is
+// printed.  The hybrid technique filters on the host (candidate generation
+// of hybrid::allpairs, hybrid_algorithms.hpp:209-291) and verifies the
+// batches on the device through the device_handler.
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hyset_amd/handlers.hpp"
+
+namespace ha = hyset_amd;
+
+namespace {
+
+struct Options {
+  std::string input, output, algorithm = "fgss", deviceMemory = "512M";
+  double threshold = 0.95;
+  unsigned block = 10000, bitmap = 64, scenario = 1, threads = 32;
+  bool overlap = false;
+};
+
+[[noreturn]] void usage(const char* msg) {
+  std::fprintf(stderr, "error: %s\nusage: hyset_join --input FILE [--threshold T] [--block B] "
+                       "[--algorithm fgss|bitmap|hybrid] [--bitmap 64|128|256] [--scenario 1|2] [--threads N] "
+                       "[--device-memory 512M] [--output FILE] [--overlap]\n", msg);
+  std::exit(1);
+}
+
+Options parse(int argc, char** argv) {
+  Options o;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    auto val = [&]() -> std::string {
+      if (i + 1 >= argc) usage(("missing value for " + a).c_str());
+      return argv[++i];
+    };
+    if (a == "--input") o.input = val();
+    else if (a == "--output") o.output = val();
+    else if (a == "--algorithm") o.algorithm = val();
+    else if (a == "--threshold") o.threshold = std::strtod(val().c_str(), nullptr);
+    else if (a == "--block") o.block = (unsigned)std::strtoul(val().c_str(), nullptr, 10);
+    else if (a == "--bitmap") o.bitmap = (unsigned)std::strtoul(val().c_str(), nullptr, 10);
+    else if (a == "--scenario") o.scenario = (unsigned)std::strtoul(val().c_str(), nullptr, 10);
+    else if (a == "--threads") o.threads = (unsigned)std::strtoul(val().c_str(), nullptr, 10);
+    else if (a == "--device-memory") o.deviceMemory = val();
+    else if (a == "--overlap") o.overlap = true;
+    else usage(("unknown option " + a).c_str());
+  }
+  if (o.input.empty()) usage("no input dataset given");
+  if (o.block == 0) usage("--block must be positive");
+  return o;
+}
+
+// "512M" -> number of 4-byte candidates (include/hyset/helpers.hpp:17-34).
+unsigned long long candidates_for(const std::string& mem) {
+  if (mem.empty()) return 1ull << 27;
+  char scale = mem.back();
+  double v = std::strtod(mem.substr(0, mem.size() - 1).c_str(), nullptr);
+  double bytes = v * (scale == 'M' ? 1e6 : 1e9);
+  return (unsigned long long)(bytes / 4.0);
+}
+
+// Host candidate generation for the hybrid driver: AllPairs filtering
+// without verification (hybrid_algorithms.hpp:209-291) over a host index of
+// midprefix entries sorted by (token, set id).
+void hybrid_filter(const hs_host_collection& c, double t, ha::algorithms::hybrid::device_handler& dh) {
+  std::vector<unsigned> offs(c.universe + 1, 0), ids;
+  std::vector<uint32_t> b(5);
+  auto bnd = [&](uint32_t l1, uint32_t l2) {
+    ha::check(hs_bounds(HS_JACCARD, t, HS_BOUNDS_REFERENCE, l1, l2, b.data()), nullptr, "hs_bounds");
+    return b;
+  };
+  const uint32_t maxlen = c.n ? c.sizes[c.n - 1] : 0;
+  std::vector<uint32_t> minsize(maxlen + 1), maxsize(maxlen + 1), maxprefix(maxlen + 1), midprefix(maxlen + 1);
+  for (uint32_t l = 1; l <= maxlen; ++l) {
+    auto v = bnd(l, l);
+    minsize[l] = v[0];
+    maxsize[l] = v[1];
+    maxprefix[l] = v[3];
+    midprefix[l] = v[4];
+  }
+  for (uint32_t i = 0; i < c.n; ++i)
+    for (uint32_t k = 0; k < midprefix[c.sizes[i]]; ++k) offs[c.tokens[c.starts[i] + k] + 1]++;
+  for (uint32_t u = 0; u < c.universe; ++u) offs[u + 1] += offs[u];
+  ids.resize(offs[c.universe]);
+  std::vector<unsigned> cur(offs.begin(), offs.end() - 1);
+  for (uint32_t i = 0; i < c.n; ++i)
+    for (uint32_t k = 0; k < midprefix[c.sizes[i]]; ++k) ids[cur[c.tokens[c.starts[i] + k]]++] = i;
+  std::vector<unsigned> seen(c.n, 0xFFFFFFFFu), cand;
+  for (uint32_t p = 0; p < c.n; ++p) {
+    const uint32_t l = c.sizes[p];
+    cand.clear();
+    for (uint32_t j = 0; j < maxprefix[l]; ++j) {
+      const uint32_t tok = c.tokens[c.starts[p] + j];
+      for (unsigned it = offs[tok]; it < offs[tok + 1]; ++it) {
+        const unsigned s = ids[it];
+        if (c.sizes[s] < minsize[l]) continue;
+        if (s >= p || c.sizes[s] > maxsize[l]) break;
+        if (seen[s] != p) {
+          seen[s] = p;
+          cand.push_back(s);
+        }
+      }
+    }
+    if (!cand.empty()) dh.addCandidates(p, cand);
+  }
+  dh.flush();
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  Options o = parse(argc, argv);
+  hs_host_collection hc{};
+  if (hs_read_collection_text(o.input.c_str(), &hc) != HS_OK) {
+    std::fprintf(stderr, "Error: cannot read %s\n", o.input.c_str());
+    return 1;
+  }
+  try {
+    ha::context ctx(0);
+    std::vector<unsigned> toks(hc.tokens, hc.tokens + hc.n_tokens), starts(hc.starts, hc.starts + hc.n),
+        sizes(hc.sizes, hc.sizes + hc.n);
+    ha::collection::device_collection dc(ctx, toks, starts, sizes);
+    std::vector<ha::structs::block> blocks = ha::structs::make_blocks(hc.n, o.block);
+    ha::structs::partition part(0);
+    for (auto& b : blocks) part.blocks.push_back(&b);
+    const bool aggregate = o.output.empty();
+    std::shared_ptr<ha::output::handler> out;
+    if (aggregate) out = std::make_shared<ha::output::count_handler>();
+    else out = std::make_shared<ha::output::pairs_handler>();
+
+    auto t0 = std::chrono::steady_clock::now();
+    if (hc.n > 0) {
+      if (o.algorithm == "fgss") {
+        ha::algorithms::device::fgssjoin::handler h(ctx, dc, o.block, aggregate, o.threshold);
+        h.setOutputHandler(out);
+        h.join(&part, &part);
+      } else if (o.algorithm == "bitmap") {
+        ha::algorithms::device::bitmap::handler h(ctx, dc, o.bitmap, o.block, aggregate, o.threshold);
+        h.setOutputHandler(out);
+        h.join(&part, &part);
+      } else if (o.algorithm == "hybrid") {
+        ha::algorithms::hybrid::device_handler dh(ctx, dc, dc, o.threads, o.scenario, aggregate, hc.n,
+                                                  (unsigned)std::min<unsigned long long>(candidates_for(o.deviceMemory),
+                                                                                         0xFFFFFFFFull),
+                                                  o.threshold);
+        dh.setMaxSetSize(hc.n ? hc.sizes[hc.n - 1] : 0);
+        dh.setOutputHandler(out);
+        hybrid_filter(hc, o.threshold, dh);
+      } else {
+        usage("unknown algorithm");
+      }
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    std::printf("Total time without I/O (ms): %.3f\n", std::chrono::duration<double, std::milli>(t1 - t0).count());
+    if (aggregate) {
+      std::printf("Result: %lu\n", out->getCount());
+    } else {
+      auto& pairs = static_cast<ha::output::pairs_handler&>(*out).pairs;
+      std::vector<uint32_t> r, s, ov;
+      for (auto& p : pairs) {
+        r.push_back(p.first);
+        s.push_back(p.second);
+        ov.push_back(p.overlap);
+      }
+      if (hs_write_pairs_text(o.output.c_str(), r.data(), s.data(), o.overlap ? ov.data() : nullptr, r.size()) !=
+          HS_OK) {
+        std::fprintf(stderr, "Error writing to file\n");
+        hs_host_collection_free(&hc);
+        return 1;
+      }
+      std::printf("Result: %zu pairs written to %s\n", pairs.size(), o.output.c_str());
+    }
+  } catch (const ha::error& e) {
+    std::fprintf(stderr, "hyset_join: %s\n", e.what());
+    hs_host_collection_free(&hc);
+    return 2;
+  }
+  hs_host_collection_free(&hc);
+  return 0;
+}

@@ -1,0 +1,309 @@
+// C++ host layer over the C ABI of libhyset_amd.so that restores the
+// reference's handler interface, so its drivers' call sites keep their shape.
+//
+// Reference interface each type mirrors (paths relative to the reference):
+//   output::handler / count_handler / pairs_handler   include/hyset/output.hpp:13-37
+//   structs::block / structs::partition               include/hyset/structs.hpp:10-54, :105-148
+//   collection::device_collection                     include/hyset/collection.hpp:54-71
+//   algorithms::device::bitmap::handler               include/hyset/device_algorithms.hpp:120-260
+//   algorithms::device::fgssjoin::handler             include/hyset/device_algorithms.hpp:305-446
+//   algorithms::hybrid::device_handler                include/hyset/hybrid_algorithms.hpp:16-205
+//
+// Differences from the reference, all deliberate:
+//   * errors throw hyset_amd::error carrying the HS_E_* code and message
+//     (the reference lets CUDA exceptions escape);
+//   * the output sink is chosen by the `aggregate` flag exactly as in the
+//     reference, but pairs_handler also keeps the overlap of every pair;
+//   * handlers take a hyset_amd::context (one HIP stream) instead of a
+//     device timer; timings are available through context::stats().
+#pragma once
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../hyset_amd.h"
+
+namespace hyset_amd {
+
+struct error : std::runtime_error {
+  int code;
+  error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+inline void check(int rc, hs_ctx* ctx, const char* what) {
+  if (rc != HS_OK) throw error(rc, std::string(what) + ": " + (ctx ? hs_last_error(ctx) : "failed"));
+}
+
+// ------------------------------------------------------------------ output
+
+namespace output {
+struct handler {
+  virtual void addPair(const unsigned int& rec1, const unsigned int& rec2, unsigned int overlap) = 0;
+  virtual unsigned long getCount() const = 0;
+  virtual ~handler() = default;
+};
+
+struct count_handler : handler {
+  unsigned long count = 0;
+  void addPair(const unsigned int&, const unsigned int&, unsigned int) override { ++count; }
+  unsigned long getCount() const override { return count; }
+};
+
+struct pair {
+  unsigned int first, second, overlap;
+};
+
+struct pairs_handler : handler {
+  std::vector<pair> pairs;
+  void addPair(const unsigned int& a, const unsigned int& b, unsigned int ov) override { pairs.push_back({a, b, ov}); }
+  unsigned long getCount() const override { return (unsigned long)pairs.size(); }
+};
+}  // namespace output
+
+// ----------------------------------------------------------------- structs
+
+namespace structs {
+// A run of consecutive set ids (include/hyset/structs.hpp:10-54).
+struct block {
+  unsigned int id = 0, startID = 0, endID = 0;
+  unsigned int size() const { return endID - startID + 1; }
+};
+
+// A list of blocks, or a plain id range (include/hyset/structs.hpp:105-148).
+struct partition {
+  unsigned int id = 0;
+  std::vector<block*> blocks;
+  bool blockBased = true;
+  unsigned int startID = 0, endID = 0;
+  explicit partition(unsigned int i) : id(i) {}
+  partition(unsigned int i, unsigned int s, unsigned int e) : id(i), blockBased(false), startID(s), endID(e) {}
+  unsigned int start_id() const { return blockBased ? blocks.front()->startID : startID; }
+  unsigned int end_id() const { return blockBased ? blocks.back()->endID : endID; }
+};
+
+// Cuts [0, n) into blocks of blockSize sets (collection.hpp:147-172).
+inline std::vector<block> make_blocks(unsigned int n, unsigned int blockSize) {
+  std::vector<block> out;
+  for (unsigned int s = 0, id = 0; s < n; s += blockSize, ++id)
+    out.push_back(block{id, s, (n - s > blockSize ? s + blockSize : n) - 1});
+  return out;
+}
+}  // namespace structs
+
+// ----------------------------------------------------- context, collection
+
+class context {
+ public:
+  explicit context(int device = 0) { check(hs_create(device, &ctx_), nullptr, "hs_create"); }
+  ~context() { hs_destroy(ctx_); }
+  context(const context&) = delete;
+  context& operator=(const context&) = delete;
+  hs_ctx* get() const { return ctx_; }
+  hs_stats stats() const {
+    hs_stats s{};
+    hs_get_stats(ctx_, &s);
+    return s;
+  }
+
+ private:
+  hs_ctx* ctx_ = nullptr;
+};
+
+namespace collection {
+// Device-resident collection (collection.hpp:54-71).
+class device_collection {
+ public:
+  device_collection(context& ctx, const std::vector<unsigned int>& tokens, const std::vector<unsigned int>& starts,
+                    const std::vector<unsigned int>& sizes)
+      : sizes_(sizes) {
+    check(hs_upload_collection(ctx.get(), tokens.data(), starts.data(), sizes.data(), (uint32_t)sizes.size(),
+                               tokens.size(), &coll_),
+          ctx.get(), "hs_upload_collection");
+  }
+  ~device_collection() { hs_free_collection(coll_); }
+  device_collection(const device_collection&) = delete;
+  device_collection& operator=(const device_collection&) = delete;
+  hs_coll* get() const { return coll_; }
+  const std::vector<unsigned int>& sizes() const { return sizes_; }
+
+ private:
+  hs_coll* coll_ = nullptr;
+  std::vector<unsigned int> sizes_;
+};
+}  // namespace collection
+
+namespace detail {
+// Runs one device join and moves its result into the sink the reference
+// would have downcast to (device_algorithms.hpp:193-221).
+inline void run_join(context& ctx, collection::device_collection& indexed, collection::device_collection& probe,
+                     hs_range ir, hs_range pr, int32_t algo, const hs_tune& tune, bool aggregate,
+                     const std::shared_ptr<output::handler>& out) {
+  hs_result res{};
+  res.count_only = 1;
+  check(hs_join(ctx.get(), indexed.get(), probe.get(), &ir, &pr, algo, &tune, &res), ctx.get(), "hs_join");
+  if (!out) throw error(HS_E_INVALID, "no output handler set");
+  if (aggregate) {
+    auto* c = dynamic_cast<output::count_handler*>(out.get());
+    if (!c) throw error(HS_E_INVALID, "aggregate mode needs a count_handler");
+    c->count += res.count;
+    return;
+  }
+  auto* p = dynamic_cast<output::pairs_handler*>(out.get());
+  if (!p) throw error(HS_E_INVALID, "pairs mode needs a pairs_handler");
+  std::vector<uint32_t> r(res.count), s(res.count), ov(res.count);
+  hs_result cp{res.count, res.count, r.data(), s.data(), ov.data(), 0};
+  check(hs_copy_last_result(ctx.get(), &cp), ctx.get(), "hs_copy_last_result");
+  p->pairs.reserve(p->pairs.size() + res.count);
+  for (uint64_t i = 0; i < res.count; ++i) p->pairs.push_back({r[i], s[i], ov[i]});
+}
+
+// The self-join block-pair rule of length_filter (device_algorithms.hpp:21-53):
+// pairs with s < r in [first.start, first.end] x [second.start, second.end].
+inline hs_range range_of(const structs::partition* p) { return hs_range{p->start_id(), p->end_id()}; }
+inline hs_range range_of(const structs::block* b) { return hs_range{b->startID, b->endID}; }
+}  // namespace detail
+
+// ------------------------------------------------------------ techniques
+
+namespace algorithms {
+namespace device {
+
+namespace bitmap {
+struct handler {
+  handler(context& ctx, collection::device_collection& coll, unsigned int bitmapSize, unsigned int blockSize,
+          bool aggregate, double threshold)
+      : ctx_(ctx), coll_(coll), blockSize(blockSize), aggregate(aggregate), threshold(threshold) {
+    tune_.bitmap_bits = bitmapSize;
+    check(hs_set_threshold(ctx.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx.get(), "hs_set_threshold");
+  }
+  void setOutputHandler(std::shared_ptr<output::handler>& h) { output_ = h; }
+  void join(const structs::block* first, const structs::block* second) { run(detail::range_of(first), detail::range_of(second)); }
+  void join(const structs::partition* first, const structs::partition* second) {
+    run(detail::range_of(first), detail::range_of(second));
+  }
+  unsigned int blockSize;
+  bool aggregate;
+  double threshold;
+
+ private:
+  void run(hs_range ir, hs_range pr) {
+    check(hs_set_threshold(ctx_.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx_.get(), "hs_set_threshold");
+    detail::run_join(ctx_, coll_, coll_, ir, pr, HS_ALGO_BITMAP, tune_, aggregate, output_);
+  }
+  context& ctx_;
+  collection::device_collection& coll_;
+  hs_tune tune_{};
+  std::shared_ptr<output::handler> output_;
+};
+}  // namespace bitmap
+
+namespace fgssjoin {
+struct handler {
+  handler(context& ctx, collection::device_collection& coll, unsigned int blockSize, bool aggregate, double threshold)
+      : ctx_(ctx), coll_(coll), blockSize(blockSize), aggregate(aggregate), threshold(threshold) {
+    check(hs_set_threshold(ctx.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx.get(), "hs_set_threshold");
+  }
+  void setOutputHandler(std::shared_ptr<output::handler>& h) { output_ = h; }
+  // The reference copies a host-built index here (device_algorithms.hpp:354-361);
+  // the device builds its own index inside every join, so this is a no-op kept
+  // for call-site compatibility.
+  template <typename Index>
+  void transferIndex(Index*) const {}
+  void join(const structs::block* first, const structs::block* second) { run(detail::range_of(first), detail::range_of(second)); }
+  void join(const structs::partition* first, const structs::partition* second) {
+    run(detail::range_of(first), detail::range_of(second));
+  }
+  unsigned int blockSize;
+  bool aggregate;
+  double threshold;
+
+ private:
+  void run(hs_range ir, hs_range pr) {
+    check(hs_set_threshold(ctx_.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx_.get(), "hs_set_threshold");
+    detail::run_join(ctx_, coll_, coll_, ir, pr, HS_ALGO_FGSS, tune_, aggregate, output_);
+  }
+  context& ctx_;
+  collection::device_collection& coll_;
+  hs_tune tune_{};
+  std::shared_ptr<output::handler> output_;
+};
+}  // namespace fgssjoin
+
+}  // namespace device
+
+namespace hybrid {
+// CPU-filtered candidates verified on the device
+// (include/hyset/hybrid_algorithms.hpp:16-205).  Candidates accumulate in a
+// host batch of maxCandidates; a full batch, join() or flush() sends it to
+// hs_verify_candidates.
+struct device_handler {
+  device_handler(context& ctx, collection::device_collection& indexed, collection::device_collection& probe,
+                 unsigned int threads, unsigned int scenario, bool aggregate, unsigned int maxProbes,
+                 unsigned int maxCandidates, double threshold)
+      : threads(threads), scenario(scenario), aggregate(aggregate), threshold(threshold), maxProbes(maxProbes),
+        maxCandidates(maxCandidates), ctx_(ctx), indexed_(indexed), probe_(probe) {
+    check(hs_set_threshold(ctx.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx.get(), "hs_set_threshold");
+  }
+  void setMaxSetSize(unsigned int s) { maxSetSize = s; }
+  void setOutputHandler(std::shared_ptr<output::handler>& h) { output_ = h; }
+  // hybrid_algorithms.hpp:71-105: a candidate list may be split across two
+  // batches when it does not fit.
+  void addCandidates(unsigned int probeID, const std::vector<unsigned int>& candidateSet) {
+    size_t done = 0;
+    while (done < candidateSet.size()) {
+      size_t room = maxCandidates > candidates_.size() ? maxCandidates - candidates_.size() : 0;
+      if (room == 0 || probes_.size() >= maxProbes) {
+        join();
+        continue;
+      }
+      size_t take = std::min(room, candidateSet.size() - done);
+      candidates_.insert(candidates_.end(), candidateSet.begin() + done, candidateSet.begin() + done + take);
+      probes_.push_back(probeID);
+      offsets_.push_back((unsigned int)candidates_.size());
+      done += take;
+    }
+  }
+  void join() {
+    if (probes_.empty()) return;
+    check(hs_set_threshold(ctx_.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx_.get(), "hs_set_threshold");
+    hs_result res{};
+    res.count_only = 1;
+    check(hs_verify_candidates(ctx_.get(), indexed_.get(), probe_.get(), probes_.data(), offsets_.data(),
+                               candidates_.data(), (uint32_t)probes_.size(), (int32_t)scenario, &res),
+          ctx_.get(), "hs_verify_candidates");
+    probes_.clear();
+    offsets_.clear();
+    candidates_.clear();
+    deliver(res.count);
+  }
+  void flush() { join(); }
+
+  unsigned int threads, scenario;
+  bool aggregate;
+  double threshold;
+  unsigned int maxProbes, maxCandidates, maxSetSize = 0;
+
+ private:
+  void deliver(uint64_t count) {
+    if (!output_) throw error(HS_E_INVALID, "no output handler set");
+    if (aggregate) {
+      dynamic_cast<output::count_handler&>(*output_).count += count;
+      return;
+    }
+    std::vector<uint32_t> r(count), s(count), ov(count);
+    hs_result cp{count, count, r.data(), s.data(), ov.data(), 0};
+    check(hs_copy_last_result(ctx_.get(), &cp), ctx_.get(), "hs_copy_last_result");
+    auto& p = dynamic_cast<output::pairs_handler&>(*output_);
+    for (uint64_t i = 0; i < count; ++i) p.pairs.push_back({r[i], s[i], ov[i]});
+  }
+  context& ctx_;
+  collection::device_collection& indexed_;
+  collection::device_collection& probe_;
+  std::vector<unsigned int> probes_, offsets_, candidates_;
+  std::shared_ptr<output::handler> output_;
+};
+}  // namespace hybrid
+}  // namespace algorithms
+}  // namespace hyset_amd

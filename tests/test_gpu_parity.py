@@ -73,8 +73,13 @@ def test_fgss_config2_full(ctx, t):
         want = O.cpu_multi(c, t, 8)
         _check(ctx.join(d, algo="fgss"), want, f"fgss 1M t={t}")
         st = ctx.stats()
-        assert st["candidates"] == want.candidates
+        # postings inside the length windows: the same count as the oracle's
         assert st["posting_visits"] == want.posting_visits
+        # verified candidates: at most the oracle's (positional filter drops more)
+        assert 0 < st["candidates"] <= want.candidates
+        sizes, counts = np.unique(c.sizes, return_counts=True)
+        mids = sum(int(k) * O.bounds("jaccard", t, int(l))[4] for l, k in zip(sizes, counts))
+        assert st["index_entries"] == mids
     finally:
         d.free()
 
@@ -103,3 +108,42 @@ def test_verify_candidates(ctx, small):
         r, s, ov = H.sort_pairs(*ctx.verify_candidates(d, d, probes, np.array(offs, dtype=np.uint32),
                                                         np.array(cands, dtype=np.uint32), scen))
         assert list(zip(r.tolist(), s.tolist(), ov.tolist())) == exp, f"scenario {scen}"
+
+
+@pytest.fixture(scope="module")
+def rs_small(ctx):
+    R, S = H.generate_rs(H.gen_params(1500, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=3),
+                         H.gen_params(1800, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=4),
+                         dup_frac=0.3)
+    return R, S, ctx.upload(R), ctx.upload(S)
+
+
+@pytest.mark.parametrize("sim", ["jaccard", "cosine", "dice"])
+@pytest.mark.parametrize("algo", ["fgss", "bitmap", "hybrid"])
+def test_rs_join_small(ctx, rs_small, sim, algo):
+    """R-S (foreign) join: probes of R against an index over S, no id order."""
+    R, S, dR, dS = rs_small
+    for t in (0.6, 0.8):
+        ctx.set_threshold(t, sim)
+        want = O.rs_join(R, S, t, sim)
+        assert want.count > 50
+        _check(ctx.join(dS, dR, algo=algo), want, f"{algo} {sim} {t}")
+
+
+def test_rs_long_sets_cosine(ctx):
+    """The long-set R-S configuration at 1/10 scale: 20,000 x 20,000 sets of
+    100-700 tokens over a 50,000-token Zipf(0.8) universe, 20% of S planted
+    near-duplicates, cosine 0.8."""
+    R, S = H.generate_rs(H.gen_params(20000, size_dist=H.SIZE_UNIFORM, lo=100, hi=700, universe=50000, zipf=0.8,
+                                      seed=1),
+                         H.gen_params(20000, size_dist=H.SIZE_UNIFORM, lo=100, hi=700, universe=50000, zipf=0.8,
+                                      seed=2), dup_frac=0.2, edit_lo=0.05, edit_hi=0.15)
+    dR, dS = ctx.upload(R), ctx.upload(S)
+    try:
+        ctx.set_threshold(0.8, "cosine")
+        want = O.rs_join(R, S, 0.8, "cosine")
+        assert want.count >= 3000
+        _check(ctx.join(dS, dR, algo="fgss"), want, "fgss long cosine")
+    finally:
+        dR.free()
+        dS.free()
