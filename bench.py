@@ -96,7 +96,7 @@ def main() -> int:
     alg = fgss_bytes(st, coll.n, int(coll.tokens.shape[0]))
     achieved = alg / (kern_ms * 1e-3) / 1e9
     traffic = None
-    tfile = ROOT / "profiles" / "fgss_1M_t0.8_traffic.json"
+    tfile = ROOT / "profiles" / "k_fgss_1M_jaccard0.8_traffic.json"
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
 

@@ -89,9 +89,10 @@ struct hs_coll {
   uint64_t n_tokens = 0;
   uint32_t universe = 0;
   uint32_t maxlen = 0;
-  DevBuf tokens, starts, sizes, size_first, sigs;
+  DevBuf tokens, starts, sizes, size_first, sigs, tiles;
   uint32_t sig_words = 0;
   std::vector<uint32_t> h_tokens, h_starts, h_sizes, h_size_first;
+  std::vector<uint32_t> h_tiles;  // tile plan: tile k = [h_tiles[k], h_tiles[k+1])
 
   hs::DevColl dev() const {
     return hs::DevColl{tokens.as<uint32_t>(), starts.as<uint32_t>(), sizes.as<uint32_t>(), n};
@@ -517,6 +518,22 @@ int hs_upload_collection(hs_ctx* ctx, const uint32_t* tokens, const uint32_t* st
       while (i < n && c->h_sizes[i] < l) ++i;
       c->h_size_first[l] = i;
     }
+    // tile plan for the prefix-filter kernel: greedy runs of at most
+    // TILE_PROBES probes and TILE_TOKENS tokens (a longer set gets a tile
+    // of its own)
+    c->h_tiles.clear();
+    for (uint32_t i = 0; i < n;) {
+      c->h_tiles.push_back(i);
+      uint64_t tok = 0;
+      uint32_t cnt = 0;
+      while (i < n && cnt < hs::TILE_PROBES && (cnt == 0 || tok + c->h_sizes[i] <= hs::TILE_TOKENS)) {
+        tok += c->h_sizes[i];
+        ++cnt;
+        ++i;
+      }
+    }
+    c->h_tiles.push_back(n);
+    c->tiles.ensure(c->h_tiles.size() * 4);
     c->tokens.ensure(std::max<uint64_t>(pos, 1) * 4);
     c->starts.ensure(std::max<uint32_t>(n, 1) * 4);
     c->sizes.ensure(std::max<uint32_t>(n, 1) * 4);
@@ -528,6 +545,7 @@ int hs_upload_collection(hs_ctx* ctx, const uint32_t* tokens, const uint32_t* st
     }
     HS_CHECK_HIP(hipMemcpy(c->size_first.ptr, c->h_size_first.data(), c->h_size_first.size() * 4,
                            hipMemcpyHostToDevice));
+    HS_CHECK_HIP(hipMemcpy(c->tiles.ptr, c->h_tiles.data(), c->h_tiles.size() * 4, hipMemcpyHostToDevice));
   });
   if (rc == HS_OK) *out = c.release();
   return rc;
@@ -613,7 +631,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         a.probe = probe->dev();
         a.indexed = indexed->dev();
         a.index = hs::DevIndex{ctx->ib.offsets.as<uint32_t>(), ctx->ib.set_ids.as<uint32_t>(),
-                               ctx->ib.positions.as<uint16_t>(), indexed->universe};
+                               ctx->ib.sstart.as<uint32_t>(), ctx->ib.meta.as<uint32_t>(), indexed->universe};
         a.bounds = ctx->dev_bounds;
         a.size_first = indexed->size_first.as<uint32_t>();
         a.size_first_len = (uint32_t)indexed->h_size_first.size();
@@ -623,12 +641,15 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         a.idx_last = ir.last;
         a.self = self;
         a.out = dev_out(ctx);
-        uint32_t ppt = tune.probes_per_tile;
+        const uint32_t ppt = tune.probes_per_tile;
         if (ppt == 0) {
-          // as many probes per workgroup as keep their tokens within the LDS stage
-          const uint64_t ntok = (uint64_t)probe->h_starts[pr.last] + probe->h_sizes[pr.last] - probe->h_starts[pr.first];
-          const double avg = (double)ntok / ((double)pr.last - pr.first + 1);
-          ppt = (uint32_t)std::min(256.0, std::max(4.0, 3072.0 / std::max(avg, 1.0)));
+          // the collection's tile plan, clipped to the probe range
+          const auto& T = probe->h_tiles;
+          const uint32_t ft = (uint32_t)(std::upper_bound(T.begin(), T.end(), pr.first) - T.begin()) - 1;
+          const uint32_t lt = (uint32_t)(std::upper_bound(T.begin(), T.end(), pr.last) - T.begin()) - 1;
+          a.tile_starts = probe->tiles.as<uint32_t>();
+          a.first_tile = ft;
+          a.n_plan_tiles = lt - ft + 1;
         }
         HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
         hs::launch_fgss(st, a, ppt);

@@ -9,7 +9,8 @@
 // of r with the index prefix of s and reach minoverlap(|r|, |s|).
 //
 // Design here:
-//   * a workgroup owns a tile of consecutive probes; their tokens are one
+//   * a workgroup owns a tile of consecutive probes (at most 256 probes and
+//     4096 tokens, planned once per collection); their tokens are one
 //     contiguous CSR range and are staged in LDS with coalesced loads;
 //   * the length window of a probe is an id interval (sets are sorted by
 //     size), so each (probe, prefix token) "item" binary-searches its slice
@@ -34,8 +35,9 @@
 
 namespace hs {
 
-constexpr int FG_TPB = 256;          // threads per workgroup, also max probes per tile
-constexpr uint32_t FG_TOKCAP = 4096;  // staged probe tokens (16 KiB)
+constexpr int FG_TPB = 256;                   // threads per workgroup, also max probes per tile
+constexpr uint32_t FG_TOKCAP = TILE_TOKENS;    // staged probe tokens (16 KiB)
+static_assert(TILE_PROBES == FG_TPB, "one probe per thread in phase A");
 constexpr uint32_t FG_ITEMCAP = 1024; // items per batch
 constexpr uint32_t NO_LDS = 0xFFFFFFFFu;
 
@@ -97,8 +99,15 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
   if (tile == 0xFFFFFFFFu) return;
   const uint32_t tid = threadIdx.x;
-  const uint32_t p0 = A.probe_first + tile * ppt;
-  const uint32_t p1 = min(p0 + ppt, A.probe_last + 1);
+  uint32_t p0, p1;
+  if (A.tile_starts) {
+    const uint32_t t = A.first_tile + tile;
+    p0 = max(A.tile_starts[t], A.probe_first);
+    p1 = min(A.tile_starts[t + 1], A.probe_last + 1);
+  } else {
+    p0 = A.probe_first + tile * ppt;
+    p1 = min(p0 + ppt, A.probe_last + 1);
+  }
   const uint32_t np = p1 - p0;
 
   if (tid < 4) s_stats[tid] = 0;
@@ -152,9 +161,24 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
         if (token < A.index.universe) {
           const uint32_t lb = A.index.offsets[token];
           const uint32_t le = A.index.offsets[token + 1];
-          a0 = lower_bound_u32(A.index.set_ids, lb, le, s_lo[pl]);
-          const uint32_t b0 = lower_bound_u32(A.index.set_ids, a0, le, s_hi[pl]);
-          len = b0 - a0;
+          const uint32_t lo = s_lo[pl], hi = s_hi[pl];
+          if (le - lb <= 8) {
+            // short list: independent loads, count ids below each bound
+            uint32_t below_lo = 0, below_hi = 0;
+#pragma unroll
+            for (uint32_t q = 0; q < 8; ++q) {
+              if (lb + q < le) {
+                const uint32_t id = A.index.set_ids[lb + q];
+                below_lo += id < lo;
+                below_hi += id < hi;
+              }
+            }
+            a0 = lb + below_lo;
+            len = below_hi - below_lo;
+          } else {
+            a0 = lower_bound_u32(A.index.set_ids, lb, le, lo);
+            len = lower_bound_u32(A.index.set_ids, a0, le, hi) - a0;
+          }
         }
         s_it_a[it] = a0;
         s_it_pj[it] = pl | (j << 8);
@@ -190,21 +214,22 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
       if (vi < total_vis) {
         const uint32_t it = segment_of(s_vis_off, nb, vi);
         const uint32_t k = vi - s_vis_off[it];
-        sid = A.index.set_ids[s_it_a[it] + k];
-        const uint32_t kp = A.index.positions[s_it_a[it] + k];
+        const uint32_t e = s_it_a[it] + k;
+        sid = A.index.set_ids[e];
+        const uint32_t sst = A.index.sstart[e];
+        const uint32_t meta = A.index.meta[e];
+        const uint32_t kp = meta & 0xFFFFu;
         const uint32_t pj = s_it_pj[it];
         const uint32_t pl = pj & 0xFFu;
         const uint32_t j = pj >> 8;
         rid = p0 + pl;
         const uint32_t lr = s_psize[pl];
-        const uint32_t ls = A.indexed.sizes[sid];
+        const uint32_t ls = (meta >> 16) == 0xFFFFu ? A.indexed.sizes[sid] : (meta >> 16);
         const uint32_t mo = dev_minoverlap(A.bounds, lr, ls);
         // Positional filter (helpers.hpp:46-50) at this visit's token: if it is
         // the first common token, |r n s| <= 1 + min(|r|-1-j, |s|-1-kp).  Only
         // the first-common visit can emit the pair, so a visit failing the
-        // bound is dropped without reading s.  (starts is loaded beside sizes
-        // to shorten the dependent-load chain.)
-        const uint32_t sst = A.indexed.starts[sid];
+        // bound is dropped without reading s.
         if (kp == 0xFFFFu || 1u + min(lr - 1u - j, ls - 1u - kp) >= mo) {
           checked_len = ls;  // a verified candidate
           const uint32_t* sp = A.indexed.tokens + sst;
@@ -240,7 +265,7 @@ void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt) {
   if (a.probe.n == 0 || a.probe_first > a.probe_last) return;
   if (ppt == 0 || ppt > (uint32_t)FG_TPB) ppt = FG_TPB;
   const uint32_t nprobes = a.probe_last - a.probe_first + 1;
-  const uint32_t ntiles = (nprobes + ppt - 1) / ppt;
+  const uint32_t ntiles = a.tile_starts ? a.n_plan_tiles : (nprobes + ppt - 1) / ppt;
   const uint32_t grid = (ntiles + 7) / 8 * 8;
   hipLaunchKernelGGL(k_fgss, dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
   HS_CHECK_HIP(hipGetLastError());

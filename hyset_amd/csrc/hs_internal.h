@@ -96,10 +96,15 @@ struct DevColl {
   uint32_t n;
 };
 
+// Index entry e of list(t) = [offsets[t], offsets[t+1]): set_ids[e] (lists
+// ascending), sstart[e] = CSR start of that set, meta[e] = (min(|s|, 0xFFFF)
+// << 16) | min(position of t in s, 0xFFFF).  Start and size ride along so a
+// posting visit reads s without first reading starts[] and sizes[].
 struct DevIndex {
-  const uint32_t* offsets;    // universe + 1
-  const uint32_t* set_ids;    // entries, sorted by set id inside every list
-  const uint16_t* positions;  // position of the token inside its set (0xFFFF: >= 65535)
+  const uint32_t* offsets;  // universe + 1
+  const uint32_t* set_ids;
+  const uint32_t* sstart;
+  const uint32_t* meta;
   uint32_t universe;
 };
 
@@ -124,7 +129,7 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
                            DevBuf& scratch);
 
 struct IndexBuild {
-  DevBuf offsets, set_ids, positions;  // the index
+  DevBuf offsets, set_ids, positions, sstart, meta;  // the index
   DevBuf keys[2], vals, pos, hist, scan_scratch;  // radix-sort work space
   uint32_t universe = 0;
   uint64_t entries = 0;
@@ -147,8 +152,15 @@ struct FgssArgs {
   uint32_t probe_first, probe_last;  // inclusive
   uint32_t idx_first, idx_last;      // inclusive
   int self;                          // 1: restrict to s < r
+  // tile plan of the probe collection: tile k covers probes
+  // [tile_starts[k], tile_starts[k+1]) (<= 256 probes, <= 4096 tokens);
+  // NULL selects fixed tiles of probes_per_tile probes
+  const uint32_t* tile_starts;
+  uint32_t first_tile, n_plan_tiles;
   DevOut out;
 };
+constexpr uint32_t TILE_TOKENS = 4096;  // probe tokens staged in LDS per tile
+constexpr uint32_t TILE_PROBES = 256;
 void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile);
 
 struct BitmapArgs {

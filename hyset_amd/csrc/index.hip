@@ -10,6 +10,8 @@
 // the positional filter.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "hs_internal.h"
 
@@ -175,7 +177,8 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 //    Per pass: per-tile digit histograms (LDS atomics), a device scan of the
 //    digit-major histogram table, and a scatter that ranks each element
 //    inside its tile with wave ballots (no global atomics anywhere).
-// 3. offsets[t] = first entry with token >= t (binary search per token).
+// 3. offsets[t] = first entry with token >= t (binary search per token), and
+//    the start and size of each entry's set.
 
 constexpr int RS_TPB = 256;
 constexpr int RS_ROUNDS = 16;
@@ -267,12 +270,20 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
   }
 }
 
-// offsets[t] = number of entries with token < t, for t in [0, universe].
-__global__ void k_index_offsets(const uint32_t* __restrict__ keys, uint32_t n, uint32_t universe,
-                                uint32_t* __restrict__ offsets) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t > universe) return;
-  offsets[t] = lower_bound_u32(keys, 0, n, t);
+// offsets[t] = number of entries with token < t, for t in [0, universe];
+// and, for every entry, the CSR start and size of its set next to the
+// token position (one random read per entry here instead of per visit).
+__global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, uint32_t universe,
+                               uint32_t* __restrict__ offsets, DevColl c, const uint32_t* __restrict__ set_ids,
+                               const uint16_t* __restrict__ positions, uint32_t* __restrict__ sstart,
+                               uint32_t* __restrict__ meta) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i <= universe) offsets[i] = lower_bound_u32(keys, 0, n, i);
+  if (i < n) {
+    const uint32_t sid = set_ids[i];
+    sstart[i] = c.starts[sid];
+    meta[i] = (min(c.sizes[sid], 0xFFFFu) << 16) | positions[i];
+  }
 }
 
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
@@ -286,6 +297,8 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
   ib.offsets.ensure(((size_t)universe + 1) * sizeof(uint32_t));
   ib.set_ids.ensure(cap * sizeof(uint32_t));
   ib.positions.ensure(cap * sizeof(uint16_t));
+  ib.sstart.ensure(cap * sizeof(uint32_t));
+  ib.meta.ensure(cap * sizeof(uint32_t));
   ib.keys[0].ensure(cap * sizeof(uint32_t));
   ib.keys[1].ensure(cap * sizeof(uint32_t));
   ib.vals.ensure(cap * sizeof(uint32_t));
@@ -316,8 +329,10 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
                          shift, ib.hist.as<uint32_t>(), ntiles, kbuf[cur ^ 1], vbuf[cur ^ 1], pbuf[cur ^ 1]);
       cur ^= 1;
     }
-    hipLaunchKernelGGL(k_index_offsets, dim3((universe + 1 + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
-                       ib.offsets.as<uint32_t>());
+    const uint32_t span = std::max(universe + 1, n);
+    hipLaunchKernelGGL(k_index_finish, dim3((span + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
+                       ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(),
+                       ib.sstart.as<uint32_t>(), ib.meta.as<uint32_t>());
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }
