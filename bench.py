@@ -40,6 +40,77 @@ def fgss_bytes(st: dict, n_probes: int, n_probe_tokens: int) -> int:
             + 8 * st["candidates"] + 4 * st["candidate_tokens"] + 12 * st["pairs"])
 
 
+def bench_batched(ctx, H) -> dict:
+    """512 independent Jaccard-0.75 self-joins (the batch workload): one
+    persistent launch over all of them vs one hs_join per join."""
+    import numpy as np
+
+    from hyset_amd import workloads as W
+    t0 = time.perf_counter()
+    colls = W.batch()
+    log(f"batch: {len(colls)} joins, {sum(c.n for c in colls)} sets, "
+        f"{sum(int(c.tokens.shape[0]) for c in colls)} tokens, generated in {time.perf_counter() - t0:.1f}s")
+    ctx.set_threshold(0.75)
+    singles = [ctx.upload(c) for c in colls]
+    b = ctx.upload_batch(colls)
+    # plain: one join (index build + launch + count read-back) per collection
+    for d in singles[:8]:
+        ctx.join(d, count_only=True)
+    plain_counts, plain_ms = [], []
+    for rep in range(2):
+        a = time.perf_counter()
+        counts = [ctx.join(d, count_only=True) for d in singles]
+        plain_ms.append((time.perf_counter() - a) * 1e3)
+        plain_counts = counts
+    plain = min(plain_ms)
+    total = sum(plain_counts)
+    res = {"batch_joins": len(colls), "batch_sets": int(sum(c.n for c in colls)), "batch_pairs": int(total),
+           "batch_plain_ms": round(plain, 2)}
+
+    def timed(tune, reps=3):
+        ctx.join_batch(b, tune=tune, count_only=True)
+        ms = []
+        for _ in range(reps):
+            a = time.perf_counter()
+            c = ctx.join_batch(b, tune=tune, count_only=True)
+            ms.append((time.perf_counter() - a) * 1e3)
+        return c, min(ms)
+
+    c, ms = timed(None)
+    assert c == total, (c, total)
+    res["batch_persistent_ms"] = round(ms, 2)
+    c, ms_g = timed(H.Tune(flags=H.TUNE_GRAPH))
+    assert c == total
+    res["batch_persistent_graph_ms"] = round(ms_g, 2)
+    tune, _ = ctx.autotune_batch(b, reps=1)
+    tune.flags = H.TUNE_GRAPH
+    c, ms_t = timed(tune)
+    assert c == total
+    res["batch_autotuned_ms"] = round(ms_t, 2)
+    res["batch_autotuned_config"] = {"probes_per_tile": tune.probes_per_tile, "tile_tokens": tune.tile_tokens,
+                                     "workgroups_per_cu": tune.workgroups_per_cu}
+    best = min(ms, ms_g, ms_t)
+    res["batch_speedup_vs_plain"] = round(plain / best, 2)
+    # per-join result parity of the batched form against the plain form on a sample of joins
+    r, s, ov = ctx.join_batch(b)
+    jf = b.join_first
+    jr = np.searchsorted(jf, r, side="right") - 1
+    ok = bool(np.array_equal(np.bincount(jr, minlength=len(colls)), np.array(plain_counts)))
+    for j in (0, 7, 100, 311, 511):
+        m = jr == j
+        want = H.sort_pairs(*ctx.join(singles[j]))
+        got = H.sort_pairs(r[m] - jf[j], s[m] - jf[j], ov[m])
+        ok = ok and all(np.array_equal(x, y) for x, y in zip(got, want))
+    res["batch_parity_vs_plain"] = ok
+    log(f"batch: plain {plain:.1f} ms, persistent {ms:.1f} ms, +graph {ms_g:.1f} ms, autotuned {ms_t:.1f} ms "
+        f"({res['batch_autotuned_config']}), speedup {res['batch_speedup_vs_plain']}x, parity={ok}")
+    for d in singles:
+        d.free()
+    b.free()
+    ctx.set_threshold(0.8)
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -47,6 +118,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--quick", action="store_true", help="headline only: no side measurements, no CPU baseline")
     ap.add_argument("--threshold", type=float, default=0.8)
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-join measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
@@ -156,6 +228,9 @@ def main() -> int:
         extra["hybrid_ms_wall"] = round((time.perf_counter() - a) * 1e3, 2)
         assert c == pairs
         log(f"hybrid ({min(16, ncores)} host threads): {extra['hybrid_ms_wall']} ms")
+
+    if not args.quick and not args.no_batch:
+        extra.update(bench_batched(ctx, H))
 
     out = {
         "metric": "pairs/s (1M-set Jaccard>=0.8 self-join, fgss)",

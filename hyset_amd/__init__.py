@@ -52,7 +52,11 @@ class Range(C.Structure):
 class Tune(C.Structure):
     _fields_ = [("bitmap_bits", C.c_uint32), ("probes_per_tile", C.c_uint32),
                 ("verify_scenario", C.c_uint32), ("host_threads", C.c_uint32),
-                ("batch_candidates", C.c_uint64), ("flags", C.c_uint32)]
+                ("batch_candidates", C.c_uint64), ("flags", C.c_uint32), ("tile_tokens", C.c_uint32),
+                ("workgroups_per_cu", C.c_uint32)]
+
+
+TUNE_GRAPH = 1
 
 
 class Result(C.Structure):
@@ -96,6 +100,11 @@ _sig("hs_join", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.c_
 _sig("hs_verify_candidates", C.c_int, _vp, _vp, _vp, _u32p, _u32p, _u32p, C.c_uint32, C.c_int32,
      C.POINTER(Result))
 _sig("hs_copy_last_result", C.c_int, _vp, C.POINTER(Result))
+_u32pp = C.POINTER(_u32p)
+_sig("hs_upload_batch", C.c_int, _vp, C.c_uint32, _u32pp, _u32pp, _u32pp, _u32p, C.POINTER(C.c_uint64),
+     C.POINTER(_vp))
+_sig("hs_join_batch", C.c_int, _vp, _vp, C.POINTER(Tune), C.POINTER(Result))
+_sig("hs_autotune_batch", C.c_int, _vp, _vp, C.c_int32, C.POINTER(Tune), C.POINTER(C.c_double))
 _sig("hs_get_stats", C.c_int, _vp, C.POINTER(Stats))
 _sig("hs_copy_bandwidth", C.c_int, _vp, C.c_uint64, C.c_int32, C.POINTER(C.c_double))
 _sig("hs_bounds", C.c_int, C.c_int32, C.c_double, C.c_int32, C.c_uint32, C.c_uint32, _u32p)
@@ -273,6 +282,39 @@ class Context:
         self._check(_lib.hs_copy_last_result(self._h, C.byref(res)), "hs_copy_last_result")
         return r[:count], s[:count], ov[:count]
 
+    def upload_batch(self, colls) -> "DeviceCollection":
+        """One batched device collection from a list of SetCollections."""
+        arrs = [(_u32(c.tokens), _u32(c.starts), _u32(c.sizes)) for c in colls]
+        n = len(colls)
+        toks = (_u32p * n)(*[_ptr(a[0]) for a in arrs])
+        sts = (_u32p * n)(*[_ptr(a[1]) for a in arrs])
+        szs = (_u32p * n)(*[_ptr(a[2]) for a in arrs])
+        nsets = np.array([c.n for c in colls], dtype=np.uint32)
+        ntok = np.array([a[0].shape[0] for a in arrs], dtype=np.uint64)
+        h = _vp()
+        self._check(_lib.hs_upload_batch(self._h, n, toks, sts, szs, _ptr(nsets),
+                                         ntok.ctypes.data_as(C.POINTER(C.c_uint64)), C.byref(h)), "hs_upload_batch")
+        dc = DeviceCollection(h, None)
+        dc.join_first = np.concatenate([[0], np.cumsum(nsets.astype(np.int64))])
+        self._colls.append(dc)
+        return dc
+
+    def join_batch(self, batch: "DeviceCollection", tune: Tune | None = None, count_only=False):
+        """Self-joins every collection of a batch; pairs are in batch ids."""
+        res = Result()
+        res.count_only = 1
+        self._check(_lib.hs_join_batch(self._h, batch._h, C.byref(tune) if tune else None, C.byref(res)),
+                    "hs_join_batch")
+        if count_only:
+            return int(res.count)
+        return self.last_result(int(res.count))
+
+    def autotune_batch(self, batch: "DeviceCollection", reps=2):
+        t = Tune()
+        ms = C.c_double()
+        self._check(_lib.hs_autotune_batch(self._h, batch._h, reps, C.byref(t), C.byref(ms)), "hs_autotune_batch")
+        return t, ms.value
+
     def verify_candidates(self, indexed, probe, probes, offsets, candidates, scenario=0):
         pr, of, ca = _u32(probes), _u32(offsets), _u32(candidates)
         res = Result()
@@ -320,7 +362,7 @@ class DeviceCollection:
 
     @property
     def n(self):
-        return self.host.n
+        return self.host.n if self.host is not None else int(self.join_first[-1])
 
     def free(self):
         if self._h:

@@ -1,0 +1,219 @@
+// Context and collection state shared by the host-side translation units
+// of libhyset_amd.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hs_internal.h"
+
+using hs::DevBuf;
+using hs::Error;
+
+// Page-locked host staging buffer (hipHostMalloc; grow-only).
+struct PinnedBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t need) {
+    if (need <= bytes && ptr) return;
+    release();
+    size_t want = need < 256 ? 256 : need + need / 8;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      throw Error{HS_E_OOM, "hipHostMalloc failed"};
+    }
+    ptr = p;
+    bytes = want;
+  }
+  void release() {
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  template <typename T>
+  T* as() const { return static_cast<T*>(ptr); }
+  ~PinnedBuf() { release(); }
+};
+
+// ------------------------------------------------------------ structures
+
+struct hs_coll {
+  int device = 0;
+  uint32_t n = 0;
+  uint64_t n_tokens = 0;
+  uint32_t universe = 0;
+  uint32_t maxlen = 0;
+  DevBuf tokens, starts, sizes, size_first, sigs, tiles;
+  uint32_t sig_words = 0;
+  std::vector<uint32_t> h_tokens, h_starts, h_sizes, h_size_first;
+  std::vector<uint32_t> h_tiles;  // tile plan: tile k = [h_tiles[k], h_tiles[k+1])
+  // batched collections (hs_upload_batch): join j owns set ids
+  // [h_join_first[j], h_join_first[j+1]) and its size table
+  // h_size_first[h_sf_base[j] .. h_sf_base[j+1]) (global set ids); tiles of
+  // the batched launch in queue order
+  uint32_t n_joins = 0;
+  std::vector<uint32_t> h_join_first, h_sf_base, h_join_maxlen, h_join_order;
+  std::vector<uint32_t> h_tile_lo, h_tile_hi, h_tile_join;
+  DevBuf join_first, sf_base, tile_lo, tile_hi, tile_join;
+  uint32_t plan_probes = 0, plan_tokens = 0;
+
+  hs::DevColl dev() const {
+    return hs::DevColl{tokens.as<uint32_t>(), starts.as<uint32_t>(), sizes.as<uint32_t>(), n};
+  }
+};
+
+// One double-buffered slot of the hybrid candidate pipeline.
+struct HybridSlot {
+  PinnedBuf h_probes, h_offsets, h_cands;
+  DevBuf d_probes, d_offsets, d_cands;
+  hipEvent_t done = nullptr;
+  bool used = false;
+};
+
+struct hs_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  bool have_threshold = false;
+  int kind = HS_JACCARD, mode = HS_BOUNDS_REFERENCE;
+  double t = 0.0;
+  hs::BoundTables tables;
+  bool tables_valid = false;
+  DevBuf d_tables;
+  hs::DevBounds dev_bounds{};
+  hs::IndexBuild ib;
+  DevBuf out_r, out_s, out_ov, counters;
+  uint64_t out_cap = 0;
+  PinnedBuf h_counters, h_cum;
+  DevBuf d_cum;
+  hipEvent_t ev[6] = {};
+  hs_stats stats{};
+  uint64_t last_count = 0;
+  HybridSlot slots[2];
+  std::mutex submit_mu;
+  std::vector<uint32_t> h_offsets, h_set_ids;  // host copy of the index (hybrid)
+  int num_cus = 256;
+  DevBuf batch_queue;  // per-XCD tile counters of the batched launch
+  // captured batched join: replayed while (collection, bounds, plan) match
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  const hs_coll* graph_coll = nullptr;
+  std::string graph_key;
+  PinnedBuf graph_cum;
+  uint64_t graph_entries = 0;
+};
+
+
+namespace hs {
+
+template <typename F>
+inline int guarded(hs_ctx* ctx, F&& body) {
+  try {
+    if (ctx) {
+      hipError_t e = hipSetDevice(ctx->device);
+      if (e != hipSuccess) throw Error{HS_E_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e)};
+    }
+    body();
+    if (ctx) ctx->err.clear();
+    return HS_OK;
+  } catch (const Error& e) {
+    if (ctx) ctx->err = e.msg;
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    if (ctx) ctx->err = "host allocation failed";
+    return HS_E_OOM;
+  } catch (const std::exception& e) {
+    if (ctx) ctx->err = e.what();
+    return HS_E_INVALID;
+  } catch (...) {
+    if (ctx) ctx->err = "unknown error";
+    return HS_E_INVALID;
+  }
+}
+
+inline float elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0.f;
+  }
+  return ms;
+}
+
+// Bound tables covering set sizes up to L, uploaded to the device.
+inline void ensure_bounds(hs_ctx* ctx, uint32_t L) {
+  HS_REQUIRE(ctx->have_threshold, HS_E_INVALID, "hs_set_threshold has not been called");
+  if (ctx->tables_valid && ctx->tables.L >= L) return;
+  uint32_t want = std::max<uint32_t>(L, 64);
+  ctx->tables.build(ctx->kind, ctx->mode, ctx->t, want);
+  const auto& T = ctx->tables;
+  const size_t nl = T.L + 2;
+  const size_t total = 4 * nl + T.mo.size();
+  std::vector<uint32_t> flat(total);
+  std::copy(T.minsize.begin(), T.minsize.end(), flat.begin());
+  std::copy(T.maxsize.begin(), T.maxsize.end(), flat.begin() + nl);
+  std::copy(T.maxprefix.begin(), T.maxprefix.end(), flat.begin() + 2 * nl);
+  std::copy(T.midprefix.begin(), T.midprefix.end(), flat.begin() + 3 * nl);
+  std::copy(T.mo.begin(), T.mo.end(), flat.begin() + 4 * nl);
+  ctx->d_tables.ensure(total * sizeof(uint32_t));
+  HS_CHECK_HIP(hipMemcpy(ctx->d_tables.ptr, flat.data(), total * sizeof(uint32_t), hipMemcpyHostToDevice));
+  const uint32_t* base = ctx->d_tables.as<uint32_t>();
+  ctx->dev_bounds = hs::DevBounds{base, base + nl, base + 2 * nl, base + 3 * nl, base + 4 * nl, T.L, T.kind};
+  ctx->tables_valid = true;
+}
+
+inline void ensure_output(hs_ctx* ctx, uint64_t cap) {
+  if (cap <= ctx->out_cap && ctx->out_r.ptr) return;
+  ctx->out_r.ensure(cap * sizeof(uint32_t));
+  ctx->out_s.ensure(cap * sizeof(uint32_t));
+  ctx->out_ov.ensure(cap * sizeof(uint32_t));
+  ctx->out_cap = cap;
+}
+
+inline hs::DevOut dev_out(hs_ctx* ctx) {
+  return hs::DevOut{ctx->out_r.as<uint32_t>(), ctx->out_s.as<uint32_t>(), ctx->out_ov.as<uint32_t>(),
+                    ctx->counters.as<unsigned long long>(), ctx->out_cap};
+}
+
+inline void reset_counters(hs_ctx* ctx) {
+  hs::launch_fill_u64(ctx->stream, ctx->counters.as<unsigned long long>(), hs::CNT_WORDS, 0);
+}
+
+inline void read_counters(hs_ctx* ctx, unsigned long long* dst) {
+  HS_CHECK_HIP(hipMemcpyAsync(ctx->h_counters.ptr, ctx->counters.ptr, hs::CNT_WORDS * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::memcpy(dst, ctx->h_counters.ptr, hs::CNT_WORDS * sizeof(unsigned long long));
+}
+
+inline void copy_pairs(hs_ctx* ctx, uint64_t count, hs_result* res) {
+  if (count == 0) return;
+  HS_REQUIRE(res->r && res->s, HS_E_INVALID, "result arrays are NULL");
+  HS_CHECK_HIP(hipMemcpyAsync(res->r, ctx->out_r.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  HS_CHECK_HIP(hipMemcpyAsync(res->s, ctx->out_s.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+  if (res->overlap)
+    HS_CHECK_HIP(hipMemcpyAsync(res->overlap, ctx->out_ov.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                ctx->stream));
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+inline hs_range clamp_range(const hs_range* r, uint32_t n) {
+  hs_range out{0, 0};
+  if (n == 0) return hs_range{1, 0};
+  if (!r) return hs_range{0, n - 1};
+  out = *r;
+  if (out.last >= n) out.last = n - 1;
+  return out;
+}
+
+inline bool range_empty(const hs_range& r) { return r.first > r.last; }
+
+
+}  // namespace hs

@@ -82,68 +82,69 @@ __device__ __forceinline__ bool verify_visit(const uint32_t* __restrict__ r, uin
   return found >= mo;
 }
 
-__global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint32_t ntiles) {
-  __shared__ uint32_t s_tok[FG_TOKCAP];
-  __shared__ uint32_t s_pstart[FG_TPB];
-  __shared__ uint32_t s_psize[FG_TPB];
-  __shared__ uint32_t s_plds[FG_TPB];
-  __shared__ uint32_t s_lo[FG_TPB];
-  __shared__ uint32_t s_hi[FG_TPB];
-  __shared__ uint32_t s_item_off[FG_TPB + 1];
-  __shared__ uint32_t s_it_a[FG_ITEMCAP];
-  __shared__ uint32_t s_it_pj[FG_ITEMCAP];
-  __shared__ uint32_t s_vis_off[FG_ITEMCAP + 1];
-  __shared__ uint32_t s_tmp[FG_TPB / WAVE + 1];
-  __shared__ unsigned long long s_stats[4];
+// Workgroup-shared state of one tile.
+struct FgssLds {
+  uint32_t tok[FG_TOKCAP];
+  uint32_t pstart[FG_TPB];
+  uint32_t psize[FG_TPB];
+  uint32_t plds[FG_TPB];
+  uint32_t lo[FG_TPB];
+  uint32_t hi[FG_TPB];
+  uint32_t item_off[FG_TPB + 1];
+  uint32_t it_a[FG_ITEMCAP];
+  uint32_t it_pj[FG_ITEMCAP];
+  uint32_t vis_off[FG_ITEMCAP + 1];
+  uint32_t tmp[FG_TPB / WAVE + 1];
+  unsigned long long stats[4];
+  uint32_t next;
+};
 
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
-  if (tile == 0xFFFFFFFFu) return;
+// The probes of a tile and the part of the indexed collection they join.
+struct TileScope {
+  uint32_t p0, p1;                // probes [p0, p1)
+  uint32_t idx_first, idx_last;   // indexed ids, inclusive
+  const uint32_t* size_first;     // first indexed id of size >= l
+  uint32_t size_first_len;
+};
+
+__device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T, FgssLds& L, uint32_t& my_cands,
+                                          uint32_t& my_cand_tokens) {
   const uint32_t tid = threadIdx.x;
-  uint32_t p0, p1;
-  if (A.tile_starts) {
-    const uint32_t t = A.first_tile + tile;
-    p0 = max(A.tile_starts[t], A.probe_first);
-    p1 = min(A.tile_starts[t + 1], A.probe_last + 1);
-  } else {
-    p0 = A.probe_first + tile * ppt;
-    p1 = min(p0 + ppt, A.probe_last + 1);
-  }
-  const uint32_t np = p1 - p0;
-
-  if (tid < 4) s_stats[tid] = 0;
+  const uint32_t p0 = T.p0;
+  const uint32_t np = T.p1 - T.p0;
 
   // ---- phase A: probe metadata, windows, LDS staging of probe tokens
   const uint32_t base = A.probe.starts[p0];
-  const uint32_t tok_end = A.probe.starts[p1 - 1] + A.probe.sizes[p1 - 1];
+  const uint32_t tok_end = A.probe.starts[T.p1 - 1] + A.probe.sizes[T.p1 - 1];
   const uint32_t ntok = tok_end - base;
   uint32_t n_items = 0;
   if (tid < np) {
     const uint32_t p = p0 + tid;
     const uint32_t l = A.probe.sizes[p];
     const uint32_t st = A.probe.starts[p];
-    const uint32_t last = A.size_first_len - 1;
+    const uint32_t last = T.size_first_len - 1;
     const uint32_t mins = min(A.bounds.minsize[l], last);
     const uint32_t maxs1 = min(A.bounds.maxsize[l], last - 1) + 1;
-    uint32_t lo = max(A.size_first[mins], A.idx_first);
-    uint32_t hi = min(A.size_first[maxs1], A.idx_last + 1);
+    uint32_t lo = max(T.size_first[mins], T.idx_first);
+    uint32_t hi = min(T.size_first[maxs1], T.idx_last + 1);
     if (A.self) hi = min(hi, p);
-    s_pstart[tid] = st;
-    s_psize[tid] = l;
-    s_lo[tid] = lo;
-    s_hi[tid] = hi;
+    L.pstart[tid] = st;
+    L.psize[tid] = l;
+    L.lo[tid] = lo;
+    L.hi[tid] = hi;
     const uint32_t off = st - base;
-    s_plds[tid] = (off + l <= FG_TOKCAP) ? off : NO_LDS;
+    L.plds[tid] = (off + l <= FG_TOKCAP) ? off : NO_LDS;
     n_items = (lo < hi) ? A.bounds.maxprefix[l] : 0;
   }
-  for (uint32_t i = tid; i < ntok && i < FG_TOKCAP; i += FG_TPB) s_tok[i] = A.probe.tokens[base + i];
+  for (uint32_t i = tid; i < ntok && i < FG_TOKCAP; i += FG_TPB) L.tok[i] = A.probe.tokens[base + i];
   uint32_t total_items;
-  uint32_t ioff = block_exclusive_scan<FG_TPB>(n_items, s_tmp, &total_items);
-  if (tid < np) s_item_off[tid] = ioff;
-  if (tid == 0) s_item_off[np] = total_items;
+  uint32_t ioff = block_exclusive_scan<FG_TPB>(n_items, L.tmp, &total_items);
+  if (tid < np) L.item_off[tid] = ioff;
+  if (tid == 0) {
+    L.item_off[np] = total_items;
+    L.stats[3] += total_items;
+  }
   __syncthreads();
-
-  uint32_t my_cands = 0, my_cand_tokens = 0;
-  if (tid == 0) s_stats[3] = total_items;
 
   // ---- phase B: batches of items
   for (uint32_t ib = 0; ib < total_items; ib += FG_ITEMCAP) {
@@ -153,15 +154,15 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
       uint32_t len = 0;
       if (it < nb) {
         const uint32_t item = ib + it;
-        const uint32_t pl = segment_of(s_item_off, np, item);
-        const uint32_t j = item - s_item_off[pl];
-        const uint32_t* r = (s_plds[pl] != NO_LDS) ? (s_tok + s_plds[pl]) : (A.probe.tokens + s_pstart[pl]);
+        const uint32_t pl = segment_of(L.item_off, np, item);
+        const uint32_t j = item - L.item_off[pl];
+        const uint32_t* r = (L.plds[pl] != NO_LDS) ? (L.tok + L.plds[pl]) : (A.probe.tokens + L.pstart[pl]);
         const uint32_t token = r[j];
         uint32_t a0 = 0;
         if (token < A.index.universe) {
           const uint32_t lb = A.index.offsets[token];
           const uint32_t le = A.index.offsets[token + 1];
-          const uint32_t lo = s_lo[pl], hi = s_hi[pl];
+          const uint32_t lo = L.lo[pl], hi = L.hi[pl];
           if (le - lb <= 8) {
             // short list: independent loads, count ids below each bound
             uint32_t below_lo = 0, below_hi = 0;
@@ -180,10 +181,10 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
             len = lower_bound_u32(A.index.set_ids, a0, le, hi) - a0;
           }
         }
-        s_it_a[it] = a0;
-        s_it_pj[it] = pl | (j << 8);
+        L.it_a[it] = a0;
+        L.it_pj[it] = pl | (j << 8);
       }
-      s_vis_off[it] = len;
+      L.vis_off[it] = len;
     }
     __syncthreads();
     // exclusive scan of the slice lengths (4 per thread)
@@ -192,19 +193,21 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
     uint32_t local = 0;
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-      v[k] = s_vis_off[tid * PER + k];
+      v[k] = L.vis_off[tid * PER + k];
       local += v[k];
     }
     uint32_t total_vis;
-    uint32_t run = block_exclusive_scan<FG_TPB>(local, s_tmp, &total_vis);
+    uint32_t run = block_exclusive_scan<FG_TPB>(local, L.tmp, &total_vis);
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
-      s_vis_off[tid * PER + k] = run;
+      L.vis_off[tid * PER + k] = run;
       run += v[k];
     }
-    if (tid == 0) s_vis_off[FG_ITEMCAP] = total_vis;
+    if (tid == 0) {
+      L.vis_off[FG_ITEMCAP] = total_vis;
+      L.stats[0] += total_vis;
+    }
     __syncthreads();
-    if (tid == 0) s_stats[0] += total_vis;
 
     // one visit per lane
     for (uint32_t vb = 0; vb < total_vis; vb += FG_TPB) {
@@ -212,18 +215,18 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
       bool pass = false;
       uint32_t rid = 0, sid = 0, ov = 0, checked_len = 0;
       if (vi < total_vis) {
-        const uint32_t it = segment_of(s_vis_off, nb, vi);
-        const uint32_t k = vi - s_vis_off[it];
-        const uint32_t e = s_it_a[it] + k;
+        const uint32_t it = segment_of(L.vis_off, nb, vi);
+        const uint32_t k = vi - L.vis_off[it];
+        const uint32_t e = L.it_a[it] + k;
         sid = A.index.set_ids[e];
         const uint32_t sst = A.index.sstart[e];
         const uint32_t meta = A.index.meta[e];
         const uint32_t kp = meta & 0xFFFFu;
-        const uint32_t pj = s_it_pj[it];
+        const uint32_t pj = L.it_pj[it];
         const uint32_t pl = pj & 0xFFu;
         const uint32_t j = pj >> 8;
         rid = p0 + pl;
-        const uint32_t lr = s_psize[pl];
+        const uint32_t lr = L.psize[pl];
         const uint32_t ls = (meta >> 16) == 0xFFFFu ? A.indexed.sizes[sid] : (meta >> 16);
         const uint32_t mo = dev_minoverlap(A.bounds, lr, ls);
         // Positional filter (helpers.hpp:46-50) at this visit's token: if it is
@@ -233,9 +236,9 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
         if (kp == 0xFFFFu || 1u + min(lr - 1u - j, ls - 1u - kp) >= mo) {
           checked_len = ls;  // a verified candidate
           const uint32_t* sp = A.indexed.tokens + sst;
-          const uint32_t off = s_plds[pl];
-          if (off != NO_LDS) pass = verify_visit<true>(s_tok + off, lr, sp, ls, j, mo, &ov);
-          else pass = verify_visit<false>(A.probe.tokens + s_pstart[pl], lr, sp, ls, j, mo, &ov);
+          const uint32_t off = L.plds[pl];
+          if (off != NO_LDS) pass = verify_visit<true>(L.tok + off, lr, sp, ls, j, mo, &ov);
+          else pass = verify_visit<false>(A.probe.tokens + L.pstart[pl], lr, sp, ls, j, mo, &ov);
         }
       }
       my_cands += checked_len != 0;
@@ -244,20 +247,73 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
     }
     __syncthreads();
   }
+}
+
+// Pulls the next tile of a batched launch: each workgroup drains the queue
+// of its own XCD (a contiguous run of tiles, so neighbouring tiles share an
+// L2), then steals from the other XCDs' queues.
+__device__ __forceinline__ uint32_t next_batch_tile(const FgssBatch& B, uint32_t ntiles, uint32_t home) {
+  constexpr uint32_t NXCD = 8;
+  const uint32_t per = (ntiles + NXCD - 1) / NXCD;
+  for (uint32_t q = 0; q < NXCD; ++q) {
+    const uint32_t x = (home + q) % NXCD;
+    const uint32_t lo = x * per, hi = min(lo + per, ntiles);
+    if (lo >= hi) continue;
+    if (B.queue[x] >= hi - lo) continue;  // drained (read without claiming)
+    const uint32_t k = atomicAdd(&B.queue[x], 1u);
+    if (k < hi - lo) return lo + k;
+  }
+  return 0xFFFFFFFFu;
+}
+
+__global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint32_t ntiles) {
+  __shared__ FgssLds L;
+  const uint32_t tid = threadIdx.x;
+  uint32_t my_cands = 0, my_cand_tokens = 0;
+  if (A.batch.tile_join) {
+    // persistent: one launch for every tile of every join of the batch
+    if (tid < 4) L.stats[tid] = 0;
+    for (;;) {
+      __syncthreads();
+      if (tid == 0) L.next = next_batch_tile(A.batch, ntiles, blockIdx.x % 8);
+      __syncthreads();
+      const uint32_t k = L.next;
+      if (k == 0xFFFFFFFFu) break;
+      const uint32_t j = A.batch.tile_join[k];
+      TileScope T{A.batch.tile_lo[k], A.batch.tile_hi[k], A.batch.join_first[j], A.batch.join_first[j + 1] - 1,
+                  A.size_first + A.batch.sf_base[j], A.batch.sf_base[j + 1] - A.batch.sf_base[j]};
+      fgss_tile(A, T, L, my_cands, my_cand_tokens);
+    }
+  } else {
+    const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
+    if (tile == 0xFFFFFFFFu) return;
+    if (tid < 4) L.stats[tid] = 0;
+    uint32_t p0, p1;
+    if (A.tile_starts) {
+      const uint32_t t = A.first_tile + tile;
+      p0 = max(A.tile_starts[t], A.probe_first);
+      p1 = min(A.tile_starts[t + 1], A.probe_last + 1);
+    } else {
+      p0 = A.probe_first + tile * ppt;
+      p1 = min(p0 + ppt, A.probe_last + 1);
+    }
+    TileScope T{p0, p1, A.idx_first, A.idx_last, A.size_first, A.size_first_len};
+    fgss_tile(A, T, L, my_cands, my_cand_tokens);
+  }
 
   // ---- statistics: one global atomic per workgroup and counter
   my_cands = wave_sum(my_cands);
   my_cand_tokens = wave_sum(my_cand_tokens);
   if (lane_id() == 0 && my_cands) {
-    atomicAdd(&s_stats[1], (unsigned long long)my_cands);
-    atomicAdd(&s_stats[2], (unsigned long long)my_cand_tokens);
+    atomicAdd(&L.stats[1], (unsigned long long)my_cands);
+    atomicAdd(&L.stats[2], (unsigned long long)my_cand_tokens);
   }
   __syncthreads();
   if (tid == 0) {
-    if (s_stats[0]) atomicAdd(&A.out.counters[CNT_VISITS], s_stats[0]);
-    if (s_stats[1]) atomicAdd(&A.out.counters[CNT_CANDS], s_stats[1]);
-    if (s_stats[2]) atomicAdd(&A.out.counters[CNT_CAND_TOKENS], s_stats[2]);
-    if (s_stats[3]) atomicAdd(&A.out.counters[CNT_ITEMS], s_stats[3]);
+    if (L.stats[0]) atomicAdd(&A.out.counters[CNT_VISITS], L.stats[0]);
+    if (L.stats[1]) atomicAdd(&A.out.counters[CNT_CANDS], L.stats[1]);
+    if (L.stats[2]) atomicAdd(&A.out.counters[CNT_CAND_TOKENS], L.stats[2]);
+    if (L.stats[3]) atomicAdd(&A.out.counters[CNT_ITEMS], L.stats[3]);
   }
 }
 
@@ -268,6 +324,13 @@ void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt) {
   const uint32_t ntiles = a.tile_starts ? a.n_plan_tiles : (nprobes + ppt - 1) / ppt;
   const uint32_t grid = (ntiles + 7) / 8 * 8;
   hipLaunchKernelGGL(k_fgss, dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+  HS_CHECK_HIP(hipGetLastError());
+}
+
+void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups) {
+  if (ntiles == 0) return;
+  launch_fill_u32(st, a.batch.queue, 8, 0);
+  hipLaunchKernelGGL(k_fgss, dim3(workgroups), dim3(FG_TPB), 0, st, a, 0u, ntiles);
   HS_CHECK_HIP(hipGetLastError());
 }
 

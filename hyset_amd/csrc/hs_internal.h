@@ -141,6 +141,24 @@ struct IndexBuild {
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib);
+// The same over every join of a batched collection at once (token ids of
+// different joins are disjoint, so one index serves all joins).
+void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
+                       const uint32_t* prefix_len, const uint32_t* size_first, const uint32_t* sf_base,
+                       const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib);
+
+// Batched launch: tile k covers probes [tile_lo[k], tile_hi[k]) of join
+// tile_join[k]; join j owns set ids [join_first[j], join_first[j+1]) and its
+// size table size_first[sf_base[j] .. sf_base[j+1]); queue[8] are the
+// per-XCD tile counters of the device work queue.
+struct FgssBatch {
+  const uint32_t* tile_lo;
+  const uint32_t* tile_hi;
+  const uint32_t* tile_join;
+  const uint32_t* join_first;
+  const uint32_t* sf_base;
+  uint32_t* queue;
+};
 
 struct FgssArgs {
   DevColl probe;
@@ -157,11 +175,13 @@ struct FgssArgs {
   // NULL selects fixed tiles of probes_per_tile probes
   const uint32_t* tile_starts;
   uint32_t first_tile, n_plan_tiles;
+  FgssBatch batch;  // tile_join != NULL: persistent batched launch
   DevOut out;
 };
 constexpr uint32_t TILE_TOKENS = 4096;  // probe tokens staged in LDS per tile
 constexpr uint32_t TILE_PROBES = 256;
 void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile);
+void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups);
 
 struct BitmapArgs {
   DevColl probe;

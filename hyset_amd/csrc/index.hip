@@ -201,6 +201,28 @@ __global__ void k_index_emit(DevColl c, uint32_t first, uint32_t last, const uin
   }
 }
 
+// Batched collections: set i belongs to join j = last j with join_first[j]
+// <= i; its sizes table and entry table start at sf_base[j].
+__global__ void k_index_emit_batch(DevColl c, const uint32_t* __restrict__ join_first, uint32_t n_joins,
+                                   const uint32_t* __restrict__ prefix_len, const uint32_t* __restrict__ size_first,
+                                   const uint32_t* __restrict__ sf_base, const uint32_t* __restrict__ cum,
+                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                   uint16_t* __restrict__ pos) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.n) return;
+  const uint32_t j = segment_of(join_first, n_joins, i);
+  const uint32_t l = c.sizes[i];
+  const uint32_t pl = prefix_len[l];
+  const uint32_t b = sf_base[j];
+  const uint32_t o = cum[b + l] + (i - size_first[b + l]) * pl;
+  const uint32_t* t = c.tokens + c.starts[i];
+  for (uint32_t k = 0; k < pl; ++k) {
+    keys[o + k] = t[k];
+    vals[o + k] = i;
+    pos[o + k] = (uint16_t)min(k, 0xFFFFu);
+  }
+}
+
 __global__ __launch_bounds__(RS_TPB) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
                                                        uint32_t* __restrict__ hist, uint32_t ntiles) {
   __shared__ uint32_t h[256];
@@ -286,11 +308,8 @@ __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, ui
   }
 }
 
-void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
-                 const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                 IndexBuild& ib) {
+static void prepare(IndexBuild& ib, uint32_t universe, uint64_t entries) {
   HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
-  const uint32_t n = (uint32_t)entries;
   ib.universe = universe;
   ib.entries = entries;
   const size_t cap = entries ? entries : 1;
@@ -303,36 +322,78 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
   ib.keys[1].ensure(cap * sizeof(uint32_t));
   ib.vals.ensure(cap * sizeof(uint32_t));
   ib.pos.ensure(cap * sizeof(uint16_t));
+}
+
+static uint32_t radix_passes(uint32_t universe) {
+  uint32_t passes = 1;
+  while (passes < 4 && (uint64_t)(universe - 1) >> (8 * passes)) ++passes;
+  return passes;
+}
+
+// Sorts the emitted entries (in buffer `cur`) by token and writes the index.
+static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32_t universe, IndexBuild& ib,
+                            int cur, uint32_t passes) {
+  const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
+  ib.hist.ensure((size_t)256 * ntiles * sizeof(uint32_t));
+  uint32_t* kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
+  uint32_t* vbuf[2] = {ib.vals.as<uint32_t>(), ib.set_ids.as<uint32_t>()};
+  uint16_t* pbuf[2] = {ib.pos.as<uint16_t>(), ib.positions.as<uint16_t>()};
+  for (uint32_t p = 0; p < passes; ++p) {
+    const uint32_t shift = 8 * p;
+    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift, ib.hist.as<uint32_t>(),
+                       ntiles);
+    launch_exclusive_scan(st, ib.hist.as<uint32_t>(), ib.hist.as<uint32_t>(), nullptr, (size_t)256 * ntiles,
+                          ib.scan_scratch);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], vbuf[cur], pbuf[cur], n, shift,
+                       ib.hist.as<uint32_t>(), ntiles, kbuf[cur ^ 1], vbuf[cur ^ 1], pbuf[cur ^ 1]);
+    cur ^= 1;
+  }
+  const uint32_t span = std::max(universe + 1, n);
+  hipLaunchKernelGGL(k_index_finish, dim3((span + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
+                     ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(),
+                     ib.sstart.as<uint32_t>(), ib.meta.as<uint32_t>());
+  HS_CHECK_HIP(hipGetLastError());
+}
+
+// The emit writes into the buffer pair that makes the last radix pass land in
+// (keys[1], set_ids, positions): buffer 0 for an odd number of passes.
+static int emit_buffer(uint32_t passes) { return (passes % 2 == 1) ? 0 : 1; }
+
+void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
+                 const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
+                 IndexBuild& ib) {
+  prepare(ib, universe, entries);
+  const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && first <= last && n > 0) {
-    // passes: enough 8-bit digits for the largest token
-    uint32_t passes = 1;
-    while (passes < 4 && (uint64_t)(universe - 1) >> (8 * passes)) ++passes;
-    const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-    ib.hist.ensure((size_t)256 * ntiles * sizeof(uint32_t));
-    // ping-pong between (keys[0], vals, pos) and (keys[1], set_ids,
-    // positions); emit into the buffer that makes the last pass land in the
-    // second one
-    uint32_t* kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
-    uint32_t* vbuf[2] = {ib.vals.as<uint32_t>(), ib.set_ids.as<uint32_t>()};
-    uint16_t* pbuf[2] = {ib.pos.as<uint16_t>(), ib.positions.as<uint16_t>()};
-    int cur = (passes % 2 == 1) ? 0 : 1;
+    const uint32_t passes = radix_passes(universe);
+    const int cur = emit_buffer(passes);
+    uint32_t* kb = cur ? ib.keys[1].as<uint32_t>() : ib.keys[0].as<uint32_t>();
+    uint32_t* vb = cur ? ib.set_ids.as<uint32_t>() : ib.vals.as<uint32_t>();
+    uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
     const uint32_t nsets = last - first + 1;
     hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
-                       size_first, cum, kbuf[cur], vbuf[cur], pbuf[cur]);
-    for (uint32_t p = 0; p < passes; ++p) {
-      const uint32_t shift = 8 * p;
-      hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift,
-                         ib.hist.as<uint32_t>(), ntiles);
-      launch_exclusive_scan(st, ib.hist.as<uint32_t>(), ib.hist.as<uint32_t>(), nullptr, (size_t)256 * ntiles,
-                            ib.scan_scratch);
-      hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], vbuf[cur], pbuf[cur], n,
-                         shift, ib.hist.as<uint32_t>(), ntiles, kbuf[cur ^ 1], vbuf[cur ^ 1], pbuf[cur ^ 1]);
-      cur ^= 1;
-    }
-    const uint32_t span = std::max(universe + 1, n);
-    hipLaunchKernelGGL(k_index_finish, dim3((span + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
-                       ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(),
-                       ib.sstart.as<uint32_t>(), ib.meta.as<uint32_t>());
+                       size_first, cum, kb, vb, pb);
+    sort_and_finish(st, c, n, universe, ib, cur, passes);
+  } else {
+    launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
+  }
+  HS_CHECK_HIP(hipGetLastError());
+}
+
+void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
+                       const uint32_t* prefix_len, const uint32_t* size_first, const uint32_t* sf_base,
+                       const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib) {
+  prepare(ib, universe, entries);
+  const uint32_t n = (uint32_t)entries;
+  if (c.n > 0 && n > 0) {
+    const uint32_t passes = radix_passes(universe);
+    const int cur = emit_buffer(passes);
+    uint32_t* kb = cur ? ib.keys[1].as<uint32_t>() : ib.keys[0].as<uint32_t>();
+    uint32_t* vb = cur ? ib.set_ids.as<uint32_t>() : ib.vals.as<uint32_t>();
+    uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
+    hipLaunchKernelGGL(k_index_emit_batch, dim3((c.n + 255) / 256), dim3(256), 0, st, c, join_first, n_joins,
+                       prefix_len, size_first, sf_base, cum, kb, vb, pb);
+    sort_and_finish(st, c, n, universe, ib, cur, passes);
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }

@@ -147,8 +147,14 @@ typedef struct {
                                  candidate, 0 pick by set size (default 0) */
   uint32_t host_threads;      /* hybrid: CPU filter threads (default 1) */
   uint64_t batch_candidates;  /* hybrid: candidates per device batch (default 1<<22) */
-  uint32_t flags;             /* reserved, must be 0 */
+  uint32_t flags;             /* HS_TUNE_* bits (batched joins only) */
+  uint32_t tile_tokens;       /* batched fgss: probe tokens per tile, <= 4096 (default 4096) */
+  uint32_t workgroups_per_cu; /* batched fgss: persistent workgroups per CU (default 4) */
 } hs_tune;
+
+/* hs_join_batch: capture the device sequence in a HIP graph on first use and
+ * replay it while the collection, the bounds and the tile plan are unchanged. */
+enum { HS_TUNE_GRAPH = 1 };
 
 /* Result sink.  count is always written.  With count_only == 0 the pairs
  * (r = probe id, s = indexed id, overlap = |r n s|) are copied into the
@@ -212,6 +218,29 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
 int hs_verify_candidates(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const uint32_t* probes,
                          const uint32_t* offsets, const uint32_t* candidates, uint32_t n_probes,
                          int32_t scenario, hs_result* result);
+
+/* ------------------------------------------------------- batched joins */
+
+/* Uploads n_joins independent collections (each sorted by size, tokens
+ * ascending; starts[j] may be NULL for a dense CSR) as one batch.  Set ids of
+ * join j are offset by the number of sets of the joins before it, token ids
+ * by their universes, so results of hs_join_batch are in batch ids:
+ * join j owns ids [sum_{k<j} n_sets[k], sum_{k<=j} n_sets[k]).
+ * Replaces the block-pair queue of src/queue.cu:142-279 by one device work
+ * queue; see hs_join_batch. */
+int hs_upload_batch(hs_ctx* ctx, uint32_t n_joins, const uint32_t* const* tokens, const uint32_t* const* starts,
+                    const uint32_t* const* sizes, const uint32_t* n_sets, const uint64_t* n_tokens, hs_coll** out);
+
+/* Self-joins every collection of a batch with the context's threshold: one
+ * index build over all of them and one persistent prefix-filter launch that
+ * drains a device work queue of probe tiles.  tune: probes_per_tile,
+ * tile_tokens, workgroups_per_cu, flags (HS_TUNE_GRAPH). */
+int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune, hs_result* result);
+
+/* Times hs_join_batch over a grid of tile sizes and persistent-grid sizes
+ * (reps timed runs each after one warm-up) and returns the fastest setting;
+ * *best_ms receives its mean device time. */
+int hs_autotune_batch(hs_ctx* ctx, hs_coll* batch, int32_t reps, hs_tune* best, double* best_ms);
 
 /* Copies the pairs of the last join into result (count_only ignored). */
 int hs_copy_last_result(hs_ctx* ctx, hs_result* result);

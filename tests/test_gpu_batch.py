@@ -1,0 +1,87 @@
+"""Batched joins: one index build and one persistent launch over many
+independent self-joins give, join by join, the oracle's result."""
+import random
+
+import numpy as np
+import pytest
+
+import hyset_amd as H
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def make_batch(k, seed):
+    rnd = random.Random(seed)
+    colls = []
+    for j in range(k):
+        n = int(round(np.exp(rnd.uniform(np.log(50), np.log(4000)))))
+        colls.append(H.generate_self(H.gen_params(n, median=rnd.uniform(4, 20), sigma=0.6, lo=1, hi=120,
+                                                  universe=max(n, 20), zipf=rnd.uniform(0.8, 1.3), seed=seed + j)))
+    return colls
+
+
+def split(batch_dc, r, s, ov):
+    """Batch ids -> per-join local (r, s, overlap) triples, sorted."""
+    jf = batch_dc.join_first
+    out = {}
+    j_of = np.searchsorted(jf, r, side="right") - 1
+    assert np.array_equal(j_of, np.searchsorted(jf, s, side="right") - 1), "pair crosses joins"
+    for j in range(len(jf) - 1):
+        m = j_of == j
+        rr, ss, oo = H.sort_pairs(r[m] - jf[j], s[m] - jf[j], ov[m])
+        out[j] = list(zip(rr.tolist(), ss.tolist(), oo.tolist()))
+    return out
+
+
+def expected(colls, t):
+    return {j: list(zip(*(x.tolist() for x in (res.r, res.s, res.overlap))))
+            for j, res in enumerate(O.self_join(c, t) for c in colls)}
+
+
+@pytest.fixture(scope="module")
+def batch(ctx):
+    colls = make_batch(40, 100)
+    return colls, ctx.upload_batch(colls)
+
+
+@pytest.mark.parametrize("t", [0.6, 0.75, 0.9])
+def test_batch_equals_per_join_oracle(ctx, batch, t):
+    colls, b = batch
+    ctx.set_threshold(t)
+    want = expected(colls, t)
+    assert sum(len(v) for v in want.values()) > 100
+    got = split(b, *ctx.join_batch(b))
+    assert got == want
+
+
+def test_batch_tile_sizes_and_grid(ctx, batch):
+    colls, b = batch
+    ctx.set_threshold(0.75)
+    want = expected(colls, 0.75)
+    for probes, wpc in ((64, 2), (128, 8), (256, 1)):
+        tune = H.Tune(probes_per_tile=probes, tile_tokens=probes * 16, workgroups_per_cu=wpc)
+        assert split(b, *ctx.join_batch(b, tune=tune)) == want, (probes, wpc)
+
+
+def test_batch_graph_replays(ctx, batch):
+    """Replays of the captured graph, with eager joins in between, all equal
+    the eager result."""
+    colls, b = batch
+    ctx.set_threshold(0.75)
+    eager = split(b, *ctx.join_batch(b))
+    single = ctx.upload(colls[3])
+    tune = H.Tune(flags=H.TUNE_GRAPH)
+    for rep in range(4):
+        assert split(b, *ctx.join_batch(b, tune=tune)) == eager, rep
+        ctx.join(single)  # eager work between replays
+    ctx.set_threshold(0.6)  # a new threshold re-captures
+    assert split(b, *ctx.join_batch(b, tune=tune)) == expected(colls, 0.6)
+
+
+def test_autotune_batch(ctx, batch):
+    colls, b = batch
+    ctx.set_threshold(0.75)
+    tune, ms = ctx.autotune_batch(b, reps=1)
+    assert tune.probes_per_tile in (64, 128, 256) and tune.workgroups_per_cu in (2, 4, 8) and ms > 0
+    assert split(b, *ctx.join_batch(b, tune=tune)) == expected(colls, 0.75)
