@@ -111,6 +111,98 @@ def bench_batched(ctx, H) -> dict:
     return res
 
 
+def bench_serving(H) -> dict:
+    """Serving workload: device-resident index over 2,000,000 sets; requests of
+    round(exp(U(0, ln 4096))) query sets at thresholds drawn from {0.6, 0.7,
+    0.8, 0.9}; saturated throughput, then Poisson arrivals at half of it."""
+    import random
+    import threading
+
+    import numpy as np
+
+    from hyset_amd import workloads as W
+    t0 = time.perf_counter()
+    idx, pool = W.serving()
+    log(f"serving: index {idx.n} sets, query pool {pool.n} sets, generated in {time.perf_counter() - t0:.1f}s")
+    rnd = random.Random(2)
+    reqs, pos = [], 0
+    for _ in range(600):
+        n = max(1, int(round(np.exp(rnd.uniform(0, np.log(4096))))))
+        if pos + n > pool.n:
+            pos = 0
+        st = pool.starts[pos]
+        en = pool.starts[pos + n - 1] + pool.sizes[pos + n - 1]
+        qc = H.SetCollection(pool.tokens[st:en], pool.starts[pos:pos + n] - st, pool.sizes[pos:pos + n], pool.universe)
+        reqs.append((qc, rnd.choice([0.6, 0.7, 0.8, 0.9])))
+        pos += n
+    ctx = H.Context(0)
+    d = ctx.upload(idx)
+    res = {"serve_index_sets": idx.n, "serve_requests": len(reqs),
+           "serve_queries": int(sum(q.n for q, _ in reqs))}
+    with H.Server(d, slots=4, hbm_budget_bytes=2 << 30) as srv:
+        for qc, t in reqs[:20]:
+            srv.wait(srv.submit(qc, t))
+        # saturation: everything queued at once
+        a = time.perf_counter()
+        ids = [srv.submit(qc, t) for qc, t in reqs]
+        pairs = sum(srv.wait(i)[0] for i in ids)
+        sat_s = time.perf_counter() - a
+        sat_rps = len(reqs) / sat_s
+        res.update(serve_saturated_req_per_s=round(sat_rps, 1), serve_saturated_ms=round(sat_s * 1e3, 1),
+                   serve_pairs=int(pairs), serve_saturated_pairs_per_s=round(pairs / sat_s, 1))
+        # offered load: Poisson arrivals at half the saturated request rate
+        lam = 0.5 * sat_rps
+        prng = random.Random(3)
+        gaps = [prng.expovariate(lam) for _ in reqs]
+        lat, done_pairs = [], [0]
+        ids = []
+        lock = threading.Lock()
+
+        def waiter():
+            k = 0
+            while True:
+                with lock:
+                    if k >= len(ids):
+                        if len(ids) == len(reqs):
+                            return
+                        rid = None
+                    else:
+                        rid = ids[k]
+                if rid is None:
+                    time.sleep(0.0002)
+                    continue
+                c, l = srv.wait(rid)
+                lat.append(l)
+                done_pairs[0] += c
+                k += 1
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        a = time.perf_counter()
+        nxt = a
+        for (qc, t), g in zip(reqs, gaps):
+            nxt += g
+            dt = nxt - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            rid = srv.submit(qc, t)
+            with lock:
+                ids.append(rid)
+        th.join()
+        el = time.perf_counter() - a
+        st = srv.stats()
+    lat_arr = np.array(lat)
+    res.update(serve_offered_req_per_s=round(lam, 1), serve_achieved_req_per_s=round(len(reqs) / el, 1),
+               serve_p50_ms=round(float(np.percentile(lat_arr, 50)), 3),
+               serve_p99_ms=round(float(np.percentile(lat_arr, 99)), 3),
+               serve_pairs_per_s=round(done_pairs[0] / el, 1), serve_batches=st["batches"],
+               serve_max_in_flight=st["max_in_flight"], serve_deferred=st["deferred"])
+    log(f"serving: saturated {sat_rps:.0f} req/s; at {lam:.0f} req/s offered: p50 {res['serve_p50_ms']} ms, "
+        f"p99 {res['serve_p99_ms']} ms, {st['batches']} batches, max {st['max_in_flight']} in flight")
+    ctx.close()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +211,7 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline only: no side measurements, no CPU baseline")
     ap.add_argument("--threshold", type=float, default=0.8)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-join measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
@@ -231,6 +324,8 @@ def main() -> int:
 
     if not args.quick and not args.no_batch:
         extra.update(bench_batched(ctx, H))
+    if not args.quick and not args.no_serve:
+        extra.update(bench_serving(H))
 
     out = {
         "metric": "pairs/s (1M-set Jaccard>=0.8 self-join, fgss)",

@@ -72,6 +72,16 @@ class Stats(C.Structure):
                 ("candidate_tokens", C.c_uint64)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("min_threshold", C.c_double), ("slots", C.c_uint32), ("hbm_budget_bytes", C.c_uint64),
+                ("max_batch_queries", C.c_uint32), ("max_query_size", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("submitted", "batches", "deferred", "rejected", "max_in_flight",
+                                          "max_bytes_in_flight")]
+
+
 def _sig(name, restype, *argtypes):
     f = getattr(_lib, name)
     f.restype = restype
@@ -107,6 +117,11 @@ _sig("hs_join_batch", C.c_int, _vp, _vp, C.POINTER(Tune), C.POINTER(Result))
 _sig("hs_autotune_batch", C.c_int, _vp, _vp, C.c_int32, C.POINTER(Tune), C.POINTER(C.c_double))
 _sig("hs_get_stats", C.c_int, _vp, C.POINTER(Stats))
 _sig("hs_copy_bandwidth", C.c_int, _vp, C.c_uint64, C.c_int32, C.POINTER(C.c_double))
+_sig("hs_server_create", C.c_int, C.c_int, _vp, C.c_int32, C.c_int32, C.POINTER(ServerConfig), C.POINTER(_vp))
+_sig("hs_server_submit", C.c_int, _vp, _u32p, _u32p, C.c_uint32, C.c_uint64, C.c_double, C.POINTER(C.c_uint64))
+_sig("hs_server_wait", C.c_int, _vp, C.c_uint64, C.POINTER(Result), C.POINTER(C.c_double))
+_sig("hs_server_get_stats", C.c_int, _vp, C.POINTER(ServerStats))
+_sig("hs_server_destroy", None, _vp)
 _sig("hs_bounds", C.c_int, C.c_int32, C.c_double, C.c_int32, C.c_uint32, C.c_uint32, _u32p)
 
 
@@ -375,3 +390,72 @@ def sort_pairs(r, s, ov):
     key = (np.asarray(r, dtype=np.uint64) << np.uint64(32)) | np.asarray(s, dtype=np.uint64)
     order = np.argsort(key, kind="stable")
     return np.asarray(r)[order], np.asarray(s)[order], np.asarray(ov)[order]
+
+
+class Server:
+    """Continuous-batching query server over a device-resident index."""
+
+    def __init__(self, index: DeviceCollection, device=0, sim="jaccard", bounds=BOUNDS_REFERENCE,
+                 min_threshold=0.6, slots=4, hbm_budget_bytes=1 << 30, max_batch_queries=65536, max_query_size=0):
+        cfg = ServerConfig(min_threshold, slots, hbm_budget_bytes, max_batch_queries, max_query_size)
+        h = _vp()
+        kind = SIMS[sim] if isinstance(sim, str) else int(sim)
+        rc = _lib.hs_server_create(device, index._h, kind, int(bounds), C.byref(cfg), C.byref(h))
+        if rc != HS_OK:
+            raise HysetError(rc, "hs_server_create")
+        self._h = h
+        self._keep = index
+
+    def submit(self, queries: SetCollection, threshold: float) -> int:
+        t, sz = _u32(queries.tokens), _u32(queries.sizes)
+        rid = C.c_uint64()
+        rc = _lib.hs_server_submit(self._h, _ptr(t), _ptr(sz), queries.n, t.shape[0], float(threshold), C.byref(rid))
+        if rc != HS_OK:
+            raise HysetError(rc, "hs_server_submit")
+        return int(rid.value)
+
+    def wait(self, rid: int, count_only=False):
+        """(r, s, overlap, latency_ms), or (count, latency_ms) when count_only."""
+        res = Result()
+        res.count_only = 1
+        lat = C.c_double()
+        rc = _lib.hs_server_wait(self._h, rid, C.byref(res), C.byref(lat))
+        if rc != HS_OK:
+            raise HysetError(rc, "hs_server_wait")
+        return int(res.count), lat.value
+
+    def wait_pairs(self, rid: int):
+        """(r, s, overlap, latency_ms) of a request; r indexes its query sets."""
+        res = Result(0, 0, None, None, None, 0)
+        lat = C.c_double()
+        rc = _lib.hs_server_wait(self._h, rid, C.byref(res), C.byref(lat))
+        if rc == HS_E_OVERFLOW:
+            n = int(res.count)
+            r = np.empty(n, dtype=np.uint32)
+            s = np.empty_like(r)
+            ov = np.empty_like(r)
+            res = Result(0, n, _ptr(r), _ptr(s), _ptr(ov), 0)
+            rc = _lib.hs_server_wait(self._h, rid, C.byref(res), C.byref(lat))
+            if rc != HS_OK:
+                raise HysetError(rc, "hs_server_wait")
+            return r, s, ov, lat.value
+        if rc != HS_OK:
+            raise HysetError(rc, "hs_server_wait")
+        z = np.zeros(0, dtype=np.uint32)
+        return z, z.copy(), z.copy(), lat.value
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        _lib.hs_server_get_stats(self._h, C.byref(st))
+        return {k: int(getattr(st, k)) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self._h:
+            _lib.hs_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -20,7 +20,7 @@ LIB = HERE / "libhyset_amd.so"
 DRIVER = HERE / "hyset_join"
 
 HIP_SOURCES = ["index.hip", "fgss.hip", "bitmap.hip", "verify.hip"]
-CPP_SOURCES = ["api.cpp", "batch.cpp", "bounds.cpp", "datagen.cpp", "host_tools.cpp"]
+CPP_SOURCES = ["api.cpp", "batch.cpp", "serve.cpp", "bounds.cpp", "datagen.cpp", "host_tools.cpp"]
 ARCH = "gfx950"
 
 

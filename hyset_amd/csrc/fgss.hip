@@ -97,6 +97,8 @@ struct FgssLds {
   uint32_t tmp[FG_TPB / WAVE + 1];
   unsigned long long stats[4];
   uint32_t next;
+  uint8_t ptidx[FG_TPB];            // per-probe threshold table (serving)
+  DevBounds bounds[MAX_THRESHOLDS];  // the tables of a multi-threshold launch
 };
 
 // The probes of a tile and the part of the indexed collection they join.
@@ -122,9 +124,12 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     const uint32_t p = p0 + tid;
     const uint32_t l = A.probe.sizes[p];
     const uint32_t st = A.probe.starts[p];
+    const uint32_t ti = A.probe_tidx ? A.probe_tidx[p] : 0u;
+    const DevBounds& B = A.probe_tidx ? L.bounds[ti] : A.bounds;
+    L.ptidx[tid] = (uint8_t)ti;
     const uint32_t last = T.size_first_len - 1;
-    const uint32_t mins = min(A.bounds.minsize[l], last);
-    const uint32_t maxs1 = min(A.bounds.maxsize[l], last - 1) + 1;
+    const uint32_t mins = min(B.minsize[l], last);
+    const uint32_t maxs1 = min(B.maxsize[l], last - 1) + 1;
     uint32_t lo = max(T.size_first[mins], T.idx_first);
     uint32_t hi = min(T.size_first[maxs1], T.idx_last + 1);
     if (A.self) hi = min(hi, p);
@@ -134,7 +139,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     L.hi[tid] = hi;
     const uint32_t off = st - base;
     L.plds[tid] = (off + l <= FG_TOKCAP) ? off : NO_LDS;
-    n_items = (lo < hi) ? A.bounds.maxprefix[l] : 0;
+    n_items = (lo < hi) ? B.maxprefix[l] : 0;
   }
   for (uint32_t i = tid; i < ntok && i < FG_TOKCAP; i += FG_TPB) L.tok[i] = A.probe.tokens[base + i];
   uint32_t total_items;
@@ -228,7 +233,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
         rid = p0 + pl;
         const uint32_t lr = L.psize[pl];
         const uint32_t ls = (meta >> 16) == 0xFFFFu ? A.indexed.sizes[sid] : (meta >> 16);
-        const uint32_t mo = dev_minoverlap(A.bounds, lr, ls);
+        const uint32_t mo = dev_minoverlap(A.probe_tidx ? L.bounds[L.ptidx[pl]] : A.bounds, lr, ls);
         // Positional filter (helpers.hpp:46-50) at this visit's token: if it is
         // the first common token, |r n s| <= 1 + min(|r|-1-j, |s|-1-kp).  Only
         // the first-common visit can emit the pair, so a visit failing the
@@ -269,6 +274,10 @@ __device__ __forceinline__ uint32_t next_batch_tile(const FgssBatch& B, uint32_t
 __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint32_t ntiles) {
   __shared__ FgssLds L;
   const uint32_t tid = threadIdx.x;
+  if (A.probe_tidx) {
+    if (tid < MAX_THRESHOLDS) L.bounds[tid] = A.probe_bounds[tid];
+    __syncthreads();
+  }
   uint32_t my_cands = 0, my_cand_tokens = 0;
   if (A.batch.tile_join) {
     // persistent: one launch for every tile of every join of the batch

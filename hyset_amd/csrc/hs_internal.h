@@ -176,8 +176,13 @@ struct FgssArgs {
   const uint32_t* tile_starts;
   uint32_t first_tile, n_plan_tiles;
   FgssBatch batch;  // tile_join != NULL: persistent batched launch
+  // serving: per-probe thresholds.  probe_tidx[p] selects probe_bounds[...]
+  // (at most MAX_THRESHOLDS tables); NULL uses `bounds` for every probe
+  const DevBounds* probe_bounds;
+  const uint8_t* probe_tidx;
   DevOut out;
 };
+constexpr uint32_t MAX_THRESHOLDS = 8;
 constexpr uint32_t TILE_TOKENS = 4096;  // probe tokens staged in LDS per tile
 constexpr uint32_t TILE_PROBES = 256;
 void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile);

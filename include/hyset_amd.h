@@ -242,6 +242,42 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune, hs_result* r
  * *best_ms receives its mean device time. */
 int hs_autotune_batch(hs_ctx* ctx, hs_coll* batch, int32_t reps, hs_tune* best, double* best_ms);
 
+/* -------------------------------------------------------------- serving */
+
+/* A device-resident index over one collection answering requests (query
+ * sets + threshold) with continuous batching.  No reference counterpart
+ * (the reference runs offline joins only); results are those of hs_join with
+ * the query sets as probes and the index collection as the indexed side. */
+typedef struct hs_server hs_server;
+
+typedef struct {
+  double min_threshold;      /* lowest threshold accepted; the index prefix is built for it (default 0.6) */
+  uint32_t slots;            /* worker streams = device batches in flight (default 4) */
+  uint64_t hbm_budget_bytes; /* device bytes the batches in flight may hold (default 1 GiB) */
+  uint32_t max_batch_queries;/* query sets per device batch (default 65536) */
+  uint32_t max_query_size;   /* largest query set (default 4096; 1024 for cosine) */
+} hs_server_config;
+
+typedef struct {
+  uint64_t submitted, batches, deferred, rejected, max_in_flight, max_bytes_in_flight;
+} hs_server_stats;
+
+/* index: a collection uploaded on `device` (kept alive by the caller). */
+int hs_server_create(int device, hs_coll* index, int32_t sim, int32_t bounds_mode, const hs_server_config* cfg,
+                     hs_server** out);
+/* Queues n_queries query sets (dense CSR: sizes[i] tokens each, ascending)
+ * at `threshold`; returns at once with a request id. */
+int hs_server_submit(hs_server* server, const uint32_t* tokens, const uint32_t* sizes, uint32_t n_queries,
+                     uint64_t n_tokens, double threshold, uint64_t* request_id);
+/* Blocks until the request is answered; r = query index inside the request,
+ * s = index set id.  *latency_ms = submit-to-answer time.  HS_E_BUSY: the
+ * request could never fit the device budget.  HS_E_OVERFLOW (count set):
+ * the arrays are too small; the answer is kept for another call.  Otherwise
+ * the request is released. */
+int hs_server_wait(hs_server* server, uint64_t request_id, hs_result* result, double* latency_ms);
+int hs_server_get_stats(hs_server* server, hs_server_stats* out);
+void hs_server_destroy(hs_server* server);
+
 /* Copies the pairs of the last join into result (count_only ignored). */
 int hs_copy_last_result(hs_ctx* ctx, hs_result* result);
 
