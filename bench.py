@@ -40,7 +40,7 @@ def fgss_bytes(st: dict, n_probes: int, n_probe_tokens: int) -> int:
             + 8 * st["candidates"] + 4 * st["candidate_tokens"] + 12 * st["pairs"])
 
 
-def bench_batched(ctx, H) -> dict:
+def bench_batched(ctx, H, split=True) -> dict:
     """512 independent Jaccard-0.75 self-joins (the batch workload): one
     persistent launch over all of them vs one hs_join per join."""
     import numpy as np
@@ -104,11 +104,62 @@ def bench_batched(ctx, H) -> dict:
     res["batch_parity_vs_plain"] = ok
     log(f"batch: plain {plain:.1f} ms, persistent {ms:.1f} ms, +graph {ms_g:.1f} ms, autotuned {ms_t:.1f} ms "
         f"({res['batch_autotuned_config']}), speedup {res['batch_speedup_vs_plain']}x, parity={ok}")
+    b.free()
+    if split:
+        res.update(bench_split(ctx, H, singles, np.array(plain_counts, dtype=np.uint64)))
     for d in singles:
         d.free()
-    b.free()
     ctx.set_threshold(0.8)
     return res
+
+
+def bench_split(ctx, H, units, want_counts) -> dict:
+    """Learned vs analytic CPU/GPU split on the batch mix (512 Jaccard-0.75
+    self-joins).  Both models are trained on timings of a disjoint training
+    mix (192 joins drawn like the batch mix, seed 1001), then plan the batch
+    mix; every unit's four techniques are timed to score the plans, and each
+    plan is executed by hs_split_run (device units on the context stream,
+    CPU units on host threads) with its pair counts checked."""
+    import numpy as np
+
+    from hyset_amd import split as S
+    from hyset_amd import workloads as W
+    threads = S.default_threads()
+    t0 = time.perf_counter()
+    train = [ctx.upload(c) for c in W.batch(k=192, seed=1001)]
+    Ftr, Ttr, _ = S.measure_units(ctx, train, 0.75)
+    for d in train:
+        d.free()
+    t_train_mix = time.perf_counter() - t0
+    a = time.perf_counter()
+    sp = S.Split(ctx).fit(Ftr, Ttr)
+    t_fit = time.perf_counter() - a
+    F, T, counts = S.measure_units(ctx, units, 0.75)
+    assert np.array_equal(counts.astype(np.uint64), want_counts)
+    log(f"split: timed {len(train)} training + {len(units)} batch joins x 4 techniques in "
+        f"{time.perf_counter() - t0:.1f}s; fit {t_fit:.1f}s, held-out log-MSE {sp.val}")
+    pa, pl = sp.analytic.predict(F), sp.learned.predict(F)
+    plans = {"all_fgss": np.full(len(units), H.TECH_FGSS, dtype=np.int32),
+             "analytic_argmin": S.argmin_plan(pa), "learned_argmin": S.argmin_plan(pl),
+             "analytic_concurrent": S.concurrent_plan(pa, threads),
+             "learned_concurrent": S.concurrent_plan(pl, threads),
+             "measured_best_concurrent": S.concurrent_plan(T, threads)}
+    summary = S.summarize(plans, T, threads)
+    for name, plan in plans.items():
+        got, _, wall = ctx.split_run(units, plan, cpu_threads=threads)
+        assert np.array_equal(got, want_counts), name
+        summary[name]["run_wall_ms"] = round(wall, 2)
+        log(f"split {name}: run {wall:.1f} ms, modelled sum {summary[name]['sum_ms']} ms "
+            f"(regret {summary[name]['regret_ms']}), makespan {summary[name]['makespan_ms']} ms, "
+            f"share {summary[name]['share']}")
+    judge = sp.judge(F, T)
+    log(f"split judge: {judge}")
+    return {"split": {"cpu_threads": threads, "training_units": len(train), "units": len(units),
+                      "training_mix_s": round(t_train_mix, 1), "fit_s": round(t_fit, 2),
+                      "heldout_log_mse": {k: round(v, 4) for k, v in sp.val.items()},
+                      "eval_log_mse": {"analytic": round(S.log_error(pa, T), 4),
+                                       "learned": round(S.log_error(pl, T), 4)},
+                      "uses_learned": sp.use_learned, "plans": summary, "judge": judge}}
 
 
 def bench_serving(H) -> dict:
@@ -212,6 +263,7 @@ def main() -> int:
     ap.add_argument("--threshold", type=float, default=0.8)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-join measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-split", action="store_true", help="skip the learned/analytic split measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
@@ -323,7 +375,7 @@ def main() -> int:
         log(f"hybrid ({min(16, ncores)} host threads): {extra['hybrid_ms_wall']} ms")
 
     if not args.quick and not args.no_batch:
-        extra.update(bench_batched(ctx, H))
+        extra.update(bench_batched(ctx, H, split=not args.no_split))
     if not args.quick and not args.no_serve:
         extra.update(bench_serving(H))
 

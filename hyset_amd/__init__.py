@@ -110,6 +110,7 @@ _sig("hs_join", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.c_
 _sig("hs_verify_candidates", C.c_int, _vp, _vp, _vp, _u32p, _u32p, _u32p, C.c_uint32, C.c_int32,
      C.POINTER(Result))
 _sig("hs_copy_last_result", C.c_int, _vp, C.POINTER(Result))
+_sig("hs_join_host", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.c_uint32, C.POINTER(Result))
 _u32pp = C.POINTER(_u32p)
 _sig("hs_upload_batch", C.c_int, _vp, C.c_uint32, _u32pp, _u32pp, _u32pp, _u32p, C.POINTER(C.c_uint64),
      C.POINTER(_vp))
@@ -123,6 +124,28 @@ _sig("hs_server_wait", C.c_int, _vp, C.c_uint64, C.POINTER(Result), C.POINTER(C.
 _sig("hs_server_get_stats", C.c_int, _vp, C.POINTER(ServerStats))
 _sig("hs_server_destroy", None, _vp)
 _sig("hs_bounds", C.c_int, C.c_int32, C.c_double, C.c_int32, C.c_uint32, C.c_uint32, _u32p)
+_f32p = C.POINTER(C.c_float)
+_f64p = C.POINTER(C.c_double)
+_sig("hs_features", C.c_int, _vp, _vp, C.POINTER(Range), _f32p)
+_sig("hs_linear_fit", C.c_int, _f64p, _f64p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, _f64p)
+_sig("hs_linear_predict", C.c_int, _f64p, _f64p, C.c_uint32, C.c_uint32, C.c_uint32, _f64p)
+_sig("hs_mlp_create", C.c_int, _vp, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.POINTER(_vp))
+_sig("hs_mlp_destroy", None, _vp)
+_sig("hs_mlp_num_params", C.c_uint32, _vp)
+_sig("hs_mlp_set_normalization", C.c_int, _vp, _f32p, _f32p)
+_sig("hs_mlp_get_params", C.c_int, _vp, _f32p)
+_sig("hs_mlp_set_params", C.c_int, _vp, _f32p)
+_sig("hs_mlp_predict", C.c_int, _vp, _f32p, C.c_uint32, _f32p)
+_sig("hs_mlp_loss_grad", C.c_int, _vp, _f32p, _f32p, C.c_uint32, _f32p, _f32p)
+_sig("hs_mlp_train", C.c_int, _vp, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_float, _f32p)
+_sig("hs_split_run", C.c_int, _vp, C.c_uint32, C.POINTER(_vp), C.POINTER(C.c_int32), C.c_uint32,
+     C.POINTER(C.c_uint64), _f64p, _f64p)
+
+NFEATURES = 7
+FEATURES = ("size", "averageSetSize", "setSizeStd", "tokenFreqDistribution", "universeSize", "indexSize",
+            "threshold")
+TECH_FGSS, TECH_BITMAP, TECH_HYBRID, TECH_CPU = 0, 1, 2, 3
+TECHNIQUES = ("fgss", "bitmap", "hybrid", "cpu")
 
 
 def bounds(sim, t, l1, l2=None, mode=BOUNDS_REFERENCE):
@@ -289,6 +312,27 @@ class Context:
             return int(res.count)
         return self.last_result(int(res.count))
 
+    def join_host(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, threads=1,
+                  indexed_range=None, probe_range=None):
+        """The CPU partition (hs_join_host): (r, s, overlap) computed on host cores."""
+        probe = indexed if probe is None else probe
+        ir = Range(*indexed_range) if indexed_range is not None else None
+        pr = Range(*probe_range) if probe_range is not None else None
+        cap = 1 << 16
+        while True:
+            r = np.empty(cap, dtype=np.uint32)
+            s = np.empty_like(r)
+            ov = np.empty_like(r)
+            res = Result(0, cap, _ptr(r), _ptr(s), _ptr(ov), 0)
+            rc = _lib.hs_join_host(self._h, indexed._h, probe._h, C.byref(ir) if ir else None,
+                                   C.byref(pr) if pr else None, threads, C.byref(res))
+            if rc == HS_E_OVERFLOW:
+                cap = int(res.count)
+                continue
+            self._check(rc, "hs_join_host")
+            n = int(res.count)
+            return r[:n], s[:n], ov[:n]
+
     def last_result(self, count: int):
         r = np.empty(max(count, 1), dtype=np.uint32)
         s = np.empty_like(r)
@@ -338,6 +382,32 @@ class Context:
                                               pr.shape[0], scenario, C.byref(res)), "hs_verify_candidates")
         return self.last_result(int(res.count))
 
+    def features(self, coll: "DeviceCollection", rng=None) -> np.ndarray:
+        """Partition features of sets [rng] (hs_features) under the current threshold."""
+        out = np.zeros(NFEATURES, dtype=np.float32)
+        r = Range(*rng) if rng is not None else None
+        self._check(_lib.hs_features(self._h, coll._h, C.byref(r) if r else None,
+                                     out.ctypes.data_as(_f32p)), "hs_features")
+        return out
+
+    def split_run(self, colls, techniques, cpu_threads=1):
+        """Runs every collection's self-join with its technique (hs_split_run);
+        returns (counts, per-unit ms, wall ms)."""
+        n = len(colls)
+        hs = (_vp * n)(*[c._h for c in colls])
+        tech = np.ascontiguousarray(techniques, dtype=np.int32)
+        assert tech.shape == (n,)
+        counts = np.zeros(n, dtype=np.uint64)
+        unit_ms = np.zeros(n, dtype=np.float64)
+        wall = C.c_double()
+        self._check(_lib.hs_split_run(self._h, n, hs, tech.ctypes.data_as(C.POINTER(C.c_int32)), cpu_threads,
+                                      counts.ctypes.data_as(C.POINTER(C.c_uint64)), unit_ms.ctypes.data_as(_f64p),
+                                      C.byref(wall)), "hs_split_run")
+        return counts, unit_ms, wall.value
+
+    def mlp(self, n_in, h1, h2, n_out, seed=1) -> "Mlp":
+        return Mlp(self, n_in, h1, h2, n_out, seed)
+
     def stats(self) -> dict:
         st = Stats()
         _lib.hs_get_stats(self._h, C.byref(st))
@@ -383,6 +453,111 @@ class DeviceCollection:
         if self._h:
             _lib.hs_free_collection(self._h)
             self._h = None
+
+
+def _f32(a, cols) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if a.ndim == 1:
+        a = a.reshape(-1, cols)
+    assert a.shape[1] == cols, (a.shape, cols)
+    return a
+
+
+class Mlp:
+    """The learned cost model's network (hs_mlp): trained and evaluated by
+    the library's device kernels."""
+
+    def __init__(self, ctx: Context, n_in, h1, h2, n_out, seed=1):
+        h = _vp()
+        ctx._check(_lib.hs_mlp_create(ctx._h, n_in, h1, h2, n_out, seed, C.byref(h)), "hs_mlp_create")
+        self._ctx, self._h = ctx, h
+        self.shape = (n_in, h1, h2, n_out)
+
+    def _check(self, rc, what):
+        self._ctx._check(rc, what)
+
+    @property
+    def num_params(self) -> int:
+        return int(_lib.hs_mlp_num_params(self._h))
+
+    def set_normalization(self, mean, stdv):
+        m = np.ascontiguousarray(mean, dtype=np.float32)
+        s = np.ascontiguousarray(stdv, dtype=np.float32)
+        assert m.shape == s.shape == (self.shape[0],)
+        self._check(_lib.hs_mlp_set_normalization(self._h, m.ctypes.data_as(_f32p), s.ctypes.data_as(_f32p)),
+                    "hs_mlp_set_normalization")
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.num_params, dtype=np.float32)
+        self._check(_lib.hs_mlp_get_params(self._h, out.ctypes.data_as(_f32p)), "hs_mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (self.num_params,)
+        self._check(_lib.hs_mlp_set_params(self._h, p.ctypes.data_as(_f32p)), "hs_mlp_set_params")
+
+    def predict(self, X) -> np.ndarray:
+        X = _f32(X, self.shape[0])
+        Y = np.empty((X.shape[0], self.shape[3]), dtype=np.float32)
+        self._check(_lib.hs_mlp_predict(self._h, X.ctypes.data_as(_f32p), X.shape[0], Y.ctypes.data_as(_f32p)),
+                    "hs_mlp_predict")
+        return Y
+
+    def loss_grad(self, X, Y):
+        X, Y = _f32(X, self.shape[0]), _f32(Y, self.shape[3])
+        loss = C.c_float()
+        g = np.empty(self.num_params, dtype=np.float32)
+        self._check(_lib.hs_mlp_loss_grad(self._h, X.ctypes.data_as(_f32p), Y.ctypes.data_as(_f32p), X.shape[0],
+                                          C.byref(loss), g.ctypes.data_as(_f32p)), "hs_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, X, Y, epochs=2000, lr=1e-2) -> float:
+        X, Y = _f32(X, self.shape[0]), _f32(Y, self.shape[3])
+        loss = C.c_float()
+        self._check(_lib.hs_mlp_train(self._h, X.ctypes.data_as(_f32p), Y.ctypes.data_as(_f32p), X.shape[0],
+                                      epochs, lr, C.byref(loss)), "hs_mlp_train")
+        return loss.value
+
+    def close(self):
+        if self._h:
+            _lib.hs_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def linear_fit(X, Y, ridge=0.0) -> np.ndarray:
+    """Ridge least squares (hs_linear_fit): coef of shape (d + 1, n_out), last row = intercepts."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    if Y.ndim == 1:
+        Y = Y[:, None]
+    n, d = X.shape
+    coef = np.empty((d + 1, Y.shape[1]), dtype=np.float64)
+    rc = _lib.hs_linear_fit(X.ctypes.data_as(_f64p), Y.ctypes.data_as(_f64p), n, d, Y.shape[1], ridge,
+                            coef.ctypes.data_as(_f64p))
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_linear_fit: singular or invalid system")
+    return coef
+
+
+def linear_predict(coef, X) -> np.ndarray:
+    """|X coef + intercept| per output (hs_linear_predict)."""
+    coef = np.ascontiguousarray(coef, dtype=np.float64)
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, d = X.shape
+    assert coef.shape[0] == d + 1
+    Y = np.empty((n, coef.shape[1]), dtype=np.float64)
+    rc = _lib.hs_linear_predict(coef.ctypes.data_as(_f64p), X.ctypes.data_as(_f64p), n, d, coef.shape[1],
+                                Y.ctypes.data_as(_f64p))
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_linear_predict")
+    return Y
 
 
 def sort_pairs(r, s, ov):

@@ -243,6 +243,7 @@ void run_hybrid(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const 
 
 void check_pair(hs_ctx* ctx, hs_coll* a, hs_coll* b) {
   HS_REQUIRE(ctx && a && b, HS_E_INVALID, "NULL context or collection");
+  HS_REQUIRE(ctx->device >= 0, HS_E_INVALID, "a host-only context runs hs_join_host only");
   HS_REQUIRE(a->n_joins == 0 && b->n_joins == 0, HS_E_INVALID, "batched collections join with hs_join_batch");
   HS_REQUIRE(a->device == ctx->device && b->device == ctx->device, HS_E_INVALID,
              "collection lives on another device");
@@ -264,6 +265,10 @@ int hs_create(int device, hs_ctx** out) {
     return HS_E_OOM;
   }
   ctx->device = device;
+  if (device < 0) {  // host-only context: CPU joins, bounds and cost models
+    *out = ctx;
+    return HS_OK;
+  }
   int rc = guarded(ctx, [&] {
     int count = 0;
     HS_CHECK_HIP(hipGetDeviceCount(&count));
@@ -287,6 +292,10 @@ int hs_create(int device, hs_ctx** out) {
 
 void hs_destroy(hs_ctx* ctx) {
   if (!ctx) return;
+  if (ctx->device < 0) {
+    delete ctx;
+    return;
+  }
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
@@ -360,6 +369,7 @@ int hs_upload_collection(hs_ctx* ctx, const uint32_t* tokens, const uint32_t* st
       }
     }
     c->h_tiles.push_back(n);
+    if (ctx->device < 0) return;  // host-only collection
     c->tiles.ensure(c->h_tiles.size() * 4);
     c->tokens.ensure(std::max<uint64_t>(pos, 1) * 4);
     c->starts.ensure(std::max<uint32_t>(n, 1) * 4);
@@ -380,8 +390,10 @@ int hs_upload_collection(hs_ctx* ctx, const uint32_t* tokens, const uint32_t* st
 
 void hs_free_collection(hs_coll* coll) {
   if (!coll) return;
-  (void)hipSetDevice(coll->device);
-  (void)hipDeviceSynchronize();
+  if (coll->device >= 0) {
+    (void)hipSetDevice(coll->device);
+    (void)hipDeviceSynchronize();
+  }
   delete coll;
 }
 

@@ -101,6 +101,7 @@ struct hs_ctx {
   std::vector<uint32_t> h_offsets, h_set_ids;  // host copy of the index (hybrid)
   int num_cus = 256;
   DevBuf batch_queue;  // per-XCD tile counters of the batched launch
+  DevBuf feat;         // partition-feature accumulators (hs_features)
   // captured batched join: replayed while (collection, bounds, plan) match
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -116,7 +117,7 @@ namespace hs {
 template <typename F>
 inline int guarded(hs_ctx* ctx, F&& body) {
   try {
-    if (ctx) {
+    if (ctx && ctx->device >= 0) {
       hipError_t e = hipSetDevice(ctx->device);
       if (e != hipSuccess) throw Error{HS_E_DEVICE, std::string("hipSetDevice: ") + hipGetErrorString(e)};
     }
@@ -214,6 +215,14 @@ inline hs_range clamp_range(const hs_range* r, uint32_t n) {
 }
 
 inline bool range_empty(const hs_range& r) { return r.first > r.last; }
+
+struct Triples {
+  std::vector<uint32_t> r, s, ov;
+};
+// CPU join (host_join.cpp): pairs of probe range pr against indexed range ir;
+// reads only host copies and the given tables, so it may run on any thread.
+uint64_t host_join(const hs_coll& indexed, const hs_coll& probe, bool self, hs_range ir, hs_range pr,
+                   const BoundTables& T, uint32_t threads, std::vector<Triples>* parts_out);
 
 
 }  // namespace hs

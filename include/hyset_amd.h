@@ -191,6 +191,8 @@ typedef struct {
 
 /* ------------------------------------------------------------- contexts */
 
+/* device < 0 creates a host-only context: collections stay in host memory
+ * and only hs_join_host, the bounds and the cost models are available. */
 int hs_create(int device, hs_ctx** out);
 void hs_destroy(hs_ctx* ctx);
 const char* hs_last_error(const hs_ctx* ctx);
@@ -277,6 +279,67 @@ int hs_server_submit(hs_server* server, const uint32_t* tokens, const uint32_t* 
 int hs_server_wait(hs_server* server, uint64_t request_id, hs_result* result, double* latency_ms);
 int hs_server_get_stats(hs_server* server, hs_server_stats* out);
 void hs_server_destroy(hs_server* server);
+
+/* The CPU partition: the same join on host cores (host::allpairs of
+ * include/hyset/host_algorithms.hpp:16-129 with the thread layout of
+ * src/cpu_multi.cu:118-207), with verification and exact overlaps.  Works on
+ * device and host-only contexts. */
+int hs_join_host(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* indexed_range,
+                 const hs_range* probe_range, uint32_t threads, hs_result* result);
+
+/* ------------------------------------------------- cost models, split */
+
+/* Partition features of include/hyset/statistics.hpp:163-283, computed on
+ * the device for sets [range] of a device collection under the context's
+ * threshold: 0 number of sets, 1 mean set size, 2 set-size standard
+ * deviation, 3 standard deviation of the prefix-token list lengths over the
+ * universe, 4 universe, 5 prefix-index entries, 6 threshold. */
+#define HS_NFEATURES 7
+int hs_features(hs_ctx* ctx, hs_coll* coll, const hs_range* range, float out[HS_NFEATURES]);
+
+/* The analytic cost model: per technique a linear function of the features
+ * whose absolute value is the predicted cost (statistics.hpp:132-161 with the
+ * coefficient table of src/cost_model.cu:296-323), fitted here by ridge least
+ * squares.  X is n x d row-major, Y is n x n_out; coef receives (d + 1) x
+ * n_out (row d = intercepts). */
+int hs_linear_fit(const double* X, const double* Y, uint32_t n, uint32_t d, uint32_t n_out, double ridge,
+                  double* coef);
+int hs_linear_predict(const double* coef, const double* X, uint32_t n, uint32_t d, uint32_t n_out, double* Y);
+
+/* The learned cost model: an MLP (n_in -> h1 -> h2 -> n_out, ReLU) trained
+ * and evaluated by device kernels.  No reference counterpart: it replaces
+ * the linear costs above where they mispredict.  Inputs are standardised by
+ * the mean/stdv set with hs_mlp_set_normalization; parameters are laid out
+ * W1 (h1 x n_in), b1, W2 (h2 x h1), b2, W3 (n_out x h2), b3. */
+typedef struct hs_mlp hs_mlp;
+int hs_mlp_create(hs_ctx* ctx, uint32_t n_in, uint32_t h1, uint32_t h2, uint32_t n_out, uint64_t seed, hs_mlp** out);
+void hs_mlp_destroy(hs_mlp* mlp);
+uint32_t hs_mlp_num_params(const hs_mlp* mlp);
+int hs_mlp_set_normalization(hs_mlp* mlp, const float* mean, const float* stdv);
+int hs_mlp_get_params(const hs_mlp* mlp, float* out);
+int hs_mlp_set_params(hs_mlp* mlp, const float* in);
+/* X: n x n_in host rows; Y: n x n_out host outputs. */
+int hs_mlp_predict(hs_mlp* mlp, const float* X, uint32_t n, float* Y);
+/* Mean squared error over n x n_out and its gradient (num_params floats). */
+int hs_mlp_loss_grad(hs_mlp* mlp, const float* X, const float* Y, uint32_t n, float* loss, float* grad);
+/* Full-batch Adam for `epochs` steps; *final_loss (may be NULL) = loss after them. */
+int hs_mlp_train(hs_mlp* mlp, const float* X, const float* Y, uint32_t n, uint32_t epochs, float lr,
+                 float* final_loss);
+
+/* Techniques a split may give a unit (include/hyset/cost.hpp:28-58 chooses
+ * among the device techniques and the CPU). */
+#define HS_TECH_FGSS 0
+#define HS_TECH_BITMAP 1
+#define HS_TECH_HYBRID 2
+#define HS_TECH_CPU 3
+
+/* Runs a split: every unit (a collection self-join under the context's
+ * threshold) with its technique; HS_TECH_CPU units run on cpu_threads host
+ * threads while the device units run one after another on the context's
+ * stream.  counts[u] = pairs of unit u; unit_ms[u] (may be NULL) its time;
+ * *wall_ms (may be NULL) the time of the whole split. */
+int hs_split_run(hs_ctx* ctx, uint32_t n_units, hs_coll* const* colls, const int32_t* technique,
+                 uint32_t cpu_threads, uint64_t* counts, double* unit_ms, double* wall_ms);
 
 /* Copies the pairs of the last join into result (count_only ignored). */
 int hs_copy_last_result(hs_ctx* ctx, hs_result* result);

@@ -1,0 +1,110 @@
+"""CPU tests of the cost-model layer: the oracle pinned against hand-computed
+features and finite differences, the host linear model against numpy, and
+the planners against brute force."""
+import itertools
+
+import numpy as np
+import pytest
+
+import hyset_amd as H
+from hyset_amd import split as S
+from oracle import bounds as or_bounds
+from oracle import cost as OC
+
+
+def _coll(sets):
+    return H.SetCollection.from_sets(sets)
+
+
+def test_oracle_features_hand_computed():
+    # sizes 2, 3, 3, 5; Jaccard 0.8 -> midprefix = l - ceil(2*0.8*l/1.8) + 1
+    sets = [[0, 1], [0, 2, 3], [1, 2, 4], [0, 1, 2, 3, 5]]
+    c = _coll(sets)
+    mids = [or_bounds("jaccard", 0.8, l)[4] for l in (2, 3, 3, 5)]
+    assert mids == [1, 1, 1, 1]
+    f = OC.features(c, 0.8)
+    # size stats over sizes[0:3] (the last set is excluded as in statistics.hpp:179-180)
+    sz = np.array([2, 3, 3], dtype=float)
+    assert f[0] == 4
+    assert f[1] == pytest.approx(sz.mean())
+    assert f[2] == pytest.approx(np.sqrt(((sz - sz.mean()) ** 2).mean()))
+    # prefix tokens: 0, 0, 1, 0 over universe 6 -> counts [3, 1, 0, 0, 0, 0]
+    cnt = np.array([3, 1, 0, 0, 0, 0], dtype=float)
+    assert f[3] == pytest.approx(np.sqrt(((cnt - cnt.mean()) ** 2).mean()))
+    assert f[4] == 6 and f[5] == 4 and f[6] == pytest.approx(0.8)
+    # a sub-range and the one-set range
+    g = OC.features(c, 0.8, first=1, last=2)
+    assert g[0] == 2 and g[1] == 3 and g[2] == 0 and g[5] == 2
+    one = OC.features(c, 0.8, first=3, last=3)
+    assert one[0] == 1 and one[1] == 5 and one[2] == 0
+
+
+def test_oracle_mlp_gradient_matches_finite_differences():
+    rng = np.random.default_rng(3)
+    shape = (5, 7, 6, 3)
+    n_par = 7 * 5 + 7 + 6 * 7 + 6 + 3 * 6 + 3
+    p = rng.normal(0, 0.5, n_par)
+    X = rng.normal(0, 1, (11, 5))
+    Y = rng.normal(0, 1, (11, 3))
+    mean, stdv = rng.normal(0, 0.3, 5), rng.uniform(0.5, 2, 5)
+    loss, g = OC.mlp_loss_grad(p, shape, X, Y, mean, stdv)
+    assert loss == pytest.approx(OC.mlp_loss(p, shape, X, Y, mean, stdv))
+    eps = 1e-6
+    fd = np.empty_like(p)
+    for i in range(n_par):
+        e = np.zeros_like(p)
+        e[i] = eps
+        fd[i] = (OC.mlp_loss(p + e, shape, X, Y, mean, stdv) - OC.mlp_loss(p - e, shape, X, Y, mean, stdv)) / (2 * eps)
+    np.testing.assert_allclose(g, fd, rtol=1e-5, atol=1e-8)
+
+
+def test_oracle_mlp_layout_is_the_textbook_network():
+    # with identity-like weights the network reduces to relu(relu(x)) summed
+    shape = (2, 2, 2, 1)
+    p = np.array([1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 0, 0, 1, 1, 0.5])
+    P, _ = OC.mlp_forward(p, shape, np.array([[1.0, -2.0], [3.0, 4.0]]))
+    np.testing.assert_allclose(P[:, 0], [1.5, 7.5])
+
+
+def test_linear_fit_matches_numpy_ridge():
+    rng = np.random.default_rng(7)
+    X = rng.normal(0, 1, (200, 6))
+    Y = X @ rng.normal(0, 1, (6, 3)) + rng.normal(0, 1, 3) + 0.01 * rng.normal(0, 1, (200, 3))
+    for ridge in (0.0, 0.5):
+        coef = H.linear_fit(X, Y, ridge)
+        A = np.hstack([X, np.ones((200, 1))])
+        R = ridge * np.eye(7)
+        R[6, 6] = 0.0  # the intercept is not penalised
+        want = np.linalg.solve(A.T @ A + R, A.T @ Y)
+        np.testing.assert_allclose(coef, want, rtol=1e-8, atol=1e-10)
+        np.testing.assert_allclose(H.linear_predict(coef, X), np.abs(A @ want), rtol=1e-9, atol=1e-12)
+
+
+def test_linear_fit_rejects_singular_system():
+    X = np.ones((10, 2))
+    with pytest.raises(H.HysetError):
+        H.linear_fit(X, np.arange(10.0), 0.0)
+
+
+def test_argmin_plan_is_cost_run():
+    pred = np.array([[3.0, 1.0, 2.0, 9.0], [1.0, 1.5, 2.0, 0.5]])
+    np.testing.assert_array_equal(S.argmin_plan(pred), [1, 3])
+
+
+def test_concurrent_plan_never_worse_than_all_device_by_prediction():
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        pred = rng.lognormal(0, 1.5, (12, 4))
+        for threads in (1, 3):
+            plan = S.concurrent_plan(pred, threads)
+            gbest = np.argmin(pred[:, :3], axis=1)
+            assert S.plan_cost(plan, pred, threads) <= S.plan_cost(gbest, pred, threads) + 1e-9
+
+
+def test_plan_cost_makespan_brute_force():
+    T = np.array([[2.0, 9, 9, 3], [4.0, 9, 9, 1], [1.0, 9, 9, 5], [3.0, 9, 9, 2]])
+    best = min(S.plan_cost(np.array(p), T, 1)
+               for p in itertools.product([0, 3], repeat=4))
+    plan = S.concurrent_plan(T, 1)
+    assert S.plan_cost(plan, T, 1) == pytest.approx(best)
+    assert S.plan_cost(np.array([0, 0, 0, 0]), T) == 10.0

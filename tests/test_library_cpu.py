@@ -95,3 +95,33 @@ def test_text_round_trip(tmp_path):
     assert pairs.read_text() == "3 1\n4 2\n"
     H.write_pairs_text(pairs, [3], [1], [7])
     assert pairs.read_text() == "3 1 7\n"
+
+
+def test_host_join_matches_oracle():
+    """hs_join_host on a host-only context: the library's own CPU partition."""
+    import oracle as O
+    c = H.generate_self(H.gen_params(3000, size_dist=H.SIZE_UNIFORM, lo=2, hi=40, universe=800, zipf=1.0, seed=5))
+    with H.Context(-1) as ctx:
+        d = ctx.upload(c)
+        for sim in ("jaccard", "cosine", "dice"):
+            for t in (0.6, 0.8, 0.9):
+                ctx.set_threshold(t, sim)
+                want = O.self_join(c, t, sim)
+                for threads in (1, 4):
+                    r, s, ov = H.sort_pairs(*ctx.join_host(d, threads=threads))
+                    assert np.array_equal(r, want.r) and np.array_equal(s, want.s)
+                    assert np.array_equal(ov, want.overlap)
+        r2, s2 = H.generate_rs(H.gen_params(700, size_dist=H.SIZE_UNIFORM, lo=5, hi=40, universe=500, zipf=0.8, seed=3),
+                               H.gen_params(800, size_dist=H.SIZE_UNIFORM, lo=5, hi=40, universe=500, zipf=0.8, seed=4),
+                               dup_frac=0.3)
+        dr, ds = ctx.upload(r2), ctx.upload(s2)
+        ctx.set_threshold(0.7, "cosine")
+        want = O.rs_join(r2, s2, 0.7, "cosine")
+        r, s, ov = H.sort_pairs(*ctx.join_host(ds, dr, threads=3))
+        assert np.array_equal(r, want.r) and np.array_equal(s, want.s) and np.array_equal(ov, want.overlap)
+        # device entry points refuse a host-only context
+        try:
+            ctx.join(d)
+            raise AssertionError("hs_join ran on a host-only context")
+        except H.HysetError as e:
+            assert e.code == H.HS_E_INVALID
