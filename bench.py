@@ -128,6 +128,7 @@ def bench_split(ctx, H, units, want_counts) -> dict:
     t0 = time.perf_counter()
     train = [ctx.upload(c) for c in W.batch(k=192, seed=1001)]
     Ftr, Ttr, _ = S.measure_units(ctx, train, 0.75)
+    cpu_scale = S.calibrate_cpu_scale(ctx, train, Ttr, threads)
     for d in train:
         d.free()
     t_train_mix = time.perf_counter() - t0
@@ -137,14 +138,15 @@ def bench_split(ctx, H, units, want_counts) -> dict:
     F, T, counts = S.measure_units(ctx, units, 0.75)
     assert np.array_equal(counts.astype(np.uint64), want_counts)
     log(f"split: timed {len(train)} training + {len(units)} batch joins x 4 techniques in "
-        f"{time.perf_counter() - t0:.1f}s; fit {t_fit:.1f}s, held-out log-MSE {sp.val}")
+        f"{time.perf_counter() - t0:.1f}s; CPU slowdown under load {cpu_scale:.2f}; fit {t_fit:.1f}s, "
+        f"held-out log-MSE {sp.val}")
     pa, pl = sp.analytic.predict(F), sp.learned.predict(F)
     plans = {"all_fgss": np.full(len(units), H.TECH_FGSS, dtype=np.int32),
              "analytic_argmin": S.argmin_plan(pa), "learned_argmin": S.argmin_plan(pl),
-             "analytic_concurrent": S.concurrent_plan(pa, threads),
-             "learned_concurrent": S.concurrent_plan(pl, threads),
-             "measured_best_concurrent": S.concurrent_plan(T, threads)}
-    summary = S.summarize(plans, T, threads)
+             "analytic_concurrent": S.concurrent_plan(pa, threads, cpu_scale),
+             "learned_concurrent": S.concurrent_plan(pl, threads, cpu_scale),
+             "measured_best_concurrent": S.concurrent_plan(T, threads, cpu_scale)}
+    summary = S.summarize(plans, T, threads, cpu_scale)
     for name, plan in plans.items():
         got, _, wall = ctx.split_run(units, plan, cpu_threads=threads)
         assert np.array_equal(got, want_counts), name
@@ -154,7 +156,8 @@ def bench_split(ctx, H, units, want_counts) -> dict:
             f"share {summary[name]['share']}")
     judge = sp.judge(F, T)
     log(f"split judge: {judge}")
-    return {"split": {"cpu_threads": threads, "training_units": len(train), "units": len(units),
+    return {"split": {"cpu_threads": threads, "cpu_scale": round(cpu_scale, 3), "training_units": len(train),
+                      "units": len(units),
                       "training_mix_s": round(t_train_mix, 1), "fit_s": round(t_fit, 2),
                       "heldout_log_mse": {k: round(v, 4) for k, v in sp.val.items()},
                       "eval_log_mse": {"analytic": round(S.log_error(pa, T), 4),
@@ -359,6 +362,15 @@ def main() -> int:
             extra[f"fgss_t{tt}_pairs"] = c
             log(f"fgss t={tt}: {c} pairs {statistics.median(ms):.3f} ms")
         ctx.set_threshold(t)
+        # the same join with one workgroup per tile in plan order (no tile queue)
+        st_tune = H.Tune(flags=H.TUNE_STATIC_TILES)
+        ctx.join(d, algo="fgss", tune=st_tune, count_only=True)
+        ks = []
+        for _ in range(5):
+            assert ctx.join(d, algo="fgss", tune=st_tune, count_only=True) == pairs
+            ks.append(ctx.stats()["kernel_ms"])
+        extra["k_fgss_static_tiles_ms"] = round(statistics.median(ks), 4)
+        log(f"k_fgss with static tiles: {extra['k_fgss_static_tiles_ms']} ms (tile queue: {kern_ms:.4f} ms)")
         for bits in (64, 128, 256):
             tune = H.Tune(bitmap_bits=bits)
             c = ctx.join(d, algo="bitmap", tune=tune, count_only=True)

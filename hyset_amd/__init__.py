@@ -57,6 +57,7 @@ class Tune(C.Structure):
 
 
 TUNE_GRAPH = 1
+TUNE_STATIC_TILES = 2
 
 
 class Result(C.Structure):

@@ -490,8 +490,36 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
           a.first_tile = ft;
           a.n_plan_tiles = lt - ft + 1;
         }
-        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
-        hs::launch_fgss(st, a, ppt);
+        uint32_t wgs = 0;
+        if (ppt == 0 && !(tune.flags & HS_TUNE_STATIC_TILES) &&
+            (uint64_t)(pr.last - pr.first + 1) * 1024 < 0xFFFFFFFFull && probe->maxlen < 0xFFFFu) {
+          // item slices first, then tiles of equal work claimed by a persistent grid
+          const auto& mp = ctx->tables.maxprefix;
+          const auto& psf = probe->h_size_first;
+          const uint32_t nl = (uint32_t)psf.size();
+          ctx->h_icum.ensure((size_t)nl * 4);
+          ctx->d_icum.ensure((size_t)nl * 4);
+          uint32_t* icum = ctx->h_icum.as<uint32_t>();
+          uint64_t items = 0;
+          for (uint32_t l = 0; l + 1 < nl; ++l) {
+            icum[l] = (uint32_t)items;
+            const uint32_t lo = std::max(pr.first, psf[l]);
+            const uint32_t hi = std::min(pr.last + 1, psf[l + 1]);
+            if (hi > lo) items += (uint64_t)(hi - lo) * mp[l];
+            HS_REQUIRE(items < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "probe range limited to 2^32-1 prefix items");
+          }
+          icum[nl - 1] = (uint32_t)items;
+          HS_CHECK_HIP(hipMemcpyAsync(ctx->d_icum.ptr, icum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
+          hs::BalancedPlan bp{ctx->d_icum.as<uint32_t>(), probe->size_first.as<uint32_t>(), nl, items,
+                              &ctx->bal_slice_a, &ctx->bal_slice_n, &ctx->bal_cost, &ctx->bal_tiles,
+                              &ctx->bal_ntiles, &ctx->ib.scan_scratch, &ctx->bal_rec, &ctx->bal_slot};
+          hs::plan_balanced_tiles(st, a, bp);
+          ctx->batch_queue.ensure(8 * sizeof(uint32_t));
+          a.tile_queue = ctx->batch_queue.as<uint32_t>();
+          wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : 4);
+        }
+        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));  // kernel_ms: the fgss launch alone
+        hs::launch_fgss(st, a, ppt, wgs);
       } else if (algo == HS_ALGO_BITMAP) {
         const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
         HS_REQUIRE(bits == 64 || bits == 128 || bits == 256, HS_E_INVALID, "bitmap width must be 64, 128 or 256");

@@ -102,6 +102,10 @@ struct hs_ctx {
   int num_cus = 256;
   DevBuf batch_queue;  // per-XCD tile counters of the batched launch
   DevBuf feat;         // partition-feature accumulators (hs_features)
+  DevBuf bal_cost, bal_tiles, bal_ntiles;  // work-balanced fgss tile plan
+  DevBuf bal_slice_a, bal_slice_n, d_icum;  // item slices and the item table
+  DevBuf bal_rec, bal_slot;                 // per-probe records of the plan
+  PinnedBuf h_icum;
   // captured batched join: replayed while (collection, bounds, plan) match
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

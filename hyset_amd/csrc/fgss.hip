@@ -30,6 +30,8 @@
 //   * passing pairs leave through one atomic per wave.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_common.h"
 #include "hs_internal.h"
 
@@ -40,8 +42,9 @@ constexpr uint32_t FG_TOKCAP = TILE_TOKENS;    // staged probe tokens (16 KiB)
 static_assert(TILE_PROBES == FG_TPB, "one probe per thread in phase A");
 constexpr uint32_t FG_ITEMCAP = 1024; // items per batch
 constexpr uint32_t NO_LDS = 0xFFFFFFFFu;
+constexpr uint32_t FG_MOCAP = 1024;   // minoverlap entries kept in LDS (|r| + |s| < 1024)
 
-constexpr uint32_t SREG = 16;  // tokens of s held in registers per refill
+constexpr uint32_t SREG = 12;  // tokens of s held in registers per refill
 
 // Verify (r, s) from the visit reached through r[j]: an s-driven merge.  The
 // tokens of s are loaded SREG at a time into registers with independent
@@ -97,6 +100,10 @@ struct FgssLds {
   uint32_t tmp[FG_TPB / WAVE + 1];
   unsigned long long stats[4];
   uint32_t next;
+  uint32_t nxt[2];                  // claimed tiles, current and next (persistent single join)
+  uint32_t slot0;                   // first item slot of the tile (item-slice pass)
+  uint32_t mo_lds;                  // 1: mo[] holds the minoverlap table (keyed by |r| + |s|)
+  uint32_t mo[FG_MOCAP];
   uint8_t ptidx[FG_TPB];            // per-probe threshold table (serving)
   DevBounds bounds[MAX_THRESHOLDS];  // the tables of a multi-threshold launch
 };
@@ -109,6 +116,65 @@ struct TileScope {
   uint32_t size_first_len;
 };
 
+// lower_bound of k_lo[k] in ids[f_lo[k], f_lo[k] + n_lo[k]) and of k_hi[k]
+// in ids[f_hi[k], f_hi[k] + n_hi[k]) for K items at once: the 2K dependent
+// load chains advance in lockstep, so they overlap (one memory latency per
+// halving step instead of one per step and chain).  Results in f_lo, f_hi.
+template <uint32_t K>
+__device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids, uint32_t (&f_lo)[K],
+                                                uint32_t (&n_lo)[K], uint32_t (&f_hi)[K], uint32_t (&n_hi)[K],
+                                                const uint32_t (&k_lo)[K], const uint32_t (&k_hi)[K]) {
+  for (;;) {
+    uint32_t v_lo[K], v_hi[K];
+    uint32_t live = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+      live |= n_lo[k] | n_hi[k];
+      v_lo[k] = n_lo[k] ? ids[f_lo[k] + (n_lo[k] >> 1)] : 0u;
+      v_hi[k] = n_hi[k] ? ids[f_hi[k] + (n_hi[k] >> 1)] : 0u;
+    }
+    if (!live) return;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+      if (n_lo[k]) {
+        const uint32_t h = n_lo[k] >> 1;
+        if (v_lo[k] < k_lo[k]) {
+          f_lo[k] += h + 1;
+          n_lo[k] -= h + 1;
+        } else {
+          n_lo[k] = h;
+        }
+      }
+      if (n_hi[k]) {
+        const uint32_t h = n_hi[k] >> 1;
+        if (v_hi[k] < k_hi[k]) {
+          f_hi[k] += h + 1;
+          n_hi[k] -= h + 1;
+        } else {
+          n_hi[k] = h;
+        }
+      }
+    }
+  }
+}
+
+// Indexed ids [lo, hi) that probe p of size l joins: its length window
+// (an id interval, the indexed sets being sorted by size) cut to the
+// indexed range, and to ids below p in a self-join.
+__device__ __forceinline__ void probe_window(const DevBounds& B, uint32_t l, uint32_t p, const uint32_t* size_first,
+                                             uint32_t size_first_len, uint32_t idx_first, uint32_t idx_last,
+                                             int self, uint32_t& lo, uint32_t& hi) {
+  const uint32_t last = size_first_len - 1;
+  const uint32_t mins = min(B.minsize[l], last);
+  const uint32_t maxs1 = min(B.maxsize[l], last - 1) + 1;
+  lo = max(size_first[mins], idx_first);
+  hi = min(size_first[maxs1], idx_last + 1);
+  if (self) hi = min(hi, p);
+}
+
+// kSlices: item slices come from the item-slice pass; kMulti: per-probe
+// threshold tables (serving).
+template <bool kSlices, bool kMulti>
 __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T, FgssLds& L, uint32_t& my_cands,
                                           uint32_t& my_cand_tokens) {
   const uint32_t tid = threadIdx.x;
@@ -119,81 +185,103 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
   const uint32_t base = A.probe.starts[p0];
   const uint32_t tok_end = A.probe.starts[T.p1 - 1] + A.probe.sizes[T.p1 - 1];
   const uint32_t ntok = tok_end - base;
-  uint32_t n_items = 0;
+  uint32_t n_items = 0, n_active = 0;
   if (tid < np) {
     const uint32_t p = p0 + tid;
-    const uint32_t l = A.probe.sizes[p];
-    const uint32_t st = A.probe.starts[p];
-    const uint32_t ti = A.probe_tidx ? A.probe_tidx[p] : 0u;
-    const DevBounds& B = A.probe_tidx ? L.bounds[ti] : A.bounds;
-    L.ptidx[tid] = (uint8_t)ti;
-    const uint32_t last = T.size_first_len - 1;
-    const uint32_t mins = min(B.minsize[l], last);
-    const uint32_t maxs1 = min(B.maxsize[l], last - 1) + 1;
-    uint32_t lo = max(T.size_first[mins], T.idx_first);
-    uint32_t hi = min(T.size_first[maxs1], T.idx_last + 1);
-    if (A.self) hi = min(hi, p);
+    uint32_t l, st, lo, hi, mp;
+    if constexpr (kSlices) {
+      // one record per probe from the planning pass: start, size | maxprefix << 16, window
+      const uint4 rec = A.probe_rec[p - A.probe_first];
+      st = rec.x;
+      l = rec.y & 0xFFFFu;
+      mp = rec.y >> 16;
+      lo = rec.z;
+      hi = rec.w;
+      if (tid == 0) L.slot0 = A.probe_slot[p - A.probe_first];
+    } else {
+      l = A.probe.sizes[p];
+      st = A.probe.starts[p];
+      const uint32_t ti = kMulti ? A.probe_tidx[p] : 0u;
+      const DevBounds& B = kMulti ? L.bounds[ti] : A.bounds;
+      L.ptidx[tid] = (uint8_t)ti;
+      probe_window(B, l, p, T.size_first, T.size_first_len, T.idx_first, T.idx_last, A.self, lo, hi);
+      mp = B.maxprefix[l];
+    }
     L.pstart[tid] = st;
     L.psize[tid] = l;
     L.lo[tid] = lo;
     L.hi[tid] = hi;
     const uint32_t off = st - base;
     L.plds[tid] = (off + l <= FG_TOKCAP) ? off : NO_LDS;
-    n_items = (lo < hi) ? B.maxprefix[l] : 0;
+    n_active = (lo < hi) ? mp : 0;
+    // the slice pass lays out maxprefix(l) items for every probe, in probe order
+    n_items = kSlices ? mp : n_active;
   }
   for (uint32_t i = tid; i < ntok && i < FG_TOKCAP; i += FG_TPB) L.tok[i] = A.probe.tokens[base + i];
+  n_active = wave_sum(n_active);
+  if (lane_id() == 0 && n_active) atomicAdd(&L.stats[3], (unsigned long long)n_active);
   uint32_t total_items;
   uint32_t ioff = block_exclusive_scan<FG_TPB>(n_items, L.tmp, &total_items);
   if (tid < np) L.item_off[tid] = ioff;
-  if (tid == 0) {
-    L.item_off[np] = total_items;
-    L.stats[3] += total_items;
-  }
+  if (tid == 0) L.item_off[np] = total_items;
   __syncthreads();
 
   // ---- phase B: batches of items
   for (uint32_t ib = 0; ib < total_items; ib += FG_ITEMCAP) {
     const uint32_t nb = min(FG_ITEMCAP, total_items - ib);
-    // slice of the posting list for every item of the batch
-    for (uint32_t it = tid; it < FG_ITEMCAP; it += FG_TPB) {
-      uint32_t len = 0;
-      if (it < nb) {
-        const uint32_t item = ib + it;
-        const uint32_t pl = segment_of(L.item_off, np, item);
-        const uint32_t j = item - L.item_off[pl];
-        const uint32_t* r = (L.plds[pl] != NO_LDS) ? (L.tok + L.plds[pl]) : (A.probe.tokens + L.pstart[pl]);
-        const uint32_t token = r[j];
-        uint32_t a0 = 0;
-        if (token < A.index.universe) {
-          const uint32_t lb = A.index.offsets[token];
-          const uint32_t le = A.index.offsets[token + 1];
-          const uint32_t lo = L.lo[pl], hi = L.hi[pl];
-          if (le - lb <= 8) {
-            // short list: independent loads, count ids below each bound
-            uint32_t below_lo = 0, below_hi = 0;
+    // slice of the posting list for every item of the batch: read from the
+    // item-slice pass when there is one, otherwise searched here, PER items
+    // per thread in lockstep
+    constexpr uint32_t PER = FG_ITEMCAP / FG_TPB;
+    if constexpr (kSlices) {
 #pragma unroll
-            for (uint32_t q = 0; q < 8; ++q) {
-              if (lb + q < le) {
-                const uint32_t id = A.index.set_ids[lb + q];
-                below_lo += id < lo;
-                below_hi += id < hi;
-              }
-            }
-            a0 = lb + below_lo;
-            len = below_hi - below_lo;
-          } else {
-            a0 = lower_bound_u32(A.index.set_ids, lb, le, lo);
-            len = lower_bound_u32(A.index.set_ids, a0, le, hi) - a0;
+      for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t it = tid + k * FG_TPB;
+        uint32_t len = 0;
+        if (it < nb) {
+          const uint32_t item = ib + it;
+          const uint32_t pl = segment_of(L.item_off, np, item);
+          const uint32_t j = item - L.item_off[pl];
+          L.it_pj[it] = pl | (j << 8);
+          L.it_a[it] = A.slice_a[L.slot0 + item];
+          len = A.slice_n[L.slot0 + item];
+        }
+        L.vis_off[it] = len;
+      }
+    } else {
+      uint32_t f_lo[PER], n_lo[PER], f_hi[PER], n_hi[PER], k_lo[PER], k_hi[PER];
+#pragma unroll
+      for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t it = tid + k * FG_TPB;
+        f_lo[k] = f_hi[k] = n_lo[k] = n_hi[k] = k_lo[k] = k_hi[k] = 0;
+        if (it < nb) {
+          const uint32_t item = ib + it;
+          const uint32_t pl = segment_of(L.item_off, np, item);
+          const uint32_t j = item - L.item_off[pl];
+          const uint32_t* r = (L.plds[pl] != NO_LDS) ? (L.tok + L.plds[pl]) : (A.probe.tokens + L.pstart[pl]);
+          const uint32_t token = r[j];
+          L.it_pj[it] = pl | (j << 8);
+          k_lo[k] = L.lo[pl];
+          k_hi[k] = L.hi[pl];
+          if (token < A.index.universe) {
+            const uint32_t lb = A.index.offsets[token];
+            const uint32_t le = A.index.offsets[token + 1];
+            f_lo[k] = f_hi[k] = lb;
+            n_lo[k] = n_hi[k] = le - lb;
           }
         }
-        L.it_a[it] = a0;
-        L.it_pj[it] = pl | (j << 8);
       }
-      L.vis_off[it] = len;
+      lockstep_bounds<PER>(A.index.set_ids, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
+#pragma unroll
+      for (uint32_t k = 0; k < PER; ++k) {
+        const uint32_t it = tid + k * FG_TPB;
+        // items exist only for lo < hi, so the hi bound never lies below lo
+        L.it_a[it] = f_lo[k];
+        L.vis_off[it] = (it < nb) ? f_hi[k] - f_lo[k] : 0u;
+      }
     }
     __syncthreads();
-    // exclusive scan of the slice lengths (4 per thread)
-    constexpr uint32_t PER = FG_ITEMCAP / FG_TPB;
+    // exclusive scan of the slice lengths (PER per thread)
     uint32_t v[PER];
     uint32_t local = 0;
 #pragma unroll
@@ -233,7 +321,9 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
         rid = p0 + pl;
         const uint32_t lr = L.psize[pl];
         const uint32_t ls = (meta >> 16) == 0xFFFFu ? A.indexed.sizes[sid] : (meta >> 16);
-        const uint32_t mo = dev_minoverlap(A.probe_tidx ? L.bounds[L.ptidx[pl]] : A.bounds, lr, ls);
+        uint32_t mo;
+        if (!kMulti && L.mo_lds) mo = min(L.mo[lr + ls], min(lr, ls));
+        else mo = dev_minoverlap(kMulti ? L.bounds[L.ptidx[pl]] : A.bounds, lr, ls);
         // Positional filter (helpers.hpp:46-50) at this visit's token: if it is
         // the first common token, |r n s| <= 1 + min(|r|-1-j, |s|-1-kp).  Only
         // the first-common visit can emit the pair, so a visit failing the
@@ -271,15 +361,59 @@ __device__ __forceinline__ uint32_t next_batch_tile(const FgssBatch& B, uint32_t
   return 0xFFFFFFFFu;
 }
 
-__global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint32_t ntiles) {
+// Pulls the next tile of a persistent single-join launch: XCD x drains the
+// contiguous run [x * per, (x + 1) * per) of the plan (neighbouring tiles
+// read neighbouring sets, which then stay in that XCD's L2), then steals
+// from the other runs.
+__device__ __forceinline__ uint32_t next_plain_tile(uint32_t* queue, uint32_t ntiles, uint32_t home) {
+  constexpr uint32_t NXCD = 8;
+  const uint32_t per = (ntiles + NXCD - 1) / NXCD;
+  for (uint32_t q = 0; q < NXCD; ++q) {
+    const uint32_t x = (home + q) % NXCD;
+    const uint32_t lo = x * per, hi = min(lo + per, ntiles);
+    if (lo >= hi || queue[x] >= hi - lo) continue;
+    const uint32_t k = atomicAdd(&queue[x], 1u);
+    if (k < hi - lo) return lo + k;
+  }
+  return 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ void plain_tile_range(const FgssArgs& A, uint32_t tile, uint32_t ppt, uint32_t& p0,
+                                                 uint32_t& p1) {
+  if (A.tile_starts) {
+    const uint32_t t = A.first_tile + tile;
+    p0 = max(A.tile_starts[t], A.probe_first);
+    p1 = min(A.tile_starts[t + 1], A.probe_last + 1);
+  } else {
+    p0 = A.probe_first + tile * ppt;
+    p1 = min(p0 + ppt, A.probe_last + 1);
+  }
+}
+
+// Launch modes: one workgroup per tile, a persistent grid over the tiles of
+// one join, a persistent grid over the tiles of a batch of joins.
+enum FgssMode { FG_TILES = 0, FG_QUEUE = 1, FG_BATCH = 2 };
+
+template <int kMode, bool kSlices, bool kMulti>
+__global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_fgss(FgssArgs A, uint32_t ppt,
+                                                                                        uint32_t ntiles) {
   __shared__ FgssLds L;
   const uint32_t tid = threadIdx.x;
-  if (A.probe_tidx) {
+  if constexpr (kMulti) {
     if (tid < MAX_THRESHOLDS) L.bounds[tid] = A.probe_bounds[tid];
     __syncthreads();
   }
   uint32_t my_cands = 0, my_cand_tokens = 0;
-  if (A.batch.tile_join) {
+  if constexpr (!kMulti) {
+    // the 1-D minoverlap table (Jaccard, Dice) when it fits: one LDS read per visit
+    const uint32_t nmo = 2 * A.bounds.L + 1;
+    const bool fits = A.bounds.kind != HS_COSINE && nmo <= FG_MOCAP;
+    if (fits)
+      for (uint32_t i = tid; i < nmo; i += FG_TPB) L.mo[i] = A.bounds.mo[i];
+    if (tid == 0) L.mo_lds = fits;
+    __syncthreads();
+  }
+  if constexpr (kMode == FG_BATCH) {
     // persistent: one launch for every tile of every join of the batch
     if (tid < 4) L.stats[tid] = 0;
     for (;;) {
@@ -291,23 +425,37 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
       const uint32_t j = A.batch.tile_join[k];
       TileScope T{A.batch.tile_lo[k], A.batch.tile_hi[k], A.batch.join_first[j], A.batch.join_first[j + 1] - 1,
                   A.size_first + A.batch.sf_base[j], A.batch.sf_base[j + 1] - A.batch.sf_base[j]};
-      fgss_tile(A, T, L, my_cands, my_cand_tokens);
+      fgss_tile<kSlices, kMulti>(A, T, L, my_cands, my_cand_tokens);
+    }
+  } else if constexpr (kMode == FG_QUEUE) {
+    // persistent over the tiles of one join (tile count on the device when
+    // the plan was made there)
+    if (A.dev_ntiles) ntiles = *A.dev_ntiles;
+    if (tid < 4) L.stats[tid] = 0;
+    // the claim of the next tile is issued when a tile starts; its latency
+    // overlaps the tile's own loads
+    if (tid == 0) L.nxt[0] = next_plain_tile(A.tile_queue, ntiles, blockIdx.x % 8);
+    __syncthreads();
+    for (uint32_t it = 0;; ++it) {
+      const uint32_t tile = L.nxt[it & 1];
+      if (tile == 0xFFFFFFFFu) break;
+      uint32_t claimed = 0;
+      if (tid == 0) claimed = next_plain_tile(A.tile_queue, ntiles, blockIdx.x % 8);
+      uint32_t p0, p1;
+      plain_tile_range(A, tile, ppt, p0, p1);
+      TileScope T{p0, p1, A.idx_first, A.idx_last, A.size_first, A.size_first_len};
+      fgss_tile<kSlices, kMulti>(A, T, L, my_cands, my_cand_tokens);
+      if (tid == 0) L.nxt[(it + 1) & 1] = claimed;
+      __syncthreads();
     }
   } else {
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
     if (tile == 0xFFFFFFFFu) return;
     if (tid < 4) L.stats[tid] = 0;
     uint32_t p0, p1;
-    if (A.tile_starts) {
-      const uint32_t t = A.first_tile + tile;
-      p0 = max(A.tile_starts[t], A.probe_first);
-      p1 = min(A.tile_starts[t + 1], A.probe_last + 1);
-    } else {
-      p0 = A.probe_first + tile * ppt;
-      p1 = min(p0 + ppt, A.probe_last + 1);
-    }
+    plain_tile_range(A, tile, ppt, p0, p1);
     TileScope T{p0, p1, A.idx_first, A.idx_last, A.size_first, A.size_first_len};
-    fgss_tile(A, T, L, my_cands, my_cand_tokens);
+    fgss_tile<kSlices, kMulti>(A, T, L, my_cands, my_cand_tokens);
   }
 
   // ---- statistics: one global atomic per workgroup and counter
@@ -326,20 +474,145 @@ __global__ __launch_bounds__(FG_TPB) void k_fgss(FgssArgs A, uint32_t ppt, uint3
   }
 }
 
-void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt) {
+// ---- item slices and the work-balanced tile plan
+//
+// Item slot s of the probe range belongs to probe p and prefix position j
+// with s = item_cum[l] + (p - first probe of size l) * maxprefix(l) + j
+// (sets are sorted by size, so the slots of one size are one run).  One
+// thread per slot finds its posting-list slice [a, a + n) of ids inside the
+// probe's window; probes whose window is empty get n = 0.
+
+__global__ void k_item_slices(FgssArgs A, uint32_t nl, uint32_t total, uint32_t* __restrict__ slice_a,
+                              uint32_t* __restrict__ slice_n) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= total) return;
+  // size class: the last l with item_cum[l] <= s (and a non-empty run)
+  uint32_t lo_l = 0, hi_l = nl - 1;  // item_cum[nl - 1] = total > s
+  while (hi_l - lo_l > 1) {
+    const uint32_t mid = (lo_l + hi_l) >> 1;
+    if (A.item_cum[mid] <= s) lo_l = mid;
+    else hi_l = mid;
+  }
+  const uint32_t l = lo_l;
+  const uint32_t mp = A.bounds.maxprefix[l];
+  const uint32_t q = s - A.item_cum[l];
+  const uint32_t p = max(A.probe_size_first[l], A.probe_first) + q / mp;
+  const uint32_t j = q % mp;
+  uint32_t lo, hi;
+  probe_window(A.bounds, l, p, A.size_first, A.size_first_len, A.idx_first, A.idx_last, A.self, lo, hi);
+  uint32_t f_lo[1] = {0}, n_lo[1] = {0}, f_hi[1] = {0}, n_hi[1] = {0};
+  const uint32_t k_lo[1] = {lo}, k_hi[1] = {hi};
+  if (lo < hi) {
+    const uint32_t token = A.probe.tokens[A.probe.starts[p] + j];
+    if (token < A.index.universe) {
+      const uint32_t lb = A.index.offsets[token];
+      f_lo[0] = f_hi[0] = lb;
+      n_lo[0] = n_hi[0] = A.index.offsets[token + 1] - lb;
+    }
+  }
+  lockstep_bounds<1>(A.index.set_ids, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
+  slice_a[s] = f_lo[0];
+  slice_n[s] = (lo < hi) ? f_hi[0] - f_lo[0] : 0u;
+}
+
+// Per probe a cost in units of COST_W-th of a tile: the largest of 4 (so a
+// tile holds at most 256 probes), a quarter per token (at most 4096 tokens
+// besides the last probe's) and an eighth per posting it visits (about 8K
+// visits per tile), capped at one tile.  Taking the largest term keeps all
+// three limits while a tile of light probes still takes 256 of them.  Tile
+// k takes the probes whose exclusive cost prefix lies in [k W, (k+1) W).  A
+// probe costs at most W, so consecutive probes never skip a tile: every
+// tile is non-empty.
+constexpr uint32_t COST_W = 1024;
+
+__global__ void k_probe_cost(FgssArgs A, uint32_t* __restrict__ cost, uint4* __restrict__ rec,
+                             uint32_t* __restrict__ slot) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t p = A.probe_first + i;
+  if (p > A.probe_last) return;
+  const uint32_t l = A.probe.sizes[p];
+  const uint32_t mp = A.bounds.maxprefix[l];
+  const uint32_t base = A.item_cum[l] + (p - max(A.probe_size_first[l], A.probe_first)) * mp;
+  uint32_t lo, hi;
+  probe_window(A.bounds, l, p, A.size_first, A.size_first_len, A.idx_first, A.idx_last, A.self, lo, hi);
+  rec[i] = make_uint4(A.probe.starts[p], l | (mp << 16), lo, hi);
+  slot[i] = base;
+  uint32_t v = 0;
+  for (uint32_t j = 0; j < mp && v < 8 * COST_W; ++j) v += A.slice_n[base + j];
+  const uint32_t c = max(4u, max((l + 3) / 4, (min(v, 8 * COST_W) + 7) / 8));
+  cost[i] = min(c, COST_W);
+}
+
+__global__ void k_balanced_tiles(const uint32_t* __restrict__ cum, uint32_t n, uint32_t probe_first,
+                                 uint32_t* __restrict__ tile_starts, uint32_t* __restrict__ ntiles) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = cum[i] / COST_W;
+  if (i == 0 || cum[i - 1] / COST_W != k) tile_starts[k] = probe_first + i;
+  if (i == n - 1) {
+    tile_starts[k + 1] = probe_first + n;
+    *ntiles = k + 1;
+  }
+}
+
+void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp) {
+  const uint32_t n = a.probe_last - a.probe_first + 1;
+  HS_REQUIRE((uint64_t)n * COST_W < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "probe range too large for a balanced plan");
+  bp.slice_a->ensure((size_t)std::max<uint64_t>(bp.total_items, 1) * 4);
+  bp.slice_n->ensure((size_t)std::max<uint64_t>(bp.total_items, 1) * 4);
+  bp.cost->ensure((size_t)n * 4);
+  bp.rec->ensure((size_t)n * 16);
+  bp.slot->ensure((size_t)n * 4);
+  bp.tiles->ensure(((size_t)n + 1) * 4);
+  bp.ntiles->ensure(4);
+  a.item_cum = bp.item_cum;
+  a.probe_size_first = bp.probe_size_first;
+  a.slice_a = bp.slice_a->as<uint32_t>();
+  a.slice_n = bp.slice_n->as<uint32_t>();
+  if (bp.total_items)
+    hipLaunchKernelGGL(k_item_slices, dim3((unsigned)((bp.total_items + 255) / 256)), dim3(256), 0, st, a, bp.nl,
+                       (uint32_t)bp.total_items, bp.slice_a->as<uint32_t>(), bp.slice_n->as<uint32_t>());
+  uint32_t* c = bp.cost->as<uint32_t>();
+  hipLaunchKernelGGL(k_probe_cost, dim3((n + 255) / 256), dim3(256), 0, st, a, c, bp.rec->as<uint4>(),
+                     bp.slot->as<uint32_t>());
+  a.probe_rec = bp.rec->as<uint4>();
+  a.probe_slot = bp.slot->as<uint32_t>();
+  launch_exclusive_scan(st, c, c, nullptr, n, *bp.scan_scratch);
+  hipLaunchKernelGGL(k_balanced_tiles, dim3((n + 255) / 256), dim3(256), 0, st, c, n, a.probe_first,
+                     bp.tiles->as<uint32_t>(), bp.ntiles->as<uint32_t>());
+  HS_CHECK_HIP(hipGetLastError());
+  a.tile_starts = bp.tiles->as<uint32_t>();
+  a.first_tile = 0;
+  a.n_plan_tiles = n;  // an upper bound; the kernel reads the count
+  a.dev_ntiles = bp.ntiles->as<uint32_t>();
+}
+
+void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persistent_wgs) {
   if (a.probe.n == 0 || a.probe_first > a.probe_last) return;
   if (ppt == 0 || ppt > (uint32_t)FG_TPB) ppt = FG_TPB;
   const uint32_t nprobes = a.probe_last - a.probe_first + 1;
   const uint32_t ntiles = a.tile_starts ? a.n_plan_tiles : (nprobes + ppt - 1) / ppt;
-  const uint32_t grid = (ntiles + 7) / 8 * 8;
-  hipLaunchKernelGGL(k_fgss, dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+  if (ntiles == 0) return;
+  uint32_t grid = (ntiles + 7) / 8 * 8;
+  if (a.tile_queue) {
+    launch_fill_u32(st, a.tile_queue, 8, 0);
+    if (persistent_wgs) grid = std::min(grid, (persistent_wgs + 7) / 8 * 8);
+  }
+  if (a.tile_queue) {
+    if (a.slice_a) hipLaunchKernelGGL((k_fgss<FG_QUEUE, true, false>), dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+    else hipLaunchKernelGGL((k_fgss<FG_QUEUE, false, false>), dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+  } else if (a.probe_tidx) {
+    hipLaunchKernelGGL((k_fgss<FG_TILES, false, true>), dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+  } else {
+    hipLaunchKernelGGL((k_fgss<FG_TILES, false, false>), dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+  }
   HS_CHECK_HIP(hipGetLastError());
 }
 
 void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups) {
   if (ntiles == 0) return;
   launch_fill_u32(st, a.batch.queue, 8, 0);
-  hipLaunchKernelGGL(k_fgss, dim3(workgroups), dim3(FG_TPB), 0, st, a, 0u, ntiles);
+  hipLaunchKernelGGL((k_fgss<FG_BATCH, false, false>), dim3(workgroups), dim3(FG_TPB), 0, st, a, 0u, ntiles);
   HS_CHECK_HIP(hipGetLastError());
 }
 

@@ -180,13 +180,41 @@ struct FgssArgs {
   // (at most MAX_THRESHOLDS tables); NULL uses `bounds` for every probe
   const DevBounds* probe_bounds;
   const uint8_t* probe_tidx;
+  // non-NULL: a persistent grid claims plan tiles from 8 per-XCD counters
+  // (zeroed by launch_fgss), heaviest tiles first; NULL: one workgroup per tile
+  uint32_t* tile_queue;
+  const uint32_t* dev_ntiles;  // non-NULL: the tile count lives on the device
+  // item-slice pass (plain joins): per item slot the posting-list slice of
+  // ids in the probe's window; item_cum[l] = first slot of size l in the
+  // probe range, probe_size_first = the probe collection's size table
+  const uint32_t* slice_a;
+  const uint32_t* slice_n;
+  const uint32_t* item_cum;
+  const uint32_t* probe_size_first;
+  // per probe of the range (planning pass): {start, size | maxprefix << 16,
+  // window lo, window hi} and its first item slot
+  const uint4* probe_rec;
+  const uint32_t* probe_slot;
   DevOut out;
 };
 constexpr uint32_t MAX_THRESHOLDS = 8;
 constexpr uint32_t TILE_TOKENS = 4096;  // probe tokens staged in LDS per tile
 constexpr uint32_t TILE_PROBES = 256;
-void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile);
+// persistent_wgs: grid size when a.tile_queue is set (0: one per tile)
+void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile, uint32_t persistent_wgs = 0);
 void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups);
+// Work buffers and the host-made item table of the item-slice pass.
+struct BalancedPlan {
+  const uint32_t* item_cum;          // device, nl entries
+  const uint32_t* probe_size_first;  // device
+  uint32_t nl;                       // size classes + 1
+  uint64_t total_items;
+  DevBuf *slice_a, *slice_n, *cost, *tiles, *ntiles, *scan_scratch, *rec, *slot;
+};
+// Runs the item-slice pass and plans work-balanced tiles of a's probe range
+// on the device (after the index build), pointing a at both; launch with
+// a.tile_queue set.
+void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp);
 
 struct BitmapArgs {
   DevColl probe;

@@ -162,16 +162,18 @@ def argmin_plan(pred: np.ndarray) -> np.ndarray:
     return np.argmin(pred, axis=1).astype(np.int32)
 
 
-def concurrent_plan(pred: np.ndarray, cpu_threads: int) -> np.ndarray:
+def concurrent_plan(pred: np.ndarray, cpu_threads: int, cpu_scale: float = 1.0) -> np.ndarray:
     """Longest-first list scheduling of units onto one device queue and
     cpu_threads host threads, by predicted time: each unit goes where it
     would finish first (the device takes its cheapest device technique).
     Runs the split the way hs_split_run executes it: CPU units concurrently
-    with the device units."""
+    with the device units.  cpu_scale stretches the CPU predictions (times
+    measured on one idle core) to what a core achieves while all the others
+    are busy too (calibrate_cpu_scale)."""
     pred = np.asarray(pred, dtype=np.float64)
     gtech = np.array(GPU_TECHS)[np.argmin(pred[:, list(GPU_TECHS)], axis=1)]
     g = pred[np.arange(len(pred)), gtech]
-    c = pred[:, TECH_CPU]
+    c = pred[:, TECH_CPU] * cpu_scale
     plan = gtech.astype(np.int32).copy()
     gpu_load = 0.0
     cpu = [0.0] * max(1, cpu_threads)
@@ -184,11 +186,14 @@ def concurrent_plan(pred: np.ndarray, cpu_threads: int) -> np.ndarray:
     return plan
 
 
-def plan_cost(plan: np.ndarray, T: np.ndarray, cpu_threads: int = 0) -> float:
+def plan_cost(plan: np.ndarray, T: np.ndarray, cpu_threads: int = 0, cpu_scale: float = 1.0) -> float:
     """Measured cost of a plan: the sum of its units' times (sequential, as
     cost_model.cu runs them) or, with cpu_threads > 0, the makespan of the
-    device queue against the host threads (list-scheduled longest first)."""
+    device queue against the host threads (list-scheduled longest first,
+    CPU times stretched by cpu_scale)."""
     t = T[np.arange(len(plan)), plan]
+    if cpu_threads > 0:
+        t = np.where(plan == TECH_CPU, t * cpu_scale, t)
     if cpu_threads <= 0:
         return float(t.sum())
     gpu = float(t[plan != TECH_CPU].sum())
@@ -196,6 +201,17 @@ def plan_cost(plan: np.ndarray, T: np.ndarray, cpu_threads: int = 0) -> float:
     for x in sorted(t[plan == TECH_CPU], reverse=True):
         heapq.heapreplace(cpu, cpu[0] + x)
     return max(gpu, max(cpu))
+
+
+def calibrate_cpu_scale(ctx: Context, units, T: np.ndarray, cpu_threads: int) -> float:
+    """Median slowdown of CPU units run all at once on cpu_threads threads
+    (hs_split_run) against their one-at-a-time times T[:, cpu]."""
+    plan = np.full(len(units), TECH_CPU, dtype=np.int32)
+    _, unit_ms, _ = ctx.split_run(units, plan, cpu_threads=cpu_threads)
+    m = T[:, TECH_CPU] >= 1.0  # long enough to time
+    if not np.any(m):
+        return 1.0
+    return float(max(1.0, np.median(unit_ms[m] / T[m, TECH_CPU])))
 
 
 class Split:
@@ -247,22 +263,23 @@ class Split:
                 "analytic_mispredicted": int((pa != best).sum())}
 
 
-def summarize(plans: dict, T: np.ndarray, cpu_threads: int) -> dict:
+def summarize(plans: dict, T: np.ndarray, cpu_threads: int, cpu_scale: float = 1.0) -> dict:
     """Measured cost of each plan against the best per-unit choice."""
     best = float(T.min(axis=1).sum())
     out = {}
     for name, plan in plans.items():
         seq = plan_cost(plan, T)
         out[name] = {"sum_ms": round(seq, 3), "regret_ms": round(seq - best, 3),
-                     "makespan_ms": round(plan_cost(plan, T, cpu_threads), 3),
+                     "makespan_ms": round(plan_cost(plan, T, cpu_threads, cpu_scale), 3),
                      "share": {TECHNIQUES[k]: int((plan == k).sum()) for k in range(N_TECH)}}
     return out
 
 
 def default_threads() -> int:
     import os
-    return max(1, min(15, len(os.sched_getaffinity(0)) - 1))
+    return max(1, min(14, len(os.sched_getaffinity(0)) - 2))
 
 
 __all__ = ["AnalyticModel", "LearnedModel", "Split", "measure_units", "argmin_plan", "concurrent_plan",
+           "calibrate_cpu_scale",
            "plan_cost", "summarize", "log_error", "reference_terms", "learned_inputs", "default_threads"]

@@ -147,7 +147,7 @@ typedef struct {
                                  candidate, 0 pick by set size (default 0) */
   uint32_t host_threads;      /* hybrid: CPU filter threads (default 1) */
   uint64_t batch_candidates;  /* hybrid: candidates per device batch (default 1<<22) */
-  uint32_t flags;             /* HS_TUNE_* bits (batched joins only) */
+  uint32_t flags;             /* HS_TUNE_* bits */
   uint32_t tile_tokens;       /* batched fgss: probe tokens per tile, <= 4096 (default 4096) */
   uint32_t workgroups_per_cu; /* batched fgss: persistent workgroups per CU (default 4) */
 } hs_tune;
@@ -155,6 +155,11 @@ typedef struct {
 /* hs_join_batch: capture the device sequence in a HIP graph on first use and
  * replay it while the collection, the bounds and the tile plan are unchanged. */
 enum { HS_TUNE_GRAPH = 1 };
+/* hs_join fgss: one workgroup per tile of the collection's upload-time plan
+ * (at most 256 probes and 4096 tokens) instead of the default: tiles of equal
+ * estimated work planned on the device after the index build and claimed
+ * by a persistent grid (kept for comparison). */
+enum { HS_TUNE_STATIC_TILES = 2 };
 
 /* Result sink.  count is always written.  With count_only == 0 the pairs
  * (r = probe id, s = indexed id, overlap = |r n s|) are copied into the

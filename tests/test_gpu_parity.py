@@ -33,6 +33,22 @@ def test_fgss_small(ctx, small, t, sim):
     _check(ctx.join(d, algo="fgss"), want, f"fgss {sim} {t}")
 
 
+@pytest.mark.parametrize("tune", ["static", "ppt64", "ppt1"])
+@pytest.mark.parametrize("t", [0.6, 0.8])
+def test_fgss_tile_plans(ctx, small, t, tune):
+    """The upload-time plan (one workgroup per tile) and fixed-size tiles give
+    the balanced plan's result."""
+    c, d = small
+    ctx.set_threshold(t)
+    want = O.self_join(c, t)
+    tu = {"static": H.Tune(flags=H.TUNE_STATIC_TILES), "ppt64": H.Tune(probes_per_tile=64),
+          "ppt1": H.Tune(probes_per_tile=1)}[tune]
+    _check(ctx.join(d, algo="fgss", tune=tu), want, f"fgss {tune} {t}")
+    for rng in ((0, 0), (1, 1), (2999, 2999), (100, 2500)):
+        want = O.self_join_blocks(c, t, rng, rng)
+        _check(ctx.join(d, algo="fgss", indexed_range=rng, probe_range=rng), want, f"fgss balanced {rng}")
+
+
 @pytest.mark.parametrize("bits", [64, 128, 256])
 @pytest.mark.parametrize("t", [0.6, 0.8, 0.9])
 def test_bitmap_small(ctx, small, t, bits):
@@ -73,6 +89,8 @@ def test_fgss_config2_full(ctx, t):
         want = O.cpu_multi(c, t, 8)
         _check(ctx.join(d, algo="fgss"), want, f"fgss 1M t={t}")
         st = ctx.stats()
+        if t == 0.8:
+            _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_STATIC_TILES)), want, "fgss 1M static tiles")
         # postings inside the length windows: the same count as the oracle's
         assert st["posting_visits"] == want.posting_visits
         # verified candidates: at most the oracle's (positional filter drops more)
