@@ -494,23 +494,33 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         if (ppt == 0 && !(tune.flags & HS_TUNE_STATIC_TILES) &&
             (uint64_t)(pr.last - pr.first + 1) * 1024 < 0xFFFFFFFFull && probe->maxlen < 0xFFFFu) {
           // item slices first, then tiles of equal work claimed by a persistent grid
-          const auto& mp = ctx->tables.maxprefix;
+          // per size class of the probe range: first item slot, first probe,
+          // window of indexed ids, maxprefix
+          const auto& T = ctx->tables;
           const auto& psf = probe->h_size_first;
+          const auto& isf = indexed->h_size_first;
           const uint32_t nl = (uint32_t)psf.size();
-          ctx->h_icum.ensure((size_t)nl * 4);
-          ctx->d_icum.ensure((size_t)nl * 4);
-          uint32_t* icum = ctx->h_icum.as<uint32_t>();
+          const uint32_t ilast = (uint32_t)isf.size() - 1;
+          ctx->h_icum.ensure((size_t)nl * 20);
+          ctx->d_icum.ensure((size_t)nl * 20);
+          uint32_t* tab = ctx->h_icum.as<uint32_t>();
           uint64_t items = 0;
-          for (uint32_t l = 0; l + 1 < nl; ++l) {
-            icum[l] = (uint32_t)items;
-            const uint32_t lo = std::max(pr.first, psf[l]);
-            const uint32_t hi = std::min(pr.last + 1, psf[l + 1]);
-            if (hi > lo) items += (uint64_t)(hi - lo) * mp[l];
-            HS_REQUIRE(items < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "probe range limited to 2^32-1 prefix items");
+          for (uint32_t l = 0; l < nl; ++l) {
+            tab[l] = (uint32_t)items;
+            tab[nl + l] = std::max(pr.first, psf[l]);
+            const uint32_t mins = std::min(T.minsize[std::min(l, T.L)], ilast);
+            const uint32_t maxs1 = std::min(T.maxsize[std::min(l, T.L)], ilast - 1) + 1;
+            tab[2 * nl + l] = std::max(isf[mins], ir.first);
+            tab[3 * nl + l] = std::min(isf[maxs1], ir.last + 1);
+            tab[4 * nl + l] = T.maxprefix[std::min(l, T.L)];
+            if (l + 1 < nl) {
+              const uint32_t hi = std::min(pr.last + 1, psf[l + 1]);
+              if (hi > tab[nl + l]) items += (uint64_t)(hi - tab[nl + l]) * tab[4 * nl + l];
+              HS_REQUIRE(items < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "probe range limited to 2^32-1 prefix items");
+            }
           }
-          icum[nl - 1] = (uint32_t)items;
-          HS_CHECK_HIP(hipMemcpyAsync(ctx->d_icum.ptr, icum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
-          hs::BalancedPlan bp{ctx->d_icum.as<uint32_t>(), probe->size_first.as<uint32_t>(), nl, items,
+          HS_CHECK_HIP(hipMemcpyAsync(ctx->d_icum.ptr, tab, (size_t)nl * 20, hipMemcpyHostToDevice, st));
+          hs::BalancedPlan bp{ctx->d_icum.as<uint32_t>(), nl, items,
                               &ctx->bal_slice_a, &ctx->bal_slice_n, &ctx->bal_cost, &ctx->bal_tiles,
                               &ctx->bal_ntiles, &ctx->ib.scan_scratch, &ctx->bal_rec, &ctx->bal_slot};
           hs::plan_balanced_tiles(st, a, bp);

@@ -116,38 +116,99 @@ struct TileScope {
   uint32_t size_first_len;
 };
 
-// lower_bound of k_lo[k] in ids[f_lo[k], f_lo[k] + n_lo[k]) and of k_hi[k]
-// in ids[f_hi[k], f_hi[k] + n_hi[k]) for K items at once: the 2K dependent
-// load chains advance in lockstep, so they overlap (one memory latency per
-// halving step instead of one per step and chain).  Results in f_lo, f_hi.
+// Window slices of K posting lists at once: for each k, a = lower_bound of
+// k_lo[k] in ids[f_lo[k], f_lo[k] + n_lo[k]) by bisection, then the end of
+// the slice, lower_bound of k_hi[k] >= k_lo[k], by galloping from a (probes
+// at a, a+1, a+3, a+7, ... then bisection inside the last gap).  Slices are
+// short, so the gallop stays on the lines the bisection just read.  All K
+// chains advance in lockstep, so their loads overlap (one memory latency
+// per step instead of one per step and chain).  On return f_lo[k] = a and
+// n_hi[k] = slice length; f_hi is scratch.  n_hi must come in as n_lo.
 template <uint32_t K>
 __device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids, uint32_t (&f_lo)[K],
                                                 uint32_t (&n_lo)[K], uint32_t (&f_hi)[K], uint32_t (&n_hi)[K],
                                                 const uint32_t (&k_lo)[K], const uint32_t (&k_hi)[K]) {
+  uint32_t end[K];
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) end[k] = f_lo[k] + n_lo[k];
+  // bisection for the slice start
   for (;;) {
-    uint32_t v_lo[K], v_hi[K];
+    uint32_t v[K];
     uint32_t live = 0;
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
-      live |= n_lo[k] | n_hi[k];
-      v_lo[k] = n_lo[k] ? ids[f_lo[k] + (n_lo[k] >> 1)] : 0u;
-      v_hi[k] = n_hi[k] ? ids[f_hi[k] + (n_hi[k] >> 1)] : 0u;
+      live |= n_lo[k];
+      v[k] = n_lo[k] ? ids[f_lo[k] + (n_lo[k] >> 1)] : 0u;
     }
-    if (!live) return;
+    if (!live) break;
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
       if (n_lo[k]) {
         const uint32_t h = n_lo[k] >> 1;
-        if (v_lo[k] < k_lo[k]) {
+        if (v[k] < k_lo[k]) {
           f_lo[k] += h + 1;
           n_lo[k] -= h + 1;
         } else {
           n_lo[k] = h;
         }
       }
+    }
+  }
+  // gallop for the slice end: ids[f_lo, left) < k_hi, the end lies in [left, right]
+  uint32_t left[K], step[K], right[K];
+  bool act[K];
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) {
+    left[k] = f_lo[k];
+    step[k] = 1;
+    right[k] = end[k];
+    act[k] = n_hi[k] != 0 && left[k] < right[k];
+  }
+  for (;;) {
+    uint32_t v[K];
+    bool any = false;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+      const uint32_t idx = left[k] + step[k] - 1;
+      if (act[k] && idx >= right[k]) act[k] = false;  // past the list: the end lies in [left, right)
+      v[k] = act[k] ? ids[idx] : 0u;
+      any |= act[k];
+    }
+    if (!any) break;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+      if (act[k]) {
+        const uint32_t idx = left[k] + step[k] - 1;
+        if (v[k] < k_hi[k]) {
+          left[k] = idx + 1;
+          step[k] <<= 1;
+        } else {
+          right[k] = idx;
+          act[k] = false;
+        }
+      }
+    }
+  }
+  // bisection inside [left, right)
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) {
+    f_hi[k] = left[k];
+    n_hi[k] = n_hi[k] ? right[k] - left[k] : 0u;
+  }
+  for (;;) {
+    uint32_t v[K];
+    uint32_t live = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+      live |= n_hi[k];
+      v[k] = n_hi[k] ? ids[f_hi[k] + (n_hi[k] >> 1)] : 0u;
+    }
+    if (!live) break;
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
       if (n_hi[k]) {
         const uint32_t h = n_hi[k] >> 1;
-        if (v_hi[k] < k_hi[k]) {
+        if (v[k] < k_hi[k]) {
           f_hi[k] += h + 1;
           n_hi[k] -= h + 1;
         } else {
@@ -156,6 +217,8 @@ __device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids
       }
     }
   }
+#pragma unroll
+  for (uint32_t k = 0; k < K; ++k) n_hi[k] = f_hi[k] - f_lo[k];
 }
 
 // Indexed ids [lo, hi) that probe p of size l joins: its length window
@@ -275,9 +338,8 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
 #pragma unroll
       for (uint32_t k = 0; k < PER; ++k) {
         const uint32_t it = tid + k * FG_TPB;
-        // items exist only for lo < hi, so the hi bound never lies below lo
         L.it_a[it] = f_lo[k];
-        L.vis_off[it] = (it < nb) ? f_hi[k] - f_lo[k] : 0u;
+        L.vis_off[it] = (it < nb) ? n_hi[k] : 0u;
       }
     }
     __syncthreads();
@@ -477,42 +539,76 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
 // ---- item slices and the work-balanced tile plan
 //
 // Item slot s of the probe range belongs to probe p and prefix position j
-// with s = item_cum[l] + (p - first probe of size l) * maxprefix(l) + j
-// (sets are sorted by size, so the slots of one size are one run).  One
+// with s = cum[l] + (p - first[l]) * maxprefix(l) + j, where first[l] is the
+// first probe of size l in the range (sets are sorted by size, so the slots
+// of one size are one run).  The host makes, per size class l, the table
+// {cum, first, window lo, window hi, maxprefix} (the window of a size is an
+// id interval of the indexed range; a self-join further cuts it at p).  One
 // thread per slot finds its posting-list slice [a, a + n) of ids inside the
-// probe's window; probes whose window is empty get n = 0.
+// window; probes whose window is empty get n = 0.
+constexpr uint32_t SZ_CAP = 1024;  // size classes staged in LDS
 
-__global__ void k_item_slices(FgssArgs A, uint32_t nl, uint32_t total, uint32_t* __restrict__ slice_a,
-                              uint32_t* __restrict__ slice_n) {
-  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= total) return;
-  // size class: the last l with item_cum[l] <= s (and a non-empty run)
-  uint32_t lo_l = 0, hi_l = nl - 1;  // item_cum[nl - 1] = total > s
-  while (hi_l - lo_l > 1) {
-    const uint32_t mid = (lo_l + hi_l) >> 1;
-    if (A.item_cum[mid] <= s) lo_l = mid;
-    else hi_l = mid;
+struct SizeTab {
+  const uint32_t *cum, *first, *lo, *hi, *mp;
+};
+
+__device__ __forceinline__ SizeTab size_tab(const uint32_t* t, uint32_t nl) {
+  return SizeTab{t, t + nl, t + 2 * nl, t + 3 * nl, t + 4 * nl};
+}
+
+constexpr uint32_t IS_PER = 4;  // item slots per thread, searched in lockstep
+
+__global__ __launch_bounds__(256) void k_item_slices(FgssArgs A, const uint32_t* __restrict__ gtab, uint32_t nl,
+                                                     uint32_t total, uint32_t* __restrict__ slice_a,
+                                                     uint32_t* __restrict__ slice_n) {
+  __shared__ uint32_t stab[5 * SZ_CAP];
+  const uint32_t* tab = gtab;
+  if (nl <= SZ_CAP) {
+    for (uint32_t i = threadIdx.x; i < 5 * nl; i += blockDim.x) stab[i] = gtab[i];
+    __syncthreads();
+    tab = stab;
   }
-  const uint32_t l = lo_l;
-  const uint32_t mp = A.bounds.maxprefix[l];
-  const uint32_t q = s - A.item_cum[l];
-  const uint32_t p = max(A.probe_size_first[l], A.probe_first) + q / mp;
-  const uint32_t j = q % mp;
-  uint32_t lo, hi;
-  probe_window(A.bounds, l, p, A.size_first, A.size_first_len, A.idx_first, A.idx_last, A.self, lo, hi);
-  uint32_t f_lo[1] = {0}, n_lo[1] = {0}, f_hi[1] = {0}, n_hi[1] = {0};
-  const uint32_t k_lo[1] = {lo}, k_hi[1] = {hi};
-  if (lo < hi) {
-    const uint32_t token = A.probe.tokens[A.probe.starts[p] + j];
-    if (token < A.index.universe) {
-      const uint32_t lb = A.index.offsets[token];
-      f_lo[0] = f_hi[0] = lb;
-      n_lo[0] = n_hi[0] = A.index.offsets[token + 1] - lb;
+  const SizeTab T = size_tab(tab, nl);
+  // slots base + threadIdx.x + k * 256: coalesced stores
+  const uint32_t base = blockIdx.x * (256 * IS_PER) + threadIdx.x;
+  uint32_t f_lo[IS_PER], n_lo[IS_PER], f_hi[IS_PER], n_hi[IS_PER], k_lo[IS_PER], k_hi[IS_PER];
+#pragma unroll
+  for (uint32_t k = 0; k < IS_PER; ++k) {
+    f_lo[k] = f_hi[k] = n_lo[k] = n_hi[k] = k_lo[k] = k_hi[k] = 0;
+    const uint32_t s = base + k * 256;
+    if (s >= total) continue;
+    // size class: the last l with cum[l] <= s (its run is non-empty)
+    uint32_t lo_l = 0, hi_l = nl - 1;  // cum[nl - 1] = total > s
+    while (hi_l - lo_l > 1) {
+      const uint32_t mid = (lo_l + hi_l) >> 1;
+      if (T.cum[mid] <= s) lo_l = mid;
+      else hi_l = mid;
+    }
+    const uint32_t l = lo_l;
+    const uint32_t mp = T.mp[l];
+    const uint32_t q = s - T.cum[l];
+    const uint32_t p = T.first[l] + q / mp;
+    const uint32_t j = q % mp;
+    k_lo[k] = T.lo[l];
+    k_hi[k] = A.self ? min(T.hi[l], p) : T.hi[l];
+    if (k_lo[k] < k_hi[k]) {
+      const uint32_t token = A.probe.tokens[A.probe.starts[p] + j];
+      if (token < A.index.universe) {
+        const uint32_t lb = A.index.offsets[token];
+        f_lo[k] = f_hi[k] = lb;
+        n_lo[k] = n_hi[k] = A.index.offsets[token + 1] - lb;
+      }
     }
   }
-  lockstep_bounds<1>(A.index.set_ids, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
-  slice_a[s] = f_lo[0];
-  slice_n[s] = (lo < hi) ? f_hi[0] - f_lo[0] : 0u;
+  lockstep_bounds<IS_PER>(A.index.set_ids, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
+#pragma unroll
+  for (uint32_t k = 0; k < IS_PER; ++k) {
+    const uint32_t s = base + k * 256;
+    if (s < total) {
+      slice_a[s] = f_lo[k];
+      slice_n[s] = (k_lo[k] < k_hi[k]) ? n_hi[k] : 0u;
+    }
+  }
 }
 
 // Per probe a cost in units of COST_W-th of a tile: the largest of 4 (so a
@@ -525,16 +621,25 @@ __global__ void k_item_slices(FgssArgs A, uint32_t nl, uint32_t total, uint32_t*
 // tile is non-empty.
 constexpr uint32_t COST_W = 1024;
 
-__global__ void k_probe_cost(FgssArgs A, uint32_t* __restrict__ cost, uint4* __restrict__ rec,
-                             uint32_t* __restrict__ slot) {
+__global__ __launch_bounds__(256) void k_probe_cost(FgssArgs A, const uint32_t* __restrict__ gtab, uint32_t nl,
+                                                    uint32_t* __restrict__ cost, uint4* __restrict__ rec,
+                                                    uint32_t* __restrict__ slot) {
+  __shared__ uint32_t stab[5 * SZ_CAP];
+  const uint32_t* tab = gtab;
+  if (nl <= SZ_CAP) {
+    for (uint32_t k = threadIdx.x; k < 5 * nl; k += blockDim.x) stab[k] = gtab[k];
+    __syncthreads();
+    tab = stab;
+  }
+  const SizeTab T = size_tab(tab, nl);
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t p = A.probe_first + i;
   if (p > A.probe_last) return;
   const uint32_t l = A.probe.sizes[p];
-  const uint32_t mp = A.bounds.maxprefix[l];
-  const uint32_t base = A.item_cum[l] + (p - max(A.probe_size_first[l], A.probe_first)) * mp;
-  uint32_t lo, hi;
-  probe_window(A.bounds, l, p, A.size_first, A.size_first_len, A.idx_first, A.idx_last, A.self, lo, hi);
+  const uint32_t mp = T.mp[l];
+  const uint32_t base = T.cum[l] + (p - T.first[l]) * mp;
+  const uint32_t lo = T.lo[l];
+  const uint32_t hi = A.self ? min(T.hi[l], p) : T.hi[l];
   rec[i] = make_uint4(A.probe.starts[p], l | (mp << 16), lo, hi);
   slot[i] = base;
   uint32_t v = 0;
@@ -565,16 +670,16 @@ void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp) {
   bp.slot->ensure((size_t)n * 4);
   bp.tiles->ensure(((size_t)n + 1) * 4);
   bp.ntiles->ensure(4);
-  a.item_cum = bp.item_cum;
-  a.probe_size_first = bp.probe_size_first;
   a.slice_a = bp.slice_a->as<uint32_t>();
   a.slice_n = bp.slice_n->as<uint32_t>();
   if (bp.total_items)
-    hipLaunchKernelGGL(k_item_slices, dim3((unsigned)((bp.total_items + 255) / 256)), dim3(256), 0, st, a, bp.nl,
-                       (uint32_t)bp.total_items, bp.slice_a->as<uint32_t>(), bp.slice_n->as<uint32_t>());
+    hipLaunchKernelGGL(k_item_slices, dim3((unsigned)((bp.total_items + 256 * IS_PER - 1) / (256 * IS_PER))),
+                       dim3(256), 0, st, a,
+                       bp.size_tab, bp.nl, (uint32_t)bp.total_items, bp.slice_a->as<uint32_t>(),
+                       bp.slice_n->as<uint32_t>());
   uint32_t* c = bp.cost->as<uint32_t>();
-  hipLaunchKernelGGL(k_probe_cost, dim3((n + 255) / 256), dim3(256), 0, st, a, c, bp.rec->as<uint4>(),
-                     bp.slot->as<uint32_t>());
+  hipLaunchKernelGGL(k_probe_cost, dim3((n + 255) / 256), dim3(256), 0, st, a, bp.size_tab, bp.nl, c,
+                     bp.rec->as<uint4>(), bp.slot->as<uint32_t>());
   a.probe_rec = bp.rec->as<uint4>();
   a.probe_slot = bp.slot->as<uint32_t>();
   launch_exclusive_scan(st, c, c, nullptr, n, *bp.scan_scratch);

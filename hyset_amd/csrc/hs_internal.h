@@ -185,12 +185,9 @@ struct FgssArgs {
   uint32_t* tile_queue;
   const uint32_t* dev_ntiles;  // non-NULL: the tile count lives on the device
   // item-slice pass (plain joins): per item slot the posting-list slice of
-  // ids in the probe's window; item_cum[l] = first slot of size l in the
-  // probe range, probe_size_first = the probe collection's size table
+  // ids in the probe's window
   const uint32_t* slice_a;
   const uint32_t* slice_n;
-  const uint32_t* item_cum;
-  const uint32_t* probe_size_first;
   // per probe of the range (planning pass): {start, size | maxprefix << 16,
   // window lo, window hi} and its first item slot
   const uint4* probe_rec;
@@ -205,9 +202,11 @@ void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile, ui
 void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups);
 // Work buffers and the host-made item table of the item-slice pass.
 struct BalancedPlan {
-  const uint32_t* item_cum;          // device, nl entries
-  const uint32_t* probe_size_first;  // device
-  uint32_t nl;                       // size classes + 1
+  // device table of 5 * nl entries, per size class l of the probe range:
+  // first item slot, first probe, window lo, window hi (before the self-join
+  // cut at the probe id), maxprefix
+  const uint32_t* size_tab;
+  uint32_t nl;  // size classes + 1 (cum[nl - 1] = total_items)
   uint64_t total_items;
   DevBuf *slice_a, *slice_n, *cost, *tiles, *ntiles, *scan_scratch, *rec, *slot;
 };

@@ -173,12 +173,13 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 //    per-size table (sets are sorted by size, so all sets of one size are
 //    contiguous): off(i) = cum[l] + (i - first id of size l) * prefix_len[l].
 // 2. a stable LSD radix sort by token, 8 bits per pass, only as many passes
-//    as the universe needs; stability keeps every list in set-id order.
-//    Per pass: per-tile digit histograms (LDS atomics), a device scan of the
+//    as the universe needs; stability keeps every list in set-id order.  Per pass:
+//    per-tile digit histograms (LDS atomics), a device scan of the
 //    digit-major histogram table, and a scatter that ranks each element
 //    inside its tile with wave ballots (no global atomics anywhere).
-// 3. offsets[t] = first entry with token >= t (binary search per token), and
-//    the start and size of each entry's set.
+// 3. offsets[t] = first entry with token >= t, written at the token
+//    boundaries of the sorted keys, and the start and size of each entry's
+//    set.
 
 constexpr int RS_TPB = 256;
 constexpr int RS_ROUNDS = 16;
@@ -223,84 +224,115 @@ __global__ void k_index_emit_batch(DevColl c, const uint32_t* __restrict__ join_
   }
 }
 
+// Digit histogram of one tile: LDS atomics, then one column of the
+// digit-major table hist[digit * ntiles + tile].
+template <int kBits>
 __global__ __launch_bounds__(RS_TPB) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, uint32_t shift,
                                                        uint32_t* __restrict__ hist, uint32_t ntiles) {
-  __shared__ uint32_t h[256];
-  h[threadIdx.x] = 0;
+  constexpr uint32_t NB = 1u << kBits;
+  __shared__ uint32_t h[NB];
+  for (uint32_t d = threadIdx.x; d < NB; d += RS_TPB) h[d] = 0;
   __syncthreads();
   const uint32_t base = blockIdx.x * RS_TILE;
 #pragma unroll 4
   for (int r = 0; r < RS_ROUNDS; ++r) {
     const uint32_t idx = base + r * RS_TPB + threadIdx.x;
-    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & 255u], 1u);
+    if (idx < n) atomicAdd(&h[(keys[idx] >> shift) & (NB - 1)], 1u);
   }
   __syncthreads();
-  hist[threadIdx.x * ntiles + blockIdx.x] = h[threadIdx.x];
+  for (uint32_t d = threadIdx.x; d < NB; d += RS_TPB) hist[d * ntiles + blockIdx.x] = h[d];
 }
 
+// Stable scatter of one tile.  The tile's keys, values and positions are
+// loaded into registers up front (one memory latency per tile); each round
+// of 256 entries ranks lanes of equal digit with kBits ballots per wave and
+// orders the waves of the round through per-wave digit counts in LDS.
+template <int kBits>
 __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                           const uint32_t* __restrict__ vin,
                                                           const uint16_t* __restrict__ pin, uint32_t n,
                                                           uint32_t shift, const uint32_t* __restrict__ offs,
                                                           uint32_t ntiles, uint32_t* __restrict__ kout,
                                                           uint32_t* __restrict__ vout, uint16_t* __restrict__ pout) {
-  __shared__ uint32_t s_base[256];
-  __shared__ uint32_t s_run[256];
-  __shared__ uint32_t s_wcnt[RS_WAVES][256];
+  constexpr uint32_t NB = 1u << kBits;
+  __shared__ uint32_t s_base[NB];
+  __shared__ uint32_t s_wcnt[RS_WAVES][NB];
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / WAVE;
-  const uint32_t lane = lane_id();
-  s_base[tid] = offs[tid * ntiles + blockIdx.x];
-  s_run[tid] = 0;
-#pragma unroll
-  for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][tid] = 0;
-  __syncthreads();
   const uint32_t base = blockIdx.x * RS_TILE;
+  uint32_t key[RS_ROUNDS], val[RS_ROUNDS], ps[RS_ROUNDS];
+#pragma unroll
   for (int r = 0; r < RS_ROUNDS; ++r) {
     const uint32_t idx = base + r * RS_TPB + tid;
     const bool valid = idx < n;
-    const uint32_t key = valid ? kin[idx] : 0u;
-    const uint32_t d = (key >> shift) & 255u;
+    key[r] = valid ? kin[idx] : 0u;
+    val[r] = valid ? vin[idx] : 0u;
+    ps[r] = valid ? pin[idx] : 0u;
+  }
+  for (uint32_t d = tid; d < NB; d += RS_TPB) {
+    s_base[d] = offs[d * ntiles + blockIdx.x];
+#pragma unroll
+    for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][d] = 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ROUNDS; ++r) {
+    const uint32_t idx = base + r * RS_TPB + tid;
+    const bool valid = idx < n;
+    const uint32_t d = (key[r] >> shift) & (NB - 1);
     // lanes of this wave holding the same digit
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int b = 0; b < 8; ++b) {
+    for (int b = 0; b < kBits; ++b) {
       const bool bit = (d >> b) & 1u;
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
     const uint32_t rank = lanes_below(peers);
-    if (valid && rank == 0) s_wcnt[wave][d] = __popcll(peers);
+    const bool leader = valid && rank == 0;
+    if (leader) s_wcnt[wave][d] = __popcll(peers);
     __syncthreads();
+    uint32_t total = 0;
+    bool last = true;  // no later wave of the round holds digit d
     if (valid) {
-      uint32_t before = s_run[d];
-      for (uint32_t w = 0; w < wave; ++w) before += s_wcnt[w][d];
+      uint32_t before = 0;
+#pragma unroll
+      for (uint32_t w = 0; w < RS_WAVES; ++w) {
+        const uint32_t cnt = s_wcnt[w][d];
+        before += w < wave ? cnt : 0u;
+        total += cnt;
+        last = last && (w <= wave || cnt == 0);
+      }
       const uint32_t o = s_base[d] + before + rank;
-      kout[o] = key;
-      vout[o] = vin[idx];
-      pout[o] = pin[idx];
+      kout[o] = key[r];
+      vout[o] = val[r];
+      pout[o] = (uint16_t)ps[r];
     }
     __syncthreads();
-    uint32_t add = 0;
-#pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) {
-      add += s_wcnt[w][tid];
-      s_wcnt[w][tid] = 0;
+    // the leader of the last wave holding d advances its base; every leader
+    // clears its count
+    if (leader) {
+      s_wcnt[wave][d] = 0;
+      if (last) s_base[d] += total;
     }
-    s_run[tid] += add;
     __syncthreads();
   }
 }
 
-// offsets[t] = number of entries with token < t, for t in [0, universe];
-// and, for every entry, the CSR start and size of its set next to the
-// token position (one random read per entry here instead of per visit).
+// offsets[t] = number of entries with token < t, for t in [0, universe],
+// from the sorted keys: entry i writes the offsets of the tokens in
+// (keys[i-1], keys[i]] (every t is written once); and, for every entry, the
+// CSR start and size of its set next to the token position (one random
+// read per entry here instead of per visit).
 __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, uint32_t universe,
                                uint32_t* __restrict__ offsets, DevColl c, const uint32_t* __restrict__ set_ids,
                                const uint16_t* __restrict__ positions, uint32_t* __restrict__ sstart,
                                uint32_t* __restrict__ meta) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i <= universe) offsets[i] = lower_bound_u32(keys, 0, n, i);
+  if (i > n) return;
+  const uint32_t prev = i == 0 ? 0u : keys[i - 1] + 1;   // first token not yet written
+  const uint32_t cur = i == n ? universe : keys[i];      // tokens prev..cur start at entry i
+  for (uint32_t t = prev; t <= cur && t <= universe; ++t) offsets[t] = i;
   if (i < n) {
     const uint32_t sid = set_ids[i];
     sstart[i] = c.starts[sid];
@@ -324,32 +356,46 @@ static void prepare(IndexBuild& ib, uint32_t universe, uint64_t entries) {
   ib.pos.ensure(cap * sizeof(uint16_t));
 }
 
-static uint32_t radix_passes(uint32_t universe) {
-  uint32_t passes = 1;
-  while (passes < 4 && (uint64_t)(universe - 1) >> (8 * passes)) ++passes;
-  return passes;
+// Digit plan of the LSD sort for keys below `universe`: 8-bit digits, as
+// many passes as the key bits need.  (Wider digits halve the passes but
+// leave ~4 entries per digit and tile, and the scattered stores then cost
+// more than the saved pass: 10-bit digits measured slower on the 1M-set
+// index.)
+struct RadixPlan {
+  uint32_t passes, bits;
+};
+static RadixPlan radix_plan(uint32_t universe) {
+  uint32_t need = 1;
+  while (need < 32 && ((uint64_t)(universe - 1) >> need)) ++need;
+  return RadixPlan{(need + 7) / 8, 8u};
 }
 
 // Sorts the emitted entries (in buffer `cur`) by token and writes the index.
+template <int kBits>
+static void radix_pass(hipStream_t st, uint32_t n, uint32_t shift, uint32_t ntiles, IndexBuild& ib,
+                       uint32_t* const kbuf[2], uint32_t* const vbuf[2], uint16_t* const pbuf[2], int cur) {
+  constexpr uint32_t NB = 1u << kBits;
+  hipLaunchKernelGGL((k_radix_hist<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift,
+                     ib.hist.as<uint32_t>(), ntiles);
+  launch_exclusive_scan(st, ib.hist.as<uint32_t>(), ib.hist.as<uint32_t>(), nullptr, (size_t)NB * ntiles,
+                        ib.scan_scratch);
+  hipLaunchKernelGGL((k_radix_scatter<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], vbuf[cur], pbuf[cur], n,
+                     shift, ib.hist.as<uint32_t>(), ntiles, kbuf[cur ^ 1], vbuf[cur ^ 1], pbuf[cur ^ 1]);
+}
+
 static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32_t universe, IndexBuild& ib,
-                            int cur, uint32_t passes) {
+                            int cur, RadixPlan plan) {
   const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-  ib.hist.ensure((size_t)256 * ntiles * sizeof(uint32_t));
-  uint32_t* kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
-  uint32_t* vbuf[2] = {ib.vals.as<uint32_t>(), ib.set_ids.as<uint32_t>()};
-  uint16_t* pbuf[2] = {ib.pos.as<uint16_t>(), ib.positions.as<uint16_t>()};
-  for (uint32_t p = 0; p < passes; ++p) {
-    const uint32_t shift = 8 * p;
-    hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift, ib.hist.as<uint32_t>(),
-                       ntiles);
-    launch_exclusive_scan(st, ib.hist.as<uint32_t>(), ib.hist.as<uint32_t>(), nullptr, (size_t)256 * ntiles,
-                          ib.scan_scratch);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], vbuf[cur], pbuf[cur], n, shift,
-                       ib.hist.as<uint32_t>(), ntiles, kbuf[cur ^ 1], vbuf[cur ^ 1], pbuf[cur ^ 1]);
+  ib.hist.ensure(((size_t)ntiles << plan.bits) * sizeof(uint32_t));
+  uint32_t* const kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
+  uint32_t* const vbuf[2] = {ib.vals.as<uint32_t>(), ib.set_ids.as<uint32_t>()};
+  uint16_t* const pbuf[2] = {ib.pos.as<uint16_t>(), ib.positions.as<uint16_t>()};
+  for (uint32_t p = 0; p < plan.passes; ++p) {
+    const uint32_t shift = plan.bits * p;
+    radix_pass<8>(st, n, shift, ntiles, ib, kbuf, vbuf, pbuf, cur);
     cur ^= 1;
   }
-  const uint32_t span = std::max(universe + 1, n);
-  hipLaunchKernelGGL(k_index_finish, dim3((span + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
+  hipLaunchKernelGGL(k_index_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
                      ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(),
                      ib.sstart.as<uint32_t>(), ib.meta.as<uint32_t>());
   HS_CHECK_HIP(hipGetLastError());
@@ -365,15 +411,15 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
   prepare(ib, universe, entries);
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && first <= last && n > 0) {
-    const uint32_t passes = radix_passes(universe);
-    const int cur = emit_buffer(passes);
+    const RadixPlan plan = radix_plan(universe);
+    const int cur = emit_buffer(plan.passes);
     uint32_t* kb = cur ? ib.keys[1].as<uint32_t>() : ib.keys[0].as<uint32_t>();
     uint32_t* vb = cur ? ib.set_ids.as<uint32_t>() : ib.vals.as<uint32_t>();
     uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
     const uint32_t nsets = last - first + 1;
     hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
                        size_first, cum, kb, vb, pb);
-    sort_and_finish(st, c, n, universe, ib, cur, passes);
+    sort_and_finish(st, c, n, universe, ib, cur, plan);
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }
@@ -386,14 +432,14 @@ void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_fi
   prepare(ib, universe, entries);
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && n > 0) {
-    const uint32_t passes = radix_passes(universe);
-    const int cur = emit_buffer(passes);
+    const RadixPlan plan = radix_plan(universe);
+    const int cur = emit_buffer(plan.passes);
     uint32_t* kb = cur ? ib.keys[1].as<uint32_t>() : ib.keys[0].as<uint32_t>();
     uint32_t* vb = cur ? ib.set_ids.as<uint32_t>() : ib.vals.as<uint32_t>();
     uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
     hipLaunchKernelGGL(k_index_emit_batch, dim3((c.n + 255) / 256), dim3(256), 0, st, c, join_first, n_joins,
                        prefix_len, size_first, sf_base, cum, kb, vb, pb);
-    sort_and_finish(st, c, n, universe, ib, cur, passes);
+    sort_and_finish(st, c, n, universe, ib, cur, plan);
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }
