@@ -33,11 +33,39 @@ def log(msg: str) -> None:
 
 def fgss_bytes(st: dict, n_probes: int, n_probe_tokens: int) -> int:
     """Algorithmic HBM bytes of one k_fgss launch:
-    probe CSR (4 B per token + starts and sizes), per probe-prefix item the two
-    list offsets, 4 B per posting visited, per verified candidate its start and
-    size plus 4 B per token, 12 B per result triple."""
-    return (4 * n_probe_tokens + 8 * n_probes + 8 * st["probe_prefix_tokens"] + 4 * st["posting_visits"]
+    probe tokens (4 B each) and a 16 B record per probe (start, size and
+    prefix length, window), per probe-prefix item its posting-list slice
+    (8 B), 4 B per posting visited, per verified candidate its start and size
+    plus 4 B per token, 12 B per result triple."""
+    return (4 * n_probe_tokens + 16 * n_probes + 8 * st["probe_prefix_tokens"] + 4 * st["posting_visits"]
             + 8 * st["candidates"] + 4 * st["candidate_tokens"] + 12 * st["pairs"])
+
+
+def bench_long_rs(ctx, H) -> dict:
+    """The long-set R-S workload: R = S = 200,000 sets of 100-700 tokens over
+    a 50,000-token Zipf(0.8) universe, 20% of S edited copies of R sets,
+    cosine 0.8, fgss (index over S, probes R)."""
+    from hyset_amd import workloads as W
+    a = time.perf_counter()
+    R, S = W.long_rs()
+    log(f"long R-S: {R.n} x {S.n} sets, {R.tokens.shape[0] + S.tokens.shape[0]} tokens, "
+        f"generated in {time.perf_counter() - a:.1f}s")
+    dR, dS = ctx.upload(R), ctx.upload(S)
+    ctx.set_threshold(0.8, "cosine")
+    c = ctx.join(dS, dR, algo="fgss", count_only=True)
+    ms = []
+    for _ in range(2):
+        assert ctx.join(dS, dR, algo="fgss", count_only=True) == c
+        ms.append(ctx.stats()["total_ms"])
+    st = ctx.stats()
+    res = {"long_rs_pairs": int(c), "long_rs_fgss_ms": round(min(ms), 1),
+           "long_rs_posting_visits": int(st["posting_visits"]), "long_rs_candidates": int(st["candidates"])}
+    log(f"long R-S cosine 0.8: {c} pairs, fgss {res['long_rs_fgss_ms']} ms, {st['posting_visits']} visits, "
+        f"{st['candidates']} verified candidates")
+    dR.free()
+    dS.free()
+    ctx.set_threshold(0.8)
+    return res
 
 
 def bench_batched(ctx, H, split=True) -> dict:
@@ -267,6 +295,7 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-join measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-split", action="store_true", help="skip the learned/analytic split measurement")
+    ap.add_argument("--no-long", action="store_true", help="skip the long-set R-S measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
@@ -386,6 +415,8 @@ def main() -> int:
         assert c == pairs
         log(f"hybrid ({min(16, ncores)} host threads): {extra['hybrid_ms_wall']} ms")
 
+    if not args.quick and not args.no_long:
+        extra.update(bench_long_rs(ctx, H))
     if not args.quick and not args.no_batch:
         extra.update(bench_batched(ctx, H, split=not args.no_split))
     if not args.quick and not args.no_serve:

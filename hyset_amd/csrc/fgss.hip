@@ -364,18 +364,32 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     }
     __syncthreads();
 
-    // one visit per lane
+    // one visit per lane; the index entry of the lane's next visit is
+    // fetched before this one is verified, so its latency overlaps the
+    // verification's loads
+    uint32_t n_it = 0, n_sid = 0, n_sst = 0, n_meta = 0;
+    if (tid < total_vis) {
+      n_it = segment_of(L.vis_off, nb, tid);
+      const uint32_t e = L.it_a[n_it] + (tid - L.vis_off[n_it]);
+      n_sid = A.index.set_ids[e];
+      n_sst = A.index.sstart[e];
+      n_meta = A.index.meta[e];
+    }
     for (uint32_t vb = 0; vb < total_vis; vb += FG_TPB) {
       const uint32_t vi = vb + tid;
+      const uint32_t it = n_it, c_sid = n_sid, sst = n_sst, meta = n_meta;
+      const uint32_t vn = vi + FG_TPB;
+      if (vn < total_vis) {
+        n_it = segment_of(L.vis_off, nb, vn);
+        const uint32_t e = L.it_a[n_it] + (vn - L.vis_off[n_it]);
+        n_sid = A.index.set_ids[e];
+        n_sst = A.index.sstart[e];
+        n_meta = A.index.meta[e];
+      }
       bool pass = false;
       uint32_t rid = 0, sid = 0, ov = 0, checked_len = 0;
       if (vi < total_vis) {
-        const uint32_t it = segment_of(L.vis_off, nb, vi);
-        const uint32_t k = vi - L.vis_off[it];
-        const uint32_t e = L.it_a[it] + k;
-        sid = A.index.set_ids[e];
-        const uint32_t sst = A.index.sstart[e];
-        const uint32_t meta = A.index.meta[e];
+        sid = c_sid;
         const uint32_t kp = meta & 0xFFFFu;
         const uint32_t pj = L.it_pj[it];
         const uint32_t pl = pj & 0xFFu;
