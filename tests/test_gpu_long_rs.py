@@ -24,7 +24,10 @@ T = 0.8
 def _csr(c):
     indptr = np.concatenate([c.starts.astype(np.int64), [c.tokens.shape[0]]])
     data = np.ones(c.tokens.shape[0], dtype=np.float32)
-    return sp.csr_matrix((data, c.tokens.astype(np.int64), indptr), shape=(c.n, 50_000))
+    # token ids run from 1 to the universe size inclusive; c.universe is max id + 1
+    m = sp.csr_matrix((data, c.tokens.astype(np.int64), indptr), shape=(c.n, c.universe))
+    m.check_format(full_check=True)
+    return m
 
 
 def _bounds(l1, l2, cache={}):
