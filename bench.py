@@ -32,12 +32,15 @@ def log(msg: str) -> None:
 
 
 def fgss_bytes(st: dict, n_probes: int, n_probe_tokens: int) -> int:
-    """Algorithmic HBM bytes of one k_fgss launch:
-    probe tokens (4 B each) and a 16 B record per probe (start, size and
-    prefix length, window), per probe-prefix item its posting-list slice
-    (8 B), 4 B per posting visited, per verified candidate its start and size
-    plus 4 B per token, 12 B per result triple."""
-    return (4 * n_probe_tokens + 16 * n_probes + 8 * st["probe_prefix_tokens"] + 4 * st["posting_visits"]
+    """Algorithmic HBM bytes of one k_fgss launch: probe tokens (4 B each),
+    a 16 B record and the signature (W words of 8 B) per probe, per
+    probe-prefix item its posting-list slice (8 B), per posting visited its
+    set id and size/position word (8 B), per signature test the indexed
+    set's signature (8 W B), per verified candidate its start and size plus
+    4 B per token, 12 B per result triple."""
+    w = st["signature_bits"] // 64
+    return (4 * n_probe_tokens + (16 + 8 * w) * n_probes + 8 * st["probe_prefix_tokens"]
+            + 8 * st["posting_visits"] + 8 * w * st["signature_tests"]
             + 8 * st["candidates"] + 4 * st["candidate_tokens"] + 12 * st["pairs"])
 
 
@@ -54,14 +57,18 @@ def bench_long_rs(ctx, H) -> dict:
     ctx.set_threshold(0.8, "cosine")
     c = ctx.join(dS, dR, algo="fgss", count_only=True)
     ms = []
-    for _ in range(2):
+    for _ in range(3):
         assert ctx.join(dS, dR, algo="fgss", count_only=True) == c
         ms.append(ctx.stats()["total_ms"])
     st = ctx.stats()
-    res = {"long_rs_pairs": int(c), "long_rs_fgss_ms": round(min(ms), 1),
-           "long_rs_posting_visits": int(st["posting_visits"]), "long_rs_candidates": int(st["candidates"])}
+    res = {"long_rs_pairs": int(c), "long_rs_fgss_ms": round(min(ms), 2),
+           "long_rs_posting_visits": int(st["posting_visits"]), "long_rs_signature_tests": int(st["signature_tests"]),
+           "long_rs_signature_bits": int(st["signature_bits"]), "long_rs_candidates": int(st["candidates"])}
     log(f"long R-S cosine 0.8: {c} pairs, fgss {res['long_rs_fgss_ms']} ms, {st['posting_visits']} visits, "
-        f"{st['candidates']} verified candidates")
+        f"{st['signature_tests']} signature tests ({st['signature_bits']} bits), {st['candidates']} verified")
+    assert ctx.join(dS, dR, algo="fgss", tune=H.Tune(flags=H.TUNE_NO_SIGNATURE), count_only=True) == c
+    res["long_rs_fgss_no_signature_ms"] = round(ctx.stats()["total_ms"], 1)
+    log(f"long R-S without the signature test: {res['long_rs_fgss_no_signature_ms']} ms")
     dR.free()
     dS.free()
     ctx.set_threshold(0.8)
@@ -353,6 +360,7 @@ def main() -> int:
              "join_ms_with_d2h": round(ms_with_d2h, 4), "pairs": int(pairs),
              "candidates": int(st["candidates"]), "posting_visits": int(st["posting_visits"]),
              "index_entries": int(st["index_entries"]), "probe_prefix_tokens": int(st["probe_prefix_tokens"]),
+             "signature_tests": int(st["signature_tests"]), "signature_bits": int(st["signature_bits"]),
              "hbm_copy_kernel_gbps": round(copy_gbps, 1)}
 
     cpu = None
@@ -400,6 +408,19 @@ def main() -> int:
             ks.append(ctx.stats()["kernel_ms"])
         extra["k_fgss_static_tiles_ms"] = round(statistics.median(ks), 4)
         log(f"k_fgss with static tiles: {extra['k_fgss_static_tiles_ms']} ms (tile queue: {kern_ms:.4f} ms)")
+        # the same join without the signature test (every positional survivor merged)
+        ns_tune = H.Tune(flags=H.TUNE_NO_SIGNATURE)
+        ctx.join(d, algo="fgss", tune=ns_tune, count_only=True)
+        ks, ds = [], []
+        for _ in range(5):
+            assert ctx.join(d, algo="fgss", tune=ns_tune, count_only=True) == pairs
+            ks.append(ctx.stats()["kernel_ms"])
+            ds.append(ctx.stats()["total_ms"])
+        extra["k_fgss_no_signature_ms"] = round(statistics.median(ks), 4)
+        extra["join_ms_device_no_signature"] = round(statistics.median(ds), 4)
+        extra["candidates_no_signature"] = int(ctx.stats()["candidates"])
+        log(f"k_fgss without the signature test: {extra['k_fgss_no_signature_ms']} ms, "
+            f"{extra['candidates_no_signature']} candidates verified")
         for bits in (64, 128, 256):
             tune = H.Tune(bitmap_bits=bits)
             c = ctx.join(d, algo="bitmap", tune=tune, count_only=True)

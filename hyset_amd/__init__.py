@@ -53,11 +53,12 @@ class Tune(C.Structure):
     _fields_ = [("bitmap_bits", C.c_uint32), ("probes_per_tile", C.c_uint32),
                 ("verify_scenario", C.c_uint32), ("host_threads", C.c_uint32),
                 ("batch_candidates", C.c_uint64), ("flags", C.c_uint32), ("tile_tokens", C.c_uint32),
-                ("workgroups_per_cu", C.c_uint32)]
+                ("workgroups_per_cu", C.c_uint32), ("signature_bits", C.c_uint32)]
 
 
 TUNE_GRAPH = 1
 TUNE_STATIC_TILES = 2
+TUNE_NO_SIGNATURE = 4
 
 
 class Result(C.Structure):
@@ -70,7 +71,7 @@ class Stats(C.Structure):
                 ("total_ms", C.c_double), ("host_filter_ms", C.c_double), ("kernel_ms", C.c_double),
                 ("index_entries", C.c_uint64), ("posting_visits", C.c_uint64),
                 ("candidates", C.c_uint64), ("pairs", C.c_uint64), ("probe_prefix_tokens", C.c_uint64),
-                ("candidate_tokens", C.c_uint64)]
+                ("candidate_tokens", C.c_uint64), ("signature_tests", C.c_uint64), ("signature_bits", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -80,7 +81,7 @@ class ServerConfig(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("submitted", "batches", "deferred", "rejected", "max_in_flight",
-                                          "max_bytes_in_flight")]
+                                          "max_bytes_in_flight", "signature_bits")]
 
 
 def _sig(name, restype, *argtypes):

@@ -428,6 +428,14 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     hipStream_t st = ctx->stream;
     HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
 
+    // ---- parity signatures for the fgss signature test (timed with the index)
+    uint32_t sig_words = 0;
+    if (algo == HS_ALGO_FGSS && !(tune.flags & HS_TUNE_NO_SIGNATURE) && !range_empty(ir) && !range_empty(pr)) {
+      sig_words = hs::fgss_signature_words(indexed, tune.signature_bits);
+      hs::compute_signatures(st, indexed, sig_words);
+      if (!self) hs::compute_signatures(st, probe, sig_words);
+    }
+
     // ---- index over the indexed range (not needed by the bitmap technique)
     const uint32_t* prefix_tab = self ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
     if (algo != HS_ALGO_BITMAP && !range_empty(ir)) {
@@ -480,6 +488,9 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         a.idx_last = ir.last;
         a.self = self;
         a.out = dev_out(ctx);
+        a.sig_words = sig_words;
+        a.probe_sigs = sig_words ? probe->sigs.as<uint64_t>() : nullptr;
+        a.indexed_sigs = sig_words ? indexed->sigs.as<uint64_t>() : nullptr;
         const uint32_t ppt = tune.probes_per_tile;
         if (ppt == 0) {
           // the collection's tile plan, clipped to the probe range
@@ -534,16 +545,8 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
         HS_REQUIRE(bits == 64 || bits == 128 || bits == 256, HS_E_INVALID, "bitmap width must be 64, 128 or 256");
         const uint32_t words = bits / 64;
-        hs_coll* colls[2] = {indexed, probe};
-        for (hs_coll* c : colls) {
-          if (c->sig_words != words) {
-            c->sigs.ensure((size_t)std::max<uint32_t>(c->n, 1) * words * 8);
-            c->sig_words = 0;
-          }
-          hs::launch_signatures(st, c->dev(), words, c->sigs.as<uint64_t>());
-          c->sig_words = words;
-          if (self) break;
-        }
+        hs::compute_signatures(st, indexed, words);
+        if (!self) hs::compute_signatures(st, probe, words);
         hs::BitmapArgs b{};
         b.probe = probe->dev();
         b.indexed = indexed->dev();
@@ -593,7 +596,11 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       ctx->stats.candidate_tokens = cnt[hs::CNT_CAND_TOKENS];
       ctx->stats.probe_prefix_tokens = cnt[hs::CNT_ITEMS];
     }
-    if (algo == HS_ALGO_FGSS) ctx->stats.index_entries = cnt[hs::CNT_ENTRIES];
+    if (algo == HS_ALGO_FGSS) {
+      ctx->stats.index_entries = cnt[hs::CNT_ENTRIES];
+      ctx->stats.signature_tests = cnt[hs::CNT_SIG_TESTS];
+      ctx->stats.signature_bits = sig_words * 64;
+    }
   });
 }
 

@@ -77,10 +77,10 @@ uint64_t batch_entry_table(const hs_coll* c, const std::vector<uint32_t>& prefix
   return acc;
 }
 
-std::string graph_key(const hs_ctx* ctx, const hs_coll* c, uint32_t workgroups) {
+std::string graph_key(const hs_ctx* ctx, const hs_coll* c, uint32_t workgroups, uint32_t sig_words) {
   return std::to_string(ctx->kind) + "/" + std::to_string(ctx->mode) + "/" + std::to_string(ctx->t) + "/" +
          std::to_string(ctx->tables.L) + "/" + std::to_string(c->plan_probes) + "x" + std::to_string(c->plan_tokens) +
-         "/" + std::to_string(workgroups);
+         "/" + std::to_string(workgroups) + "/" + std::to_string(sig_words);
 }
 
 void drop_graph(hs_ctx* ctx) {
@@ -92,10 +92,12 @@ void drop_graph(hs_ctx* ctx) {
   ctx->graph_key.clear();
 }
 
-// Enqueues the whole batched join (entry table upload, index build, counter
-// reset, persistent launch) on the context stream.
-void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entries, uint32_t workgroups) {
+// Enqueues the whole batched join (entry table upload, signatures, index
+// build, counter reset, persistent launch) on the context stream.
+void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entries, uint32_t workgroups,
+                   uint32_t sig_words) {
   hipStream_t st = ctx->stream;
+  if (sig_words) compute_signatures(st, c, sig_words);
   const uint32_t ntab = (uint32_t)c->h_size_first.size();
   ctx->d_cum.ensure((size_t)ntab * 4);
   HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, h_cum, (size_t)ntab * 4, hipMemcpyHostToDevice, st));
@@ -116,6 +118,8 @@ void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entr
   a.batch = FgssBatch{c->tile_lo.as<uint32_t>(), c->tile_hi.as<uint32_t>(), c->tile_join.as<uint32_t>(),
                       c->join_first.as<uint32_t>(), c->sf_base.as<uint32_t>(), ctx->batch_queue.as<uint32_t>()};
   a.out = dev_out(ctx);
+  a.sig_words = sig_words;
+  a.probe_sigs = a.indexed_sigs = sig_words ? c->sigs.as<uint64_t>() : nullptr;
   launch_fgss_batch(st, a, (uint32_t)c->h_tile_lo.size(), workgroups);
 }
 
@@ -229,8 +233,11 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
     HS_REQUIRE(result, HS_E_INVALID, "NULL result");
     hs_tune tune{};
     if (tune_in) tune = *tune_in;
-    HS_REQUIRE((tune.flags & ~(uint32_t)HS_TUNE_GRAPH) == 0, HS_E_INVALID, "unknown tune flags");
+    HS_REQUIRE((tune.flags & ~(uint32_t)(HS_TUNE_GRAPH | HS_TUNE_NO_SIGNATURE)) == 0, HS_E_INVALID,
+               "unknown tune flags");
     ensure_bounds(ctx, batch->maxlen);
+    const uint32_t sig_words =
+        (tune.flags & HS_TUNE_NO_SIGNATURE) ? 0u : fgss_signature_words(batch, tune.signature_bits);
     // tile plan of the requested granularity
     const uint32_t probes = tune.probes_per_tile ? std::min<uint32_t>(tune.probes_per_tile, TILE_PROBES) : TILE_PROBES;
     const uint32_t tokens = tune.tile_tokens ? std::min<uint32_t>(tune.tile_tokens, TILE_TOKENS) : TILE_TOKENS;
@@ -241,7 +248,7 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
     ctx->stats = hs_stats{};
     hipStream_t st = ctx->stream;
     const bool use_graph = (tune.flags & HS_TUNE_GRAPH) != 0;
-    const std::string key = graph_key(ctx, batch, workgroups);
+    const std::string key = graph_key(ctx, batch, workgroups, sig_words);
     HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
     unsigned long long cnt[CNT_WORDS];
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -256,10 +263,10 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
           // an eager run first allocates every work buffer; the capture then
           // records a sequence that allocates nothing and replays as is
           drop_graph(ctx);
-          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups);
+          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words);
           HS_CHECK_HIP(hipStreamSynchronize(st));
           HS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups);
+          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words);
           HS_CHECK_HIP(hipStreamEndCapture(st, &ctx->graph));
           HS_CHECK_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
           ctx->graph_coll = batch;
@@ -267,7 +274,7 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
           ctx->graph_entries = entries;
           HS_CHECK_HIP(hipGraphLaunch(ctx->graph_exec, st));
         } else {
-          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups);
+          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words);
         }
         ctx->stats.index_entries = entries;
       }
@@ -298,6 +305,8 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
     ctx->stats.candidates = cnt[CNT_CANDS];
     ctx->stats.candidate_tokens = cnt[CNT_CAND_TOKENS];
     ctx->stats.probe_prefix_tokens = cnt[CNT_ITEMS];
+    ctx->stats.signature_tests = cnt[CNT_SIG_TESTS];
+    ctx->stats.signature_bits = sig_words * 64;
   });
 }
 
@@ -309,26 +318,39 @@ int hs_autotune_batch(hs_ctx* ctx, hs_coll* batch, int32_t reps, hs_tune* best, 
     const int r = reps > 0 ? reps : 2;
     double best_t = 1e300;
     hs_tune bt{};
+    auto trial = [&](const hs_tune& t) {
+      double sum = 0;
+      for (int k = 0; k <= r; ++k) {
+        hs_result res{};
+        res.count_only = 1;
+        int rc = hs_join_batch(ctx, batch, &t, &res);
+        if (rc != HS_OK) throw Error{rc, ctx->err};
+        if (k > 0) sum += ctx->stats.total_ms;  // the first run warms up
+      }
+      if (sum / r < best_t) {
+        best_t = sum / r;
+        bt = t;
+      }
+    };
+    // tile granularity and grid at the default signature width, then the width
     for (uint32_t probes : {64u, 128u, 256u}) {
       for (uint32_t wpc : {2u, 4u, 8u}) {
         hs_tune t{};
         t.probes_per_tile = probes;
         t.tile_tokens = probes * 16;
         t.workgroups_per_cu = wpc;
-        double sum = 0;
-        for (int k = 0; k <= r; ++k) {
-          hs_result res{};
-          res.count_only = 1;
-          int rc = hs_join_batch(ctx, batch, &t, &res);
-          if (rc != HS_OK) throw Error{rc, ctx->err};
-          if (k > 0) sum += ctx->stats.total_ms;  // the first run warms up
-        }
-        if (sum / r < best_t) {
-          best_t = sum / r;
-          bt = t;
-        }
+        trial(t);
       }
     }
+    const hs_tune base = bt;
+    for (uint32_t bits : {64u, 128u, 256u}) {
+      hs_tune t = base;
+      t.signature_bits = bits;
+      trial(t);
+    }
+    hs_tune off = base;
+    off.flags |= HS_TUNE_NO_SIGNATURE;
+    trial(off);
     *best = bt;
     if (best_ms) *best_ms = best_t;
   });

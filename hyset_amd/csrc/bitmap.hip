@@ -45,14 +45,39 @@ __global__ void k_signatures(DevColl c, uint64_t* __restrict__ sigs) {
   for (int k = 0; k < W; ++k) sigs[(size_t)i * W + k] = w[k];
 }
 
+// Wide signatures (512, 1024 bits) of long sets: one wave per set reads the
+// set's tokens coalesced and toggles bits of an LDS copy of the signature
+// with LDS atomic XOR (32-bit words; word k of the u64 layout is dwords 2k,
+// 2k+1), then 2W lanes store it.
+template <int W>
+__global__ __launch_bounds__(256) void k_signatures_wave(DevColl c, uint64_t* __restrict__ sigs) {
+  __shared__ uint32_t s[256 / WAVE][2 * W];
+  const uint32_t wave = threadIdx.x / WAVE, lane = lane_id();
+  const uint32_t i = blockIdx.x * (256 / WAVE) + wave;
+  if (lane < 2 * W) s[wave][lane] = 0;
+  __syncthreads();
+  if (i < c.n) {
+    const uint32_t* t = c.tokens + c.starts[i];
+    const uint32_t l = c.sizes[i];
+    for (uint32_t k = lane; k < l; k += WAVE) {
+      const uint32_t bit = t[k] & (64u * W - 1u);
+      atomicXor(&s[wave][bit >> 5], 1u << (bit & 31u));
+    }
+  }
+  __syncthreads();
+  if (i < c.n && lane < 2 * W) reinterpret_cast<uint32_t*>(sigs)[(size_t)i * 2 * W + lane] = s[wave][lane];
+}
+
 void launch_signatures(hipStream_t st, const DevColl& c, uint32_t words, uint64_t* sigs) {
   if (c.n == 0) return;
-  dim3 g((c.n + 255) / 256), b(256);
+  dim3 g((c.n + 255) / 256), b(256), gw((c.n + 3) / 4);
   switch (words) {
     case 1: hipLaunchKernelGGL(k_signatures<1>, g, b, 0, st, c, sigs); break;
     case 2: hipLaunchKernelGGL(k_signatures<2>, g, b, 0, st, c, sigs); break;
     case 4: hipLaunchKernelGGL(k_signatures<4>, g, b, 0, st, c, sigs); break;
-    default: throw Error{HS_E_INVALID, "bitmap width must be 64, 128 or 256 bits"};
+    case 8: hipLaunchKernelGGL(k_signatures_wave<8>, gw, b, 0, st, c, sigs); break;
+    case 16: hipLaunchKernelGGL(k_signatures_wave<16>, gw, b, 0, st, c, sigs); break;
+    default: throw Error{HS_E_INVALID, "signature width must be 64, 128, 256, 512 or 1024 bits"};
   }
   HS_CHECK_HIP(hipGetLastError());
 }

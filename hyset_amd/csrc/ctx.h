@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -150,6 +151,23 @@ inline float elapsed(hipEvent_t a, hipEvent_t b) {
     return 0.f;
   }
   return ms;
+}
+
+// Parity signatures of every set of c (words 64-bit words each) into
+// c->sigs, on the stream.
+inline void compute_signatures(hipStream_t st, hs_coll* c, uint32_t words) {
+  c->sigs.ensure((size_t)std::max<uint32_t>(c->n, 1) * words * 8);
+  c->sig_words = words;
+  launch_signatures(st, c->dev(), words, c->sigs.as<uint64_t>());
+}
+
+// Signature words of the fgss signature test: `bits` when given (64, 128,
+// 256, 512 or 1024), else chosen from the mean size of the indexed sets.
+inline uint32_t fgss_signature_words(const hs_coll* indexed, uint32_t bits) {
+  if (bits == 0) return auto_signature_words(indexed->n ? (double)indexed->n_tokens / indexed->n : 0.0);
+  HS_REQUIRE(bits == 64 || bits == 128 || bits == 256 || bits == 512 || bits == 1024, HS_E_INVALID,
+             "signature width must be 64, 128, 256, 512 or 1024 bits");
+  return bits / 64;
 }
 
 // Bound tables covering set sizes up to L, uploaded to the device.

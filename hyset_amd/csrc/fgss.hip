@@ -23,7 +23,16 @@
 //     the reference's dense count matrix by a merge that starts anyway;
 //   * PPJoin's positional filter (host_algorithms.hpp:196-199) runs first,
 //     from the token positions stored in the index;
-//   * the merge is driven by s, whose tokens arrive 16 at a time in
+//   * then the bitmap filter of the reference's bitmap technique
+//     (device_algorithms.hpp:76-118): |r n s| <= (|r| + |s| -
+//     popcount(sig_r ^ sig_s)) / 2 for XOR-parity signatures of 64-1024
+//     bits, so a visit with |r| + |s| - popcount < 2 minoverlap cannot be a
+//     result.  Both filters only drop visits that cannot pass, so the result
+//     set is unchanged;
+//   * visits that pass both are pushed onto an LDS stack (one LDS atomic per
+//     wave) and verified 256 at a time, one per lane, so the merges run with
+//     full waves instead of in the few lanes whose visit passed;
+//   * the merge is driven by s, whose tokens arrive 12 at a time in
 //     registers, against r in LDS; it keeps the early-termination bound of
 //     verify_pair (verification.hpp:9-31) for failing pairs and runs to the
 //     end for passing ones, which yields the exact overlap;
@@ -40,9 +49,12 @@ namespace hs {
 constexpr int FG_TPB = 256;                   // threads per workgroup, also max probes per tile
 constexpr uint32_t FG_TOKCAP = TILE_TOKENS;    // staged probe tokens (16 KiB)
 static_assert(TILE_PROBES == FG_TPB, "one probe per thread in phase A");
-constexpr uint32_t FG_ITEMCAP = 1024; // items per batch
+constexpr uint32_t FG_ITEMCAP = 768;  // items per batch
 constexpr uint32_t NO_LDS = 0xFFFFFFFFu;
 constexpr uint32_t FG_MOCAP = 1024;   // minoverlap entries kept in LDS (|r| + |s| < 1024)
+// Candidate stack: at most 255 entries wait between rounds and a round of
+// 256 visits adds at most 256.
+constexpr uint32_t FG_QCAP = 2 * FG_TPB;
 
 constexpr uint32_t SREG = 12;  // tokens of s held in registers per refill
 
@@ -98,12 +110,17 @@ struct FgssLds {
   uint32_t it_pj[FG_ITEMCAP];
   uint32_t vis_off[FG_ITEMCAP + 1];
   uint32_t tmp[FG_TPB / WAVE + 1];
-  unsigned long long stats[4];
+  unsigned long long stats[5];
   uint32_t next;
   uint32_t nxt[2];                  // claimed tiles, current and next (persistent single join)
   uint32_t slot0;                   // first item slot of the tile (item-slice pass)
   uint32_t mo_lds;                  // 1: mo[] holds the minoverlap table (keyed by |r| + |s|)
-  uint32_t mo[FG_MOCAP];
+  uint16_t mo[FG_MOCAP];
+  // candidate stack: probe slot | position in r << 8, indexed set id; the
+  // entries of round k are counted in qadd[k % 3] (cleared two rounds ahead)
+  uint32_t q_pj[FG_QCAP];
+  uint32_t q_sid[FG_QCAP];
+  uint32_t qadd[3];
   uint8_t ptidx[FG_TPB];            // per-probe threshold table (serving)
   DevBounds bounds[MAX_THRESHOLDS];  // the tables of a multi-threshold launch
 };
@@ -235,11 +252,50 @@ __device__ __forceinline__ void probe_window(const DevBounds& B, uint32_t l, uin
   if (self) hi = min(hi, p);
 }
 
-// kSlices: item slices come from the item-slice pass; kMulti: per-probe
-// threshold tables (serving).
-template <bool kSlices, bool kMulti>
-__device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T, FgssLds& L, uint32_t& my_cands,
-                                          uint32_t& my_cand_tokens) {
+// Work counters of one workgroup's lanes.
+struct FgssCounts {
+  uint32_t cands = 0, cand_tokens = 0, sig_tests = 0;
+};
+
+template <bool kMulti>
+__device__ __forceinline__ uint32_t tile_minoverlap(const FgssArgs& A, const FgssLds& L, uint32_t pl, uint32_t lr,
+                                                    uint32_t ls) {
+  if (!kMulti && L.mo_lds) return min((uint32_t)L.mo[lr + ls], min(lr, ls));
+  return dev_minoverlap(kMulti ? L.bounds[L.ptidx[pl]] : A.bounds, lr, ls);
+}
+
+// Verifies one candidate of the stack (valid lanes) and appends the passing
+// pairs; called by every lane of the wave.  pj = probe slot | j << 8, where
+// r[j] is the token through which the candidate was reached.
+template <bool kMulti>
+__device__ __forceinline__ void verify_candidate(const FgssArgs& A, const FgssLds& L, uint32_t p0, bool valid,
+                                                 uint32_t pj, uint32_t sid, FgssCounts& C) {
+  bool pass = false;
+  uint32_t ov = 0, rid = 0;
+  if (valid) {
+    const uint32_t pl = pj & 0xFFu;
+    const uint32_t j = pj >> 8;
+    rid = p0 + pl;
+    const uint32_t sst = A.indexed.starts[sid];
+    const uint32_t ls = A.indexed.sizes[sid];
+    const uint32_t lr = L.psize[pl];
+    const uint32_t mo = tile_minoverlap<kMulti>(A, L, pl, lr, ls);
+    const uint32_t* sp = A.indexed.tokens + sst;
+    const uint32_t off = L.plds[pl];
+    if (off != NO_LDS) pass = verify_visit<true>(L.tok + off, lr, sp, ls, j, mo, &ov);
+    else pass = verify_visit<false>(A.probe.tokens + L.pstart[pl], lr, sp, ls, j, mo, &ov);
+    ++C.cands;
+    C.cand_tokens += ls;
+  }
+  emit_pairs(pass, rid, sid, ov, A.out.r, A.out.s, A.out.ov, &A.out.counters[CNT_PAIRS], A.out.capacity);
+}
+
+// kW: signature words (0: no signature test); kSlices: item slices come
+// from the item-slice pass; kMulti: per-probe threshold tables (serving).
+// rnd counts the workgroup's visit rounds (it selects the stack counter).
+template <int kW, bool kSlices, bool kMulti>
+__device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T, FgssLds& L, FgssCounts& C,
+                                          uint32_t& rnd) {
   const uint32_t tid = threadIdx.x;
   const uint32_t p0 = T.p0;
   const uint32_t np = T.p1 - T.p0;
@@ -290,6 +346,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
   __syncthreads();
 
   // ---- phase B: batches of items
+  uint32_t qn = 0;  // entries on the candidate stack (the same in every lane)
   for (uint32_t ib = 0; ib < total_items; ib += FG_ITEMCAP) {
     const uint32_t nb = min(FG_ITEMCAP, total_items - ib);
     // slice of the posting list for every item of the batch: read from the
@@ -365,56 +422,82 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     __syncthreads();
 
     // one visit per lane; the index entry of the lane's next visit is
-    // fetched before this one is verified, so its latency overlaps the
-    // verification's loads
-    uint32_t n_it = 0, n_sid = 0, n_sst = 0, n_meta = 0;
+    // fetched before this one is filtered, so its latency overlaps the
+    // filter's loads.  Visits passing both filters go onto the LDS stack;
+    // whenever it holds 256 they are verified, one per lane.  The last batch
+    // of the tile ends with one round without visits that verifies the rest
+    // of the stack (one verification site keeps the register budget).
+    const bool last_batch = ib + FG_ITEMCAP >= total_items;
+    const uint32_t vis_end = total_vis + (last_batch ? FG_TPB : 0u);
+    uint32_t n_it = 0, n_sid = 0, n_meta = 0;
     if (tid < total_vis) {
       n_it = segment_of(L.vis_off, nb, tid);
       const uint32_t e = L.it_a[n_it] + (tid - L.vis_off[n_it]);
       n_sid = A.index.set_ids[e];
-      n_sst = A.index.sstart[e];
       n_meta = A.index.meta[e];
     }
-    for (uint32_t vb = 0; vb < total_vis; vb += FG_TPB) {
+    for (uint32_t vb = 0; vb < vis_end; vb += FG_TPB, ++rnd) {
       const uint32_t vi = vb + tid;
-      const uint32_t it = n_it, c_sid = n_sid, sst = n_sst, meta = n_meta;
+      const uint32_t it = n_it, sid = n_sid, meta = n_meta;
       const uint32_t vn = vi + FG_TPB;
       if (vn < total_vis) {
         n_it = segment_of(L.vis_off, nb, vn);
         const uint32_t e = L.it_a[n_it] + (vn - L.vis_off[n_it]);
         n_sid = A.index.set_ids[e];
-        n_sst = A.index.sstart[e];
         n_meta = A.index.meta[e];
       }
-      bool pass = false;
-      uint32_t rid = 0, sid = 0, ov = 0, checked_len = 0;
+      if (tid == 0) L.qadd[(rnd + 1) % 3] = 0;  // its last reader passed the previous barrier
+      bool keep = false;
+      uint32_t pj = 0;
       if (vi < total_vis) {
-        sid = c_sid;
         const uint32_t kp = meta & 0xFFFFu;
-        const uint32_t pj = L.it_pj[it];
+        pj = L.it_pj[it];
         const uint32_t pl = pj & 0xFFu;
         const uint32_t j = pj >> 8;
-        rid = p0 + pl;
         const uint32_t lr = L.psize[pl];
         const uint32_t ls = (meta >> 16) == 0xFFFFu ? A.indexed.sizes[sid] : (meta >> 16);
-        uint32_t mo;
-        if (!kMulti && L.mo_lds) mo = min(L.mo[lr + ls], min(lr, ls));
-        else mo = dev_minoverlap(kMulti ? L.bounds[L.ptidx[pl]] : A.bounds, lr, ls);
+        const uint32_t mo = tile_minoverlap<kMulti>(A, L, pl, lr, ls);
         // Positional filter (helpers.hpp:46-50) at this visit's token: if it is
         // the first common token, |r n s| <= 1 + min(|r|-1-j, |s|-1-kp).  Only
         // the first-common visit can emit the pair, so a visit failing the
         // bound is dropped without reading s.
-        if (kp == 0xFFFFu || 1u + min(lr - 1u - j, ls - 1u - kp) >= mo) {
-          checked_len = ls;  // a verified candidate
-          const uint32_t* sp = A.indexed.tokens + sst;
-          const uint32_t off = L.plds[pl];
-          if (off != NO_LDS) pass = verify_visit<true>(L.tok + off, lr, sp, ls, j, mo, &ov);
-          else pass = verify_visit<false>(A.probe.tokens + L.pstart[pl], lr, sp, ls, j, mo, &ov);
+        keep = kp == 0xFFFFu || 1u + min(lr - 1u - j, ls - 1u - kp) >= mo;
+        if constexpr (kW > 0) {
+          if (keep) {
+            ++C.sig_tests;
+            const uint64_t* sr = A.probe_sigs + (size_t)(T.p0 + pl) * kW;
+            const uint64_t* ss = A.indexed_sigs + (size_t)sid * kW;
+            uint32_t pc = 0;
+#pragma unroll 4
+            for (int q = 0; q < kW; ++q) pc += (uint32_t)__popcll(sr[q] ^ ss[q]);
+            keep = lr + ls >= 2u * mo + pc;
+          }
         }
       }
-      my_cands += checked_len != 0;
-      my_cand_tokens += checked_len;
-      emit_pairs(pass, rid, sid, ov, A.out.r, A.out.s, A.out.ov, &A.out.counters[CNT_PAIRS], A.out.capacity);
+      const uint64_t m = __ballot(keep);
+      if (m) {
+        const uint32_t leader = __ffsll((unsigned long long)m) - 1;
+        uint32_t b = 0;
+        if (lane_id() == leader) b = atomicAdd(&L.qadd[rnd % 3], (uint32_t)__popcll(m));
+        b = __shfl(b, leader);
+        if (keep) {
+          const uint32_t slot = qn + b + lanes_below(m);
+          L.q_pj[slot] = pj;
+          L.q_sid[slot] = sid;
+        }
+      }
+      __syncthreads();
+      qn += L.qadd[rnd % 3];
+      // 256 entries, or in the final round (no visits, fewer than 256 left) the rest
+      const uint32_t take = qn >= FG_TPB ? FG_TPB : (vb + FG_TPB >= vis_end && last_batch ? qn : 0u);
+      if (take) {
+        const bool valid = tid < take;
+        const uint32_t e = qn - take + tid;
+        const uint32_t qpj = valid ? L.q_pj[e] : 0u, qsid = valid ? L.q_sid[e] : 0u;
+        __syncthreads();  // every lane holds its entry before the next round reuses the slots
+        qn -= take;
+        verify_candidate<kMulti>(A, L, p0, valid, qpj, qsid, C);
+      }
     }
     __syncthreads();
   }
@@ -470,7 +553,7 @@ __device__ __forceinline__ void plain_tile_range(const FgssArgs& A, uint32_t til
 // one join, a persistent grid over the tiles of a batch of joins.
 enum FgssMode { FG_TILES = 0, FG_QUEUE = 1, FG_BATCH = 2 };
 
-template <int kMode, bool kSlices, bool kMulti>
+template <int kMode, bool kSlices, bool kMulti, int kW>
 __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_fgss(FgssArgs A, uint32_t ppt,
                                                                                         uint32_t ntiles) {
   __shared__ FgssLds L;
@@ -479,19 +562,21 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (tid < MAX_THRESHOLDS) L.bounds[tid] = A.probe_bounds[tid];
     __syncthreads();
   }
-  uint32_t my_cands = 0, my_cand_tokens = 0;
+  FgssCounts C;
+  uint32_t rnd = 0;
+  if (tid < 5) L.stats[tid] = 0;
+  if (tid < 3) L.qadd[tid] = 0;
   if constexpr (!kMulti) {
     // the 1-D minoverlap table (Jaccard, Dice) when it fits: one LDS read per visit
     const uint32_t nmo = 2 * A.bounds.L + 1;
     const bool fits = A.bounds.kind != HS_COSINE && nmo <= FG_MOCAP;
     if (fits)
-      for (uint32_t i = tid; i < nmo; i += FG_TPB) L.mo[i] = A.bounds.mo[i];
+      for (uint32_t i = tid; i < nmo; i += FG_TPB) L.mo[i] = (uint16_t)A.bounds.mo[i];
     if (tid == 0) L.mo_lds = fits;
-    __syncthreads();
   }
+  __syncthreads();
   if constexpr (kMode == FG_BATCH) {
     // persistent: one launch for every tile of every join of the batch
-    if (tid < 4) L.stats[tid] = 0;
     for (;;) {
       __syncthreads();
       if (tid == 0) L.next = next_batch_tile(A.batch, ntiles, blockIdx.x % 8);
@@ -501,13 +586,12 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
       const uint32_t j = A.batch.tile_join[k];
       TileScope T{A.batch.tile_lo[k], A.batch.tile_hi[k], A.batch.join_first[j], A.batch.join_first[j + 1] - 1,
                   A.size_first + A.batch.sf_base[j], A.batch.sf_base[j + 1] - A.batch.sf_base[j]};
-      fgss_tile<kSlices, kMulti>(A, T, L, my_cands, my_cand_tokens);
+      fgss_tile<kW, kSlices, kMulti>(A, T, L, C, rnd);
     }
   } else if constexpr (kMode == FG_QUEUE) {
     // persistent over the tiles of one join (tile count on the device when
     // the plan was made there)
     if (A.dev_ntiles) ntiles = *A.dev_ntiles;
-    if (tid < 4) L.stats[tid] = 0;
     // the claim of the next tile is issued when a tile starts; its latency
     // overlaps the tile's own loads
     if (tid == 0) L.nxt[0] = next_plain_tile(A.tile_queue, ntiles, blockIdx.x % 8);
@@ -520,26 +604,29 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
       uint32_t p0, p1;
       plain_tile_range(A, tile, ppt, p0, p1);
       TileScope T{p0, p1, A.idx_first, A.idx_last, A.size_first, A.size_first_len};
-      fgss_tile<kSlices, kMulti>(A, T, L, my_cands, my_cand_tokens);
+      fgss_tile<kW, kSlices, kMulti>(A, T, L, C, rnd);
       if (tid == 0) L.nxt[(it + 1) & 1] = claimed;
       __syncthreads();
     }
   } else {
     const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
     if (tile == 0xFFFFFFFFu) return;
-    if (tid < 4) L.stats[tid] = 0;
     uint32_t p0, p1;
     plain_tile_range(A, tile, ppt, p0, p1);
     TileScope T{p0, p1, A.idx_first, A.idx_last, A.size_first, A.size_first_len};
-    fgss_tile<kSlices, kMulti>(A, T, L, my_cands, my_cand_tokens);
+    fgss_tile<kW, kSlices, kMulti>(A, T, L, C, rnd);
   }
 
   // ---- statistics: one global atomic per workgroup and counter
-  my_cands = wave_sum(my_cands);
-  my_cand_tokens = wave_sum(my_cand_tokens);
-  if (lane_id() == 0 && my_cands) {
-    atomicAdd(&L.stats[1], (unsigned long long)my_cands);
-    atomicAdd(&L.stats[2], (unsigned long long)my_cand_tokens);
+  const uint32_t cands = wave_sum(C.cands);
+  const uint32_t cand_tokens = wave_sum(C.cand_tokens);
+  const uint32_t sig_tests = wave_sum(C.sig_tests);
+  if (lane_id() == 0) {
+    if (cands) {
+      atomicAdd(&L.stats[1], (unsigned long long)cands);
+      atomicAdd(&L.stats[2], (unsigned long long)cand_tokens);
+    }
+    if (sig_tests) atomicAdd(&L.stats[4], (unsigned long long)sig_tests);
   }
   __syncthreads();
   if (tid == 0) {
@@ -547,6 +634,7 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
     if (L.stats[1]) atomicAdd(&A.out.counters[CNT_CANDS], L.stats[1]);
     if (L.stats[2]) atomicAdd(&A.out.counters[CNT_CAND_TOKENS], L.stats[2]);
     if (L.stats[3]) atomicAdd(&A.out.counters[CNT_ITEMS], L.stats[3]);
+    if (L.stats[4]) atomicAdd(&A.out.counters[CNT_SIG_TESTS], L.stats[4]);
   }
 }
 
@@ -706,8 +794,29 @@ void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp) {
   a.dev_ntiles = bp.ntiles->as<uint32_t>();
 }
 
+// One k_fgss instance per signature width.
+template <int kMode, bool kSlices, bool kMulti>
+static void launch_width(hipStream_t st, const FgssArgs& a, dim3 grid, uint32_t ppt, uint32_t ntiles) {
+  const dim3 block(FG_TPB);
+  switch (a.sig_words) {
+    case 0: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 0>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 1: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 1>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 2: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 2>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 4: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 4>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 8: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 8>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 16: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 16>), grid, block, 0, st, a, ppt, ntiles); break;
+    default: throw Error{HS_E_INVALID, "signature width must be 0, 64, 128, 256, 512 or 1024 bits"};
+  }
+  HS_CHECK_HIP(hipGetLastError());
+}
+
+static void check_sigs(const FgssArgs& a) {
+  HS_REQUIRE(a.sig_words == 0 || (a.probe_sigs && a.indexed_sigs), HS_E_INVALID, "signature arrays missing");
+}
+
 void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persistent_wgs) {
   if (a.probe.n == 0 || a.probe_first > a.probe_last) return;
+  check_sigs(a);
   if (ppt == 0 || ppt > (uint32_t)FG_TPB) ppt = FG_TPB;
   const uint32_t nprobes = a.probe_last - a.probe_first + 1;
   const uint32_t ntiles = a.tile_starts ? a.n_plan_tiles : (nprobes + ppt - 1) / ppt;
@@ -718,21 +827,26 @@ void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persi
     if (persistent_wgs) grid = std::min(grid, (persistent_wgs + 7) / 8 * 8);
   }
   if (a.tile_queue) {
-    if (a.slice_a) hipLaunchKernelGGL((k_fgss<FG_QUEUE, true, false>), dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
-    else hipLaunchKernelGGL((k_fgss<FG_QUEUE, false, false>), dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+    if (a.slice_a) launch_width<FG_QUEUE, true, false>(st, a, dim3(grid), ppt, ntiles);
+    else launch_width<FG_QUEUE, false, false>(st, a, dim3(grid), ppt, ntiles);
   } else if (a.probe_tidx) {
-    hipLaunchKernelGGL((k_fgss<FG_TILES, false, true>), dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+    launch_width<FG_TILES, false, true>(st, a, dim3(grid), ppt, ntiles);
   } else {
-    hipLaunchKernelGGL((k_fgss<FG_TILES, false, false>), dim3(grid), dim3(FG_TPB), 0, st, a, ppt, ntiles);
+    launch_width<FG_TILES, false, false>(st, a, dim3(grid), ppt, ntiles);
   }
-  HS_CHECK_HIP(hipGetLastError());
 }
 
 void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups) {
   if (ntiles == 0) return;
+  check_sigs(a);
   launch_fill_u32(st, a.batch.queue, 8, 0);
-  hipLaunchKernelGGL((k_fgss<FG_BATCH, false, false>), dim3(workgroups), dim3(FG_TPB), 0, st, a, 0u, ntiles);
-  HS_CHECK_HIP(hipGetLastError());
+  launch_width<FG_BATCH, false, false>(st, a, dim3(workgroups), 0u, ntiles);
+}
+
+uint32_t auto_signature_words(double mean) {
+  uint32_t w = 1;
+  while (w < 16 && 64.0 * w < 2.0 * mean) w <<= 1;
+  return w;
 }
 
 }  // namespace hs

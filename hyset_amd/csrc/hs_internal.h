@@ -110,8 +110,12 @@ struct DevIndex {
 
 // Output cursor and work counters (64-bit): pairs found, posting visits
 // (bitmap: pair tests), distinct candidates verified (bitmap: filter
-// survivors), tokens of the verified candidate sets, probe-prefix items.
-enum { CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_CAND_TOKENS = 3, CNT_ITEMS = 4, CNT_ENTRIES = 5, CNT_WORDS = 8 };
+// survivors), tokens of the verified candidate sets, probe-prefix items,
+// index entries, and (fgss) visits that reached the signature test.
+enum {
+  CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_CAND_TOKENS = 3, CNT_ITEMS = 4, CNT_ENTRIES = 5,
+  CNT_SIG_TESTS = 6, CNT_WORDS = 8
+};
 
 struct DevOut {
   uint32_t* r;
@@ -192,6 +196,11 @@ struct FgssArgs {
   // window lo, window hi} and its first item slot
   const uint4* probe_rec;
   const uint32_t* probe_slot;
+  // parity signatures, sig_words 64-bit words per set (probe and indexed
+  // collection, indexed by set id); sig_words == 0 skips the signature test
+  const uint64_t* probe_sigs;
+  const uint64_t* indexed_sigs;
+  uint32_t sig_words;
   DevOut out;
 };
 constexpr uint32_t MAX_THRESHOLDS = 8;
@@ -229,7 +238,13 @@ struct BitmapArgs {
   int self;
   DevOut out;
 };
+// words: 1, 2, 4, 8 or 16 (64 ... 1024 bits)
 void launch_signatures(hipStream_t st, const DevColl& c, uint32_t words, uint64_t* sigs);
+// Signature width (64-bit words) for the fgss signature test of sets of
+// mean size `mean`: the smallest power of two of at least 2 * mean bits,
+// within [64, 1024] -- wide enough that unrelated sets differ in about half
+// of the bits, which rules them out.
+uint32_t auto_signature_words(double mean);
 void launch_bitmap(hipStream_t st, const BitmapArgs& a);
 
 struct VerifyArgs {

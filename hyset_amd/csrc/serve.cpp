@@ -47,7 +47,7 @@ struct Request {
 
 struct Slot {
   hipStream_t stream = nullptr;
-  DevBuf tokens, starts, sizes, tidx, tiles, out_r, out_s, out_ov, counters;
+  DevBuf tokens, starts, sizes, tidx, tiles, sigs, out_r, out_s, out_ov, counters;
   PinnedBuf h_tokens, h_starts, h_sizes, h_tidx, h_tiles, h_counters, h_r, h_s, h_ov;
   uint64_t out_cap = 0;
 };
@@ -68,6 +68,10 @@ struct hs_server {
   DevBounds hbounds[MAX_THRESHOLDS] = {};
   DevBuf d_bounds;
   IndexBuild ib;
+  // parity signatures of the indexed sets for the fgss signature test (the
+  // server's own copy: joins on a context may rewrite the collection's)
+  uint32_t sig_words = 0;
+  DevBuf isigs;
   // requests
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -175,6 +179,7 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     sl.out_ov.ensure(cap * 4);
     sl.out_cap = cap;
   }
+  if (S->sig_words) sl.sigs.ensure((size_t)std::max<uint32_t>(nq, 1) * S->sig_words * 8);
   hipStream_t s = sl.stream;
   HS_CHECK_HIP(hipMemcpyAsync(sl.tokens.ptr, tk, ntok * 4, hipMemcpyHostToDevice, s));
   HS_CHECK_HIP(hipMemcpyAsync(sl.starts.ptr, st, (size_t)nq * 4, hipMemcpyHostToDevice, s));
@@ -203,6 +208,12 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     a.n_plan_tiles = (uint32_t)tiles.size() - 1;
     a.probe_bounds = S->d_bounds.as<DevBounds>();
     a.probe_tidx = sl.tidx.as<uint8_t>();
+    if (S->sig_words) {
+      if (attempt == 0) launch_signatures(s, a.probe, S->sig_words, sl.sigs.as<uint64_t>());
+      a.sig_words = S->sig_words;
+      a.probe_sigs = sl.sigs.as<uint64_t>();
+      a.indexed_sigs = S->isigs.as<uint64_t>();
+    }
     a.out = DevOut{sl.out_r.as<uint32_t>(), sl.out_s.as<uint32_t>(), sl.out_ov.as<uint32_t>(),
                    sl.counters.as<unsigned long long>(), sl.out_cap};
     launch_fgss(s, a, 0);
@@ -321,8 +332,10 @@ void worker(hs_server* S) {
 }
 
 // Device bytes a request holds while in flight: its query CSR, threshold
-// indices and tile plan, plus room for 8 result triples per query.
-uint64_t request_bytes(uint32_t n, uint64_t ntok) { return 4 * ntok + 13ull * n + 96ull * n + 4096; }
+// indices, tile plan and signatures, plus room for 8 result triples per query.
+uint64_t request_bytes(uint32_t n, uint64_t ntok, uint32_t sig_words) {
+  return 4 * ntok + (13ull + 8ull * sig_words) * n + 96ull * n + 4096;
+}
 
 }  // namespace
 
@@ -374,6 +387,10 @@ int hs_server_create(int device, hs_coll* index, int32_t sim, int32_t bounds_mod
     HS_CHECK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
     build_index(st, index->dev(), 0, index->n - 1, S->hbounds[0].maxprefix, index->size_first.as<uint32_t>(),
                 d_cum.as<uint32_t>(), index->universe, acc, S->ib);
+    S->sig_words = auto_signature_words(index->n ? (double)index->n_tokens / index->n : 0.0);
+    S->stats.signature_bits = S->sig_words * 64;
+    S->isigs.ensure((size_t)std::max<uint32_t>(index->n, 1) * S->sig_words * 8);
+    launch_signatures(st, index->dev(), S->sig_words, S->isigs.as<uint64_t>());
     HS_CHECK_HIP(hipStreamSynchronize(st));
     HS_CHECK_HIP(hipStreamDestroy(st));
     for (uint32_t k = 0; k < c.slots; ++k) S->workers.emplace_back(worker, S.get());
@@ -404,7 +421,7 @@ int hs_server_submit(hs_server* S, const uint32_t* tokens, const uint32_t* sizes
     HS_REQUIRE(total == n_tokens, HS_E_INVALID, "n_tokens does not match the sizes");
     r->tokens.assign(tokens, tokens + n_tokens);
     r->tidx = (uint8_t)table_for(S, threshold);
-    r->bytes = request_bytes(n_queries, n_tokens);
+    r->bytes = request_bytes(n_queries, n_tokens, S->sig_words);
     std::lock_guard<std::mutex> g(S->mu);
     HS_REQUIRE(!S->stop, HS_E_INVALID, "server is shutting down");
     r->id = S->next_id++;

@@ -150,6 +150,10 @@ typedef struct {
   uint32_t flags;             /* HS_TUNE_* bits */
   uint32_t tile_tokens;       /* batched fgss: probe tokens per tile, <= 4096 (default 4096) */
   uint32_t workgroups_per_cu; /* batched fgss: persistent workgroups per CU (default 4) */
+  uint32_t signature_bits;    /* fgss: width of the parity-signature test that runs between the
+                                 positional filter and verification, 64, 128, 256, 512 or 1024
+                                 (default: the smallest power of two >= 2 x the mean indexed set
+                                 size, within 64..1024) */
 } hs_tune;
 
 /* hs_join_batch: capture the device sequence in a HIP graph on first use and
@@ -160,6 +164,10 @@ enum { HS_TUNE_GRAPH = 1 };
  * estimated work planned on the device after the index build and claimed
  * by a persistent grid (kept for comparison). */
 enum { HS_TUNE_STATIC_TILES = 2 };
+/* fgss: no signature test (every visit passing the positional filter is
+ * verified by a merge, as in the reference's fgssjoin).  The pair set is the
+ * same either way. */
+enum { HS_TUNE_NO_SIGNATURE = 4 };
 
 /* Result sink.  count is always written.  With count_only == 0 the pairs
  * (r = probe id, s = indexed id, overlap = |r n s|) are copied into the
@@ -192,6 +200,9 @@ typedef struct {
   uint64_t pairs;
   uint64_t probe_prefix_tokens;
   uint64_t candidate_tokens; /* fgss: sum of |s| over verified candidates */
+  uint64_t signature_tests;  /* fgss: visits that passed the positional filter and were tested
+                                by signature (0 without the signature test) */
+  uint32_t signature_bits;   /* fgss: signature width used (0: none) */
 } hs_stats;
 
 /* ------------------------------------------------------------- contexts */
@@ -241,12 +252,14 @@ int hs_upload_batch(hs_ctx* ctx, uint32_t n_joins, const uint32_t* const* tokens
 /* Self-joins every collection of a batch with the context's threshold: one
  * index build over all of them and one persistent prefix-filter launch that
  * drains a device work queue of probe tiles.  tune: probes_per_tile,
- * tile_tokens, workgroups_per_cu, flags (HS_TUNE_GRAPH). */
+ * tile_tokens, workgroups_per_cu, signature_bits, flags (HS_TUNE_GRAPH,
+ * HS_TUNE_NO_SIGNATURE). */
 int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune, hs_result* result);
 
-/* Times hs_join_batch over a grid of tile sizes and persistent-grid sizes
- * (reps timed runs each after one warm-up) and returns the fastest setting;
- * *best_ms receives its mean device time. */
+/* Times hs_join_batch over a grid of tile sizes and persistent-grid sizes,
+ * then over signature widths (64, 128, 256 bits, none) at the best grid
+ * point (reps timed runs each after one warm-up) and returns the fastest
+ * setting; *best_ms receives its mean device time. */
 int hs_autotune_batch(hs_ctx* ctx, hs_coll* batch, int32_t reps, hs_tune* best, double* best_ms);
 
 /* -------------------------------------------------------------- serving */
@@ -265,8 +278,11 @@ typedef struct {
   uint32_t max_query_size;   /* largest query set (default 4096; 1024 for cosine) */
 } hs_server_config;
 
+/* signature_bits: width of the fgss signature test the server runs (chosen
+ * from the index's mean set size); a request of n query sets and T tokens
+ * holds 4 T + (109 + signature_bits / 8) n + 4096 device bytes. */
 typedef struct {
-  uint64_t submitted, batches, deferred, rejected, max_in_flight, max_bytes_in_flight;
+  uint64_t submitted, batches, deferred, rejected, max_in_flight, max_bytes_in_flight, signature_bits;
 } hs_server_stats;
 
 /* index: a collection uploaded on `device` (kept alive by the caller). */

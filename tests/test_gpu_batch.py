@@ -62,6 +62,9 @@ def test_batch_tile_sizes_and_grid(ctx, batch):
     for probes, wpc in ((64, 2), (128, 8), (256, 1)):
         tune = H.Tune(probes_per_tile=probes, tile_tokens=probes * 16, workgroups_per_cu=wpc)
         assert split(b, *ctx.join_batch(b, tune=tune)) == want, (probes, wpc)
+    # signature widths and no signature test
+    for tune in (H.Tune(signature_bits=128), H.Tune(signature_bits=1024), H.Tune(flags=H.TUNE_NO_SIGNATURE)):
+        assert split(b, *ctx.join_batch(b, tune=tune)) == want, (tune.signature_bits, tune.flags)
 
 
 def test_batch_graph_replays(ctx, batch):
@@ -84,4 +87,5 @@ def test_autotune_batch(ctx, batch):
     ctx.set_threshold(0.75)
     tune, ms = ctx.autotune_batch(b, reps=1)
     assert tune.probes_per_tile in (64, 128, 256) and tune.workgroups_per_cu in (2, 4, 8) and ms > 0
+    assert tune.signature_bits in (0, 64, 128, 256)
     assert split(b, *ctx.join_batch(b, tune=tune)) == expected(colls, 0.75)

@@ -49,6 +49,52 @@ def test_fgss_tile_plans(ctx, small, t, tune):
         _check(ctx.join(d, algo="fgss", indexed_range=rng, probe_range=rng), want, f"fgss balanced {rng}")
 
 
+@pytest.mark.parametrize("bits", [64, 128, 256, 512, 1024, "none"])
+@pytest.mark.parametrize("t,sim", [(0.6, "jaccard"), (0.8, "jaccard"), (0.9, "jaccard"), (0.7, "cosine"),
+                                   (0.8, "dice")])
+def test_fgss_signature_widths(ctx, small, t, sim, bits):
+    """The signature test between the positional filter and verification
+    only drops visits that cannot pass: every width, and no test at all,
+    give the oracle's result; the test runs and prunes."""
+    c, d = small
+    ctx.set_threshold(t, sim)
+    want = O.self_join(c, t, sim)
+    tu = H.Tune(flags=H.TUNE_NO_SIGNATURE) if bits == "none" else H.Tune(signature_bits=bits)
+    _check(ctx.join(d, algo="fgss", tune=tu), want, f"fgss signature {bits} {sim} {t}")
+    st = ctx.stats()
+    if bits == "none":
+        assert st["signature_bits"] == 0 and st["signature_tests"] == 0
+    else:
+        assert st["signature_bits"] == bits
+        assert st["signature_tests"] >= st["candidates"]
+    rng = (100, 2500)
+    _check(ctx.join(d, algo="fgss", tune=tu, indexed_range=rng, probe_range=rng),
+           O.self_join_blocks(c, t, rng, rng, sim), f"fgss signature {bits} block {rng}")
+
+
+def test_fgss_signature_prunes_candidates(ctx):
+    """On the 1M-set workload's distribution (50K sets) the 64-bit signature
+    test leaves far fewer merges than the positional filter alone, and the
+    same pairs."""
+    c = H.generate_self(H.gen_params(50_000, median=10, sigma=0.6, lo=2, hi=200, universe=50_000, zipf=1.1,
+                                     seed=3))
+    d = ctx.upload(c)
+    try:
+        ctx.set_threshold(0.8)
+        a = H.sort_pairs(*ctx.join(d, algo="fgss"))
+        with_sig = ctx.stats()
+        b = H.sort_pairs(*ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_NO_SIGNATURE)))
+        without = ctx.stats()
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+        assert with_sig["signature_bits"] == 64
+        assert with_sig["signature_tests"] == without["candidates"]
+        assert with_sig["candidates"] * 5 < without["candidates"]
+        _check(a, O.self_join(c, 0.8), "fgss 50K")
+    finally:
+        d.free()
+
+
 @pytest.mark.parametrize("bits", [64, 128, 256])
 @pytest.mark.parametrize("t", [0.6, 0.8, 0.9])
 def test_bitmap_small(ctx, small, t, bits):

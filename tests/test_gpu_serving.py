@@ -78,7 +78,10 @@ def test_budget_defers_and_refuses(served):
     idx, q, d = served
     reqs = requests(q, 16, 11)
     big = max(reqs, key=lambda x: x[0].n)
-    need = 4 * int(big[0].tokens.shape[0]) + 109 * big[0].n + 4096
+    with H.Server(d, slots=1) as probe_srv:
+        sig_bytes = probe_srv.stats()["signature_bits"] // 8
+    assert sig_bytes == 8  # 64 bits for sets of mean size 12
+    need = 4 * int(big[0].tokens.shape[0]) + (109 + sig_bytes) * big[0].n + 4096
     with H.Server(d, slots=4, hbm_budget_bytes=need) as srv:
         ids = [srv.submit(qc, t) for qc, t in reqs]
         got = [triples(*srv.wait_pairs(i)[:3]) for i in ids]
