@@ -63,7 +63,8 @@ def bench_long_rs(ctx, H) -> dict:
     st = ctx.stats()
     res = {"long_rs_pairs": int(c), "long_rs_fgss_ms": round(min(ms), 2),
            "long_rs_posting_visits": int(st["posting_visits"]), "long_rs_signature_tests": int(st["signature_tests"]),
-           "long_rs_signature_bits": int(st["signature_bits"]), "long_rs_candidates": int(st["candidates"])}
+           "long_rs_signature_bits": int(st["signature_bits"]), "long_rs_candidates": int(st["candidates"]),
+           "long_rs_tile_plan": int(st["tile_plan"])}
     log(f"long R-S cosine 0.8: {c} pairs, fgss {res['long_rs_fgss_ms']} ms, {st['posting_visits']} visits, "
         f"{st['signature_tests']} signature tests ({st['signature_bits']} bits), {st['candidates']} verified")
     assert ctx.join(dS, dR, algo="fgss", tune=H.Tune(flags=H.TUNE_NO_SIGNATURE), count_only=True) == c
@@ -361,6 +362,7 @@ def main() -> int:
              "candidates": int(st["candidates"]), "posting_visits": int(st["posting_visits"]),
              "index_entries": int(st["index_entries"]), "probe_prefix_tokens": int(st["probe_prefix_tokens"]),
              "signature_tests": int(st["signature_tests"]), "signature_bits": int(st["signature_bits"]),
+             "tile_plan": int(st["tile_plan"]),
              "hbm_copy_kernel_gbps": round(copy_gbps, 1)}
 
     cpu = None
@@ -390,24 +392,36 @@ def main() -> int:
         # the other thresholds and techniques of the configuration
         for tt in (0.6, 0.7, 0.9):
             ctx.set_threshold(tt)
-            ctx.join(d, algo="fgss", count_only=True)
-            ms = []
-            for _ in range(3):
-                c = ctx.join(d, algo="fgss", count_only=True)
-                ms.append(ctx.stats()["total_ms"])
-            extra[f"fgss_t{tt}_ms"] = round(statistics.median(ms), 3)
+            for plan, flag in (("", 0), ("_static_tiles", H.TUNE_STATIC_TILES), ("_tile_queue", H.TUNE_TILE_QUEUE),
+                               ("_balanced_tiles", H.TUNE_BALANCED_TILES)):
+                tu = H.Tune(flags=flag)
+                c0 = ctx.join(d, algo="fgss", tune=tu, count_only=True)
+                ms = []
+                for _ in range(3):
+                    c = ctx.join(d, algo="fgss", tune=tu, count_only=True)
+                    assert c == c0
+                    ms.append(ctx.stats()["total_ms"])
+                extra[f"fgss_t{tt}{plan}_ms"] = round(statistics.median(ms), 3)
+                if not plan:
+                    extra[f"fgss_t{tt}_tile_plan"] = int(ctx.stats()["tile_plan"])
+                log(f"fgss t={tt}{plan}: {c} pairs {statistics.median(ms):.3f} ms (plan {ctx.stats()['tile_plan']})")
             extra[f"fgss_t{tt}_pairs"] = c
-            log(f"fgss t={tt}: {c} pairs {statistics.median(ms):.3f} ms")
         ctx.set_threshold(t)
-        # the same join with one workgroup per tile in plan order (no tile queue)
-        st_tune = H.Tune(flags=H.TUNE_STATIC_TILES)
-        ctx.join(d, algo="fgss", tune=st_tune, count_only=True)
-        ks = []
-        for _ in range(5):
-            assert ctx.join(d, algo="fgss", tune=st_tune, count_only=True) == pairs
-            ks.append(ctx.stats()["kernel_ms"])
-        extra["k_fgss_static_tiles_ms"] = round(statistics.median(ks), 4)
-        log(f"k_fgss with static tiles: {extra['k_fgss_static_tiles_ms']} ms (tile queue: {kern_ms:.4f} ms)")
+        # the other tile plans: the upload-time tiles on a persistent grid,
+        # and work-balanced tiles planned on the device (item-slice pass)
+        for name, flag in (("static_tiles", H.TUNE_STATIC_TILES), ("tile_queue", H.TUNE_TILE_QUEUE),
+                           ("balanced_tiles", H.TUNE_BALANCED_TILES)):
+            tu = H.Tune(flags=flag)
+            ctx.join(d, algo="fgss", tune=tu, count_only=True)
+            ks, ds = [], []
+            for _ in range(5):
+                assert ctx.join(d, algo="fgss", tune=tu, count_only=True) == pairs
+                ks.append(ctx.stats()["kernel_ms"])
+                ds.append(ctx.stats()["total_ms"])
+            extra[f"k_fgss_{name}_ms"] = round(statistics.median(ks), 4)
+            extra[f"join_ms_device_{name}"] = round(statistics.median(ds), 4)
+            log(f"{name}: k_fgss {extra[f'k_fgss_{name}_ms']} ms, join {extra[f'join_ms_device_{name}']} ms "
+                f"(default: k_fgss {kern_ms:.4f} ms, join {dev_ms:.4f} ms)")
         # the same join without the signature test (every positional survivor merged)
         ns_tune = H.Tune(flags=H.TUNE_NO_SIGNATURE)
         ctx.join(d, algo="fgss", tune=ns_tune, count_only=True)

@@ -57,8 +57,11 @@ class Tune(C.Structure):
 
 
 TUNE_GRAPH = 1
-TUNE_STATIC_TILES = 2
+TUNE_TILE_QUEUE = 2
 TUNE_NO_SIGNATURE = 4
+TUNE_BALANCED_TILES = 8
+TUNE_STATIC_TILES = 16
+PLAN_FIXED, PLAN_UPLOAD, PLAN_QUEUE, PLAN_BALANCED = 1, 2, 3, 4
 
 
 class Result(C.Structure):
@@ -71,7 +74,8 @@ class Stats(C.Structure):
                 ("total_ms", C.c_double), ("host_filter_ms", C.c_double), ("kernel_ms", C.c_double),
                 ("index_entries", C.c_uint64), ("posting_visits", C.c_uint64),
                 ("candidates", C.c_uint64), ("pairs", C.c_uint64), ("probe_prefix_tokens", C.c_uint64),
-                ("candidate_tokens", C.c_uint64), ("signature_tests", C.c_uint64), ("signature_bits", C.c_uint32)]
+                ("candidate_tokens", C.c_uint64), ("signature_tests", C.c_uint64), ("signature_bits", C.c_uint32),
+                ("tile_plan", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

@@ -438,6 +438,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
 
     // ---- index over the indexed range (not needed by the bitmap technique)
     const uint32_t* prefix_tab = self ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
+    uint64_t index_entries = 0;
     if (algo != HS_ALGO_BITMAP && !range_empty(ir)) {
       // entries per size class of the range (sets are sorted by size)
       const auto& ptab = self ? ctx->tables.midprefix : ctx->tables.maxprefix;
@@ -455,6 +456,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         HS_REQUIRE(acc < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
       }
       cum[nl - 1] = (uint32_t)acc;
+      index_entries = acc;
       HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, cum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
       hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
                       ctx->d_cum.as<uint32_t>(), indexed->universe, acc, ctx->ib);
@@ -465,6 +467,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
 
     // ---- candidate generation + verification
     unsigned long long cnt[hs::CNT_WORDS];
+    uint32_t tile_plan = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
       reset_counters(ctx);
       if (algo == HS_ALGO_FGSS && !range_empty(ir))  // index size, read back with the counters
@@ -502,8 +505,30 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
           a.n_plan_tiles = lt - ft + 1;
         }
         uint32_t wgs = 0;
-        if (ppt == 0 && !(tune.flags & HS_TUNE_STATIC_TILES) &&
-            (uint64_t)(pr.last - pr.first + 1) * 1024 < 0xFFFFFFFFull && probe->maxlen < 0xFFFFu) {
+        // Tile plan.  The work-balanced plan costs a pass over every probe-prefix
+        // item; it pays when tiles differ much in work, which follows the mean
+        // posting-list length (index entries per token of the universe): the
+        // visits of a tile grow with it.  Measured on 1M sets (sizes median 10,
+        // Zipf 1.1, 647,502 distinct tokens): upload-time tiles win at mean length
+        // 1.8 and 2.9 (Jaccard 0.9, 0.8), balanced tiles at 4.1 and 5.6 (0.7, 0.6)
+        // and on 200K x 200K sets of 100-700 tokens (length ~600); small joins keep
+        // the upload-time tiles.
+        bool balanced = false;
+        if (ppt == 0 && (uint64_t)(pr.last - pr.first + 1) * 1024 < 0xFFFFFFFFull && probe->maxlen < 0xFFFFu) {
+          if (tune.flags & HS_TUNE_BALANCED_TILES) {
+            balanced = true;
+          } else if (!(tune.flags & (HS_TUNE_STATIC_TILES | HS_TUNE_TILE_QUEUE))) {
+            const auto& psf = probe->h_size_first;
+            uint64_t items = 0;
+            for (uint32_t l = 0; l + 1 < (uint32_t)psf.size(); ++l) {
+              const uint32_t lo = std::max(pr.first, psf[l]), hi = std::min(pr.last + 1, psf[l + 1]);
+              if (hi > lo) items += (uint64_t)(hi - lo) * ctx->tables.maxprefix[std::min(l, ctx->tables.L)];
+            }
+            const double mean_list = (double)index_entries / std::max<uint32_t>(indexed->universe, 1);
+            balanced = items >= (1u << 18) && mean_list >= 3.5;
+          }
+        }
+        if (balanced) {
           // item slices first, then tiles of equal work claimed by a persistent grid
           // per size class of the probe range: first item slot, first probe,
           // window of indexed ids, maxprefix
@@ -535,6 +560,11 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
                               &ctx->bal_slice_a, &ctx->bal_slice_n, &ctx->bal_cost, &ctx->bal_tiles,
                               &ctx->bal_ntiles, &ctx->ib.scan_scratch, &ctx->bal_rec, &ctx->bal_slot};
           hs::plan_balanced_tiles(st, a, bp);
+        }
+        tile_plan = ppt ? HS_PLAN_FIXED : balanced ? HS_PLAN_BALANCED
+                                        : (tune.flags & HS_TUNE_TILE_QUEUE) ? HS_PLAN_QUEUE : HS_PLAN_UPLOAD;
+        if (ppt == 0 && (balanced || (tune.flags & HS_TUNE_TILE_QUEUE))) {
+          // a persistent grid claims the tiles from per-XCD queues
           ctx->batch_queue.ensure(8 * sizeof(uint32_t));
           a.tile_queue = ctx->batch_queue.as<uint32_t>();
           wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : 4);
@@ -597,6 +627,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       ctx->stats.probe_prefix_tokens = cnt[hs::CNT_ITEMS];
     }
     if (algo == HS_ALGO_FGSS) {
+      ctx->stats.tile_plan = tile_plan;
       ctx->stats.index_entries = cnt[hs::CNT_ENTRIES];
       ctx->stats.signature_tests = cnt[hs::CNT_SIG_TESTS];
       ctx->stats.signature_bits = sig_words * 64;

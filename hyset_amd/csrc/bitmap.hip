@@ -26,6 +26,10 @@
 
 namespace hs {
 
+// One thread per set; the set's tokens are read SIG_CHUNK at a time with
+// independent loads (one memory latency per chunk, not per token).
+constexpr uint32_t SIG_CHUNK = 8;
+
 template <int W>
 __global__ void k_signatures(DevColl c, uint64_t* __restrict__ sigs) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -35,11 +39,18 @@ __global__ void k_signatures(DevColl c, uint64_t* __restrict__ sigs) {
   for (int k = 0; k < W; ++k) w[k] = 0;
   const uint32_t* t = c.tokens + c.starts[i];
   const uint32_t l = c.sizes[i];
-  for (uint32_t k = 0; k < l; ++k) {
-    uint32_t bit = t[k] & (64u * W - 1u);
+  for (uint32_t k0 = 0; k0 < l; k0 += SIG_CHUNK) {
+    uint32_t v[SIG_CHUNK];
 #pragma unroll
-    for (int q = 0; q < W; ++q)
-      if ((bit >> 6) == (uint32_t)q) w[q] ^= 1ull << (bit & 63u);
+    for (uint32_t q = 0; q < SIG_CHUNK; ++q) v[q] = k0 + q < l ? t[k0 + q] : 0u;
+#pragma unroll
+    for (uint32_t q = 0; q < SIG_CHUNK; ++q) {
+      if (k0 + q >= l) break;
+      const uint32_t bit = v[q] & (64u * W - 1u);
+#pragma unroll
+      for (int b = 0; b < W; ++b)
+        if ((bit >> 6) == (uint32_t)b) w[b] ^= 1ull << (bit & 63u);
+    }
   }
 #pragma unroll
   for (int k = 0; k < W; ++k) sigs[(size_t)i * W + k] = w[k];

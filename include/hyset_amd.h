@@ -149,7 +149,7 @@ typedef struct {
   uint64_t batch_candidates;  /* hybrid: candidates per device batch (default 1<<22) */
   uint32_t flags;             /* HS_TUNE_* bits */
   uint32_t tile_tokens;       /* batched fgss: probe tokens per tile, <= 4096 (default 4096) */
-  uint32_t workgroups_per_cu; /* batched fgss: persistent workgroups per CU (default 4) */
+  uint32_t workgroups_per_cu; /* fgss, batched fgss: persistent workgroups per CU (default 4) */
   uint32_t signature_bits;    /* fgss: width of the parity-signature test that runs between the
                                  positional filter and verification, 64, 128, 256, 512 or 1024
                                  (default: the smallest power of two >= 2 x the mean indexed set
@@ -159,11 +159,19 @@ typedef struct {
 /* hs_join_batch: capture the device sequence in a HIP graph on first use and
  * replay it while the collection, the bounds and the tile plan are unchanged. */
 enum { HS_TUNE_GRAPH = 1 };
-/* hs_join fgss: one workgroup per tile of the collection's upload-time plan
- * (at most 256 probes and 4096 tokens) instead of the default: tiles of equal
- * estimated work planned on the device after the index build and claimed
- * by a persistent grid (kept for comparison). */
-enum { HS_TUNE_STATIC_TILES = 2 };
+/* hs_join fgss tiles.  HS_TUNE_STATIC_TILES: one workgroup per tile of the
+ * collection's upload-time plan (at most 256 probes and 4096 tokens).
+ * HS_TUNE_TILE_QUEUE: the same tiles claimed by a persistent grid
+ * (workgroups_per_cu per CU) from per-XCD queues.  HS_TUNE_BALANCED_TILES:
+ * tiles of equal estimated work, planned on the device after the index
+ * build by a pass that locates every probe-prefix item's posting-list
+ * slice, claimed by the persistent grid.  Default (none of these): the
+ * balanced plan for joins of at least 2^18 probe-prefix items whose index
+ * holds at least 3.5 entries per token of the universe (long posting lists,
+ * so tiles differ much in work), the upload-time tiles otherwise. */
+enum { HS_TUNE_TILE_QUEUE = 2, HS_TUNE_BALANCED_TILES = 8, HS_TUNE_STATIC_TILES = 16 };
+/* hs_stats.tile_plan: the plan a fgss join ran with. */
+enum { HS_PLAN_FIXED = 1, HS_PLAN_UPLOAD = 2, HS_PLAN_QUEUE = 3, HS_PLAN_BALANCED = 4 };
 /* fgss: no signature test (every visit passing the positional filter is
  * verified by a merge, as in the reference's fgssjoin).  The pair set is the
  * same either way. */
@@ -203,6 +211,7 @@ typedef struct {
   uint64_t signature_tests;  /* fgss: visits that passed the positional filter and were tested
                                 by signature (0 without the signature test) */
   uint32_t signature_bits;   /* fgss: signature width used (0: none) */
+  uint32_t tile_plan;        /* fgss: HS_PLAN_* (0 for other techniques) */
 } hs_stats;
 
 /* ------------------------------------------------------------- contexts */

@@ -33,20 +33,26 @@ def test_fgss_small(ctx, small, t, sim):
     _check(ctx.join(d, algo="fgss"), want, f"fgss {sim} {t}")
 
 
-@pytest.mark.parametrize("tune", ["static", "ppt64", "ppt1"])
+@pytest.mark.parametrize("tune", ["default", "static", "queue", "balanced", "ppt64", "ppt1", "wpc1"])
 @pytest.mark.parametrize("t", [0.6, 0.8])
 def test_fgss_tile_plans(ctx, small, t, tune):
-    """The upload-time plan (one workgroup per tile) and fixed-size tiles give
-    the balanced plan's result."""
+    """Every tile plan gives the oracle's result: the automatic choice, one
+    workgroup per upload-time tile, the same tiles on a persistent grid
+    (also a small one), the work-balanced plan, fixed-size tiles; whole
+    range and sub-ranges."""
     c, d = small
     ctx.set_threshold(t)
     want = O.self_join(c, t)
-    tu = {"static": H.Tune(flags=H.TUNE_STATIC_TILES), "ppt64": H.Tune(probes_per_tile=64),
-          "ppt1": H.Tune(probes_per_tile=1)}[tune]
+    tu = {"default": H.Tune(), "static": H.Tune(flags=H.TUNE_STATIC_TILES), "queue": H.Tune(flags=H.TUNE_TILE_QUEUE),
+          "balanced": H.Tune(flags=H.TUNE_BALANCED_TILES), "ppt64": H.Tune(probes_per_tile=64),
+          "ppt1": H.Tune(probes_per_tile=1), "wpc1": H.Tune(flags=H.TUNE_TILE_QUEUE, workgroups_per_cu=1)}[tune]
     _check(ctx.join(d, algo="fgss", tune=tu), want, f"fgss {tune} {t}")
+    plan = {"default": H.PLAN_UPLOAD, "static": H.PLAN_UPLOAD, "queue": H.PLAN_QUEUE, "balanced": H.PLAN_BALANCED,
+            "ppt64": H.PLAN_FIXED, "ppt1": H.PLAN_FIXED, "wpc1": H.PLAN_QUEUE}[tune]
+    assert ctx.stats()["tile_plan"] == plan  # 3000 sets: too few items for the automatic balanced plan
     for rng in ((0, 0), (1, 1), (2999, 2999), (100, 2500)):
         want = O.self_join_blocks(c, t, rng, rng)
-        _check(ctx.join(d, algo="fgss", indexed_range=rng, probe_range=rng), want, f"fgss balanced {rng}")
+        _check(ctx.join(d, algo="fgss", tune=tu, indexed_range=rng, probe_range=rng), want, f"fgss {tune} {rng}")
 
 
 @pytest.mark.parametrize("bits", [64, 128, 256, 512, 1024, "none"])
@@ -70,6 +76,32 @@ def test_fgss_signature_widths(ctx, small, t, sim, bits):
     rng = (100, 2500)
     _check(ctx.join(d, algo="fgss", tune=tu, indexed_range=rng, probe_range=rng),
            O.self_join_blocks(c, t, rng, rng, sim), f"fgss signature {bits} block {rng}")
+
+
+@pytest.mark.parametrize("n", [3000, 10000])
+def test_index_long_lists(ctx, n):
+    """Posting lists longer than a workgroup sorts in LDS (2048 keys): every
+    set shares tokens 1 and 2 (not frequency-ranked on purpose), so lists 1
+    and 2 hold every set and are sorted in runs and merged.  Results, index
+    size and posting visits equal the oracle's."""
+    rng = np.random.default_rng(n)
+    sets = []
+    for i in range(n):
+        k = 3 + i * 6 // n  # sizes 3..8, ascending
+        rest = np.sort(rng.choice(np.arange(3, 400), size=k - 2, replace=False))
+        sets.append([1, 2] + rest.tolist())
+    c = H.SetCollection.from_sets(sets)
+    d = ctx.upload(c)
+    try:
+        for t in (0.5, 0.6, 0.8):
+            ctx.set_threshold(t)
+            want = O.self_join(c, t)
+            _check(ctx.join(d, algo="fgss"), want, f"long lists n={n} t={t}")
+            st = ctx.stats()
+            assert st["index_entries"] == want.index_entries
+            assert st["posting_visits"] == want.posting_visits
+    finally:
+        d.free()
 
 
 def test_fgss_signature_prunes_candidates(ctx):
@@ -135,8 +167,11 @@ def test_fgss_config2_full(ctx, t):
         want = O.cpu_multi(c, t, 8)
         _check(ctx.join(d, algo="fgss"), want, f"fgss 1M t={t}")
         st = ctx.stats()
+        # the automatic plan: 4.1 index entries per token at t = 0.7, 2.9 and 1.8 above
+        assert st["tile_plan"] == (H.PLAN_BALANCED if t == 0.7 else H.PLAN_UPLOAD)
         if t == 0.8:
-            _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_STATIC_TILES)), want, "fgss 1M static tiles")
+            _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_TILE_QUEUE)), want, "fgss 1M tile queue")
+            _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_BALANCED_TILES)), want, "fgss 1M balanced")
         # postings inside the length windows: the same count as the oracle's
         assert st["posting_visits"] == want.posting_visits
         # verified candidates: at most the oracle's (positional filter drops more)
