@@ -459,7 +459,8 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       index_entries = acc;
       HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, cum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
       hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
-                      ctx->d_cum.as<uint32_t>(), indexed->universe, acc, ctx->ib);
+                      ctx->d_cum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
+                      sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr);
     } else {
       ctx->ib.universe = 0;
     }
@@ -480,8 +481,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         hs::FgssArgs a{};
         a.probe = probe->dev();
         a.indexed = indexed->dev();
-        a.index = hs::DevIndex{ctx->ib.offsets.as<uint32_t>(), ctx->ib.set_ids.as<uint32_t>(),
-                               ctx->ib.sstart.as<uint32_t>(), ctx->ib.meta.as<uint32_t>(), indexed->universe};
+        a.index = hs::dev_index(ctx->ib, sig_words == 1);
         a.bounds = ctx->dev_bounds;
         a.size_first = indexed->size_first.as<uint32_t>();
         a.size_first_len = (uint32_t)indexed->h_size_first.size();

@@ -103,13 +103,12 @@ void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entr
   HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, h_cum, (size_t)ntab * 4, hipMemcpyHostToDevice, st));
   build_index_batch(st, c->dev(), c->join_first.as<uint32_t>(), c->n_joins, ctx->dev_bounds.midprefix,
                     c->size_first.as<uint32_t>(), c->sf_base.as<uint32_t>(), ctx->d_cum.as<uint32_t>(), c->universe,
-                    entries, ctx->ib);
+                    entries, ctx->ib, sig_words == 1 ? c->sigs.as<uint64_t>() : nullptr);
   reset_counters(ctx);
   FgssArgs a{};
   a.probe = c->dev();
   a.indexed = c->dev();
-  a.index = DevIndex{ctx->ib.offsets.as<uint32_t>(), ctx->ib.set_ids.as<uint32_t>(), ctx->ib.sstart.as<uint32_t>(),
-                     ctx->ib.meta.as<uint32_t>(), c->universe};
+  a.index = dev_index(ctx->ib, sig_words == 1);
   a.bounds = ctx->dev_bounds;
   a.size_first = c->size_first.as<uint32_t>();
   a.self = 1;

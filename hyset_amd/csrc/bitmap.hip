@@ -28,7 +28,7 @@ namespace hs {
 
 // One thread per set; the set's tokens are read SIG_CHUNK at a time with
 // independent loads (one memory latency per chunk, not per token).
-constexpr uint32_t SIG_CHUNK = 8;
+constexpr uint32_t SIG_CHUNK = 16;
 
 template <int W>
 __global__ void k_signatures(DevColl c, uint64_t* __restrict__ sigs) {

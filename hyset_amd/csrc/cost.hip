@@ -135,7 +135,7 @@ int hs_features(hs_ctx* ctx, hs_coll* coll, const hs_range* range, float out[HS_
     cum[nl - 1] = (uint32_t)acc;
     HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, cum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
     build_index(st, coll->dev(), r.first, r.last, ctx->dev_bounds.midprefix, coll->size_first.as<uint32_t>(),
-                ctx->d_cum.as<uint32_t>(), coll->universe, acc, ctx->ib);
+                ctx->d_cum.as<uint32_t>(), coll->universe, acc, ctx->ib, nullptr);
     ctx->feat.ensure(4 * sizeof(double));
     double* dacc = ctx->feat.as<double>();
     hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, dacc, 4u);

@@ -28,7 +28,9 @@
 //     popcount(sig_r ^ sig_s)) / 2 for XOR-parity signatures of 64-1024
 //     bits, so a visit with |r| + |s| - popcount < 2 minoverlap cannot be a
 //     result.  Both filters only drop visits that cannot pass, so the result
-//     set is unchanged;
+//     set is unchanged.  64-bit signatures of indexed sets are stored in the
+//     index entries and those of the tile's probes in LDS, so the test reads
+//     nothing beyond the entry; wider ones are read from the per-set table;
 //   * visits that pass both are pushed onto an LDS stack (one LDS atomic per
 //     wave) and verified 256 at a time, one per lane, so the merges run with
 //     full waves instead of in the few lanes whose visit passed;
@@ -121,6 +123,7 @@ struct FgssLds {
   uint32_t q_pj[FG_QCAP];
   uint32_t q_sid[FG_QCAP];
   uint32_t qadd[3];
+  uint64_t psig[FG_TPB];            // probe signatures (64-bit signature tests)
   uint8_t ptidx[FG_TPB];            // per-probe threshold table (serving)
   DevBounds bounds[MAX_THRESHOLDS];  // the tables of a multi-threshold launch
 };
@@ -134,108 +137,90 @@ struct TileScope {
 };
 
 // Window slices of K posting lists at once: for each k, a = lower_bound of
-// k_lo[k] in ids[f_lo[k], f_lo[k] + n_lo[k]) by bisection, then the end of
-// the slice, lower_bound of k_hi[k] >= k_lo[k], by galloping from a (probes
-// at a, a+1, a+3, a+7, ... then bisection inside the last gap).  Slices are
-// short, so the gallop stays on the lines the bisection just read.  All K
-// chains advance in lockstep, so their loads overlap (one memory latency
-// per step instead of one per step and chain).  On return f_lo[k] = a and
-// n_hi[k] = slice length; f_hi is scratch.  n_hi must come in as n_lo.
+// k_lo[k] in ids[f_lo[k], f_lo[k] + n_lo[k]), then b = lower_bound of
+// k_hi[k] >= k_lo[k] in [a, end).  Each search is KARY-ary: a round reads
+// KARY evenly spaced ids of the interval that holds the answer
+// (independent loads) and keeps the one of the KARY + 1 gaps that holds it,
+// so an interval of n ids takes about log_{KARY+1}(n) rounds; an interval
+// of at most KARY ids is read whole, and a list that short is read once for
+// both bounds.  All K searches advance in lockstep, so their loads overlap.
+// (4-ary measured as fast as 8-ary on the 1M-set index and keeps fewer
+// registers live.)  On return f_lo[k] = a and n_hi[k] = b - a; f_hi is
+// scratch.  n_hi must come in as n_lo.
+constexpr uint32_t KARY = 4;
+
 template <uint32_t K>
 __device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids, uint32_t (&f_lo)[K],
                                                 uint32_t (&n_lo)[K], uint32_t (&f_hi)[K], uint32_t (&n_hi)[K],
                                                 const uint32_t (&k_lo)[K], const uint32_t (&k_hi)[K]) {
-  uint32_t end[K];
-#pragma unroll
-  for (uint32_t k = 0; k < K; ++k) end[k] = f_lo[k] + n_lo[k];
-  // bisection for the slice start
-  for (;;) {
-    uint32_t v[K];
-    uint32_t live = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < K; ++k) {
-      live |= n_lo[k];
-      v[k] = n_lo[k] ? ids[f_lo[k] + (n_lo[k] >> 1)] : 0u;
-    }
-    if (!live) break;
-#pragma unroll
-    for (uint32_t k = 0; k < K; ++k) {
-      if (n_lo[k]) {
-        const uint32_t h = n_lo[k] >> 1;
-        if (v[k] < k_lo[k]) {
-          f_lo[k] += h + 1;
-          n_lo[k] -= h + 1;
-        } else {
-          n_lo[k] = h;
-        }
-      }
-    }
-  }
-  // gallop for the slice end: ids[f_lo, left) < k_hi, the end lies in [left, right]
-  uint32_t left[K], step[K], right[K];
-  bool act[K];
+  // per search: the answer lies in [lo, hi] (ids below lo are < key, ids at
+  // hi and beyond are >= key); phase 0 looks for k_lo, phase 1 for k_hi, 2 done
+  uint32_t lo[K], hi[K], ph[K], a[K];
 #pragma unroll
   for (uint32_t k = 0; k < K; ++k) {
-    left[k] = f_lo[k];
-    step[k] = 1;
-    right[k] = end[k];
-    act[k] = n_hi[k] != 0 && left[k] < right[k];
+    lo[k] = 0;
+    hi[k] = n_lo[k];
+    a[k] = 0;
+    ph[k] = n_lo[k] ? 0u : 2u;
+    f_hi[k] = 0;
   }
   for (;;) {
-    uint32_t v[K];
     bool any = false;
 #pragma unroll
-    for (uint32_t k = 0; k < K; ++k) {
-      const uint32_t idx = left[k] + step[k] - 1;
-      if (act[k] && idx >= right[k]) act[k] = false;  // past the list: the end lies in [left, right)
-      v[k] = act[k] ? ids[idx] : 0u;
-      any |= act[k];
-    }
+    for (uint32_t k = 0; k < K; ++k) any |= ph[k] < 2;
     if (!any) break;
+    uint32_t v[K][KARY];
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
-      if (act[k]) {
-        const uint32_t idx = left[k] + step[k] - 1;
-        if (v[k] < k_hi[k]) {
-          left[k] = idx + 1;
-          step[k] <<= 1;
+      const uint32_t len = hi[k] - lo[k];
+#pragma unroll
+      for (uint32_t q = 0; q < KARY; ++q) {
+        const uint32_t pos = len <= KARY ? lo[k] + q : lo[k] + (uint32_t)((uint64_t)(q + 1) * len / (KARY + 1));
+        const bool rd = ph[k] < 2 && (len > KARY || q < len);
+        v[k][q] = rd ? ids[f_lo[k] + pos] : 0xFFFFFFFFu;
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k) {
+      if (ph[k] >= 2) continue;
+      const uint32_t len = hi[k] - lo[k];
+      const uint32_t key = ph[k] == 0 ? k_lo[k] : k_hi[k];
+      uint32_t c = 0;
+#pragma unroll
+      for (uint32_t q = 0; q < KARY; ++q) c += v[k][q] < key;  // unread slots hold 0xFFFFFFFF
+      if (len <= KARY) {
+        const uint32_t ans = lo[k] + c;
+        if (ph[k] == 0 && lo[k] == 0 && hi[k] == n_lo[k]) {
+          // the whole list was read: the second bound from the same ids
+          uint32_t c2 = 0;
+#pragma unroll
+          for (uint32_t q = 0; q < KARY; ++q) c2 += v[k][q] < k_hi[k];
+          a[k] = ans;
+          hi[k] = max(c2, ans);  // b
+          ph[k] = 2;
+        } else if (ph[k] == 0) {
+          a[k] = ans;
+          lo[k] = ans;
+          hi[k] = n_lo[k];
+          ph[k] = 1;
         } else {
-          right[k] = idx;
-          act[k] = false;
+          hi[k] = ans;  // b
+          ph[k] = 2;
         }
+      } else {
+        const uint32_t nlo = c == 0 ? lo[k] : lo[k] + (uint32_t)((uint64_t)c * len / (KARY + 1)) + 1;
+        const uint32_t nhi = c == KARY ? hi[k] : lo[k] + (uint32_t)((uint64_t)(c + 1) * len / (KARY + 1));
+        lo[k] = nlo;
+        hi[k] = nhi;
       }
     }
   }
-  // bisection inside [left, right)
 #pragma unroll
   for (uint32_t k = 0; k < K; ++k) {
-    f_hi[k] = left[k];
-    n_hi[k] = n_hi[k] ? right[k] - left[k] : 0u;
+    const uint32_t b = n_lo[k] ? hi[k] : 0u;
+    f_lo[k] += a[k];
+    n_hi[k] = b - a[k];
   }
-  for (;;) {
-    uint32_t v[K];
-    uint32_t live = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < K; ++k) {
-      live |= n_hi[k];
-      v[k] = n_hi[k] ? ids[f_hi[k] + (n_hi[k] >> 1)] : 0u;
-    }
-    if (!live) break;
-#pragma unroll
-    for (uint32_t k = 0; k < K; ++k) {
-      if (n_hi[k]) {
-        const uint32_t h = n_hi[k] >> 1;
-        if (v[k] < k_hi[k]) {
-          f_hi[k] += h + 1;
-          n_hi[k] -= h + 1;
-        } else {
-          n_hi[k] = h;
-        }
-      }
-    }
-  }
-#pragma unroll
-  for (uint32_t k = 0; k < K; ++k) n_hi[k] = f_hi[k] - f_lo[k];
 }
 
 // Indexed ids [lo, hi) that probe p of size l joins: its length window
@@ -330,13 +315,24 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     L.psize[tid] = l;
     L.lo[tid] = lo;
     L.hi[tid] = hi;
+    if constexpr (kW == 1) L.psig[tid] = A.probe_sigs[p];
     const uint32_t off = st - base;
     L.plds[tid] = (off + l <= FG_TOKCAP) ? off : NO_LDS;
     n_active = (lo < hi) ? mp : 0;
     // the slice pass lays out maxprefix(l) items for every probe, in probe order
     n_items = kSlices ? mp : n_active;
   }
-  for (uint32_t i = tid; i < ntok && i < FG_TOKCAP; i += FG_TPB) L.tok[i] = A.probe.tokens[base + i];
+  {
+    // every load of the staging is issued before the first LDS store
+    constexpr uint32_t R = FG_TOKCAP / FG_TPB;
+    const uint32_t m = min(ntok, FG_TOKCAP);
+    uint32_t tv[R];
+#pragma unroll
+    for (uint32_t k = 0; k < R; ++k) tv[k] = tid + k * FG_TPB < m ? A.probe.tokens[base + tid + k * FG_TPB] : 0u;
+#pragma unroll
+    for (uint32_t k = 0; k < R; ++k)
+      if (tid + k * FG_TPB < m) L.tok[tid + k * FG_TPB] = tv[k];
+  }
   n_active = wave_sum(n_active);
   if (lane_id() == 0 && n_active) atomicAdd(&L.stats[3], (unsigned long long)n_active);
   uint32_t total_items;
@@ -429,22 +425,27 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     // of the stack (one verification site keeps the register budget).
     const bool last_batch = ib + FG_ITEMCAP >= total_items;
     const uint32_t vis_end = total_vis + (last_batch ? FG_TPB : 0u);
+    // (64-bit signatures ride in the index entry and arrive with it)
     uint32_t n_it = 0, n_sid = 0, n_meta = 0;
+    uint64_t n_sig = 0;
     if (tid < total_vis) {
       n_it = segment_of(L.vis_off, nb, tid);
       const uint32_t e = L.it_a[n_it] + (tid - L.vis_off[n_it]);
       n_sid = A.index.set_ids[e];
       n_meta = A.index.meta[e];
+      if constexpr (kW == 1) n_sig = A.index.esig[e];
     }
     for (uint32_t vb = 0; vb < vis_end; vb += FG_TPB, ++rnd) {
       const uint32_t vi = vb + tid;
       const uint32_t it = n_it, sid = n_sid, meta = n_meta;
+      const uint64_t ssig = n_sig;
       const uint32_t vn = vi + FG_TPB;
       if (vn < total_vis) {
         n_it = segment_of(L.vis_off, nb, vn);
         const uint32_t e = L.it_a[n_it] + (vn - L.vis_off[n_it]);
         n_sid = A.index.set_ids[e];
         n_meta = A.index.meta[e];
+        if constexpr (kW == 1) n_sig = A.index.esig[e];
       }
       if (tid == 0) L.qadd[(rnd + 1) % 3] = 0;  // its last reader passed the previous barrier
       bool keep = false;
@@ -462,7 +463,13 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
         // the first-common visit can emit the pair, so a visit failing the
         // bound is dropped without reading s.
         keep = kp == 0xFFFFu || 1u + min(lr - 1u - j, ls - 1u - kp) >= mo;
-        if constexpr (kW > 0) {
+        if constexpr (kW == 1) {
+          // both signatures at hand: the probe's in LDS, the set's from its entry
+          if (keep) {
+            ++C.sig_tests;
+            keep = lr + ls >= 2u * mo + (uint32_t)__popcll(L.psig[pl] ^ ssig);
+          }
+        } else if constexpr (kW > 1) {
           if (keep) {
             ++C.sig_tests;
             const uint64_t* sr = A.probe_sigs + (size_t)(T.p0 + pl) * kW;
@@ -812,6 +819,7 @@ static void launch_width(hipStream_t st, const FgssArgs& a, dim3 grid, uint32_t 
 
 static void check_sigs(const FgssArgs& a) {
   HS_REQUIRE(a.sig_words == 0 || (a.probe_sigs && a.indexed_sigs), HS_E_INVALID, "signature arrays missing");
+  HS_REQUIRE(a.sig_words != 1 || a.index.esig, HS_E_INVALID, "64-bit signature test needs an index with signatures");
 }
 
 void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persistent_wgs) {

@@ -97,13 +97,15 @@ struct DevColl {
 };
 
 // Index entry e of list(t) = [offsets[t], offsets[t+1]): set_ids[e] (lists
-// ascending), sstart[e] = CSR start of that set, meta[e] = (min(|s|, 0xFFFF)
-// << 16) | min(position of t in s, 0xFFFF).  Start and size ride along so a
-// posting visit reads s without first reading starts[] and sizes[].
+// ascending), meta[e] = (min(|s|, 0xFFFF) << 16) | min(position of t in s,
+// 0xFFFF), and, for 64-bit signature tests, esig[e] = the set's signature.
+// Size, position and signature ride along so a posting visit is filtered
+// from coalesced loads of its own entry, without reading sizes[] or the
+// signature table at random.
 struct DevIndex {
   const uint32_t* offsets;  // universe + 1
   const uint32_t* set_ids;
-  const uint32_t* sstart;
+  const uint64_t* esig;     // NULL unless the index was built with 64-bit signatures
   const uint32_t* meta;
   uint32_t universe;
 };
@@ -133,7 +135,7 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
                            DevBuf& scratch);
 
 struct IndexBuild {
-  DevBuf offsets, set_ids, positions, sstart, meta;  // the index
+  DevBuf offsets, set_ids, positions, esig, meta;  // the index
   DevBuf keys[2], vals, pos, hist, scan_scratch;  // radix-sort work space
   uint32_t universe = 0;
   uint64_t entries = 0;
@@ -142,14 +144,22 @@ struct IndexBuild {
 // of `c` (sets sorted by size).  prefix_len is indexed by set size;
 // size_first[l] is the first set id of size >= l; cum[l] is the number of
 // prefix entries of the range's sets shorter than l; `entries` is the total.
+// sigs (may be NULL): one 64-bit signature per set of c, copied into the
+// entries (esig).
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                 IndexBuild& ib);
+                 IndexBuild& ib, const uint64_t* sigs);
 // The same over every join of a batched collection at once (token ids of
 // different joins are disjoint, so one index serves all joins).
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
                        const uint32_t* prefix_len, const uint32_t* size_first, const uint32_t* sf_base,
-                       const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib);
+                       const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib,
+                       const uint64_t* sigs);
+// The device view of a built index (esig only when it was built with sigs).
+inline DevIndex dev_index(const IndexBuild& ib, bool with_sigs) {
+  return DevIndex{ib.offsets.as<uint32_t>(), ib.set_ids.as<uint32_t>(), with_sigs ? ib.esig.as<uint64_t>() : nullptr,
+                  ib.meta.as<uint32_t>(), ib.universe};
+}
 
 // Batched launch: tile k covers probes [tile_lo[k], tile_hi[k]) of join
 // tile_join[k]; join j owns set ids [join_first[j], join_first[j+1]) and its

@@ -322,12 +322,12 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
 // offsets[t] = number of entries with token < t, for t in [0, universe],
 // from the sorted keys: entry i writes the offsets of the tokens in
 // (keys[i-1], keys[i]] (every t is written once); and, for every entry, the
-// CSR start and size of its set next to the token position (one random
-// read per entry here instead of per visit).
+// size and (when given) the signature of its set next to the token
+// position (one random read per entry here instead of per visit).
 __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, uint32_t universe,
                                uint32_t* __restrict__ offsets, DevColl c, const uint32_t* __restrict__ set_ids,
-                               const uint16_t* __restrict__ positions, uint32_t* __restrict__ sstart,
-                               uint32_t* __restrict__ meta) {
+                               const uint16_t* __restrict__ positions, const uint64_t* __restrict__ sigs,
+                               uint64_t* __restrict__ esig, uint32_t* __restrict__ meta) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   const uint32_t prev = i == 0 ? 0u : keys[i - 1] + 1;   // first token not yet written
@@ -335,7 +335,7 @@ __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, ui
   for (uint32_t t = prev; t <= cur && t <= universe; ++t) offsets[t] = i;
   if (i < n) {
     const uint32_t sid = set_ids[i];
-    sstart[i] = c.starts[sid];
+    if (sigs) esig[i] = sigs[sid];
     meta[i] = (min(c.sizes[sid], 0xFFFFu) << 16) | positions[i];
   }
 }
@@ -348,7 +348,7 @@ static void prepare(IndexBuild& ib, uint32_t universe, uint64_t entries) {
   ib.offsets.ensure(((size_t)universe + 1) * sizeof(uint32_t));
   ib.set_ids.ensure(cap * sizeof(uint32_t));
   ib.positions.ensure(cap * sizeof(uint16_t));
-  ib.sstart.ensure(cap * sizeof(uint32_t));
+  ib.esig.ensure(cap * sizeof(uint64_t));
   ib.meta.ensure(cap * sizeof(uint32_t));
   ib.keys[0].ensure(cap * sizeof(uint32_t));
   ib.keys[1].ensure(cap * sizeof(uint32_t));
@@ -384,7 +384,7 @@ static void radix_pass(hipStream_t st, uint32_t n, uint32_t shift, uint32_t ntil
 }
 
 static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32_t universe, IndexBuild& ib,
-                            int cur, RadixPlan plan) {
+                            int cur, RadixPlan plan, const uint64_t* sigs) {
   const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
   ib.hist.ensure(((size_t)ntiles << plan.bits) * sizeof(uint32_t));
   uint32_t* const kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
@@ -396,8 +396,8 @@ static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32
     cur ^= 1;
   }
   hipLaunchKernelGGL(k_index_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
-                     ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(),
-                     ib.sstart.as<uint32_t>(), ib.meta.as<uint32_t>());
+                     ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(), sigs,
+                     ib.esig.as<uint64_t>(), ib.meta.as<uint32_t>());
   HS_CHECK_HIP(hipGetLastError());
 }
 
@@ -407,7 +407,7 @@ static int emit_buffer(uint32_t passes) { return (passes % 2 == 1) ? 0 : 1; }
 
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                 IndexBuild& ib) {
+                 IndexBuild& ib, const uint64_t* sigs) {
   prepare(ib, universe, entries);
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && first <= last && n > 0) {
@@ -419,7 +419,7 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
     const uint32_t nsets = last - first + 1;
     hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
                        size_first, cum, kb, vb, pb);
-    sort_and_finish(st, c, n, universe, ib, cur, plan);
+    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs);
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }
@@ -428,7 +428,8 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
 
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
                        const uint32_t* prefix_len, const uint32_t* size_first, const uint32_t* sf_base,
-                       const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib) {
+                       const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib,
+                       const uint64_t* sigs) {
   prepare(ib, universe, entries);
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && n > 0) {
@@ -439,7 +440,7 @@ void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_fi
     uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
     hipLaunchKernelGGL(k_index_emit_batch, dim3((c.n + 255) / 256), dim3(256), 0, st, c, join_first, n_joins,
                        prefix_len, size_first, sf_base, cum, kb, vb, pb);
-    sort_and_finish(st, c, n, universe, ib, cur, plan);
+    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs);
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }

@@ -193,8 +193,7 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     FgssArgs a{};
     a.probe = DevColl{sl.tokens.as<uint32_t>(), sl.starts.as<uint32_t>(), sl.sizes.as<uint32_t>(), nq};
     a.indexed = I->dev();
-    a.index = DevIndex{S->ib.offsets.as<uint32_t>(), S->ib.set_ids.as<uint32_t>(), S->ib.sstart.as<uint32_t>(),
-                       S->ib.meta.as<uint32_t>(), I->universe};
+    a.index = dev_index(S->ib, S->sig_words == 1);
     a.bounds = S->hbounds[0];
     a.size_first = I->size_first.as<uint32_t>();
     a.size_first_len = (uint32_t)I->h_size_first.size();
@@ -385,12 +384,13 @@ int hs_server_create(int device, hs_coll* index, int32_t sim, int32_t bounds_mod
     HS_CHECK_HIP(hipMemcpy(d_cum.ptr, cum.data(), cum.size() * 4, hipMemcpyHostToDevice));
     hipStream_t st;
     HS_CHECK_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
-    build_index(st, index->dev(), 0, index->n - 1, S->hbounds[0].maxprefix, index->size_first.as<uint32_t>(),
-                d_cum.as<uint32_t>(), index->universe, acc, S->ib);
     S->sig_words = auto_signature_words(index->n ? (double)index->n_tokens / index->n : 0.0);
     S->stats.signature_bits = S->sig_words * 64;
     S->isigs.ensure((size_t)std::max<uint32_t>(index->n, 1) * S->sig_words * 8);
     launch_signatures(st, index->dev(), S->sig_words, S->isigs.as<uint64_t>());
+    build_index(st, index->dev(), 0, index->n - 1, S->hbounds[0].maxprefix, index->size_first.as<uint32_t>(),
+                d_cum.as<uint32_t>(), index->universe, acc, S->ib,
+                S->sig_words == 1 ? S->isigs.as<uint64_t>() : nullptr);
     HS_CHECK_HIP(hipStreamSynchronize(st));
     HS_CHECK_HIP(hipStreamDestroy(st));
     for (uint32_t k = 0; k < c.slots; ++k) S->workers.emplace_back(worker, S.get());
