@@ -243,10 +243,16 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_hist(const uint32_t* __restric
   for (uint32_t d = threadIdx.x; d < NB; d += RS_TPB) hist[d * ntiles + blockIdx.x] = h[d];
 }
 
-// Stable scatter of one tile.  The tile's keys, values and positions are
-// loaded into registers up front (one memory latency per tile); each round
-// of 256 entries ranks lanes of equal digit with kBits ballots per wave and
-// orders the waves of the round through per-wave digit counts in LDS.
+// Stable scatter of one tile.  Each wave owns a contiguous quarter of the
+// tile (RS_ROUNDS rounds of 64 entries, loaded into registers up front), so
+// the tile's order is wave by wave.  The waves count their digits (LDS
+// atomics into wave-private rows); one barrier later every wave knows where
+// its entries of digit d start inside the tile sorted by digit, ranks its
+// rounds on its own (lanes of equal digit found with kBits ballots, the
+// leader of each digit advancing the wave's cursor) and places them in an
+// LDS copy of the tile in digit order.  The sorted tile is then written out
+// with consecutive lanes on consecutive slots, so each digit's run of the
+// tile leaves as contiguous stores instead of one scattered store per lane.
 template <int kBits>
 __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                           const uint32_t* __restrict__ vin,
@@ -255,32 +261,73 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
                                                           uint32_t ntiles, uint32_t* __restrict__ kout,
                                                           uint32_t* __restrict__ vout, uint16_t* __restrict__ pout) {
   constexpr uint32_t NB = 1u << kBits;
-  __shared__ uint32_t s_base[NB];
-  __shared__ uint32_t s_wcnt[RS_WAVES][NB];
+  __shared__ uint32_t s_cnt[RS_WAVES][NB];  // per-wave digit counts, then the wave's cursors
+  __shared__ uint32_t s_lo[NB];             // first slot of digit d in the sorted tile
+  __shared__ uint32_t s_base[NB];           // output position of that slot
+  __shared__ uint32_t s_key[RS_TILE];
+  __shared__ uint32_t s_val[RS_TILE];
+  __shared__ uint16_t s_pos[RS_TILE];
+  __shared__ uint32_t s_tmp[RS_TPB / WAVE + 1];
   const uint32_t tid = threadIdx.x;
-  const uint32_t wave = tid / WAVE;
-  const uint32_t base = blockIdx.x * RS_TILE;
+  const uint32_t wave = tid / WAVE, lane = lane_id();
+  const uint32_t tbase = blockIdx.x * RS_TILE;
+  const uint32_t wbase = tbase + wave * (RS_ROUNDS * WAVE);
+  const uint32_t tn = min(RS_TILE, n - tbase);  // entries of this tile
   uint32_t key[RS_ROUNDS], val[RS_ROUNDS], ps[RS_ROUNDS];
 #pragma unroll
   for (int r = 0; r < RS_ROUNDS; ++r) {
-    const uint32_t idx = base + r * RS_TPB + tid;
+    const uint32_t idx = wbase + r * WAVE + lane;
     const bool valid = idx < n;
     key[r] = valid ? kin[idx] : 0u;
     val[r] = valid ? vin[idx] : 0u;
     ps[r] = valid ? pin[idx] : 0u;
   }
   for (uint32_t d = tid; d < NB; d += RS_TPB) {
-    s_base[d] = offs[d * ntiles + blockIdx.x];
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) s_wcnt[w][d] = 0;
+    for (int w = 0; w < RS_WAVES; ++w) s_cnt[w][d] = 0;
+    s_base[d] = offs[d * ntiles + blockIdx.x];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < RS_ROUNDS; ++r)
+    if (wbase + r * WAVE + lane < n) atomicAdd(&s_cnt[wave][(key[r] >> shift) & (NB - 1)], 1u);
+  __syncthreads();
+  // the tile sorted by digit: digit d occupies [s_lo[d], s_lo[d] + total(d)),
+  // wave w's entries of d start after those of the earlier waves
+  static_assert(NB % RS_TPB == 0 || NB < RS_TPB, "digits per thread");
+  constexpr uint32_t DPT = NB > RS_TPB ? NB / RS_TPB : 1;
+  uint32_t tot[DPT], sum = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < DPT; ++k) {
+    const uint32_t d = tid * DPT + k;
+    tot[k] = 0;
+    if (d < NB)
+#pragma unroll
+      for (int w = 0; w < RS_WAVES; ++w) tot[k] += s_cnt[w][d];
+    sum += tot[k];
+  }
+  uint32_t all;
+  uint32_t run = block_exclusive_scan<RS_TPB>(sum, s_tmp, &all);
+#pragma unroll
+  for (uint32_t k = 0; k < DPT; ++k) {
+    const uint32_t d = tid * DPT + k;
+    if (d < NB) {
+      s_lo[d] = run;
+      uint32_t c = run;
+#pragma unroll
+      for (int w = 0; w < RS_WAVES; ++w) {
+        const uint32_t x = s_cnt[w][d];
+        s_cnt[w][d] = c;
+        c += x;
+      }
+    }
+    run += tot[k];
   }
   __syncthreads();
 #pragma unroll
   for (int r = 0; r < RS_ROUNDS; ++r) {
-    const uint32_t idx = base + r * RS_TPB + tid;
-    const bool valid = idx < n;
+    const bool valid = wbase + r * WAVE + lane < n;
     const uint32_t d = (key[r] >> shift) & (NB - 1);
-    // lanes of this wave holding the same digit
     uint64_t peers = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < kBits; ++b) {
@@ -288,34 +335,28 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
-    const uint32_t rank = lanes_below(peers);
-    const bool leader = valid && rank == 0;
-    if (leader) s_wcnt[wave][d] = __popcll(peers);
-    __syncthreads();
-    uint32_t total = 0;
-    bool last = true;  // no later wave of the round holds digit d
     if (valid) {
-      uint32_t before = 0;
-#pragma unroll
-      for (uint32_t w = 0; w < RS_WAVES; ++w) {
-        const uint32_t cnt = s_wcnt[w][d];
-        before += w < wave ? cnt : 0u;
-        total += cnt;
-        last = last && (w <= wave || cnt == 0);
-      }
-      const uint32_t o = s_base[d] + before + rank;
-      kout[o] = key[r];
-      vout[o] = val[r];
-      pout[o] = (uint16_t)ps[r];
+      const uint32_t o = s_cnt[wave][d] + lanes_below(peers);
+      s_key[o] = key[r];
+      s_val[o] = val[r];
+      s_pos[o] = (uint16_t)ps[r];
     }
-    __syncthreads();
-    // the leader of the last wave holding d advances its base; every leader
-    // clears its count
-    if (leader) {
-      s_wcnt[wave][d] = 0;
-      if (last) s_base[d] += total;
-    }
-    __syncthreads();
+    // the leader of each digit advances the wave's cursor; the next round's
+    // reads come after it in this wave's program order
+    __builtin_amdgcn_wave_barrier();
+    if (valid && lanes_below(peers) == 0) s_cnt[wave][d] += (uint32_t)__popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // slot i of the sorted tile goes to base(d) + (i - lo(d))
+  for (uint32_t i = tid; i < tn; i += RS_TPB) {
+    const uint32_t k = s_key[i];
+    const uint32_t d = (k >> shift) & (NB - 1);
+    const uint32_t o = s_base[d] + (i - s_lo[d]);
+    kout[o] = k;
+    vout[o] = s_val[i];
+    pout[o] = s_pos[i];
   }
 }
 
