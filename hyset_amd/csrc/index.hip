@@ -174,12 +174,14 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 //    contiguous): off(i) = cum[l] + (i - first id of size l) * prefix_len[l].
 // 2. a stable LSD radix sort by token, 8 bits per pass, only as many passes
 //    as the universe needs; stability keeps every list in set-id order.  Per pass:
-//    per-tile digit histograms (LDS atomics), a device scan of the
-//    digit-major histogram table, and a scatter that ranks each element
-//    inside its tile with wave ballots (no global atomics anywhere).
+//    per-tile digit histograms (LDS atomics), a scan of each digit's row of
+//    the histogram table (one workgroup per digit), and a scatter that
+//    ranks each element inside its tile with wave ballots, reorders the
+//    tile by digit in LDS and writes contiguous runs (no global atomics
+//    anywhere: device-scope atomics are slow across the eight XCDs).
 // 3. offsets[t] = first entry with token >= t, written at the token
-//    boundaries of the sorted keys, and the start and size of each entry's
-//    set.
+//    boundaries of the sorted keys, and the size (and, for 64-bit signature
+//    tests, the signature) of each entry's set.
 
 constexpr int RS_TPB = 256;
 constexpr int RS_ROUNDS = 16;
@@ -243,6 +245,38 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_hist(const uint32_t* __restric
   for (uint32_t d = threadIdx.x; d < NB; d += RS_TPB) hist[d * ntiles + blockIdx.x] = h[d];
 }
 
+// The scan of the digit-major histogram table in one launch: workgroup d
+// turns row d (the counts of digit d in every tile) into its exclusive
+// prefix in place and stores the row total; the scatter adds the totals of
+// the smaller digits itself.  One launch instead of a three-kernel scan of
+// the whole table, whose launches cost more than the work at this size.
+__global__ __launch_bounds__(RS_TPB) void k_radix_rows(uint32_t* __restrict__ hist, uint32_t ntiles,
+                                                      uint32_t* __restrict__ totals) {
+  __shared__ uint32_t tmp[RS_TPB / WAVE + 1];
+  uint32_t* row = hist + (size_t)blockIdx.x * ntiles;
+  constexpr uint32_t PT = 8;  // counts per thread and chunk
+  uint32_t carry = 0;
+  for (uint32_t c0 = 0; c0 < ntiles; c0 += RS_TPB * PT) {
+    uint32_t v[PT], local = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < PT; ++k) {
+      const uint32_t i = c0 + threadIdx.x * PT + k;
+      v[k] = i < ntiles ? row[i] : 0u;
+      local += v[k];
+    }
+    uint32_t total;
+    uint32_t run = carry + block_exclusive_scan<RS_TPB>(local, tmp, &total);
+#pragma unroll
+    for (uint32_t k = 0; k < PT; ++k) {
+      const uint32_t i = c0 + threadIdx.x * PT + k;
+      if (i < ntiles) row[i] = run;
+      run += v[k];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) totals[blockIdx.x] = carry;
+}
+
 // Stable scatter of one tile.  Each wave owns a contiguous quarter of the
 // tile (RS_ROUNDS rounds of 64 entries, loaded into registers up front), so
 // the tile's order is wave by wave.  The waves count their digits (LDS
@@ -258,9 +292,11 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
                                                           const uint32_t* __restrict__ vin,
                                                           const uint16_t* __restrict__ pin, uint32_t n,
                                                           uint32_t shift, const uint32_t* __restrict__ offs,
-                                                          uint32_t ntiles, uint32_t* __restrict__ kout,
-                                                          uint32_t* __restrict__ vout, uint16_t* __restrict__ pout) {
+                                                          const uint32_t* __restrict__ totals, uint32_t ntiles,
+                                                          uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                          uint16_t* __restrict__ pout) {
   constexpr uint32_t NB = 1u << kBits;
+  static_assert(NB <= RS_TPB, "one digit total per thread");
   __shared__ uint32_t s_cnt[RS_WAVES][NB];  // per-wave digit counts, then the wave's cursors
   __shared__ uint32_t s_lo[NB];             // first slot of digit d in the sorted tile
   __shared__ uint32_t s_base[NB];           // output position of that slot
@@ -282,10 +318,16 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
     val[r] = valid ? vin[idx] : 0u;
     ps[r] = valid ? pin[idx] : 0u;
   }
-  for (uint32_t d = tid; d < NB; d += RS_TPB) {
+  // output base of digit d in this tile: the entries of smaller digits (a
+  // scan of the digit totals) plus those of digit d in earlier tiles
+  {
+    uint32_t all;
+    const uint32_t below = block_exclusive_scan<RS_TPB>(tid < NB ? totals[tid] : 0u, s_tmp, &all);
+    if (tid < NB) {
 #pragma unroll
-    for (int w = 0; w < RS_WAVES; ++w) s_cnt[w][d] = 0;
-    s_base[d] = offs[d * ntiles + blockIdx.x];
+      for (int w = 0; w < RS_WAVES; ++w) s_cnt[w][tid] = 0;
+      s_base[tid] = below + offs[tid * ntiles + blockIdx.x];
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -418,10 +460,12 @@ static void radix_pass(hipStream_t st, uint32_t n, uint32_t shift, uint32_t ntil
   constexpr uint32_t NB = 1u << kBits;
   hipLaunchKernelGGL((k_radix_hist<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift,
                      ib.hist.as<uint32_t>(), ntiles);
-  launch_exclusive_scan(st, ib.hist.as<uint32_t>(), ib.hist.as<uint32_t>(), nullptr, (size_t)NB * ntiles,
-                        ib.scan_scratch);
+  ib.scan_scratch.ensure(NB * sizeof(uint32_t));
+  hipLaunchKernelGGL(k_radix_rows, dim3(NB), dim3(RS_TPB), 0, st, ib.hist.as<uint32_t>(), ntiles,
+                     ib.scan_scratch.as<uint32_t>());
   hipLaunchKernelGGL((k_radix_scatter<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], vbuf[cur], pbuf[cur], n,
-                     shift, ib.hist.as<uint32_t>(), ntiles, kbuf[cur ^ 1], vbuf[cur ^ 1], pbuf[cur ^ 1]);
+                     shift, ib.hist.as<uint32_t>(), ib.scan_scratch.as<uint32_t>(), ntiles, kbuf[cur ^ 1],
+                     vbuf[cur ^ 1], pbuf[cur ^ 1]);
 }
 
 static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32_t universe, IndexBuild& ib,
