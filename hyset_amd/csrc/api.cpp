@@ -430,15 +430,23 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
 
     // ---- parity signatures for the fgss signature test (timed with the index)
     uint32_t sig_words = 0;
+    bool sig_in_emit = false;  // the index's emit pass computes them (self-join over the whole collection)
     if (algo == HS_ALGO_FGSS && !(tune.flags & HS_TUNE_NO_SIGNATURE) && !range_empty(ir) && !range_empty(pr)) {
       sig_words = hs::fgss_signature_words(indexed, tune.signature_bits);
-      hs::compute_signatures(st, indexed, sig_words);
-      if (!self) hs::compute_signatures(st, probe, sig_words);
+      sig_in_emit = sig_words == 1 && self && ir.first == 0 && ir.last + 1 == indexed->n;
+      if (sig_in_emit) {
+        indexed->sigs.ensure((size_t)std::max<uint32_t>(indexed->n, 1) * 8);
+        indexed->sig_words = 1;
+      } else {
+        hs::compute_signatures(st, indexed, sig_words);
+        if (!self) hs::compute_signatures(st, probe, sig_words);
+      }
     }
 
     // ---- index over the indexed range (not needed by the bitmap technique)
     const uint32_t* prefix_tab = self ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
     uint64_t index_entries = 0;
+    bool counters_ready = false;  // the index build prepared the work counters
     if (algo != HS_ALGO_BITMAP && !range_empty(ir)) {
       // entries per size class of the range (sets are sorted by size)
       const auto& ptab = self ? ctx->tables.midprefix : ctx->tables.maxprefix;
@@ -460,7 +468,10 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, cum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
       hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
                       ctx->d_cum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
-                      sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr);
+                      sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr,
+                      sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
+                      algo == HS_ALGO_FGSS ? ctx->counters.as<unsigned long long>() : nullptr);
+      counters_ready = algo == HS_ALGO_FGSS && acc > 0 && indexed->n > 0;
     } else {
       ctx->ib.universe = 0;
     }
@@ -470,11 +481,13 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     unsigned long long cnt[hs::CNT_WORDS];
     uint32_t tile_plan = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
-      reset_counters(ctx);
-      if (algo == HS_ALGO_FGSS && !range_empty(ir))  // index size, read back with the counters
-        HS_CHECK_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + hs::CNT_ENTRIES,
-                                    ctx->ib.offsets.as<uint32_t>() + indexed->universe, 4,
-                                    hipMemcpyDeviceToDevice, st));
+      if (attempt > 0 || !counters_ready) {
+        reset_counters(ctx);
+        if (algo == HS_ALGO_FGSS && !range_empty(ir))  // index size, read back with the counters
+          HS_CHECK_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + hs::CNT_ENTRIES,
+                                      ctx->ib.offsets.as<uint32_t>() + indexed->universe, 4,
+                                      hipMemcpyDeviceToDevice, st));
+      }
       if (range_empty(pr) || range_empty(ir)) {
         HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
       } else if (algo == HS_ALGO_FGSS) {

@@ -97,13 +97,20 @@ void drop_graph(hs_ctx* ctx) {
 void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entries, uint32_t workgroups,
                    uint32_t sig_words) {
   hipStream_t st = ctx->stream;
-  if (sig_words) compute_signatures(st, c, sig_words);
+  // 64-bit signatures come out of the index's emit pass (it reads every set)
+  if (sig_words == 1) {
+    c->sigs.ensure((size_t)std::max<uint32_t>(c->n, 1) * 8);
+    c->sig_words = 1;
+  } else if (sig_words) {
+    compute_signatures(st, c, sig_words);
+  }
   const uint32_t ntab = (uint32_t)c->h_size_first.size();
   ctx->d_cum.ensure((size_t)ntab * 4);
   HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, h_cum, (size_t)ntab * 4, hipMemcpyHostToDevice, st));
   build_index_batch(st, c->dev(), c->join_first.as<uint32_t>(), c->n_joins, ctx->dev_bounds.midprefix,
                     c->size_first.as<uint32_t>(), c->sf_base.as<uint32_t>(), ctx->d_cum.as<uint32_t>(), c->universe,
-                    entries, ctx->ib, sig_words == 1 ? c->sigs.as<uint64_t>() : nullptr);
+                    entries, ctx->ib, sig_words == 1 ? c->sigs.as<uint64_t>() : nullptr,
+                    sig_words == 1 ? c->sigs.as<uint64_t>() : nullptr);
   reset_counters(ctx);
   FgssArgs a{};
   a.probe = c->dev();

@@ -145,16 +145,21 @@ struct IndexBuild {
 // size_first[l] is the first set id of size >= l; cum[l] is the number of
 // prefix entries of the range's sets shorter than l; `entries` is the total.
 // sigs (may be NULL): one 64-bit signature per set of c, copied into the
-// entries (esig).
+// entries (esig).  sigs_out (may be NULL): the emit pass computes the
+// 64-bit signatures of the range's sets into it on the way (then pass the
+// same array as sigs).  counters (may be NULL): CNT_WORDS work counters the
+// last kernel zeroes, with the index size in CNT_ENTRIES (only when the
+// index has entries).
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                 IndexBuild& ib, const uint64_t* sigs);
+                 IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out = nullptr,
+                 unsigned long long* counters = nullptr);
 // The same over every join of a batched collection at once (token ids of
 // different joins are disjoint, so one index serves all joins).
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
                        const uint32_t* prefix_len, const uint32_t* size_first, const uint32_t* sf_base,
                        const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib,
-                       const uint64_t* sigs);
+                       const uint64_t* sigs, uint64_t* sigs_out = nullptr);
 // The device view of a built index (esig only when it was built with sigs).
 inline DevIndex dev_index(const IndexBuild& ib, bool with_sigs) {
   return DevIndex{ib.offsets.as<uint32_t>(), ib.set_ids.as<uint32_t>(), with_sigs ? ib.esig.as<uint64_t>() : nullptr,

@@ -188,20 +188,47 @@ constexpr int RS_ROUNDS = 16;
 constexpr uint32_t RS_TILE = RS_TPB * RS_ROUNDS;  // 4096 entries per tile
 constexpr int RS_WAVES = RS_TPB / WAVE;
 
+// One set's prefix entries at entry offset o, read IX_CHUNK tokens at a
+// time with independent loads; with sig != NULL all of the set's tokens are
+// read and its 64-bit XOR-parity signature (bit token mod 64) is stored too,
+// so a join that indexes the whole collection needs no separate signature
+// pass.
+constexpr uint32_t IX_CHUNK = 16;
+
+__device__ __forceinline__ void emit_set(const DevColl& c, uint32_t i, uint32_t l, uint32_t pl, uint32_t o,
+                                         uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
+                                         uint16_t* __restrict__ pos, uint64_t* __restrict__ sig) {
+  const uint32_t* t = c.tokens + c.starts[i];
+  const uint32_t len = sig ? l : pl;
+  uint64_t w = 0;
+  for (uint32_t k0 = 0; k0 < len; k0 += IX_CHUNK) {
+    uint32_t v[IX_CHUNK];
+#pragma unroll
+    for (uint32_t q = 0; q < IX_CHUNK; ++q) v[q] = k0 + q < len ? t[k0 + q] : 0u;
+#pragma unroll
+    for (uint32_t q = 0; q < IX_CHUNK; ++q) {
+      const uint32_t k = k0 + q;
+      if (k < pl) {
+        keys[o + k] = v[q];
+        vals[o + k] = i;
+        pos[o + k] = (uint16_t)min(k, 0xFFFFu);
+      }
+      if (k < len) w ^= 1ull << (v[q] & 63u);
+    }
+  }
+  if (sig) sig[i] = w;
+}
+
 __global__ void k_index_emit(DevColl c, uint32_t first, uint32_t last, const uint32_t* __restrict__ prefix_len,
                              const uint32_t* __restrict__ size_first, const uint32_t* __restrict__ cum,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint16_t* __restrict__ pos) {
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint16_t* __restrict__ pos,
+                             uint64_t* __restrict__ sig) {
   const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
   if (i > last) return;
   const uint32_t l = c.sizes[i];
   const uint32_t pl = prefix_len[l];
   const uint32_t o = cum[l] + (i - max(size_first[l], first)) * pl;
-  const uint32_t* t = c.tokens + c.starts[i];
-  for (uint32_t k = 0; k < pl; ++k) {
-    keys[o + k] = t[k];
-    vals[o + k] = i;
-    pos[o + k] = (uint16_t)min(k, 0xFFFFu);
-  }
+  emit_set(c, i, l, pl, o, keys, vals, pos, sig);
 }
 
 // Batched collections: set i belongs to join j = last j with join_first[j]
@@ -210,7 +237,7 @@ __global__ void k_index_emit_batch(DevColl c, const uint32_t* __restrict__ join_
                                    const uint32_t* __restrict__ prefix_len, const uint32_t* __restrict__ size_first,
                                    const uint32_t* __restrict__ sf_base, const uint32_t* __restrict__ cum,
                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                   uint16_t* __restrict__ pos) {
+                                   uint16_t* __restrict__ pos, uint64_t* __restrict__ sig) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.n) return;
   const uint32_t j = segment_of(join_first, n_joins, i);
@@ -218,12 +245,7 @@ __global__ void k_index_emit_batch(DevColl c, const uint32_t* __restrict__ join_
   const uint32_t pl = prefix_len[l];
   const uint32_t b = sf_base[j];
   const uint32_t o = cum[b + l] + (i - size_first[b + l]) * pl;
-  const uint32_t* t = c.tokens + c.starts[i];
-  for (uint32_t k = 0; k < pl; ++k) {
-    keys[o + k] = t[k];
-    vals[o + k] = i;
-    pos[o + k] = (uint16_t)min(k, 0xFFFFu);
-  }
+  emit_set(c, i, l, pl, o, keys, vals, pos, sig);
 }
 
 // Digit histogram of one tile: LDS atomics, then one column of the
@@ -407,12 +429,20 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
 // (keys[i-1], keys[i]] (every t is written once); and, for every entry, the
 // size and (when given) the signature of its set next to the token
 // position (one random read per entry here instead of per visit).
+// With counters != NULL the thread of entry n also prepares the join's work
+// counters (all zero, the index size in CNT_ENTRIES), which saves the join
+// a fill and a copy launch.
 __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, uint32_t universe,
                                uint32_t* __restrict__ offsets, DevColl c, const uint32_t* __restrict__ set_ids,
                                const uint16_t* __restrict__ positions, const uint64_t* __restrict__ sigs,
-                               uint64_t* __restrict__ esig, uint32_t* __restrict__ meta) {
+                               uint64_t* __restrict__ esig, uint32_t* __restrict__ meta,
+                               unsigned long long* __restrict__ counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
+  if (i == n && counters) {
+    for (int k = 0; k < CNT_WORDS; ++k) counters[k] = 0;
+    counters[CNT_ENTRIES] = n;
+  }
   const uint32_t prev = i == 0 ? 0u : keys[i - 1] + 1;   // first token not yet written
   const uint32_t cur = i == n ? universe : keys[i];      // tokens prev..cur start at entry i
   for (uint32_t t = prev; t <= cur && t <= universe; ++t) offsets[t] = i;
@@ -469,7 +499,7 @@ static void radix_pass(hipStream_t st, uint32_t n, uint32_t shift, uint32_t ntil
 }
 
 static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32_t universe, IndexBuild& ib,
-                            int cur, RadixPlan plan, const uint64_t* sigs) {
+                            int cur, RadixPlan plan, const uint64_t* sigs, unsigned long long* counters) {
   const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
   ib.hist.ensure(((size_t)ntiles << plan.bits) * sizeof(uint32_t));
   uint32_t* const kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
@@ -482,7 +512,7 @@ static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32
   }
   hipLaunchKernelGGL(k_index_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
                      ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(), sigs,
-                     ib.esig.as<uint64_t>(), ib.meta.as<uint32_t>());
+                     ib.esig.as<uint64_t>(), ib.meta.as<uint32_t>(), counters);
   HS_CHECK_HIP(hipGetLastError());
 }
 
@@ -492,7 +522,7 @@ static int emit_buffer(uint32_t passes) { return (passes % 2 == 1) ? 0 : 1; }
 
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                 IndexBuild& ib, const uint64_t* sigs) {
+                 IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters) {
   prepare(ib, universe, entries);
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && first <= last && n > 0) {
@@ -503,8 +533,8 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
     uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
     const uint32_t nsets = last - first + 1;
     hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
-                       size_first, cum, kb, vb, pb);
-    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs);
+                       size_first, cum, kb, vb, pb, sigs_out);
+    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs, counters);
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }
@@ -514,7 +544,8 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
                        const uint32_t* prefix_len, const uint32_t* size_first, const uint32_t* sf_base,
                        const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib,
-                       const uint64_t* sigs) {
+                       const uint64_t* sigs, uint64_t* sigs_out) {
+  unsigned long long* counters = nullptr;
   prepare(ib, universe, entries);
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && n > 0) {
@@ -524,8 +555,8 @@ void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_fi
     uint32_t* vb = cur ? ib.set_ids.as<uint32_t>() : ib.vals.as<uint32_t>();
     uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
     hipLaunchKernelGGL(k_index_emit_batch, dim3((c.n + 255) / 256), dim3(256), 0, st, c, join_first, n_joins,
-                       prefix_len, size_first, sf_base, cum, kb, vb, pb);
-    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs);
+                       prefix_len, size_first, sf_base, cum, kb, vb, pb, sigs_out);
+    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs, counters);
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
   }
