@@ -425,7 +425,11 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     // of the stack (one verification site keeps the register budget).
     const bool last_batch = ib + FG_ITEMCAP >= total_items;
     const uint32_t vis_end = total_vis + (last_batch ? FG_TPB : 0u);
-    // (64-bit signatures ride in the index entry and arrive with it)
+    // (64-bit signatures ride in the index entry and arrive with it).  A
+    // lane's next visit lies 256 further on: with long slices (long sets) it
+    // is in the same item or a near one, found by galloping from the current
+    // item; with short slices a bisection over the batch's items is cheaper.
+    const bool gallop = total_vis >= 64u * nb;
     uint32_t n_it = 0, n_sid = 0, n_meta = 0;
     uint64_t n_sig = 0;
     if (tid < total_vis) {
@@ -441,7 +445,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       const uint64_t ssig = n_sig;
       const uint32_t vn = vi + FG_TPB;
       if (vn < total_vis) {
-        n_it = segment_of(L.vis_off, nb, vn);
+        n_it = gallop ? segment_from(L.vis_off, nb, vn, n_it) : segment_of(L.vis_off, nb, vn);
         const uint32_t e = L.it_a[n_it] + (vn - L.vis_off[n_it]);
         n_sid = A.index.set_ids[e];
         n_meta = A.index.meta[e];
@@ -470,14 +474,19 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
             keep = lr + ls >= 2u * mo + (uint32_t)__popcll(L.psig[pl] ^ ssig);
           }
         } else if constexpr (kW > 1) {
+          // the popcount of a prefix of the words is a lower bound of the
+          // whole one, so the test stops at the first pair of words (128 bits)
+          // that already rules the visit out: unrelated long sets differ in
+          // about half of their bits and leave after a few words
           if (keep) {
             ++C.sig_tests;
             const uint64_t* sr = A.probe_sigs + (size_t)(T.p0 + pl) * kW;
             const uint64_t* ss = A.indexed_sigs + (size_t)sid * kW;
+            const uint32_t slack = lr + ls - 2u * mo;  // mo <= min(lr, ls)
             uint32_t pc = 0;
-#pragma unroll 4
-            for (int q = 0; q < kW; ++q) pc += (uint32_t)__popcll(sr[q] ^ ss[q]);
-            keep = lr + ls >= 2u * mo + pc;
+            for (int q = 0; q < kW && pc <= slack; q += 2)
+              pc += (uint32_t)__popcll(sr[q] ^ ss[q]) + (uint32_t)__popcll(sr[q + 1] ^ ss[q + 1]);
+            keep = pc <= slack;
           }
         }
       }
