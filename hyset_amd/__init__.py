@@ -105,6 +105,20 @@ _sig("hs_host_collection_free", None, C.POINTER(HostCollection))
 _sig("hs_read_collection_text", C.c_int, C.c_char_p, C.POINTER(HostCollection))
 _sig("hs_write_collection_text", C.c_int, C.c_char_p, _u32p, _u32p, _u32p, C.c_uint32)
 _sig("hs_write_pairs_text", C.c_int, C.c_char_p, _u32p, _u32p, _u32p, C.c_uint64)
+
+
+class PrepParams(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("q", C.c_uint32), ("lowercase", C.c_int32)]
+
+
+TOKENS_WORDS, TOKENS_QGRAMS, TOKENS_INTEGERS = 0, 1, 2
+_TOKEN_MODES = {"words": TOKENS_WORDS, "qgrams": TOKENS_QGRAMS, "integers": TOKENS_INTEGERS}
+_sig("hs_preprocess_text", C.c_int, C.c_char_p, C.c_uint64, C.POINTER(PrepParams), C.POINTER(HostCollection),
+     C.POINTER(_u32p))
+_sig("hs_preprocess_file", C.c_int, C.c_char_p, C.POINTER(PrepParams), C.POINTER(HostCollection), C.POINTER(_u32p))
+_sig("hs_free_array", None, C.c_void_p)
+_sig("hs_write_collection_binary", C.c_int, C.c_char_p, _u32p, _u32p, _u32p, C.c_uint32)
+_sig("hs_read_collection_binary", C.c_int, C.c_char_p, C.POINTER(HostCollection))
 _sig("hs_create", C.c_int, C.c_int, C.POINTER(_vp))
 _sig("hs_destroy", None, _vp)
 _sig("hs_last_error", C.c_char_p, _vp)
@@ -260,6 +274,58 @@ def write_collection_text(path, c: SetCollection) -> None:
     rc = _lib.hs_write_collection_text(str(path).encode(), _ptr(t), _ptr(st), _ptr(sz), c.n)
     if rc != HS_OK:
         raise HysetError(rc, f"cannot write {path}")
+
+
+def _prep(mode, q, lowercase):
+    return PrepParams(_TOKEN_MODES[mode] if isinstance(mode, str) else int(mode), int(q), int(bool(lowercase)))
+
+
+def _take_records(hc: HostCollection, rec) -> tuple:
+    try:
+        n = hc.n
+        lines = np.ctypeslib.as_array(rec, shape=(max(n, 1),))[:n].copy() if rec else np.zeros(0, np.uint32)
+    finally:
+        if rec:
+            _lib.hs_free_array(rec)
+    return _take(hc), lines
+
+
+def preprocess_text(text, mode="words", q=3, lowercase=False):
+    """Raw records (one per line) -> (collection, record_of): tokens by
+    word, character q-gram or integer id, repeats numbered, ids ranked by
+    ascending frequency from 1, sets sorted by size; record_of[i] is the line
+    of set i (hs_preprocess_text)."""
+    data = text.encode() if isinstance(text, str) else bytes(text)
+    hc, rec = HostCollection(), _u32p()
+    p = _prep(mode, q, lowercase)
+    rc = _lib.hs_preprocess_text(data, len(data), C.byref(p), C.byref(hc), C.byref(rec))
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_preprocess_text")
+    return _take_records(hc, rec)
+
+
+def preprocess_file(path, mode="words", q=3, lowercase=False):
+    hc, rec = HostCollection(), _u32p()
+    p = _prep(mode, q, lowercase)
+    rc = _lib.hs_preprocess_file(str(path).encode(), C.byref(p), C.byref(hc), C.byref(rec))
+    if rc != HS_OK:
+        raise HysetError(rc, f"cannot preprocess {path}")
+    return _take_records(hc, rec)
+
+
+def write_collection_binary(path, c: SetCollection) -> None:
+    t, st, sz = _u32(c.tokens), _u32(c.starts), _u32(c.sizes)
+    rc = _lib.hs_write_collection_binary(str(path).encode(), _ptr(t), _ptr(st), _ptr(sz), c.n)
+    if rc != HS_OK:
+        raise HysetError(rc, f"cannot write {path}")
+
+
+def read_collection_binary(path) -> SetCollection:
+    hc = HostCollection()
+    rc = _lib.hs_read_collection_binary(str(path).encode(), C.byref(hc))
+    if rc != HS_OK:
+        raise HysetError(rc, f"cannot read {path}")
+    return _take(hc)
 
 
 def write_pairs_text(path, r, s, overlap=None) -> None:

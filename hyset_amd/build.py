@@ -20,7 +20,8 @@ LIB = HERE / "libhyset_amd.so"
 DRIVER = HERE / "hyset_join"
 
 HIP_SOURCES = ["index.hip", "fgss.hip", "bitmap.hip", "verify.hip", "mlp.hip", "cost.hip"]
-CPP_SOURCES = ["api.cpp", "batch.cpp", "serve.cpp", "host_join.cpp", "bounds.cpp", "datagen.cpp", "host_tools.cpp"]
+CPP_SOURCES = ["api.cpp", "batch.cpp", "serve.cpp", "host_join.cpp", "bounds.cpp", "datagen.cpp", "host_tools.cpp",
+               "preprocess.cpp"]
 ARCH = "gfx950"
 
 

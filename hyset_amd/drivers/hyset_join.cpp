@@ -5,6 +5,15 @@
 //              [--algorithm fgss|bitmap|hybrid] [--bitmap 64]
 //              [--scenario 1] [--threads 32] [--device-memory 512M]
 //              [--output FILE] [--overlap]
+//              [--prepare words|qgrams|integers [--q 3] [--lowercase]]
+//              [--binary] [--save-binary FILE]
+//
+// --prepare reads raw records (one per line) and prepares them with
+// hs_preprocess_file (the step the reference leaves to external scripts,
+// README.md:14); pair ids are then set ids of the prepared collection, and
+// --output also writes FILE.records with the input line of every set id.
+// --binary reads the binary CSR cache instead of text; --save-binary writes
+// the (prepared) collection to it.
 //
 // The whole collection is one partition of blocks joined with itself
 // (src/fgss.cu:108-116).  With --output the pairs are written as "r s" lines
@@ -28,15 +37,18 @@ namespace {
 
 struct Options {
   std::string input, output, algorithm = "fgss", deviceMemory = "512M";
+  std::string prepare, saveBinary;
   double threshold = 0.95;
-  unsigned block = 10000, bitmap = 64, scenario = 1, threads = 32;
-  bool overlap = false;
+  unsigned block = 10000, bitmap = 64, scenario = 1, threads = 32, q = 3;
+  bool overlap = false, binary = false, lowercase = false;
 };
 
 [[noreturn]] void usage(const char* msg) {
   std::fprintf(stderr, "error: %s\nusage: hyset_join --input FILE [--threshold T] [--block B] "
                        "[--algorithm fgss|bitmap|hybrid] [--bitmap 64|128|256] [--scenario 1|2] [--threads N] "
-                       "[--device-memory 512M] [--output FILE] [--overlap]\n", msg);
+                       "[--device-memory 512M] [--output FILE] [--overlap] "
+                       "[--prepare words|qgrams|integers [--q N] [--lowercase]] [--binary] [--save-binary FILE]\n",
+               msg);
   std::exit(1);
 }
 
@@ -58,9 +70,17 @@ Options parse(int argc, char** argv) {
     else if (a == "--threads") o.threads = (unsigned)std::strtoul(val().c_str(), nullptr, 10);
     else if (a == "--device-memory") o.deviceMemory = val();
     else if (a == "--overlap") o.overlap = true;
+    else if (a == "--prepare") o.prepare = val();
+    else if (a == "--q") o.q = (unsigned)std::strtoul(val().c_str(), nullptr, 10);
+    else if (a == "--lowercase") o.lowercase = true;
+    else if (a == "--binary") o.binary = true;
+    else if (a == "--save-binary") o.saveBinary = val();
     else usage(("unknown option " + a).c_str());
   }
   if (o.input.empty()) usage("no input dataset given");
+  if (!o.prepare.empty() && o.prepare != "words" && o.prepare != "qgrams" && o.prepare != "integers")
+    usage("--prepare takes words, qgrams or integers");
+  if (!o.prepare.empty() && o.binary) usage("--prepare reads raw text, --binary a prepared cache");
   if (o.block == 0) usage("--block must be positive");
   return o;
 }
@@ -126,10 +146,36 @@ void hybrid_filter(const hs_host_collection& c, double t, ha::algorithms::hybrid
 int main(int argc, char** argv) {
   Options o = parse(argc, argv);
   hs_host_collection hc{};
-  if (hs_read_collection_text(o.input.c_str(), &hc) != HS_OK) {
+  uint32_t* record_of = nullptr;
+  int rc;
+  if (!o.prepare.empty()) {
+    hs_prep_params pp{};
+    pp.mode = o.prepare == "words" ? HS_TOKENS_WORDS : o.prepare == "qgrams" ? HS_TOKENS_QGRAMS : HS_TOKENS_INTEGERS;
+    pp.q = o.q;
+    pp.lowercase = o.lowercase;
+    rc = hs_preprocess_file(o.input.c_str(), &pp, &hc, &record_of);
+  } else {
+    rc = o.binary ? hs_read_collection_binary(o.input.c_str(), &hc) : hs_read_collection_text(o.input.c_str(), &hc);
+  }
+  if (rc != HS_OK) {
     std::fprintf(stderr, "Error: cannot read %s\n", o.input.c_str());
     return 1;
   }
+  if (!o.saveBinary.empty() &&
+      hs_write_collection_binary(o.saveBinary.c_str(), hc.tokens, hc.starts, hc.sizes, hc.n) != HS_OK) {
+    std::fprintf(stderr, "Error: cannot write %s\n", o.saveBinary.c_str());
+    return 1;
+  }
+  if (record_of && !o.output.empty()) {
+    FILE* f = std::fopen((o.output + ".records").c_str(), "w");
+    if (!f) {
+      std::fprintf(stderr, "Error: cannot write %s.records\n", o.output.c_str());
+      return 1;
+    }
+    for (uint32_t i = 0; i < hc.n; ++i) std::fprintf(f, "%u %u\n", i, record_of[i]);
+    std::fclose(f);
+  }
+  hs_free_array(record_of);
   try {
     ha::context ctx(0);
     std::vector<unsigned> toks(hc.tokens, hc.tokens + hc.n_tokens), starts(hc.starts, hc.starts + hc.n),

@@ -101,6 +101,44 @@ int hs_read_collection_text(const char* path, hs_host_collection* out);
 int hs_write_collection_text(const char* path, const uint32_t* tokens, const uint32_t* starts,
                              const uint32_t* sizes, uint32_t n);
 
+/* ----------------------------------------------------- dataset preparation */
+
+/* Raw records to the format above -- the preparation the reference leaves to
+ * the external scripts of the set-similarity-join benchmark it points to
+ * (README.md:14; the format it then assumes, collection.hpp:117-135).  One
+ * record per line; tokens are whitespace-separated words (HS_TOKENS_WORDS),
+ * the character q-grams of the line padded with q-1 marks on both sides
+ * (HS_TOKENS_QGRAMS, q from 1 to 64, default 3) or integer ids
+ * (HS_TOKENS_INTEGERS).  The k-th occurrence of a token inside a record is
+ * a token of its own, so records are sets.  Token ids are ranked by
+ * ascending number of records holding them (ties: first appearance),
+ * starting at 1; tokens ascend inside each set; sets are sorted by (size,
+ * tokens); records without tokens are dropped.  record_of (may be NULL)
+ * receives a malloc'd array: the 0-based line number of every set (free it
+ * with hs_free_array). */
+enum { HS_TOKENS_WORDS = 0, HS_TOKENS_QGRAMS = 1, HS_TOKENS_INTEGERS = 2 };
+typedef struct {
+  int32_t mode;      /* HS_TOKENS_* */
+  uint32_t q;        /* q-gram length (0: 3) */
+  int32_t lowercase; /* fold ASCII letters to lower case first */
+} hs_prep_params;
+
+int hs_preprocess_text(const char* text, uint64_t len, const hs_prep_params* params, hs_host_collection* out,
+                       uint32_t** record_of);
+int hs_preprocess_file(const char* path, const hs_prep_params* params, hs_host_collection* out,
+                       uint32_t** record_of);
+void hs_free_array(void* p);
+
+/* Binary CSR cache of a collection: the 8 bytes "HSCSR1\0\0", u32 n, u32
+ * universe, u64 n_tokens, then sizes u32[n] and tokens u32[n_tokens] (sets
+ * in order, dense), little-endian.  The reader checks the magic, that the
+ * sizes add up to n_tokens, that no token reaches the universe and that the
+ * file ends there; anything else is HS_E_INVALID.  starts may be NULL for a
+ * dense CSR. */
+int hs_write_collection_binary(const char* path, const uint32_t* tokens, const uint32_t* starts,
+                               const uint32_t* sizes, uint32_t n);
+int hs_read_collection_binary(const char* path, hs_host_collection* out);
+
 /* Writes pairs as "r s\n" lines (include/hyset/output.hpp:39-76), or
  * "r s overlap\n" when overlap is not NULL. */
 int hs_write_pairs_text(const char* path, const uint32_t* r, const uint32_t* s, const uint32_t* overlap,

@@ -63,3 +63,67 @@ def test_cpp_driver(tmp_path, algo, extra):
         res = subprocess.run([str(DRIVER), "--input", str(col), "--threshold", str(t), "--algorithm", algo, *extra],
                              capture_output=True, text=True, timeout=120)
         assert res.returncode == 0 and f"Result: {len(got)}" in res.stdout
+
+
+def _names(n, seed):
+    """Strings of 8-20 letters, 30% of them copies of earlier ones with 1-2
+    character edits (the string-matching use of the join)."""
+    import random
+    rnd = random.Random(seed)
+    abc = "abcdefghijklmnopqrstuvwxyz"
+    out = []
+    for _ in range(n):
+        if out and rnd.random() < 0.3:
+            s = list(rnd.choice(out))
+            for _ in range(rnd.randint(1, 2)):
+                k = rnd.randrange(len(s))
+                s[k] = rnd.choice(abc)
+            out.append("".join(s))
+        else:
+            out.append("".join(rnd.choice(abc) for _ in range(rnd.randint(8, 20))))
+    return out
+
+
+@pytest.mark.parametrize("algo", ["fgss", "bitmap"])
+def test_prepared_qgram_join(ctx, algo):
+    """Raw strings prepared into 3-gram sets join on the device exactly as in
+    the oracle."""
+    import oracle as O
+    c, rec = H.preprocess_text("\n".join(_names(4000, 5)), mode="qgrams", q=3)
+    d = ctx.upload(c)
+    try:
+        for t in (0.6, 0.8):
+            ctx.set_threshold(t)
+            want = O.self_join(c, t)
+            assert want.count > 20
+            got = triples(*ctx.join(d, algo=algo))
+            assert got == list(zip(want.r.tolist(), want.s.tolist(), want.overlap.tolist())), t
+    finally:
+        d.free()
+
+
+def test_cpp_driver_prepare_and_binary(tmp_path):
+    """The driver prepares raw text (--prepare), saves and reloads the binary
+    cache (--save-binary, --binary), and maps set ids back to input lines."""
+    import oracle as O
+    names = _names(3000, 6)
+    raw = tmp_path / "names.txt"
+    raw.write_text("\n".join(names) + "\n")
+    c, rec = H.preprocess_text("\n".join(names), mode="qgrams", q=3)
+    want = O.self_join(c, 0.7)
+    want_t = sorted(zip(want.r.tolist(), want.s.tolist()))
+    out, cache = tmp_path / "pairs.txt", tmp_path / "names.bin"
+    res = subprocess.run([str(DRIVER), "--input", str(raw), "--prepare", "qgrams", "--q", "3", "--threshold", "0.7",
+                          "--save-binary", str(cache), "--output", str(out)], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stderr
+    got = sorted(tuple(int(x) for x in l.split()[:2]) for l in out.read_text().splitlines() if l.strip())
+    assert got == want_t
+    recs = [tuple(int(x) for x in l.split()) for l in (tmp_path / "pairs.txt.records").read_text().splitlines()]
+    assert recs == list(enumerate(rec.tolist()))
+    out2 = tmp_path / "pairs2.txt"
+    res = subprocess.run([str(DRIVER), "--input", str(cache), "--binary", "--threshold", "0.7", "--output", str(out2)],
+                         capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stderr
+    got2 = sorted(tuple(int(x) for x in l.split()[:2]) for l in out2.read_text().splitlines() if l.strip())
+    assert got2 == want_t
