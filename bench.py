@@ -18,6 +18,7 @@ import json
 import os
 import statistics
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -406,6 +407,26 @@ def main() -> int:
                     extra[f"fgss_t{tt}_tile_plan"] = int(ctx.stats()["tile_plan"])
                 log(f"fgss t={tt}{plan}: {c} pairs {statistics.median(ms):.3f} ms (plan {ctx.stats()['tile_plan']})")
             extra[f"fgss_t{tt}_pairs"] = c
+            if tt == 0.6:
+                # the result sink on a large result: the pairs leave the device in
+                # pinned chunks, into host arrays or streamed into a text file
+                ctx.join(d, algo="fgss", count_only=True)
+                cp, tx = [], []
+                with tempfile.TemporaryDirectory() as td:
+                    tf = Path(td) / "pairs.txt"
+                    for _ in range(3):
+                        a = time.perf_counter()
+                        ctx.last_result(c)
+                        cp.append((time.perf_counter() - a) * 1e3)
+                        a = time.perf_counter()
+                        ctx.write_last_result_text(tf)
+                        tx.append((time.perf_counter() - a) * 1e3)
+                    extra["sink_t0.6_text_bytes"] = tf.stat().st_size
+                extra["sink_t0.6_copy_ms"] = round(statistics.median(cp), 3)
+                extra["sink_t0.6_copy_gbps"] = round(12 * c / (statistics.median(cp) * 1e-3) / 1e9, 2)
+                extra["sink_t0.6_text_ms"] = round(statistics.median(tx), 3)
+                log(f"result sink t=0.6: {c} pairs to host arrays {extra['sink_t0.6_copy_ms']} ms "
+                    f"({extra['sink_t0.6_copy_gbps']} GB/s), to text {extra['sink_t0.6_text_ms']} ms")
         ctx.set_threshold(t)
         # the other tile plans: the upload-time tiles on a persistent grid,
         # and work-balanced tiles planned on the device (item-slice pass)

@@ -130,6 +130,9 @@ _sig("hs_join", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.c_
 _sig("hs_verify_candidates", C.c_int, _vp, _vp, _vp, _u32p, _u32p, _u32p, C.c_uint32, C.c_int32,
      C.POINTER(Result))
 _sig("hs_copy_last_result", C.c_int, _vp, C.POINTER(Result))
+PairSink = C.CFUNCTYPE(C.c_int, C.c_void_p, _u32p, _u32p, _u32p, C.c_uint64)
+_sig("hs_stream_last_result", C.c_int, _vp, C.c_uint64, C.c_int32, PairSink, C.c_void_p)
+_sig("hs_write_last_result_text", C.c_int, _vp, C.c_char_p, C.c_int32)
 _sig("hs_join_host", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.c_uint32, C.POINTER(Result))
 _u32pp = C.POINTER(_u32p)
 _sig("hs_upload_batch", C.c_int, _vp, C.c_uint32, _u32pp, _u32pp, _u32pp, _u32p, C.POINTER(C.c_uint64),
@@ -412,6 +415,29 @@ class Context:
         res = Result(0, count, _ptr(r), _ptr(s), _ptr(ov), 0)
         self._check(_lib.hs_copy_last_result(self._h, C.byref(res)), "hs_copy_last_result")
         return r[:count], s[:count], ov[:count]
+
+    def stream_last_result(self, consume, *, chunk=0, overlap=True):
+        """Feeds the pairs of the last join to consume(r, s, overlap or None)
+        chunk by chunk (hs_stream_last_result); the arrays are copies.  A
+        falsy return from consume continues, anything else stops the
+        stream with HysetError."""
+        def sink(_user, r, s, ov, n):
+            n = int(n)
+            view = lambda p: np.ctypeslib.as_array(p, shape=(n,)).copy() if n else np.empty(0, np.uint32)
+            try:
+                stop = consume(view(r), view(s), view(ov) if ov else None)
+            except Exception:  # an exception must not cross the C frame
+                return 1
+            return 1 if stop else 0
+        cb = PairSink(sink)
+        self._check(_lib.hs_stream_last_result(self._h, int(chunk), 1 if overlap else 0, cb, None),
+                    "hs_stream_last_result")
+
+    def write_last_result_text(self, path, *, overlap=False) -> None:
+        """Streams the pairs of the last join into a text file of "r s" (or
+        "r s overlap") lines."""
+        self._check(_lib.hs_write_last_result_text(self._h, str(path).encode(), 1 if overlap else 0),
+                    "hs_write_last_result_text")
 
     def upload_batch(self, colls) -> "DeviceCollection":
         """One batched device collection from a list of SetCollections."""

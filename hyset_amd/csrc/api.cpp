@@ -29,6 +29,7 @@
 #include <vector>
 
 #include "ctx.h"
+#include "text_out.h"
 
 using namespace hs;
 
@@ -52,6 +53,63 @@ void hs::DevBuf::release() {
   if (ptr) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
+}
+
+// ------------------------------------------------------- result stream
+
+void hs::stream_pairs(hs_ctx* ctx, uint64_t count, bool with_overlap, uint64_t chunk,
+                      const std::function<void(const uint32_t*, const uint32_t*, const uint32_t*, uint64_t,
+                                               uint64_t)>& consume) {
+  if (count == 0) return;
+  hipStream_t st = ctx->stream;
+  chunk = std::min(chunk ? chunk : kStreamChunk, count);
+  const size_t arrays = with_overlap ? 3 : 2;
+  for (int k = 0; k < 2; ++k) {
+    if (!ctx->stage_ev[k]) HS_CHECK_HIP(hipEventCreateWithFlags(&ctx->stage_ev[k], hipEventDisableTiming));
+  }
+  const uint64_t nch = (count + chunk - 1) / chunk;
+  // a slot holds [r | s | overlap] of one chunk; the second slot is needed
+  // only when there is a second chunk
+  for (int k = 0; k < (nch > 1 ? 2 : 1); ++k) {
+    if (ctx->stage[k].bytes < chunk * 4 * arrays) {
+      HS_CHECK_HIP(hipStreamSynchronize(st));  // nothing may still be copying into the old buffer
+      ctx->stage[k].ensure(chunk * 4 * arrays);
+    }
+  }
+  auto issue = [&](uint64_t c) {
+    const uint64_t off = c * chunk, n = std::min(chunk, count - off);
+    uint32_t* dst = ctx->stage[c & 1].as<uint32_t>();
+    HS_CHECK_HIP(hipMemcpyAsync(dst, ctx->out_r.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost, st));
+    HS_CHECK_HIP(hipMemcpyAsync(dst + chunk, ctx->out_s.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost, st));
+    if (with_overlap)
+      HS_CHECK_HIP(
+          hipMemcpyAsync(dst + 2 * chunk, ctx->out_ov.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost, st));
+    HS_CHECK_HIP(hipEventRecord(ctx->stage_ev[c & 1], st));
+  };
+  try {
+    issue(0);
+    for (uint64_t c = 0; c < nch; ++c) {
+      // slot (c+1)&1 held chunk c-1, which consume has finished with
+      if (c + 1 < nch) issue(c + 1);
+      HS_CHECK_HIP(hipEventSynchronize(ctx->stage_ev[c & 1]));
+      const uint32_t* b = ctx->stage[c & 1].as<uint32_t>();
+      consume(b, b + chunk, with_overlap ? b + 2 * chunk : nullptr, c * chunk, std::min(chunk, count - c * chunk));
+    }
+  } catch (...) {
+    (void)hipStreamSynchronize(st);  // no copy may land in a slot after the caller has moved on
+    throw;
+  }
+}
+
+void hs::copy_pairs(hs_ctx* ctx, uint64_t count, hs_result* res) {
+  if (count == 0) return;
+  HS_REQUIRE(res->r && res->s, HS_E_INVALID, "result arrays are NULL");
+  stream_pairs(ctx, count, res->overlap != nullptr, 0,
+               [&](const uint32_t* r, const uint32_t* s, const uint32_t* ov, uint64_t first, uint64_t n) {
+                 std::memcpy(res->r + first, r, n * 4);
+                 std::memcpy(res->s + first, s, n * 4);
+                 if (ov) std::memcpy(res->overlap + first, ov, n * 4);
+               });
 }
 
 namespace {
@@ -302,6 +360,8 @@ void hs_destroy(hs_ctx* ctx) {
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
   for (auto& sl : ctx->slots)
     if (sl.done) (void)hipEventDestroy(sl.done);
+  for (auto& e : ctx->stage_ev)
+    if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -722,6 +782,36 @@ int hs_copy_last_result(hs_ctx* ctx, hs_result* result) {
     result->count = ctx->last_count;
     HS_REQUIRE(ctx->last_count <= result->capacity, HS_E_OVERFLOW, "result capacity too small");
     copy_pairs(ctx, ctx->last_count, result);
+  });
+}
+
+int hs_stream_last_result(hs_ctx* ctx, uint64_t chunk_pairs, int32_t with_overlap, hs_pair_sink sink, void* user) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    HS_REQUIRE(sink, HS_E_INVALID, "NULL sink");
+    HS_REQUIRE(ctx->device >= 0, HS_E_INVALID, "a host-only context holds no device result");
+    stream_pairs(ctx, ctx->last_count, with_overlap != 0, chunk_pairs,
+                 [&](const uint32_t* r, const uint32_t* s, const uint32_t* ov, uint64_t first, uint64_t n) {
+                   const int rc = sink(user, r, s, ov, n);
+                   HS_REQUIRE(rc == 0, HS_E_INVALID,
+                              "the sink stopped the stream at pair " + std::to_string(first) + " (it returned " +
+                                  std::to_string(rc) + ")");
+                 });
+  });
+}
+
+int hs_write_last_result_text(hs_ctx* ctx, const char* path, int32_t with_overlap) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    HS_REQUIRE(path, HS_E_INVALID, "NULL path");
+    HS_REQUIRE(ctx->device >= 0, HS_E_INVALID, "a host-only context holds no device result");
+    hs::PairTextWriter w(path);
+    HS_REQUIRE(w.opened(), HS_E_INVALID, std::string("cannot open ") + path);
+    stream_pairs(ctx, ctx->last_count, with_overlap != 0, 0,
+                 [&](const uint32_t* r, const uint32_t* s, const uint32_t* ov, uint64_t, uint64_t n) {
+                   w.add(r, s, ov, n);
+                 });
+    HS_REQUIRE(w.close(), HS_E_INVALID, std::string("writing ") + path + " failed");
   });
 }
 

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -100,6 +101,8 @@ struct hs_ctx {
   HybridSlot slots[2];
   std::mutex submit_mu;
   std::vector<uint32_t> h_offsets, h_set_ids;  // host copy of the index (hybrid)
+  PinnedBuf stage[2];            // pinned chunks of the result stream
+  hipEvent_t stage_ev[2] = {};   // chunk k has landed in stage[k]
   int num_cus = 256;
   DevBuf batch_queue;  // per-XCD tile counters of the batched launch
   DevBuf feat;         // partition-feature accumulators (hs_features)
@@ -216,16 +219,19 @@ inline void read_counters(hs_ctx* ctx, unsigned long long* dst) {
   std::memcpy(dst, ctx->h_counters.ptr, hs::CNT_WORDS * sizeof(unsigned long long));
 }
 
-inline void copy_pairs(hs_ctx* ctx, uint64_t count, hs_result* res) {
-  if (count == 0) return;
-  HS_REQUIRE(res->r && res->s, HS_E_INVALID, "result arrays are NULL");
-  HS_CHECK_HIP(hipMemcpyAsync(res->r, ctx->out_r.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  HS_CHECK_HIP(hipMemcpyAsync(res->s, ctx->out_s.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-  if (res->overlap)
-    HS_CHECK_HIP(hipMemcpyAsync(res->overlap, ctx->out_ov.ptr, count * sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                ctx->stream));
-  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-}
+// Pairs per chunk of the result stream (12 bytes each with the overlap).
+constexpr uint64_t kStreamChunk = 1ull << 18;
+
+// Sends pairs [0, count) of the device result buffers to `consume` in chunks
+// of at most `chunk` pairs (0: kStreamChunk), staged through two pinned
+// buffers: chunk k+1 is in flight on the context stream while consume sees
+// chunk k.  consume(r, s, overlap or NULL, first pair index, n).
+void stream_pairs(hs_ctx* ctx, uint64_t count, bool with_overlap, uint64_t chunk,
+                  const std::function<void(const uint32_t*, const uint32_t*, const uint32_t*, uint64_t, uint64_t)>&
+                      consume);
+
+// Pairs [0, count) into the caller's arrays of res (through stream_pairs).
+void copy_pairs(hs_ctx* ctx, uint64_t count, hs_result* res);
 
 inline hs_range clamp_range(const hs_range* r, uint32_t n) {
   hs_range out{0, 0};

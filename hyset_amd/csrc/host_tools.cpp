@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "hyset_amd.h"
+#include "text_out.h"
 
 extern "C" {
 
@@ -96,13 +97,10 @@ int hs_write_collection_text(const char* path, const uint32_t* tokens, const uin
 int hs_write_pairs_text(const char* path, const uint32_t* r, const uint32_t* s, const uint32_t* overlap,
                         uint64_t count) {
   if (!path || (count && (!r || !s))) return HS_E_INVALID;
-  FILE* f = std::fopen(path, "wb");
-  if (!f) return HS_E_INVALID;
-  for (uint64_t i = 0; i < count; ++i) {
-    if (overlap) std::fprintf(f, "%u %u %u\n", r[i], s[i], overlap[i]);
-    else std::fprintf(f, "%u %u\n", r[i], s[i]);
-  }
-  return std::fclose(f) == 0 ? HS_OK : HS_E_INVALID;
+  hs::PairTextWriter w(path);
+  if (!w.opened()) return HS_E_INVALID;
+  w.add(r, s, overlap, count);
+  return w.close() ? HS_OK : HS_E_INVALID;
 }
 
 }  // extern "C"

@@ -409,8 +409,28 @@ int hs_mlp_train(hs_mlp* mlp, const float* X, const float* Y, uint32_t n, uint32
 int hs_split_run(hs_ctx* ctx, uint32_t n_units, hs_coll* const* colls, const int32_t* technique,
                  uint32_t cpu_threads, uint64_t* counts, double* unit_ms, double* wall_ms);
 
-/* Copies the pairs of the last join into result (count_only ignored). */
+/* Copies the pairs of the last join into result (count_only ignored).  The
+ * pairs travel in chunks through pinned host buffers, the next chunk's copy
+ * in flight while the previous one is moved into the caller's arrays. */
 int hs_copy_last_result(hs_ctx* ctx, hs_result* result);
+
+/* Streaming result sink (the reference's output handlers,
+ * include/hyset/output.hpp:13-93, fed chunk by chunk instead of from one
+ * host array).  The pairs of the last join on the context reach `sink` in
+ * chunks of at most chunk_pairs pairs (0: 262144), in the order of the
+ * device result buffer; overlap is NULL unless with_overlap.  The arrays are
+ * pinned staging buffers valid only during the call; the next chunk is being
+ * copied while the sink runs.  A nonzero return from the sink stops the
+ * stream and the call returns HS_E_INVALID.  The sink must not call into the
+ * same context. */
+typedef int (*hs_pair_sink)(void* user, const uint32_t* r, const uint32_t* s, const uint32_t* overlap,
+                            uint64_t n);
+int hs_stream_last_result(hs_ctx* ctx, uint64_t chunk_pairs, int32_t with_overlap, hs_pair_sink sink, void* user);
+
+/* Writes the pairs of the last join as hs_write_pairs_text does, streamed
+ * from the device in pinned chunks (formatting overlaps the next chunk's
+ * copy); no host array of the whole result is built. */
+int hs_write_last_result_text(hs_ctx* ctx, const char* path, int32_t with_overlap);
 
 int hs_get_stats(const hs_ctx* ctx, hs_stats* out);
 

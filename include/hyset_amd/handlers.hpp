@@ -135,6 +135,18 @@ class device_collection {
 }  // namespace collection
 
 namespace detail {
+// Appends the pairs of the last join on ctx to a pairs sink, streamed from
+// the device in pinned chunks (hs_stream_last_result).
+inline int append_chunk(void* user, const uint32_t* r, const uint32_t* s, const uint32_t* ov, uint64_t n) {
+  auto& pairs = static_cast<output::pairs_handler*>(user)->pairs;
+  for (uint64_t i = 0; i < n; ++i) pairs.push_back({r[i], s[i], ov[i]});
+  return 0;
+}
+inline void deliver_pairs(hs_ctx* ctx, uint64_t count, output::pairs_handler& p) {
+  p.pairs.reserve(p.pairs.size() + count);
+  check(hs_stream_last_result(ctx, 0, 1, append_chunk, &p), ctx, "hs_stream_last_result");
+}
+
 // Runs one device join and moves its result into the sink the reference
 // would have downcast to (device_algorithms.hpp:193-221).
 inline void run_join(context& ctx, collection::device_collection& indexed, collection::device_collection& probe,
@@ -152,11 +164,7 @@ inline void run_join(context& ctx, collection::device_collection& indexed, colle
   }
   auto* p = dynamic_cast<output::pairs_handler*>(out.get());
   if (!p) throw error(HS_E_INVALID, "pairs mode needs a pairs_handler");
-  std::vector<uint32_t> r(res.count), s(res.count), ov(res.count);
-  hs_result cp{res.count, res.count, r.data(), s.data(), ov.data(), 0};
-  check(hs_copy_last_result(ctx.get(), &cp), ctx.get(), "hs_copy_last_result");
-  p->pairs.reserve(p->pairs.size() + res.count);
-  for (uint64_t i = 0; i < res.count; ++i) p->pairs.push_back({r[i], s[i], ov[i]});
+  deliver_pairs(ctx.get(), res.count, *p);
 }
 
 // The self-join block-pair rule of length_filter (device_algorithms.hpp:21-53):
@@ -292,11 +300,7 @@ struct device_handler {
       dynamic_cast<output::count_handler&>(*output_).count += count;
       return;
     }
-    std::vector<uint32_t> r(count), s(count), ov(count);
-    hs_result cp{count, count, r.data(), s.data(), ov.data(), 0};
-    check(hs_copy_last_result(ctx_.get(), &cp), ctx_.get(), "hs_copy_last_result");
-    auto& p = dynamic_cast<output::pairs_handler&>(*output_);
-    for (uint64_t i = 0; i < count; ++i) p.pairs.push_back({r[i], s[i], ov[i]});
+    detail::deliver_pairs(ctx_.get(), count, dynamic_cast<output::pairs_handler&>(*output_));
   }
   context& ctx_;
   collection::device_collection& indexed_;

@@ -5,6 +5,7 @@ import re
 from pathlib import Path
 
 import numpy as np
+import pytest
 
 import hyset_amd as H
 
@@ -95,6 +96,27 @@ def test_text_round_trip(tmp_path):
     assert pairs.read_text() == "3 1\n4 2\n"
     H.write_pairs_text(pairs, [3], [1], [7])
     assert pairs.read_text() == "3 1 7\n"
+
+
+def test_pairs_text_writer_formatting(tmp_path):
+    """The buffered writer's own decimal formatting against Python's, over
+    every digit count, the extremes of uint32 and more than one 1 MiB buffer."""
+    rng = np.random.default_rng(3)
+    edge = np.array([0, 1, 9, 10, 99, 100, 4294967295, 4294967294, 1000000000, 999999999], dtype=np.uint32)
+    r = np.concatenate([edge, rng.integers(0, 2**32, 60000, dtype=np.uint64).astype(np.uint32)])
+    s = np.concatenate([edge[::-1], (rng.integers(0, 2**32, 60000, dtype=np.uint64) >> rng.integers(0, 32, 60000,
+                                                                                              dtype=np.uint64)
+                                     ).astype(np.uint32)])
+    ov = (r % 1000).astype(np.uint32)
+    p = tmp_path / "p.txt"
+    H.write_pairs_text(p, r, s, ov)
+    assert p.read_text() == "".join(f"{a} {b} {c}\n" for a, b, c in zip(r.tolist(), s.tolist(), ov.tolist()))
+    H.write_pairs_text(p, r, s)
+    assert p.read_text() == "".join(f"{a} {b}\n" for a, b in zip(r.tolist(), s.tolist()))
+    H.write_pairs_text(p, [], [])
+    assert p.read_text() == ""
+    with pytest.raises(H.HysetError):
+        H.write_pairs_text(tmp_path / "no" / "such" / "dir.txt", [1], [2])
 
 
 def test_host_join_matches_oracle():
