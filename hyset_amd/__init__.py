@@ -130,6 +130,25 @@ _sig("hs_join", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.c_
 _sig("hs_verify_candidates", C.c_int, _vp, _vp, _vp, _u32p, _u32p, _u32p, C.c_uint32, C.c_int32,
      C.POINTER(Result))
 _sig("hs_copy_last_result", C.c_int, _vp, C.POINTER(Result))
+
+
+class Coprocess(C.Structure):
+    _fields_ = [("mode", C.c_int32), ("algo", C.c_int32), ("cpu_threads", C.c_uint32), ("cpu_block", C.c_uint32),
+                ("device_min_claim", C.c_uint32), ("cpu_share", C.c_double), ("tune", Tune)]
+
+
+class CoprocessStats(C.Structure):
+    _fields_ = [("device_pairs", C.c_uint64), ("host_pairs", C.c_uint64), ("device_probes", C.c_uint32),
+                ("host_probes", C.c_uint32), ("device_claims", C.c_uint32), ("host_claims", C.c_uint32),
+                ("split", C.c_uint32), ("host_threads", C.c_uint32), ("index_ms", C.c_double),
+                ("copy_ms", C.c_double), ("device_ms", C.c_double), ("host_ms", C.c_double),
+                ("wall_ms", C.c_double)]
+
+
+COPROC_QUEUE, COPROC_STATIC = 0, 1
+_COPROC_MODES = {"queue": COPROC_QUEUE, "static": COPROC_STATIC}
+_sig("hs_join_coprocess", C.c_int, _vp, _vp, _vp, C.POINTER(Coprocess), C.POINTER(Result),
+     C.POINTER(CoprocessStats))
 PairSink = C.CFUNCTYPE(C.c_int, C.c_void_p, _u32p, _u32p, _u32p, C.c_uint64)
 _sig("hs_stream_last_result", C.c_int, _vp, C.c_uint64, C.c_int32, PairSink, C.c_void_p)
 _sig("hs_write_last_result_text", C.c_int, _vp, C.c_char_p, C.c_int32)
@@ -386,6 +405,25 @@ class Context:
         if count_only:
             return int(res.count)
         return self.last_result(int(res.count))
+
+    def join_coprocess(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, mode="queue",
+                       algo="fgss", cpu_threads=0, cpu_block=0, device_min_claim=0, cpu_share=0.0,
+                       tune: Tune | None = None, count_only=False):
+        """One join shared by the device and host threads (hs_join_coprocess).
+        Returns ((r, s, overlap) or the pair count, stats dict)."""
+        probe = indexed if probe is None else probe
+        opt = Coprocess(_COPROC_MODES[mode] if isinstance(mode, str) else int(mode),
+                        ALGOS[algo] if isinstance(algo, str) else int(algo), int(cpu_threads), int(cpu_block),
+                        int(device_min_claim), float(cpu_share), tune if tune is not None else Tune())
+        res = Result()
+        res.count_only = 1
+        st = CoprocessStats()
+        self._check(_lib.hs_join_coprocess(self._h, indexed._h, probe._h, C.byref(opt), C.byref(res), C.byref(st)),
+                    "hs_join_coprocess")
+        info = {k: getattr(st, k) for k, _ in CoprocessStats._fields_}
+        if count_only:
+            return int(res.count), info
+        return self.last_result(int(res.count)), info
 
     def join_host(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, threads=1,
                   indexed_range=None, probe_range=None):

@@ -362,6 +362,14 @@ void hs_destroy(hs_ctx* ctx) {
     if (sl.done) (void)hipEventDestroy(sl.done);
   for (auto& e : ctx->stage_ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto& e : ctx->claim_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (ctx->index_built) (void)hipEventDestroy(ctx->index_built);
+  if (ctx->index_copied) (void)hipEventDestroy(ctx->index_copied);
+  if (ctx->copy_stream) {
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    (void)hipStreamDestroy(ctx->copy_stream);
+  }
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -470,6 +478,200 @@ int hs_set_threshold(hs_ctx* ctx, int32_t sim, double threshold, int32_t bounds_
   });
 }
 
+}  // extern "C"
+
+// ------------------------------------------- device join, in two steps
+
+void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_range ir, hs_range pr, int32_t algo,
+                             const hs_tune& tune, bool with_index, DevicePlan& P) {
+  hipStream_t st = ctx->stream;
+  P = DevicePlan{};
+  const bool self = indexed == probe;
+  P.self = self;
+  P.ir = ir;
+  // ---- parity signatures for the fgss signature test (timed with the index)
+  bool sig_in_emit = false;  // the index's emit pass computes them (self-join over the whole collection)
+  if (algo == HS_ALGO_FGSS && !(tune.flags & HS_TUNE_NO_SIGNATURE) && !range_empty(ir) && !range_empty(pr)) {
+    P.sig_words = hs::fgss_signature_words(indexed, tune.signature_bits);
+    sig_in_emit = P.sig_words == 1 && self && ir.first == 0 && ir.last + 1 == indexed->n;
+    if (sig_in_emit) {
+      indexed->sigs.ensure((size_t)std::max<uint32_t>(indexed->n, 1) * 8);
+      indexed->sig_words = 1;
+    } else {
+      hs::compute_signatures(st, indexed, P.sig_words);
+      if (!self) hs::compute_signatures(st, probe, P.sig_words);
+    }
+  }
+
+  // ---- index over the indexed range (not needed by the bitmap technique)
+  const uint32_t* prefix_tab = self ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
+  if ((algo != HS_ALGO_BITMAP || with_index) && !range_empty(ir)) {
+    // entries per size class of the range (sets are sorted by size)
+    const auto& ptab = self ? ctx->tables.midprefix : ctx->tables.maxprefix;
+    const auto& sf = indexed->h_size_first;
+    const uint32_t nl = (uint32_t)sf.size();  // maxlen + 2
+    ctx->h_cum.ensure((size_t)nl * 4);
+    ctx->d_cum.ensure((size_t)nl * 4);
+    uint32_t* cum = ctx->h_cum.as<uint32_t>();
+    uint64_t acc = 0;
+    for (uint32_t l = 0; l + 1 < nl; ++l) {
+      cum[l] = (uint32_t)acc;
+      const uint32_t lo = std::max(ir.first, sf[l]);
+      const uint32_t hi = std::min(ir.last + 1, sf[l + 1]);
+      if (hi > lo) acc += (uint64_t)(hi - lo) * ptab[l];
+      HS_REQUIRE(acc < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
+    }
+    cum[nl - 1] = (uint32_t)acc;
+    P.index_entries = acc;
+    HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, cum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
+    hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
+                    ctx->d_cum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
+                    P.sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr,
+                    sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
+                    algo == HS_ALGO_FGSS ? ctx->counters.as<unsigned long long>() : nullptr);
+    P.counters_ready = algo == HS_ALGO_FGSS && acc > 0 && indexed->n > 0;
+  } else {
+    ctx->ib.universe = 0;
+  }
+}
+
+uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, DevicePlan& P, hs_range pr,
+                                int32_t algo, const hs_tune& tune) {
+  hipStream_t st = ctx->stream;
+  const bool self = P.self;
+  const hs_range ir = P.ir;
+  uint32_t tile_plan = 0;
+  if (range_empty(pr) || range_empty(ir)) {
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
+  } else if (algo == HS_ALGO_FGSS) {
+    hs::FgssArgs a{};
+    a.probe = probe->dev();
+    a.indexed = indexed->dev();
+    a.index = hs::dev_index(ctx->ib, P.sig_words == 1);
+    a.bounds = ctx->dev_bounds;
+    a.size_first = indexed->size_first.as<uint32_t>();
+    a.size_first_len = (uint32_t)indexed->h_size_first.size();
+    a.probe_first = pr.first;
+    a.probe_last = pr.last;
+    a.idx_first = ir.first;
+    a.idx_last = ir.last;
+    a.self = self;
+    a.out = dev_out(ctx);
+    a.sig_words = P.sig_words;
+    a.probe_sigs = P.sig_words ? probe->sigs.as<uint64_t>() : nullptr;
+    a.indexed_sigs = P.sig_words ? indexed->sigs.as<uint64_t>() : nullptr;
+    const uint32_t ppt = tune.probes_per_tile;
+    if (ppt == 0) {
+      // the collection's tile plan, clipped to the probe range
+      const auto& T = probe->h_tiles;
+      const uint32_t ft = (uint32_t)(std::upper_bound(T.begin(), T.end(), pr.first) - T.begin()) - 1;
+      const uint32_t lt = (uint32_t)(std::upper_bound(T.begin(), T.end(), pr.last) - T.begin()) - 1;
+      a.tile_starts = probe->tiles.as<uint32_t>();
+      a.first_tile = ft;
+      a.n_plan_tiles = lt - ft + 1;
+    }
+    uint32_t wgs = 0;
+    // Tile plan.  The work-balanced plan costs a pass over every probe-prefix
+    // item; it pays when tiles differ much in work, which follows the mean
+    // posting-list length (index entries per token of the universe): the
+    // visits of a tile grow with it.  Measured on 1M sets (sizes median 10,
+    // Zipf 1.1, 647,502 distinct tokens): upload-time tiles win at mean length
+    // 1.8 and 2.9 (Jaccard 0.9, 0.8), balanced tiles at 4.1 and 5.6 (0.7, 0.6)
+    // and on 200K x 200K sets of 100-700 tokens (length ~600); small joins keep
+    // the upload-time tiles.
+    bool balanced = false;
+    if (ppt == 0 && (uint64_t)(pr.last - pr.first + 1) * 1024 < 0xFFFFFFFFull && probe->maxlen < 0xFFFFu) {
+      if (tune.flags & HS_TUNE_BALANCED_TILES) {
+        balanced = true;
+      } else if (!(tune.flags & (HS_TUNE_STATIC_TILES | HS_TUNE_TILE_QUEUE))) {
+        const auto& psf = probe->h_size_first;
+        uint64_t items = 0;
+        for (uint32_t l = 0; l + 1 < (uint32_t)psf.size(); ++l) {
+          const uint32_t lo = std::max(pr.first, psf[l]), hi = std::min(pr.last + 1, psf[l + 1]);
+          if (hi > lo) items += (uint64_t)(hi - lo) * ctx->tables.maxprefix[std::min(l, ctx->tables.L)];
+        }
+        const double mean_list = (double)P.index_entries / std::max<uint32_t>(indexed->universe, 1);
+        balanced = items >= (1u << 18) && mean_list >= 3.5;
+      }
+    }
+    if (balanced) {
+      // item slices first, then tiles of equal work claimed by a persistent grid
+      // per size class of the probe range: first item slot, first probe,
+      // window of indexed ids, maxprefix
+      const auto& T = ctx->tables;
+      const auto& psf = probe->h_size_first;
+      const auto& isf = indexed->h_size_first;
+      const uint32_t nl = (uint32_t)psf.size();
+      const uint32_t ilast = (uint32_t)isf.size() - 1;
+      ctx->h_icum.ensure((size_t)nl * 20);
+      ctx->d_icum.ensure((size_t)nl * 20);
+      uint32_t* tab = ctx->h_icum.as<uint32_t>();
+      uint64_t items = 0;
+      for (uint32_t l = 0; l < nl; ++l) {
+        tab[l] = (uint32_t)items;
+        tab[nl + l] = std::max(pr.first, psf[l]);
+        const uint32_t mins = std::min(T.minsize[std::min(l, T.L)], ilast);
+        const uint32_t maxs1 = std::min(T.maxsize[std::min(l, T.L)], ilast - 1) + 1;
+        tab[2 * nl + l] = std::max(isf[mins], ir.first);
+        tab[3 * nl + l] = std::min(isf[maxs1], ir.last + 1);
+        tab[4 * nl + l] = T.maxprefix[std::min(l, T.L)];
+        if (l + 1 < nl) {
+          const uint32_t hi = std::min(pr.last + 1, psf[l + 1]);
+          if (hi > tab[nl + l]) items += (uint64_t)(hi - tab[nl + l]) * tab[4 * nl + l];
+          HS_REQUIRE(items < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "probe range limited to 2^32-1 prefix items");
+        }
+      }
+      HS_CHECK_HIP(hipMemcpyAsync(ctx->d_icum.ptr, tab, (size_t)nl * 20, hipMemcpyHostToDevice, st));
+      hs::BalancedPlan bp{ctx->d_icum.as<uint32_t>(), nl, items,
+                          &ctx->bal_slice_a, &ctx->bal_slice_n, &ctx->bal_cost, &ctx->bal_tiles,
+                          &ctx->bal_ntiles, &ctx->ib.scan_scratch, &ctx->bal_rec, &ctx->bal_slot};
+      hs::plan_balanced_tiles(st, a, bp);
+    }
+    tile_plan = ppt ? HS_PLAN_FIXED : balanced ? HS_PLAN_BALANCED
+                                    : (tune.flags & HS_TUNE_TILE_QUEUE) ? HS_PLAN_QUEUE : HS_PLAN_UPLOAD;
+    if (ppt == 0 && (balanced || (tune.flags & HS_TUNE_TILE_QUEUE))) {
+      // a persistent grid claims the tiles from per-XCD queues
+      ctx->batch_queue.ensure(8 * sizeof(uint32_t));
+      a.tile_queue = ctx->batch_queue.as<uint32_t>();
+      wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : 4);
+    }
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));  // kernel_ms: the fgss launch alone
+    hs::launch_fgss(st, a, ppt, wgs);
+  } else if (algo == HS_ALGO_BITMAP) {
+    const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
+    HS_REQUIRE(bits == 64 || bits == 128 || bits == 256, HS_E_INVALID, "bitmap width must be 64, 128 or 256");
+    const uint32_t words = bits / 64;
+    if (P.bitmap_words != words) {  // once per join, whatever the number of launches
+      hs::compute_signatures(st, indexed, words);
+      if (!self) hs::compute_signatures(st, probe, words);
+      P.bitmap_words = words;
+    }
+    hs::BitmapArgs b{};
+    b.probe = probe->dev();
+    b.indexed = indexed->dev();
+    b.bounds = ctx->dev_bounds;
+    b.probe_sigs = probe->sigs.as<uint64_t>();
+    b.indexed_sigs = indexed->sigs.as<uint64_t>();
+    b.size_first = indexed->size_first.as<uint32_t>();
+    b.size_first_len = (uint32_t)indexed->h_size_first.size();
+    b.words = words;
+    b.probe_first = pr.first;
+    b.probe_last = pr.last;
+    b.idx_first = ir.first;
+    b.idx_last = ir.last;
+    b.self = self;
+    b.out = dev_out(ctx);
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
+    hs::launch_bitmap(st, b);
+  } else {
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
+    run_hybrid(ctx, indexed, probe, self, ir, pr, tune);
+  }
+  return tile_plan;
+}
+
+extern "C" {
+
 int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* indexed_range,
             const hs_range* probe_range, int32_t algo, const hs_tune* tune_in, hs_result* result) {
   if (!ctx) return HS_E_INVALID;
@@ -488,189 +690,22 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     hipStream_t st = ctx->stream;
     HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
 
-    // ---- parity signatures for the fgss signature test (timed with the index)
-    uint32_t sig_words = 0;
-    bool sig_in_emit = false;  // the index's emit pass computes them (self-join over the whole collection)
-    if (algo == HS_ALGO_FGSS && !(tune.flags & HS_TUNE_NO_SIGNATURE) && !range_empty(ir) && !range_empty(pr)) {
-      sig_words = hs::fgss_signature_words(indexed, tune.signature_bits);
-      sig_in_emit = sig_words == 1 && self && ir.first == 0 && ir.last + 1 == indexed->n;
-      if (sig_in_emit) {
-        indexed->sigs.ensure((size_t)std::max<uint32_t>(indexed->n, 1) * 8);
-        indexed->sig_words = 1;
-      } else {
-        hs::compute_signatures(st, indexed, sig_words);
-        if (!self) hs::compute_signatures(st, probe, sig_words);
-      }
-    }
-
-    // ---- index over the indexed range (not needed by the bitmap technique)
-    const uint32_t* prefix_tab = self ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
-    uint64_t index_entries = 0;
-    bool counters_ready = false;  // the index build prepared the work counters
-    if (algo != HS_ALGO_BITMAP && !range_empty(ir)) {
-      // entries per size class of the range (sets are sorted by size)
-      const auto& ptab = self ? ctx->tables.midprefix : ctx->tables.maxprefix;
-      const auto& sf = indexed->h_size_first;
-      const uint32_t nl = (uint32_t)sf.size();  // maxlen + 2
-      ctx->h_cum.ensure((size_t)nl * 4);
-      ctx->d_cum.ensure((size_t)nl * 4);
-      uint32_t* cum = ctx->h_cum.as<uint32_t>();
-      uint64_t acc = 0;
-      for (uint32_t l = 0; l + 1 < nl; ++l) {
-        cum[l] = (uint32_t)acc;
-        const uint32_t lo = std::max(ir.first, sf[l]);
-        const uint32_t hi = std::min(ir.last + 1, sf[l + 1]);
-        if (hi > lo) acc += (uint64_t)(hi - lo) * ptab[l];
-        HS_REQUIRE(acc < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
-      }
-      cum[nl - 1] = (uint32_t)acc;
-      index_entries = acc;
-      HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, cum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
-      hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
-                      ctx->d_cum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
-                      sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr,
-                      sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
-                      algo == HS_ALGO_FGSS ? ctx->counters.as<unsigned long long>() : nullptr);
-      counters_ready = algo == HS_ALGO_FGSS && acc > 0 && indexed->n > 0;
-    } else {
-      ctx->ib.universe = 0;
-    }
+    hs::DevicePlan P;
+    hs::prepare_device_join(ctx, indexed, probe, ir, pr, algo, tune, false, P);
     HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
 
     // ---- candidate generation + verification
     unsigned long long cnt[hs::CNT_WORDS];
     uint32_t tile_plan = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
-      if (attempt > 0 || !counters_ready) {
+      if (attempt > 0 || !P.counters_ready) {
         reset_counters(ctx);
         if (algo == HS_ALGO_FGSS && !range_empty(ir))  // index size, read back with the counters
           HS_CHECK_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + hs::CNT_ENTRIES,
                                       ctx->ib.offsets.as<uint32_t>() + indexed->universe, 4,
                                       hipMemcpyDeviceToDevice, st));
       }
-      if (range_empty(pr) || range_empty(ir)) {
-        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
-      } else if (algo == HS_ALGO_FGSS) {
-        hs::FgssArgs a{};
-        a.probe = probe->dev();
-        a.indexed = indexed->dev();
-        a.index = hs::dev_index(ctx->ib, sig_words == 1);
-        a.bounds = ctx->dev_bounds;
-        a.size_first = indexed->size_first.as<uint32_t>();
-        a.size_first_len = (uint32_t)indexed->h_size_first.size();
-        a.probe_first = pr.first;
-        a.probe_last = pr.last;
-        a.idx_first = ir.first;
-        a.idx_last = ir.last;
-        a.self = self;
-        a.out = dev_out(ctx);
-        a.sig_words = sig_words;
-        a.probe_sigs = sig_words ? probe->sigs.as<uint64_t>() : nullptr;
-        a.indexed_sigs = sig_words ? indexed->sigs.as<uint64_t>() : nullptr;
-        const uint32_t ppt = tune.probes_per_tile;
-        if (ppt == 0) {
-          // the collection's tile plan, clipped to the probe range
-          const auto& T = probe->h_tiles;
-          const uint32_t ft = (uint32_t)(std::upper_bound(T.begin(), T.end(), pr.first) - T.begin()) - 1;
-          const uint32_t lt = (uint32_t)(std::upper_bound(T.begin(), T.end(), pr.last) - T.begin()) - 1;
-          a.tile_starts = probe->tiles.as<uint32_t>();
-          a.first_tile = ft;
-          a.n_plan_tiles = lt - ft + 1;
-        }
-        uint32_t wgs = 0;
-        // Tile plan.  The work-balanced plan costs a pass over every probe-prefix
-        // item; it pays when tiles differ much in work, which follows the mean
-        // posting-list length (index entries per token of the universe): the
-        // visits of a tile grow with it.  Measured on 1M sets (sizes median 10,
-        // Zipf 1.1, 647,502 distinct tokens): upload-time tiles win at mean length
-        // 1.8 and 2.9 (Jaccard 0.9, 0.8), balanced tiles at 4.1 and 5.6 (0.7, 0.6)
-        // and on 200K x 200K sets of 100-700 tokens (length ~600); small joins keep
-        // the upload-time tiles.
-        bool balanced = false;
-        if (ppt == 0 && (uint64_t)(pr.last - pr.first + 1) * 1024 < 0xFFFFFFFFull && probe->maxlen < 0xFFFFu) {
-          if (tune.flags & HS_TUNE_BALANCED_TILES) {
-            balanced = true;
-          } else if (!(tune.flags & (HS_TUNE_STATIC_TILES | HS_TUNE_TILE_QUEUE))) {
-            const auto& psf = probe->h_size_first;
-            uint64_t items = 0;
-            for (uint32_t l = 0; l + 1 < (uint32_t)psf.size(); ++l) {
-              const uint32_t lo = std::max(pr.first, psf[l]), hi = std::min(pr.last + 1, psf[l + 1]);
-              if (hi > lo) items += (uint64_t)(hi - lo) * ctx->tables.maxprefix[std::min(l, ctx->tables.L)];
-            }
-            const double mean_list = (double)index_entries / std::max<uint32_t>(indexed->universe, 1);
-            balanced = items >= (1u << 18) && mean_list >= 3.5;
-          }
-        }
-        if (balanced) {
-          // item slices first, then tiles of equal work claimed by a persistent grid
-          // per size class of the probe range: first item slot, first probe,
-          // window of indexed ids, maxprefix
-          const auto& T = ctx->tables;
-          const auto& psf = probe->h_size_first;
-          const auto& isf = indexed->h_size_first;
-          const uint32_t nl = (uint32_t)psf.size();
-          const uint32_t ilast = (uint32_t)isf.size() - 1;
-          ctx->h_icum.ensure((size_t)nl * 20);
-          ctx->d_icum.ensure((size_t)nl * 20);
-          uint32_t* tab = ctx->h_icum.as<uint32_t>();
-          uint64_t items = 0;
-          for (uint32_t l = 0; l < nl; ++l) {
-            tab[l] = (uint32_t)items;
-            tab[nl + l] = std::max(pr.first, psf[l]);
-            const uint32_t mins = std::min(T.minsize[std::min(l, T.L)], ilast);
-            const uint32_t maxs1 = std::min(T.maxsize[std::min(l, T.L)], ilast - 1) + 1;
-            tab[2 * nl + l] = std::max(isf[mins], ir.first);
-            tab[3 * nl + l] = std::min(isf[maxs1], ir.last + 1);
-            tab[4 * nl + l] = T.maxprefix[std::min(l, T.L)];
-            if (l + 1 < nl) {
-              const uint32_t hi = std::min(pr.last + 1, psf[l + 1]);
-              if (hi > tab[nl + l]) items += (uint64_t)(hi - tab[nl + l]) * tab[4 * nl + l];
-              HS_REQUIRE(items < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "probe range limited to 2^32-1 prefix items");
-            }
-          }
-          HS_CHECK_HIP(hipMemcpyAsync(ctx->d_icum.ptr, tab, (size_t)nl * 20, hipMemcpyHostToDevice, st));
-          hs::BalancedPlan bp{ctx->d_icum.as<uint32_t>(), nl, items,
-                              &ctx->bal_slice_a, &ctx->bal_slice_n, &ctx->bal_cost, &ctx->bal_tiles,
-                              &ctx->bal_ntiles, &ctx->ib.scan_scratch, &ctx->bal_rec, &ctx->bal_slot};
-          hs::plan_balanced_tiles(st, a, bp);
-        }
-        tile_plan = ppt ? HS_PLAN_FIXED : balanced ? HS_PLAN_BALANCED
-                                        : (tune.flags & HS_TUNE_TILE_QUEUE) ? HS_PLAN_QUEUE : HS_PLAN_UPLOAD;
-        if (ppt == 0 && (balanced || (tune.flags & HS_TUNE_TILE_QUEUE))) {
-          // a persistent grid claims the tiles from per-XCD queues
-          ctx->batch_queue.ensure(8 * sizeof(uint32_t));
-          a.tile_queue = ctx->batch_queue.as<uint32_t>();
-          wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : 4);
-        }
-        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));  // kernel_ms: the fgss launch alone
-        hs::launch_fgss(st, a, ppt, wgs);
-      } else if (algo == HS_ALGO_BITMAP) {
-        const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
-        HS_REQUIRE(bits == 64 || bits == 128 || bits == 256, HS_E_INVALID, "bitmap width must be 64, 128 or 256");
-        const uint32_t words = bits / 64;
-        hs::compute_signatures(st, indexed, words);
-        if (!self) hs::compute_signatures(st, probe, words);
-        hs::BitmapArgs b{};
-        b.probe = probe->dev();
-        b.indexed = indexed->dev();
-        b.bounds = ctx->dev_bounds;
-        b.probe_sigs = probe->sigs.as<uint64_t>();
-        b.indexed_sigs = indexed->sigs.as<uint64_t>();
-        b.size_first = indexed->size_first.as<uint32_t>();
-        b.size_first_len = (uint32_t)indexed->h_size_first.size();
-        b.words = words;
-        b.probe_first = pr.first;
-        b.probe_last = pr.last;
-        b.idx_first = ir.first;
-        b.idx_last = ir.last;
-        b.self = self;
-        b.out = dev_out(ctx);
-        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
-        hs::launch_bitmap(st, b);
-      } else {
-        HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
-        run_hybrid(ctx, indexed, probe, self, ir, pr, tune);
-      }
+      tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
       HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
       read_counters(ctx, cnt);
       if (cnt[hs::CNT_PAIRS] <= ctx->out_cap) break;
@@ -703,7 +738,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       ctx->stats.tile_plan = tile_plan;
       ctx->stats.index_entries = cnt[hs::CNT_ENTRIES];
       ctx->stats.signature_tests = cnt[hs::CNT_SIG_TESTS];
-      ctx->stats.signature_bits = sig_words * 64;
+      ctx->stats.signature_bits = P.sig_words * 64;
     }
   });
 }

@@ -1,0 +1,338 @@
+// One join shared by the device and host cores running concurrently.
+//
+// Reference: the work-sharing driver src/queue.cu:142-279 (a queue of block
+// pairs drained by a device thread and by the host thread) and the static
+// split of src/dichotomy.cu:143-285 (the host joins the small-set partition
+// Rs with itself, the device R against the rest Rl).  Here the unit of work
+// is a stripe of consecutive probe ids joined against the whole indexed
+// range, so any set of disjoint stripes covering the probe ids gives exactly
+// the join.  Sets are sorted by size, so the work per probe grows with its
+// id: the device claims from the top, where its wide launches pay, the host
+// threads from the bottom, in stripes small enough that the last one ends
+// soon after the device's.
+//
+// The host needs no index of its own: the device builds the prefix index
+// once and a copy of it goes to pinned host memory on a second stream while
+// the device already runs its first stripe.  Device stripes accumulate pairs
+// in the context's result buffers (every launch only adds to the counters);
+// host pairs are appended behind them at the end, so the context holds the
+// whole result as after hs_join.
+#include <sched.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "ctx.h"
+
+using namespace hs;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+uint32_t usable_cpus() {
+  cpu_set_t set;
+  if (sched_getaffinity(0, sizeof(set), &set) == 0) return std::max(1, CPU_COUNT(&set));
+  return std::max(1u, std::thread::hardware_concurrency());
+}
+
+// Unclaimed probe ids [lo, hi): host claims move lo up, device claims move
+// hi down; the host never claims at or above host_limit.
+struct Stripes {
+  std::mutex mu;
+  uint32_t lo = 0, hi = 0, host_limit = 0;
+
+  bool take_low(uint32_t want, uint32_t& first, uint32_t& n) {
+    std::lock_guard<std::mutex> g(mu);
+    const uint32_t lim = std::min(hi, host_limit);
+    if (lo >= lim) return false;
+    n = std::min(want, lim - lo);
+    first = lo;
+    lo += n;
+    return true;
+  }
+  // Guided claim: half of what is left, at least min_claim sets.
+  bool take_high(uint32_t min_claim, uint32_t& first, uint32_t& n) {
+    std::lock_guard<std::mutex> g(mu);
+    if (hi <= lo) return false;
+    const uint32_t rem = hi - lo;
+    n = std::min(rem, std::max(min_claim, rem - rem / 2));
+    hi -= n;
+    first = hi;
+    return true;
+  }
+};
+
+struct HostWorker {
+  Triples out;
+  std::vector<uint32_t> stamp;
+  uint32_t probes = 0, claims = 0;
+  Clock::time_point done;
+  bool failed = false;
+  Error err{HS_OK, ""};
+};
+
+// Host stripe size when the caller gives none: about this much work per
+// claim at the thread's last measured rate.
+constexpr double kHostClaimMs = 0.25;
+
+// Device results: grows the pair buffers to hold `need` pairs, keeping the
+// first `keep`.
+void grow_output_keep(hs_ctx* ctx, uint64_t need, uint64_t keep) {
+  if (need <= ctx->out_cap) return;
+  DevBuf r, s, ov;
+  r.ensure(need * 4);
+  s.ensure(need * 4);
+  ov.ensure(need * 4);
+  if (keep) {
+    HS_CHECK_HIP(hipMemcpyAsync(r.ptr, ctx->out_r.ptr, keep * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HS_CHECK_HIP(hipMemcpyAsync(s.ptr, ctx->out_s.ptr, keep * 4, hipMemcpyDeviceToDevice, ctx->stream));
+    HS_CHECK_HIP(hipMemcpyAsync(ov.ptr, ctx->out_ov.ptr, keep * 4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  std::swap(r.ptr, ctx->out_r.ptr);
+  std::swap(r.bytes, ctx->out_r.bytes);
+  std::swap(s.ptr, ctx->out_s.ptr);
+  std::swap(s.bytes, ctx->out_s.bytes);
+  std::swap(ov.ptr, ctx->out_ov.ptr);
+  std::swap(ov.bytes, ctx->out_ov.bytes);
+  ctx->out_cap = need;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_coprocess* options,
+                      hs_result* result, hs_coprocess_stats* stats) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    HS_REQUIRE(options && result, HS_E_INVALID, "NULL options or result");
+    HS_REQUIRE(indexed && probe, HS_E_INVALID, "NULL collection");
+    HS_REQUIRE(ctx->device >= 0, HS_E_INVALID, "co-processing needs a device context");
+    HS_REQUIRE(indexed->n_joins == 0 && probe->n_joins == 0, HS_E_INVALID, "batched collections are not supported");
+    HS_REQUIRE(indexed->device == ctx->device && probe->device == ctx->device, HS_E_INVALID,
+               "collection lives on another device");
+    const hs_coprocess opt = *options;
+    HS_REQUIRE(opt.mode == HS_COPROC_QUEUE || opt.mode == HS_COPROC_STATIC, HS_E_INVALID, "unknown mode");
+    HS_REQUIRE(opt.algo == HS_ALGO_FGSS || opt.algo == HS_ALGO_BITMAP, HS_E_INVALID,
+               "the device technique must be fgss or bitmap");
+    HS_REQUIRE(opt.mode != HS_COPROC_STATIC || (opt.cpu_share >= 0.0 && opt.cpu_share <= 1.0), HS_E_INVALID,
+               "cpu_share must lie in [0, 1]");
+    hs_tune tune = opt.tune;
+    // launches overlap on the stream: no plan that stages per-launch tables
+    // through shared host buffers
+    tune.flags = (tune.flags & ~(uint32_t)(HS_TUNE_BALANCED_TILES | HS_TUNE_TILE_QUEUE)) | HS_TUNE_STATIC_TILES;
+    const auto t_start = Clock::now();
+    hs_coprocess_stats cs{};
+    const bool self = indexed == probe;
+    const hs_range ir = clamp_range(nullptr, indexed->n);
+    const hs_range pr = clamp_range(nullptr, probe->n);
+    ensure_bounds(ctx, std::max(indexed->maxlen, probe->maxlen));
+    const BoundTables T = ctx->tables;  // the host threads' copy
+    ctx->stats = hs_stats{};
+    hipStream_t st = ctx->stream;
+
+    if (range_empty(ir) || range_empty(pr)) {
+      ctx->last_count = 0;
+      result->count = 0;
+      cs.wall_ms = ms_between(t_start, Clock::now());
+      if (stats) *stats = cs;
+      return;
+    }
+    if (!ctx->copy_stream) HS_CHECK_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+    if (!ctx->index_built) HS_CHECK_HIP(hipEventCreate(&ctx->index_built));
+    if (!ctx->index_copied) HS_CHECK_HIP(hipEventCreate(&ctx->index_copied));
+    for (auto& e : ctx->claim_ev)
+      if (!e) HS_CHECK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+
+    // ---- the index, once, and its copy to the host behind it
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
+    DevicePlan P;
+    prepare_device_join(ctx, indexed, probe, ir, pr, opt.algo, tune, true, P);
+    if (P.counters_ready) {
+      // counters zeroed by the index build; nothing more to prepare
+    } else {
+      reset_counters(ctx);
+    }
+    HS_CHECK_HIP(hipEventRecord(ctx->index_built, st));
+    const uint32_t U = indexed->universe;
+    const uint64_t entries = P.index_entries;
+    ctx->h_index_offsets.ensure(((size_t)U + 1) * 4);
+    ctx->h_index_ids.ensure(std::max<uint64_t>(entries, 1) * 4);
+    HS_CHECK_HIP(hipStreamWaitEvent(ctx->copy_stream, ctx->index_built, 0));
+    HS_CHECK_HIP(hipMemcpyAsync(ctx->h_index_offsets.ptr, ctx->ib.offsets.ptr, ((size_t)U + 1) * 4,
+                                hipMemcpyDeviceToHost, ctx->copy_stream));
+    if (entries)
+      HS_CHECK_HIP(hipMemcpyAsync(ctx->h_index_ids.ptr, ctx->ib.set_ids.ptr, entries * 4, hipMemcpyDeviceToHost,
+                                  ctx->copy_stream));
+    HS_CHECK_HIP(hipEventRecord(ctx->index_copied, ctx->copy_stream));
+
+    // ---- the split of the probe ids
+    Stripes S;
+    S.lo = pr.first;
+    S.hi = pr.last + 1;
+    S.host_limit = S.hi;
+    const uint32_t np = pr.last - pr.first + 1;
+    if (opt.mode == HS_COPROC_STATIC) {
+      const uint32_t cut = pr.first + (uint32_t)std::llround(opt.cpu_share * (double)np);
+      S.host_limit = cut;
+      S.hi = cut;  // the device's share [cut, n) is claimed below, in one launch
+    }
+    const uint32_t nthreads = opt.cpu_threads ? opt.cpu_threads : std::max(1u, usable_cpus() - 1);
+    cs.host_threads = nthreads;
+    const uint32_t min_claim = opt.device_min_claim ? opt.device_min_claim : 16384u;
+
+    // ---- host consumers
+    const uint32_t* h_off = ctx->h_index_offsets.as<uint32_t>();
+    const uint32_t* h_ids = ctx->h_index_ids.as<uint32_t>();
+    const int device = ctx->device;
+    hipEvent_t copied = ctx->index_copied;
+    std::vector<HostWorker> workers(nthreads);
+    std::vector<std::thread> pool;
+    const auto t_host = Clock::now();
+    for (uint32_t k = 0; k < nthreads; ++k) {
+      pool.emplace_back([&, k] {
+        HostWorker& w = workers[k];
+        w.done = Clock::now();
+        try {
+          HS_CHECK_HIP(hipSetDevice(device));
+          HS_CHECK_HIP(hipEventSynchronize(copied));
+          uint32_t want = opt.cpu_block ? opt.cpu_block : 16u, first = 0, n = 0;
+          while (S.take_low(want, first, n)) {
+            const auto a = Clock::now();
+            host_probe_range(*probe, *indexed, h_off, h_ids, T, self, ir, first, first + n - 1, w.out, w.stamp);
+            const double dt = ms_between(a, Clock::now());
+            w.probes += n;
+            ++w.claims;
+            if (!opt.cpu_block) {
+              const double per = std::max(dt / n, 1e-5);
+              want = (uint32_t)std::clamp(kHostClaimMs / per, 1.0, 65536.0);
+            }
+          }
+        } catch (const Error& e) {
+          w.failed = true;
+          w.err = e;
+        } catch (...) {
+          w.failed = true;
+          w.err = Error{HS_E_INVALID, "host stripe failed"};
+        }
+        w.done = Clock::now();
+      });
+    }
+
+    // ---- device consumer (this thread): up to two stripes in flight
+    uint32_t dev_first = pr.last + 1;  // the device's share is [dev_first, n)
+    uint32_t tile_plan = 0;
+    std::string dev_err;
+    int dev_rc = HS_OK;
+    const auto t_dev = Clock::now();
+    try {
+      std::deque<uint32_t> inflight;  // claim numbers
+      uint32_t claims = 0;
+      auto launch = [&](uint32_t first, uint32_t n) {
+        tile_plan = launch_device_join(ctx, indexed, probe, P, hs_range{first, first + n - 1}, opt.algo, tune);
+        HS_CHECK_HIP(hipEventRecord(ctx->claim_ev[claims & 1], st));
+        inflight.push_back(claims++);
+        dev_first = first;
+        cs.device_probes += n;
+      };
+      if (opt.mode == HS_COPROC_STATIC) {
+        if (S.host_limit <= pr.last) launch(S.host_limit, pr.last + 1 - S.host_limit);
+      }
+      for (;;) {
+        uint32_t first = 0, n = 0;
+        while (opt.mode == HS_COPROC_QUEUE && inflight.size() < 2 && S.take_high(min_claim, first, n))
+          launch(first, n);
+        if (inflight.empty()) break;
+        HS_CHECK_HIP(hipEventSynchronize(ctx->claim_ev[inflight.front() & 1]));
+        inflight.pop_front();
+      }
+      cs.device_claims = claims;
+    } catch (const Error& e) {
+      dev_rc = e.code;
+      dev_err = e.msg;
+      std::lock_guard<std::mutex> g(S.mu);
+      S.host_limit = 0;  // stop the host threads after their current stripe
+    }
+    const auto t_dev_end = Clock::now();
+    for (auto& th : pool) th.join();
+    if (dev_rc != HS_OK) throw Error{dev_rc, dev_err};
+    for (auto& w : workers)
+      if (w.failed) throw w.err;
+    HS_REQUIRE(S.lo == S.hi, HS_E_DEVICE, "probe stripes left unclaimed");
+    cs.split = dev_first;
+
+    // ---- the device's pairs; a second pass over its share if they overflowed
+    unsigned long long cnt[CNT_WORDS];
+    read_counters(ctx, cnt);
+    if (cnt[CNT_PAIRS] > ctx->out_cap) {
+      ensure_output(ctx, cnt[CNT_PAIRS] + cnt[CNT_PAIRS] / 4 + 1024);
+      reset_counters(ctx);
+      if (dev_first <= pr.last)
+        launch_device_join(ctx, indexed, probe, P, hs_range{dev_first, pr.last}, opt.algo, tune);
+      unsigned long long again[CNT_WORDS];
+      read_counters(ctx, again);
+      HS_REQUIRE(again[CNT_PAIRS] == cnt[CNT_PAIRS], HS_E_DEVICE, "result count changed between runs");
+    }
+    const uint64_t g = cnt[CNT_PAIRS];
+
+    // ---- host pairs behind the device's
+    uint64_t h = 0;
+    for (auto& w : workers) {
+      h += w.out.r.size();
+      cs.host_probes += w.probes;
+      cs.host_claims += w.claims;
+      cs.host_ms = std::max(cs.host_ms, ms_between(t_host, w.done));
+    }
+    grow_output_keep(ctx, g + h, g);
+    uint64_t at = g;
+    for (auto& w : workers) {
+      const size_t k = w.out.r.size();
+      if (!k) continue;
+      HS_CHECK_HIP(hipMemcpyAsync(ctx->out_r.as<uint32_t>() + at, w.out.r.data(), k * 4, hipMemcpyHostToDevice, st));
+      HS_CHECK_HIP(hipMemcpyAsync(ctx->out_s.as<uint32_t>() + at, w.out.s.data(), k * 4, hipMemcpyHostToDevice, st));
+      HS_CHECK_HIP(
+          hipMemcpyAsync(ctx->out_ov.as<uint32_t>() + at, w.out.ov.data(), k * 4, hipMemcpyHostToDevice, st));
+      at += k;
+    }
+    HS_CHECK_HIP(hipStreamSynchronize(st));
+    const uint64_t total = g + h;
+    ctx->last_count = total;
+    result->count = total;
+    if (!result->count_only && total) {
+      HS_REQUIRE(total <= result->capacity, HS_E_OVERFLOW, "result capacity too small");
+      copy_pairs(ctx, total, result);
+    }
+
+    cs.device_pairs = g;
+    cs.host_pairs = h;
+    cs.index_ms = elapsed(ctx->ev[0], ctx->index_built);
+    cs.copy_ms = elapsed(ctx->index_built, ctx->index_copied);
+    cs.device_ms = ms_between(t_dev, t_dev_end);
+    cs.wall_ms = ms_between(t_start, Clock::now());
+    ctx->stats.pairs = total;
+    ctx->stats.index_ms = cs.index_ms;
+    ctx->stats.total_ms = cs.wall_ms;
+    ctx->stats.posting_visits = cnt[CNT_VISITS];
+    ctx->stats.candidates = cnt[CNT_CANDS];
+    ctx->stats.candidate_tokens = cnt[CNT_CAND_TOKENS];
+    ctx->stats.probe_prefix_tokens = cnt[CNT_ITEMS];
+    ctx->stats.index_entries = entries;
+    ctx->stats.tile_plan = tile_plan;
+    ctx->stats.signature_tests = cnt[CNT_SIG_TESTS];
+    ctx->stats.signature_bits = P.sig_words * 64;
+    if (stats) *stats = cs;
+  });
+}
+
+}  // extern "C"

@@ -101,6 +101,12 @@ struct hs_ctx {
   HybridSlot slots[2];
   std::mutex submit_mu;
   std::vector<uint32_t> h_offsets, h_set_ids;  // host copy of the index (hybrid)
+  // CPU/GPU co-processing join: a copy stream for the index, pinned host
+  // copies of the index, claim-completion events
+  hipStream_t copy_stream = nullptr;
+  PinnedBuf h_index_offsets, h_index_ids;
+  hipEvent_t index_built = nullptr, index_copied = nullptr;
+  hipEvent_t claim_ev[2] = {};
   PinnedBuf stage[2];            // pinned chunks of the result stream
   hipEvent_t stage_ev[2] = {};   // chunk k has landed in stage[k]
   int num_cus = 256;
@@ -244,9 +250,41 @@ inline hs_range clamp_range(const hs_range* r, uint32_t n) {
 
 inline bool range_empty(const hs_range& r) { return r.first > r.last; }
 
+// One device join between its set-up and its launches: hs_join runs one
+// launch over its probe range, the CPU/GPU co-processing join one launch per
+// probe stripe it claims, all against one index.
+struct DevicePlan {
+  bool self = false;
+  hs_range ir{1, 0};          // indexed range of the index
+  uint32_t sig_words = 0;     // fgss signature test width (64-bit words)
+  uint32_t bitmap_words = 0;  // signatures computed for the bitmap technique
+  uint64_t index_entries = 0;
+  bool counters_ready = false;  // the index build zeroed the work counters
+};
+
+// Signatures and the prefix index over ir on the context stream (the index
+// also for the bitmap technique when with_index).
+void prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_range ir, hs_range pr, int32_t algo,
+                         const hs_tune& tune, bool with_index, DevicePlan& P);
+
+// Queues the technique over probe range pr against the prepared join; pairs
+// and work counters accumulate in the context's result buffers.  Records
+// ev[5] just before the main kernel.  Returns the tile plan (HS_PLAN_*).
+uint32_t launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, DevicePlan& P, hs_range pr,
+                            int32_t algo, const hs_tune& tune);
+
 struct Triples {
   std::vector<uint32_t> r, s, ov;
 };
+// CPU join of probes [first, last] of P against the prefix index (offsets,
+// ids: id-ascending lists over S's universe) of S's sets in ir, on the
+// calling thread; pairs are appended to out.  stamp is the thread's
+// deduplication array (S.n entries, allocated on first use; keep it across
+// calls of one thread).
+void host_probe_range(const hs_coll& P, const hs_coll& S, const uint32_t* offsets, const uint32_t* ids,
+                      const BoundTables& T, bool self, const hs_range& ir, uint32_t first, uint32_t last,
+                      Triples& out, std::vector<uint32_t>& stamp);
+
 // CPU join (host_join.cpp): pairs of probe range pr against indexed range ir;
 // reads only host copies and the given tables, so it may run on any thread.
 uint64_t host_join(const hs_coll& indexed, const hs_coll& probe, bool self, hs_range ir, hs_range pr,

@@ -45,7 +45,17 @@ void build_host_index(const hs_coll& S, uint32_t first, uint32_t last, const std
 
 void join_probes(const hs_coll& P, const hs_coll& S, const HostIndex& X, const BoundTables& T, bool self,
                  const hs_range& ir, uint32_t first, uint32_t last, Triples& out) {
-  std::vector<uint32_t> stamp(S.n, 0xFFFFFFFFu), cand;
+  std::vector<uint32_t> stamp(S.n, 0xFFFFFFFFu);
+  hs::host_probe_range(P, S, X.offsets.data(), X.ids.data(), T, self, ir, first, last, out, stamp);
+}
+
+}  // namespace
+
+void hs::host_probe_range(const hs_coll& P, const hs_coll& S, const uint32_t* offsets, const uint32_t* ids,
+                          const BoundTables& T, bool self, const hs_range& ir, uint32_t first, uint32_t last,
+                          Triples& out, std::vector<uint32_t>& stamp) {
+  std::vector<uint32_t> cand;
+  if (stamp.size() < S.n) stamp.assign(S.n, 0xFFFFFFFFu);
   const uint32_t last_len = (uint32_t)S.h_size_first.size() - 1;
   for (uint32_t p = first; p <= last && first <= last; ++p) {
     const uint32_t l = P.h_sizes[p];
@@ -60,8 +70,8 @@ void join_probes(const hs_coll& P, const hs_coll& S, const HostIndex& X, const B
     for (uint32_t j = 0; j < T.maxprefix[l]; ++j) {
       const uint32_t tok = rt[j];
       if (tok >= S.universe) continue;
-      const uint32_t* a = X.ids.data() + X.offsets[tok];
-      const uint32_t* e = X.ids.data() + X.offsets[tok + 1];
+      const uint32_t* a = ids + offsets[tok];
+      const uint32_t* e = ids + offsets[tok + 1];
       for (const uint32_t* it = std::lower_bound(a, e, lo); it != e && *it < hi; ++it)
         if (stamp[*it] != p) {
           stamp[*it] = p;
@@ -97,8 +107,6 @@ void join_probes(const hs_coll& P, const hs_coll& S, const HostIndex& X, const B
     }
   }
 }
-
-}  // namespace
 
 namespace hs {
 

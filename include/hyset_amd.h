@@ -355,6 +355,55 @@ void hs_server_destroy(hs_server* server);
 int hs_join_host(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* indexed_range,
                  const hs_range* probe_range, uint32_t threads, hs_result* result);
 
+/* One join shared between the device and host cores running concurrently,
+ * over the whole probe collection against the whole indexed one (a
+ * self-join when indexed == probe).  The device builds the prefix index
+ * once; a copy of it travels to the host on a second stream while the
+ * device starts probing.  The device takes probe stripes from the top of the
+ * (size-sorted) probe ids, the host threads take small stripes from the
+ * bottom, each with the CPU partition's join (hs_join_host's per-probe
+ * filter and verification).
+ *   HS_COPROC_QUEUE: both sides claim stripes until they meet -- the shared
+ *     block-pair queue of src/queue.cu:142-279 with a device consumer and
+ *     host consumers.  Device claims halve the unclaimed range (at least
+ *     device_min_claim sets), two in flight at a time; host claims are
+ *     cpu_block sets, or sized per thread to about 1 ms of its measured rate
+ *     when cpu_block is 0.
+ *   HS_COPROC_STATIC: the split of src/dichotomy.cu:143-285: the host joins
+ *     the probe sets below cpu_share * n (Rs, the small sets), the device
+ *     the rest (Rl) against everything.
+ * The result is the same set of pairs as hs_join: afterwards the context's
+ * device result holds the device's pairs followed by the host's, count-only
+ * or copied into result as for hs_join, and hs_copy_last_result,
+ * hs_stream_last_result work on it.  Device tile plans are the upload-time
+ * tiles (or tune.probes_per_tile). */
+enum { HS_COPROC_QUEUE = 0, HS_COPROC_STATIC = 1 };
+typedef struct {
+  int32_t mode;              /* HS_COPROC_* */
+  int32_t algo;              /* device technique: HS_ALGO_FGSS or HS_ALGO_BITMAP */
+  uint32_t cpu_threads;      /* host threads (0: all hardware threads) */
+  uint32_t cpu_block;        /* queue: probe sets per host claim (0: about 1 ms of work) */
+  uint32_t device_min_claim; /* queue: smallest device claim in probe sets (0: 16384) */
+  double cpu_share;          /* static: fraction of the probe sets the host joins */
+  hs_tune tune;              /* device knobs */
+} hs_coprocess;
+
+typedef struct {
+  uint64_t device_pairs, host_pairs;
+  uint32_t device_probes, host_probes; /* probe sets joined by each side */
+  uint32_t device_claims, host_claims; /* stripes each side claimed */
+  uint32_t split;                      /* first probe id of the device's share */
+  uint32_t host_threads;
+  double index_ms;  /* device index build and signatures */
+  double copy_ms;   /* index copy to the host, from the end of the build */
+  double device_ms; /* first device claim to the device's last stripe done */
+  double host_ms;   /* host threads: start to the last stripe done */
+  double wall_ms;   /* the whole call */
+} hs_coprocess_stats;
+
+int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_coprocess* options,
+                      hs_result* result, hs_coprocess_stats* stats);
+
 /* ------------------------------------------------- cost models, split */
 
 /* Partition features of include/hyset/statistics.hpp:163-283, computed on

@@ -1,0 +1,117 @@
+"""One join shared by the device and host threads (hs_join_coprocess): the
+work-sharing queue and the static split give the oracle's pairs for every
+way the probe ids are divided, self-join and R-S, and the per-side counts
+add up."""
+import numpy as np
+import pytest
+
+import hyset_amd as H
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _triples(r, s, ov):
+    return sorted(zip(r.tolist(), s.tolist(), ov.tolist()))
+
+
+def _want(res):
+    return sorted(zip(res.r.tolist(), res.s.tolist(), res.overlap.tolist()))
+
+
+@pytest.fixture(scope="module")
+def col(ctx):
+    c = H.generate_self(H.gen_params(20000, median=10, sigma=0.6, lo=2, hi=200, universe=20000, zipf=1.1, seed=5))
+    d = ctx.upload(c)
+    yield c, d
+    d.free()
+
+
+def _check_stats(info, n, count):
+    assert info["device_probes"] + info["host_probes"] == n
+    assert info["device_pairs"] + info["host_pairs"] == count
+    assert info["split"] == n - info["device_probes"]
+
+
+@pytest.mark.parametrize("algo", ["fgss", "bitmap"])
+@pytest.mark.parametrize("kw", [dict(cpu_threads=4), dict(cpu_threads=1, cpu_block=1, device_min_claim=256),
+                                dict(cpu_threads=3, cpu_block=64, device_min_claim=1000)])
+def test_queue_self_join(ctx, col, algo, kw):
+    c, d = col
+    for t in (0.6, 0.8):
+        ctx.set_threshold(t)
+        want = O.self_join(c, t)
+        (r, s, ov), info = ctx.join_coprocess(d, algo=algo, **kw)
+        assert _triples(r, s, ov) == _want(want), (t, kw)
+        _check_stats(info, c.n, want.count)
+        if kw.get("device_min_claim") == 256:
+            assert info["device_claims"] >= 2
+
+
+@pytest.mark.parametrize("share", [0.0, 0.05, 0.5, 1.0])
+def test_static_split_self_join(ctx, col, share):
+    c, d = col
+    ctx.set_threshold(0.7)
+    want = O.self_join(c, 0.7)
+    (r, s, ov), info = ctx.join_coprocess(d, mode="static", cpu_share=share, cpu_threads=4)
+    assert _triples(r, s, ov) == _want(want), share
+    _check_stats(info, c.n, want.count)
+    assert info["split"] == round(share * c.n)
+    assert info["device_claims"] == (0 if share == 1.0 else 1)
+
+
+def test_rs_join_both_modes(ctx):
+    R, S = H.generate_rs(H.gen_params(3000, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=3),
+                         H.gen_params(2500, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=4),
+                         dup_frac=0.3)
+    dR, dS = ctx.upload(R), ctx.upload(S)
+    try:
+        for sim in ("jaccard", "cosine"):
+            ctx.set_threshold(0.7, sim)
+            want = O.rs_join(R, S, 0.7, sim)
+            assert want.count > 100
+            for kw in (dict(mode="queue", cpu_threads=2, cpu_block=8, device_min_claim=300),
+                       dict(mode="static", cpu_share=0.3, cpu_threads=2)):
+                (r, s, ov), info = ctx.join_coprocess(dS, dR, **kw)
+                assert _triples(r, s, ov) == _want(want), (sim, kw)
+                _check_stats(info, R.n, want.count)
+    finally:
+        dR.free()
+        dS.free()
+
+
+def test_count_only_then_copy_and_dense_host_share(ctx):
+    """Many pairs on both sides: the host's pairs land behind the device's in
+    the context's result, where hs_copy_last_result and the stream sink find
+    them."""
+    c = H.generate_self(H.gen_params(3000, size_dist=H.SIZE_UNIFORM, lo=4, hi=6, universe=12, zipf=0.5, seed=9))
+    d = ctx.upload(c)
+    try:
+        ctx.set_threshold(0.5)
+        want = O.self_join(c, 0.5)
+        n, info = ctx.join_coprocess(d, mode="static", cpu_share=0.6, cpu_threads=4, count_only=True)
+        assert n == want.count and info["host_pairs"] > 100000 and info["device_pairs"] > 100000
+        r, s, ov = ctx.last_result(n)
+        assert _triples(r, s, ov) == _want(want)
+        # device pairs first, then the host's (ids below the split)
+        g = info["device_pairs"]
+        assert np.all(r[:g] >= info["split"]) and np.all(r[g:] < info["split"])
+        got = []
+        ctx.stream_last_result(lambda a, b, o: got.append((a, b, o)), chunk=100000)
+        assert np.array_equal(np.concatenate([x[0] for x in got]), r)
+        assert np.array_equal(np.concatenate([x[2] for x in got]), ov)
+        # an ordinary join afterwards is unaffected
+        assert ctx.join(d, algo="fgss", count_only=True) == want.count
+    finally:
+        d.free()
+
+
+def test_bad_options(ctx, col):
+    _, d = col
+    ctx.set_threshold(0.8)
+    with pytest.raises(H.HysetError):
+        ctx.join_coprocess(d, algo="hybrid")
+    with pytest.raises(H.HysetError):
+        ctx.join_coprocess(d, mode="static", cpu_share=1.5)
+    with pytest.raises(H.HysetError):
+        ctx.join_coprocess(d, mode=7)
