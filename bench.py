@@ -45,6 +45,41 @@ def fgss_bytes(st: dict, n_probes: int, n_probe_tokens: int) -> int:
             + 8 * st["candidates"] + 4 * st["candidate_tokens"] + 12 * st["pairs"])
 
 
+def bench_coprocess(ctx, dI, dP, name, threads, want, reps=3) -> dict:
+    """The join shared by the device and `threads` host threads (0: the
+    library's default, the usable CPUs less one) through the work-sharing
+    queue, against the device alone; both timed as wall clock around the
+    call, median of `reps`."""
+    alone, shared, infos = [], [], []
+    for _ in range(reps + 1):
+        a = time.perf_counter()
+        assert ctx.join(dI, dP, algo="fgss", count_only=True) == want
+        alone.append((time.perf_counter() - a) * 1e3)
+        a = time.perf_counter()
+        c, info = ctx.join_coprocess(dI, dP, cpu_threads=threads, count_only=True)
+        shared.append((time.perf_counter() - a) * 1e3)
+        assert c == want
+        infos.append(info)
+    alone, shared, infos = alone[1:], shared[1:], infos[1:]
+    k = sorted(range(reps), key=lambda i: shared[i])[reps // 2]
+    info = infos[k]
+    n = info["device_probes"] + info["host_probes"]
+    res = {f"{name}_device_alone_ms": round(statistics.median(alone), 3),
+           f"{name}_coprocess_ms": round(shared[k], 3),
+           f"{name}_coprocess_host_threads": int(info["host_threads"]),
+           f"{name}_coprocess_host_probe_share": round(info["host_probes"] / max(n, 1), 5),
+           f"{name}_coprocess_host_pairs": int(info["host_pairs"]),
+           f"{name}_coprocess_device_claims": int(info["device_claims"]),
+           f"{name}_coprocess_host_claims": int(info["host_claims"]),
+           f"{name}_coprocess_index_copy_ms": round(info["copy_ms"], 3),
+           f"{name}_coprocess_device_busy_ms": round(info["device_ms"], 3),
+           f"{name}_coprocess_host_busy_ms": round(info["host_ms"], 3)}
+    log(f"{name}: device alone {res[f'{name}_device_alone_ms']} ms, shared with {info['host_threads']} host "
+        f"threads {res[f'{name}_coprocess_ms']} ms (host took {info['host_probes']} of {n} probes, "
+        f"{info['host_pairs']} pairs; index copy {info['copy_ms']:.3f} ms)")
+    return res
+
+
 def bench_long_rs(ctx, H) -> dict:
     """The long-set R-S workload: R = S = 200,000 sets of 100-700 tokens over
     a 50,000-token Zipf(0.8) universe, 20% of S edited copies of R sets,
@@ -71,6 +106,7 @@ def bench_long_rs(ctx, H) -> dict:
     assert ctx.join(dS, dR, algo="fgss", tune=H.Tune(flags=H.TUNE_NO_SIGNATURE), count_only=True) == c
     res["long_rs_fgss_no_signature_ms"] = round(ctx.stats()["total_ms"], 1)
     log(f"long R-S without the signature test: {res['long_rs_fgss_no_signature_ms']} ms")
+    res.update(bench_coprocess(ctx, dS, dR, "long_rs", 0, c))
     dR.free()
     dS.free()
     ctx.set_threshold(0.8)
@@ -470,6 +506,10 @@ def main() -> int:
         extra["hybrid_ms_wall"] = round((time.perf_counter() - a) * 1e3, 2)
         assert c == pairs
         log(f"hybrid ({min(16, ncores)} host threads): {extra['hybrid_ms_wall']} ms")
+        extra.update(bench_coprocess(ctx, d, d, "t0.8", 0, pairs))
+        ctx.set_threshold(0.6)
+        extra.update(bench_coprocess(ctx, d, d, "t0.6", 0, extra["fgss_t0.6_pairs"]))
+        ctx.set_threshold(t)
 
     if not args.quick and not args.no_long:
         extra.update(bench_long_rs(ctx, H))

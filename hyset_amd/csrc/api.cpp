@@ -35,9 +35,17 @@ using namespace hs;
 
 // ---------------------------------------------------------------- DevBuf
 
+namespace {
+std::atomic<uint64_t> g_buffer_epoch{0};
+}
+
+uint64_t hs::buffer_epoch() { return g_buffer_epoch.load(std::memory_order_acquire); }
+void hs::bump_buffer_epoch() { g_buffer_epoch.fetch_add(1, std::memory_order_acq_rel); }
+
 void hs::DevBuf::ensure(size_t need) {
   if (need <= bytes && ptr) return;
   release();
+  bump_buffer_epoch();
   size_t want = need < 256 ? 256 : need + need / 8;
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, want);
@@ -50,7 +58,10 @@ void hs::DevBuf::ensure(size_t need) {
 }
 
 void hs::DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) {
+    (void)hipFree(ptr);
+    bump_buffer_epoch();
+  }
   ptr = nullptr;
   bytes = 0;
 }
@@ -101,10 +112,29 @@ void hs::stream_pairs(hs_ctx* ctx, uint64_t count, bool with_overlap, uint64_t c
   }
 }
 
+void hs::stream_result(hs_ctx* ctx, bool with_overlap, uint64_t chunk,
+                       const std::function<void(const uint32_t*, const uint32_t*, const uint32_t*, uint64_t,
+                                                uint64_t)>& consume) {
+  uint64_t tail = 0;
+  for (const auto& t : ctx->host_tail) tail += t.r.size();
+  HS_REQUIRE(tail <= ctx->last_count, HS_E_DEVICE, "inconsistent result");
+  const uint64_t dev = ctx->last_count - tail;
+  stream_pairs(ctx, dev, with_overlap, chunk, consume);
+  chunk = chunk ? chunk : kStreamChunk;
+  uint64_t at = dev;
+  for (const auto& t : ctx->host_tail)
+    for (uint64_t o = 0; o < t.r.size(); o += chunk) {
+      const uint64_t n = std::min<uint64_t>(chunk, t.r.size() - o);
+      consume(t.r.data() + o, t.s.data() + o, with_overlap ? t.ov.data() + o : nullptr, at, n);
+      at += n;
+    }
+}
+
 void hs::copy_pairs(hs_ctx* ctx, uint64_t count, hs_result* res) {
   if (count == 0) return;
   HS_REQUIRE(res->r && res->s, HS_E_INVALID, "result arrays are NULL");
-  stream_pairs(ctx, count, res->overlap != nullptr, 0,
+  HS_REQUIRE(count == ctx->last_count, HS_E_DEVICE, "copy of a result that is not the last one");
+  stream_result(ctx, res->overlap != nullptr, 0,
                [&](const uint32_t* r, const uint32_t* s, const uint32_t* ov, uint64_t first, uint64_t n) {
                  std::memcpy(res->r + first, r, n * 4);
                  std::memcpy(res->s + first, s, n * 4);
@@ -364,6 +394,7 @@ void hs_destroy(hs_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   for (auto& e : ctx->claim_ev)
     if (e) (void)hipEventDestroy(e);
+  if (ctx->icum_ev) (void)hipEventDestroy(ctx->icum_ev);
   if (ctx->index_built) (void)hipEventDestroy(ctx->index_built);
   if (ctx->index_copied) (void)hipEventDestroy(ctx->index_copied);
   if (ctx->copy_stream) {
@@ -603,6 +634,10 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
       const auto& isf = indexed->h_size_first;
       const uint32_t nl = (uint32_t)psf.size();
       const uint32_t ilast = (uint32_t)isf.size() - 1;
+      // a previous launch's copy out of the staging table must have run
+      // before the table is rewritten (launches may be queued back to back)
+      if (ctx->icum_pending) HS_CHECK_HIP(hipEventSynchronize(ctx->icum_ev));
+      ctx->icum_pending = false;
       ctx->h_icum.ensure((size_t)nl * 20);
       ctx->d_icum.ensure((size_t)nl * 20);
       uint32_t* tab = ctx->h_icum.as<uint32_t>();
@@ -622,6 +657,9 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
         }
       }
       HS_CHECK_HIP(hipMemcpyAsync(ctx->d_icum.ptr, tab, (size_t)nl * 20, hipMemcpyHostToDevice, st));
+      if (!ctx->icum_ev) HS_CHECK_HIP(hipEventCreateWithFlags(&ctx->icum_ev, hipEventDisableTiming));
+      HS_CHECK_HIP(hipEventRecord(ctx->icum_ev, st));
+      ctx->icum_pending = true;
       hs::BalancedPlan bp{ctx->d_icum.as<uint32_t>(), nl, items,
                           &ctx->bal_slice_a, &ctx->bal_slice_n, &ctx->bal_cost, &ctx->bal_tiles,
                           &ctx->bal_ntiles, &ctx->ib.scan_scratch, &ctx->bal_rec, &ctx->bal_slot};
@@ -713,7 +751,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
     }
     const uint64_t count = cnt[hs::CNT_PAIRS];
-    ctx->last_count = count;
+    set_device_result(ctx, count);
     result->count = count;
     HS_CHECK_HIP(hipEventRecord(ctx->ev[3], st));
     if (!result->count_only && count > 0) {
@@ -792,7 +830,7 @@ int hs_verify_candidates(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const ui
     }
     sl.used = false;
     const uint64_t count = cnt[hs::CNT_PAIRS];
-    ctx->last_count = count;
+    set_device_result(ctx, count);
     result->count = count;
     HS_CHECK_HIP(hipEventRecord(ctx->ev[3], st));
     if (!result->count_only && count > 0) {
@@ -825,7 +863,7 @@ int hs_stream_last_result(hs_ctx* ctx, uint64_t chunk_pairs, int32_t with_overla
   return guarded(ctx, [&] {
     HS_REQUIRE(sink, HS_E_INVALID, "NULL sink");
     HS_REQUIRE(ctx->device >= 0, HS_E_INVALID, "a host-only context holds no device result");
-    stream_pairs(ctx, ctx->last_count, with_overlap != 0, chunk_pairs,
+    stream_result(ctx, with_overlap != 0, chunk_pairs,
                  [&](const uint32_t* r, const uint32_t* s, const uint32_t* ov, uint64_t first, uint64_t n) {
                    const int rc = sink(user, r, s, ov, n);
                    HS_REQUIRE(rc == 0, HS_E_INVALID,
@@ -842,7 +880,7 @@ int hs_write_last_result_text(hs_ctx* ctx, const char* path, int32_t with_overla
     HS_REQUIRE(ctx->device >= 0, HS_E_INVALID, "a host-only context holds no device result");
     hs::PairTextWriter w(path);
     HS_REQUIRE(w.opened(), HS_E_INVALID, std::string("cannot open ") + path);
-    stream_pairs(ctx, ctx->last_count, with_overlap != 0, 0,
+    stream_result(ctx, with_overlap != 0, 0,
                  [&](const uint32_t* r, const uint32_t* s, const uint32_t* ov, uint64_t, uint64_t n) {
                    w.add(r, s, ov, n);
                  });

@@ -258,7 +258,9 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
     HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
     unsigned long long cnt[CNT_WORDS];
     for (int attempt = 0; attempt < 2; ++attempt) {
-      if (use_graph && ctx->graph_exec && ctx->graph_coll == batch && ctx->graph_key == key) {
+      // replay only while every buffer the capture recorded is still where it was
+      if (use_graph && ctx->graph_exec && ctx->graph_coll == batch && ctx->graph_key == key &&
+          ctx->graph_epoch == buffer_epoch()) {
         HS_CHECK_HIP(hipGraphLaunch(ctx->graph_exec, st));
       } else {
         const uint32_t ntab = (uint32_t)batch->h_size_first.size();
@@ -278,6 +280,7 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
           ctx->graph_coll = batch;
           ctx->graph_key = key;
           ctx->graph_entries = entries;
+          ctx->graph_epoch = buffer_epoch();
           HS_CHECK_HIP(hipGraphLaunch(ctx->graph_exec, st));
         } else {
           enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words);
@@ -293,7 +296,7 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
     }
     if (use_graph) ctx->stats.index_entries = ctx->graph_entries;
     const uint64_t count = cnt[CNT_PAIRS];
-    ctx->last_count = count;
+    set_device_result(ctx, count);
     result->count = count;
     HS_CHECK_HIP(hipEventRecord(ctx->ev[3], st));
     if (!result->count_only && count > 0) {

@@ -15,11 +15,12 @@
 // once and a copy of it goes to pinned host memory on a second stream while
 // the device already runs its first stripe.  Device stripes accumulate pairs
 // in the context's result buffers (every launch only adds to the counters);
-// host pairs are appended behind them at the end, so the context holds the
-// whole result as after hs_join.
+// the host's pairs stay in host memory as the tail of the context's result,
+// which the result sinks deliver after the device's pairs.
 #include <sched.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <deque>
@@ -45,29 +46,49 @@ uint32_t usable_cpus() {
   return std::max(1u, std::thread::hardware_concurrency());
 }
 
+// Work proxy of probe ids [a, b): set size plus a fixed per-probe cost,
+// summed over the size classes the ids span (sets are sorted by size, sf[l]
+// is the first id of size l or more).
+constexpr uint64_t kProbeCost = 8;
+
+uint64_t probe_weight(const std::vector<uint32_t>& sf, uint32_t a, uint32_t b) {
+  uint64_t w = 0;
+  for (uint32_t l = 0; l + 1 < (uint32_t)sf.size(); ++l) {
+    const uint32_t lo = std::max(a, sf[l]), hi = std::min(b, sf[l + 1]);
+    if (hi > lo) w += (uint64_t)(hi - lo) * (l + kProbeCost);
+  }
+  return w;
+}
+
+// Smallest x in [a, b] with probe_weight(a, x) >= u.
+uint32_t weight_cut(const std::vector<uint32_t>& sf, uint32_t a, uint32_t b, uint64_t u) {
+  uint64_t acc = 0;
+  for (uint32_t l = 0; l + 1 < (uint32_t)sf.size(); ++l) {
+    const uint32_t lo = std::max(a, sf[l]), hi = std::min(b, sf[l + 1]);
+    if (hi <= lo) continue;
+    const uint64_t c = l + kProbeCost, cls = (uint64_t)(hi - lo) * c;
+    if (acc + cls >= u) return lo + (uint32_t)std::min<uint64_t>(hi - lo, (u - acc + c - 1) / c);
+    acc += cls;
+  }
+  return b;
+}
+
 // Unclaimed probe ids [lo, hi): host claims move lo up, device claims move
 // hi down; the host never claims at or above host_limit.
 struct Stripes {
   std::mutex mu;
   uint32_t lo = 0, hi = 0, host_limit = 0;
 
-  bool take_low(uint32_t want, uint32_t& first, uint32_t& n) {
+  // A host claim of at most `want` sets and at most an equal share of what
+  // is left between the host threads, so the last stripes are short.
+  bool take_low(uint32_t want, uint32_t threads, uint32_t& first, uint32_t& n) {
     std::lock_guard<std::mutex> g(mu);
     const uint32_t lim = std::min(hi, host_limit);
     if (lo >= lim) return false;
-    n = std::min(want, lim - lo);
+    const uint32_t share = std::max(1u, (lim - lo + threads - 1) / threads);
+    n = std::min({want, lim - lo, share});
     first = lo;
     lo += n;
-    return true;
-  }
-  // Guided claim: half of what is left, at least min_claim sets.
-  bool take_high(uint32_t min_claim, uint32_t& first, uint32_t& n) {
-    std::lock_guard<std::mutex> g(mu);
-    if (hi <= lo) return false;
-    const uint32_t rem = hi - lo;
-    n = std::min(rem, std::max(min_claim, rem - rem / 2));
-    hi -= n;
-    first = hi;
     return true;
   }
 };
@@ -83,30 +104,7 @@ struct HostWorker {
 
 // Host stripe size when the caller gives none: about this much work per
 // claim at the thread's last measured rate.
-constexpr double kHostClaimMs = 0.25;
-
-// Device results: grows the pair buffers to hold `need` pairs, keeping the
-// first `keep`.
-void grow_output_keep(hs_ctx* ctx, uint64_t need, uint64_t keep) {
-  if (need <= ctx->out_cap) return;
-  DevBuf r, s, ov;
-  r.ensure(need * 4);
-  s.ensure(need * 4);
-  ov.ensure(need * 4);
-  if (keep) {
-    HS_CHECK_HIP(hipMemcpyAsync(r.ptr, ctx->out_r.ptr, keep * 4, hipMemcpyDeviceToDevice, ctx->stream));
-    HS_CHECK_HIP(hipMemcpyAsync(s.ptr, ctx->out_s.ptr, keep * 4, hipMemcpyDeviceToDevice, ctx->stream));
-    HS_CHECK_HIP(hipMemcpyAsync(ov.ptr, ctx->out_ov.ptr, keep * 4, hipMemcpyDeviceToDevice, ctx->stream));
-  }
-  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  std::swap(r.ptr, ctx->out_r.ptr);
-  std::swap(r.bytes, ctx->out_r.bytes);
-  std::swap(s.ptr, ctx->out_s.ptr);
-  std::swap(s.bytes, ctx->out_s.bytes);
-  std::swap(ov.ptr, ctx->out_ov.ptr);
-  std::swap(ov.bytes, ctx->out_ov.bytes);
-  ctx->out_cap = need;
-}
+constexpr double kHostClaimMs = 0.1;
 
 }  // namespace
 
@@ -128,10 +126,7 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
                "the device technique must be fgss or bitmap");
     HS_REQUIRE(opt.mode != HS_COPROC_STATIC || (opt.cpu_share >= 0.0 && opt.cpu_share <= 1.0), HS_E_INVALID,
                "cpu_share must lie in [0, 1]");
-    hs_tune tune = opt.tune;
-    // launches overlap on the stream: no plan that stages per-launch tables
-    // through shared host buffers
-    tune.flags = (tune.flags & ~(uint32_t)(HS_TUNE_BALANCED_TILES | HS_TUNE_TILE_QUEUE)) | HS_TUNE_STATIC_TILES;
+    const hs_tune tune = opt.tune;
     const auto t_start = Clock::now();
     hs_coprocess_stats cs{};
     const bool self = indexed == probe;
@@ -143,7 +138,7 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     hipStream_t st = ctx->stream;
 
     if (range_empty(ir) || range_empty(pr)) {
-      ctx->last_count = 0;
+      set_device_result(ctx, 0);
       result->count = 0;
       cs.wall_ms = ms_between(t_start, Clock::now());
       if (stats) *stats = cs;
@@ -197,6 +192,9 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     const uint32_t* h_ids = ctx->h_index_ids.as<uint32_t>();
     const int device = ctx->device;
     hipEvent_t copied = ctx->index_copied;
+    const std::vector<uint32_t>& psf = probe->h_size_first;
+    std::atomic<uint64_t> host_weight_done{0};  // work proxy of the stripes the host has joined
+    std::atomic<int64_t> host_start_ns{0};      // when the first host thread had the index
     std::vector<HostWorker> workers(nthreads);
     std::vector<std::thread> pool;
     const auto t_host = Clock::now();
@@ -207,11 +205,14 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
         try {
           HS_CHECK_HIP(hipSetDevice(device));
           HS_CHECK_HIP(hipEventSynchronize(copied));
+          int64_t zero = 0;
+          host_start_ns.compare_exchange_strong(zero, Clock::now().time_since_epoch().count());
           uint32_t want = opt.cpu_block ? opt.cpu_block : 16u, first = 0, n = 0;
-          while (S.take_low(want, first, n)) {
+          while (S.take_low(want, nthreads, first, n)) {
             const auto a = Clock::now();
             host_probe_range(*probe, *indexed, h_off, h_ids, T, self, ir, first, first + n - 1, w.out, w.stamp);
             const double dt = ms_between(a, Clock::now());
+            host_weight_done += probe_weight(psf, first, first + n);
             w.probes += n;
             ++w.claims;
             if (!opt.cpu_block) {
@@ -230,32 +231,80 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
       });
     }
 
-    // ---- device consumer (this thread): up to two stripes in flight
+    // ---- device consumer (this thread): up to two stripes in flight.  Until
+    // both sides have a measured rate the device claims half of what is
+    // left; then it claims the top part of the unclaimed ids sized so that
+    // the host, joining the rest, ends when the device does.
     uint32_t dev_first = pr.last + 1;  // the device's share is [dev_first, n)
     uint32_t tile_plan = 0;
     std::string dev_err;
     int dev_rc = HS_OK;
     const auto t_dev = Clock::now();
     try {
-      std::deque<uint32_t> inflight;  // claim numbers
+      struct Claim {
+        uint64_t weight;
+        Clock::time_point launched;
+        uint32_t ev;
+      };
+      std::deque<Claim> inflight;
       uint32_t claims = 0;
+      uint64_t dev_weight_done = 0;
+      double dev_busy_ms = 0.0;
+      Clock::time_point prev_done = t_dev;
       auto launch = [&](uint32_t first, uint32_t n) {
         tile_plan = launch_device_join(ctx, indexed, probe, P, hs_range{first, first + n - 1}, opt.algo, tune);
         HS_CHECK_HIP(hipEventRecord(ctx->claim_ev[claims & 1], st));
-        inflight.push_back(claims++);
+        inflight.push_back(Claim{probe_weight(psf, first, first + n), Clock::now(), claims & 1u});
+        ++claims;
         dev_first = first;
         cs.device_probes += n;
+      };
+      // the next device claim [x, hi) under the lock, or false
+      auto decide = [&](uint32_t& first, uint32_t& n) {
+        std::lock_guard<std::mutex> g(S.mu);
+        if (S.hi <= S.lo) return false;
+        const uint32_t rem = S.hi - S.lo;
+        const uint64_t hw = host_weight_done.load();
+        const int64_t h0 = host_start_ns.load();
+        const double host_ms = h0 ? ms_between(Clock::time_point(Clock::duration(h0)), Clock::now()) : 0.0;
+        uint32_t x;
+        if (dev_busy_ms <= 0.0 || hw == 0 || host_ms <= 0.0) {
+          x = S.hi - std::min(rem, std::max(min_claim, rem - rem / 2));
+        } else {
+          const double rd = dev_weight_done / dev_busy_ms, rh = hw / host_ms;  // weight per ms
+          double queued = 0.0;  // device work already launched
+          for (const Claim& c : inflight) queued += c.weight;
+          const double W = (double)probe_weight(psf, S.lo, S.hi);
+          // host share u: u / rh = queued / rd + (W - u) / rd
+          const double u = (queued / rd + W / rd) / (1.0 / rh + 1.0 / rd);
+          x = u >= W ? S.hi : weight_cut(psf, S.lo, S.hi, (uint64_t)u);
+          if (inflight.empty() && x == S.hi) x = S.hi - 1;  // an idle device always takes something
+          if (!inflight.empty() && S.hi - x < std::max(1u, min_claim / 4)) return false;  // not worth it yet
+        }
+        if (x >= S.hi) return false;
+        first = x;
+        n = S.hi - x;
+        S.hi = x;
+        return true;
       };
       if (opt.mode == HS_COPROC_STATIC) {
         if (S.host_limit <= pr.last) launch(S.host_limit, pr.last + 1 - S.host_limit);
       }
       for (;;) {
         uint32_t first = 0, n = 0;
-        while (opt.mode == HS_COPROC_QUEUE && inflight.size() < 2 && S.take_high(min_claim, first, n))
-          launch(first, n);
-        if (inflight.empty()) break;
-        HS_CHECK_HIP(hipEventSynchronize(ctx->claim_ev[inflight.front() & 1]));
+        while (opt.mode == HS_COPROC_QUEUE && inflight.size() < 2 && decide(first, n)) launch(first, n);
+        if (inflight.empty()) {
+          std::lock_guard<std::mutex> g(S.mu);
+          if (S.hi <= S.lo || opt.mode == HS_COPROC_STATIC) break;
+          continue;  // unreachable in practice: an idle device with work left always claims
+        }
+        const Claim c = inflight.front();
+        HS_CHECK_HIP(hipEventSynchronize(ctx->claim_ev[c.ev]));
         inflight.pop_front();
+        const auto now = Clock::now();
+        dev_busy_ms += ms_between(std::max(c.launched, prev_done), now);
+        dev_weight_done += c.weight;
+        prev_done = now;
       }
       cs.device_claims = claims;
     } catch (const Error& e) {
@@ -286,26 +335,16 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     }
     const uint64_t g = cnt[CNT_PAIRS];
 
-    // ---- host pairs behind the device's
+    // ---- host pairs: kept on the host as the tail of the context's result
     uint64_t h = 0;
+    set_device_result(ctx, g);
     for (auto& w : workers) {
       h += w.out.r.size();
       cs.host_probes += w.probes;
       cs.host_claims += w.claims;
       cs.host_ms = std::max(cs.host_ms, ms_between(t_host, w.done));
+      if (!w.out.r.empty()) ctx->host_tail.push_back(std::move(w.out));
     }
-    grow_output_keep(ctx, g + h, g);
-    uint64_t at = g;
-    for (auto& w : workers) {
-      const size_t k = w.out.r.size();
-      if (!k) continue;
-      HS_CHECK_HIP(hipMemcpyAsync(ctx->out_r.as<uint32_t>() + at, w.out.r.data(), k * 4, hipMemcpyHostToDevice, st));
-      HS_CHECK_HIP(hipMemcpyAsync(ctx->out_s.as<uint32_t>() + at, w.out.s.data(), k * 4, hipMemcpyHostToDevice, st));
-      HS_CHECK_HIP(
-          hipMemcpyAsync(ctx->out_ov.as<uint32_t>() + at, w.out.ov.data(), k * 4, hipMemcpyHostToDevice, st));
-      at += k;
-    }
-    HS_CHECK_HIP(hipStreamSynchronize(st));
     const uint64_t total = g + h;
     ctx->last_count = total;
     result->count = total;

@@ -25,6 +25,7 @@ struct PinnedBuf {
   void ensure(size_t need) {
     if (need <= bytes && ptr) return;
     release();
+    hs::bump_buffer_epoch();
     size_t want = need < 256 ? 256 : need + need / 8;
     void* p = nullptr;
     if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess) {
@@ -35,7 +36,10 @@ struct PinnedBuf {
     bytes = want;
   }
   void release() {
-    if (ptr) (void)hipHostFree(ptr);
+    if (ptr) {
+      (void)hipHostFree(ptr);
+      hs::bump_buffer_epoch();
+    }
     ptr = nullptr;
     bytes = 0;
   }
@@ -71,6 +75,12 @@ struct hs_coll {
   }
 };
 
+namespace hs {
+struct Triples {
+  std::vector<uint32_t> r, s, ov;
+};
+}  // namespace hs
+
 // One double-buffered slot of the hybrid candidate pipeline.
 struct HybridSlot {
   PinnedBuf h_probes, h_offsets, h_cands;
@@ -97,7 +107,8 @@ struct hs_ctx {
   DevBuf d_cum;
   hipEvent_t ev[6] = {};
   hs_stats stats{};
-  uint64_t last_count = 0;
+  uint64_t last_count = 0;  // pairs of the last result: on the device, then host_tail
+  std::vector<hs::Triples> host_tail;  // pairs the host joined (co-processing), after the device's
   HybridSlot slots[2];
   std::mutex submit_mu;
   std::vector<uint32_t> h_offsets, h_set_ids;  // host copy of the index (hybrid)
@@ -116,6 +127,8 @@ struct hs_ctx {
   DevBuf bal_slice_a, bal_slice_n, d_icum;  // item slices and the item table
   DevBuf bal_rec, bal_slot;                 // per-probe records of the plan
   PinnedBuf h_icum;
+  hipEvent_t icum_ev = nullptr;  // recorded after the last copy out of h_icum
+  bool icum_pending = false;
   // captured batched join: replayed while (collection, bounds, plan) match
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -123,6 +136,7 @@ struct hs_ctx {
   std::string graph_key;
   PinnedBuf graph_cum;
   uint64_t graph_entries = 0;
+  uint64_t graph_epoch = 0;  // buffer epoch the capture saw
 };
 
 
@@ -236,8 +250,20 @@ void stream_pairs(hs_ctx* ctx, uint64_t count, bool with_overlap, uint64_t chunk
                   const std::function<void(const uint32_t*, const uint32_t*, const uint32_t*, uint64_t, uint64_t)>&
                       consume);
 
-// Pairs [0, count) into the caller's arrays of res (through stream_pairs).
+// The last result on the context: its device pairs through stream_pairs,
+// then the host-held tail, in chunks of at most `chunk` pairs.
+void stream_result(hs_ctx* ctx, bool with_overlap, uint64_t chunk,
+                   const std::function<void(const uint32_t*, const uint32_t*, const uint32_t*, uint64_t, uint64_t)>&
+                       consume);
+
+// The last result (count pairs) into the caller's arrays of res.
 void copy_pairs(hs_ctx* ctx, uint64_t count, hs_result* res);
+
+// A join left `count` pairs in the device result buffers and nothing on the host.
+inline void set_device_result(hs_ctx* ctx, uint64_t count) {
+  ctx->last_count = count;
+  ctx->host_tail.clear();
+}
 
 inline hs_range clamp_range(const hs_range* r, uint32_t n) {
   hs_range out{0, 0};
@@ -273,9 +299,6 @@ void prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_range
 uint32_t launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, DevicePlan& P, hs_range pr,
                             int32_t algo, const hs_tune& tune);
 
-struct Triples {
-  std::vector<uint32_t> r, s, ov;
-};
 // CPU join of probes [first, last] of P against the prefix index (offsets,
 // ids: id-ascending lists over S's universe) of S's sets in ir, on the
 // calling thread; pairs are appended to out.  stamp is the thread's

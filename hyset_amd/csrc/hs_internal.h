@@ -33,6 +33,12 @@ struct Error {
 
 // ----------------------------------------------------------- device memory
 
+// Allocation epoch of the process: every device or pinned allocation and
+// free bumps it.  A captured graph holds raw buffer addresses, so it is
+// replayed only while the epoch still reads what it read at capture.
+uint64_t buffer_epoch();
+void bump_buffer_epoch();
+
 // A growable device allocation.  Memory is only ever grown, never shrunk,
 // so steady-state joins allocate nothing.
 struct DevBuf {

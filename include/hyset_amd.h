@@ -365,24 +365,29 @@ int hs_join_host(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* 
  * filter and verification).
  *   HS_COPROC_QUEUE: both sides claim stripes until they meet -- the shared
  *     block-pair queue of src/queue.cu:142-279 with a device consumer and
- *     host consumers.  Device claims halve the unclaimed range (at least
- *     device_min_claim sets), two in flight at a time; host claims are
- *     cpu_block sets, or sized per thread to about 1 ms of its measured rate
- *     when cpu_block is 0.
+ *     host consumers.  Until both sides have a measured rate the device
+ *     claims half of the unclaimed range (at least device_min_claim sets);
+ *     then it claims the top part sized so that the host, joining the rest
+ *     at its measured rate, ends when the device does.  Two device stripes
+ *     are in flight at a time.  Host claims are cpu_block sets, or sized per
+ *     thread to about 0.1 ms at its measured rate when cpu_block is 0, and
+ *     never more than an equal share of what is left.  Rates are taken per
+ *     unit of a work proxy: set size + 8 per probe.
  *   HS_COPROC_STATIC: the split of src/dichotomy.cu:143-285: the host joins
  *     the probe sets below cpu_share * n (Rs, the small sets), the device
  *     the rest (Rl) against everything.
  * The result is the same set of pairs as hs_join: afterwards the context's
- * device result holds the device's pairs followed by the host's, count-only
- * or copied into result as for hs_join, and hs_copy_last_result,
- * hs_stream_last_result work on it.  Device tile plans are the upload-time
- * tiles (or tune.probes_per_tile). */
+ * last result is the device's pairs followed by the host's (those stay in
+ * host memory, no copy to the device), count-only or copied into result as
+ * for hs_join, and hs_copy_last_result, hs_stream_last_result and
+ * hs_write_last_result_text deliver it in that order.  Each device stripe chooses its tile
+ * plan as hs_join does for a probe range. */
 enum { HS_COPROC_QUEUE = 0, HS_COPROC_STATIC = 1 };
 typedef struct {
   int32_t mode;              /* HS_COPROC_* */
   int32_t algo;              /* device technique: HS_ALGO_FGSS or HS_ALGO_BITMAP */
-  uint32_t cpu_threads;      /* host threads (0: all hardware threads) */
-  uint32_t cpu_block;        /* queue: probe sets per host claim (0: about 1 ms of work) */
+  uint32_t cpu_threads;      /* host threads (0: the usable CPUs less one, for the device thread) */
+  uint32_t cpu_block;        /* queue: probe sets per host claim (0: about 0.1 ms of work) */
   uint32_t device_min_claim; /* queue: smallest device claim in probe sets (0: 16384) */
   double cpu_share;          /* static: fraction of the probe sets the host joins */
   hs_tune tune;              /* device knobs */

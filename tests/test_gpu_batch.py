@@ -82,6 +82,22 @@ def test_batch_graph_replays(ctx, batch):
     assert split(b, *ctx.join_batch(b, tune=tune)) == expected(colls, 0.6)
 
 
+def test_batch_graph_recaptured_after_buffers_move(ctx, batch):
+    """A graph records raw buffer addresses.  After a finer tile plan (a
+    larger tile table) and wider signatures reallocate the buffers of the
+    batch, the next graph join with the captured configuration captures
+    anew instead of replaying, and every join equals the oracle."""
+    colls, b = batch
+    ctx.set_threshold(0.75)
+    want = expected(colls, 0.75)
+    g = H.Tune(flags=H.TUNE_GRAPH)
+    assert split(b, *ctx.join_batch(b, tune=g)) == want
+    finer = H.Tune(probes_per_tile=16, tile_tokens=256, signature_bits=1024)
+    assert split(b, *ctx.join_batch(b, tune=finer)) == want
+    for rep in range(3):
+        assert split(b, *ctx.join_batch(b, tune=g)) == want, rep
+
+
 def test_autotune_batch(ctx, batch):
     colls, b = batch
     ctx.set_threshold(0.75)
@@ -89,3 +105,7 @@ def test_autotune_batch(ctx, batch):
     assert tune.probes_per_tile in (64, 128, 256) and tune.workgroups_per_cu in (2, 4, 8) and ms > 0
     assert tune.signature_bits in (0, 64, 128, 256)
     assert split(b, *ctx.join_batch(b, tune=tune)) == expected(colls, 0.75)
+    # the tuned configuration as a graph, after the trials moved the buffers
+    tune.flags |= H.TUNE_GRAPH
+    for rep in range(2):
+        assert split(b, *ctx.join_batch(b, tune=tune)) == expected(colls, 0.75), rep
