@@ -238,10 +238,11 @@ def bench_split(ctx, H, units, want_counts) -> dict:
                       "uses_learned": sp.use_learned, "plans": summary, "judge": judge}}
 
 
-def bench_serving(H) -> dict:
+def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=600, offered=None) -> dict:
     """Serving workload: device-resident index over 2,000,000 sets; requests of
     round(exp(U(0, ln 4096))) query sets at thresholds drawn from {0.6, 0.7,
-    0.8, 0.9}; saturated throughput, then Poisson arrivals at half of it."""
+    0.8, 0.9}; saturated throughput, then Poisson arrivals at `offered`
+    requests/s (default: half the saturated rate)."""
     import random
     import threading
 
@@ -253,7 +254,7 @@ def bench_serving(H) -> dict:
     log(f"serving: index {idx.n} sets, query pool {pool.n} sets, generated in {time.perf_counter() - t0:.1f}s")
     rnd = random.Random(2)
     reqs, pos = [], 0
-    for _ in range(600):
+    for _ in range(n_requests):
         n = max(1, int(round(np.exp(rnd.uniform(0, np.log(4096))))))
         if pos + n > pool.n:
             pos = 0
@@ -266,7 +267,8 @@ def bench_serving(H) -> dict:
     d = ctx.upload(idx)
     res = {"serve_index_sets": idx.n, "serve_requests": len(reqs),
            "serve_queries": int(sum(q.n for q, _ in reqs))}
-    with H.Server(d, slots=4, hbm_budget_bytes=2 << 30) as srv:
+    res.update(serve_slots=slots, serve_max_batch_queries=max_batch_queries)
+    with H.Server(d, slots=slots, hbm_budget_bytes=2 << 30, max_batch_queries=max_batch_queries) as srv:
         for qc, t in reqs[:20]:
             srv.wait(srv.submit(qc, t))
         # saturation: everything queued at once
@@ -278,7 +280,7 @@ def bench_serving(H) -> dict:
         res.update(serve_saturated_req_per_s=round(sat_rps, 1), serve_saturated_ms=round(sat_s * 1e3, 1),
                    serve_pairs=int(pairs), serve_saturated_pairs_per_s=round(pairs / sat_s, 1))
         # offered load: Poisson arrivals at half the saturated request rate
-        lam = 0.5 * sat_rps
+        lam = offered if offered else 0.5 * sat_rps
         prng = random.Random(3)
         gaps = [prng.expovariate(lam) for _ in reqs]
         lat, done_pairs = [], [0]

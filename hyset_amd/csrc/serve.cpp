@@ -60,6 +60,7 @@ struct hs_server {
   int kind = HS_JACCARD, mode = HS_BOUNDS_REFERENCE;
   hs_server_config cfg{};
   uint32_t L = 0;  // largest set size the tables cover
+  uint32_t num_cus = 256;
   // threshold tables: entry 0 is t_min, others are added on first use
   std::mutex tmu;
   std::vector<double> thresholds;
@@ -118,6 +119,40 @@ int table_for(hs_server* S, double t) {
   return (int)S->thresholds.size() - 1;
 }
 
+// Grows a slot buffer to at least `need` bytes, at least doubling it: the
+// free inside a reallocation synchronizes the whole device (every slot's
+// work in flight), so it has to stay rare.
+template <class B>
+void grow(B& b, size_t need) {
+  if (need > b.bytes || !b.ptr) b.ensure(std::max(need, 2 * b.bytes));
+}
+
+// Sizes a slot for a batch of max_batch_queries queries of 16 tokens on
+// average with 4 result triples per query before it serves anything.
+void reserve_slot(const hs_server* S, Slot& sl) {
+  const size_t q = std::max<uint32_t>(S->cfg.max_batch_queries, 1), tok = q * 16, out = 4 * q;
+  grow(sl.h_tokens, tok * 4);
+  grow(sl.h_starts, q * 4);
+  grow(sl.h_sizes, q * 4);
+  grow(sl.h_tidx, q);
+  grow(sl.h_tiles, (q + 1) * 4);
+  grow(sl.tokens, tok * 4);
+  grow(sl.starts, q * 4);
+  grow(sl.sizes, q * 4);
+  grow(sl.tidx, q);
+  grow(sl.tiles, (q + 1) * 4);
+  grow(sl.counters, CNT_WORDS * 8);
+  grow(sl.h_counters, CNT_WORDS * 8);
+  grow(sl.out_r, out * 4);
+  grow(sl.out_s, out * 4);
+  grow(sl.out_ov, out * 4);
+  sl.out_cap = out;
+  grow(sl.h_r, out * 4);
+  grow(sl.h_s, out * 4);
+  grow(sl.h_ov, out * 4);
+  if (S->sig_words) grow(sl.sigs, q * S->sig_words * 8);
+}
+
 void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
   // concatenated probe collection of the batch
   uint64_t ntok = 0;
@@ -126,10 +161,10 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     ntok += r->tokens.size();
     nq += r->n;
   }
-  sl.h_tokens.ensure(std::max<uint64_t>(ntok, 1) * 4);
-  sl.h_starts.ensure((size_t)std::max<uint32_t>(nq, 1) * 4);
-  sl.h_sizes.ensure((size_t)std::max<uint32_t>(nq, 1) * 4);
-  sl.h_tidx.ensure(std::max<uint32_t>(nq, 1));
+  grow(sl.h_tokens, std::max<uint64_t>(ntok, 1) * 4);
+  grow(sl.h_starts, (size_t)std::max<uint32_t>(nq, 1) * 4);
+  grow(sl.h_sizes, (size_t)std::max<uint32_t>(nq, 1) * 4);
+  grow(sl.h_tidx, std::max<uint32_t>(nq, 1));
   std::vector<uint32_t> first(batch.size() + 1, 0);
   uint32_t* tk = sl.h_tokens.as<uint32_t>();
   uint32_t* st = sl.h_starts.as<uint32_t>();
@@ -150,36 +185,40 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     }
   }
   first[batch.size()] = q;
-  // tile plan of the batch (<= TILE_PROBES probes, <= TILE_TOKENS tokens)
+  // tile plan of the batch: <= TILE_TOKENS tokens and at most enough probes
+  // per tile that the batch spreads over about 8 tiles per CU (a lone
+  // request of a few thousand queries in 256-probe tiles would occupy a
+  // handful of CUs and set the latency tail)
+  const uint32_t ppt = std::clamp<uint32_t>((nq + 8 * S->num_cus - 1) / (8 * S->num_cus), 1, TILE_PROBES);
   std::vector<uint32_t> tiles;
   for (uint32_t i = 0; i < nq;) {
     tiles.push_back(i);
     uint64_t t = 0;
     uint32_t c = 0;
-    while (i < nq && c < TILE_PROBES && (c == 0 || t + sz[i] <= TILE_TOKENS)) {
+    while (i < nq && c < ppt && (c == 0 || t + sz[i] <= TILE_TOKENS)) {
       t += sz[i];
       ++c;
       ++i;
     }
   }
   tiles.push_back(nq);
-  sl.h_tiles.ensure(tiles.size() * 4);
+  grow(sl.h_tiles, tiles.size() * 4);
   std::copy(tiles.begin(), tiles.end(), sl.h_tiles.as<uint32_t>());
-  sl.tokens.ensure(std::max<uint64_t>(ntok, 1) * 4);
-  sl.starts.ensure((size_t)std::max<uint32_t>(nq, 1) * 4);
-  sl.sizes.ensure((size_t)std::max<uint32_t>(nq, 1) * 4);
-  sl.tidx.ensure(std::max<uint32_t>(nq, 1));
-  sl.tiles.ensure(tiles.size() * 4);
-  sl.counters.ensure(CNT_WORDS * 8);
-  sl.h_counters.ensure(CNT_WORDS * 8);
-  if (sl.out_cap == 0 || sl.out_cap < 4ull * nq) {
+  grow(sl.tokens, std::max<uint64_t>(ntok, 1) * 4);
+  grow(sl.starts, (size_t)std::max<uint32_t>(nq, 1) * 4);
+  grow(sl.sizes, (size_t)std::max<uint32_t>(nq, 1) * 4);
+  grow(sl.tidx, std::max<uint32_t>(nq, 1));
+  grow(sl.tiles, tiles.size() * 4);
+  grow(sl.counters, CNT_WORDS * 8);
+  grow(sl.h_counters, CNT_WORDS * 8);
+  if (sl.out_cap < 4ull * nq) {
     const uint64_t cap = std::max<uint64_t>(1u << 16, 4ull * nq);
-    sl.out_r.ensure(cap * 4);
-    sl.out_s.ensure(cap * 4);
-    sl.out_ov.ensure(cap * 4);
-    sl.out_cap = cap;
+    grow(sl.out_r, cap * 4);
+    grow(sl.out_s, cap * 4);
+    grow(sl.out_ov, cap * 4);
+    sl.out_cap = std::min({sl.out_r.bytes, sl.out_s.bytes, sl.out_ov.bytes}) / 4;
   }
-  if (S->sig_words) sl.sigs.ensure((size_t)std::max<uint32_t>(nq, 1) * S->sig_words * 8);
+  if (S->sig_words) grow(sl.sigs, (size_t)std::max<uint32_t>(nq, 1) * S->sig_words * 8);
   hipStream_t s = sl.stream;
   HS_CHECK_HIP(hipMemcpyAsync(sl.tokens.ptr, tk, ntok * 4, hipMemcpyHostToDevice, s));
   HS_CHECK_HIP(hipMemcpyAsync(sl.starts.ptr, st, (size_t)nq * 4, hipMemcpyHostToDevice, s));
@@ -221,16 +260,16 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     std::memcpy(cnt, sl.h_counters.ptr, sizeof(cnt));
     if (cnt[CNT_PAIRS] <= sl.out_cap) break;
     const uint64_t cap = cnt[CNT_PAIRS] + cnt[CNT_PAIRS] / 4 + 1024;
-    sl.out_r.ensure(cap * 4);
-    sl.out_s.ensure(cap * 4);
-    sl.out_ov.ensure(cap * 4);
-    sl.out_cap = cap;
+    grow(sl.out_r, cap * 4);
+    grow(sl.out_s, cap * 4);
+    grow(sl.out_ov, cap * 4);
+    sl.out_cap = std::min({sl.out_r.bytes, sl.out_s.bytes, sl.out_ov.bytes}) / 4;
   }
   const uint64_t np = cnt[CNT_PAIRS];
   if (np) {
-    sl.h_r.ensure(np * 4);
-    sl.h_s.ensure(np * 4);
-    sl.h_ov.ensure(np * 4);
+    grow(sl.h_r, np * 4);
+    grow(sl.h_s, np * 4);
+    grow(sl.h_ov, np * 4);
     HS_CHECK_HIP(hipMemcpyAsync(sl.h_r.ptr, sl.out_r.ptr, np * 4, hipMemcpyDeviceToHost, s));
     HS_CHECK_HIP(hipMemcpyAsync(sl.h_s.ptr, sl.out_s.ptr, np * 4, hipMemcpyDeviceToHost, s));
     HS_CHECK_HIP(hipMemcpyAsync(sl.h_ov.ptr, sl.out_ov.ptr, np * 4, hipMemcpyDeviceToHost, s));
@@ -253,6 +292,13 @@ void worker(hs_server* S) {
   std::string fail;
   if (hipSetDevice(S->device) != hipSuccess || hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess)
     fail = "cannot create a stream";
+  if (fail.empty()) {
+    try {
+      reserve_slot(S, sl);
+    } catch (const Error& e) {
+      fail = e.msg;
+    }
+  }
   for (;;) {
     std::vector<Request*> batch;
     uint64_t bytes = 0;
@@ -351,6 +397,9 @@ int hs_server_create(int device, hs_coll* index, int32_t sim, int32_t bounds_mod
     HS_CHECK_HIP(hipSetDevice(device));
     S->device = device;
     S->index = index;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+      S->num_cus = (uint32_t)cus;
     S->kind = sim;
     S->mode = bounds_mode;
     hs_server_config c{0.6, 4, 1ull << 30, 65536, 0};
