@@ -622,7 +622,7 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
           if (hi > lo) items += (uint64_t)(hi - lo) * ctx->tables.maxprefix[std::min(l, ctx->tables.L)];
         }
         const double mean_list = (double)P.index_entries / std::max<uint32_t>(indexed->universe, 1);
-        balanced = items >= (1u << 18) && mean_list >= 3.5;
+        balanced = items >= (1u << 21) && mean_list >= 3.5;
       }
     }
     if (balanced) {

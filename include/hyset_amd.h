@@ -204,7 +204,7 @@ enum { HS_TUNE_GRAPH = 1 };
  * tiles of equal estimated work, planned on the device after the index
  * build by a pass that locates every probe-prefix item's posting-list
  * slice, claimed by the persistent grid.  Default (none of these): the
- * balanced plan for joins of at least 2^18 probe-prefix items whose index
+ * balanced plan for joins of at least 2^21 probe-prefix items whose index
  * holds at least 3.5 entries per token of the universe (long posting lists,
  * so tiles differ much in work), the upload-time tiles otherwise. */
 enum { HS_TUNE_TILE_QUEUE = 2, HS_TUNE_BALANCED_TILES = 8, HS_TUNE_STATIC_TILES = 16 };
