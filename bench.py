@@ -238,7 +238,7 @@ def bench_split(ctx, H, units, want_counts) -> dict:
                       "uses_learned": sp.use_learned, "plans": summary, "judge": judge}}
 
 
-def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=600, offered=None) -> dict:
+def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=2000, offered=None) -> dict:
     """Serving workload: device-resident index over 2,000,000 sets; requests of
     round(exp(U(0, ln 4096))) query sets at thresholds drawn from {0.6, 0.7,
     0.8, 0.9}; saturated throughput, then Poisson arrivals at `offered`
