@@ -494,15 +494,26 @@ def main() -> int:
         extra["candidates_no_signature"] = int(ctx.stats()["candidates"])
         log(f"k_fgss without the signature test: {extra['k_fgss_no_signature_ms']} ms, "
             f"{extra['candidates_no_signature']} candidates verified")
+        # the bitmap filter is bound by integer VALU work, not HBM: its pair
+        # tests (2b/32 + 4 operations each) against the measured rate of
+        # XOR-popcount chains on this device
+        popc_gops = ctx.popcount_rate()
+        extra["popcount_peak_gops"] = round(popc_gops, 1)
         for bits in (64, 128, 256):
             tune = H.Tune(bitmap_bits=bits)
             c = ctx.join(d, algo="bitmap", tune=tune, count_only=True)
             sb = ctx.stats()
             extra[f"bitmap{bits}_ms"] = round(sb["total_ms"], 3)
+            extra[f"bitmap{bits}_kernel_ms"] = round(sb["kernel_ms"], 3)
             extra[f"bitmap{bits}_pair_tests"] = int(sb["posting_visits"])
             extra[f"bitmap{bits}_survivors"] = int(sb["candidates"])
+            ops = sb["posting_visits"] * (2 * bits // 32 + 4)
+            gops = ops / (sb["kernel_ms"] * 1e-3) / 1e9
+            extra[f"bitmap{bits}_valu"] = {"achieved_gops": round(gops, 1), "peak_gops": round(popc_gops, 1),
+                                           "frac": round(gops / popc_gops, 3)}
             assert c == pairs
-            log(f"bitmap{bits}: {sb['total_ms']:.2f} ms, {sb['posting_visits']} tests, {sb['candidates']} survivors")
+            log(f"bitmap{bits}: {sb['total_ms']:.2f} ms, {sb['posting_visits']} tests, {sb['candidates']} survivors, "
+                f"{gops:.0f} Gop/s of {popc_gops:.0f} ({gops / popc_gops:.2f})")
         a = time.perf_counter()
         c = ctx.join(d, algo="hybrid", tune=H.Tune(host_threads=min(16, ncores)), count_only=True)
         extra["hybrid_ms_wall"] = round((time.perf_counter() - a) * 1e3, 2)

@@ -160,6 +160,7 @@ _sig("hs_join_batch", C.c_int, _vp, _vp, C.POINTER(Tune), C.POINTER(Result))
 _sig("hs_autotune_batch", C.c_int, _vp, _vp, C.c_int32, C.POINTER(Tune), C.POINTER(C.c_double))
 _sig("hs_get_stats", C.c_int, _vp, C.POINTER(Stats))
 _sig("hs_copy_bandwidth", C.c_int, _vp, C.c_uint64, C.c_int32, C.POINTER(C.c_double))
+_sig("hs_popcount_rate", C.c_int, _vp, C.c_int32, C.POINTER(C.c_double))
 _sig("hs_server_create", C.c_int, C.c_int, _vp, C.c_int32, C.c_int32, C.POINTER(ServerConfig), C.POINTER(_vp))
 _sig("hs_server_submit", C.c_int, _vp, _u32p, _u32p, C.c_uint32, C.c_uint64, C.c_double, C.POINTER(C.c_uint64))
 _sig("hs_server_wait", C.c_int, _vp, C.c_uint64, C.POINTER(Result), C.POINTER(C.c_double))
@@ -548,6 +549,12 @@ class Context:
         st = Stats()
         _lib.hs_get_stats(self._h, C.byref(st))
         return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+    def popcount_rate(self, reps=10) -> float:
+        """Giga-operations/s of XOR-popcount chains (hs_popcount_rate)."""
+        g = C.c_double()
+        self._check(_lib.hs_popcount_rate(self._h, int(reps), C.byref(g)), "hs_popcount_rate")
+        return g.value
 
     def copy_bandwidth(self, nbytes=1 << 30, reps=10) -> float:
         """GB/s (read + write) of a plain device copy kernel."""

@@ -907,6 +907,25 @@ int hs_bounds(int32_t sim, double threshold, int32_t bounds_mode, uint32_t l1, u
   });
 }
 
+int hs_popcount_rate(hs_ctx* ctx, int32_t reps, double* gops) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    HS_REQUIRE(gops && reps > 0, HS_E_INVALID, "bad popcount-rate arguments");
+    HS_REQUIRE(ctx->device >= 0, HS_E_INVALID, "a host-only context has no device");
+    DevBuf out;
+    out.ensure(4);
+    const uint32_t grid = (uint32_t)ctx->num_cus * 16, iters = 2048;
+    hs::launch_popc_peak(ctx->stream, grid, iters, out.as<uint32_t>());  // warm-up
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[0], ctx->stream));
+    for (int i = 0; i < reps; ++i) hs::launch_popc_peak(ctx->stream, grid, iters, out.as<uint32_t>());
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[1], ctx->stream));
+    HS_CHECK_HIP(hipEventSynchronize(ctx->ev[1]));
+    const double ms = elapsed(ctx->ev[0], ctx->ev[1]);
+    const double ops = (double)grid * 256.0 * iters * 4 * 8 * 2 * reps;
+    *gops = ms > 0 ? ops / (ms * 1e-3) / 1e9 : 0.0;
+  });
+}
+
 int hs_copy_bandwidth(hs_ctx* ctx, uint64_t bytes, int32_t reps, double* gbps) {
   if (!ctx) return HS_E_INVALID;
   return guarded(ctx, [&] {

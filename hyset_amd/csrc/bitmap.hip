@@ -232,4 +232,32 @@ void launch_bitmap(hipStream_t st, const BitmapArgs& a) {
   HS_CHECK_HIP(hipGetLastError());
 }
 
+// ---- integer-op peak of the XOR-popcount test, for the bitmap roofline.
+// Every lane runs 8 independent chains a = popcount(a ^ m) + a (one XOR and
+// one popcount-with-accumulate per step, 2 VALU operations); the result is
+// stored only when it equals a value it cannot take, which keeps the
+// chains live without per-lane stores.
+__global__ __launch_bounds__(256) void k_popc_peak(uint32_t seed, uint32_t iters, uint32_t* out) {
+  uint32_t a[8];
+  const uint32_t m = seed * 0x9E3779B9u + 1u;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) a[k] = seed + (blockIdx.x * blockDim.x + threadIdx.x) * 8u + (uint32_t)k;
+  for (uint32_t i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] = (uint32_t)__popc(a[k] ^ m) + a[k];
+    }
+  }
+  uint32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += a[k];
+  if (s == 0xFFFFFFFFu) out[0] = s;
+}
+
+void launch_popc_peak(hipStream_t st, uint32_t grid, uint32_t iters, uint32_t* out) {
+  hipLaunchKernelGGL(k_popc_peak, dim3(grid), dim3(256), 0, st, 12345u, iters, out);
+  HS_CHECK_HIP(hipGetLastError());
+}
+
 }  // namespace hs

@@ -137,6 +137,8 @@ struct DevOut {
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_fill_u64(hipStream_t st, unsigned long long* p, size_t n, unsigned long long v);
 void launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
+// XOR-popcount chains: grid x 256 lanes x iters x 4 x 8 steps of 2 VALU ops.
+void launch_popc_peak(hipStream_t st, uint32_t grid, uint32_t iters, uint32_t* out);
 void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* out2, size_t n,
                            DevBuf& scratch);
 

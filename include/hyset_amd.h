@@ -494,6 +494,13 @@ int hs_get_stats(const hs_ctx* ctx, hs_stats* out);
  * roofline fractions on the machine at hand. */
 int hs_copy_bandwidth(hs_ctx* ctx, uint64_t bytes, int32_t reps, double* gbps);
 
+/* The integer-op rate of the XOR-popcount test on this device: lanes run
+ * independent chains of a = popcount(a ^ m) + a (2 VALU operations a step)
+ * over 16 workgroups of 256 lanes per CU, `reps` timed launches; *gops
+ * receives operations per second / 1e9.  The reference rate for the bitmap
+ * filter's roofline (it is bound by these operations, not by HBM). */
+int hs_popcount_rate(hs_ctx* ctx, int32_t reps, double* gops);
+
 #ifdef __cplusplus
 }
 #endif
