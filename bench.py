@@ -36,12 +36,13 @@ def fgss_bytes(st: dict, n_probes: int, n_probe_tokens: int) -> int:
     """Algorithmic HBM bytes of one k_fgss launch: probe tokens (4 B each),
     a 16 B record and the signature (W words of 8 B) per probe, per
     probe-prefix item its posting-list slice (8 B), per posting visited its
-    set id and size/position word (8 B), per signature test the indexed
-    set's signature (8 W B), per verified candidate its start and size plus
+    index record (set id, size/position word and, with 64-bit signatures,
+    the set's signature: 16 B; else 8 B, and per signature test the indexed
+    set's W-word signature), per verified candidate its start and size plus
     4 B per token, 12 B per result triple."""
     w = st["signature_bits"] // 64
-    return (4 * n_probe_tokens + (16 + 8 * w) * n_probes + 8 * st["probe_prefix_tokens"]
-            + 8 * st["posting_visits"] + 8 * w * st["signature_tests"]
+    visit = 16 * st["posting_visits"] if w == 1 else 8 * st["posting_visits"] + 8 * w * st["signature_tests"]
+    return (4 * n_probe_tokens + (16 + 8 * w) * n_probes + 8 * st["probe_prefix_tokens"] + visit
             + 8 * st["candidates"] + 4 * st["candidate_tokens"] + 12 * st["pairs"])
 
 

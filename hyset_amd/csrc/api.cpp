@@ -578,7 +578,7 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     hs::FgssArgs a{};
     a.probe = probe->dev();
     a.indexed = indexed->dev();
-    a.index = hs::dev_index(ctx->ib, P.sig_words == 1);
+    a.index = hs::dev_index(ctx->ib);
     a.bounds = ctx->dev_bounds;
     a.size_first = indexed->size_first.as<uint32_t>();
     a.size_first_len = (uint32_t)indexed->h_size_first.size();

@@ -115,7 +115,7 @@ void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entr
   FgssArgs a{};
   a.probe = c->dev();
   a.indexed = c->dev();
-  a.index = dev_index(ctx->ib, sig_words == 1);
+  a.index = dev_index(ctx->ib);
   a.bounds = ctx->dev_bounds;
   a.size_first = c->size_first.as<uint32_t>();
   a.self = 1;

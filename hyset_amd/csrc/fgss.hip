@@ -149,8 +149,25 @@ struct TileScope {
 // scratch.  n_hi must come in as n_lo.
 constexpr uint32_t KARY = 4;
 
+// Index record e: set id, meta and (64-bit signature tests) the signature,
+// in one load.
+template <int kW>
+__device__ __forceinline__ void load_entry(const DevIndex& X, uint32_t e, uint32_t& sid, uint32_t& meta,
+                                           uint64_t& sig) {
+  if constexpr (kW == 1) {
+    const uint4 v = reinterpret_cast<const uint4*>(X.ent)[e];
+    sid = v.x;
+    meta = v.y;
+    sig = ((uint64_t)v.w << 32) | v.z;
+  } else {
+    const uint2 v = *reinterpret_cast<const uint2*>(X.ent + (size_t)e * X.ent_words);
+    sid = v.x;
+    meta = v.y;
+  }
+}
+
 template <uint32_t K>
-__device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids, uint32_t (&f_lo)[K],
+__device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids, uint32_t stride, uint32_t (&f_lo)[K],
                                                 uint32_t (&n_lo)[K], uint32_t (&f_hi)[K], uint32_t (&n_hi)[K],
                                                 const uint32_t (&k_lo)[K], const uint32_t (&k_hi)[K]) {
   // per search: the answer lies in [lo, hi] (ids below lo are < key, ids at
@@ -177,7 +194,7 @@ __device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids
       for (uint32_t q = 0; q < KARY; ++q) {
         const uint32_t pos = len <= KARY ? lo[k] + q : lo[k] + (uint32_t)((uint64_t)(q + 1) * len / (KARY + 1));
         const bool rd = ph[k] < 2 && (len > KARY || q < len);
-        v[k][q] = rd ? ids[f_lo[k] + pos] : 0xFFFFFFFFu;
+        v[k][q] = rd ? ids[(size_t)(f_lo[k] + pos) * stride] : 0xFFFFFFFFu;
       }
     }
 #pragma unroll
@@ -387,7 +404,9 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
           }
         }
       }
-      lockstep_bounds<PER>(A.index.set_ids, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
+      // on the compact ids: a list is searched over more entries than its
+      // window visits, so 4-byte ids touch fewer lines than the records
+      lockstep_bounds<PER>(A.index.set_ids, 1u, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
 #pragma unroll
       for (uint32_t k = 0; k < PER; ++k) {
         const uint32_t it = tid + k * FG_TPB;
@@ -434,10 +453,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     uint64_t n_sig = 0;
     if (tid < total_vis) {
       n_it = segment_of(L.vis_off, nb, tid);
-      const uint32_t e = L.it_a[n_it] + (tid - L.vis_off[n_it]);
-      n_sid = A.index.set_ids[e];
-      n_meta = A.index.meta[e];
-      if constexpr (kW == 1) n_sig = A.index.esig[e];
+      load_entry<kW>(A.index, L.it_a[n_it] + (tid - L.vis_off[n_it]), n_sid, n_meta, n_sig);
     }
     for (uint32_t vb = 0; vb < vis_end; vb += FG_TPB, ++rnd) {
       const uint32_t vi = vb + tid;
@@ -446,10 +462,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       const uint32_t vn = vi + FG_TPB;
       if (vn < total_vis) {
         n_it = gallop ? segment_from(L.vis_off, nb, vn, n_it) : segment_of(L.vis_off, nb, vn);
-        const uint32_t e = L.it_a[n_it] + (vn - L.vis_off[n_it]);
-        n_sid = A.index.set_ids[e];
-        n_meta = A.index.meta[e];
-        if constexpr (kW == 1) n_sig = A.index.esig[e];
+        load_entry<kW>(A.index, L.it_a[n_it] + (vn - L.vis_off[n_it]), n_sid, n_meta, n_sig);
       }
       if (tid == 0) L.qadd[(rnd + 1) % 3] = 0;  // its last reader passed the previous barrier
       bool keep = false;
@@ -718,7 +731,7 @@ __global__ __launch_bounds__(256) void k_item_slices(FgssArgs A, const uint32_t*
       }
     }
   }
-  lockstep_bounds<IS_PER>(A.index.set_ids, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
+  lockstep_bounds<IS_PER>(A.index.set_ids, 1u, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
 #pragma unroll
   for (uint32_t k = 0; k < IS_PER; ++k) {
     const uint32_t s = base + k * 256;
@@ -828,7 +841,8 @@ static void launch_width(hipStream_t st, const FgssArgs& a, dim3 grid, uint32_t 
 
 static void check_sigs(const FgssArgs& a) {
   HS_REQUIRE(a.sig_words == 0 || (a.probe_sigs && a.indexed_sigs), HS_E_INVALID, "signature arrays missing");
-  HS_REQUIRE(a.sig_words != 1 || a.index.esig, HS_E_INVALID, "64-bit signature test needs an index with signatures");
+  HS_REQUIRE(a.sig_words != 1 || a.index.ent_words == 4, HS_E_INVALID,
+             "64-bit signature test needs an index with signatures");
 }
 
 void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persistent_wgs) {

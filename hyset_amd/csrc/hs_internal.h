@@ -103,16 +103,19 @@ struct DevColl {
 };
 
 // Index entry e of list(t) = [offsets[t], offsets[t+1]): set_ids[e] (lists
-// ascending), meta[e] = (min(|s|, 0xFFFF) << 16) | min(position of t in s,
-// 0xFFFF), and, for 64-bit signature tests, esig[e] = the set's signature.
-// Size, position and signature ride along so a posting visit is filtered
-// from coalesced loads of its own entry, without reading sizes[] or the
-// signature table at random.
+// ascending) and the record ent[e * ent_words ..] = {set id, meta, and for
+// an index built with 64-bit signatures the set's signature (low, high
+// word)}, meta = (min(|s|, 0xFFFF) << 16) | min(position of t in s,
+// 0xFFFF).  Size, position and signature ride along so a posting visit is
+// filtered from its own entry, without reading sizes[] or the signature
+// table at random; interleaved, a visit reads one record instead of three
+// arrays.  The window search reads the compact set_ids (a list is searched
+// over more entries than its window visits).
 struct DevIndex {
   const uint32_t* offsets;  // universe + 1
-  const uint32_t* set_ids;
-  const uint64_t* esig;     // NULL unless the index was built with 64-bit signatures
-  const uint32_t* meta;
+  const uint32_t* set_ids;  // the ids alone (host copies, the item-slice pass)
+  const uint32_t* ent;      // ent_words words per entry
+  uint32_t ent_words;       // 4 with 64-bit signatures, else 2
   uint32_t universe;
 };
 
@@ -143,8 +146,9 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
                            DevBuf& scratch);
 
 struct IndexBuild {
-  DevBuf offsets, set_ids, positions, esig, meta;  // the index
+  DevBuf offsets, set_ids, positions, ent;  // the index
   DevBuf keys[2], vals, pos, hist, scan_scratch;  // radix-sort work space
+  uint32_t ent_words = 2;
   uint32_t universe = 0;
   uint64_t entries = 0;
 };
@@ -153,7 +157,7 @@ struct IndexBuild {
 // size_first[l] is the first set id of size >= l; cum[l] is the number of
 // prefix entries of the range's sets shorter than l; `entries` is the total.
 // sigs (may be NULL): one 64-bit signature per set of c, copied into the
-// entries (esig).  sigs_out (may be NULL): the emit pass computes the
+// entries (4-word records).  sigs_out (may be NULL): the emit pass computes the
 // 64-bit signatures of the range's sets into it on the way (then pass the
 // same array as sigs).  counters (may be NULL): CNT_WORDS work counters the
 // last kernel zeroes, with the index size in CNT_ENTRIES (only when the
@@ -168,10 +172,10 @@ void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_fi
                        const uint32_t* prefix_len, const uint32_t* size_first, const uint32_t* sf_base,
                        const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib,
                        const uint64_t* sigs, uint64_t* sigs_out = nullptr);
-// The device view of a built index (esig only when it was built with sigs).
-inline DevIndex dev_index(const IndexBuild& ib, bool with_sigs) {
-  return DevIndex{ib.offsets.as<uint32_t>(), ib.set_ids.as<uint32_t>(), with_sigs ? ib.esig.as<uint64_t>() : nullptr,
-                  ib.meta.as<uint32_t>(), ib.universe};
+// The device view of a built index.
+inline DevIndex dev_index(const IndexBuild& ib) {
+  return DevIndex{ib.offsets.as<uint32_t>(), ib.set_ids.as<uint32_t>(), ib.ent.as<uint32_t>(), ib.ent_words,
+                  ib.universe};
 }
 
 // Batched launch: tile k covers probes [tile_lo[k], tile_hi[k]) of join

@@ -435,7 +435,7 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
 __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, uint32_t universe,
                                uint32_t* __restrict__ offsets, DevColl c, const uint32_t* __restrict__ set_ids,
                                const uint16_t* __restrict__ positions, const uint64_t* __restrict__ sigs,
-                               uint64_t* __restrict__ esig, uint32_t* __restrict__ meta,
+                               uint32_t* __restrict__ ent,
                                unsigned long long* __restrict__ counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
@@ -448,8 +448,13 @@ __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, ui
   for (uint32_t t = prev; t <= cur && t <= universe; ++t) offsets[t] = i;
   if (i < n) {
     const uint32_t sid = set_ids[i];
-    if (sigs) esig[i] = sigs[sid];
-    meta[i] = (min(c.sizes[sid], 0xFFFFu) << 16) | positions[i];
+    const uint32_t m = (min(c.sizes[sid], 0xFFFFu) << 16) | positions[i];
+    if (sigs) {
+      const uint64_t g = sigs[sid];
+      reinterpret_cast<uint4*>(ent)[i] = make_uint4(sid, m, (uint32_t)g, (uint32_t)(g >> 32));
+    } else {
+      reinterpret_cast<uint2*>(ent)[i] = make_uint2(sid, m);
+    }
   }
 }
 
@@ -461,8 +466,7 @@ static void prepare(IndexBuild& ib, uint32_t universe, uint64_t entries) {
   ib.offsets.ensure(((size_t)universe + 1) * sizeof(uint32_t));
   ib.set_ids.ensure(cap * sizeof(uint32_t));
   ib.positions.ensure(cap * sizeof(uint16_t));
-  ib.esig.ensure(cap * sizeof(uint64_t));
-  ib.meta.ensure(cap * sizeof(uint32_t));
+  ib.ent.ensure(cap * 4 * sizeof(uint32_t));
   ib.keys[0].ensure(cap * sizeof(uint32_t));
   ib.keys[1].ensure(cap * sizeof(uint32_t));
   ib.vals.ensure(cap * sizeof(uint32_t));
@@ -512,7 +516,7 @@ static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32
   }
   hipLaunchKernelGGL(k_index_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
                      ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(), sigs,
-                     ib.esig.as<uint64_t>(), ib.meta.as<uint32_t>(), counters);
+                     ib.ent.as<uint32_t>(), counters);
   HS_CHECK_HIP(hipGetLastError());
 }
 
@@ -524,6 +528,7 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters) {
   prepare(ib, universe, entries);
+  ib.ent_words = sigs ? 4u : 2u;
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && first <= last && n > 0) {
     const RadixPlan plan = radix_plan(universe);
@@ -547,6 +552,7 @@ void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_fi
                        const uint64_t* sigs, uint64_t* sigs_out) {
   unsigned long long* counters = nullptr;
   prepare(ib, universe, entries);
+  ib.ent_words = sigs ? 4u : 2u;
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && n > 0) {
     const RadixPlan plan = radix_plan(universe);

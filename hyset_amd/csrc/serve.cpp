@@ -232,7 +232,7 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     FgssArgs a{};
     a.probe = DevColl{sl.tokens.as<uint32_t>(), sl.starts.as<uint32_t>(), sl.sizes.as<uint32_t>(), nq};
     a.indexed = I->dev();
-    a.index = dev_index(S->ib, S->sig_words == 1);
+    a.index = dev_index(S->ib);
     a.bounds = S->hbounds[0];
     a.size_first = I->size_first.as<uint32_t>();
     a.size_first_len = (uint32_t)I->h_size_first.size();
