@@ -671,7 +671,7 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
       // a persistent grid claims the tiles from per-XCD queues
       ctx->batch_queue.ensure(8 * sizeof(uint32_t));
       a.tile_queue = ctx->batch_queue.as<uint32_t>();
-      wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : 4);
+      wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : 2);
     }
     HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));  // kernel_ms: the fgss launch alone
     hs::launch_fgss(st, a, ppt, wgs);

@@ -248,7 +248,7 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
     const uint32_t probes = tune.probes_per_tile ? std::min<uint32_t>(tune.probes_per_tile, TILE_PROBES) : TILE_PROBES;
     const uint32_t tokens = tune.tile_tokens ? std::min<uint32_t>(tune.tile_tokens, TILE_TOKENS) : TILE_TOKENS;
     if (probes != batch->plan_probes || tokens != batch->plan_tokens) plan_batch_tiles(batch, probes, tokens);
-    const uint32_t wpc = tune.workgroups_per_cu ? tune.workgroups_per_cu : 4;
+    const uint32_t wpc = tune.workgroups_per_cu ? tune.workgroups_per_cu : 2;
     const uint32_t workgroups = (uint32_t)ctx->num_cus * wpc;
     ctx->batch_queue.ensure(8 * 4);
     ctx->stats = hs_stats{};
@@ -342,8 +342,8 @@ int hs_autotune_batch(hs_ctx* ctx, hs_coll* batch, int32_t reps, hs_tune* best, 
       }
     };
     // tile granularity and grid at the default signature width, then the width
-    for (uint32_t probes : {64u, 128u, 256u}) {
-      for (uint32_t wpc : {2u, 4u, 8u}) {
+    for (uint32_t probes : {64u, 128u, 256u, 512u}) {
+      for (uint32_t wpc : {1u, 2u, 4u}) {
         hs_tune t{};
         t.probes_per_tile = probes;
         t.tile_tokens = probes * 16;

@@ -9,9 +9,14 @@
 // of r with the index prefix of s and reach minoverlap(|r|, |s|).
 //
 // Design here:
-//   * a workgroup owns a tile of consecutive probes (at most 256 probes and
-//     4096 tokens, planned once per collection); their tokens are one
-//     contiguous CSR range and are staged in LDS with coalesced loads;
+//   * a workgroup of 512 lanes owns a tile of consecutive probes (at most
+//     512 probes and 8192 tokens, planned once per collection); their
+//     tokens are one contiguous CSR range and are staged in LDS with
+//     coalesced loads.  A tile's work is a chain of dependent loads and
+//     barriers whose length hardly depends on the tile's size, so the
+//     kernel's time follows the number of tiles: 512-probe tiles in two
+//     workgroups per CU ran 8-18% faster than 256-probe tiles in four (the
+//     same 16 waves per CU), 2048-token tiles 1.5x slower;
 //   * the length window of a probe is an id interval (sets are sorted by
 //     size), so each (probe, prefix token) "item" binary-searches its slice
 //     of the id-sorted posting list -- no length checks per posting;
@@ -32,7 +37,7 @@
 //     index entries and those of the tile's probes in LDS, so the test reads
 //     nothing beyond the entry; wider ones are read from the per-set table;
 //   * visits that pass both are pushed onto an LDS stack (one LDS atomic per
-//     wave) and verified 256 at a time, one per lane, so the merges run with
+//     wave) and verified 512 at a time, one per lane, so the merges run with
 //     full waves instead of in the few lanes whose visit passed;
 //   * the merge is driven by s, whose tokens arrive 12 at a time in
 //     registers, against r in LDS; it keeps the early-termination bound of
@@ -48,14 +53,14 @@
 
 namespace hs {
 
-constexpr int FG_TPB = 256;                   // threads per workgroup, also max probes per tile
-constexpr uint32_t FG_TOKCAP = TILE_TOKENS;    // staged probe tokens (16 KiB)
+constexpr int FG_TPB = 512;                   // threads per workgroup, also max probes per tile
+constexpr uint32_t FG_TOKCAP = TILE_TOKENS;    // staged probe tokens (32 KiB)
 static_assert(TILE_PROBES == FG_TPB, "one probe per thread in phase A");
-constexpr uint32_t FG_ITEMCAP = 768;  // items per batch
+constexpr uint32_t FG_ITEMCAP = 1536; // items per batch
 constexpr uint32_t NO_LDS = 0xFFFFFFFFu;
 constexpr uint32_t FG_MOCAP = 1024;   // minoverlap entries kept in LDS (|r| + |s| < 1024)
-// Candidate stack: at most 255 entries wait between rounds and a round of
-// 256 visits adds at most 256.
+// Candidate stack: fewer than FG_TPB entries wait between rounds and a round
+// of FG_TPB visits adds at most FG_TPB.
 constexpr uint32_t FG_QCAP = 2 * FG_TPB;
 
 constexpr uint32_t SREG = 12;  // tokens of s held in registers per refill
@@ -118,7 +123,7 @@ struct FgssLds {
   uint32_t slot0;                   // first item slot of the tile (item-slice pass)
   uint32_t mo_lds;                  // 1: mo[] holds the minoverlap table (keyed by |r| + |s|)
   uint16_t mo[FG_MOCAP];
-  // candidate stack: probe slot | position in r << 8, indexed set id; the
+  // candidate stack: probe slot | position in r << 9, indexed set id; the
   // entries of round k are counted in qadd[k % 3] (cleared two rounds ahead)
   uint32_t q_pj[FG_QCAP];
   uint32_t q_sid[FG_QCAP];
@@ -267,7 +272,7 @@ __device__ __forceinline__ uint32_t tile_minoverlap(const FgssArgs& A, const Fgs
 }
 
 // Verifies one candidate of the stack (valid lanes) and appends the passing
-// pairs; called by every lane of the wave.  pj = probe slot | j << 8, where
+// pairs; called by every lane of the wave.  pj = probe slot | j << 9, where
 // r[j] is the token through which the candidate was reached.
 template <bool kMulti>
 __device__ __forceinline__ void verify_candidate(const FgssArgs& A, const FgssLds& L, uint32_t p0, bool valid,
@@ -275,8 +280,8 @@ __device__ __forceinline__ void verify_candidate(const FgssArgs& A, const FgssLd
   bool pass = false;
   uint32_t ov = 0, rid = 0;
   if (valid) {
-    const uint32_t pl = pj & 0xFFu;
-    const uint32_t j = pj >> 8;
+    const uint32_t pl = pj & 0x1FFu;
+    const uint32_t j = pj >> 9;
     rid = p0 + pl;
     const uint32_t sst = A.indexed.starts[sid];
     const uint32_t ls = A.indexed.sizes[sid];
@@ -375,7 +380,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
           const uint32_t item = ib + it;
           const uint32_t pl = segment_of(L.item_off, np, item);
           const uint32_t j = item - L.item_off[pl];
-          L.it_pj[it] = pl | (j << 8);
+          L.it_pj[it] = pl | (j << 9);
           L.it_a[it] = A.slice_a[L.slot0 + item];
           len = A.slice_n[L.slot0 + item];
         }
@@ -393,7 +398,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
           const uint32_t j = item - L.item_off[pl];
           const uint32_t* r = (L.plds[pl] != NO_LDS) ? (L.tok + L.plds[pl]) : (A.probe.tokens + L.pstart[pl]);
           const uint32_t token = r[j];
-          L.it_pj[it] = pl | (j << 8);
+          L.it_pj[it] = pl | (j << 9);
           k_lo[k] = L.lo[pl];
           k_hi[k] = L.hi[pl];
           if (token < A.index.universe) {
@@ -439,13 +444,13 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     // one visit per lane; the index entry of the lane's next visit is
     // fetched before this one is filtered, so its latency overlaps the
     // filter's loads.  Visits passing both filters go onto the LDS stack;
-    // whenever it holds 256 they are verified, one per lane.  The last batch
+    // whenever it holds FG_TPB they are verified, one per lane.  The last batch
     // of the tile ends with one round without visits that verifies the rest
     // of the stack (one verification site keeps the register budget).
     const bool last_batch = ib + FG_ITEMCAP >= total_items;
     const uint32_t vis_end = total_vis + (last_batch ? FG_TPB : 0u);
     // (64-bit signatures ride in the index entry and arrive with it).  A
-    // lane's next visit lies 256 further on: with long slices (long sets) it
+    // lane's next visit lies FG_TPB further on: with long slices (long sets) it
     // is in the same item or a near one, found by galloping from the current
     // item; with short slices a bisection over the batch's items is cheaper.
     const bool gallop = total_vis >= 64u * nb;
@@ -470,8 +475,8 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       if (vi < total_vis) {
         const uint32_t kp = meta & 0xFFFFu;
         pj = L.it_pj[it];
-        const uint32_t pl = pj & 0xFFu;
-        const uint32_t j = pj >> 8;
+        const uint32_t pl = pj & 0x1FFu;
+        const uint32_t j = pj >> 9;
         const uint32_t lr = L.psize[pl];
         const uint32_t ls = (meta >> 16) == 0xFFFFu ? A.indexed.sizes[sid] : (meta >> 16);
         const uint32_t mo = tile_minoverlap<kMulti>(A, L, pl, lr, ls);
@@ -517,7 +522,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       }
       __syncthreads();
       qn += L.qadd[rnd % 3];
-      // 256 entries, or in the final round (no visits, fewer than 256 left) the rest
+      // FG_TPB entries, or in the final round (no visits, fewer left) the rest
       const uint32_t take = qn >= FG_TPB ? FG_TPB : (vb + FG_TPB >= vis_end && last_batch ? qn : 0u);
       if (take) {
         const bool valid = tid < take;
@@ -742,11 +747,11 @@ __global__ __launch_bounds__(256) void k_item_slices(FgssArgs A, const uint32_t*
   }
 }
 
-// Per probe a cost in units of COST_W-th of a tile: the largest of 4 (so a
-// tile holds at most 256 probes), a quarter per token (at most 4096 tokens
-// besides the last probe's) and an eighth per posting it visits (about 8K
-// visits per tile), capped at one tile.  Taking the largest term keeps all
-// three limits while a tile of light probes still takes 256 of them.  Tile
+// Per probe a cost in units of COST_W-th of a tile: the largest of 2 (so a
+// tile holds at most 512 probes), an eighth per token (at most 8192 tokens
+// besides the last probe's) and a sixteenth per posting it visits (about
+// 16K visits per tile), capped at one tile.  Taking the largest term keeps
+// all three limits while a tile of light probes still takes 512 of them.  Tile
 // k takes the probes whose exclusive cost prefix lies in [k W, (k+1) W).  A
 // probe costs at most W, so consecutive probes never skip a tile: every
 // tile is non-empty.
@@ -774,8 +779,8 @@ __global__ __launch_bounds__(256) void k_probe_cost(FgssArgs A, const uint32_t* 
   rec[i] = make_uint4(A.probe.starts[p], l | (mp << 16), lo, hi);
   slot[i] = base;
   uint32_t v = 0;
-  for (uint32_t j = 0; j < mp && v < 8 * COST_W; ++j) v += A.slice_n[base + j];
-  const uint32_t c = max(4u, max((l + 3) / 4, (min(v, 8 * COST_W) + 7) / 8));
+  for (uint32_t j = 0; j < mp && v < 16 * COST_W; ++j) v += A.slice_n[base + j];
+  const uint32_t c = max(2u, max((l + 7) / 8, (min(v, 16 * COST_W) + 15) / 16));
   cost[i] = min(c, COST_W);
 }
 

@@ -202,7 +202,7 @@ struct FgssArgs {
   uint32_t idx_first, idx_last;      // inclusive
   int self;                          // 1: restrict to s < r
   // tile plan of the probe collection: tile k covers probes
-  // [tile_starts[k], tile_starts[k+1]) (<= 256 probes, <= 4096 tokens);
+  // [tile_starts[k], tile_starts[k+1]) (<= 512 probes, <= 8192 tokens);
   // NULL selects fixed tiles of probes_per_tile probes
   const uint32_t* tile_starts;
   uint32_t first_tile, n_plan_tiles;
@@ -231,8 +231,8 @@ struct FgssArgs {
   DevOut out;
 };
 constexpr uint32_t MAX_THRESHOLDS = 8;
-constexpr uint32_t TILE_TOKENS = 4096;  // probe tokens staged in LDS per tile
-constexpr uint32_t TILE_PROBES = 256;
+constexpr uint32_t TILE_TOKENS = 8192;  // probe tokens staged in LDS per tile
+constexpr uint32_t TILE_PROBES = 512;
 // persistent_wgs: grid size when a.tile_queue is set (0: one per tile)
 void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile, uint32_t persistent_wgs = 0);
 void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups);

@@ -187,7 +187,8 @@ typedef struct {
   uint64_t batch_candidates;  /* hybrid: candidates per device batch (default 1<<22) */
   uint32_t flags;             /* HS_TUNE_* bits */
   uint32_t tile_tokens;       /* batched fgss: probe tokens per tile, <= 4096 (default 4096) */
-  uint32_t workgroups_per_cu; /* fgss, batched fgss: persistent workgroups per CU (default 4) */
+  uint32_t workgroups_per_cu; /* fgss, batched fgss: persistent workgroups per CU (default 2, the
+                                 number of 512-lane fgss workgroups a CU holds) */
   uint32_t signature_bits;    /* fgss: width of the parity-signature test that runs between the
                                  positional filter and verification, 64, 128, 256, 512 or 1024
                                  (default: the smallest power of two >= 2 x the mean indexed set
@@ -198,7 +199,7 @@ typedef struct {
  * replay it while the collection, the bounds and the tile plan are unchanged. */
 enum { HS_TUNE_GRAPH = 1 };
 /* hs_join fgss tiles.  HS_TUNE_STATIC_TILES: one workgroup per tile of the
- * collection's upload-time plan (at most 256 probes and 4096 tokens).
+ * collection's upload-time plan (at most 512 probes and 8192 tokens).
  * HS_TUNE_TILE_QUEUE: the same tiles claimed by a persistent grid
  * (workgroups_per_cu per CU) from per-XCD queues.  HS_TUNE_BALANCED_TILES:
  * tiles of equal estimated work, planned on the device after the index

@@ -102,7 +102,7 @@ def test_autotune_batch(ctx, batch):
     colls, b = batch
     ctx.set_threshold(0.75)
     tune, ms = ctx.autotune_batch(b, reps=1)
-    assert tune.probes_per_tile in (64, 128, 256) and tune.workgroups_per_cu in (2, 4, 8) and ms > 0
+    assert tune.probes_per_tile in (64, 128, 256, 512) and tune.workgroups_per_cu in (1, 2, 4) and ms > 0
     assert tune.signature_bits in (0, 64, 128, 256)
     assert split(b, *ctx.join_batch(b, tune=tune)) == expected(colls, 0.75)
     # the tuned configuration as a graph, after the trials moved the buffers
