@@ -611,6 +611,7 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     // and on 200K x 200K sets of 100-700 tokens (length ~600); small joins keep
     // the upload-time tiles.
     bool balanced = false;
+    const double mean_list = (double)P.index_entries / std::max<uint32_t>(indexed->universe, 1);
     if (ppt == 0 && (uint64_t)(pr.last - pr.first + 1) * 1024 < 0xFFFFFFFFull && probe->maxlen < 0xFFFFu) {
       if (tune.flags & HS_TUNE_BALANCED_TILES) {
         balanced = true;
@@ -621,10 +622,15 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
           const uint32_t lo = std::max(pr.first, psf[l]), hi = std::min(pr.last + 1, psf[l + 1]);
           if (hi > lo) items += (uint64_t)(hi - lo) * ctx->tables.maxprefix[std::min(l, ctx->tables.L)];
         }
-        const double mean_list = (double)P.index_entries / std::max<uint32_t>(indexed->universe, 1);
         balanced = items >= (1u << 21) && mean_list >= 3.5;
       }
     }
+    // Very long posting lists (tens of entries per token and more) put one
+    // heavy probe in each balanced tile; such tiles run in 256-lane
+    // workgroups, where a round of visits waits on 4 waves' random
+    // signature gathers instead of 8 (measured on 200K x 200K sets of
+    // 100-700 tokens, ~580 entries per token).
+    a.narrow_tiles = balanced && mean_list >= 32.0;
     if (balanced) {
       // item slices first, then tiles of equal work claimed by a persistent grid
       // per size class of the probe range: first item slot, first probe,
@@ -671,7 +677,8 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
       // a persistent grid claims the tiles from per-XCD queues
       ctx->batch_queue.ensure(8 * sizeof(uint32_t));
       a.tile_queue = ctx->batch_queue.as<uint32_t>();
-      wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : 2);
+      // as many workgroups per CU as fit: two of 512 lanes, four of 256
+      wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : a.narrow_tiles ? 4 : 2);
     }
     HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));  // kernel_ms: the fgss launch alone
     hs::launch_fgss(st, a, ppt, wgs);

@@ -53,15 +53,22 @@
 
 namespace hs {
 
-constexpr int FG_TPB = 512;                   // threads per workgroup, also max probes per tile
-constexpr uint32_t FG_TOKCAP = TILE_TOKENS;    // staged probe tokens (32 KiB)
-static_assert(TILE_PROBES == FG_TPB, "one probe per thread in phase A");
-constexpr uint32_t FG_ITEMCAP = 1536; // items per batch
+constexpr int FG_TPB = 512;         // lanes per workgroup of the usual launch, also max probes per tile
+constexpr int FG_TPB_NARROW = 256;  // lanes of the narrow launch (balanced tiles over very long lists)
+static_assert(TILE_PROBES == FG_TPB && TILE_TOKENS == 16u * FG_TPB, "one probe per lane in phase A");
+
+// Geometry of a launch of TPB lanes: one probe and 16 staged tokens per
+// lane, 3 items per lane and batch, a candidate stack of 2 TPB (fewer than
+// TPB entries wait between rounds and a round of TPB visits adds at most
+// TPB).
+template <int TPB>
+struct FgGeom {
+  static constexpr uint32_t TOKCAP = 16u * TPB;
+  static constexpr uint32_t ITEMCAP = 3u * TPB;
+  static constexpr uint32_t QCAP = 2u * TPB;
+};
 constexpr uint32_t NO_LDS = 0xFFFFFFFFu;
 constexpr uint32_t FG_MOCAP = 1024;   // minoverlap entries kept in LDS (|r| + |s| < 1024)
-// Candidate stack: fewer than FG_TPB entries wait between rounds and a round
-// of FG_TPB visits adds at most FG_TPB.
-constexpr uint32_t FG_QCAP = 2 * FG_TPB;
 
 constexpr uint32_t SREG = 12;  // tokens of s held in registers per refill
 
@@ -105,18 +112,19 @@ __device__ __forceinline__ bool verify_visit(const uint32_t* __restrict__ r, uin
 }
 
 // Workgroup-shared state of one tile.
+template <int TPB>
 struct FgssLds {
-  uint32_t tok[FG_TOKCAP];
-  uint32_t pstart[FG_TPB];
-  uint32_t psize[FG_TPB];
-  uint32_t plds[FG_TPB];
-  uint32_t lo[FG_TPB];
-  uint32_t hi[FG_TPB];
-  uint32_t item_off[FG_TPB + 1];
-  uint32_t it_a[FG_ITEMCAP];
-  uint32_t it_pj[FG_ITEMCAP];
-  uint32_t vis_off[FG_ITEMCAP + 1];
-  uint32_t tmp[FG_TPB / WAVE + 1];
+  uint32_t tok[FgGeom<TPB>::TOKCAP];
+  uint32_t pstart[TPB];
+  uint32_t psize[TPB];
+  uint32_t plds[TPB];
+  uint32_t lo[TPB];
+  uint32_t hi[TPB];
+  uint32_t item_off[TPB + 1];
+  uint32_t it_a[FgGeom<TPB>::ITEMCAP];
+  uint32_t it_pj[FgGeom<TPB>::ITEMCAP];
+  uint32_t vis_off[FgGeom<TPB>::ITEMCAP + 1];
+  uint32_t tmp[TPB / WAVE + 1];
   unsigned long long stats[5];
   uint32_t next;
   uint32_t nxt[2];                  // claimed tiles, current and next (persistent single join)
@@ -125,11 +133,11 @@ struct FgssLds {
   uint16_t mo[FG_MOCAP];
   // candidate stack: probe slot | position in r << 9, indexed set id; the
   // entries of round k are counted in qadd[k % 3] (cleared two rounds ahead)
-  uint32_t q_pj[FG_QCAP];
-  uint32_t q_sid[FG_QCAP];
+  uint32_t q_pj[FgGeom<TPB>::QCAP];
+  uint32_t q_sid[FgGeom<TPB>::QCAP];
   uint32_t qadd[3];
-  uint64_t psig[FG_TPB];            // probe signatures (64-bit signature tests)
-  uint8_t ptidx[FG_TPB];            // per-probe threshold table (serving)
+  uint64_t psig[TPB];               // probe signatures (64-bit signature tests)
+  uint8_t ptidx[TPB];               // per-probe threshold table (serving)
   DevBounds bounds[MAX_THRESHOLDS];  // the tables of a multi-threshold launch
 };
 
@@ -264,8 +272,8 @@ struct FgssCounts {
   uint32_t cands = 0, cand_tokens = 0, sig_tests = 0;
 };
 
-template <bool kMulti>
-__device__ __forceinline__ uint32_t tile_minoverlap(const FgssArgs& A, const FgssLds& L, uint32_t pl, uint32_t lr,
+template <int TPB, bool kMulti>
+__device__ __forceinline__ uint32_t tile_minoverlap(const FgssArgs& A, const FgssLds<TPB>& L, uint32_t pl, uint32_t lr,
                                                     uint32_t ls) {
   if (!kMulti && L.mo_lds) return min((uint32_t)L.mo[lr + ls], min(lr, ls));
   return dev_minoverlap(kMulti ? L.bounds[L.ptidx[pl]] : A.bounds, lr, ls);
@@ -274,8 +282,8 @@ __device__ __forceinline__ uint32_t tile_minoverlap(const FgssArgs& A, const Fgs
 // Verifies one candidate of the stack (valid lanes) and appends the passing
 // pairs; called by every lane of the wave.  pj = probe slot | j << 9, where
 // r[j] is the token through which the candidate was reached.
-template <bool kMulti>
-__device__ __forceinline__ void verify_candidate(const FgssArgs& A, const FgssLds& L, uint32_t p0, bool valid,
+template <int TPB, bool kMulti>
+__device__ __forceinline__ void verify_candidate(const FgssArgs& A, const FgssLds<TPB>& L, uint32_t p0, bool valid,
                                                  uint32_t pj, uint32_t sid, FgssCounts& C) {
   bool pass = false;
   uint32_t ov = 0, rid = 0;
@@ -286,7 +294,7 @@ __device__ __forceinline__ void verify_candidate(const FgssArgs& A, const FgssLd
     const uint32_t sst = A.indexed.starts[sid];
     const uint32_t ls = A.indexed.sizes[sid];
     const uint32_t lr = L.psize[pl];
-    const uint32_t mo = tile_minoverlap<kMulti>(A, L, pl, lr, ls);
+    const uint32_t mo = tile_minoverlap<TPB, kMulti>(A, L, pl, lr, ls);
     const uint32_t* sp = A.indexed.tokens + sst;
     const uint32_t off = L.plds[pl];
     if (off != NO_LDS) pass = verify_visit<true>(L.tok + off, lr, sp, ls, j, mo, &ov);
@@ -300,8 +308,8 @@ __device__ __forceinline__ void verify_candidate(const FgssArgs& A, const FgssLd
 // kW: signature words (0: no signature test); kSlices: item slices come
 // from the item-slice pass; kMulti: per-probe threshold tables (serving).
 // rnd counts the workgroup's visit rounds (it selects the stack counter).
-template <int kW, bool kSlices, bool kMulti>
-__device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T, FgssLds& L, FgssCounts& C,
+template <int TPB, int kW, bool kSlices, bool kMulti>
+__device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T, FgssLds<TPB>& L, FgssCounts& C,
                                           uint32_t& rnd) {
   const uint32_t tid = threadIdx.x;
   const uint32_t p0 = T.p0;
@@ -339,42 +347,42 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     L.hi[tid] = hi;
     if constexpr (kW == 1) L.psig[tid] = A.probe_sigs[p];
     const uint32_t off = st - base;
-    L.plds[tid] = (off + l <= FG_TOKCAP) ? off : NO_LDS;
+    L.plds[tid] = (off + l <= FgGeom<TPB>::TOKCAP) ? off : NO_LDS;
     n_active = (lo < hi) ? mp : 0;
     // the slice pass lays out maxprefix(l) items for every probe, in probe order
     n_items = kSlices ? mp : n_active;
   }
   {
     // every load of the staging is issued before the first LDS store
-    constexpr uint32_t R = FG_TOKCAP / FG_TPB;
-    const uint32_t m = min(ntok, FG_TOKCAP);
+    constexpr uint32_t R = FgGeom<TPB>::TOKCAP / TPB;
+    const uint32_t m = min(ntok, FgGeom<TPB>::TOKCAP);
     uint32_t tv[R];
 #pragma unroll
-    for (uint32_t k = 0; k < R; ++k) tv[k] = tid + k * FG_TPB < m ? A.probe.tokens[base + tid + k * FG_TPB] : 0u;
+    for (uint32_t k = 0; k < R; ++k) tv[k] = tid + k * TPB < m ? A.probe.tokens[base + tid + k * TPB] : 0u;
 #pragma unroll
     for (uint32_t k = 0; k < R; ++k)
-      if (tid + k * FG_TPB < m) L.tok[tid + k * FG_TPB] = tv[k];
+      if (tid + k * TPB < m) L.tok[tid + k * TPB] = tv[k];
   }
   n_active = wave_sum(n_active);
   if (lane_id() == 0 && n_active) atomicAdd(&L.stats[3], (unsigned long long)n_active);
   uint32_t total_items;
-  uint32_t ioff = block_exclusive_scan<FG_TPB>(n_items, L.tmp, &total_items);
+  uint32_t ioff = block_exclusive_scan<TPB>(n_items, L.tmp, &total_items);
   if (tid < np) L.item_off[tid] = ioff;
   if (tid == 0) L.item_off[np] = total_items;
   __syncthreads();
 
   // ---- phase B: batches of items
   uint32_t qn = 0;  // entries on the candidate stack (the same in every lane)
-  for (uint32_t ib = 0; ib < total_items; ib += FG_ITEMCAP) {
-    const uint32_t nb = min(FG_ITEMCAP, total_items - ib);
+  for (uint32_t ib = 0; ib < total_items; ib += FgGeom<TPB>::ITEMCAP) {
+    const uint32_t nb = min(FgGeom<TPB>::ITEMCAP, total_items - ib);
     // slice of the posting list for every item of the batch: read from the
     // item-slice pass when there is one, otherwise searched here, PER items
     // per thread in lockstep
-    constexpr uint32_t PER = FG_ITEMCAP / FG_TPB;
+    constexpr uint32_t PER = FgGeom<TPB>::ITEMCAP / TPB;
     if constexpr (kSlices) {
 #pragma unroll
       for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t it = tid + k * FG_TPB;
+        const uint32_t it = tid + k * TPB;
         uint32_t len = 0;
         if (it < nb) {
           const uint32_t item = ib + it;
@@ -390,7 +398,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       uint32_t f_lo[PER], n_lo[PER], f_hi[PER], n_hi[PER], k_lo[PER], k_hi[PER];
 #pragma unroll
       for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t it = tid + k * FG_TPB;
+        const uint32_t it = tid + k * TPB;
         f_lo[k] = f_hi[k] = n_lo[k] = n_hi[k] = k_lo[k] = k_hi[k] = 0;
         if (it < nb) {
           const uint32_t item = ib + it;
@@ -414,7 +422,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       lockstep_bounds<PER>(A.index.set_ids, 1u, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
 #pragma unroll
       for (uint32_t k = 0; k < PER; ++k) {
-        const uint32_t it = tid + k * FG_TPB;
+        const uint32_t it = tid + k * TPB;
         L.it_a[it] = f_lo[k];
         L.vis_off[it] = (it < nb) ? n_hi[k] : 0u;
       }
@@ -429,14 +437,14 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       local += v[k];
     }
     uint32_t total_vis;
-    uint32_t run = block_exclusive_scan<FG_TPB>(local, L.tmp, &total_vis);
+    uint32_t run = block_exclusive_scan<TPB>(local, L.tmp, &total_vis);
 #pragma unroll
     for (uint32_t k = 0; k < PER; ++k) {
       L.vis_off[tid * PER + k] = run;
       run += v[k];
     }
     if (tid == 0) {
-      L.vis_off[FG_ITEMCAP] = total_vis;
+      L.vis_off[FgGeom<TPB>::ITEMCAP] = total_vis;
       L.stats[0] += total_vis;
     }
     __syncthreads();
@@ -444,13 +452,13 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     // one visit per lane; the index entry of the lane's next visit is
     // fetched before this one is filtered, so its latency overlaps the
     // filter's loads.  Visits passing both filters go onto the LDS stack;
-    // whenever it holds FG_TPB they are verified, one per lane.  The last batch
+    // whenever it holds TPB they are verified, one per lane.  The last batch
     // of the tile ends with one round without visits that verifies the rest
     // of the stack (one verification site keeps the register budget).
-    const bool last_batch = ib + FG_ITEMCAP >= total_items;
-    const uint32_t vis_end = total_vis + (last_batch ? FG_TPB : 0u);
+    const bool last_batch = ib + FgGeom<TPB>::ITEMCAP >= total_items;
+    const uint32_t vis_end = total_vis + (last_batch ? TPB : 0u);
     // (64-bit signatures ride in the index entry and arrive with it).  A
-    // lane's next visit lies FG_TPB further on: with long slices (long sets) it
+    // lane's next visit lies TPB further on: with long slices (long sets) it
     // is in the same item or a near one, found by galloping from the current
     // item; with short slices a bisection over the batch's items is cheaper.
     const bool gallop = total_vis >= 64u * nb;
@@ -460,11 +468,11 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       n_it = segment_of(L.vis_off, nb, tid);
       load_entry<kW>(A.index, L.it_a[n_it] + (tid - L.vis_off[n_it]), n_sid, n_meta, n_sig);
     }
-    for (uint32_t vb = 0; vb < vis_end; vb += FG_TPB, ++rnd) {
+    for (uint32_t vb = 0; vb < vis_end; vb += TPB, ++rnd) {
       const uint32_t vi = vb + tid;
       const uint32_t it = n_it, sid = n_sid, meta = n_meta;
       const uint64_t ssig = n_sig;
-      const uint32_t vn = vi + FG_TPB;
+      const uint32_t vn = vi + TPB;
       if (vn < total_vis) {
         n_it = gallop ? segment_from(L.vis_off, nb, vn, n_it) : segment_of(L.vis_off, nb, vn);
         load_entry<kW>(A.index, L.it_a[n_it] + (vn - L.vis_off[n_it]), n_sid, n_meta, n_sig);
@@ -479,7 +487,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
         const uint32_t j = pj >> 9;
         const uint32_t lr = L.psize[pl];
         const uint32_t ls = (meta >> 16) == 0xFFFFu ? A.indexed.sizes[sid] : (meta >> 16);
-        const uint32_t mo = tile_minoverlap<kMulti>(A, L, pl, lr, ls);
+        const uint32_t mo = tile_minoverlap<TPB, kMulti>(A, L, pl, lr, ls);
         // Positional filter (helpers.hpp:46-50) at this visit's token: if it is
         // the first common token, |r n s| <= 1 + min(|r|-1-j, |s|-1-kp).  Only
         // the first-common visit can emit the pair, so a visit failing the
@@ -522,15 +530,15 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       }
       __syncthreads();
       qn += L.qadd[rnd % 3];
-      // FG_TPB entries, or in the final round (no visits, fewer left) the rest
-      const uint32_t take = qn >= FG_TPB ? FG_TPB : (vb + FG_TPB >= vis_end && last_batch ? qn : 0u);
+      // TPB entries, or in the final round (no visits, fewer left) the rest
+      const uint32_t take = qn >= TPB ? TPB : (vb + TPB >= vis_end && last_batch ? qn : 0u);
       if (take) {
         const bool valid = tid < take;
         const uint32_t e = qn - take + tid;
         const uint32_t qpj = valid ? L.q_pj[e] : 0u, qsid = valid ? L.q_sid[e] : 0u;
         __syncthreads();  // every lane holds its entry before the next round reuses the slots
         qn -= take;
-        verify_candidate<kMulti>(A, L, p0, valid, qpj, qsid, C);
+        verify_candidate<TPB, kMulti>(A, L, p0, valid, qpj, qsid, C);
       }
     }
     __syncthreads();
@@ -587,10 +595,10 @@ __device__ __forceinline__ void plain_tile_range(const FgssArgs& A, uint32_t til
 // one join, a persistent grid over the tiles of a batch of joins.
 enum FgssMode { FG_TILES = 0, FG_QUEUE = 1, FG_BATCH = 2 };
 
-template <int kMode, bool kSlices, bool kMulti, int kW>
-__global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_fgss(FgssArgs A, uint32_t ppt,
-                                                                                        uint32_t ntiles) {
-  __shared__ FgssLds L;
+template <int TPB, int kMode, bool kSlices, bool kMulti, int kW>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k_fgss(FgssArgs A, uint32_t ppt,
+                                                                                     uint32_t ntiles) {
+  __shared__ FgssLds<TPB> L;
   const uint32_t tid = threadIdx.x;
   if constexpr (kMulti) {
     if (tid < MAX_THRESHOLDS) L.bounds[tid] = A.probe_bounds[tid];
@@ -605,7 +613,7 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
     const uint32_t nmo = 2 * A.bounds.L + 1;
     const bool fits = A.bounds.kind != HS_COSINE && nmo <= FG_MOCAP;
     if (fits)
-      for (uint32_t i = tid; i < nmo; i += FG_TPB) L.mo[i] = (uint16_t)A.bounds.mo[i];
+      for (uint32_t i = tid; i < nmo; i += TPB) L.mo[i] = (uint16_t)A.bounds.mo[i];
     if (tid == 0) L.mo_lds = fits;
   }
   __syncthreads();
@@ -620,7 +628,7 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
       const uint32_t j = A.batch.tile_join[k];
       TileScope T{A.batch.tile_lo[k], A.batch.tile_hi[k], A.batch.join_first[j], A.batch.join_first[j + 1] - 1,
                   A.size_first + A.batch.sf_base[j], A.batch.sf_base[j + 1] - A.batch.sf_base[j]};
-      fgss_tile<kW, kSlices, kMulti>(A, T, L, C, rnd);
+      fgss_tile<TPB, kW, kSlices, kMulti>(A, T, L, C, rnd);
     }
   } else if constexpr (kMode == FG_QUEUE) {
     // persistent over the tiles of one join (tile count on the device when
@@ -638,7 +646,7 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
       uint32_t p0, p1;
       plain_tile_range(A, tile, ppt, p0, p1);
       TileScope T{p0, p1, A.idx_first, A.idx_last, A.size_first, A.size_first_len};
-      fgss_tile<kW, kSlices, kMulti>(A, T, L, C, rnd);
+      fgss_tile<TPB, kW, kSlices, kMulti>(A, T, L, C, rnd);
       if (tid == 0) L.nxt[(it + 1) & 1] = claimed;
       __syncthreads();
     }
@@ -648,7 +656,7 @@ __global__ __launch_bounds__(FG_TPB) __attribute__((amdgpu_waves_per_eu(4))) voi
     uint32_t p0, p1;
     plain_tile_range(A, tile, ppt, p0, p1);
     TileScope T{p0, p1, A.idx_first, A.idx_last, A.size_first, A.size_first_len};
-    fgss_tile<kW, kSlices, kMulti>(A, T, L, C, rnd);
+    fgss_tile<TPB, kW, kSlices, kMulti>(A, T, L, C, rnd);
   }
 
   // ---- statistics: one global atomic per workgroup and counter
@@ -747,11 +755,12 @@ __global__ __launch_bounds__(256) void k_item_slices(FgssArgs A, const uint32_t*
   }
 }
 
-// Per probe a cost in units of COST_W-th of a tile: the largest of 2 (so a
-// tile holds at most 512 probes), an eighth per token (at most 8192 tokens
-// besides the last probe's) and a sixteenth per posting it visits (about
-// 16K visits per tile), capped at one tile.  Taking the largest term keeps
-// all three limits while a tile of light probes still takes 512 of them.  Tile
+// Per probe a cost in units of COST_W-th of a tile of `lanes` lanes: the
+// largest of COST_W / lanes (so a tile holds at most `lanes` probes),
+// COST_W / (16 lanes) per token (at most 16 lanes tokens besides the last
+// probe's) and COST_W / (32 lanes) per posting it visits (about 32 lanes
+// visits per tile), capped at one tile.  Taking the largest term keeps all
+// three limits while a tile of light probes still takes `lanes` of them.  Tile
 // k takes the probes whose exclusive cost prefix lies in [k W, (k+1) W).  A
 // probe costs at most W, so consecutive probes never skip a tile: every
 // tile is non-empty.
@@ -759,7 +768,7 @@ constexpr uint32_t COST_W = 1024;
 
 __global__ __launch_bounds__(256) void k_probe_cost(FgssArgs A, const uint32_t* __restrict__ gtab, uint32_t nl,
                                                     uint32_t* __restrict__ cost, uint4* __restrict__ rec,
-                                                    uint32_t* __restrict__ slot) {
+                                                    uint32_t* __restrict__ slot, uint32_t lanes) {
   __shared__ uint32_t stab[5 * SZ_CAP];
   const uint32_t* tab = gtab;
   if (nl <= SZ_CAP) {
@@ -778,9 +787,12 @@ __global__ __launch_bounds__(256) void k_probe_cost(FgssArgs A, const uint32_t* 
   const uint32_t hi = A.self ? min(T.hi[l], p) : T.hi[l];
   rec[i] = make_uint4(A.probe.starts[p], l | (mp << 16), lo, hi);
   slot[i] = base;
+  const uint32_t vcap = 32u * lanes;  // visits of a full tile
   uint32_t v = 0;
-  for (uint32_t j = 0; j < mp && v < 16 * COST_W; ++j) v += A.slice_n[base + j];
-  const uint32_t c = max(2u, max((l + 7) / 8, (min(v, 16 * COST_W) + 15) / 16));
+  for (uint32_t j = 0; j < mp && v < vcap; ++j) v += A.slice_n[base + j];
+  const uint32_t c_tok = (uint32_t)(((uint64_t)l * COST_W + 16u * lanes - 1) / (16u * lanes));
+  const uint32_t c_vis = (uint32_t)(((uint64_t)min(v, vcap) * COST_W + vcap - 1) / vcap);
+  const uint32_t c = max(COST_W / lanes, max(c_tok, c_vis));
   cost[i] = min(c, COST_W);
 }
 
@@ -815,7 +827,8 @@ void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp) {
                        bp.slice_n->as<uint32_t>());
   uint32_t* c = bp.cost->as<uint32_t>();
   hipLaunchKernelGGL(k_probe_cost, dim3((n + 255) / 256), dim3(256), 0, st, a, bp.size_tab, bp.nl, c,
-                     bp.rec->as<uint4>(), bp.slot->as<uint32_t>());
+                     bp.rec->as<uint4>(), bp.slot->as<uint32_t>(),
+                     (uint32_t)(a.narrow_tiles ? FG_TPB_NARROW : FG_TPB));
   a.probe_rec = bp.rec->as<uint4>();
   a.probe_slot = bp.slot->as<uint32_t>();
   launch_exclusive_scan(st, c, c, nullptr, n, *bp.scan_scratch);
@@ -829,16 +842,16 @@ void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp) {
 }
 
 // One k_fgss instance per signature width.
-template <int kMode, bool kSlices, bool kMulti>
+template <int TPB, int kMode, bool kSlices, bool kMulti>
 static void launch_width(hipStream_t st, const FgssArgs& a, dim3 grid, uint32_t ppt, uint32_t ntiles) {
-  const dim3 block(FG_TPB);
+  const dim3 block(TPB);
   switch (a.sig_words) {
-    case 0: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 0>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 1: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 1>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 2: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 2>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 4: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 4>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 8: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 8>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 16: hipLaunchKernelGGL((k_fgss<kMode, kSlices, kMulti, 16>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 0: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 0>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 1: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 1>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 2: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 2>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 4: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 4>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 8: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 8>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 16: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 16>), grid, block, 0, st, a, ppt, ntiles); break;
     default: throw Error{HS_E_INVALID, "signature width must be 0, 64, 128, 256, 512 or 1024 bits"};
   }
   HS_CHECK_HIP(hipGetLastError());
@@ -863,12 +876,13 @@ void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persi
     if (persistent_wgs) grid = std::min(grid, (persistent_wgs + 7) / 8 * 8);
   }
   if (a.tile_queue) {
-    if (a.slice_a) launch_width<FG_QUEUE, true, false>(st, a, dim3(grid), ppt, ntiles);
-    else launch_width<FG_QUEUE, false, false>(st, a, dim3(grid), ppt, ntiles);
+    if (a.slice_a && a.narrow_tiles) launch_width<FG_TPB_NARROW, FG_QUEUE, true, false>(st, a, dim3(grid), ppt, ntiles);
+    else if (a.slice_a) launch_width<FG_TPB, FG_QUEUE, true, false>(st, a, dim3(grid), ppt, ntiles);
+    else launch_width<FG_TPB, FG_QUEUE, false, false>(st, a, dim3(grid), ppt, ntiles);
   } else if (a.probe_tidx) {
-    launch_width<FG_TILES, false, true>(st, a, dim3(grid), ppt, ntiles);
+    launch_width<FG_TPB, FG_TILES, false, true>(st, a, dim3(grid), ppt, ntiles);
   } else {
-    launch_width<FG_TILES, false, false>(st, a, dim3(grid), ppt, ntiles);
+    launch_width<FG_TPB, FG_TILES, false, false>(st, a, dim3(grid), ppt, ntiles);
   }
 }
 
@@ -876,7 +890,7 @@ void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint3
   if (ntiles == 0) return;
   check_sigs(a);
   launch_fill_u32(st, a.batch.queue, 8, 0);
-  launch_width<FG_BATCH, false, false>(st, a, dim3(workgroups), 0u, ntiles);
+  launch_width<FG_TPB, FG_BATCH, false, false>(st, a, dim3(workgroups), 0u, ntiles);
 }
 
 uint32_t auto_signature_words(double mean) {

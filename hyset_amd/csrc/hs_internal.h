@@ -217,6 +217,7 @@ struct FgssArgs {
   const uint32_t* dev_ntiles;  // non-NULL: the tile count lives on the device
   // item-slice pass (plain joins): per item slot the posting-list slice of
   // ids in the probe's window
+  uint32_t narrow_tiles;  // balanced plan: tiles of 256 lanes and probes (very long posting lists)
   const uint32_t* slice_a;
   const uint32_t* slice_n;
   // per probe of the range (planning pass): {start, size | maxprefix << 16,

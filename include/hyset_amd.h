@@ -187,8 +187,8 @@ typedef struct {
   uint64_t batch_candidates;  /* hybrid: candidates per device batch (default 1<<22) */
   uint32_t flags;             /* HS_TUNE_* bits */
   uint32_t tile_tokens;       /* batched fgss: probe tokens per tile, <= 4096 (default 4096) */
-  uint32_t workgroups_per_cu; /* fgss, batched fgss: persistent workgroups per CU (default 2, the
-                                 number of 512-lane fgss workgroups a CU holds) */
+  uint32_t workgroups_per_cu; /* fgss, batched fgss: persistent workgroups per CU (default: as
+                                 many as a CU holds, 2 of 512 lanes or 4 of 256) */
   uint32_t signature_bits;    /* fgss: width of the parity-signature test that runs between the
                                  positional filter and verification, 64, 128, 256, 512 or 1024
                                  (default: the smallest power of two >= 2 x the mean indexed set

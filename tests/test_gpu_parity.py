@@ -243,6 +243,9 @@ def test_rs_long_sets_cosine(ctx):
         want = O.rs_join(R, S, 0.8, "cosine")
         assert want.count >= 3000
         _check(ctx.join(dS, dR, algo="fgss"), want, "fgss long cosine")
+        # ~58 index entries per token: balanced tiles in 256-lane workgroups
+        assert ctx.stats()["tile_plan"] == H.PLAN_BALANCED
+        _check(ctx.join(dS, dR, algo="fgss", tune=H.Tune(flags=H.TUNE_TILE_QUEUE)), want, "fgss long cosine queue")
     finally:
         dR.free()
         dS.free()
