@@ -382,6 +382,22 @@ def main() -> int:
     log(f"fgss t={t}: {pairs} pairs, {ms_per_step:.3f} ms/step wall, device {dev_ms:.3f} ms "
         f"(index {idx_ms:.3f}, k_fgss {kern_ms:.3f})")
 
+    # the same join replayed from a HIP graph (HS_TUNE_GRAPH): every kernel
+    # runs each step; the host enqueues one graph instead of the sequence
+    gtune = H.Tune(flags=H.TUNE_GRAPH)
+    for _ in range(max(args.warmup, 2)):
+        assert ctx.join(d, algo="fgss", tune=gtune, count_only=True) == pairs
+    gsteps = []
+    a = time.perf_counter()
+    for _ in range(args.steps):
+        assert ctx.join(d, algo="fgss", tune=gtune, count_only=True) == pairs
+        gsteps.append(ctx.stats())
+    ms_per_step_graph = (time.perf_counter() - a) * 1e3 / args.steps
+    replayed = all(g["graph"] == 2 for g in gsteps)
+    dev_ms_graph = statistics.median(g["total_ms"] for g in gsteps)
+    log(f"fgss t={t} graph replay: {ms_per_step_graph:.3f} ms/step wall, device {dev_ms_graph:.3f} ms "
+        f"(replayed every step: {replayed})")
+
     # with the explicit device-to-host copy of the pairs
     d2h = []
     for _ in range(5):
@@ -397,7 +413,9 @@ def main() -> int:
     if tfile.exists():
         traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
 
-    extra = {"join_ms_device": round(dev_ms, 4), "index_ms": round(idx_ms, 4), "k_fgss_ms": round(kern_ms, 4),
+    extra = {"ms_per_step_graph": round(ms_per_step_graph, 4), "join_ms_device_graph": round(dev_ms_graph, 4),
+             "graph_replayed_every_step": replayed,
+             "join_ms_device": round(dev_ms, 4), "index_ms": round(idx_ms, 4), "k_fgss_ms": round(kern_ms, 4),
              "join_ms_with_d2h": round(ms_with_d2h, 4), "pairs": int(pairs),
              "candidates": int(st["candidates"]), "posting_visits": int(st["posting_visits"]),
              "index_entries": int(st["index_entries"]), "probe_prefix_tokens": int(st["probe_prefix_tokens"]),

@@ -75,7 +75,7 @@ class Stats(C.Structure):
                 ("index_entries", C.c_uint64), ("posting_visits", C.c_uint64),
                 ("candidates", C.c_uint64), ("pairs", C.c_uint64), ("probe_prefix_tokens", C.c_uint64),
                 ("candidate_tokens", C.c_uint64), ("signature_tests", C.c_uint64), ("signature_bits", C.c_uint32),
-                ("tile_plan", C.c_uint32)]
+                ("tile_plan", C.c_uint32), ("graph", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):

@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -388,6 +389,8 @@ void hs_destroy(hs_ctx* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   if (ctx->graph_exec) (void)hipGraphExecDestroy(ctx->graph_exec);
   if (ctx->graph) (void)hipGraphDestroy(ctx->graph);
+  if (ctx->jgraph_exec) (void)hipGraphExecDestroy(ctx->jgraph_exec);
+  if (ctx->jgraph) (void)hipGraphDestroy(ctx->jgraph);
   for (auto& sl : ctx->slots)
     if (sl.done) (void)hipEventDestroy(sl.done);
   for (auto& e : ctx->stage_ev)
@@ -541,9 +544,10 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
     const auto& ptab = self ? ctx->tables.midprefix : ctx->tables.maxprefix;
     const auto& sf = indexed->h_size_first;
     const uint32_t nl = (uint32_t)sf.size();  // maxlen + 2
-    ctx->h_cum.ensure((size_t)nl * 4);
+    PinnedBuf& hcum = ctx->capturing_join ? ctx->jgraph_cum : ctx->h_cum;
+    hcum.ensure((size_t)nl * 4);
     ctx->d_cum.ensure((size_t)nl * 4);
-    uint32_t* cum = ctx->h_cum.as<uint32_t>();
+    uint32_t* cum = hcum.as<uint32_t>();
     uint64_t acc = 0;
     for (uint32_t l = 0; l + 1 < nl; ++l) {
       cum[l] = (uint32_t)acc;
@@ -573,7 +577,7 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
   const hs_range ir = P.ir;
   uint32_t tile_plan = 0;
   if (range_empty(pr) || range_empty(ir)) {
-    HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
+    if (!ctx->capturing_join) HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
   } else if (algo == HS_ALGO_FGSS) {
     hs::FgssArgs a{};
     a.probe = probe->dev();
@@ -680,7 +684,7 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
       // as many workgroups per CU as fit: two of 512 lanes, four of 256
       wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : a.narrow_tiles ? 4 : 2);
     }
-    HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));  // kernel_ms: the fgss launch alone
+    if (!ctx->capturing_join) HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));  // kernel_ms: the fgss launch alone
     hs::launch_fgss(st, a, ppt, wgs);
   } else if (algo == HS_ALGO_BITMAP) {
     const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
@@ -706,7 +710,7 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     b.idx_last = ir.last;
     b.self = self;
     b.out = dev_out(ctx);
-    HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
+    if (!ctx->capturing_join) HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
     hs::launch_bitmap(st, b);
   } else {
     HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
@@ -714,6 +718,80 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
   }
   return tile_plan;
 }
+
+namespace {
+
+// Identity of a join for graph replay: collections, ranges, technique,
+// knobs, bound tables and result capacity.
+std::string join_graph_key(const hs_ctx* ctx, const hs_coll* indexed, const hs_coll* probe, hs_range ir,
+                           hs_range pr, int32_t algo, const hs_tune& t) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "%p/%p/%u-%u/%u-%u/%d/%u/%u/%u/%u/%u/%u/%d/%d/%.17g/%u/%llu", (const void*)indexed,
+                (const void*)probe, ir.first, ir.last, pr.first, pr.last, algo, t.bitmap_bits, t.probes_per_tile,
+                t.tile_tokens, t.workgroups_per_cu, t.signature_bits, t.flags, ctx->kind, ctx->mode, ctx->t,
+                ctx->tables.L, (unsigned long long)ctx->out_cap);
+  return buf;
+}
+
+void drop_join_graph(hs_ctx* ctx) {
+  if (ctx->jgraph_exec) (void)hipGraphExecDestroy(ctx->jgraph_exec);
+  if (ctx->jgraph) (void)hipGraphDestroy(ctx->jgraph);
+  ctx->jgraph_exec = nullptr;
+  ctx->jgraph = nullptr;
+  ctx->jgraph_key.clear();
+}
+
+// Records the join's device sequence (events, signatures, index, counters,
+// launch) into ctx->jgraph.  The bound-entry table comes from a pinned
+// buffer of the graph's own: a graph reads host memory when it replays.  A
+// capture that fails leaves no graph and no error: the join's result is
+// already there.
+template <typename Reset>
+bool capture_join_graph(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_range ir, hs_range pr, int32_t algo,
+                        const hs_tune& tune, const std::string& key, Reset&& reset_for_launch) {
+  drop_join_graph(ctx);
+  hipStream_t st = ctx->stream;
+  ctx->jgraph_cum.ensure(indexed->h_size_first.size() * 4);
+  HS_CHECK_HIP(hipStreamSynchronize(st));
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  hs::DevicePlan Q;
+  uint32_t plan = 0;
+  bool ok = true;
+  ctx->capturing_join = true;
+  try {
+    // no event records in the graph: the replay is timed from outside it
+    hs::prepare_device_join(ctx, indexed, probe, ir, pr, algo, tune, false, Q);
+    if (!Q.counters_ready) reset_for_launch(Q);
+    plan = hs::launch_device_join(ctx, indexed, probe, Q, pr, algo, tune);
+  } catch (...) {
+    ok = false;
+  }
+  ctx->capturing_join = false;
+  hipGraph_t g = nullptr;
+  if (hipStreamEndCapture(st, &g) != hipSuccess || !ok) {
+    (void)hipGetLastError();
+    if (g) (void)hipGraphDestroy(g);
+    return false;
+  }
+  hipGraphExec_t ge = nullptr;
+  if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipGraphDestroy(g);
+    return false;
+  }
+  ctx->jgraph = g;
+  ctx->jgraph_exec = ge;
+  ctx->jgraph_key = key;
+  ctx->jgraph_plan = Q;
+  ctx->jgraph_tile_plan = plan;
+  ctx->jgraph_epoch = buffer_epoch();
+  return true;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -733,29 +811,65 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     ensure_bounds(ctx, std::max(indexed->maxlen, probe->maxlen));
     ctx->stats = hs_stats{};
     hipStream_t st = ctx->stream;
-    HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
-
-    hs::DevicePlan P;
-    hs::prepare_device_join(ctx, indexed, probe, ir, pr, algo, tune, false, P);
-    HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
-
-    // ---- candidate generation + verification
     unsigned long long cnt[hs::CNT_WORDS];
     uint32_t tile_plan = 0;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      if (attempt > 0 || !P.counters_ready) {
-        reset_counters(ctx);
-        if (algo == HS_ALGO_FGSS && !range_empty(ir))  // index size, read back with the counters
-          HS_CHECK_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + hs::CNT_ENTRIES,
-                                      ctx->ib.offsets.as<uint32_t>() + indexed->universe, 4,
-                                      hipMemcpyDeviceToDevice, st));
-      }
-      tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
+    hs::DevicePlan P;
+    // counters cleared and the index size placed where the kernels read it
+    auto reset_for_launch = [&](const hs::DevicePlan& Q) {
+      reset_counters(ctx);
+      if (algo == HS_ALGO_FGSS && !range_empty(Q.ir))  // index size, read back with the counters
+        HS_CHECK_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + hs::CNT_ENTRIES,
+                                    ctx->ib.offsets.as<uint32_t>() + indexed->universe, 4, hipMemcpyDeviceToDevice,
+                                    st));
+    };
+    // a launch sequence captured by an earlier call with the same join,
+    // replayed while no buffer it recorded has moved (HS_TUNE_GRAPH)
+    const bool want_graph = (tune.flags & HS_TUNE_GRAPH) && algo != HS_ALGO_HYBRID;
+    const std::string key = want_graph ? join_graph_key(ctx, indexed, probe, ir, pr, algo, tune) : std::string();
+    const bool replay = want_graph && ctx->jgraph_exec && ctx->jgraph_key == key && ctx->jgraph_epoch == buffer_epoch();
+    uint32_t graph_state = 0;
+    if (replay) {
+      graph_state = 2;
+      // the graph is timed as a whole: filter_ms covers index and launch
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
+      HS_CHECK_HIP(hipGraphLaunch(ctx->jgraph_exec, st));
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
       HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
+      P = ctx->jgraph_plan;
+      tile_plan = ctx->jgraph_tile_plan;
       read_counters(ctx, cnt);
-      if (cnt[hs::CNT_PAIRS] <= ctx->out_cap) break;
-      HS_REQUIRE(attempt == 0, HS_E_DEVICE, "result count changed between runs");
-      ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
+      if (cnt[hs::CNT_PAIRS] > ctx->out_cap) {  // more pairs than the buffers hold: grow, run again eagerly
+        drop_join_graph(ctx);
+        ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
+        reset_for_launch(P);
+        HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
+        tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
+        HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
+        read_counters(ctx, cnt);
+        HS_REQUIRE(cnt[hs::CNT_PAIRS] <= ctx->out_cap, HS_E_DEVICE, "result count changed between runs");
+      }
+    } else {
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
+      hs::prepare_device_join(ctx, indexed, probe, ir, pr, algo, tune, false, P);
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
+
+      // ---- candidate generation + verification
+      for (int attempt = 0; attempt < 2; ++attempt) {
+        if (attempt > 0 || !P.counters_ready) reset_for_launch(P);
+        tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
+        HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
+        read_counters(ctx, cnt);
+        if (cnt[hs::CNT_PAIRS] <= ctx->out_cap) break;
+        HS_REQUIRE(attempt == 0, HS_E_DEVICE, "result count changed between runs");
+        ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
+      }
+      // Capture the sequence just run for the next calls.  The eager run
+      // allocated every buffer, so the capture allocates nothing.  Plans that
+      // stage per-launch tables through host buffers (balanced, tile queue)
+      // stay eager.
+      if (want_graph && (algo == HS_ALGO_BITMAP || tile_plan == HS_PLAN_UPLOAD || tile_plan == HS_PLAN_FIXED))
+        if (capture_join_graph(ctx, indexed, probe, ir, pr, algo, tune, key, reset_for_launch)) graph_state = 1;
     }
     const uint64_t count = cnt[hs::CNT_PAIRS];
     set_device_result(ctx, count);
@@ -773,6 +887,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
     ctx->stats.kernel_ms = elapsed(ctx->ev[5], ctx->ev[2]);
     ctx->stats.pairs = count;
+    ctx->stats.graph = graph_state;
     if (algo != HS_ALGO_HYBRID) {
       ctx->stats.posting_visits = cnt[hs::CNT_VISITS];
       ctx->stats.candidates = cnt[hs::CNT_CANDS];

@@ -79,6 +79,18 @@ namespace hs {
 struct Triples {
   std::vector<uint32_t> r, s, ov;
 };
+
+// One device join between its set-up and its launches: hs_join runs one
+// launch over its probe range, the CPU/GPU co-processing join one launch per
+// probe stripe it claims, all against one index.
+struct DevicePlan {
+  bool self = false;
+  hs_range ir{1, 0};          // indexed range of the index
+  uint32_t sig_words = 0;     // fgss signature test width (64-bit words)
+  uint32_t bitmap_words = 0;  // signatures computed for the bitmap technique
+  uint64_t index_entries = 0;
+  bool counters_ready = false;  // the index build zeroed the work counters
+};
 }  // namespace hs
 
 // One double-buffered slot of the hybrid candidate pipeline.
@@ -136,6 +148,16 @@ struct hs_ctx {
   std::string graph_key;
   PinnedBuf graph_cum;
   uint64_t graph_entries = 0;
+  // captured single join (hs_join with HS_TUNE_GRAPH): replayed while the
+  // join's key and the buffer epoch are those of the capture
+  hipGraph_t jgraph = nullptr;
+  hipGraphExec_t jgraph_exec = nullptr;
+  std::string jgraph_key;
+  uint64_t jgraph_epoch = 0;
+  hs::DevicePlan jgraph_plan;
+  uint32_t jgraph_tile_plan = 0;
+  PinnedBuf jgraph_cum;         // the captured bound-entry table
+  bool capturing_join = false;  // prepare_device_join stages through jgraph_cum
   uint64_t graph_epoch = 0;  // buffer epoch the capture saw
 };
 
@@ -275,18 +297,6 @@ inline hs_range clamp_range(const hs_range* r, uint32_t n) {
 }
 
 inline bool range_empty(const hs_range& r) { return r.first > r.last; }
-
-// One device join between its set-up and its launches: hs_join runs one
-// launch over its probe range, the CPU/GPU co-processing join one launch per
-// probe stripe it claims, all against one index.
-struct DevicePlan {
-  bool self = false;
-  hs_range ir{1, 0};          // indexed range of the index
-  uint32_t sig_words = 0;     // fgss signature test width (64-bit words)
-  uint32_t bitmap_words = 0;  // signatures computed for the bitmap technique
-  uint64_t index_entries = 0;
-  bool counters_ready = false;  // the index build zeroed the work counters
-};
 
 // Signatures and the prefix index over ir on the context stream (the index
 // also for the bitmap technique when with_index).

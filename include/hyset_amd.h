@@ -195,8 +195,12 @@ typedef struct {
                                  size, within 64..1024) */
 } hs_tune;
 
-/* hs_join_batch: capture the device sequence in a HIP graph on first use and
- * replay it while the collection, the bounds and the tile plan are unchanged. */
+/* hs_join_batch and hs_join: capture the device sequence in a HIP graph on
+ * first use and replay it while the join (collections, ranges, bounds, knobs,
+ * result capacity) is the same and no device or pinned buffer has been
+ * allocated or freed since.  hs_join replays its index build and launch for
+ * the upload-time and fixed tile plans and the bitmap technique; the
+ * balanced and tile-queue plans and the hybrid technique run eagerly. */
 enum { HS_TUNE_GRAPH = 1 };
 /* hs_join fgss tiles.  HS_TUNE_STATIC_TILES: one workgroup per tile of the
  * collection's upload-time plan (at most 512 probes and 8192 tokens).
@@ -251,6 +255,10 @@ typedef struct {
                                 by signature (0 without the signature test) */
   uint32_t signature_bits;   /* fgss: signature width used (0: none) */
   uint32_t tile_plan;        /* fgss: HS_PLAN_* (0 for other techniques) */
+  uint32_t graph;            /* HS_TUNE_GRAPH joins: 1 the sequence was captured, 2 replayed (0: eager);
+                                a replayed join is timed as a whole: index_ms and kernel_ms are 0,
+                                filter_ms covers the index build and the launch */
+  uint32_t reserved;
 } hs_stats;
 
 /* ------------------------------------------------------------- contexts */

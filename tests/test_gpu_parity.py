@@ -249,3 +249,42 @@ def test_rs_long_sets_cosine(ctx):
     finally:
         dR.free()
         dS.free()
+
+
+@pytest.mark.parametrize("algo", ["fgss", "bitmap"])
+def test_join_graph_replays(ctx, small, rs_small, algo):
+    """hs_join with HS_TUNE_GRAPH: the first call runs eagerly and captures,
+    later calls replay; every replay, with eager joins of other collections
+    and thresholds in between (which rewrite the shared staging table,
+    signatures and index), equals the oracle.  A new threshold or range
+    captures anew."""
+    c, d = small
+    R, S, dR, dS = rs_small
+    g = H.Tune(flags=H.TUNE_GRAPH, bitmap_bits=128 if algo == "bitmap" else 0)
+    for t in (0.8, 0.6):
+        ctx.set_threshold(t)
+        want = O.self_join(c, t)
+        for rep in range(4):
+            _check(ctx.join(d, algo=algo, tune=g), want, f"{algo} graph t={t} rep {rep}")
+            st = ctx.stats()
+            assert st["pairs"] == want.count
+            # captured on the first call; replayed once the eager joins in
+            # between have stopped growing buffers (a growth captures anew)
+            if rep == 0:
+                assert st["graph"] == 1, rep
+            if rep >= 2:
+                assert st["graph"] == 2, rep
+            # eager work between replays: another collection, another threshold
+            ctx.set_threshold(0.7, "cosine")
+            ctx.join(dS, dR, algo=algo)
+            ctx.set_threshold(t)
+        rng = (100, 2500)
+        want_b = O.self_join_blocks(c, t, rng, rng)
+        for rep in range(2):
+            _check(ctx.join(d, algo=algo, tune=g, indexed_range=rng, probe_range=rng), want_b, f"{algo} graph block")
+    # R-S through the graph
+    ctx.set_threshold(0.8, "cosine")
+    want = O.rs_join(R, S, 0.8, "cosine")
+    for rep in range(3):
+        _check(ctx.join(dS, dR, algo=algo, tune=g), want, f"{algo} graph R-S rep {rep}")
+    ctx.set_threshold(0.8)
