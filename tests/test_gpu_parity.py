@@ -288,3 +288,74 @@ def test_join_graph_replays(ctx, small, rs_small, algo):
     for rep in range(3):
         _check(ctx.join(dS, dR, algo=algo, tune=g), want, f"{algo} graph R-S rep {rep}")
     ctx.set_threshold(0.8)
+
+
+def test_sets_longer_than_16_bit_fields(ctx):
+    """Sets of more than 65,535 tokens: the size and position fields of an
+    index record saturate and the kernels read sizes[] instead, the set's
+    tokens do not fit the LDS stage, the minoverlap table lives in HBM, the
+    balanced plan is not used.  Six near-duplicates of a 70,000-token set
+    among small sets, Jaccard 0.8 and 0.9, both techniques, as the oracle."""
+    rng = np.random.default_rng(17)
+    base = np.sort(rng.choice(np.arange(1, 300_000), size=70_000, replace=False))
+    longs = []
+    for k in range(6):
+        keep = np.sort(rng.choice(base, size=70_000 - 500 * k, replace=False))
+        extra = rng.choice(np.arange(300_000, 320_000), size=300 * k, replace=False)
+        longs.append(np.unique(np.concatenate([keep, extra])).tolist())
+    smalls = [np.sort(rng.choice(np.arange(1, 2000), size=int(rng.integers(3, 12)), replace=False)).tolist()
+              for _ in range(300)]
+    sets = sorted(smalls + longs, key=lambda s: (len(s), s))
+    c = H.SetCollection.from_sets(sets)
+    assert int(c.sizes.max()) > 65_535
+    d = ctx.upload(c)
+    try:
+        for t in (0.8, 0.9):
+            ctx.set_threshold(t)
+            want = O.self_join(c, t)
+            assert int((np.asarray(want.overlap) > 65_535).sum()) > 0
+            _check(ctx.join(d, algo="fgss"), want, f"long sets fgss t={t}")
+            _check(ctx.join(d, algo="bitmap", tune=H.Tune(bitmap_bits=256)), want, f"long sets bitmap t={t}")
+    finally:
+        d.free()
+
+
+@pytest.mark.parametrize("algo", ["fgss", "bitmap", "hybrid"])
+def test_degenerate_inputs(ctx, algo):
+    """Empty collection, one set, 400 identical sets (every pair a result),
+    singletons only, threshold 1.0 (exact duplicates), and an empty side of
+    an R-S join: every technique, and the co-processing join, as the
+    oracle."""
+    cases = {
+        "empty": [],
+        "one": [[3, 9, 12]],
+        "identical": [[2, 5, 7, 11, 13]] * 400,
+        "singletons": [[i % 50] for i in range(600)],
+        "dups_and_near": sorted([[1, 2, 3, 4]] * 30 + [[1, 2, 3, 5]] * 20 + [[1, 2, 3, 4, 6]] * 10,
+                                key=lambda s: (len(s), s)),
+    }
+    for name, sets in cases.items():
+        c = H.SetCollection.from_sets(sorted(sets, key=lambda s: (len(s), s)))
+        d = ctx.upload(c)
+        try:
+            for t in (0.5, 0.8, 1.0):
+                ctx.set_threshold(t)
+                want = O.self_join(c, t)
+                _check(ctx.join(d, algo=algo), want, f"{algo} {name} t={t}")
+                if algo != "hybrid":
+                    (r, s, ov), _ = ctx.join_coprocess(d, algo=algo, cpu_threads=2, cpu_block=7, device_min_claim=64)
+                    _check((r, s, ov), want, f"coprocess {algo} {name} t={t}")
+        finally:
+            d.free()
+    # R-S with an empty side
+    full = H.SetCollection.from_sets([[1, 2, 3], [1, 2, 4], [2, 3, 4, 5]])
+    none = H.SetCollection.from_sets([])
+    df, dn = ctx.upload(full), ctx.upload(none)
+    try:
+        ctx.set_threshold(0.5)
+        for idx, prb in ((df, dn), (dn, df)):
+            r, s, ov = ctx.join(idx, prb, algo=algo)
+            assert r.shape[0] == 0
+    finally:
+        df.free()
+        dn.free()
