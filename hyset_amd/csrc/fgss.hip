@@ -449,32 +449,44 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
     }
     __syncthreads();
 
-    // one visit per lane; the index entry of the lane's next visit is
-    // fetched before this one is filtered, so its latency overlaps the
-    // filter's loads.  Visits passing both filters go onto the LDS stack;
-    // whenever it holds TPB they are verified, one per lane.  The last batch
-    // of the tile ends with one round without visits that verifies the rest
-    // of the stack (one verification site keeps the register budget).
+    // one visit per lane and sub-round; the index entry of the lane's next
+    // visit is fetched before this one is filtered, so its latency overlaps
+    // the filter's loads.  Visits passing both filters go onto the LDS
+    // stack; whenever it holds TPB they are verified, one per lane.  The
+    // last batch of the tile ends with one sub-round without visits that
+    // verifies the rest of the stack (one verification site keeps the
+    // register budget).
     const bool last_batch = ib + FgGeom<TPB>::ITEMCAP >= total_items;
-    const uint32_t vis_end = total_vis + (last_batch ? TPB : 0u);
-    // (64-bit signatures ride in the index entry and arrive with it).  A
-    // lane's next visit lies TPB further on: with long slices (long sets) it
-    // is in the same item or a near one, found by galloping from the current
-    // item; with short slices a bisection over the batch's items is cheaper.
+    // sub-rounds of TPB visits: a lane takes two consecutive visits, the
+    // even one in sub-round 2k and the odd one in 2k + 1, so one search
+    // finds the items of both (the odd visit's item is the even one's or a
+    // later one a step or two on); the last batch adds a sub-round without
+    // visits that drains the stack
+    const uint32_t n_sub = 2u * ((total_vis + 2u * TPB - 1u) / (2u * TPB)) + (last_batch ? 1u : 0u);
+    // (64-bit signatures ride in the index entry and arrive with it).  After
+    // its odd visit a lane's next visit lies 2 TPB - 1 further on: with long
+    // slices (long sets) it is in the same item or a near one, found by
+    // galloping from the current item; with short slices a bisection over
+    // the batch's items is cheaper.  The search was a fifth of the kernel's
+    // time before pairs halved the number of searches.
     const bool gallop = total_vis >= 64u * nb;
     uint32_t n_it = 0, n_sid = 0, n_meta = 0;
     uint64_t n_sig = 0;
-    if (tid < total_vis) {
-      n_it = segment_of(L.vis_off, nb, tid);
-      load_entry<kW>(A.index, L.it_a[n_it] + (tid - L.vis_off[n_it]), n_sid, n_meta, n_sig);
+    if (2u * tid < total_vis) {
+      n_it = segment_of(L.vis_off, nb, 2u * tid);
+      load_entry<kW>(A.index, L.it_a[n_it] + (2u * tid - L.vis_off[n_it]), n_sid, n_meta, n_sig);
     }
-    for (uint32_t vb = 0; vb < vis_end; vb += TPB, ++rnd) {
-      const uint32_t vi = vb + tid;
+    for (uint32_t sr = 0; sr < n_sub; ++sr, ++rnd) {
+      const uint32_t vi = (sr >> 1) * (2u * TPB) + 2u * tid + (sr & 1u);
       const uint32_t it = n_it, sid = n_sid, meta = n_meta;
       const uint64_t ssig = n_sig;
-      const uint32_t vn = vi + TPB;
+      const uint32_t vn = (sr & 1u) ? vi - 1u + 2u * TPB : vi + 1u;
       if (vn < total_vis) {
-        n_it = gallop ? segment_from(L.vis_off, nb, vn, n_it) : segment_of(L.vis_off, nb, vn);
+        if (sr & 1u) {
+          n_it = gallop ? segment_from(L.vis_off, nb, vn, n_it) : segment_of(L.vis_off, nb, vn);
+        } else {
+          while (L.vis_off[n_it + 1] <= vn) ++n_it;  // vis_off[nb] = total_vis > vn ends it
+        }
         load_entry<kW>(A.index, L.it_a[n_it] + (vn - L.vis_off[n_it]), n_sid, n_meta, n_sig);
       }
       if (tid == 0) L.qadd[(rnd + 1) % 3] = 0;  // its last reader passed the previous barrier
@@ -531,7 +543,7 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
       __syncthreads();
       qn += L.qadd[rnd % 3];
       // TPB entries, or in the final round (no visits, fewer left) the rest
-      const uint32_t take = qn >= TPB ? TPB : (vb + TPB >= vis_end && last_batch ? qn : 0u);
+      const uint32_t take = qn >= TPB ? TPB : (sr + 1u == n_sub && last_batch ? qn : 0u);
       if (take) {
         const bool valid = tid < take;
         const uint32_t e = qn - take + tid;
