@@ -150,17 +150,24 @@ struct TileScope {
 };
 
 // Window slices of K posting lists at once: for each k, a = lower_bound of
-// k_lo[k] in ids[f_lo[k], f_lo[k] + n_lo[k]), then b = lower_bound of
-// k_hi[k] >= k_lo[k] in [a, end).  Each search is KARY-ary: a round reads
-// KARY evenly spaced ids of the interval that holds the answer
-// (independent loads) and keeps the one of the KARY + 1 gaps that holds it,
-// so an interval of n ids takes about log_{KARY+1}(n) rounds; an interval
-// of at most KARY ids is read whole, and a list that short is read once for
-// both bounds.  All K searches advance in lockstep, so their loads overlap.
-// (4-ary measured as fast as 8-ary on the 1M-set index and keeps fewer
-// registers live.)  On return f_lo[k] = a and n_hi[k] = b - a; f_hi is
-// scratch.  n_hi must come in as n_lo.
-constexpr uint32_t KARY = 4;
+// k_lo[k] and b = lower_bound of k_hi[k] >= k_lo[k] in the sorted ids
+// ids[f_lo[k], f_lo[k] + n_lo[k]).  Both bounds are searched together, and
+// a round reads whole aligned quads of ids (one 16-byte load each): every
+// id x of the list that a round reads tightens both answers at once
+// (ids[x] < key puts the answer above x, otherwise at or below x).  A round
+// gives an item WQ quads: all the quads of the interval still open when
+// they fit, else evenly spaced pivot quads, split between the two bounds
+// once their intervals part.  Lists of up to 13 ids resolve in one round
+// and every round shrinks each open interval, so an interval of n ids takes
+// about log_{WQ + 1}(n / 4) rounds instead of the 2 log_5(n) of a
+// 4-ary search per bound over single ids.  All K searches advance in
+// lockstep, so their loads overlap.  The quads may read up to 3 ids past
+// the end of the last list: the id array carries IDS_PAD ids of padding.
+// On return f_lo[k] = start of the slice and n_hi[k] = b - a.
+constexpr uint32_t NO_QUAD = 0xFFFFFFFFu;
+// quads per item and round in k_fgss, whose tile state already holds most
+// of the registers
+constexpr uint32_t FG_SEARCH_QUADS = 2;
 
 // Index record e: set id, meta and (64-bit signature tests) the signature,
 // in one load.
@@ -179,77 +186,91 @@ __device__ __forceinline__ void load_entry(const DevIndex& X, uint32_t e, uint32
   }
 }
 
-template <uint32_t K>
-__device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids, uint32_t stride, uint32_t (&f_lo)[K],
-                                                uint32_t (&n_lo)[K], uint32_t (&f_hi)[K], uint32_t (&n_hi)[K],
+// Plans m quads over the open interval [lo, hi) of a list that starts at
+// global position f: every quad of the interval when at most m of them hold
+// it, else m pivot quads, the i-th at position lo + (i + 1) (hi - lo) / (m + 1).
+template <uint32_t M>
+__device__ __forceinline__ void plan_quads(uint32_t f, uint32_t lo, uint32_t hi, uint32_t (&q)[M], uint32_t o,
+                                           uint32_t m) {
+  const uint32_t first = (f + lo) >> 2, last = (f + hi - 1) >> 2;
+  const uint32_t span = last - first + 1;
+#pragma unroll
+  for (uint32_t i = 0; i < M; ++i) {
+    if (i < o || i >= o + m) continue;
+    const uint32_t r = i - o;
+    if (span <= m) q[i] = r < span ? first + r : NO_QUAD;
+    else q[i] = (f + lo + (uint32_t)((uint64_t)(r + 1) * (hi - lo) / (m + 1))) >> 2;
+  }
+}
+
+template <uint32_t K, uint32_t WQ>
+__device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids, uint32_t (&f_lo)[K],
+                                                const uint32_t (&n_lo)[K], uint32_t (&n_hi)[K],
                                                 const uint32_t (&k_lo)[K], const uint32_t (&k_hi)[K]) {
-  // per search: the answer lies in [lo, hi] (ids below lo are < key, ids at
-  // hi and beyond are >= key); phase 0 looks for k_lo, phase 1 for k_hi, 2 done
-  uint32_t lo[K], hi[K], ph[K], a[K];
+  // a lies in [alo, ahi], b in [blo, bhi] (counts of ids below the key)
+  uint32_t alo[K], ahi[K], blo[K], bhi[K];
 #pragma unroll
   for (uint32_t k = 0; k < K; ++k) {
-    lo[k] = 0;
-    hi[k] = n_lo[k];
-    a[k] = 0;
-    ph[k] = n_lo[k] ? 0u : 2u;
-    f_hi[k] = 0;
+    alo[k] = blo[k] = 0;
+    ahi[k] = bhi[k] = n_lo[k];
   }
   for (;;) {
     bool any = false;
 #pragma unroll
-    for (uint32_t k = 0; k < K; ++k) any |= ph[k] < 2;
+    for (uint32_t k = 0; k < K; ++k) any |= alo[k] < ahi[k] || blo[k] < bhi[k];
     if (!any) break;
-    uint32_t v[K][KARY];
+    uint32_t q[K][WQ];
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
-      const uint32_t len = hi[k] - lo[k];
 #pragma unroll
-      for (uint32_t q = 0; q < KARY; ++q) {
-        const uint32_t pos = len <= KARY ? lo[k] + q : lo[k] + (uint32_t)((uint64_t)(q + 1) * len / (KARY + 1));
-        const bool rd = ph[k] < 2 && (len > KARY || q < len);
-        v[k][q] = rd ? ids[(size_t)(f_lo[k] + pos) * stride] : 0xFFFFFFFFu;
+      for (uint32_t i = 0; i < WQ; ++i) q[k][i] = NO_QUAD;
+      const bool oa = alo[k] < ahi[k], ob = blo[k] < bhi[k];
+      const uint32_t f = f_lo[k];
+      if (oa && ob) {
+        const uint32_t span = ((f + bhi[k] - 1) >> 2) - ((f + alo[k]) >> 2) + 1;
+        if (span <= WQ || (alo[k] == blo[k] && ahi[k] == bhi[k])) {
+          plan_quads(f, alo[k], bhi[k], q[k], 0, WQ);  // one interval for both bounds
+        } else {
+          plan_quads(f, alo[k], ahi[k], q[k], 0, WQ / 2);
+          plan_quads(f, blo[k], bhi[k], q[k], WQ / 2, WQ / 2);
+        }
+      } else if (oa) {
+        plan_quads(f, alo[k], ahi[k], q[k], 0, WQ);
+      } else if (ob) {
+        plan_quads(f, blo[k], bhi[k], q[k], 0, WQ);
       }
     }
+    uint4 v[K][WQ];
+#pragma unroll
+    for (uint32_t k = 0; k < K; ++k)
+#pragma unroll
+      for (uint32_t i = 0; i < WQ; ++i)
+        v[k][i] = q[k][i] != NO_QUAD ? reinterpret_cast<const uint4*>(ids)[q[k][i]] : make_uint4(0, 0, 0, 0);
 #pragma unroll
     for (uint32_t k = 0; k < K; ++k) {
-      if (ph[k] >= 2) continue;
-      const uint32_t len = hi[k] - lo[k];
-      const uint32_t key = ph[k] == 0 ? k_lo[k] : k_hi[k];
-      uint32_t c = 0;
 #pragma unroll
-      for (uint32_t q = 0; q < KARY; ++q) c += v[k][q] < key;  // unread slots hold 0xFFFFFFFF
-      if (len <= KARY) {
-        const uint32_t ans = lo[k] + c;
-        if (ph[k] == 0 && lo[k] == 0 && hi[k] == n_lo[k]) {
-          // the whole list was read: the second bound from the same ids
-          uint32_t c2 = 0;
+      for (uint32_t i = 0; i < WQ; ++i) {
+        if (q[k][i] == NO_QUAD) continue;
+        const uint32_t e[4] = {v[k][i].x, v[k][i].y, v[k][i].z, v[k][i].w};
 #pragma unroll
-          for (uint32_t q = 0; q < KARY; ++q) c2 += v[k][q] < k_hi[k];
-          a[k] = ans;
-          hi[k] = max(c2, ans);  // b
-          ph[k] = 2;
-        } else if (ph[k] == 0) {
-          a[k] = ans;
-          lo[k] = ans;
-          hi[k] = n_lo[k];
-          ph[k] = 1;
-        } else {
-          hi[k] = ans;  // b
-          ph[k] = 2;
+        for (uint32_t c = 0; c < 4; ++c) {
+          const uint32_t x = 4u * q[k][i] + c - f_lo[k];  // list position (wraps below the list)
+          if (x >= n_lo[k]) continue;
+          if (e[c] < k_lo[k]) alo[k] = max(alo[k], x + 1);
+          else ahi[k] = min(ahi[k], x);
+          if (e[c] < k_hi[k]) blo[k] = max(blo[k], x + 1);
+          else bhi[k] = min(bhi[k], x);
         }
-      } else {
-        const uint32_t nlo = c == 0 ? lo[k] : lo[k] + (uint32_t)((uint64_t)c * len / (KARY + 1)) + 1;
-        const uint32_t nhi = c == KARY ? hi[k] : lo[k] + (uint32_t)((uint64_t)(c + 1) * len / (KARY + 1));
-        lo[k] = nlo;
-        hi[k] = nhi;
       }
+      // a <= b
+      blo[k] = max(blo[k], alo[k]);
+      ahi[k] = min(ahi[k], bhi[k]);
     }
   }
 #pragma unroll
   for (uint32_t k = 0; k < K; ++k) {
-    const uint32_t b = n_lo[k] ? hi[k] : 0u;
-    f_lo[k] += a[k];
-    n_hi[k] = b - a[k];
+    f_lo[k] += alo[k];
+    n_hi[k] = bhi[k] > alo[k] ? bhi[k] - alo[k] : 0u;
   }
 }
 
@@ -395,11 +416,11 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
         L.vis_off[it] = len;
       }
     } else {
-      uint32_t f_lo[PER], n_lo[PER], f_hi[PER], n_hi[PER], k_lo[PER], k_hi[PER];
+      uint32_t f_lo[PER], n_lo[PER], n_hi[PER], k_lo[PER], k_hi[PER];
 #pragma unroll
       for (uint32_t k = 0; k < PER; ++k) {
         const uint32_t it = tid + k * TPB;
-        f_lo[k] = f_hi[k] = n_lo[k] = n_hi[k] = k_lo[k] = k_hi[k] = 0;
+        f_lo[k] = n_lo[k] = n_hi[k] = k_lo[k] = k_hi[k] = 0;
         if (it < nb) {
           const uint32_t item = ib + it;
           const uint32_t pl = segment_of(L.item_off, np, item);
@@ -412,14 +433,14 @@ __device__ __forceinline__ void fgss_tile(const FgssArgs& A, const TileScope& T,
           if (token < A.index.universe) {
             const uint32_t lb = A.index.offsets[token];
             const uint32_t le = A.index.offsets[token + 1];
-            f_lo[k] = f_hi[k] = lb;
+            f_lo[k] = lb;
             n_lo[k] = n_hi[k] = le - lb;
           }
         }
       }
       // on the compact ids: a list is searched over more entries than its
       // window visits, so 4-byte ids touch fewer lines than the records
-      lockstep_bounds<PER>(A.index.set_ids, 1u, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
+      lockstep_bounds<PER, FG_SEARCH_QUADS>(A.index.set_ids, f_lo, n_lo, n_hi, k_lo, k_hi);
 #pragma unroll
       for (uint32_t k = 0; k < PER; ++k) {
         const uint32_t it = tid + k * TPB;
@@ -712,7 +733,8 @@ __device__ __forceinline__ SizeTab size_tab(const uint32_t* t, uint32_t nl) {
   return SizeTab{t, t + nl, t + 2 * nl, t + 3 * nl, t + 4 * nl};
 }
 
-constexpr uint32_t IS_PER = 4;  // item slots per thread, searched in lockstep
+constexpr uint32_t IS_PER = 2;           // item slots per thread, searched in lockstep
+constexpr uint32_t IS_SEARCH_QUADS = 4;  // quads per item and search round
 
 __global__ __launch_bounds__(256) void k_item_slices(FgssArgs A, const uint32_t* __restrict__ gtab, uint32_t nl,
                                                      uint32_t total, uint32_t* __restrict__ slice_a,
@@ -727,10 +749,10 @@ __global__ __launch_bounds__(256) void k_item_slices(FgssArgs A, const uint32_t*
   const SizeTab T = size_tab(tab, nl);
   // slots base + threadIdx.x + k * 256: coalesced stores
   const uint32_t base = blockIdx.x * (256 * IS_PER) + threadIdx.x;
-  uint32_t f_lo[IS_PER], n_lo[IS_PER], f_hi[IS_PER], n_hi[IS_PER], k_lo[IS_PER], k_hi[IS_PER];
+  uint32_t f_lo[IS_PER], n_lo[IS_PER], n_hi[IS_PER], k_lo[IS_PER], k_hi[IS_PER];
 #pragma unroll
   for (uint32_t k = 0; k < IS_PER; ++k) {
-    f_lo[k] = f_hi[k] = n_lo[k] = n_hi[k] = k_lo[k] = k_hi[k] = 0;
+    f_lo[k] = n_lo[k] = n_hi[k] = k_lo[k] = k_hi[k] = 0;
     const uint32_t s = base + k * 256;
     if (s >= total) continue;
     // size class: the last l with cum[l] <= s (its run is non-empty)
@@ -751,12 +773,12 @@ __global__ __launch_bounds__(256) void k_item_slices(FgssArgs A, const uint32_t*
       const uint32_t token = A.probe.tokens[A.probe.starts[p] + j];
       if (token < A.index.universe) {
         const uint32_t lb = A.index.offsets[token];
-        f_lo[k] = f_hi[k] = lb;
+        f_lo[k] = lb;
         n_lo[k] = n_hi[k] = A.index.offsets[token + 1] - lb;
       }
     }
   }
-  lockstep_bounds<IS_PER>(A.index.set_ids, 1u, f_lo, n_lo, f_hi, n_hi, k_lo, k_hi);
+  lockstep_bounds<IS_PER, IS_SEARCH_QUADS>(A.index.set_ids, f_lo, n_lo, n_hi, k_lo, k_hi);
 #pragma unroll
   for (uint32_t k = 0; k < IS_PER; ++k) {
     const uint32_t s = base + k * 256;

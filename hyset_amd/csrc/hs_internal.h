@@ -110,7 +110,9 @@ struct DevColl {
 // filtered from its own entry, without reading sizes[] or the signature
 // table at random; interleaved, a visit reads one record instead of three
 // arrays.  The window search reads the compact set_ids (a list is searched
-// over more entries than its window visits).
+// over more entries than its window visits) in aligned 16-byte quads, so
+// set_ids is allocated IDS_PAD ids longer than the index.
+constexpr uint32_t IDS_PAD = 4;
 struct DevIndex {
   const uint32_t* offsets;  // universe + 1
   const uint32_t* set_ids;  // the ids alone (host copies, the item-slice pass)

@@ -464,7 +464,7 @@ static void prepare(IndexBuild& ib, uint32_t universe, uint64_t entries) {
   ib.entries = entries;
   const size_t cap = entries ? entries : 1;
   ib.offsets.ensure(((size_t)universe + 1) * sizeof(uint32_t));
-  ib.set_ids.ensure(cap * sizeof(uint32_t));
+  ib.set_ids.ensure((cap + IDS_PAD) * sizeof(uint32_t));  // the window search reads whole quads
   ib.positions.ensure(cap * sizeof(uint16_t));
   ib.ent.ensure(cap * 4 * sizeof(uint32_t));
   ib.keys[0].ensure(cap * sizeof(uint32_t));
