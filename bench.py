@@ -376,11 +376,23 @@ def main() -> int:
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     st = steps[-1]
-    kern_ms = statistics.median(s["kernel_ms"] for s in steps)
     dev_ms = statistics.median(s["total_ms"] for s in steps)
-    idx_ms = statistics.median(s["index_ms"] for s in steps)
-    log(f"fgss t={t}: {pairs} pairs, {ms_per_step:.3f} ms/step wall, device {dev_ms:.3f} ms "
-        f"(index {idx_ms:.3f}, k_fgss {kern_ms:.3f})")
+    # The same steps again with the stage events (HS_TUNE_STAGE_TIMES): the
+    # index build's end and the k_fgss dispatch's start and end, taken by the
+    # dispatches themselves on the join's stream.  They cost the device a
+    # few microseconds per join, so the throughput above runs without them.
+    stune = H.Tune(flags=H.TUNE_STAGE_TIMES)
+    ctx.join(d, algo="fgss", tune=stune, count_only=True)
+    ssteps = []
+    a = time.perf_counter()
+    for _ in range(args.steps):
+        assert ctx.join(d, algo="fgss", tune=stune, count_only=True) == pairs
+        ssteps.append(ctx.stats())
+    ms_per_step_stages = (time.perf_counter() - a) * 1e3 / args.steps
+    kern_ms = statistics.median(s["kernel_ms"] for s in ssteps)
+    idx_ms = statistics.median(s["index_ms"] for s in ssteps)
+    log(f"fgss t={t}: {pairs} pairs, {ms_per_step:.3f} ms/step wall, device {dev_ms:.3f} ms; with stage events "
+        f"{ms_per_step_stages:.3f} ms/step (index {idx_ms:.3f}, k_fgss {kern_ms:.3f})")
 
     # the same join replayed from a HIP graph (HS_TUNE_GRAPH): every kernel
     # runs each step; the host enqueues one graph instead of the sequence
@@ -416,6 +428,7 @@ def main() -> int:
     extra = {"ms_per_step_graph": round(ms_per_step_graph, 4), "join_ms_device_graph": round(dev_ms_graph, 4),
              "graph_replayed_every_step": replayed,
              "join_ms_device": round(dev_ms, 4), "index_ms": round(idx_ms, 4), "k_fgss_ms": round(kern_ms, 4),
+             "ms_per_step_stage_events": round(ms_per_step_stages, 4),
              "join_ms_with_d2h": round(ms_with_d2h, 4), "pairs": int(pairs),
              "candidates": int(st["candidates"]), "posting_visits": int(st["posting_visits"]),
              "index_entries": int(st["index_entries"]), "probe_prefix_tokens": int(st["probe_prefix_tokens"]),
@@ -487,30 +500,26 @@ def main() -> int:
         ctx.set_threshold(t)
         # the other tile plans: the upload-time tiles on a persistent grid,
         # and work-balanced tiles planned on the device (item-slice pass)
-        for name, flag in (("static_tiles", H.TUNE_STATIC_TILES), ("tile_queue", H.TUNE_TILE_QUEUE),
-                           ("balanced_tiles", H.TUNE_BALANCED_TILES)):
-            tu = H.Tune(flags=flag)
-            ctx.join(d, algo="fgss", tune=tu, count_only=True)
+        # (join times without stage events, kernel times with them)
+        def plan_times(flags):
+            pt, st_ = H.Tune(flags=flags), H.Tune(flags=flags | H.TUNE_STAGE_TIMES)
+            ctx.join(d, algo="fgss", tune=pt, count_only=True)
             ks, ds = [], []
             for _ in range(5):
-                assert ctx.join(d, algo="fgss", tune=tu, count_only=True) == pairs
-                ks.append(ctx.stats()["kernel_ms"])
+                assert ctx.join(d, algo="fgss", tune=pt, count_only=True) == pairs
                 ds.append(ctx.stats()["total_ms"])
-            extra[f"k_fgss_{name}_ms"] = round(statistics.median(ks), 4)
-            extra[f"join_ms_device_{name}"] = round(statistics.median(ds), 4)
+                assert ctx.join(d, algo="fgss", tune=st_, count_only=True) == pairs
+                ks.append(ctx.stats()["kernel_ms"])
+            return round(statistics.median(ks), 4), round(statistics.median(ds), 4), ctx.stats()
+
+        for name, flag in (("static_tiles", H.TUNE_STATIC_TILES), ("tile_queue", H.TUNE_TILE_QUEUE),
+                           ("balanced_tiles", H.TUNE_BALANCED_TILES)):
+            extra[f"k_fgss_{name}_ms"], extra[f"join_ms_device_{name}"], _ = plan_times(flag)
             log(f"{name}: k_fgss {extra[f'k_fgss_{name}_ms']} ms, join {extra[f'join_ms_device_{name}']} ms "
                 f"(default: k_fgss {kern_ms:.4f} ms, join {dev_ms:.4f} ms)")
         # the same join without the signature test (every positional survivor merged)
-        ns_tune = H.Tune(flags=H.TUNE_NO_SIGNATURE)
-        ctx.join(d, algo="fgss", tune=ns_tune, count_only=True)
-        ks, ds = [], []
-        for _ in range(5):
-            assert ctx.join(d, algo="fgss", tune=ns_tune, count_only=True) == pairs
-            ks.append(ctx.stats()["kernel_ms"])
-            ds.append(ctx.stats()["total_ms"])
-        extra["k_fgss_no_signature_ms"] = round(statistics.median(ks), 4)
-        extra["join_ms_device_no_signature"] = round(statistics.median(ds), 4)
-        extra["candidates_no_signature"] = int(ctx.stats()["candidates"])
+        extra["k_fgss_no_signature_ms"], extra["join_ms_device_no_signature"], sn = plan_times(H.TUNE_NO_SIGNATURE)
+        extra["candidates_no_signature"] = int(sn["candidates"])
         log(f"k_fgss without the signature test: {extra['k_fgss_no_signature_ms']} ms, "
             f"{extra['candidates_no_signature']} candidates verified")
         # the bitmap filter is bound by integer VALU work, not HBM: its pair
@@ -519,7 +528,7 @@ def main() -> int:
         popc_gops = ctx.popcount_rate()
         extra["popcount_peak_gops"] = round(popc_gops, 1)
         for bits in (64, 128, 256):
-            tune = H.Tune(bitmap_bits=bits)
+            tune = H.Tune(bitmap_bits=bits, flags=H.TUNE_STAGE_TIMES)
             c = ctx.join(d, algo="bitmap", tune=tune, count_only=True)
             sb = ctx.stats()
             extra[f"bitmap{bits}_ms"] = round(sb["total_ms"], 3)

@@ -61,6 +61,7 @@ TUNE_TILE_QUEUE = 2
 TUNE_NO_SIGNATURE = 4
 TUNE_BALANCED_TILES = 8
 TUNE_STATIC_TILES = 16
+TUNE_STAGE_TIMES = 32
 PLAN_FIXED, PLAN_UPLOAD, PLAN_QUEUE, PLAN_BALANCED = 1, 2, 3, 4
 
 

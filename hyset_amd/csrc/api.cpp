@@ -519,6 +519,8 @@ int hs_set_threshold(hs_ctx* ctx, int32_t sim, double threshold, int32_t bounds_
 void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_range ir, hs_range pr, int32_t algo,
                              const hs_tune& tune, bool with_index, DevicePlan& P) {
   hipStream_t st = ctx->stream;
+  // index_ms: ev[1] takes the end of the index build (stage timing only)
+  const bool timed = ctx->stage_times && !ctx->capturing_join;
   P = DevicePlan{};
   const bool self = indexed == probe;
   P.self = self;
@@ -544,10 +546,8 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
     const auto& ptab = self ? ctx->tables.midprefix : ctx->tables.maxprefix;
     const auto& sf = indexed->h_size_first;
     const uint32_t nl = (uint32_t)sf.size();  // maxlen + 2
-    PinnedBuf& hcum = ctx->capturing_join ? ctx->jgraph_cum : ctx->h_cum;
-    hcum.ensure((size_t)nl * 4);
-    ctx->d_cum.ensure((size_t)nl * 4);
-    uint32_t* cum = hcum.as<uint32_t>();
+    std::vector<uint32_t>& cum = ctx->jcum_tmp;
+    cum.resize(nl);
     uint64_t acc = 0;
     for (uint32_t l = 0; l + 1 < nl; ++l) {
       cum[l] = (uint32_t)acc;
@@ -558,15 +558,29 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
     }
     cum[nl - 1] = (uint32_t)acc;
     P.index_entries = acc;
-    HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, cum, (size_t)nl * 4, hipMemcpyHostToDevice, st));
+    ctx->d_jcum.ensure((size_t)nl * 4);
+    // the same table as the one the device holds: no copy (repeated joins)
+    const bool held = !ctx->capturing_join && ctx->jcum_valid && ctx->jcum_epoch == hs::buffer_epoch() &&
+                      ctx->jcum_host == cum;
+    if (!held) {
+      PinnedBuf& hcum = ctx->capturing_join ? ctx->jgraph_cum : ctx->h_cum;
+      hcum.ensure((size_t)nl * 4);
+      std::memcpy(hcum.ptr, cum.data(), (size_t)nl * 4);
+      HS_CHECK_HIP(hipMemcpyAsync(ctx->d_jcum.ptr, hcum.ptr, (size_t)nl * 4, hipMemcpyHostToDevice, st));
+      ctx->jcum_valid = !ctx->capturing_join;
+      ctx->jcum_host = cum;
+      ctx->jcum_epoch = hs::buffer_epoch();
+    }
     hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
-                    ctx->d_cum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
+                    ctx->d_jcum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
                     P.sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr,
                     sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
-                    algo == HS_ALGO_FGSS ? ctx->counters.as<unsigned long long>() : nullptr);
+                    algo == HS_ALGO_FGSS ? ctx->counters.as<unsigned long long>() : nullptr,
+                    timed ? ctx->ev[1] : nullptr);
     P.counters_ready = algo == HS_ALGO_FGSS && acc > 0 && indexed->n > 0;
   } else {
     ctx->ib.universe = 0;
+    if (timed) HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
   }
 }
 
@@ -576,8 +590,12 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
   const bool self = P.self;
   const hs_range ir = P.ir;
   uint32_t tile_plan = 0;
+  const bool timed = ctx->stage_times && !ctx->capturing_join;
   if (range_empty(pr) || range_empty(ir)) {
-    if (!ctx->capturing_join) HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
+    if (timed) {
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
+    }
   } else if (algo == HS_ALGO_FGSS) {
     hs::FgssArgs a{};
     a.probe = probe->dev();
@@ -684,8 +702,8 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
       // as many workgroups per CU as fit: two of 512 lanes, four of 256
       wgs = (uint32_t)ctx->num_cus * (tune.workgroups_per_cu ? tune.workgroups_per_cu : a.narrow_tiles ? 4 : 2);
     }
-    if (!ctx->capturing_join) HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));  // kernel_ms: the fgss launch alone
-    hs::launch_fgss(st, a, ppt, wgs);
+    // kernel_ms: the k_fgss dispatch alone, timed by the dispatch itself
+    hs::launch_fgss(st, a, ppt, wgs, timed ? ctx->ev[5] : nullptr, timed ? ctx->ev[2] : nullptr);
   } else if (algo == HS_ALGO_BITMAP) {
     const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
     HS_REQUIRE(bits == 64 || bits == 128 || bits == 256, HS_E_INVALID, "bitmap width must be 64, 128 or 256");
@@ -710,11 +728,13 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     b.idx_last = ir.last;
     b.self = self;
     b.out = dev_out(ctx);
-    if (!ctx->capturing_join) HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
+    if (timed) HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
     hs::launch_bitmap(st, b);
+    if (timed) HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
   } else {
     HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
     run_hybrid(ctx, indexed, probe, self, ir, pr, tune);
+    HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
   }
   return tile_plan;
 }
@@ -828,38 +848,40 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     const std::string key = want_graph ? join_graph_key(ctx, indexed, probe, ir, pr, algo, tune) : std::string();
     const bool replay = want_graph && ctx->jgraph_exec && ctx->jgraph_key == key && ctx->jgraph_epoch == buffer_epoch();
     uint32_t graph_state = 0;
+    // Marker events between kernels cost the device a few microseconds of
+    // idle time each, so by default the join is timed as a whole: ev[0]
+    // before the first launch, ev[3] after the counters' copy back.  With
+    // HS_TUNE_STAGE_TIMES ev[1] takes the end of the index build and
+    // ev[5] / ev[2] the start and end of the main kernel (fgss: from the
+    // dispatches themselves, bitmap and hybrid: markers).  The graph replay
+    // is timed as a whole.
+    const bool stages = (tune.flags & HS_TUNE_STAGE_TIMES) && algo != HS_ALGO_HYBRID;
+    ctx->stage_times = stages || algo == HS_ALGO_HYBRID;
     if (replay) {
       graph_state = 2;
-      // the graph is timed as a whole: filter_ms covers index and launch
       HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
-      HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
       HS_CHECK_HIP(hipGraphLaunch(ctx->jgraph_exec, st));
-      HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
-      HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
+      ctx->jcum_valid = false;  // the graph rewrote the entry table
       P = ctx->jgraph_plan;
       tile_plan = ctx->jgraph_tile_plan;
-      read_counters(ctx, cnt);
+      read_counters(ctx, cnt, ctx->ev[3]);
       if (cnt[hs::CNT_PAIRS] > ctx->out_cap) {  // more pairs than the buffers hold: grow, run again eagerly
         drop_join_graph(ctx);
         ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
         reset_for_launch(P);
-        HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
         tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
-        HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
-        read_counters(ctx, cnt);
+        read_counters(ctx, cnt, ctx->ev[3]);
         HS_REQUIRE(cnt[hs::CNT_PAIRS] <= ctx->out_cap, HS_E_DEVICE, "result count changed between runs");
       }
     } else {
       HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
       hs::prepare_device_join(ctx, indexed, probe, ir, pr, algo, tune, false, P);
-      HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
 
       // ---- candidate generation + verification
       for (int attempt = 0; attempt < 2; ++attempt) {
         if (attempt > 0 || !P.counters_ready) reset_for_launch(P);
         tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
-        HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
-        read_counters(ctx, cnt);
+        read_counters(ctx, cnt, ctx->ev[3]);
         if (cnt[hs::CNT_PAIRS] <= ctx->out_cap) break;
         HS_REQUIRE(attempt == 0, HS_E_DEVICE, "result count changed between runs");
         ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
@@ -874,18 +896,23 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     const uint64_t count = cnt[hs::CNT_PAIRS];
     set_device_result(ctx, count);
     result->count = count;
-    HS_CHECK_HIP(hipEventRecord(ctx->ev[3], st));
-    if (!result->count_only && count > 0) {
+    const bool copy = !result->count_only && count > 0;
+    if (copy) {
       HS_REQUIRE(count <= result->capacity, HS_E_OVERFLOW, "result capacity too small");
       copy_pairs(ctx, count, result);
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[4], st));
+      HS_CHECK_HIP(hipEventSynchronize(ctx->ev[4]));
+      ctx->stats.transfer_ms = elapsed(ctx->ev[3], ctx->ev[4]);
     }
-    HS_CHECK_HIP(hipEventRecord(ctx->ev[4], st));
-    HS_CHECK_HIP(hipEventSynchronize(ctx->ev[4]));
-    ctx->stats.index_ms = elapsed(ctx->ev[0], ctx->ev[1]);
-    ctx->stats.filter_ms = elapsed(ctx->ev[1], ctx->ev[2]);
-    ctx->stats.transfer_ms = elapsed(ctx->ev[3], ctx->ev[4]);
-    ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
-    ctx->stats.kernel_ms = elapsed(ctx->ev[5], ctx->ev[2]);
+    ctx->stats.total_ms = elapsed(ctx->ev[0], copy ? ctx->ev[4] : ctx->ev[3]);
+    if (graph_state == 2) {
+      ctx->stats.filter_ms = elapsed(ctx->ev[0], ctx->ev[3]);
+    } else if (ctx->stage_times) {
+      ctx->stats.index_ms = elapsed(ctx->ev[0], ctx->ev[1]);
+      ctx->stats.filter_ms = elapsed(ctx->ev[1], ctx->ev[2]);
+      ctx->stats.kernel_ms = elapsed(ctx->ev[5], ctx->ev[2]);
+    }
+    ctx->stage_times = false;
     ctx->stats.pairs = count;
     ctx->stats.graph = graph_state;
     if (algo != HS_ALGO_HYBRID) {

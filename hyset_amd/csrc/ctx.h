@@ -157,7 +157,15 @@ struct hs_ctx {
   hs::DevicePlan jgraph_plan;
   uint32_t jgraph_tile_plan = 0;
   PinnedBuf jgraph_cum;         // the captured bound-entry table
+  // hs_join's entry table on the device, and what it holds: the copy is
+  // skipped while the table, the buffer and the allocation epoch are as
+  // they were (graph replays rewrite it, so they clear jcum_valid)
+  DevBuf d_jcum;
+  std::vector<uint32_t> jcum_host, jcum_tmp;
+  bool jcum_valid = false;
+  uint64_t jcum_epoch = 0;
   bool capturing_join = false;  // prepare_device_join stages through jgraph_cum
+  bool stage_times = false;     // this join records events between its stages (HS_TUNE_STAGE_TIMES)
   uint64_t graph_epoch = 0;  // buffer epoch the capture saw
 };
 
@@ -254,9 +262,12 @@ inline void reset_counters(hs_ctx* ctx) {
   hs::launch_fill_u64(ctx->stream, ctx->counters.as<unsigned long long>(), hs::CNT_WORDS, 0);
 }
 
-inline void read_counters(hs_ctx* ctx, unsigned long long* dst) {
+// Reads the work counters back (synchronizes the stream); `done`, when
+// given, is recorded after the copy, so it completes with the device work.
+inline void read_counters(hs_ctx* ctx, unsigned long long* dst, hipEvent_t done = nullptr) {
   HS_CHECK_HIP(hipMemcpyAsync(ctx->h_counters.ptr, ctx->counters.ptr, hs::CNT_WORDS * sizeof(unsigned long long),
                               hipMemcpyDeviceToHost, ctx->stream));
+  if (done) HS_CHECK_HIP(hipEventRecord(done, ctx->stream));
   HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   std::memcpy(dst, ctx->h_counters.ptr, hs::CNT_WORDS * sizeof(unsigned long long));
 }

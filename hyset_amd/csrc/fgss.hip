@@ -44,6 +44,7 @@
 //     verify_pair (verification.hpp:9-31) for failing pairs and runs to the
 //     end for passing ones, which yields the exact overlap;
 //   * passing pairs leave through one atomic per wave.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -232,7 +233,7 @@ __device__ __forceinline__ void lockstep_bounds(const uint32_t* __restrict__ ids
           plan_quads(f, alo[k], bhi[k], q[k], 0, WQ);  // one interval for both bounds
         } else {
           plan_quads(f, alo[k], ahi[k], q[k], 0, WQ / 2);
-          plan_quads(f, blo[k], bhi[k], q[k], WQ / 2, WQ / 2);
+          plan_quads(f, blo[k], bhi[k], q[k], WQ / 2, WQ - WQ / 2);
         }
       } else if (oa) {
         plan_quads(f, alo[k], ahi[k], q[k], 0, WQ);
@@ -875,17 +876,19 @@ void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp) {
   a.dev_ntiles = bp.ntiles->as<uint32_t>();
 }
 
-// One k_fgss instance per signature width.
+// One k_fgss instance per signature width.  t0 / t1 (may be NULL) take the
+// dispatch's own start and end times: no marker packets between kernels.
 template <int TPB, int kMode, bool kSlices, bool kMulti>
-static void launch_width(hipStream_t st, const FgssArgs& a, dim3 grid, uint32_t ppt, uint32_t ntiles) {
+static void launch_width(hipStream_t st, const FgssArgs& a, dim3 grid, uint32_t ppt, uint32_t ntiles,
+                         hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
   const dim3 block(TPB);
   switch (a.sig_words) {
-    case 0: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 0>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 1: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 1>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 2: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 2>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 4: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 4>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 8: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 8>), grid, block, 0, st, a, ppt, ntiles); break;
-    case 16: hipLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 16>), grid, block, 0, st, a, ppt, ntiles); break;
+    case 0: hipExtLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 0>), grid, block, 0, st, t0, t1, 0, a, ppt, ntiles); break;
+    case 1: hipExtLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 1>), grid, block, 0, st, t0, t1, 0, a, ppt, ntiles); break;
+    case 2: hipExtLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 2>), grid, block, 0, st, t0, t1, 0, a, ppt, ntiles); break;
+    case 4: hipExtLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 4>), grid, block, 0, st, t0, t1, 0, a, ppt, ntiles); break;
+    case 8: hipExtLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 8>), grid, block, 0, st, t0, t1, 0, a, ppt, ntiles); break;
+    case 16: hipExtLaunchKernelGGL((k_fgss<TPB, kMode, kSlices, kMulti, 16>), grid, block, 0, st, t0, t1, 0, a, ppt, ntiles); break;
     default: throw Error{HS_E_INVALID, "signature width must be 0, 64, 128, 256, 512 or 1024 bits"};
   }
   HS_CHECK_HIP(hipGetLastError());
@@ -897,26 +900,33 @@ static void check_sigs(const FgssArgs& a) {
              "64-bit signature test needs an index with signatures");
 }
 
-void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persistent_wgs) {
-  if (a.probe.n == 0 || a.probe_first > a.probe_last) return;
+void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persistent_wgs, hipEvent_t t0,
+                 hipEvent_t t1) {
+  // no launch: the events still complete, as markers
+  auto none = [&] {
+    if (t0) HS_CHECK_HIP(hipEventRecord(t0, st));
+    if (t1) HS_CHECK_HIP(hipEventRecord(t1, st));
+  };
+  if (a.probe.n == 0 || a.probe_first > a.probe_last) return none();
   check_sigs(a);
   if (ppt == 0 || ppt > (uint32_t)FG_TPB) ppt = FG_TPB;
   const uint32_t nprobes = a.probe_last - a.probe_first + 1;
   const uint32_t ntiles = a.tile_starts ? a.n_plan_tiles : (nprobes + ppt - 1) / ppt;
-  if (ntiles == 0) return;
+  if (ntiles == 0) return none();
   uint32_t grid = (ntiles + 7) / 8 * 8;
   if (a.tile_queue) {
     launch_fill_u32(st, a.tile_queue, 8, 0);
     if (persistent_wgs) grid = std::min(grid, (persistent_wgs + 7) / 8 * 8);
   }
+  const dim3 g(grid);
   if (a.tile_queue) {
-    if (a.slice_a && a.narrow_tiles) launch_width<FG_TPB_NARROW, FG_QUEUE, true, false>(st, a, dim3(grid), ppt, ntiles);
-    else if (a.slice_a) launch_width<FG_TPB, FG_QUEUE, true, false>(st, a, dim3(grid), ppt, ntiles);
-    else launch_width<FG_TPB, FG_QUEUE, false, false>(st, a, dim3(grid), ppt, ntiles);
+    if (a.slice_a && a.narrow_tiles) launch_width<FG_TPB_NARROW, FG_QUEUE, true, false>(st, a, g, ppt, ntiles, t0, t1);
+    else if (a.slice_a) launch_width<FG_TPB, FG_QUEUE, true, false>(st, a, g, ppt, ntiles, t0, t1);
+    else launch_width<FG_TPB, FG_QUEUE, false, false>(st, a, g, ppt, ntiles, t0, t1);
   } else if (a.probe_tidx) {
-    launch_width<FG_TPB, FG_TILES, false, true>(st, a, dim3(grid), ppt, ntiles);
+    launch_width<FG_TPB, FG_TILES, false, true>(st, a, g, ppt, ntiles, t0, t1);
   } else {
-    launch_width<FG_TPB, FG_TILES, false, false>(st, a, dim3(grid), ppt, ntiles);
+    launch_width<FG_TPB, FG_TILES, false, false>(st, a, g, ppt, ntiles, t0, t1);
   }
 }
 

@@ -163,11 +163,12 @@ struct IndexBuild {
 // 64-bit signatures of the range's sets into it on the way (then pass the
 // same array as sigs).  counters (may be NULL): CNT_WORDS work counters the
 // last kernel zeroes, with the index size in CNT_ENTRIES (only when the
-// index has entries).
+// index has entries).  done (may be NULL): an event given the end of the
+// build's last kernel.
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out = nullptr,
-                 unsigned long long* counters = nullptr);
+                 unsigned long long* counters = nullptr, hipEvent_t done = nullptr);
 // The same over every join of a batched collection at once (token ids of
 // different joins are disjoint, so one index serves all joins).
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
@@ -236,8 +237,10 @@ struct FgssArgs {
 constexpr uint32_t MAX_THRESHOLDS = 8;
 constexpr uint32_t TILE_TOKENS = 8192;  // probe tokens staged in LDS per tile
 constexpr uint32_t TILE_PROBES = 512;
-// persistent_wgs: grid size when a.tile_queue is set (0: one per tile)
-void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile, uint32_t persistent_wgs = 0);
+// persistent_wgs: grid size when a.tile_queue is set (0: one per tile).
+// t0 / t1 (may be NULL): events given the k_fgss dispatch's start and end.
+void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile, uint32_t persistent_wgs = 0,
+                 hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
 void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups);
 // Work buffers and the host-made item table of the item-slice pass.
 struct BalancedPlan {

@@ -8,6 +8,7 @@
 // parallel 16-bit array keeps each entry's position inside its set (the
 // `position` of the reference's 12-byte entry, structs.hpp:56-63), used by
 // the positional filter.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -503,7 +504,8 @@ static void radix_pass(hipStream_t st, uint32_t n, uint32_t shift, uint32_t ntil
 }
 
 static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32_t universe, IndexBuild& ib,
-                            int cur, RadixPlan plan, const uint64_t* sigs, unsigned long long* counters) {
+                            int cur, RadixPlan plan, const uint64_t* sigs, unsigned long long* counters,
+                            hipEvent_t done = nullptr) {
   const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
   ib.hist.ensure(((size_t)ntiles << plan.bits) * sizeof(uint32_t));
   uint32_t* const kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
@@ -514,9 +516,9 @@ static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32
     radix_pass<8>(st, n, shift, ntiles, ib, kbuf, vbuf, pbuf, cur);
     cur ^= 1;
   }
-  hipLaunchKernelGGL(k_index_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st, kbuf[cur], n, universe,
-                     ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(), sigs,
-                     ib.ent.as<uint32_t>(), counters);
+  hipExtLaunchKernelGGL(k_index_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st, nullptr, done, 0, kbuf[cur], n,
+                        universe, ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(),
+                        ib.positions.as<uint16_t>(), sigs, ib.ent.as<uint32_t>(), counters);
   HS_CHECK_HIP(hipGetLastError());
 }
 
@@ -526,7 +528,8 @@ static int emit_buffer(uint32_t passes) { return (passes % 2 == 1) ? 0 : 1; }
 
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                 IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters) {
+                 IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
+                 hipEvent_t done) {
   prepare(ib, universe, entries);
   ib.ent_words = sigs ? 4u : 2u;
   const uint32_t n = (uint32_t)entries;
@@ -539,9 +542,10 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
     const uint32_t nsets = last - first + 1;
     hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
                        size_first, cum, kb, vb, pb, sigs_out);
-    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs, counters);
+    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs, counters, done);
   } else {
     launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
+    if (done) HS_CHECK_HIP(hipEventRecord(done, st));
   }
   HS_CHECK_HIP(hipGetLastError());
 }

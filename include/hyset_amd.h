@@ -219,6 +219,12 @@ enum { HS_PLAN_FIXED = 1, HS_PLAN_UPLOAD = 2, HS_PLAN_QUEUE = 3, HS_PLAN_BALANCE
  * verified by a merge, as in the reference's fgssjoin).  The pair set is the
  * same either way. */
 enum { HS_TUNE_NO_SIGNATURE = 4 };
+/* hs_join: record events between the stages, so that hs_stats holds
+ * index_ms, filter_ms and kernel_ms.  Each event costs the device a few
+ * microseconds of idle time, so by default a join is timed as a whole
+ * (total_ms, and transfer_ms when pairs are copied) and the stage times
+ * are 0.  The hybrid technique always records its stages. */
+enum { HS_TUNE_STAGE_TIMES = 32 };
 
 /* Result sink.  count is always written.  With count_only == 0 the pairs
  * (r = probe id, s = indexed id, overlap = |r n s|) are copied into the
@@ -237,7 +243,9 @@ typedef struct {
 } hs_result;
 
 /* Timings (ms, HIP events on the context stream) and work counters of the
- * last join on a context. */
+ * last join on a context.  hs_join fills index_ms, filter_ms and kernel_ms
+ * with HS_TUNE_STAGE_TIMES only; total_ms runs from the first launch to the
+ * counters' (and, when copied, the pairs') arrival on the host. */
 typedef struct {
   double index_ms;
   double filter_ms; /* candidate generation + verification (fused for fgss/bitmap) */
@@ -257,7 +265,7 @@ typedef struct {
   uint32_t tile_plan;        /* fgss: HS_PLAN_* (0 for other techniques) */
   uint32_t graph;            /* HS_TUNE_GRAPH joins: 1 the sequence was captured, 2 replayed (0: eager);
                                 a replayed join is timed as a whole: index_ms and kernel_ms are 0,
-                                filter_ms covers the index build and the launch */
+                                filter_ms covers the index build, the launch and the counters */
   uint32_t reserved;
 } hs_stats;
 
