@@ -141,6 +141,11 @@ struct hs_ctx {
   PinnedBuf h_icum;
   hipEvent_t icum_ev = nullptr;  // recorded after the last copy out of h_icum
   bool icum_pending = false;
+  // the item table the device holds (d_icum): the upload is skipped while
+  // the table and the allocation epoch are unchanged
+  std::vector<uint32_t> icum_host, icum_tmp;
+  bool icum_valid = false;
+  uint64_t icum_epoch = 0;
   // captured batched join: replayed while (collection, bounds, plan) match
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

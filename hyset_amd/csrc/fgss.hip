@@ -831,9 +831,12 @@ __global__ __launch_bounds__(256) void k_probe_cost(FgssArgs A, const uint32_t* 
   cost[i] = min(c, COST_W);
 }
 
+// queue (may be NULL): the persistent grid's 8 tile counters, zeroed here.
 __global__ void k_balanced_tiles(const uint32_t* __restrict__ cum, uint32_t n, uint32_t probe_first,
-                                 uint32_t* __restrict__ tile_starts, uint32_t* __restrict__ ntiles) {
+                                 uint32_t* __restrict__ tile_starts, uint32_t* __restrict__ ntiles,
+                                 uint32_t* __restrict__ queue) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (queue && i < 8) queue[i] = 0;
   if (i >= n) return;
   const uint32_t k = cum[i] / COST_W;
   if (i == 0 || cum[i - 1] / COST_W != k) tile_starts[k] = probe_first + i;
@@ -868,7 +871,8 @@ void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp) {
   a.probe_slot = bp.slot->as<uint32_t>();
   launch_exclusive_scan(st, c, c, nullptr, n, *bp.scan_scratch);
   hipLaunchKernelGGL(k_balanced_tiles, dim3((n + 255) / 256), dim3(256), 0, st, c, n, a.probe_first,
-                     bp.tiles->as<uint32_t>(), bp.ntiles->as<uint32_t>());
+                     bp.tiles->as<uint32_t>(), bp.ntiles->as<uint32_t>(), a.tile_queue);
+  a.queue_zeroed = a.tile_queue != nullptr;
   HS_CHECK_HIP(hipGetLastError());
   a.tile_starts = bp.tiles->as<uint32_t>();
   a.first_tile = 0;
@@ -915,7 +919,7 @@ void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persi
   if (ntiles == 0) return none();
   uint32_t grid = (ntiles + 7) / 8 * 8;
   if (a.tile_queue) {
-    launch_fill_u32(st, a.tile_queue, 8, 0);
+    if (!a.queue_zeroed) launch_fill_u32(st, a.tile_queue, 8, 0);
     if (persistent_wgs) grid = std::min(grid, (persistent_wgs + 7) / 8 * 8);
   }
   const dim3 g(grid);

@@ -215,8 +215,10 @@ struct FgssArgs {
   const DevBounds* probe_bounds;
   const uint8_t* probe_tidx;
   // non-NULL: a persistent grid claims plan tiles from 8 per-XCD counters
-  // (zeroed by launch_fgss), heaviest tiles first; NULL: one workgroup per tile
+  // (zeroed by launch_fgss unless queue_zeroed: the balanced planner zeroes
+  // them on its way), heaviest tiles first; NULL: one workgroup per tile
   uint32_t* tile_queue;
+  uint32_t queue_zeroed;
   const uint32_t* dev_ntiles;  // non-NULL: the tile count lives on the device
   // item-slice pass (plain joins): per item slot the posting-list slice of
   // ids in the probe's window
