@@ -336,8 +336,8 @@ def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=2000, offered=
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--quick", action="store_true", help="headline only: no side measurements, no CPU baseline")
     ap.add_argument("--threshold", type=float, default=0.8)
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-join measurement")
@@ -368,13 +368,16 @@ def main() -> int:
 
     for _ in range(args.warmup):
         ctx.join(d, algo="fgss", count_only=True)
-    steps = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pairs = ctx.join(d, algo="fgss", count_only=True)
-        steps.append(ctx.stats())
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
+    # the device time of the same joins (read after each join, outside the timed loop)
+    steps = []
+    for _ in range(args.steps):
+        assert ctx.join(d, algo="fgss", count_only=True) == pairs
+        steps.append(ctx.stats())
     st = steps[-1]
     dev_ms = statistics.median(s["total_ms"] for s in steps)
     # The same steps again with the stage events (HS_TUNE_STAGE_TIMES): the
@@ -383,12 +386,14 @@ def main() -> int:
     # few microseconds per join, so the throughput above runs without them.
     stune = H.Tune(flags=H.TUNE_STAGE_TIMES)
     ctx.join(d, algo="fgss", tune=stune, count_only=True)
-    ssteps = []
     a = time.perf_counter()
     for _ in range(args.steps):
         assert ctx.join(d, algo="fgss", tune=stune, count_only=True) == pairs
-        ssteps.append(ctx.stats())
     ms_per_step_stages = (time.perf_counter() - a) * 1e3 / args.steps
+    ssteps = []
+    for _ in range(args.steps):
+        assert ctx.join(d, algo="fgss", tune=stune, count_only=True) == pairs
+        ssteps.append(ctx.stats())
     kern_ms = statistics.median(s["kernel_ms"] for s in ssteps)
     idx_ms = statistics.median(s["index_ms"] for s in ssteps)
     log(f"fgss t={t}: {pairs} pairs, {ms_per_step:.3f} ms/step wall, device {dev_ms:.3f} ms; with stage events "
@@ -399,12 +404,14 @@ def main() -> int:
     gtune = H.Tune(flags=H.TUNE_GRAPH)
     for _ in range(max(args.warmup, 2)):
         assert ctx.join(d, algo="fgss", tune=gtune, count_only=True) == pairs
-    gsteps = []
     a = time.perf_counter()
     for _ in range(args.steps):
         assert ctx.join(d, algo="fgss", tune=gtune, count_only=True) == pairs
-        gsteps.append(ctx.stats())
     ms_per_step_graph = (time.perf_counter() - a) * 1e3 / args.steps
+    gsteps = []
+    for _ in range(args.steps):
+        assert ctx.join(d, algo="fgss", tune=gtune, count_only=True) == pairs
+        gsteps.append(ctx.stats())
     replayed = all(g["graph"] == 2 for g in gsteps)
     dev_ms_graph = statistics.median(g["total_ms"] for g in gsteps)
     log(f"fgss t={t} graph replay: {ms_per_step_graph:.3f} ms/step wall, device {dev_ms_graph:.3f} ms "
