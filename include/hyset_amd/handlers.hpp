@@ -184,6 +184,7 @@ struct handler {
           bool aggregate, double threshold)
       : ctx_(ctx), coll_(coll), blockSize(blockSize), aggregate(aggregate), threshold(threshold) {
     tune_.bitmap_bits = bitmapSize;
+    tune_.flags = HS_TUNE_STAGE_TIMES;  // per-stage times, as the reference's device timer
     check(hs_set_threshold(ctx.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx.get(), "hs_set_threshold");
   }
   void setOutputHandler(std::shared_ptr<output::handler>& h) { output_ = h; }
@@ -211,6 +212,7 @@ namespace fgssjoin {
 struct handler {
   handler(context& ctx, collection::device_collection& coll, unsigned int blockSize, bool aggregate, double threshold)
       : ctx_(ctx), coll_(coll), blockSize(blockSize), aggregate(aggregate), threshold(threshold) {
+    tune_.flags = HS_TUNE_STAGE_TIMES;  // per-stage times, as the reference's device timer
     check(hs_set_threshold(ctx.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx.get(), "hs_set_threshold");
   }
   void setOutputHandler(std::shared_ptr<output::handler>& h) { output_ = h; }
