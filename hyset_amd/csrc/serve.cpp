@@ -47,9 +47,26 @@ struct Request {
 
 struct Slot {
   hipStream_t stream = nullptr;
-  DevBuf tokens, starts, sizes, tidx, tiles, sigs, out_r, out_s, out_ov, counters;
-  PinnedBuf h_tokens, h_starts, h_sizes, h_tidx, h_tiles, h_counters, h_r, h_s, h_ov;
+  // a batch's probe arrays in one staging region (one copy per batch), laid
+  // out by StageLayout; h_stage is its pinned host side
+  DevBuf stage, sigs, out_r, out_s, out_ov, counters;
+  PinnedBuf h_stage, h_counters, h_r, h_s, h_ov;
   uint64_t out_cap = 0;
+};
+
+// Byte offsets of a batch of nq queries and ntok tokens in the staging
+// region: starts, sizes, tile starts (at most nq + 1), threshold indices,
+// then the tokens (4-byte aligned).
+struct StageLayout {
+  size_t starts, sizes, tiles, tidx, tokens, bytes;
+  StageLayout(uint64_t nq, uint64_t ntok) {
+    starts = 0;
+    sizes = 4 * nq;
+    tiles = 8 * nq;
+    tidx = tiles + 4 * (nq + 1);
+    tokens = (tidx + nq + 3) & ~size_t(3);
+    bytes = tokens + 4 * ntok;
+  }
 };
 
 }  // namespace
@@ -131,16 +148,9 @@ void grow(B& b, size_t need) {
 // average with 4 result triples per query before it serves anything.
 void reserve_slot(const hs_server* S, Slot& sl) {
   const size_t q = std::max<uint32_t>(S->cfg.max_batch_queries, 1), tok = q * 16, out = 4 * q;
-  grow(sl.h_tokens, tok * 4);
-  grow(sl.h_starts, q * 4);
-  grow(sl.h_sizes, q * 4);
-  grow(sl.h_tidx, q);
-  grow(sl.h_tiles, (q + 1) * 4);
-  grow(sl.tokens, tok * 4);
-  grow(sl.starts, q * 4);
-  grow(sl.sizes, q * 4);
-  grow(sl.tidx, q);
-  grow(sl.tiles, (q + 1) * 4);
+  const StageLayout lay(q, tok);
+  grow(sl.h_stage, lay.bytes);
+  grow(sl.stage, lay.bytes);
   grow(sl.counters, CNT_WORDS * 8);
   grow(sl.h_counters, CNT_WORDS * 8);
   grow(sl.out_r, out * 4);
@@ -161,15 +171,14 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     ntok += r->tokens.size();
     nq += r->n;
   }
-  grow(sl.h_tokens, std::max<uint64_t>(ntok, 1) * 4);
-  grow(sl.h_starts, (size_t)std::max<uint32_t>(nq, 1) * 4);
-  grow(sl.h_sizes, (size_t)std::max<uint32_t>(nq, 1) * 4);
-  grow(sl.h_tidx, std::max<uint32_t>(nq, 1));
+  const StageLayout lay(std::max<uint32_t>(nq, 1), std::max<uint64_t>(ntok, 1));
+  grow(sl.h_stage, lay.bytes);
   std::vector<uint32_t> first(batch.size() + 1, 0);
-  uint32_t* tk = sl.h_tokens.as<uint32_t>();
-  uint32_t* st = sl.h_starts.as<uint32_t>();
-  uint32_t* sz = sl.h_sizes.as<uint32_t>();
-  uint8_t* ti = sl.h_tidx.as<uint8_t>();
+  uint8_t* hb = sl.h_stage.as<uint8_t>();
+  uint32_t* tk = reinterpret_cast<uint32_t*>(hb + lay.tokens);
+  uint32_t* st = reinterpret_cast<uint32_t*>(hb + lay.starts);
+  uint32_t* sz = reinterpret_cast<uint32_t*>(hb + lay.sizes);
+  uint8_t* ti = hb + lay.tidx;
   uint64_t pos = 0;
   uint32_t q = 0;
   for (size_t k = 0; k < batch.size(); ++k) {
@@ -202,13 +211,9 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     }
   }
   tiles.push_back(nq);
-  grow(sl.h_tiles, tiles.size() * 4);
-  std::copy(tiles.begin(), tiles.end(), sl.h_tiles.as<uint32_t>());
-  grow(sl.tokens, std::max<uint64_t>(ntok, 1) * 4);
-  grow(sl.starts, (size_t)std::max<uint32_t>(nq, 1) * 4);
-  grow(sl.sizes, (size_t)std::max<uint32_t>(nq, 1) * 4);
-  grow(sl.tidx, std::max<uint32_t>(nq, 1));
-  grow(sl.tiles, tiles.size() * 4);
+  // a tile holds at least one probe: at most nq + 1 starts, as laid out
+  std::copy(tiles.begin(), tiles.end(), reinterpret_cast<uint32_t*>(hb + lay.tiles));
+  grow(sl.stage, lay.bytes);
   grow(sl.counters, CNT_WORDS * 8);
   grow(sl.h_counters, CNT_WORDS * 8);
   if (sl.out_cap < 4ull * nq) {
@@ -220,17 +225,16 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
   }
   if (S->sig_words) grow(sl.sigs, (size_t)std::max<uint32_t>(nq, 1) * S->sig_words * 8);
   hipStream_t s = sl.stream;
-  HS_CHECK_HIP(hipMemcpyAsync(sl.tokens.ptr, tk, ntok * 4, hipMemcpyHostToDevice, s));
-  HS_CHECK_HIP(hipMemcpyAsync(sl.starts.ptr, st, (size_t)nq * 4, hipMemcpyHostToDevice, s));
-  HS_CHECK_HIP(hipMemcpyAsync(sl.sizes.ptr, sz, (size_t)nq * 4, hipMemcpyHostToDevice, s));
-  HS_CHECK_HIP(hipMemcpyAsync(sl.tidx.ptr, ti, nq, hipMemcpyHostToDevice, s));
-  HS_CHECK_HIP(hipMemcpyAsync(sl.tiles.ptr, sl.h_tiles.ptr, tiles.size() * 4, hipMemcpyHostToDevice, s));
+  // one copy for every probe array of the batch
+  HS_CHECK_HIP(hipMemcpyAsync(sl.stage.ptr, sl.h_stage.ptr, lay.tokens + ntok * 4, hipMemcpyHostToDevice, s));
+  uint8_t* db = sl.stage.as<uint8_t>();
   hs_coll* I = S->index;
   unsigned long long cnt[CNT_WORDS];
   for (int attempt = 0; attempt < 2; ++attempt) {
     launch_fill_u64(s, sl.counters.as<unsigned long long>(), CNT_WORDS, 0);
     FgssArgs a{};
-    a.probe = DevColl{sl.tokens.as<uint32_t>(), sl.starts.as<uint32_t>(), sl.sizes.as<uint32_t>(), nq};
+    a.probe = DevColl{reinterpret_cast<uint32_t*>(db + lay.tokens), reinterpret_cast<uint32_t*>(db + lay.starts),
+                      reinterpret_cast<uint32_t*>(db + lay.sizes), nq};
     a.indexed = I->dev();
     a.index = dev_index(S->ib);
     a.bounds = S->hbounds[0];
@@ -241,11 +245,11 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     a.idx_first = 0;
     a.idx_last = I->n - 1;
     a.self = 0;
-    a.tile_starts = sl.tiles.as<uint32_t>();
+    a.tile_starts = reinterpret_cast<const uint32_t*>(db + lay.tiles);
     a.first_tile = 0;
     a.n_plan_tiles = (uint32_t)tiles.size() - 1;
     a.probe_bounds = S->d_bounds.as<DevBounds>();
-    a.probe_tidx = sl.tidx.as<uint8_t>();
+    a.probe_tidx = db + lay.tidx;
     if (S->sig_words) {
       if (attempt == 0) launch_signatures(s, a.probe, S->sig_words, sl.sigs.as<uint64_t>());
       a.sig_words = S->sig_words;
