@@ -49,10 +49,37 @@ struct Slot {
   hipStream_t stream = nullptr;
   // a batch's probe arrays in one staging region (one copy per batch), laid
   // out by StageLayout; h_stage is its pinned host side
-  DevBuf stage, sigs, out_r, out_s, out_ov, counters;
-  PinnedBuf h_stage, h_counters, h_r, h_s, h_ov;
-  uint64_t out_cap = 0;
+  DevBuf stage, sigs, out, counters;
+  PinnedBuf h_stage, h_counters, h_out;
+  // out holds the result triples as three rows of out_cap ids (r, s,
+  // overlap), h_out as three rows of h_cap: one 2D copy moves a prefix of
+  // all three
+  uint64_t out_cap = 0, h_cap = 0;
 };
+
+// Grows a slot buffer to at least `need` bytes, at least doubling it: the
+// free inside a reallocation synchronizes the whole device (every slot's
+// work in flight), so it has to stay rare.
+template <class B>
+void grow(B& b, size_t need) {
+  if (need > b.bytes || !b.ptr) b.ensure(std::max(need, 2 * b.bytes));
+}
+
+// Grows the device (host) result rows of a slot to at least `cap` triples.
+void grow_out(Slot& sl, uint64_t cap) {
+  grow(sl.out, cap * 12);
+  sl.out_cap = sl.out.bytes / 12;
+}
+void grow_h_out(Slot& sl, uint64_t cap) {
+  grow(sl.h_out, cap * 12);
+  sl.h_cap = sl.h_out.bytes / 12;
+}
+// Copies triples [from, to) of the device rows into the host rows.
+void copy_triples(Slot& sl, uint64_t from, uint64_t to, hipStream_t s) {
+  if (to <= from) return;
+  HS_CHECK_HIP(hipMemcpy2DAsync(sl.h_out.as<uint32_t>() + from, sl.h_cap * 4, sl.out.as<uint32_t>() + from,
+                                sl.out_cap * 4, (to - from) * 4, 3, hipMemcpyDeviceToHost, s));
+}
 
 // Byte offsets of a batch of nq queries and ntok tokens in the staging
 // region: starts, sizes, tile starts (at most nq + 1), threshold indices,
@@ -136,14 +163,6 @@ int table_for(hs_server* S, double t) {
   return (int)S->thresholds.size() - 1;
 }
 
-// Grows a slot buffer to at least `need` bytes, at least doubling it: the
-// free inside a reallocation synchronizes the whole device (every slot's
-// work in flight), so it has to stay rare.
-template <class B>
-void grow(B& b, size_t need) {
-  if (need > b.bytes || !b.ptr) b.ensure(std::max(need, 2 * b.bytes));
-}
-
 // Sizes a slot for a batch of max_batch_queries queries of 16 tokens on
 // average with 4 result triples per query before it serves anything.
 void reserve_slot(const hs_server* S, Slot& sl) {
@@ -153,13 +172,8 @@ void reserve_slot(const hs_server* S, Slot& sl) {
   grow(sl.stage, lay.bytes);
   grow(sl.counters, CNT_WORDS * 8);
   grow(sl.h_counters, CNT_WORDS * 8);
-  grow(sl.out_r, out * 4);
-  grow(sl.out_s, out * 4);
-  grow(sl.out_ov, out * 4);
-  sl.out_cap = out;
-  grow(sl.h_r, out * 4);
-  grow(sl.h_s, out * 4);
-  grow(sl.h_ov, out * 4);
+  grow_out(sl, out);
+  grow_h_out(sl, out);
   if (S->sig_words) grow(sl.sigs, q * S->sig_words * 8);
 }
 
@@ -216,13 +230,12 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
   grow(sl.stage, lay.bytes);
   grow(sl.counters, CNT_WORDS * 8);
   grow(sl.h_counters, CNT_WORDS * 8);
-  if (sl.out_cap < 4ull * nq) {
-    const uint64_t cap = std::max<uint64_t>(1u << 16, 4ull * nq);
-    grow(sl.out_r, cap * 4);
-    grow(sl.out_s, cap * 4);
-    grow(sl.out_ov, cap * 4);
-    sl.out_cap = std::min({sl.out_r.bytes, sl.out_s.bytes, sl.out_ov.bytes}) / 4;
-  }
+  if (sl.out_cap < 4ull * nq) grow_out(sl, std::max<uint64_t>(1u << 16, 4ull * nq));
+  // result triples fetched with the counters, before their count is known
+  // (about one pair per query in the measured mixes): most batches need one
+  // round trip to the host, not two
+  const uint64_t guess = std::max<uint64_t>(nq, 1024);
+  if (sl.h_cap < guess) grow_h_out(sl, guess);
   if (S->sig_words) grow(sl.sigs, (size_t)std::max<uint32_t>(nq, 1) * S->sig_words * 8);
   hipStream_t s = sl.stream;
   // one copy for every probe array of the batch
@@ -256,32 +269,31 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
       a.probe_sigs = sl.sigs.as<uint64_t>();
       a.indexed_sigs = S->isigs.as<uint64_t>();
     }
-    a.out = DevOut{sl.out_r.as<uint32_t>(), sl.out_s.as<uint32_t>(), sl.out_ov.as<uint32_t>(),
-                   sl.counters.as<unsigned long long>(), sl.out_cap};
+    uint32_t* dout = sl.out.as<uint32_t>();
+    a.out = DevOut{dout, dout + sl.out_cap, dout + 2 * sl.out_cap, sl.counters.as<unsigned long long>(), sl.out_cap};
     launch_fgss(s, a, 0);
     HS_CHECK_HIP(hipMemcpyAsync(sl.h_counters.ptr, sl.counters.ptr, CNT_WORDS * 8, hipMemcpyDeviceToHost, s));
+    copy_triples(sl, 0, std::min(guess, sl.out_cap), s);
     HS_CHECK_HIP(hipStreamSynchronize(s));
     std::memcpy(cnt, sl.h_counters.ptr, sizeof(cnt));
     if (cnt[CNT_PAIRS] <= sl.out_cap) break;
-    const uint64_t cap = cnt[CNT_PAIRS] + cnt[CNT_PAIRS] / 4 + 1024;
-    grow(sl.out_r, cap * 4);
-    grow(sl.out_s, cap * 4);
-    grow(sl.out_ov, cap * 4);
-    sl.out_cap = std::min({sl.out_r.bytes, sl.out_s.bytes, sl.out_ov.bytes}) / 4;
+    grow_out(sl, cnt[CNT_PAIRS] + cnt[CNT_PAIRS] / 4 + 1024);
   }
   const uint64_t np = cnt[CNT_PAIRS];
-  if (np) {
-    grow(sl.h_r, np * 4);
-    grow(sl.h_s, np * 4);
-    grow(sl.h_ov, np * 4);
-    HS_CHECK_HIP(hipMemcpyAsync(sl.h_r.ptr, sl.out_r.ptr, np * 4, hipMemcpyDeviceToHost, s));
-    HS_CHECK_HIP(hipMemcpyAsync(sl.h_s.ptr, sl.out_s.ptr, np * 4, hipMemcpyDeviceToHost, s));
-    HS_CHECK_HIP(hipMemcpyAsync(sl.h_ov.ptr, sl.out_ov.ptr, np * 4, hipMemcpyDeviceToHost, s));
+  const uint64_t fetched = std::min(guess, sl.out_cap);
+  if (np > fetched) {
+    // the rest; a larger host buffer starts empty, so then every triple
+    uint64_t from = fetched;
+    if (sl.h_cap < np) {
+      grow_h_out(sl, np);
+      from = 0;
+    }
+    copy_triples(sl, from, np, s);
     HS_CHECK_HIP(hipStreamSynchronize(s));
   }
-  const uint32_t* R = sl.h_r.as<uint32_t>();
-  const uint32_t* Sd = sl.h_s.as<uint32_t>();
-  const uint32_t* O = sl.h_ov.as<uint32_t>();
+  const uint32_t* R = sl.h_out.as<uint32_t>();
+  const uint32_t* Sd = R + sl.h_cap;
+  const uint32_t* O = R + 2 * sl.h_cap;
   for (uint64_t i = 0; i < np; ++i) {
     const size_t k = std::upper_bound(first.begin(), first.end(), R[i]) - first.begin() - 1;
     Request* r = batch[k];
