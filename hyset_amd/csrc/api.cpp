@@ -90,12 +90,9 @@ void hs::stream_pairs(hs_ctx* ctx, uint64_t count, bool with_overlap, uint64_t c
   }
   auto issue = [&](uint64_t c) {
     const uint64_t off = c * chunk, n = std::min(chunk, count - off);
-    uint32_t* dst = ctx->stage[c & 1].as<uint32_t>();
-    HS_CHECK_HIP(hipMemcpyAsync(dst, ctx->out_r.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost, st));
-    HS_CHECK_HIP(hipMemcpyAsync(dst + chunk, ctx->out_s.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost, st));
-    if (with_overlap)
-      HS_CHECK_HIP(
-          hipMemcpyAsync(dst + 2 * chunk, ctx->out_ov.as<uint32_t>() + off, n * 4, hipMemcpyDeviceToHost, st));
+    // rows r, s (and overlap) of the chunk in one 2D copy
+    HS_CHECK_HIP(hipMemcpy2DAsync(ctx->stage[c & 1].ptr, chunk * 4, ctx->out.as<uint32_t>() + off, ctx->out_cap * 4,
+                                  n * 4, arrays, hipMemcpyDeviceToHost, st));
     HS_CHECK_HIP(hipEventRecord(ctx->stage_ev[c & 1], st));
   };
   try {

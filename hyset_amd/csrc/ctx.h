@@ -113,7 +113,9 @@ struct hs_ctx {
   DevBuf d_tables;
   hs::DevBounds dev_bounds{};
   hs::IndexBuild ib;
-  DevBuf out_r, out_s, out_ov, counters;
+  // the result pairs as three rows of out_cap ids (r, s, overlap) in one
+  // allocation, so one 2D copy moves a range of all three
+  DevBuf out, counters;
   uint64_t out_cap = 0;
   PinnedBuf h_counters, h_cum;
   DevBuf d_cum;
@@ -251,16 +253,15 @@ inline void ensure_bounds(hs_ctx* ctx, uint32_t L) {
 }
 
 inline void ensure_output(hs_ctx* ctx, uint64_t cap) {
-  if (cap <= ctx->out_cap && ctx->out_r.ptr) return;
-  ctx->out_r.ensure(cap * sizeof(uint32_t));
-  ctx->out_s.ensure(cap * sizeof(uint32_t));
-  ctx->out_ov.ensure(cap * sizeof(uint32_t));
+  if (cap <= ctx->out_cap && ctx->out.ptr) return;
+  ctx->out.ensure(cap * 3 * sizeof(uint32_t));
   ctx->out_cap = cap;
 }
 
 inline hs::DevOut dev_out(hs_ctx* ctx) {
-  return hs::DevOut{ctx->out_r.as<uint32_t>(), ctx->out_s.as<uint32_t>(), ctx->out_ov.as<uint32_t>(),
-                    ctx->counters.as<unsigned long long>(), ctx->out_cap};
+  uint32_t* o = ctx->out.as<uint32_t>();
+  return hs::DevOut{o, o + ctx->out_cap, o + 2 * ctx->out_cap, ctx->counters.as<unsigned long long>(),
+                    ctx->out_cap};
 }
 
 inline void reset_counters(hs_ctx* ctx) {
