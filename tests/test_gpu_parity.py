@@ -359,3 +359,41 @@ def test_degenerate_inputs(ctx, algo):
     finally:
         df.free()
         dn.free()
+
+
+@pytest.mark.parametrize("algo", ["fgss", "bitmap", "hybrid"])
+def test_stage_times(ctx, small, algo):
+    """A plain hs_join is timed as a whole (total_ms; the stage times stay 0,
+    no marker events between kernels); HS_TUNE_STAGE_TIMES fills index_ms,
+    filter_ms and kernel_ms (fgss: from the dispatches' own events) within
+    total_ms; the hybrid technique always records its stages.  The pairs
+    are the oracle's either way, and a graph replay is timed as a whole."""
+    c, d = small
+    ctx.set_threshold(0.7)
+    want = O.self_join(c, 0.7)
+    plain = H.Tune(flags=0)
+    staged = H.Tune(flags=H.TUNE_STAGE_TIMES)
+    _check(ctx.join(d, algo=algo, tune=plain), want, f"{algo} plain")
+    st = ctx.stats()
+    assert st["total_ms"] > 0
+    if algo == "hybrid":
+        assert st["filter_ms"] > 0
+    else:
+        assert st["index_ms"] == 0 and st["kernel_ms"] == 0 and st["filter_ms"] == 0
+    _check(ctx.join(d, algo=algo, tune=staged), want, f"{algo} staged")
+    st = ctx.stats()
+    assert 0 < st["kernel_ms"] <= st["total_ms"]
+    assert 0 < st["filter_ms"] <= st["total_ms"]
+    if algo == "fgss":
+        assert 0 < st["index_ms"] <= st["total_ms"]
+        assert st["kernel_ms"] <= st["filter_ms"]
+    # count-only joins after a copying one keep the same timing rules
+    assert ctx.join(d, algo=algo, tune=staged, count_only=True) == want.count
+    assert ctx.stats()["kernel_ms"] > 0
+    if algo != "hybrid":
+        g = H.Tune(flags=H.TUNE_GRAPH)
+        for _ in range(3):
+            assert ctx.join(d, algo=algo, tune=g, count_only=True) == want.count
+        st = ctx.stats()
+        assert st["graph"] == 2
+        assert st["index_ms"] == 0 and st["kernel_ms"] == 0 and 0 < st["filter_ms"] <= st["total_ms"]
