@@ -58,7 +58,7 @@ def test_reference_differs_from_exact_exactly_at_mismatch_sums(t):
                 assert ref == ex
 
 
-def test_survey_example_19_tokens():
+def test_example_19_tokens_at_0_9():
     """Two 19-token sets sharing 18 tokens: J = 0.9 exactly; the reference's
     double bound demands 19, the exact bound 18."""
     assert O.bounds("jaccard", 0.9, 19, 19)[2] == 19
