@@ -115,12 +115,25 @@ __global__ __launch_bounds__(SC_TPB) void k_scan_sums(uint32_t* sums, uint32_t n
 }
 
 // in, out and out2 may alias (in-place scans): each tile is read into LDS
-// before any of it is written.
+// before any of it is written.  With sums_scanned == 0 the tile sums are
+// not scanned yet and each workgroup adds up those before its own (at most
+// SC_TILE of them: one coalesced read each), which saves the one-workgroup
+// scan launch for scans of up to SC_TILE tiles.
 __global__ __launch_bounds__(SC_TPB) void k_scan_apply(const uint32_t* in, size_t n, const uint32_t* __restrict__ sums,
-                                                       uint32_t* out, uint32_t* out2) {
+                                                       uint32_t* out, uint32_t* out2, int sums_scanned) {
   __shared__ uint32_t tile[SC_TILE];
   __shared__ uint32_t tmp[SC_TPB / WAVE + 1];
   size_t base = (size_t)blockIdx.x * SC_TILE;
+  uint32_t offset;
+  if (sums_scanned) {
+    offset = sums[blockIdx.x];
+  } else {
+    uint32_t acc = 0;
+    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += SC_TPB) acc += sums[j];
+    uint32_t all;
+    (void)block_exclusive_scan<SC_TPB>(acc, tmp, &all);
+    offset = all;
+  }
   // coalesced load into LDS
 #pragma unroll
   for (int k = 0; k < SC_IPT; ++k) {
@@ -136,7 +149,7 @@ __global__ __launch_bounds__(SC_TPB) void k_scan_apply(const uint32_t* in, size_
     local += v[k];
   }
   uint32_t total;
-  uint32_t run = sums[blockIdx.x] + block_exclusive_scan<SC_TPB>(local, tmp, &total);
+  uint32_t run = offset + block_exclusive_scan<SC_TPB>(local, tmp, &total);
 #pragma unroll
   for (int k = 0; k < SC_IPT; ++k) {
     tile[threadIdx.x * SC_IPT + k] = run;
@@ -162,8 +175,9 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
   scratch.ensure(nb * sizeof(uint32_t));
   uint32_t* sums = scratch.as<uint32_t>();
   hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nb), dim3(SC_TPB), 0, st, in, n, sums);
-  hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_TPB), 0, st, sums, (uint32_t)nb);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SC_TPB), 0, st, in, n, sums, out, out2);
+  const int scanned = nb > (size_t)SC_TILE;
+  if (scanned) hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_TPB), 0, st, sums, (uint32_t)nb);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SC_TPB), 0, st, in, n, sums, out, out2, scanned);
   HS_CHECK_HIP(hipGetLastError());
 }
 
