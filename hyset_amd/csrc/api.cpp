@@ -989,6 +989,7 @@ int hs_verify_candidates(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const ui
       HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
       read_counters(ctx, cnt);
       if (cnt[hs::CNT_PAIRS] <= ctx->out_cap) break;
+      HS_REQUIRE(attempt == 0, HS_E_DEVICE, "result count changed between runs");
       ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
     }
     sl.used = false;

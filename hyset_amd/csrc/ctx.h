@@ -252,10 +252,13 @@ inline void ensure_bounds(hs_ctx* ctx, uint32_t L) {
   ctx->tables_valid = true;
 }
 
+// The row length follows the allocation (its bytes / 12), so the positions
+// of the s and overlap rows move only when the buffer is reallocated, which
+// bumps the allocation epoch that guards graph replay.
 inline void ensure_output(hs_ctx* ctx, uint64_t cap) {
   if (cap <= ctx->out_cap && ctx->out.ptr) return;
   ctx->out.ensure(cap * 3 * sizeof(uint32_t));
-  ctx->out_cap = cap;
+  ctx->out_cap = ctx->out.bytes / (3 * sizeof(uint32_t));
 }
 
 inline hs::DevOut dev_out(hs_ctx* ctx) {

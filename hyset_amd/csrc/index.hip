@@ -60,8 +60,9 @@ void launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes) {
 }
 
 // ------------------------------------------------------------------- scan
-// Three phases: per-tile sums, one-workgroup scan of the tile sums, per-tile
-// scan with the tile offset.  A tile is 256 threads x 16 elements.
+// Two launches: per-tile sums, then per-tile scans, each workgroup adding
+// up the sums of the tiles before its own.  A tile is 256 threads x 16
+// elements; at most 4096 tiles (2^24 elements).
 
 constexpr int SC_TPB = 256;
 constexpr int SC_IPT = 16;
@@ -87,53 +88,19 @@ __global__ __launch_bounds__(SC_TPB) void k_scan_tile_sums(const uint32_t* __res
   }
 }
 
-// Exclusive scan of `sums[0..nb)` in place by a single workgroup.
-__global__ __launch_bounds__(SC_TPB) void k_scan_sums(uint32_t* sums, uint32_t nb) {
-  __shared__ uint32_t tmp[SC_TPB / WAVE + 1];
-  uint32_t carry = 0;
-  for (uint32_t base = 0; base < nb; base += SC_TPB * SC_IPT) {
-    uint32_t v[SC_IPT];
-    uint32_t local = 0;
-#pragma unroll
-    for (int k = 0; k < SC_IPT; ++k) {
-      uint32_t i = base + threadIdx.x * SC_IPT + k;
-      v[k] = i < nb ? sums[i] : 0;
-      local += v[k];
-    }
-    uint32_t total;
-    uint32_t pre = block_exclusive_scan<SC_TPB>(local, tmp, &total);
-    uint32_t run = carry + pre;
-#pragma unroll
-    for (int k = 0; k < SC_IPT; ++k) {
-      uint32_t i = base + threadIdx.x * SC_IPT + k;
-      if (i < nb) sums[i] = run;
-      run += v[k];
-    }
-    carry += total;
-    __syncthreads();
-  }
-}
-
 // in, out and out2 may alias (in-place scans): each tile is read into LDS
-// before any of it is written.  With sums_scanned == 0 the tile sums are
-// not scanned yet and each workgroup adds up those before its own (at most
-// SC_TILE of them: one coalesced read each), which saves the one-workgroup
-// scan launch for scans of up to SC_TILE tiles.
+// before any of it is written.  The tile sums are not scanned: each
+// workgroup adds up those before its own (at most SC_TILE of them, one
+// coalesced read each), so a scan is two launches for up to SC_TILE tiles.
 __global__ __launch_bounds__(SC_TPB) void k_scan_apply(const uint32_t* in, size_t n, const uint32_t* __restrict__ sums,
-                                                       uint32_t* out, uint32_t* out2, int sums_scanned) {
+                                                       uint32_t* out, uint32_t* out2) {
   __shared__ uint32_t tile[SC_TILE];
   __shared__ uint32_t tmp[SC_TPB / WAVE + 1];
   size_t base = (size_t)blockIdx.x * SC_TILE;
+  uint32_t acc = 0;
+  for (uint32_t j = threadIdx.x; j < blockIdx.x; j += SC_TPB) acc += sums[j];
   uint32_t offset;
-  if (sums_scanned) {
-    offset = sums[blockIdx.x];
-  } else {
-    uint32_t acc = 0;
-    for (uint32_t j = threadIdx.x; j < blockIdx.x; j += SC_TPB) acc += sums[j];
-    uint32_t all;
-    (void)block_exclusive_scan<SC_TPB>(acc, tmp, &all);
-    offset = all;
-  }
+  (void)block_exclusive_scan<SC_TPB>(acc, tmp, &offset);
   // coalesced load into LDS
 #pragma unroll
   for (int k = 0; k < SC_IPT; ++k) {
@@ -171,13 +138,11 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
                            DevBuf& scratch) {
   if (n == 0) return;
   size_t nb = (n + SC_TILE - 1) / SC_TILE;
-  HS_REQUIRE(nb < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "scan too large");
+  HS_REQUIRE(nb <= (size_t)SC_TILE, HS_E_UNSUPPORTED, "scan limited to 2^24 elements");
   scratch.ensure(nb * sizeof(uint32_t));
   uint32_t* sums = scratch.as<uint32_t>();
   hipLaunchKernelGGL(k_scan_tile_sums, dim3((unsigned)nb), dim3(SC_TPB), 0, st, in, n, sums);
-  const int scanned = nb > (size_t)SC_TILE;
-  if (scanned) hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(SC_TPB), 0, st, sums, (uint32_t)nb);
-  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SC_TPB), 0, st, in, n, sums, out, out2, scanned);
+  hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(SC_TPB), 0, st, in, n, sums, out, out2);
   HS_CHECK_HIP(hipGetLastError());
 }
 

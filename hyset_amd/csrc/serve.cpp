@@ -277,6 +277,7 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     HS_CHECK_HIP(hipStreamSynchronize(s));
     std::memcpy(cnt, sl.h_counters.ptr, sizeof(cnt));
     if (cnt[CNT_PAIRS] <= sl.out_cap) break;
+    HS_REQUIRE(attempt == 0, HS_E_DEVICE, "result count changed between runs");
     grow_out(sl, cnt[CNT_PAIRS] + cnt[CNT_PAIRS] / 4 + 1024);
   }
   const uint64_t np = cnt[CNT_PAIRS];

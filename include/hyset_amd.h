@@ -180,13 +180,14 @@ typedef struct {
 /* Tuning knobs; zero selects the default. */
 typedef struct {
   uint32_t bitmap_bits;       /* bitmap: signature width 64, 128 or 256 (default 64) */
-  uint32_t probes_per_tile;   /* fgss: probes per workgroup, 1..256 (default 256) */
+  uint32_t probes_per_tile;   /* fgss: fixed tiles of this many probes, 1..512 (default 0: the
+                                 collection's tile plan); batched fgss: probes per tile (default 512) */
   uint32_t verify_scenario;   /* hybrid: 1 lane per candidate, 2 wave merge-path per
                                  candidate, 0 pick by set size (default 0) */
   uint32_t host_threads;      /* hybrid: CPU filter threads (default 1) */
   uint64_t batch_candidates;  /* hybrid: candidates per device batch (default 1<<22) */
   uint32_t flags;             /* HS_TUNE_* bits */
-  uint32_t tile_tokens;       /* batched fgss: probe tokens per tile, <= 4096 (default 4096) */
+  uint32_t tile_tokens;       /* batched fgss: probe tokens per tile, <= 8192 (default 8192) */
   uint32_t workgroups_per_cu; /* fgss, batched fgss: persistent workgroups per CU (default: as
                                  many as a CU holds, 2 of 512 lanes or 4 of 256) */
   uint32_t signature_bits;    /* fgss: width of the parity-signature test that runs between the

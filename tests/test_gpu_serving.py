@@ -108,18 +108,21 @@ def test_threshold_below_minimum_rejected(served):
             srv.submit(requests(q, 1, 1)[0][0], 0.6)
 
 
-def test_result_larger_than_first_fetch(ctx):
+@pytest.mark.parametrize("max_batch_queries", [65536, 2])
+def test_result_larger_than_first_fetch(ctx, max_batch_queries):
     """A batch's result rows come back with its counters up to as many
     triples as the batch has queries (at least 1024); a batch with more
     pairs than that fetches the rest in a second copy.  Two queries that
     equal 3000 indexed sets give 6000 pairs: the second copy runs and the
-    pairs are still the oracle's."""
+    pairs are still the oracle's.  With a 2-query batch limit the slots are
+    sized for 8 result triples, so the device rows regrow (and the batch
+    runs again) and the host rows regrow and take every triple anew."""
     base = list(range(1, 11))
     sets = [base] * 3000 + [list(range(20, 20 + k)) for k in range(2, 40)]
     idx = H.SetCollection.from_sets(sorted(sets, key=lambda s: (len(s), s)))
     d = ctx.upload(idx)
     qc = H.SetCollection.from_sets([base, base])
-    with H.Server(d, slots=2) as srv:
+    with H.Server(d, slots=2, max_batch_queries=max_batch_queries) as srv:
         for t in (0.8, 0.6):
             r, s, ov, _ = srv.wait_pairs(srv.submit(qc, t))
             got = triples(r, s, ov)
