@@ -160,6 +160,9 @@ _sig("hs_upload_batch", C.c_int, _vp, C.c_uint32, _u32pp, _u32pp, _u32pp, _u32p,
 _sig("hs_join_batch", C.c_int, _vp, _vp, C.POINTER(Tune), C.POINTER(Result))
 _sig("hs_autotune_batch", C.c_int, _vp, _vp, C.c_int32, C.POINTER(Tune), C.POINTER(C.c_double))
 _sig("hs_get_stats", C.c_int, _vp, C.POINTER(Stats))
+_sig("hs_join_noalloc", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.POINTER(Tune), _vp, _vp, _vp,
+     C.c_uint64, C.POINTER(C.c_uint64))
+NOALLOC_SCRATCH_BYTES = 128 << 20
 _sig("hs_copy_bandwidth", C.c_int, _vp, C.c_uint64, C.c_int32, C.POINTER(C.c_double))
 _sig("hs_popcount_rate", C.c_int, _vp, C.c_int32, C.POINTER(C.c_double))
 _sig("hs_server_create", C.c_int, C.c_int, _vp, C.c_int32, C.c_int32, C.POINTER(ServerConfig), C.POINTER(_vp))
@@ -407,6 +410,27 @@ class Context:
         if count_only:
             return int(res.count)
         return self.last_result(int(res.count))
+
+    def join_noalloc(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, r: int, s: int,
+                     overlap: int, capacity: int, indexed_range=None, probe_range=None,
+                     tune: Tune | None = None) -> int:
+        """hs_join_noalloc: the fgss join into caller-owned DEVICE arrays (r,
+        s, overlap are device addresses of `capacity` uint32 each), with no
+        allocation by the library; returns the pair count.  Too many pairs
+        raise HysetError(HS_E_OVERFLOW) whose .count is the number needed."""
+        probe = indexed if probe is None else probe
+        ir = Range(*indexed_range) if indexed_range is not None else None
+        pr = Range(*probe_range) if probe_range is not None else None
+        n = C.c_uint64()
+        rc = _lib.hs_join_noalloc(self._h, indexed._h, probe._h, C.byref(ir) if ir else None,
+                                  C.byref(pr) if pr else None, C.byref(tune) if tune else None, r, s, overlap,
+                                  int(capacity), C.byref(n))
+        if rc == HS_E_OVERFLOW:
+            e = HysetError(rc, f"hs_join_noalloc: {n.value} pairs, capacity {capacity}")
+            e.count = int(n.value)
+            raise e
+        self._check(rc, "hs_join_noalloc")
+        return int(n.value)
 
     def join_coprocess(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, mode="queue",
                        algo="fgss", cpu_threads=0, cpu_block=0, device_min_claim=0, cpu_share=0.0,

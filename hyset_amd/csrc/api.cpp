@@ -367,6 +367,9 @@ int hs_create(int device, hs_ctx** out) {
     ctx->counters.ensure(hs::CNT_WORDS * sizeof(unsigned long long));
     ctx->h_counters.ensure(hs::CNT_WORDS * sizeof(unsigned long long));
     ensure_output(ctx, 1u << 20);
+    // the allocation-free join's scratch address on this device (once per
+    // process; if the runtime refuses, hs_join_noalloc reports it)
+    (void)hs::noalloc_init(device);
   });
   if (rc != HS_OK) {
     hs_destroy(ctx);

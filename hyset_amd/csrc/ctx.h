@@ -174,6 +174,11 @@ struct hs_ctx {
   bool capturing_join = false;  // prepare_device_join stages through jgraph_cum
   bool stage_times = false;     // this join records events between its stages (HS_TUNE_STAGE_TIMES)
   uint64_t graph_epoch = 0;  // buffer epoch the capture saw
+  // hs_join_noalloc: its host bound tables and the staging copy of tables
+  // and entry table (host memory only; the device side is the scratch array)
+  hs::BoundTables na_tables;
+  bool na_tables_valid = false;
+  std::vector<uint32_t> na_stage;
 };
 
 

@@ -154,6 +154,24 @@ struct IndexBuild {
   uint32_t universe = 0;
   uint64_t entries = 0;
 };
+// The same buffers as raw device pointers: the index (offsets, set_ids,
+// positions, ent) and the radix sort's work space (keys, vals, pos, the
+// per-tile digit histograms and the digit totals).
+struct IndexBufs {
+  uint32_t* offsets;
+  uint32_t* set_ids;
+  uint16_t* positions;
+  uint32_t* ent;
+  uint32_t* keys[2];
+  uint32_t* vals;
+  uint16_t* pos;
+  uint32_t* hist;
+  uint32_t* totals;
+};
+// Bytes an index of `entries` entries over `universe` tokens occupies laid
+// out in one region (each buffer 256-byte aligned); with base != NULL the
+// pointers into the region starting at base are stored in *out.
+size_t index_region(uint32_t universe, uint64_t entries, uint8_t* base, IndexBufs* out);
 // Builds the inverted index over the prefix entries of sets [first, last]
 // of `c` (sets sorted by size).  prefix_len is indexed by set size;
 // size_first[l] is the first set id of size >= l; cum[l] is the number of
@@ -169,6 +187,11 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out = nullptr,
                  unsigned long long* counters = nullptr, hipEvent_t done = nullptr);
+// The same into buffers the caller laid out (index_region); allocates nothing.
+void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
+                      const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
+                      const IndexBufs& b, const uint64_t* sigs, uint64_t* sigs_out,
+                      unsigned long long* counters, hipEvent_t done = nullptr);
 // The same over every join of a batched collection at once (token ids of
 // different joins are disjoint, so one index serves all joins).
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
@@ -294,5 +317,9 @@ struct VerifyArgs {
   DevOut out;
 };
 void launch_verify(hipStream_t st, const VerifyArgs& a, int scenario);
+
+// hs_join_noalloc's scratch array: takes its device address on `device`
+// (the current device) once per process; false when the runtime refused.
+bool noalloc_init(int device);
 
 }  // namespace hs

@@ -438,21 +438,6 @@ __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, ui
   }
 }
 
-static void prepare(IndexBuild& ib, uint32_t universe, uint64_t entries) {
-  HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
-  ib.universe = universe;
-  ib.entries = entries;
-  const size_t cap = entries ? entries : 1;
-  ib.offsets.ensure(((size_t)universe + 1) * sizeof(uint32_t));
-  ib.set_ids.ensure((cap + IDS_PAD) * sizeof(uint32_t));  // the window search reads whole quads
-  ib.positions.ensure(cap * sizeof(uint16_t));
-  ib.ent.ensure(cap * 4 * sizeof(uint32_t));
-  ib.keys[0].ensure(cap * sizeof(uint32_t));
-  ib.keys[1].ensure(cap * sizeof(uint32_t));
-  ib.vals.ensure(cap * sizeof(uint32_t));
-  ib.pos.ensure(cap * sizeof(uint16_t));
-}
-
 // Digit plan of the LSD sort for keys below `universe`: 8-bit digits, as
 // many passes as the key bits need.  (Wider digits halve the passes but
 // leave ~4 entries per digit and tile, and the scattered stores then cost
@@ -466,38 +451,81 @@ static RadixPlan radix_plan(uint32_t universe) {
   while (need < 32 && ((uint64_t)(universe - 1) >> need)) ++need;
   return RadixPlan{(need + 7) / 8, 8u};
 }
+constexpr uint32_t RS_DIGITS = 256;
 
-// Sorts the emitted entries (in buffer `cur`) by token and writes the index.
-template <int kBits>
-static void radix_pass(hipStream_t st, uint32_t n, uint32_t shift, uint32_t ntiles, IndexBuild& ib,
-                       uint32_t* const kbuf[2], uint32_t* const vbuf[2], uint16_t* const pbuf[2], int cur) {
-  constexpr uint32_t NB = 1u << kBits;
-  hipLaunchKernelGGL((k_radix_hist<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift,
-                     ib.hist.as<uint32_t>(), ntiles);
-  ib.scan_scratch.ensure(NB * sizeof(uint32_t));
-  hipLaunchKernelGGL(k_radix_rows, dim3(NB), dim3(RS_TPB), 0, st, ib.hist.as<uint32_t>(), ntiles,
-                     ib.scan_scratch.as<uint32_t>());
-  hipLaunchKernelGGL((k_radix_scatter<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], vbuf[cur], pbuf[cur], n,
-                     shift, ib.hist.as<uint32_t>(), ib.scan_scratch.as<uint32_t>(), ntiles, kbuf[cur ^ 1],
-                     vbuf[cur ^ 1], pbuf[cur ^ 1]);
+static uint32_t radix_tiles(uint64_t entries) { return (uint32_t)((entries + RS_TILE - 1) / RS_TILE); }
+
+size_t index_region(uint32_t universe, uint64_t entries, uint8_t* base, IndexBufs* out) {
+  const size_t cap = entries ? entries : 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t at = off;
+    off = (off + bytes + 255) & ~size_t(255);
+    return base ? base + at : nullptr;
+  };
+  IndexBufs b{};
+  b.offsets = reinterpret_cast<uint32_t*>(take(((size_t)universe + 1) * 4));
+  b.set_ids = reinterpret_cast<uint32_t*>(take((cap + IDS_PAD) * 4));  // the window search reads whole quads
+  b.positions = reinterpret_cast<uint16_t*>(take(cap * 2));
+  b.ent = reinterpret_cast<uint32_t*>(take(cap * 16));
+  b.keys[0] = reinterpret_cast<uint32_t*>(take(cap * 4));
+  b.keys[1] = reinterpret_cast<uint32_t*>(take(cap * 4));
+  b.vals = reinterpret_cast<uint32_t*>(take(cap * 4));
+  b.pos = reinterpret_cast<uint16_t*>(take(cap * 2));
+  b.hist = reinterpret_cast<uint32_t*>(take((size_t)std::max<uint32_t>(radix_tiles(entries), 1) * RS_DIGITS * 4));
+  b.totals = reinterpret_cast<uint32_t*>(take(RS_DIGITS * 4));
+  if (out) *out = b;
+  return off;
 }
 
-static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32_t universe, IndexBuild& ib,
+static IndexBufs prepare(IndexBuild& ib, uint32_t universe, uint64_t entries) {
+  HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
+  ib.universe = universe;
+  ib.entries = entries;
+  const size_t cap = entries ? entries : 1;
+  ib.offsets.ensure(((size_t)universe + 1) * sizeof(uint32_t));
+  ib.set_ids.ensure((cap + IDS_PAD) * sizeof(uint32_t));  // the window search reads whole quads
+  ib.positions.ensure(cap * sizeof(uint16_t));
+  ib.ent.ensure(cap * 4 * sizeof(uint32_t));
+  ib.keys[0].ensure(cap * sizeof(uint32_t));
+  ib.keys[1].ensure(cap * sizeof(uint32_t));
+  ib.vals.ensure(cap * sizeof(uint32_t));
+  ib.pos.ensure(cap * sizeof(uint16_t));
+  ib.hist.ensure((size_t)std::max<uint32_t>(radix_tiles(entries), 1) * RS_DIGITS * sizeof(uint32_t));
+  ib.scan_scratch.ensure(RS_DIGITS * sizeof(uint32_t));
+  return IndexBufs{ib.offsets.as<uint32_t>(), ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(),
+                   ib.ent.as<uint32_t>(), {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()},
+                   ib.vals.as<uint32_t>(), ib.pos.as<uint16_t>(), ib.hist.as<uint32_t>(),
+                   ib.scan_scratch.as<uint32_t>()};
+}
+
+// One LSD pass over the entries in buffer `cur`: histograms, row scans, scatter.
+template <int kBits>
+static void radix_pass(hipStream_t st, uint32_t n, uint32_t shift, uint32_t ntiles, const IndexBufs& b,
+                       uint32_t* const kbuf[2], uint32_t* const vbuf[2], uint16_t* const pbuf[2], int cur) {
+  constexpr uint32_t NB = 1u << kBits;
+  static_assert(NB <= RS_DIGITS, "digit totals");
+  hipLaunchKernelGGL((k_radix_hist<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift, b.hist, ntiles);
+  hipLaunchKernelGGL(k_radix_rows, dim3(NB), dim3(RS_TPB), 0, st, b.hist, ntiles, b.totals);
+  hipLaunchKernelGGL((k_radix_scatter<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], vbuf[cur], pbuf[cur], n,
+                     shift, b.hist, b.totals, ntiles, kbuf[cur ^ 1], vbuf[cur ^ 1], pbuf[cur ^ 1]);
+}
+
+// Sorts the emitted entries (in buffer `cur`) by token and writes the index.
+static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32_t universe, const IndexBufs& b,
                             int cur, RadixPlan plan, const uint64_t* sigs, unsigned long long* counters,
                             hipEvent_t done = nullptr) {
-  const uint32_t ntiles = (n + RS_TILE - 1) / RS_TILE;
-  ib.hist.ensure(((size_t)ntiles << plan.bits) * sizeof(uint32_t));
-  uint32_t* const kbuf[2] = {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()};
-  uint32_t* const vbuf[2] = {ib.vals.as<uint32_t>(), ib.set_ids.as<uint32_t>()};
-  uint16_t* const pbuf[2] = {ib.pos.as<uint16_t>(), ib.positions.as<uint16_t>()};
+  const uint32_t ntiles = radix_tiles(n);
+  uint32_t* const kbuf[2] = {b.keys[0], b.keys[1]};
+  uint32_t* const vbuf[2] = {b.vals, b.set_ids};
+  uint16_t* const pbuf[2] = {b.pos, b.positions};
   for (uint32_t p = 0; p < plan.passes; ++p) {
     const uint32_t shift = plan.bits * p;
-    radix_pass<8>(st, n, shift, ntiles, ib, kbuf, vbuf, pbuf, cur);
+    radix_pass<8>(st, n, shift, ntiles, b, kbuf, vbuf, pbuf, cur);
     cur ^= 1;
   }
   hipExtLaunchKernelGGL(k_index_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st, nullptr, done, 0, kbuf[cur], n,
-                        universe, ib.offsets.as<uint32_t>(), c, ib.set_ids.as<uint32_t>(),
-                        ib.positions.as<uint16_t>(), sigs, ib.ent.as<uint32_t>(), counters);
+                        universe, b.offsets, c, b.set_ids, b.positions, sigs, b.ent, counters);
   HS_CHECK_HIP(hipGetLastError());
 }
 
@@ -505,28 +533,37 @@ static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32
 // (keys[1], set_ids, positions): buffer 0 for an odd number of passes.
 static int emit_buffer(uint32_t passes) { return (passes % 2 == 1) ? 0 : 1; }
 
-void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
-                 const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                 IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
-                 hipEvent_t done) {
-  prepare(ib, universe, entries);
-  ib.ent_words = sigs ? 4u : 2u;
+void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
+                      const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
+                      const IndexBufs& b, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
+                      hipEvent_t done) {
+  HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && first <= last && n > 0) {
     const RadixPlan plan = radix_plan(universe);
     const int cur = emit_buffer(plan.passes);
-    uint32_t* kb = cur ? ib.keys[1].as<uint32_t>() : ib.keys[0].as<uint32_t>();
-    uint32_t* vb = cur ? ib.set_ids.as<uint32_t>() : ib.vals.as<uint32_t>();
-    uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
+    uint32_t* kb = cur ? b.keys[1] : b.keys[0];
+    uint32_t* vb = cur ? b.set_ids : b.vals;
+    uint16_t* pb = cur ? b.positions : b.pos;
     const uint32_t nsets = last - first + 1;
     hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
                        size_first, cum, kb, vb, pb, sigs_out);
-    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs, counters, done);
+    sort_and_finish(st, c, n, universe, b, cur, plan, sigs, counters, done);
   } else {
-    launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
+    launch_fill_u32(st, b.offsets, (size_t)universe + 1, 0);
     if (done) HS_CHECK_HIP(hipEventRecord(done, st));
   }
   HS_CHECK_HIP(hipGetLastError());
+}
+
+void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
+                 const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
+                 IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
+                 hipEvent_t done) {
+  const IndexBufs b = prepare(ib, universe, entries);
+  ib.ent_words = sigs ? 4u : 2u;
+  build_index_into(st, c, first, last, prefix_len, size_first, cum, universe, entries, b, sigs, sigs_out, counters,
+                   done);
 }
 
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
@@ -534,20 +571,20 @@ void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_fi
                        const uint32_t* cum, uint32_t universe, uint64_t entries, IndexBuild& ib,
                        const uint64_t* sigs, uint64_t* sigs_out) {
   unsigned long long* counters = nullptr;
-  prepare(ib, universe, entries);
+  const IndexBufs b = prepare(ib, universe, entries);
   ib.ent_words = sigs ? 4u : 2u;
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && n > 0) {
     const RadixPlan plan = radix_plan(universe);
     const int cur = emit_buffer(plan.passes);
-    uint32_t* kb = cur ? ib.keys[1].as<uint32_t>() : ib.keys[0].as<uint32_t>();
-    uint32_t* vb = cur ? ib.set_ids.as<uint32_t>() : ib.vals.as<uint32_t>();
-    uint16_t* pb = cur ? ib.positions.as<uint16_t>() : ib.pos.as<uint16_t>();
+    uint32_t* kb = cur ? b.keys[1] : b.keys[0];
+    uint32_t* vb = cur ? b.set_ids : b.vals;
+    uint16_t* pb = cur ? b.positions : b.pos;
     hipLaunchKernelGGL(k_index_emit_batch, dim3((c.n + 255) / 256), dim3(256), 0, st, c, join_first, n_joins,
                        prefix_len, size_first, sf_base, cum, kb, vb, pb, sigs_out);
-    sort_and_finish(st, c, n, universe, ib, cur, plan, sigs, counters);
+    sort_and_finish(st, c, n, universe, b, cur, plan, sigs, counters);
   } else {
-    launch_fill_u32(st, ib.offsets.as<uint32_t>(), (size_t)universe + 1, 0);
+    launch_fill_u32(st, b.offsets, (size_t)universe + 1, 0);
   }
   HS_CHECK_HIP(hipGetLastError());
 }

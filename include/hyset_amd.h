@@ -506,6 +506,33 @@ int hs_write_last_result_text(hs_ctx* ctx, const char* path, int32_t with_overla
 
 int hs_get_stats(const hs_ctx* ctx, hs_stats* out);
 
+/* ------------------------------------------------- allocation-free join */
+
+/* hs_join with the fgss technique that allocates nothing: no hipMalloc, no
+ * hipFree, no pinned or library allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  Inputs are collections
+ * already uploaded (hs_upload_collection) and the context's threshold; the
+ * result triples are written into the caller's DEVICE arrays r, s and
+ * overlap (capacity entries each) and *count receives the number of pairs.
+ * The bound tables, the signatures, the inverted index with its sort work
+ * space and the work counters live in one device array of
+ * HS_NOALLOC_SCRATCH_BYTES per device, fixed for the life of the process
+ * (its address is taken in hs_create).  A join that needs more than that
+ * returns HS_E_UNSUPPORTED (the 1M-set self-join at Jaccard 0.8 needs about
+ * 80 MB); more pairs than capacity return HS_E_OVERFLOW with *count set, the
+ * arrays then holding an unspecified `capacity` of them.  Tiles are the
+ * probe collection's upload-time plan (tune->probes_per_tile: fixed tiles);
+ * tune->signature_bits and HS_TUNE_NO_SIGNATURE apply, the other knobs are
+ * ignored.  The call synchronizes the context stream to read the count.
+ * Every call on one device shares the scratch array: calls that overlap on
+ * one device (from two contexts) race.  The context's last result
+ * (hs_copy_last_result and friends) is left as it was. */
+#define HS_NOALLOC_SCRATCH_BYTES (128ull << 20)
+int hs_join_noalloc(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* indexed_range,
+                    const hs_range* probe_range, const hs_tune* tune, uint32_t* r, uint32_t* s, uint32_t* overlap,
+                    uint64_t capacity, uint64_t* count);
+
 /* Device-to-device copy of `bytes` with a plain 16-byte-vector kernel,
  * repeated `reps` times on the context stream; *gbps receives the rate
  * (read + write bytes per second / 1e9).  The reference rate for HBM
