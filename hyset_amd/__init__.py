@@ -62,7 +62,8 @@ TUNE_NO_SIGNATURE = 4
 TUNE_BALANCED_TILES = 8
 TUNE_STATIC_TILES = 16
 TUNE_STAGE_TIMES = 32
-PLAN_FIXED, PLAN_UPLOAD, PLAN_QUEUE, PLAN_BALANCED = 1, 2, 3, 4
+TUNE_ITEM_MAJOR = 64
+PLAN_FIXED, PLAN_UPLOAD, PLAN_QUEUE, PLAN_BALANCED, PLAN_TOKEN = 1, 2, 3, 4, 5
 
 
 class Result(C.Structure):

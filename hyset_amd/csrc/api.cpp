@@ -525,6 +525,14 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
   const bool self = indexed == probe;
   P.self = self;
   P.ir = ir;
+  // the token-major plan: a fgss self-join of one range with itself, its
+  // index needed by nobody else (co-processing reads the index prefixes on
+  // the host), sizes that fit the 16-bit fields of the records
+  P.token_major = algo == HS_ALGO_FGSS && self && !with_index && !range_empty(ir) && pr.first == ir.first &&
+                  pr.last == ir.last && tune.probes_per_tile == 0 &&
+                  !(tune.flags & (HS_TUNE_STATIC_TILES | HS_TUNE_TILE_QUEUE | HS_TUNE_BALANCED_TILES |
+                                  HS_TUNE_ITEM_MAJOR)) &&
+                  indexed->maxlen < 0xFFFFu;
   // ---- parity signatures for the fgss signature test (timed with the index)
   bool sig_in_emit = false;  // the index's emit pass computes them (self-join over the whole collection)
   if (algo == HS_ALGO_FGSS && !(tune.flags & HS_TUNE_NO_SIGNATURE) && !range_empty(ir) && !range_empty(pr)) {
@@ -540,24 +548,31 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
   }
 
   // ---- index over the indexed range (not needed by the bitmap technique)
-  const uint32_t* prefix_tab = self ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
+  // the index prefix of a self-join is midprefix; the token-major plan
+  // indexes the probe prefixes (maxprefix), of which those are a part
+  const bool mid_index = self && !P.token_major;
+  const uint32_t* prefix_tab = mid_index ? ctx->dev_bounds.midprefix : ctx->dev_bounds.maxprefix;
   if ((algo != HS_ALGO_BITMAP || with_index) && !range_empty(ir)) {
     // entries per size class of the range (sets are sorted by size)
-    const auto& ptab = self ? ctx->tables.midprefix : ctx->tables.maxprefix;
+    const auto& ptab = mid_index ? ctx->tables.midprefix : ctx->tables.maxprefix;
     const auto& sf = indexed->h_size_first;
     const uint32_t nl = (uint32_t)sf.size();  // maxlen + 2
     std::vector<uint32_t>& cum = ctx->jcum_tmp;
     cum.resize(nl);
-    uint64_t acc = 0;
+    uint64_t acc = 0, side = 0;
     for (uint32_t l = 0; l + 1 < nl; ++l) {
       cum[l] = (uint32_t)acc;
       const uint32_t lo = std::max(ir.first, sf[l]);
       const uint32_t hi = std::min(ir.last + 1, sf[l + 1]);
-      if (hi > lo) acc += (uint64_t)(hi - lo) * ptab[l];
+      if (hi > lo) {
+        acc += (uint64_t)(hi - lo) * ptab[l];
+        side += (uint64_t)(hi - lo) * (self ? ctx->tables.midprefix[l] : ctx->tables.maxprefix[l]);
+      }
       HS_REQUIRE(acc < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
     }
     cum[nl - 1] = (uint32_t)acc;
     P.index_entries = acc;
+    P.index_side_entries = side;
     ctx->d_jcum.ensure((size_t)nl * 4);
     // the same table as the one the device holds: no copy (repeated joins)
     const bool held = !ctx->capturing_join && ctx->jcum_valid && ctx->jcum_epoch == hs::buffer_epoch() &&
@@ -576,7 +591,7 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
                     P.sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr,
                     sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
                     algo == HS_ALGO_FGSS ? ctx->counters.as<unsigned long long>() : nullptr,
-                    timed ? ctx->ev[1] : nullptr);
+                    timed ? ctx->ev[1] : nullptr, P.token_major ? ctx->dev_bounds.midprefix : nullptr);
     P.counters_ready = algo == HS_ALGO_FGSS && acc > 0 && indexed->n > 0;
   } else {
     ctx->ib.universe = 0;
@@ -596,6 +611,44 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
       HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
       HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));
     }
+  } else if (algo == HS_ALGO_FGSS && P.token_major) {
+    hs::TokArgs a{};
+    a.c = probe->dev();
+    a.keys = ctx->ib.keys[1].as<uint32_t>();  // the last radix pass leaves the sorted keys here
+    a.offsets = ctx->ib.offsets.as<uint32_t>();
+    a.ids = ctx->ib.set_ids.as<uint32_t>();
+    a.ent = ctx->ib.ent.as<uint32_t>();
+    a.ent_words = ctx->ib.ent_words;
+    a.n_ent = (uint32_t)P.index_entries;
+    a.bounds = ctx->dev_bounds;
+    a.size_first = indexed->size_first.as<uint32_t>();
+    a.size_first_len = (uint32_t)indexed->h_size_first.size();
+    a.first = pr.first;
+    a.last = pr.last;
+    a.sig_words = P.sig_words;
+    a.sigs = P.sig_words > 1 ? indexed->sigs.as<uint64_t>() : nullptr;
+    // survivors of both filters: a first guess of half the entries, grown
+    // after a join that found more (the join then runs again)
+    if (ctx->tok_queue_cap == 0) {
+      const uint64_t cap = std::max<uint64_t>(1u << 18, P.index_entries / 2);
+      ctx->tok_queue.ensure(cap * sizeof(uint4));
+      ctx->tok_queue_cap = (uint32_t)std::min<uint64_t>(ctx->tok_queue.bytes / sizeof(uint4), 0xFFFFFFFFull);
+    }
+    a.queue = ctx->tok_queue.as<uint4>();
+    a.queue_cap = ctx->tok_queue_cap;
+    // chunks of heavy blocks: a first guess of one per block, grown like the queue
+    if (ctx->tok_heavy_cap == 0) {
+      const uint64_t cap = std::max<uint64_t>(4096, (P.index_entries + hs::TOK_BLOCK - 1) / hs::TOK_BLOCK);
+      ctx->tok_heavy.ensure(cap * sizeof(uint4));
+      ctx->tok_heavy_cap = (uint32_t)std::min<uint64_t>(ctx->tok_heavy.bytes / sizeof(uint4), 0xFFFFFFFFull);
+    }
+    a.heavy = ctx->tok_heavy.as<uint4>();
+    a.heavy_cap = ctx->tok_heavy_cap;
+    ctx->tok_aoff.ensure(std::max<uint64_t>(P.index_entries, 1) * sizeof(uint2));
+    a.aoff = ctx->tok_aoff.as<uint2>();
+    a.out = dev_out(ctx);
+    tile_plan = HS_PLAN_TOKEN;
+    hs::launch_tok_join(st, a, (uint32_t)ctx->num_cus, timed ? ctx->ev[5] : nullptr, timed ? ctx->ev[2] : nullptr);
   } else if (algo == HS_ALGO_FGSS) {
     hs::FgssArgs a{};
     a.probe = probe->dev();
@@ -856,8 +909,28 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       reset_counters(ctx);
       if (algo == HS_ALGO_FGSS && !range_empty(Q.ir))  // index size, read back with the counters
         HS_CHECK_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + hs::CNT_ENTRIES,
-                                    ctx->ib.offsets.as<uint32_t>() + indexed->universe, 4, hipMemcpyDeviceToDevice,
+                                    ctx->ib.offsets.as<uint32_t>() + ctx->ib.universe, 4, hipMemcpyDeviceToDevice,
                                     st));
+    };
+    // the result rows and (token-major) the survivor queue hold the counts
+    auto fits = [&](const hs::DevicePlan& Q, const unsigned long long* c) {
+      return c[hs::CNT_PAIRS] <= ctx->out_cap &&
+             (!Q.token_major || (c[hs::CNT_QUEUE] <= ctx->tok_queue_cap && c[hs::CNT_HEAVY] <= ctx->tok_heavy_cap));
+    };
+    auto grow_for = [&](const hs::DevicePlan& Q, const unsigned long long* c) {
+      if (c[hs::CNT_PAIRS] > ctx->out_cap) ensure_output(ctx, c[hs::CNT_PAIRS] + c[hs::CNT_PAIRS] / 4 + 1024);
+      if (Q.token_major && c[hs::CNT_QUEUE] > ctx->tok_queue_cap) {
+        const uint64_t cap = c[hs::CNT_QUEUE] + c[hs::CNT_QUEUE] / 4 + 1024;
+        HS_REQUIRE(cap < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "more than 2^32 candidates");
+        ctx->tok_queue.ensure(cap * sizeof(uint4));
+        ctx->tok_queue_cap = (uint32_t)std::min<uint64_t>(ctx->tok_queue.bytes / sizeof(uint4), 0xFFFFFFFFull);
+      }
+      if (Q.token_major && c[hs::CNT_HEAVY] > ctx->tok_heavy_cap) {
+        const uint64_t cap = c[hs::CNT_HEAVY] + c[hs::CNT_HEAVY] / 4 + 1024;
+        HS_REQUIRE(cap < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "more than 2^32 chunks");
+        ctx->tok_heavy.ensure(cap * sizeof(uint4));
+        ctx->tok_heavy_cap = (uint32_t)std::min<uint64_t>(ctx->tok_heavy.bytes / sizeof(uint4), 0xFFFFFFFFull);
+      }
     };
     // a launch sequence captured by an earlier call with the same join,
     // replayed while no buffer it recorded has moved (HS_TUNE_GRAPH)
@@ -882,32 +955,39 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       P = ctx->jgraph_plan;
       tile_plan = ctx->jgraph_tile_plan;
       read_counters(ctx, cnt, ctx->ev[3]);
-      if (cnt[hs::CNT_PAIRS] > ctx->out_cap) {  // more pairs than the buffers hold: grow, run again eagerly
+      // more pairs (or survivors, or chunks) than the buffers hold: grow,
+      // run again eagerly (a full chunk or survivor queue leaves the later
+      // counts short, so up to three growths may follow one another)
+      for (int again = 0; !fits(P, cnt); ++again) {
+        HS_REQUIRE(again < 3, HS_E_DEVICE, "result count changed between runs");
         drop_join_graph(ctx);
-        ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
+        grow_for(P, cnt);
         reset_for_launch(P);
         tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
         read_counters(ctx, cnt, ctx->ev[3]);
-        HS_REQUIRE(cnt[hs::CNT_PAIRS] <= ctx->out_cap, HS_E_DEVICE, "result count changed between runs");
       }
     } else {
       HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
       hs::prepare_device_join(ctx, indexed, probe, ir, pr, algo, tune, false, P);
 
       // ---- candidate generation + verification
-      for (int attempt = 0; attempt < 2; ++attempt) {
+      // at most four runs: a full chunk queue leaves the survivor count
+      // short and a full survivor queue the pair count, so the survivor
+      // queue and the result rows may have to grow after it in turn
+      for (int attempt = 0; attempt < 4; ++attempt) {
         if (attempt > 0 || !P.counters_ready) reset_for_launch(P);
         tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
         read_counters(ctx, cnt, ctx->ev[3]);
-        if (cnt[hs::CNT_PAIRS] <= ctx->out_cap) break;
-        HS_REQUIRE(attempt == 0, HS_E_DEVICE, "result count changed between runs");
-        ensure_output(ctx, cnt[hs::CNT_PAIRS] + cnt[hs::CNT_PAIRS] / 4 + 1024);
+        if (fits(P, cnt)) break;
+        HS_REQUIRE(attempt < 3, HS_E_DEVICE, "result count changed between runs");
+        grow_for(P, cnt);
       }
       // Capture the sequence just run for the next calls.  The eager run
       // allocated every buffer, so the capture allocates nothing.  Plans that
       // stage per-launch tables through host buffers (balanced, tile queue)
       // stay eager.
-      if (want_graph && (algo == HS_ALGO_BITMAP || tile_plan == HS_PLAN_UPLOAD || tile_plan == HS_PLAN_FIXED))
+      if (want_graph && (algo == HS_ALGO_BITMAP || tile_plan == HS_PLAN_UPLOAD || tile_plan == HS_PLAN_FIXED ||
+                         tile_plan == HS_PLAN_TOKEN))
         if (capture_join_graph(ctx, indexed, probe, ir, pr, algo, tune, key, reset_for_launch)) graph_state = 1;
     }
     const uint64_t count = cnt[hs::CNT_PAIRS];
@@ -940,7 +1020,9 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     }
     if (algo == HS_ALGO_FGSS) {
       ctx->stats.tile_plan = tile_plan;
-      ctx->stats.index_entries = cnt[hs::CNT_ENTRIES];
+      // the index side's entries (the reference's index size); the
+      // token-major index also holds the probe-only prefix entries
+      ctx->stats.index_entries = P.token_major ? P.index_side_entries : cnt[hs::CNT_ENTRIES];
       ctx->stats.signature_tests = cnt[hs::CNT_SIG_TESTS];
       ctx->stats.signature_bits = P.sig_words * 64;
     }

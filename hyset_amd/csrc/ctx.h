@@ -88,8 +88,10 @@ struct DevicePlan {
   hs_range ir{1, 0};          // indexed range of the index
   uint32_t sig_words = 0;     // fgss signature test width (64-bit words)
   uint32_t bitmap_words = 0;  // signatures computed for the bitmap technique
-  uint64_t index_entries = 0;
+  uint64_t index_entries = 0;    // entries of the index built (token-major: probe prefixes)
+  uint64_t index_side_entries = 0;  // of those, the entries of the index prefixes
   bool counters_ready = false;  // the index build zeroed the work counters
+  bool token_major = false;     // fgss self-join of one range: the token-major plan
 };
 }  // namespace hs
 
@@ -174,6 +176,10 @@ struct hs_ctx {
   bool capturing_join = false;  // prepare_device_join stages through jgraph_cum
   bool stage_times = false;     // this join records events between its stages (HS_TUNE_STAGE_TIMES)
   uint64_t graph_epoch = 0;  // buffer epoch the capture saw
+  DevBuf tok_queue;          // token-major join: survivors of the filters, verified next
+  uint32_t tok_queue_cap = 0;
+  DevBuf tok_heavy, tok_aoff;  // token-major join: chunks of heavy blocks and the blocks' layouts
+  uint32_t tok_heavy_cap = 0;
   // hs_join_noalloc: its host bound tables and the staging copy of tables
   // and entry table (host memory only; the device side is the scratch array)
   hs::BoundTables na_tables;

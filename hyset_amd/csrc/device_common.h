@@ -72,6 +72,37 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* t
   return res;
 }
 
+// Exclusive running maximum over the workgroup's threads (0 for thread 0);
+// tmp holds NT / WAVE + 1 words.  Contains barriers: call uniformly.
+template <int NT>
+__device__ __forceinline__ uint32_t block_exclusive_max(uint32_t v, uint32_t* tmp) {
+  constexpr int NW = NT / WAVE;
+  const uint32_t lane = lane_id();
+  const uint32_t wave = threadIdx.x / WAVE;
+  uint32_t inc = v;
+#pragma unroll
+  for (int d = 1; d < WAVE; d <<= 1) {
+    const uint32_t o = __shfl_up(inc, d);
+    if (lane >= (uint32_t)d) inc = max(inc, o);
+  }
+  const uint32_t exc_w = __shfl_up(inc, 1);
+  if (lane == WAVE - 1) tmp[wave] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t x = tmp[w];
+      tmp[w] = run;
+      run = max(run, x);
+    }
+  }
+  __syncthreads();
+  const uint32_t res = max(tmp[wave], lane ? exc_w : 0u);
+  __syncthreads();
+  return res;
+}
+
 // First index i in [lo, hi) with a[i] >= key (a ascending); hi if none.
 __device__ __forceinline__ uint32_t lower_bound_u32(const uint32_t* a, uint32_t lo, uint32_t hi, uint32_t key) {
   while (lo < hi) {

@@ -947,4 +947,530 @@ uint32_t auto_signature_words(double mean) {
   return w;
 }
 
+// ---- token-major self-join
+//
+// The lists of the probe-prefix index are short (at 1M sets and Jaccard
+// 0.8 the longest of 647K lists holds 558 entries, the median 2), so a lane
+// that owns one entry finds all its visits by walking back through its own
+// list: the entries of one list lie next to each other, ascending in set
+// id, so the ids below the probe and inside its length window are the run
+// just before it.  Lanes of a wave own consecutive entries and read
+// overlapping runs: the records stream in with coalesced loads, and the only
+// random reads left are those of verification (the survivors of both
+// filters, ~1% of the visits).
+namespace {
+constexpr int TK_TPB = 512;
+constexpr uint32_t TK_EPL = TOK_BLOCK / TK_TPB;  // entries per lane
+constexpr uint32_t TK_LUT = 512;  // set sizes whose window start and minoverlap sit in LDS
+constexpr uint32_t TK_QCAP = 512;  // survivors held in LDS before one flush to the global queue
+constexpr uint32_t TK_NONE = 0xFFFFFFFFu;
+static_assert(TOK_BLOCK % TK_TPB == 0 && TOK_WINDOW >= TOK_BLOCK, "tile geometry");
+
+struct TokCounts {
+  uint32_t visits = 0, items = 0, tests = 0;
+};
+
+template <int kW>
+__device__ __forceinline__ void tok_record(const TokArgs& A, uint32_t e, uint32_t& sid, uint32_t& meta,
+                                           uint64_t& sig) {
+  if constexpr (kW == 1) {
+    const uint4 v = reinterpret_cast<const uint4*>(A.ent)[e];
+    sid = v.x;
+    meta = v.y;
+    sig = ((uint64_t)v.w << 32) | v.z;
+  } else {
+    const uint2 v = *reinterpret_cast<const uint2*>(A.ent + (size_t)e * A.ent_words);
+    sid = v.x;
+    meta = v.y;
+    sig = 0;
+  }
+}
+
+// Workgroup state of a token-major workgroup: the bound tables (window
+// start and minoverlap per size), the staged window of the index
+// (entries [ws, ws + wn) of keys, set ids, metas and 64-bit signatures),
+// the tile's runs and first visits, and the survivors not yet flushed.
+struct TokLds {
+  uint32_t lo_of[TK_LUT + 2];  // first id of the window of a probe of size l
+  uint16_t mo[2 * TK_LUT + 1];
+  uint32_t wk[TOK_WINDOW], wid[TOK_WINDOW], wm[TOK_WINDOW];
+  uint32_t ks[TOK_WINDOW];  // first staged position of each position's key
+  uint64_t wsig[TOK_WINDOW];
+  uint32_t a[TOK_BLOCK], off[TOK_BLOCK + 1];
+  uint32_t tmp[TK_TPB / WAVE + 1];
+  uint32_t q_r[TK_QCAP], q_s[TK_QCAP], q_j[TK_QCAP];
+  uint32_t qn, qbase, ws, wn, trunc;
+  uint32_t stats[3];
+};
+}  // namespace
+
+__device__ __forceinline__ bool tok_lut(const TokArgs& A) { return A.bounds.kind != HS_COSINE && A.bounds.L <= TK_LUT; }
+
+__device__ __forceinline__ uint32_t tok_window_lo(const TokArgs& A, uint32_t lr) {
+  return max(A.size_first[min(A.bounds.minsize[lr], A.size_first_len - 1)], A.first);
+}
+
+__device__ __forceinline__ void tok_tables(const TokArgs& A, TokLds& L) {
+  if (tok_lut(A)) {
+    for (uint32_t l = threadIdx.x; l < A.bounds.L + 2; l += TK_TPB) L.lo_of[l] = tok_window_lo(A, min(l, A.bounds.L));
+    for (uint32_t k = threadIdx.x; k < 2 * A.bounds.L + 1; k += TK_TPB) L.mo[k] = (uint16_t)A.bounds.mo[k];
+  }
+  if (threadIdx.x < 3) L.stats[threadIdx.x] = 0;
+  if (threadIdx.x == 0) {
+    L.qn = 0;
+    L.ws = 0;
+    L.wn = 0;
+  }
+}
+
+// Moves the survivors held in LDS to the global queue: one atomic for all
+// of them.  Called by every lane, between barriers.
+__device__ __forceinline__ void tok_flush_queue(const TokArgs& A, TokLds& L) {
+  const uint32_t n = min(L.qn, TK_QCAP);  // the others went to the global queue directly
+  if (n == 0) return;
+  if (threadIdx.x == 0) L.qbase = (uint32_t)min(atomicAdd(&A.out.counters[CNT_QUEUE], (unsigned long long)n),
+                                                 (unsigned long long)0xFFFFFFFFu);
+  __syncthreads();
+  const uint32_t base = L.qbase;
+  for (uint32_t k = threadIdx.x; k < n; k += TK_TPB) {
+    const uint64_t slot = (uint64_t)base + k;
+    if (slot < A.queue_cap) A.queue[slot] = make_uint4(L.q_r[k], L.q_s[k], L.q_j[k], 0u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) L.qn = 0;
+  __syncthreads();
+}
+
+// Entry e's set id, meta and 64-bit signature: from the staged window
+// (kMixed == false: the caller knows the window holds e), else from the
+// window when it holds e and the records otherwise.  The two forms are
+// separate code: a choice between an LDS and a global address compiles to
+// one generic (flat) load, which takes the vector memory path even for LDS.
+template <int kW, bool kMixed>
+__device__ __forceinline__ void tok_entry(const TokArgs& A, const TokLds& L, uint32_t e, uint32_t& sid,
+                                          uint32_t& meta, uint64_t& sig) {
+  const uint32_t w = e - L.ws;
+  if (!kMixed || w < L.wn) {
+    sid = L.wid[w];
+    meta = L.wm[w];
+    sig = kW == 1 ? L.wsig[w] : 0;
+  } else {
+    tok_record<kW>(A, e, sid, meta, sig);
+  }
+}
+
+// Visits [p0, p1) of tile t (entries t TOK_BLOCK + k, their runs L.a and
+// first visits L.off): each lane takes a contiguous range of them, finds
+// its first entry once and walks on through the runs, reading entries from
+// the staged window.  Survivors go to the LDS queue (to the global queue
+// directly when it is full); call tok_flush_queue between tiles.
+template <int kW, bool kMixed>
+__device__ __forceinline__ void tok_visits_(const TokArgs& A, TokLds& L, uint32_t t, uint32_t p0, uint32_t p1,
+                                            TokCounts& C) {
+  if (p1 <= p0) return;
+  const bool lut = tok_lut(A);
+  const uint32_t per = (p1 - p0 + TK_TPB - 1) / TK_TPB;
+  uint32_t p = p0 + threadIdx.x * per;
+  const uint32_t pe = min(p + per, p1);
+  if (p >= pe) return;
+  uint32_t k = segment_of(L.off, TOK_BLOCK, p);
+  uint32_t r, mr;
+  uint64_t sig_r;
+  tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
+  for (; p < pe; ++p) {
+    if (p >= L.off[k + 1]) {  // the next entry with visits
+      do ++k;
+      while (p >= L.off[k + 1]);
+      tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
+    }
+    const uint32_t j = L.a[k] + (p - L.off[k]);
+    uint32_t s, ms;
+    uint64_t sig_s;
+    tok_entry<kW, kMixed>(A, L, j, s, ms, sig_s);
+    const uint32_t lr = mr >> 16, jr = mr & 0xFFFFu, ls = ms >> 16, ps = ms & 0xFFFFu;
+    ++C.visits;
+    const uint32_t mo = lut ? min((uint32_t)L.mo[lr + ls], min(lr, ls)) : dev_minoverlap(A.bounds, lr, ls);
+    // positional filter (helpers.hpp:46-50) at this token, which only the
+    // first common token of r and s may be
+    bool keep = 1u + min(lr - 1u - jr, ls - 1u - ps) >= mo;
+    if (keep && kW == 1) {
+      ++C.tests;
+      keep = lr + ls >= 2u * mo + (uint32_t)__popcll(sig_r ^ sig_s);
+    } else if (keep && kW > 1) {
+      ++C.tests;
+      const uint64_t* sr = A.sigs + (size_t)r * kW;
+      const uint64_t* ss = A.sigs + (size_t)s * kW;
+      const uint32_t slack = lr + ls - 2u * mo;
+      uint32_t pc = 0;
+      for (int q = 0; q < kW && pc <= slack; q += 2)
+        pc += (uint32_t)__popcll(sr[q] ^ ss[q]) + (uint32_t)__popcll(sr[q + 1] ^ ss[q + 1]);
+      keep = pc <= slack;
+    }
+    if (keep) {  // rare: one LDS atomic per survivor
+      const uint32_t slot = atomicAdd(&L.qn, 1u);
+      if (slot < TK_QCAP) {
+        L.q_r[slot] = r;
+        L.q_s[slot] = s;
+        L.q_j[slot] = jr;
+      } else {
+        const unsigned long long g = atomicAdd(&A.out.counters[CNT_QUEUE], 1ull);
+        if (g < A.queue_cap) A.queue[g] = make_uint4(r, s, jr, 0u);
+      }
+    }
+  }
+}
+
+// The visits of a tile whose window holds every entry they read (no list
+// of the tile starts before its staged window) read LDS only.
+template <int kW>
+__device__ __forceinline__ void tok_visits(const TokArgs& A, TokLds& L, uint32_t t, uint32_t p0, uint32_t p1,
+                                           TokCounts& C) {
+  if (L.trunc == TK_NONE) tok_visits_<kW, false>(A, L, t, p0, p1, C);
+  else tok_visits_<kW, true>(A, L, t, p0, p1, C);
+}
+
+// Work counters: registers, then LDS, then one global atomic per
+// workgroup and counter, once at the end of a persistent workgroup.
+__device__ __forceinline__ void tok_flush(const TokArgs& A, TokLds& L, const TokCounts& C) {
+  const uint32_t v = wave_sum(C.visits), it = wave_sum(C.items), te = wave_sum(C.tests);
+  if (lane_id() == 0) {
+    if (v) atomicAdd(&L.stats[0], v);
+    if (it) atomicAdd(&L.stats[1], it);
+    if (te) atomicAdd(&L.stats[2], te);
+  }
+  __syncthreads();
+  tok_flush_queue(A, L);
+  if (threadIdx.x == 0) {
+    if (L.stats[0]) atomicAdd(&A.out.counters[CNT_VISITS], (unsigned long long)L.stats[0]);
+    if (L.stats[1]) atomicAdd(&A.out.counters[CNT_ITEMS], (unsigned long long)L.stats[1]);
+    if (L.stats[2]) atomicAdd(&A.out.counters[CNT_SIG_TESTS], (unsigned long long)L.stats[2]);
+  }
+}
+
+// First staged position p <= hi with (key, id) >= (k, v), hi qualifying:
+// gallop back from hi, then bisect the last step (runs are short: most take
+// one or two reads).
+__device__ __forceinline__ uint32_t tok_lb_lds(const TokLds& L, uint32_t hi, uint32_t k, uint32_t v) {
+  uint32_t step = 1, lo;
+  for (;;) {
+    const uint32_t j = hi >= step ? hi - step : 0u;
+    const uint32_t kj = L.wk[j];
+    if (kj < k || (kj == k && L.wid[j] < v)) {
+      lo = j + 1;
+      break;
+    }
+    hi = j;
+    if (j == 0) return 0;
+    step <<= 1;
+  }
+  while (lo < hi) {
+    const uint32_t m = (lo + hi) >> 1;
+    const uint32_t km = L.wk[m];
+    if (km < k || (km == k && L.wid[m] < v)) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+// The same over the global arrays, galloping back from hi (qualifying):
+// the entries whose lists start before the staged window.
+__device__ __forceinline__ uint32_t tok_lb_back(const TokArgs& A, uint32_t hi, uint32_t k, uint32_t v) {
+  uint32_t step = 1, lo;
+  for (;;) {
+    const uint32_t j = hi >= step ? hi - step : 0u;
+    const uint32_t kj = A.keys[j];
+    if (kj < k || (kj == k && A.ids[j] < v)) {
+      lo = j + 1;
+      break;
+    }
+    hi = j;
+    if (j == 0) return 0;
+    step <<= 1;
+  }
+  while (lo < hi) {
+    const uint32_t m = (lo + hi) >> 1;
+    const uint32_t km = A.keys[m];
+    if (km < k || (km == k && A.ids[m] < v)) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+// The staged window of tile t: from the start of its first token's list
+// (its index part) to the tile's end, at most TOK_WINDOW entries; every
+// other token of the tile lies inside it whole.  hdr = {first key of the
+// tile, first entry of that key's index part}.
+__device__ __forceinline__ uint2 tok_header(const TokArgs& A, uint32_t t) {
+  const uint32_t k0 = A.keys[t * TOK_BLOCK] & ~1u;
+  return make_uint2(k0, A.offsets[k0]);
+}
+
+// Stages tile t's window from its header; called by every lane between
+// barriers (thread 0's hdr counts).  All loads of a lane are issued before
+// its first LDS store.
+template <int kW>
+__device__ __forceinline__ void tok_stage(const TokArgs& A, TokLds& L, uint32_t t, uint2 hdr) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t t0 = t * TOK_BLOCK, t1 = min(t0 + TOK_BLOCK, A.n_ent);
+  if (tid == 0) {
+    const uint32_t ws = max(hdr.y, t1 > TOK_WINDOW ? t1 - TOK_WINDOW : 0u);
+    L.ws = ws;
+    L.wn = t1 - ws;
+    L.trunc = ws > hdr.y ? hdr.x : TK_NONE;
+  }
+  __syncthreads();
+  const uint32_t ws = L.ws, wn = L.wn;
+  constexpr uint32_t R = TOK_WINDOW / TK_TPB;
+  uint32_t key[R], sid[R], meta[R];
+  uint64_t sig[R];
+#pragma unroll
+  for (uint32_t q = 0; q < R; ++q) {
+    const uint32_t w = tid + q * TK_TPB;
+    key[q] = sid[q] = meta[q] = 0;
+    sig[q] = 0;
+    if (w < wn) {
+      tok_record<kW>(A, ws + w, sid[q], meta[q], sig[q]);
+      key[q] = A.keys[ws + w];
+    }
+  }
+#pragma unroll
+  for (uint32_t q = 0; q < R; ++q) {
+    const uint32_t w = tid + q * TK_TPB;
+    if (w < wn) {
+      L.wk[w] = key[q];
+      L.wid[w] = sid[q];
+      L.wm[w] = meta[q];
+      if (kW == 1) L.wsig[w] = sig[q];
+    }
+  }
+  __syncthreads();
+  // the first position of every position's key: a running maximum of the
+  // positions where the key changes (R consecutive positions per lane)
+  uint32_t head[R], m = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < R; ++q) {
+    const uint32_t w = tid * R + q;
+    head[q] = (w < wn && (w == 0 || L.wk[w] != L.wk[w - 1])) ? w : 0u;
+    m = max(m, head[q]);
+  }
+  uint32_t run = block_exclusive_max<TK_TPB>(m, L.tmp);
+#pragma unroll
+  for (uint32_t q = 0; q < R; ++q) {
+    run = max(run, head[q]);
+    L.ks[tid * R + q] = run;
+  }
+  __syncthreads();
+}
+
+// First staged position in [lo, hi) with set id >= v (ids ascend there); hi if none.
+__device__ __forceinline__ uint32_t tok_lb_ids(const TokLds& L, uint32_t lo, uint32_t hi, uint32_t v) {
+  while (lo < hi) {
+    const uint32_t m = (lo + hi) >> 1;
+    if (L.wid[m] < v) lo = m + 1;
+    else hi = m;
+  }
+  return lo;
+}
+
+// Persistent: workgroup g takes tiles g, g + grid, ... of TOK_BLOCK
+// entries and stages each tile's window.  Each entry's visits are the run
+// [a, b) of its token's index entries with ids in [window start, probe):
+// bisections over the staged keys and ids (over the global arrays for the
+// first token when its list starts before the staged window).  A scan
+// numbers the tile's visits, the workgroup tests the first TOK_CHUNK of
+// them, and a heavier tile queues its other chunks for k_tok_heavy.
+template <int kW>
+__global__ __launch_bounds__(TK_TPB) void k_tok_filter(TokArgs A) {
+  __shared__ TokLds L;
+  const uint32_t tid = threadIdx.x;
+  tok_tables(A, L);
+  const bool lut = tok_lut(A);
+  TokCounts C;
+  const uint32_t n_tiles = (A.n_ent + TOK_BLOCK - 1) / TOK_BLOCK;
+  // thread 0 loads the next tile's header while the current one runs
+  uint2 hdr = make_uint2(0, 0);
+  if (tid == 0 && blockIdx.x < n_tiles) hdr = tok_header(A, blockIdx.x);
+  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    __syncthreads();  // the previous tile's readers are done with L
+    tok_stage<kW>(A, L, t, hdr);
+    if (tid == 0 && t + gridDim.x < n_tiles) hdr = tok_header(A, t + gridDim.x);
+    const uint32_t t0 = t * TOK_BLOCK, t1 = min(t0 + TOK_BLOCK, A.n_ent);
+    const uint32_t ws = L.ws;
+    uint32_t cnt[TK_EPL], sum = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < TK_EPL; ++q) {
+      const uint32_t k = tid * TK_EPL + q;  // consecutive entries per lane: one scan value each
+      const uint32_t i = t0 + k;
+      cnt[q] = 0;
+      uint32_t x = i;
+      if (i < t1) {
+        const uint32_t w = i - ws;
+        const uint32_t r = L.wid[w], lr = L.wm[w] >> 16;
+        const uint32_t lo = lut ? L.lo_of[lr] : tok_window_lo(A, lr);
+        if (lo < r) {
+          ++C.items;
+          const uint32_t key = L.wk[w], k_idx = key & ~1u;
+          uint32_t b = i;
+          if (k_idx != L.trunc) {
+            // the token's index part lies in the window: it starts at ts and
+            // ends where the key's run of other probe entries starts
+            const uint32_t kst = L.ks[w];
+            uint32_t ts = kst, bw = w;
+            if (key & 1u) {
+              if (kst > 0 && L.wk[kst - 1] == k_idx) ts = L.ks[kst - 1];
+              bw = tok_lb_ids(L, ts, kst, r);  // first index entry at or above r
+            }
+            x = ws + tok_lb_ids(L, ts, bw, lo);
+            b = ws + bw;
+          } else {
+            if (key & 1u) b = tok_lb_back(A, tok_lb_back(A, i, key, 0u), k_idx, r);
+            x = tok_lb_back(A, b, k_idx, lo);
+          }
+          cnt[q] = b - x;
+        }
+      }
+      L.a[k] = x;
+      sum += cnt[q];
+    }
+    uint32_t total;
+    uint32_t run = block_exclusive_scan<TK_TPB>(sum, L.tmp, &total);
+#pragma unroll
+    for (uint32_t q = 0; q < TK_EPL; ++q) {
+      const uint32_t k = tid * TK_EPL + q;
+      L.off[k] = run;
+      if (total > TOK_CHUNK && t0 + k < t1) A.aoff[t0 + k] = make_uint2(L.a[k], run);
+      run += cnt[q];
+    }
+    if (tid == 0) {
+      L.off[TOK_BLOCK] = total;
+      if (total > TOK_CHUNK) {  // the tile's other chunks, for the second pass
+        const uint32_t n = (total - 1) / TOK_CHUNK;  // chunks 1 .. n
+        const unsigned long long q = atomicAdd(&A.out.counters[CNT_HEAVY], (unsigned long long)n);
+        for (uint32_t k = 0; k < n; ++k)
+          if (q + k < A.heavy_cap) A.heavy[q + k] = make_uint4(t, k + 1, total, 0u);
+      }
+    }
+    __syncthreads();
+    tok_visits<kW>(A, L, t, 0, min(total, TOK_CHUNK), C);
+    __syncthreads();
+    if (L.qn >= TK_QCAP / 2) tok_flush_queue(A, L);  // (a fuller queue spills to the global one)
+  }
+  tok_flush(A, L, C);
+}
+
+// The queued chunks of heavy tiles, claimed in order by a persistent grid;
+// each restages its tile's window and reloads its layout.
+template <int kW>
+__global__ __launch_bounds__(TK_TPB) void k_tok_heavy(TokArgs A) {
+  __shared__ TokLds L;
+  const uint32_t tid = threadIdx.x;
+  tok_tables(A, L);
+  TokCounts C;
+  const unsigned long long queued = A.out.counters[CNT_HEAVY];
+  const uint32_t n_items = (uint32_t)min(queued, (unsigned long long)A.heavy_cap);
+  for (uint32_t q = blockIdx.x; q < n_items; q += gridDim.x) {
+    __syncthreads();  // the previous item's readers are done with L
+    const uint4 it = A.heavy[q];
+    const uint32_t t0 = it.x * TOK_BLOCK;
+    for (uint32_t k = tid; k < TOK_BLOCK; k += TK_TPB) {
+      if (t0 + k < A.n_ent) {
+        const uint2 ao = A.aoff[t0 + k];
+        L.a[k] = ao.x;
+        L.off[k] = ao.y;
+      } else {
+        L.off[k] = it.z;  // past the last entry: no visits
+      }
+    }
+    if (tid == 0) L.off[TOK_BLOCK] = it.z;
+    uint2 hdr = make_uint2(0, 0);
+    if (tid == 0) hdr = tok_header(A, it.x);
+    tok_stage<kW>(A, L, it.x, hdr);
+    tok_visits<kW>(A, L, it.x, it.y * TOK_CHUNK, min(it.z, (it.y + 1) * TOK_CHUNK), C);
+    __syncthreads();
+    if (L.qn >= TK_QCAP / 2) tok_flush_queue(A, L);
+  }
+  __syncthreads();
+  tok_flush(A, L, C);
+}
+
+// One lane per queued candidate: the s-driven merge of k_fgss's verification
+// from the token through which the candidate was reached (a pair reached
+// through several tokens is verified by the visit of its first common one).
+__global__ __launch_bounds__(TK_TPB) void k_tok_verify(TokArgs A) {
+  __shared__ unsigned long long s_stats[2];
+  if (threadIdx.x < 2) s_stats[threadIdx.x] = 0;
+  __syncthreads();
+  const unsigned long long queued = A.out.counters[CNT_QUEUE];
+  const uint32_t total = (uint32_t)min(queued, (unsigned long long)A.queue_cap);
+  const uint32_t stride = gridDim.x * TK_TPB;
+  uint32_t cands = 0, cand_tokens = 0;
+  // wave-uniform trip count: emit_pairs needs every lane of the wave
+  for (uint32_t base = blockIdx.x * TK_TPB; base < total; base += stride) {
+    const uint32_t q = base + threadIdx.x;
+    bool pass = false;
+    uint32_t ov = 0, r = 0, s = 0;
+    if (q < total) {
+      const uint4 e = A.queue[q];
+      r = e.x;
+      s = e.y;
+      const uint32_t lr = A.c.sizes[r], ls = A.c.sizes[s];
+      const uint32_t mo = dev_minoverlap(A.bounds, lr, ls);
+      pass = verify_visit<false>(A.c.tokens + A.c.starts[r], lr, A.c.tokens + A.c.starts[s], ls, e.z, mo, &ov);
+      ++cands;
+      cand_tokens += ls;
+    }
+    emit_pairs(pass, r, s, ov, A.out.r, A.out.s, A.out.ov, &A.out.counters[CNT_PAIRS], A.out.capacity);
+  }
+  cands = wave_sum(cands);
+  cand_tokens = wave_sum(cand_tokens);
+  if (lane_id() == 0 && cands) {
+    atomicAdd(&s_stats[0], (unsigned long long)cands);
+    atomicAdd(&s_stats[1], (unsigned long long)cand_tokens);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && s_stats[0]) {
+    atomicAdd(&A.out.counters[CNT_CANDS], s_stats[0]);
+    atomicAdd(&A.out.counters[CNT_CAND_TOKENS], s_stats[1]);
+  }
+}
+
+void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEvent_t t0, hipEvent_t t1) {
+  HS_REQUIRE(a.sig_words == 0 || a.sig_words == 1 || (a.sigs && a.ent_words == 2), HS_E_INVALID,
+             "token-major join: signature table missing");
+  HS_REQUIRE(a.sig_words != 1 || a.ent_words == 4, HS_E_INVALID,
+             "token-major join: 64-bit signature test needs records with signatures");
+  const dim3 b(TK_TPB);
+  const uint32_t n_tiles = std::max<uint32_t>((a.n_ent + TOK_BLOCK - 1) / TOK_BLOCK, 1);
+  // persistent passes: as many workgroups as the CUs hold at once
+  static int per_cu = 0;
+  if (per_cu == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_tok_filter<1>), TK_TPB, 0) !=
+            hipSuccess ||
+        nb < 1) {
+      (void)hipGetLastError();
+      nb = 4;
+    }
+    per_cu = nb;
+  }
+  const dim3 gp(std::max<uint32_t>(num_cus, 1) * (uint32_t)per_cu);
+  const dim3 g(std::min<uint32_t>(n_tiles, gp.x));
+#define HS_TOK_FILTER(W)                                                           \
+  hipExtLaunchKernelGGL(k_tok_filter<W>, g, b, 0, st, t0, nullptr, 0, a);          \
+  hipExtLaunchKernelGGL(k_tok_heavy<W>, gp, b, 0, st, nullptr, nullptr, 0, a);
+  switch (a.sig_words) {
+    case 0: HS_TOK_FILTER(0) break;
+    case 1: HS_TOK_FILTER(1) break;
+    case 2: HS_TOK_FILTER(2) break;
+    case 4: HS_TOK_FILTER(4) break;
+    case 8: HS_TOK_FILTER(8) break;
+    case 16: HS_TOK_FILTER(16) break;
+    default: throw Error{HS_E_INVALID, "signature width must be 0, 64, 128, 256, 512 or 1024 bits"};
+  }
+#undef HS_TOK_FILTER
+  HS_CHECK_HIP(hipGetLastError());
+  hipExtLaunchKernelGGL(k_tok_verify, gp, b, 0, st, nullptr, t1, 0, a);
+  HS_CHECK_HIP(hipGetLastError());
+}
+
 }  // namespace hs

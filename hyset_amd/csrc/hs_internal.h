@@ -125,9 +125,11 @@ struct DevIndex {
 // (bitmap: pair tests), distinct candidates verified (bitmap: filter
 // survivors), tokens of the verified candidate sets, probe-prefix items,
 // index entries, and (fgss) visits that reached the signature test.
+// CNT_QUEUE: candidates the token-major filter appended to its queue;
+// CNT_HEAVY: chunks of list positions it left to the second filter pass.
 enum {
   CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_CAND_TOKENS = 3, CNT_ITEMS = 4, CNT_ENTRIES = 5,
-  CNT_SIG_TESTS = 6, CNT_WORDS = 8
+  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_WORDS = 16
 };
 
 struct DevOut {
@@ -148,23 +150,25 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
                            DevBuf& scratch);
 
 struct IndexBuild {
-  DevBuf offsets, set_ids, positions, ent;  // the index
+  DevBuf offsets, set_ids, metas, ent;  // the index
   DevBuf keys[2], vals, pos, hist, scan_scratch;  // radix-sort work space
   uint32_t ent_words = 2;
   uint32_t universe = 0;
   uint64_t entries = 0;
 };
 // The same buffers as raw device pointers: the index (offsets, set_ids,
-// positions, ent) and the radix sort's work space (keys, vals, pos, the
-// per-tile digit histograms and the digit totals).
+// metas, ent) and the radix sort's work space (keys, vals, pos, the
+// per-tile digit histograms and the digit totals).  A meta is the entry's
+// (min(|s|, 0xFFFF) << 16) | min(position, 0xFFFF), written by the emit
+// pass and carried through the sort.
 struct IndexBufs {
   uint32_t* offsets;
   uint32_t* set_ids;
-  uint16_t* positions;
+  uint32_t* metas;
   uint32_t* ent;
   uint32_t* keys[2];
   uint32_t* vals;
-  uint16_t* pos;
+  uint32_t* pos;
   uint32_t* hist;
   uint32_t* totals;
 };
@@ -186,12 +190,16 @@ size_t index_region(uint32_t universe, uint64_t entries, uint8_t* base, IndexBuf
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out = nullptr,
-                 unsigned long long* counters = nullptr, hipEvent_t done = nullptr);
+                 unsigned long long* counters = nullptr, hipEvent_t done = nullptr,
+                 const uint32_t* flag_len = nullptr);
 // The same into buffers the caller laid out (index_region); allocates nothing.
+// flag_len (may be NULL; indexed by set size): the token-major index, keys
+// (token << 1) | (position >= flag_len[|s|]) over 2 x universe (pass that
+// key universe here and to index_region).
 void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                       const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                       const IndexBufs& b, const uint64_t* sigs, uint64_t* sigs_out,
-                      unsigned long long* counters, hipEvent_t done = nullptr);
+                      unsigned long long* counters, hipEvent_t done = nullptr, const uint32_t* flag_len = nullptr);
 // The same over every join of a batched collection at once (token ids of
 // different joins are disjoint, so one index serves all joins).
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
@@ -281,6 +289,55 @@ struct BalancedPlan {
 // on the device (after the index build), pointing a at both; launch with
 // a.tile_queue set.
 void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp);
+
+// Token-major self-join of the id range [first, last] with itself.  The
+// index is built over the probe prefixes (maxprefix) of the range's sets,
+// keyed (token << 1) | (position >= midprefix(|s|)) and sorted by (key, set
+// id): each token's list holds its index entries (the index prefixes)
+// first, then its other probe entries, both ascending in id.  keys[e] is
+// entry e's key and ent its record (ent_words words: set id, meta and, with
+// 64-bit signatures, the set's signature); offsets[k] is the first entry of
+// key k.  The visits of a probe entry are the index entries of its token
+// with ids in its length window below the probe: one run of the token's
+// index part (ids ascend, and so do sizes).  A workgroup takes a tile of
+// TOK_BLOCK entries and stages in LDS the window of the index from the
+// start of its first token's list to the tile's end (at most TOK_WINDOW
+// entries), finds every entry's run by bisection there, and tests the
+// tile's visits -- positional filter, signature test -- from LDS; the
+// survivors {r, s, position of the token in r} go to `queue`, verified by a
+// last kernel.  A tile's first TOK_CHUNK visits are tested by its own
+// workgroup, the others through `heavy` ({tile, chunk, visits of the tile})
+// by a persistent second pass, `aoff` ({run start, first visit} per entry)
+// carrying the tile's layout there.
+constexpr uint32_t TOK_BLOCK = 512;
+constexpr uint32_t TOK_WINDOW = 1024;
+constexpr uint32_t TOK_CHUNK = 4096;
+struct TokArgs {
+  DevColl c;
+  const uint32_t* keys;
+  const uint32_t* offsets;  // first entry of each key, 2 x universe + 1 of them
+  const uint32_t* ids;      // set id of each entry (the records' first words, compact)
+  const uint32_t* ent;
+  uint32_t ent_words;
+  uint32_t n_ent;
+  DevBounds bounds;
+  const uint32_t* size_first;  // first id of size >= l, l in [0, maxlen + 1]
+  uint32_t size_first_len;     // maxlen + 2
+  uint32_t first, last;
+  uint32_t sig_words;          // 0: no signature test; 1: in the records; 2-16: from sigs
+  const uint64_t* sigs;        // per-set signatures, sig_words words each (sig_words > 1)
+  uint4* queue;
+  uint32_t queue_cap;
+  uint4* heavy;
+  uint32_t heavy_cap;
+  uint2* aoff;  // n_ent entries
+  DevOut out;
+};
+// Filter, then verify the queue (a persistent grid of num_cus * 8
+// workgroups); t0 / t1 (may be NULL) take the filter's start and the
+// verify's end.
+void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEvent_t t0 = nullptr,
+                     hipEvent_t t1 = nullptr);
 
 struct BitmapArgs {
   DevColl probe;

@@ -175,11 +175,17 @@ constexpr int RS_WAVES = RS_TPB / WAVE;
 // pass.
 constexpr uint32_t IX_CHUNK = 16;
 
+// With flag_len != NULL the key is (token << 1) | (position >= flag_len[l]):
+// the token-major index sorts a token's index-prefix entries before its
+// other probe-prefix entries.
 __device__ __forceinline__ void emit_set(const DevColl& c, uint32_t i, uint32_t l, uint32_t pl, uint32_t o,
                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                         uint16_t* __restrict__ pos, uint64_t* __restrict__ sig) {
+                                         uint32_t* __restrict__ meta, uint64_t* __restrict__ sig,
+                                         const uint32_t* __restrict__ flag_len) {
   const uint32_t* t = c.tokens + c.starts[i];
   const uint32_t len = sig ? l : pl;
+  const uint32_t size_bits = min(l, 0xFFFFu) << 16;
+  const uint32_t fl = flag_len ? flag_len[l] : 0xFFFFFFFFu;
   uint64_t w = 0;
   for (uint32_t k0 = 0; k0 < len; k0 += IX_CHUNK) {
     uint32_t v[IX_CHUNK];
@@ -189,9 +195,9 @@ __device__ __forceinline__ void emit_set(const DevColl& c, uint32_t i, uint32_t 
     for (uint32_t q = 0; q < IX_CHUNK; ++q) {
       const uint32_t k = k0 + q;
       if (k < pl) {
-        keys[o + k] = v[q];
+        keys[o + k] = flag_len ? (v[q] << 1) | (k >= fl ? 1u : 0u) : v[q];
         vals[o + k] = i;
-        pos[o + k] = (uint16_t)min(k, 0xFFFFu);
+        meta[o + k] = size_bits | min(k, 0xFFFFu);
       }
       if (k < len) w ^= 1ull << (v[q] & 63u);
     }
@@ -201,14 +207,14 @@ __device__ __forceinline__ void emit_set(const DevColl& c, uint32_t i, uint32_t 
 
 __global__ void k_index_emit(DevColl c, uint32_t first, uint32_t last, const uint32_t* __restrict__ prefix_len,
                              const uint32_t* __restrict__ size_first, const uint32_t* __restrict__ cum,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint16_t* __restrict__ pos,
-                             uint64_t* __restrict__ sig) {
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ meta,
+                             uint64_t* __restrict__ sig, const uint32_t* __restrict__ flag_len) {
   const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
   if (i > last) return;
   const uint32_t l = c.sizes[i];
   const uint32_t pl = prefix_len[l];
   const uint32_t o = cum[l] + (i - max(size_first[l], first)) * pl;
-  emit_set(c, i, l, pl, o, keys, vals, pos, sig);
+  emit_set(c, i, l, pl, o, keys, vals, meta, sig, flag_len);
 }
 
 // Batched collections: set i belongs to join j = last j with join_first[j]
@@ -217,7 +223,7 @@ __global__ void k_index_emit_batch(DevColl c, const uint32_t* __restrict__ join_
                                    const uint32_t* __restrict__ prefix_len, const uint32_t* __restrict__ size_first,
                                    const uint32_t* __restrict__ sf_base, const uint32_t* __restrict__ cum,
                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                   uint16_t* __restrict__ pos, uint64_t* __restrict__ sig) {
+                                   uint32_t* __restrict__ meta, uint64_t* __restrict__ sig) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= c.n) return;
   const uint32_t j = segment_of(join_first, n_joins, i);
@@ -225,7 +231,7 @@ __global__ void k_index_emit_batch(DevColl c, const uint32_t* __restrict__ join_
   const uint32_t pl = prefix_len[l];
   const uint32_t b = sf_base[j];
   const uint32_t o = cum[b + l] + (i - size_first[b + l]) * pl;
-  emit_set(c, i, l, pl, o, keys, vals, pos, sig);
+  emit_set(c, i, l, pl, o, keys, vals, meta, sig, nullptr);
 }
 
 // Digit histogram of one tile: LDS atomics, then one column of the
@@ -292,11 +298,11 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_rows(uint32_t* __restrict__ hi
 template <int kBits>
 __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __restrict__ kin,
                                                           const uint32_t* __restrict__ vin,
-                                                          const uint16_t* __restrict__ pin, uint32_t n,
+                                                          const uint32_t* __restrict__ pin, uint32_t n,
                                                           uint32_t shift, const uint32_t* __restrict__ offs,
                                                           const uint32_t* __restrict__ totals, uint32_t ntiles,
                                                           uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                          uint16_t* __restrict__ pout) {
+                                                          uint32_t* __restrict__ pout) {
   constexpr uint32_t NB = 1u << kBits;
   static_assert(NB <= RS_TPB, "one digit total per thread");
   __shared__ uint32_t s_cnt[RS_WAVES][NB];  // per-wave digit counts, then the wave's cursors
@@ -304,7 +310,7 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
   __shared__ uint32_t s_base[NB];           // output position of that slot
   __shared__ uint32_t s_key[RS_TILE];
   __shared__ uint32_t s_val[RS_TILE];
-  __shared__ uint16_t s_pos[RS_TILE];
+  __shared__ uint32_t s_pos[RS_TILE];
   __shared__ uint32_t s_tmp[RS_TPB / WAVE + 1];
   const uint32_t tid = threadIdx.x;
   const uint32_t wave = tid / WAVE, lane = lane_id();
@@ -383,7 +389,7 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
       const uint32_t o = s_cnt[wave][d] + lanes_below(peers);
       s_key[o] = key[r];
       s_val[o] = val[r];
-      s_pos[o] = (uint16_t)ps[r];
+      s_pos[o] = ps[r];
     }
     // the leader of each digit advances the wave's cursor; the next round's
     // reads come after it in this wave's program order
@@ -406,17 +412,16 @@ __global__ __launch_bounds__(RS_TPB) void k_radix_scatter(const uint32_t* __rest
 
 // offsets[t] = number of entries with token < t, for t in [0, universe],
 // from the sorted keys: entry i writes the offsets of the tokens in
-// (keys[i-1], keys[i]] (every t is written once); and, for every entry, the
-// size and (when given) the signature of its set next to the token
-// position (one random read per entry here instead of per visit).
-// With counters != NULL the thread of entry n also prepares the join's work
-// counters (all zero, the index size in CNT_ENTRIES), which saves the join
-// a fill and a copy launch.
+// (keys[i-1], keys[i]] (every t is written once); and, for every entry, its
+// record: set id, meta (size and position, carried through the sort) and,
+// when given, the set's signature (one random read per entry here instead
+// of per visit).  With counters != NULL the thread of entry n also prepares
+// the join's work counters (all zero, the index size in CNT_ENTRIES), which
+// saves the join a fill and a copy launch.
 __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, uint32_t universe,
-                               uint32_t* __restrict__ offsets, DevColl c, const uint32_t* __restrict__ set_ids,
-                               const uint16_t* __restrict__ positions, const uint64_t* __restrict__ sigs,
-                               uint32_t* __restrict__ ent,
-                               unsigned long long* __restrict__ counters) {
+                               uint32_t* __restrict__ offsets, const uint32_t* __restrict__ set_ids,
+                               const uint32_t* __restrict__ metas, const uint64_t* __restrict__ sigs,
+                               uint32_t* __restrict__ ent, unsigned long long* __restrict__ counters) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i > n) return;
   if (i == n && counters) {
@@ -428,7 +433,7 @@ __global__ void k_index_finish(const uint32_t* __restrict__ keys, uint32_t n, ui
   for (uint32_t t = prev; t <= cur && t <= universe; ++t) offsets[t] = i;
   if (i < n) {
     const uint32_t sid = set_ids[i];
-    const uint32_t m = (min(c.sizes[sid], 0xFFFFu) << 16) | positions[i];
+    const uint32_t m = metas[i];
     if (sigs) {
       const uint64_t g = sigs[sid];
       reinterpret_cast<uint4*>(ent)[i] = make_uint4(sid, m, (uint32_t)g, (uint32_t)(g >> 32));
@@ -466,12 +471,12 @@ size_t index_region(uint32_t universe, uint64_t entries, uint8_t* base, IndexBuf
   IndexBufs b{};
   b.offsets = reinterpret_cast<uint32_t*>(take(((size_t)universe + 1) * 4));
   b.set_ids = reinterpret_cast<uint32_t*>(take((cap + IDS_PAD) * 4));  // the window search reads whole quads
-  b.positions = reinterpret_cast<uint16_t*>(take(cap * 2));
+  b.metas = reinterpret_cast<uint32_t*>(take(cap * 4));
   b.ent = reinterpret_cast<uint32_t*>(take(cap * 16));
   b.keys[0] = reinterpret_cast<uint32_t*>(take(cap * 4));
   b.keys[1] = reinterpret_cast<uint32_t*>(take(cap * 4));
   b.vals = reinterpret_cast<uint32_t*>(take(cap * 4));
-  b.pos = reinterpret_cast<uint16_t*>(take(cap * 2));
+  b.pos = reinterpret_cast<uint32_t*>(take(cap * 4));
   b.hist = reinterpret_cast<uint32_t*>(take((size_t)std::max<uint32_t>(radix_tiles(entries), 1) * RS_DIGITS * 4));
   b.totals = reinterpret_cast<uint32_t*>(take(RS_DIGITS * 4));
   if (out) *out = b;
@@ -485,24 +490,24 @@ static IndexBufs prepare(IndexBuild& ib, uint32_t universe, uint64_t entries) {
   const size_t cap = entries ? entries : 1;
   ib.offsets.ensure(((size_t)universe + 1) * sizeof(uint32_t));
   ib.set_ids.ensure((cap + IDS_PAD) * sizeof(uint32_t));  // the window search reads whole quads
-  ib.positions.ensure(cap * sizeof(uint16_t));
+  ib.metas.ensure(cap * sizeof(uint32_t));
   ib.ent.ensure(cap * 4 * sizeof(uint32_t));
   ib.keys[0].ensure(cap * sizeof(uint32_t));
   ib.keys[1].ensure(cap * sizeof(uint32_t));
   ib.vals.ensure(cap * sizeof(uint32_t));
-  ib.pos.ensure(cap * sizeof(uint16_t));
+  ib.pos.ensure(cap * sizeof(uint32_t));
   ib.hist.ensure((size_t)std::max<uint32_t>(radix_tiles(entries), 1) * RS_DIGITS * sizeof(uint32_t));
   ib.scan_scratch.ensure(RS_DIGITS * sizeof(uint32_t));
-  return IndexBufs{ib.offsets.as<uint32_t>(), ib.set_ids.as<uint32_t>(), ib.positions.as<uint16_t>(),
+  return IndexBufs{ib.offsets.as<uint32_t>(), ib.set_ids.as<uint32_t>(), ib.metas.as<uint32_t>(),
                    ib.ent.as<uint32_t>(), {ib.keys[0].as<uint32_t>(), ib.keys[1].as<uint32_t>()},
-                   ib.vals.as<uint32_t>(), ib.pos.as<uint16_t>(), ib.hist.as<uint32_t>(),
+                   ib.vals.as<uint32_t>(), ib.pos.as<uint32_t>(), ib.hist.as<uint32_t>(),
                    ib.scan_scratch.as<uint32_t>()};
 }
 
 // One LSD pass over the entries in buffer `cur`: histograms, row scans, scatter.
 template <int kBits>
 static void radix_pass(hipStream_t st, uint32_t n, uint32_t shift, uint32_t ntiles, const IndexBufs& b,
-                       uint32_t* const kbuf[2], uint32_t* const vbuf[2], uint16_t* const pbuf[2], int cur) {
+                       uint32_t* const kbuf[2], uint32_t* const vbuf[2], uint32_t* const pbuf[2], int cur) {
   constexpr uint32_t NB = 1u << kBits;
   static_assert(NB <= RS_DIGITS, "digit totals");
   hipLaunchKernelGGL((k_radix_hist<kBits>), dim3(ntiles), dim3(RS_TPB), 0, st, kbuf[cur], n, shift, b.hist, ntiles);
@@ -518,14 +523,14 @@ static void sort_and_finish(hipStream_t st, const DevColl& c, uint32_t n, uint32
   const uint32_t ntiles = radix_tiles(n);
   uint32_t* const kbuf[2] = {b.keys[0], b.keys[1]};
   uint32_t* const vbuf[2] = {b.vals, b.set_ids};
-  uint16_t* const pbuf[2] = {b.pos, b.positions};
+  uint32_t* const pbuf[2] = {b.pos, b.metas};
   for (uint32_t p = 0; p < plan.passes; ++p) {
     const uint32_t shift = plan.bits * p;
     radix_pass<8>(st, n, shift, ntiles, b, kbuf, vbuf, pbuf, cur);
     cur ^= 1;
   }
   hipExtLaunchKernelGGL(k_index_finish, dim3((n + 1 + 255) / 256), dim3(256), 0, st, nullptr, done, 0, kbuf[cur], n,
-                        universe, b.offsets, c, b.set_ids, b.positions, sigs, b.ent, counters);
+                        universe, b.offsets, b.set_ids, b.metas, sigs, b.ent, counters);
   HS_CHECK_HIP(hipGetLastError());
 }
 
@@ -536,7 +541,7 @@ static int emit_buffer(uint32_t passes) { return (passes % 2 == 1) ? 0 : 1; }
 void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                       const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                       const IndexBufs& b, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
-                      hipEvent_t done) {
+                      hipEvent_t done, const uint32_t* flag_len) {
   HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && first <= last && n > 0) {
@@ -544,10 +549,10 @@ void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t
     const int cur = emit_buffer(plan.passes);
     uint32_t* kb = cur ? b.keys[1] : b.keys[0];
     uint32_t* vb = cur ? b.set_ids : b.vals;
-    uint16_t* pb = cur ? b.positions : b.pos;
+    uint32_t* pb = cur ? b.metas : b.pos;
     const uint32_t nsets = last - first + 1;
     hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
-                       size_first, cum, kb, vb, pb, sigs_out);
+                       size_first, cum, kb, vb, pb, sigs_out, flag_len);
     sort_and_finish(st, c, n, universe, b, cur, plan, sigs, counters, done);
   } else {
     launch_fill_u32(st, b.offsets, (size_t)universe + 1, 0);
@@ -559,11 +564,14 @@ void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
-                 hipEvent_t done) {
-  const IndexBufs b = prepare(ib, universe, entries);
+                 hipEvent_t done, const uint32_t* flag_len) {
+  // flagged keys run over twice the universe
+  const uint32_t key_universe = flag_len ? 2 * universe : universe;
+  HS_REQUIRE(!flag_len || universe < 0x80000000u, HS_E_UNSUPPORTED, "token-major index limited to 2^31 tokens");
+  const IndexBufs b = prepare(ib, key_universe, entries);
   ib.ent_words = sigs ? 4u : 2u;
-  build_index_into(st, c, first, last, prefix_len, size_first, cum, universe, entries, b, sigs, sigs_out, counters,
-                   done);
+  build_index_into(st, c, first, last, prefix_len, size_first, cum, key_universe, entries, b, sigs, sigs_out,
+                   counters, done, flag_len);
 }
 
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
@@ -579,7 +587,7 @@ void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_fi
     const int cur = emit_buffer(plan.passes);
     uint32_t* kb = cur ? b.keys[1] : b.keys[0];
     uint32_t* vb = cur ? b.set_ids : b.vals;
-    uint16_t* pb = cur ? b.positions : b.pos;
+    uint32_t* pb = cur ? b.metas : b.pos;
     hipLaunchKernelGGL(k_index_emit_batch, dim3((c.n + 255) / 256), dim3(256), 0, st, c, join_first, n_joins,
                        prefix_len, size_first, sf_base, cum, kb, vb, pb, sigs_out);
     sort_and_finish(st, c, n, universe, b, cur, plan, sigs, counters);

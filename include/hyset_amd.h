@@ -203,19 +203,29 @@ typedef struct {
  * the upload-time and fixed tile plans and the bitmap technique; the
  * balanced and tile-queue plans and the hybrid technique run eagerly. */
 enum { HS_TUNE_GRAPH = 1 };
-/* hs_join fgss tiles.  HS_TUNE_STATIC_TILES: one workgroup per tile of the
- * collection's upload-time plan (at most 512 probes and 8192 tokens).
+/* hs_join fgss plans.  A self-join of one id range with itself (indexed ==
+ * probe, equal ranges) runs token-major by default (HS_PLAN_TOKEN): the
+ * index is built over the probe prefixes, sorted by (token, set id), and
+ * one lane per index entry walks back through its token's list over the
+ * probe's length window, the survivors of the positional filter and the
+ * signature test queued and verified by a second kernel.  Every other join,
+ * and any join with one of the flags below or probes_per_tile set, runs
+ * item-major over an index of the index prefixes: a workgroup per tile of
+ * probes searches each prefix token's posting list for the probe's window.
+ * HS_TUNE_STATIC_TILES: one workgroup per tile of the collection's
+ * upload-time plan (at most 512 probes and 8192 tokens).
  * HS_TUNE_TILE_QUEUE: the same tiles claimed by a persistent grid
  * (workgroups_per_cu per CU) from per-XCD queues.  HS_TUNE_BALANCED_TILES:
  * tiles of equal estimated work, planned on the device after the index
  * build by a pass that locates every probe-prefix item's posting-list
- * slice, claimed by the persistent grid.  Default (none of these): the
- * balanced plan for joins of at least 2^21 probe-prefix items whose index
- * holds at least 3.5 entries per token of the universe (long posting lists,
- * so tiles differ much in work), the upload-time tiles otherwise. */
-enum { HS_TUNE_TILE_QUEUE = 2, HS_TUNE_BALANCED_TILES = 8, HS_TUNE_STATIC_TILES = 16 };
+ * slice, claimed by the persistent grid.  HS_TUNE_ITEM_MAJOR: item-major
+ * with the automatic tile plan: the balanced plan for joins of at least
+ * 2^21 probe-prefix items whose index holds at least 3.5 entries per token
+ * of the universe (long posting lists, so tiles differ much in work), the
+ * upload-time tiles otherwise. */
+enum { HS_TUNE_TILE_QUEUE = 2, HS_TUNE_BALANCED_TILES = 8, HS_TUNE_STATIC_TILES = 16, HS_TUNE_ITEM_MAJOR = 64 };
 /* hs_stats.tile_plan: the plan a fgss join ran with. */
-enum { HS_PLAN_FIXED = 1, HS_PLAN_UPLOAD = 2, HS_PLAN_QUEUE = 3, HS_PLAN_BALANCED = 4 };
+enum { HS_PLAN_FIXED = 1, HS_PLAN_UPLOAD = 2, HS_PLAN_QUEUE = 3, HS_PLAN_BALANCED = 4, HS_PLAN_TOKEN = 5 };
 /* fgss: no signature test (every visit passing the positional filter is
  * verified by a merge, as in the reference's fgssjoin).  The pair set is the
  * same either way. */

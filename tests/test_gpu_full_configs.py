@@ -56,11 +56,13 @@ def test_headline_fgss_t06(ctx, headline):
     assert want.count > 1_000_000
     _check(ctx.join(d, algo="fgss"), want, "fgss 1M t=0.6")
     st = ctx.stats()
-    assert st["tile_plan"] == H.PLAN_BALANCED
+    assert st["tile_plan"] == H.PLAN_TOKEN
     assert st["posting_visits"] == want.posting_visits
     sizes, counts = np.unique(c.sizes, return_counts=True)
     assert st["index_entries"] == sum(int(k) * O.bounds("jaccard", 0.6, int(l))[4] for l, k in zip(sizes, counts))
-    # the other plans at full size
+    # the item-major plans at full size (the automatic one is balanced here)
+    _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_ITEM_MAJOR)), want, "fgss 1M t=0.6 item-major")
+    assert ctx.stats()["tile_plan"] == H.PLAN_BALANCED
     for flags in (H.TUNE_STATIC_TILES, H.TUNE_TILE_QUEUE):
         _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=flags)), want, f"fgss 1M t=0.6 flags={flags}")
 

@@ -33,23 +33,28 @@ def test_fgss_small(ctx, small, t, sim):
     _check(ctx.join(d, algo="fgss"), want, f"fgss {sim} {t}")
 
 
-@pytest.mark.parametrize("tune", ["default", "static", "queue", "balanced", "ppt64", "ppt1", "wpc1"])
+@pytest.mark.parametrize("tune", ["default", "item", "static", "queue", "balanced", "ppt64", "ppt1", "wpc1"])
 @pytest.mark.parametrize("t", [0.6, 0.8])
 def test_fgss_tile_plans(ctx, small, t, tune):
-    """Every tile plan gives the oracle's result: the automatic choice, one
-    workgroup per upload-time tile, the same tiles on a persistent grid
-    (also a small one), the work-balanced plan, fixed-size tiles; whole
-    range and sub-ranges."""
+    """Every plan gives the oracle's result: the automatic choice
+    (token-major for a self-join of one range), item-major with its
+    automatic tiles, one workgroup per upload-time tile, the same tiles on a
+    persistent grid (also a small one), the work-balanced plan, fixed-size
+    tiles; whole range and sub-ranges."""
     c, d = small
     ctx.set_threshold(t)
     want = O.self_join(c, t)
-    tu = {"default": H.Tune(), "static": H.Tune(flags=H.TUNE_STATIC_TILES), "queue": H.Tune(flags=H.TUNE_TILE_QUEUE),
-          "balanced": H.Tune(flags=H.TUNE_BALANCED_TILES), "ppt64": H.Tune(probes_per_tile=64),
-          "ppt1": H.Tune(probes_per_tile=1), "wpc1": H.Tune(flags=H.TUNE_TILE_QUEUE, workgroups_per_cu=1)}[tune]
+    tu = {"default": H.Tune(), "item": H.Tune(flags=H.TUNE_ITEM_MAJOR), "static": H.Tune(flags=H.TUNE_STATIC_TILES),
+          "queue": H.Tune(flags=H.TUNE_TILE_QUEUE), "balanced": H.Tune(flags=H.TUNE_BALANCED_TILES),
+          "ppt64": H.Tune(probes_per_tile=64), "ppt1": H.Tune(probes_per_tile=1),
+          "wpc1": H.Tune(flags=H.TUNE_TILE_QUEUE, workgroups_per_cu=1)}[tune]
     _check(ctx.join(d, algo="fgss", tune=tu), want, f"fgss {tune} {t}")
-    plan = {"default": H.PLAN_UPLOAD, "static": H.PLAN_UPLOAD, "queue": H.PLAN_QUEUE, "balanced": H.PLAN_BALANCED,
-            "ppt64": H.PLAN_FIXED, "ppt1": H.PLAN_FIXED, "wpc1": H.PLAN_QUEUE}[tune]
-    assert ctx.stats()["tile_plan"] == plan  # 3000 sets: too few items for the automatic balanced plan
+    plan = {"default": H.PLAN_TOKEN, "item": H.PLAN_UPLOAD, "static": H.PLAN_UPLOAD, "queue": H.PLAN_QUEUE,
+            "balanced": H.PLAN_BALANCED, "ppt64": H.PLAN_FIXED, "ppt1": H.PLAN_FIXED, "wpc1": H.PLAN_QUEUE}[tune]
+    st = ctx.stats()
+    assert st["tile_plan"] == plan  # 3000 sets: too few items for the automatic balanced plan
+    # every plan counts the same work: postings inside the windows, the index's entries
+    assert st["posting_visits"] == want.posting_visits and st["index_entries"] == want.index_entries
     for rng in ((0, 0), (1, 1), (2999, 2999), (100, 2500)):
         want = O.self_join_blocks(c, t, rng, rng)
         _check(ctx.join(d, algo="fgss", tune=tu, indexed_range=rng, probe_range=rng), want, f"fgss {tune} {rng}")
@@ -167,8 +172,14 @@ def test_fgss_config2_full(ctx, t):
         want = O.cpu_multi(c, t, 8)
         _check(ctx.join(d, algo="fgss"), want, f"fgss 1M t={t}")
         st = ctx.stats()
-        # the automatic plan: 4.1 index entries per token at t = 0.7, 2.9 and 1.8 above
-        assert st["tile_plan"] == (H.PLAN_BALANCED if t == 0.7 else H.PLAN_UPLOAD)
+        assert st["tile_plan"] == H.PLAN_TOKEN
+        # item-major with its automatic tiles: 4.1 index entries per token at
+        # t = 0.7 (balanced tiles), 2.9 and 1.8 above (upload-time tiles)
+        _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_ITEM_MAJOR)), want, f"fgss 1M item-major t={t}")
+        sti = ctx.stats()
+        assert sti["tile_plan"] == (H.PLAN_BALANCED if t == 0.7 else H.PLAN_UPLOAD)
+        for k in ("posting_visits", "candidates", "signature_tests", "index_entries", "probe_prefix_tokens"):
+            assert sti[k] == st[k], k
         if t == 0.8:
             _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_TILE_QUEUE)), want, "fgss 1M tile queue")
             _check(ctx.join(d, algo="fgss", tune=H.Tune(flags=H.TUNE_BALANCED_TILES)), want, "fgss 1M balanced")
