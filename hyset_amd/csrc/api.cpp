@@ -586,13 +586,21 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
       ctx->jcum_host = cum;
       ctx->jcum_epoch = hs::buffer_epoch();
     }
-    hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
-                    ctx->d_jcum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
-                    P.sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr,
-                    sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
-                    algo == HS_ALGO_FGSS ? ctx->counters.as<unsigned long long>() : nullptr,
-                    timed ? ctx->ev[1] : nullptr, P.token_major ? ctx->dev_bounds.midprefix : nullptr);
-    P.counters_ready = algo == HS_ALGO_FGSS && acc > 0 && indexed->n > 0;
+    if (P.token_major) {
+      hs::build_token_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
+                            ctx->d_jcum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
+                            ctx->dev_bounds.midprefix, sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
+                            ctx->counters.as<unsigned long long>(), timed ? ctx->ev[1] : nullptr);
+      P.counters_ready = true;
+    } else {
+      hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
+                      ctx->d_jcum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
+                      P.sig_words == 1 ? indexed->sigs.as<uint64_t>() : nullptr,
+                      sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
+                      algo == HS_ALGO_FGSS ? ctx->counters.as<unsigned long long>() : nullptr,
+                      timed ? ctx->ev[1] : nullptr);
+      P.counters_ready = algo == HS_ALGO_FGSS && acc > 0 && indexed->n > 0;
+    }
   } else {
     ctx->ib.universe = 0;
     if (timed) HS_CHECK_HIP(hipEventRecord(ctx->ev[1], st));
@@ -614,11 +622,10 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
   } else if (algo == HS_ALGO_FGSS && P.token_major) {
     hs::TokArgs a{};
     a.c = probe->dev();
-    a.keys = ctx->ib.keys[1].as<uint32_t>();  // the last radix pass leaves the sorted keys here
-    a.offsets = ctx->ib.offsets.as<uint32_t>();
+    // the last radix pass leaves the sorted keys, ids and metas here
+    a.keys = ctx->ib.keys[1].as<uint32_t>();
     a.ids = ctx->ib.set_ids.as<uint32_t>();
-    a.ent = ctx->ib.ent.as<uint32_t>();
-    a.ent_words = ctx->ib.ent_words;
+    a.metas = ctx->ib.metas.as<uint32_t>();
     a.n_ent = (uint32_t)P.index_entries;
     a.bounds = ctx->dev_bounds;
     a.size_first = indexed->size_first.as<uint32_t>();
@@ -626,7 +633,7 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     a.first = pr.first;
     a.last = pr.last;
     a.sig_words = P.sig_words;
-    a.sigs = P.sig_words > 1 ? indexed->sigs.as<uint64_t>() : nullptr;
+    a.sigs = P.sig_words ? indexed->sigs.as<uint64_t>() : nullptr;
     // survivors of both filters: a first guess of half the entries, grown
     // after a join that found more (the join then runs again)
     if (ctx->tok_queue_cap == 0) {
@@ -907,7 +914,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     // counters cleared and the index size placed where the kernels read it
     auto reset_for_launch = [&](const hs::DevicePlan& Q) {
       reset_counters(ctx);
-      if (algo == HS_ALGO_FGSS && !range_empty(Q.ir))  // index size, read back with the counters
+      if (algo == HS_ALGO_FGSS && !Q.token_major && !range_empty(Q.ir))  // index size, read back with the counters
         HS_CHECK_HIP(hipMemcpyAsync(ctx->counters.as<unsigned long long>() + hs::CNT_ENTRIES,
                                     ctx->ib.offsets.as<uint32_t>() + ctx->ib.universe, 4, hipMemcpyDeviceToDevice,
                                     st));

@@ -970,20 +970,14 @@ struct TokCounts {
   uint32_t visits = 0, items = 0, tests = 0;
 };
 
+// Entry e of the sorted index: its set id, meta and (64-bit signature
+// tests) the set's signature, gathered from the per-set table.
 template <int kW>
 __device__ __forceinline__ void tok_record(const TokArgs& A, uint32_t e, uint32_t& sid, uint32_t& meta,
                                            uint64_t& sig) {
-  if constexpr (kW == 1) {
-    const uint4 v = reinterpret_cast<const uint4*>(A.ent)[e];
-    sid = v.x;
-    meta = v.y;
-    sig = ((uint64_t)v.w << 32) | v.z;
-  } else {
-    const uint2 v = *reinterpret_cast<const uint2*>(A.ent + (size_t)e * A.ent_words);
-    sid = v.x;
-    meta = v.y;
-    sig = 0;
-  }
+  sid = A.ids[e];
+  meta = A.metas[e];
+  sig = kW == 1 ? A.sigs[sid] : 0;
 }
 
 // Workgroup state of a token-major workgroup: the bound tables (window
@@ -1201,8 +1195,26 @@ __device__ __forceinline__ uint32_t tok_lb_back(const TokArgs& A, uint32_t hi, u
 // other token of the tile lies inside it whole.  hdr = {first key of the
 // tile, first entry of that key's index part}.
 __device__ __forceinline__ uint2 tok_header(const TokArgs& A, uint32_t t) {
-  const uint32_t k0 = A.keys[t * TOK_BLOCK] & ~1u;
-  return make_uint2(k0, A.offsets[k0]);
+  const uint32_t i = t * TOK_BLOCK;
+  const uint32_t k0 = A.keys[i] & ~1u;
+  // first position with key >= k0: gallop back from i, then bisect
+  uint32_t hi = i, step = 1, lo;
+  for (;;) {
+    const uint32_t j = hi >= step ? hi - step : 0u;
+    if (A.keys[j] < k0) {
+      lo = j + 1;
+      break;
+    }
+    hi = j;
+    if (j == 0) return make_uint2(k0, 0u);
+    step <<= 1;
+  }
+  while (lo < hi) {
+    const uint32_t m = (lo + hi) >> 1;
+    if (A.keys[m] < k0) lo = m + 1;
+    else hi = m;
+  }
+  return make_uint2(k0, lo);
 }
 
 // Stages tile t's window from its header; called by every lane between
@@ -1435,10 +1447,7 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_verify(TokArgs A) {
 }
 
 void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEvent_t t0, hipEvent_t t1) {
-  HS_REQUIRE(a.sig_words == 0 || a.sig_words == 1 || (a.sigs && a.ent_words == 2), HS_E_INVALID,
-             "token-major join: signature table missing");
-  HS_REQUIRE(a.sig_words != 1 || a.ent_words == 4, HS_E_INVALID,
-             "token-major join: 64-bit signature test needs records with signatures");
+  HS_REQUIRE(a.sig_words == 0 || a.sigs, HS_E_INVALID, "token-major join: signature table missing");
   const dim3 b(TK_TPB);
   const uint32_t n_tiles = std::max<uint32_t>((a.n_ent + TOK_BLOCK - 1) / TOK_BLOCK, 1);
   // persistent passes: as many workgroups as the CUs hold at once

@@ -192,6 +192,14 @@ void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out = nullptr,
                  unsigned long long* counters = nullptr, hipEvent_t done = nullptr,
                  const uint32_t* flag_len = nullptr);
+// The token-major index: flagged keys (flag_len = midprefix), sorted with
+// their set ids and metas into (keys[1], set_ids, metas) and nothing more:
+// no offsets, no records (the token-major join reads the sorted arrays).
+// counters (may be NULL) are zeroed; done (may be NULL) takes the end.
+void build_token_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
+                       const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
+                       IndexBuild& ib, const uint32_t* flag_len, uint64_t* sigs_out,
+                       unsigned long long* counters, hipEvent_t done = nullptr);
 // The same into buffers the caller laid out (index_region); allocates nothing.
 // flag_len (may be NULL; indexed by set size): the token-major index, keys
 // (token << 1) | (position >= flag_len[|s|]) over 2 x universe (pass that
@@ -296,8 +304,9 @@ void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp);
 // id): each token's list holds its index entries (the index prefixes)
 // first, then its other probe entries, both ascending in id.  keys[e] is
 // entry e's key and ent its record (ent_words words: set id, meta and, with
-// 64-bit signatures, the set's signature); offsets[k] is the first entry of
-// key k.  The visits of a probe entry are the index entries of its token
+// 64-bit signatures, the set's signature) -- here read from the sort's own
+// arrays: ids[e] and metas[e], the signature from the per-set table.  The
+// visits of a probe entry are the index entries of its token
 // with ids in its length window below the probe: one run of the token's
 // index part (ids ascend, and so do sizes).  A workgroup takes a tile of
 // TOK_BLOCK entries and stages in LDS the window of the index from the
@@ -314,18 +323,16 @@ constexpr uint32_t TOK_WINDOW = 1024;
 constexpr uint32_t TOK_CHUNK = 4096;
 struct TokArgs {
   DevColl c;
-  const uint32_t* keys;
-  const uint32_t* offsets;  // first entry of each key, 2 x universe + 1 of them
-  const uint32_t* ids;      // set id of each entry (the records' first words, compact)
-  const uint32_t* ent;
-  uint32_t ent_words;
+  const uint32_t* keys;   // key of each sorted entry
+  const uint32_t* ids;    // its set id
+  const uint32_t* metas;  // its meta (size << 16 | position, both saturated at 0xFFFF)
   uint32_t n_ent;
   DevBounds bounds;
   const uint32_t* size_first;  // first id of size >= l, l in [0, maxlen + 1]
   uint32_t size_first_len;     // maxlen + 2
   uint32_t first, last;
-  uint32_t sig_words;          // 0: no signature test; 1: in the records; 2-16: from sigs
-  const uint64_t* sigs;        // per-set signatures, sig_words words each (sig_words > 1)
+  uint32_t sig_words;          // 0: no signature test, else 1-16 words
+  const uint64_t* sigs;        // per-set signatures, sig_words words each
   uint4* queue;
   uint32_t queue_cap;
   uint4* heavy;

@@ -561,6 +561,43 @@ void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t
   HS_CHECK_HIP(hipGetLastError());
 }
 
+__global__ void k_fill_counters(unsigned long long* __restrict__ counters) {
+  if (threadIdx.x < CNT_WORDS) counters[threadIdx.x] = 0;
+}
+
+void build_token_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
+                       const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
+                       IndexBuild& ib, const uint32_t* flag_len, uint64_t* sigs_out,
+                       unsigned long long* counters, hipEvent_t done) {
+  HS_REQUIRE(universe < 0x80000000u, HS_E_UNSUPPORTED, "token-major index limited to 2^31 tokens");
+  const uint32_t key_universe = 2 * universe;
+  const IndexBufs b = prepare(ib, key_universe, entries);
+  ib.ent_words = 0;  // no records
+  HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
+  const uint32_t n = (uint32_t)entries;
+  if (counters) hipLaunchKernelGGL(k_fill_counters, dim3(1), dim3(64), 0, st, counters);
+  if (c.n > 0 && first <= last && n > 0) {
+    const RadixPlan plan = radix_plan(key_universe);
+    int cur = emit_buffer(plan.passes);
+    uint32_t* kb = cur ? b.keys[1] : b.keys[0];
+    uint32_t* vb = cur ? b.set_ids : b.vals;
+    uint32_t* pb = cur ? b.metas : b.pos;
+    const uint32_t nsets = last - first + 1;
+    hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
+                       size_first, cum, kb, vb, pb, sigs_out, flag_len);
+    const uint32_t ntiles = radix_tiles(n);
+    uint32_t* const kbuf[2] = {b.keys[0], b.keys[1]};
+    uint32_t* const vbuf[2] = {b.vals, b.set_ids};
+    uint32_t* const pbuf[2] = {b.pos, b.metas};
+    for (uint32_t p = 0; p < plan.passes; ++p) {
+      radix_pass<8>(st, n, plan.bits * p, ntiles, b, kbuf, vbuf, pbuf, cur);
+      cur ^= 1;
+    }
+  }
+  if (done) HS_CHECK_HIP(hipEventRecord(done, st));
+  HS_CHECK_HIP(hipGetLastError());
+}
+
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
