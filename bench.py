@@ -6,7 +6,8 @@ by ascending frequency, sets sorted by size, seed 1; Jaccard threshold 0.8
 with the reference's double-precision bounds; prefix-filter (fgss)
 technique.  A step is one full join from the device-resident CSR collection:
 index build + candidate generation + verification, result pairs left in HBM
-(their count read back).  Prints one JSON line on stdout; progress goes to
+(their count read back); `value_with_d2h` in the extra fields is the same
+step with every result triple copied into host memory.  Prints one JSON line on stdout; progress goes to
 stderr.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
@@ -32,18 +33,53 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def fgss_bytes(st: dict, n_probes: int, n_probe_tokens: int) -> int:
-    """Algorithmic HBM bytes of one k_fgss launch: probe tokens (4 B each),
-    a 16 B record and the signature (W words of 8 B) per probe, per
-    probe-prefix item its posting-list slice (8 B), per posting visited its
-    index record (set id, size/position word and, with 64-bit signatures,
-    the set's signature: 16 B; else 8 B, and per signature test the indexed
-    set's W-word signature), per verified candidate its start and size plus
-    4 B per token, 12 B per result triple."""
+def stage_bytes(st: dict, n_sets: int, universe: int, n_tokens: int) -> dict:
+    """Algorithmic HBM bytes of the join's stages, per unit of work counted
+    by the join itself:
+
+    * index build, per index entry: B_idx = 8 P_idx + 8 n + 8 (U + 1) (read
+      the prefix token and write the posting's set id; read each set's start
+      and size; write each token's count and offset);
+    * signatures, per token: B_sig = 4 N + 8 n + (b / 8) n;
+    * candidate generation, per posting visited: B_cand = 12 P_probe + 4 E +
+      12 C (each probe-prefix token with the two offsets around it, 4 B per
+      posting inside the length window, a 12 B candidate record);
+    * verification, per candidate: B_ver = 12 C + 4 sum_cand |s| +
+      4 sum_{r with a candidate} |r| + 12 K (the candidate record, both
+      sets' tokens, one result triple per pair)."""
+    P, E, C, K = st["probe_prefix_tokens"], st["posting_visits"], st["candidates"], st["pairs"]
+    b = st["signature_bits"]
+    return {
+        "index": 8 * st["index_entries"] + 8 * n_sets + 8 * (universe + 1),
+        "signatures": 4 * n_tokens + 8 * n_sets + (b // 8) * n_sets if b else 0,
+        "candidates": 12 * P + 4 * E + 12 * C,
+        "verify": 12 * C + 4 * st["candidate_tokens"] + 4 * st["candidate_probe_tokens"] + 12 * K,
+    }
+
+
+def token_join_design_bytes(st: dict) -> int:
+    """What the token-major join kernels read and write by design, each
+    datum counted once: every entry of the token-major index (the probe
+    prefixes: key, set id, size/position word, 12 B) staged by the filter
+    with its set's signature (W words of 8 B); per surviving candidate a
+    16 B queue record written and read back, the two sets' starts and sizes
+    (16 B) and tokens; 12 B per result triple.  Staging windows that reach
+    back into a long posting list and the second pass over heavy tiles
+    re-read entries beyond this."""
+    P, C, K = st["probe_prefix_tokens"], st["candidates"], st["pairs"]
     w = st["signature_bits"] // 64
-    visit = 16 * st["posting_visits"] if w == 1 else 8 * st["posting_visits"] + 8 * w * st["signature_tests"]
-    return (4 * n_probe_tokens + (16 + 8 * w) * n_probes + 8 * st["probe_prefix_tokens"] + visit
-            + 8 * st["candidates"] + 4 * st["candidate_tokens"] + 12 * st["pairs"])
+    return ((12 + 8 * w) * P + 48 * C + 4 * st["candidate_tokens"] + 4 * st["candidate_probe_tokens"]
+            + 12 * K)
+
+
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def bench_coprocess(ctx, dI, dP, name, threads, want, reps=3) -> dict:
@@ -381,8 +417,9 @@ def main() -> int:
     st = steps[-1]
     dev_ms = statistics.median(s["total_ms"] for s in steps)
     # The same steps again with the stage events (HS_TUNE_STAGE_TIMES): the
-    # index build's end and the k_fgss dispatch's start and end, taken by the
-    # dispatches themselves on the join's stream.  They cost the device a
+    # index build's end and the join kernels' start and end (the token-major
+    # filter's start to the verification's end), taken by the dispatches
+    # themselves on the join's stream.  They cost the device a
     # few microseconds per join, so the throughput above runs without them.
     stune = H.Tune(flags=H.TUNE_STAGE_TIMES)
     ctx.join(d, algo="fgss", tune=stune, count_only=True)
@@ -397,7 +434,7 @@ def main() -> int:
     kern_ms = statistics.median(s["kernel_ms"] for s in ssteps)
     idx_ms = statistics.median(s["index_ms"] for s in ssteps)
     log(f"fgss t={t}: {pairs} pairs, {ms_per_step:.3f} ms/step wall, device {dev_ms:.3f} ms; with stage events "
-        f"{ms_per_step_stages:.3f} ms/step (index {idx_ms:.3f}, k_fgss {kern_ms:.3f})")
+        f"{ms_per_step_stages:.3f} ms/step (index {idx_ms:.3f}, join kernels {kern_ms:.3f})")
 
     # the same join replayed from a HIP graph (HS_TUNE_GRAPH): every kernel
     # runs each step; the host enqueues one graph instead of the sequence
@@ -417,55 +454,90 @@ def main() -> int:
     log(f"fgss t={t} graph replay: {ms_per_step_graph:.3f} ms/step wall, device {dev_ms_graph:.3f} ms "
         f"(replayed every step: {replayed})")
 
-    # with the explicit device-to-host copy of the pairs
-    d2h = []
-    for _ in range(5):
-        a = time.perf_counter()
+    # with every result triple copied into host memory inside the timed
+    # region (the pairs' device-to-host copy rides with the counters' copy
+    # when the result fits the previous join's size)
+    for _ in range(2):
+        ctx.join(d, algo="fgss")
+    n_d2h = max(5, args.steps // 4)
+    a = time.perf_counter()
+    for _ in range(n_d2h):
         r, s, ov = ctx.join(d, algo="fgss")
-        d2h.append((time.perf_counter() - a) * 1e3)
-    ms_with_d2h = statistics.median(d2h)
+    ms_with_d2h = (time.perf_counter() - a) * 1e3 / n_d2h
+    assert r.shape[0] == pairs
+    dev_ms_d2h = ctx.stats()["total_ms"]
+    log(f"fgss t={t} with the pairs in host memory: {ms_with_d2h:.3f} ms/step wall, device {dev_ms_d2h:.3f} ms")
 
-    alg = fgss_bytes(st, coll.n, int(coll.tokens.shape[0]))
+    sst = ssteps[-1]  # stage-timed: also counts |r| over the probes with candidates
+    tok = sst["tile_plan"] == H.PLAN_TOKEN
+    sb = stage_bytes(sst, coll.n, coll.universe, int(coll.tokens.shape[0]))
+    alg = sb["candidates"] + sb["verify"]
     achieved = alg / (kern_ms * 1e-3) / 1e9
+    e2e = sum(sb.values())
+    e2e_gbps = e2e / (dev_ms * 1e-3) / 1e9
+    design = token_join_design_bytes(sst) if tok else None
     traffic = None
-    tfile = ROOT / "profiles" / "k_fgss_1M_jaccard0.8_traffic.json"
-    if tfile.exists():
+    tfile = ROOT / "profiles" / "tok_join_1M_jaccard0.8_traffic.json"
+    if tok and tfile.exists():
         traffic = json.loads(tfile.read_text()).get("hbm_bytes_per_launch")
+    log(f"stage bytes {sb}; join kernels {achieved:.0f} GB/s, end to end {e2e_gbps:.0f} GB/s")
 
     extra = {"ms_per_step_graph": round(ms_per_step_graph, 4), "join_ms_device_graph": round(dev_ms_graph, 4),
              "graph_replayed_every_step": replayed,
-             "join_ms_device": round(dev_ms, 4), "index_ms": round(idx_ms, 4), "k_fgss_ms": round(kern_ms, 4),
+             "join_ms_device": round(dev_ms, 4), "index_ms": round(idx_ms, 4), "join_kernels_ms": round(kern_ms, 4),
              "ms_per_step_stage_events": round(ms_per_step_stages, 4),
-             "join_ms_with_d2h": round(ms_with_d2h, 4), "pairs": int(pairs),
-             "candidates": int(st["candidates"]), "posting_visits": int(st["posting_visits"]),
-             "index_entries": int(st["index_entries"]), "probe_prefix_tokens": int(st["probe_prefix_tokens"]),
-             "signature_tests": int(st["signature_tests"]), "signature_bits": int(st["signature_bits"]),
-             "tile_plan": int(st["tile_plan"]),
+             "value_with_d2h": round(pairs / (ms_with_d2h * 1e-3), 1), "ms_per_step_with_d2h": round(ms_with_d2h, 4),
+             "join_ms_device_with_d2h": round(dev_ms_d2h, 4), "pairs": int(pairs),
+             "end_to_end": {"algorithmic_bytes": int(e2e), "stage_bytes": {k: int(v) for k, v in sb.items()},
+                            "ms": round(dev_ms, 4), "achieved_gbps": round(e2e_gbps, 1),
+                            "frac": round(e2e_gbps / HBM_PEAK_GBPS, 4)},
+             "candidates": int(sst["candidates"]), "posting_visits": int(sst["posting_visits"]),
+             "index_entries": int(sst["index_entries"]), "probe_prefix_tokens": int(sst["probe_prefix_tokens"]),
+             "candidate_tokens": int(sst["candidate_tokens"]),
+             "candidate_probe_tokens": int(sst["candidate_probe_tokens"]),
+             "signature_tests": int(sst["signature_tests"]), "signature_bits": int(sst["signature_bits"]),
+             "tile_plan": int(sst["tile_plan"]),
              "hbm_copy_kernel_gbps": round(copy_gbps, 1)}
 
     cpu = None
     if not args.quick:
         import oracle as O
         # parity of the measured result against the oracle (cpu_single restated)
-        times = []
-        for _ in range(3):
-            a = time.perf_counter()
-            want = O.self_join(coll, t)
-            times.append(time.perf_counter() - a)
         r, s, ov = H.sort_pairs(r, s, ov)
+        want = O.self_join(coll, t)  # warm-up
         extra["parity_vs_oracle"] = bool(np.array_equal(r, want.r) and np.array_equal(s, want.s)
                                          and np.array_equal(ov, want.overlap))
-        cpu_s = statistics.median(times)
-        cpu = {"value": round(want.count / cpu_s, 1), "unit": "pairs/s", "cores": 1, "kind": "port",
-               "sample": f"full workload: cpu_single AllPairs restated in C (index build + join), "
-                         f"1M sets, t={t}, median of 3 = {cpu_s * 1e3:.1f} ms"}
-        ncores = len(os.sched_getaffinity(0))
-        a = time.perf_counter()
-        O.cpu_multi(coll, t, ncores)
-        extra["cpu_multi_ms"] = round((time.perf_counter() - a) * 1e3, 1)
-        extra["cpu_multi_threads"] = ncores
-        extra["cpu_single_ms"] = round(cpu_s * 1e3, 1)
-        log(f"oracle: {want.count} pairs in {cpu_s * 1e3:.0f} ms (1 core); parity={extra['parity_vs_oracle']}")
+        # the reference's intervals on a steady clock: "join time" without
+        # the index build, "total time without I/O" with it; median of 5
+        # after the warm-up
+        runs = []
+        for _ in range(5):
+            w = O.self_join(coll, t)
+            assert w.count == want.count
+            runs.append((w.index_ms, w.join_ms))
+        join_ms = statistics.median(j for _, j in runs)
+        total_ms = statistics.median(i + j for i, j in runs)
+        ncores = max(1, min(16, len(os.sched_getaffinity(0))))  # the box's CPU share for one GPU
+        model = cpu_model()
+        cpu = {"value": round(want.count / (total_ms * 1e-3), 1), "unit": "pairs/s", "cores": 1, "kind": "port",
+               "join_ms": round(join_ms, 2), "total_ms": round(total_ms, 2), "cpu_model": model,
+               "nproc": os.cpu_count(), "usable_cpus": ncores,
+               "sample": f"full workload: cpu_single (AllPairs restated in C), 1M sets, t={t}; steady clock, median "
+                         f"of 5 after 1 warm-up: join time {join_ms:.1f} ms, total without I/O (index build + "
+                         f"join) {total_ms:.1f} ms; value = pairs / total; {model}, nproc {os.cpu_count()}"}
+        O.cpu_multi(coll, t, ncores)  # warm-up
+        mruns = []
+        for _ in range(5):
+            a = time.perf_counter()
+            w = O.cpu_multi(coll, t, ncores)
+            mruns.append((w.index_ms, w.join_ms, (time.perf_counter() - a) * 1e3))
+        extra["cpu_multi"] = {"threads": ncores, "join_ms": round(statistics.median(m[1] for m in mruns), 2),
+                              "index_ms_max_worker": round(statistics.median(m[0] for m in mruns), 2),
+                              "wall_ms": round(statistics.median(m[2] for m in mruns), 2),
+                              "pairs_per_s": round(want.count / (statistics.median(m[2] for m in mruns) * 1e-3), 1)}
+        extra["cpu_single"] = {"join_ms": cpu["join_ms"], "total_ms": cpu["total_ms"]}
+        log(f"oracle: {want.count} pairs; cpu_single join {join_ms:.1f} ms, total {total_ms:.1f} ms (1 core); "
+            f"cpu_multi {extra['cpu_multi']}; parity={extra['parity_vs_oracle']}")
 
         # the other thresholds and techniques of the configuration
         for tt in (0.6, 0.7, 0.9):
@@ -519,11 +591,11 @@ def main() -> int:
                 ks.append(ctx.stats()["kernel_ms"])
             return round(statistics.median(ks), 4), round(statistics.median(ds), 4), ctx.stats()
 
-        for name, flag in (("static_tiles", H.TUNE_STATIC_TILES), ("tile_queue", H.TUNE_TILE_QUEUE),
-                           ("balanced_tiles", H.TUNE_BALANCED_TILES)):
+        for name, flag in (("item_major", H.TUNE_ITEM_MAJOR), ("static_tiles", H.TUNE_STATIC_TILES),
+                           ("tile_queue", H.TUNE_TILE_QUEUE), ("balanced_tiles", H.TUNE_BALANCED_TILES)):
             extra[f"k_fgss_{name}_ms"], extra[f"join_ms_device_{name}"], _ = plan_times(flag)
             log(f"{name}: k_fgss {extra[f'k_fgss_{name}_ms']} ms, join {extra[f'join_ms_device_{name}']} ms "
-                f"(default: k_fgss {kern_ms:.4f} ms, join {dev_ms:.4f} ms)")
+                f"(token-major: join kernels {kern_ms:.4f} ms, join {dev_ms:.4f} ms)")
         # the same join without the signature test (every positional survivor merged)
         extra["k_fgss_no_signature_ms"], extra["join_ms_device_no_signature"], sn = plan_times(H.TUNE_NO_SIGNATURE)
         extra["candidates_no_signature"] = int(sn["candidates"])
@@ -584,7 +656,9 @@ def main() -> int:
                    "sets": coll.n, "tokens": int(coll.tokens.shape[0]), "threshold": t, "algorithm": "fgss"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic,
-                     "kernel": "k_fgss", "algorithmic_bytes": int(alg)},
+                     "kernel": "token join (k_tok_filter, k_tok_heavy, k_tok_verify)" if tok else "k_fgss",
+                     "ms": round(kern_ms, 4), "algorithmic_bytes": int(alg),
+                     "design_bytes": int(design) if design is not None else None},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -77,7 +77,8 @@ class Stats(C.Structure):
                 ("index_entries", C.c_uint64), ("posting_visits", C.c_uint64),
                 ("candidates", C.c_uint64), ("pairs", C.c_uint64), ("probe_prefix_tokens", C.c_uint64),
                 ("candidate_tokens", C.c_uint64), ("signature_tests", C.c_uint64), ("signature_bits", C.c_uint32),
-                ("tile_plan", C.c_uint32), ("graph", C.c_uint32), ("reserved", C.c_uint32)]
+                ("tile_plan", C.c_uint32), ("graph", C.c_uint32), ("reserved", C.c_uint32),
+                ("candidate_probe_tokens", C.c_uint64)]
 
 
 class ServerConfig(C.Structure):
@@ -377,6 +378,7 @@ class Context:
             raise HysetError(rc, f"hs_create({device}) failed")
         self._h = h
         self._colls = []
+        self._last_pairs = 0
 
     def _check(self, rc: int, what: str):
         if rc != HS_OK:
@@ -398,19 +400,34 @@ class Context:
     def join(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, algo="fgss",
              indexed_range=None, probe_range=None, tune: Tune | None = None, count_only=False,
              capacity=None):
-        """Returns (r, s, overlap) arrays, or the pair count when count_only."""
+        """Returns (r, s, overlap) arrays, or the pair count when count_only.
+        The arrays are passed to hs_join sized for `capacity` pairs (default:
+        a quarter more than the last join returned, at least 4096), so a
+        result that fits reaches the host in the join's own round trip; a
+        larger one is fetched afterwards (hs_copy_last_result)."""
         probe = indexed if probe is None else probe
         a = ALGOS[algo] if isinstance(algo, str) else int(algo)
         ir = Range(*indexed_range) if indexed_range is not None else None
         pr = Range(*probe_range) if probe_range is not None else None
-        res = Result()
-        res.count_only = 1
-        self._check(_lib.hs_join(self._h, indexed._h, probe._h, C.byref(ir) if ir else None,
-                                 C.byref(pr) if pr else None, a, C.byref(tune) if tune else None, C.byref(res)),
-                    "hs_join")
+        args = (self._h, indexed._h, probe._h, C.byref(ir) if ir else None, C.byref(pr) if pr else None, a,
+                C.byref(tune) if tune else None)
         if count_only:
+            res = Result()
+            res.count_only = 1
+            self._check(_lib.hs_join(*args, C.byref(res)), "hs_join")
             return int(res.count)
-        return self.last_result(int(res.count))
+        cap = int(capacity) if capacity is not None else max(4096, self._last_pairs + self._last_pairs // 4)
+        r = np.empty(max(cap, 1), dtype=np.uint32)
+        s = np.empty_like(r)
+        ov = np.empty_like(r)
+        res = Result(0, cap, _ptr(r), _ptr(s), _ptr(ov), 0)
+        rc = _lib.hs_join(*args, C.byref(res))
+        n = int(res.count)
+        self._last_pairs = n
+        if rc == HS_E_OVERFLOW:
+            return self.last_result(n)
+        self._check(rc, "hs_join")
+        return r[:n], s[:n], ov[:n]
 
     def join_noalloc(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, r: int, s: int,
                      overlap: int, capacity: int, indexed_range=None, probe_range=None,

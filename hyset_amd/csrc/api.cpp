@@ -653,6 +653,17 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     a.heavy_cap = ctx->tok_heavy_cap;
     ctx->tok_aoff.ensure(std::max<uint64_t>(P.index_entries, 1) * sizeof(uint2));
     a.aoff = ctx->tok_aoff.as<uint2>();
+    if (ctx->stage_times && !ctx->capturing_join) {
+      // stage-timed joins also count |r| over the distinct probes with candidates
+      if (ctx->tok_stamp_n < probe->n || ctx->tok_epoch == 0xFFFFFFFFu) {
+        ctx->tok_stamp.ensure((size_t)std::max<uint32_t>(probe->n, 1) * 4);
+        ctx->tok_stamp_n = ctx->tok_stamp.bytes / 4;
+        hs::launch_fill_u32(st, ctx->tok_stamp.as<uint32_t>(), ctx->tok_stamp_n, 0);
+        ctx->tok_epoch = 0;
+      }
+      a.stamp = ctx->tok_stamp.as<uint32_t>();
+      a.epoch = ++ctx->tok_epoch;
+    }
     a.out = dev_out(ctx);
     tile_plan = HS_PLAN_TOKEN;
     hs::launch_tok_join(st, a, (uint32_t)ctx->num_cus, timed ? ctx->ev[5] : nullptr, timed ? ctx->ev[2] : nullptr);
@@ -954,6 +965,28 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     // is timed as a whole.
     const bool stages = (tune.flags & HS_TUNE_STAGE_TIMES) && algo != HS_ALGO_HYBRID;
     ctx->stage_times = stages || algo == HS_ALGO_HYBRID;
+    // Pairs wanted on the host: a prefix of the result rows (as many as the
+    // last such join found, and a quarter more) is stored into mapped host
+    // memory by a kernel queued right after the launches, so it arrives with
+    // the counters; only a larger result needs the second round trip of
+    // copy_pairs.
+    const bool want_pairs =
+        !result->count_only && result->capacity > 0 && result->r && result->s && algo != HS_ALGO_HYBRID;
+    uint64_t nspec = 0;
+    auto speculate = [&]() {
+      if (!want_pairs) return;
+      nspec = std::min<uint64_t>(std::min<uint64_t>(result->capacity, ctx->out_cap),
+                                 std::max<uint64_t>(ctx->spec_guess + ctx->spec_guess / 4, 4096));
+      if (ctx->spec_cap < nspec) {
+        HS_CHECK_HIP(hipStreamSynchronize(st));  // no copy may still land in the old buffer
+        ctx->spec.ensure(nspec * 12);
+        ctx->spec_cap = ctx->spec.bytes / 12;
+      }
+      void* dst = nullptr;
+      HS_CHECK_HIP(hipHostGetDevicePointer(&dst, ctx->spec.ptr, 0));
+      hs::launch_rows_to_host(st, ctx->out.as<uint32_t>(), ctx->out_cap, ctx->counters.as<unsigned long long>(), nspec,
+                              static_cast<uint32_t*>(dst), ctx->spec_cap);
+    };
     if (replay) {
       graph_state = 2;
       HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
@@ -961,6 +994,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       ctx->jcum_valid = false;  // the graph rewrote the entry table
       P = ctx->jgraph_plan;
       tile_plan = ctx->jgraph_tile_plan;
+      speculate();
       read_counters(ctx, cnt, ctx->ev[3]);
       // more pairs (or survivors, or chunks) than the buffers hold: grow,
       // run again eagerly (a full chunk or survivor queue leaves the later
@@ -971,6 +1005,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         grow_for(P, cnt);
         reset_for_launch(P);
         tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
+        speculate();
         read_counters(ctx, cnt, ctx->ev[3]);
       }
     } else {
@@ -984,6 +1019,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       for (int attempt = 0; attempt < 4; ++attempt) {
         if (attempt > 0 || !P.counters_ready) reset_for_launch(P);
         tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
+        speculate();
         read_counters(ctx, cnt, ctx->ev[3]);
         if (fits(P, cnt)) break;
         HS_REQUIRE(attempt < 3, HS_E_DEVICE, "result count changed between runs");
@@ -1000,15 +1036,9 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     const uint64_t count = cnt[hs::CNT_PAIRS];
     set_device_result(ctx, count);
     result->count = count;
-    const bool copy = !result->count_only && count > 0;
-    if (copy) {
-      HS_REQUIRE(count <= result->capacity, HS_E_OVERFLOW, "result capacity too small");
-      copy_pairs(ctx, count, result);
-      HS_CHECK_HIP(hipEventRecord(ctx->ev[4], st));
-      HS_CHECK_HIP(hipEventSynchronize(ctx->ev[4]));
-      ctx->stats.transfer_ms = elapsed(ctx->ev[3], ctx->ev[4]);
-    }
-    ctx->stats.total_ms = elapsed(ctx->ev[0], copy ? ctx->ev[4] : ctx->ev[3]);
+    // the stage times, work counters and plan, also when the pairs then
+    // overflow the caller's arrays
+    ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[3]);
     if (graph_state == 2) {
       ctx->stats.filter_ms = elapsed(ctx->ev[0], ctx->ev[3]);
     } else if (ctx->stage_times) {
@@ -1032,6 +1062,23 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       ctx->stats.index_entries = P.token_major ? P.index_side_entries : cnt[hs::CNT_ENTRIES];
       ctx->stats.signature_tests = cnt[hs::CNT_SIG_TESTS];
       ctx->stats.signature_bits = P.sig_words * 64;
+      ctx->stats.candidate_probe_tokens = cnt[hs::CNT_CAND_PROBES];
+    }
+    if (want_pairs) ctx->spec_guess = count;
+    if (!result->count_only && count > 0) {
+      HS_REQUIRE(count <= result->capacity, HS_E_OVERFLOW, "result capacity too small");
+      if (count <= nspec) {  // the whole result came with the counters
+        const uint32_t* rows = ctx->spec.as<uint32_t>();
+        std::memcpy(result->r, rows, count * 4);
+        std::memcpy(result->s, rows + ctx->spec_cap, count * 4);
+        if (result->overlap) std::memcpy(result->overlap, rows + 2 * ctx->spec_cap, count * 4);
+      } else {
+        copy_pairs(ctx, count, result);
+      }
+      HS_CHECK_HIP(hipEventRecord(ctx->ev[4], st));
+      HS_CHECK_HIP(hipEventSynchronize(ctx->ev[4]));
+      ctx->stats.transfer_ms = elapsed(ctx->ev[3], ctx->ev[4]);
+      ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[4]);
     }
   });
 }

@@ -180,6 +180,14 @@ struct hs_ctx {
   uint32_t tok_queue_cap = 0;
   DevBuf tok_heavy, tok_aoff;  // token-major join: chunks of heavy blocks and the blocks' layouts
   uint32_t tok_heavy_cap = 0;
+  DevBuf tok_stamp;            // token-major join with stage times: probes already counted
+  uint32_t tok_epoch = 0;
+  uint64_t tok_stamp_n = 0;
+  // hs_join copying its pairs: a prefix of the result rows fetched with the
+  // counters (as many as the last such join found), so the pairs usually
+  // reach the host in the counters' round trip
+  PinnedBuf spec;
+  uint64_t spec_cap = 0, spec_guess = 0;
   // hs_join_noalloc: its host bound tables and the staging copy of tables
   // and entry table (host memory only; the device side is the scratch array)
   hs::BoundTables na_tables;

@@ -1415,7 +1415,7 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_verify(TokArgs A) {
   const unsigned long long queued = A.out.counters[CNT_QUEUE];
   const uint32_t total = (uint32_t)min(queued, (unsigned long long)A.queue_cap);
   const uint32_t stride = gridDim.x * TK_TPB;
-  uint32_t cands = 0, cand_tokens = 0;
+  uint32_t cands = 0, cand_tokens = 0, probe_tokens = 0;
   // wave-uniform trip count: emit_pairs needs every lane of the wave
   for (uint32_t base = blockIdx.x * TK_TPB; base < total; base += stride) {
     const uint32_t q = base + threadIdx.x;
@@ -1430,11 +1430,14 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_verify(TokArgs A) {
       pass = verify_visit<false>(A.c.tokens + A.c.starts[r], lr, A.c.tokens + A.c.starts[s], ls, e.z, mo, &ov);
       ++cands;
       cand_tokens += ls;
+      if (A.stamp && atomicExch(&A.stamp[r], A.epoch) != A.epoch) probe_tokens += lr;
     }
     emit_pairs(pass, r, s, ov, A.out.r, A.out.s, A.out.ov, &A.out.counters[CNT_PAIRS], A.out.capacity);
   }
   cands = wave_sum(cands);
   cand_tokens = wave_sum(cand_tokens);
+  probe_tokens = wave_sum(probe_tokens);
+  if (lane_id() == 0 && probe_tokens) atomicAdd(&A.out.counters[CNT_CAND_PROBES], (unsigned long long)probe_tokens);
   if (lane_id() == 0 && cands) {
     atomicAdd(&s_stats[0], (unsigned long long)cands);
     atomicAdd(&s_stats[1], (unsigned long long)cand_tokens);

@@ -126,10 +126,12 @@ struct DevIndex {
 // survivors), tokens of the verified candidate sets, probe-prefix items,
 // index entries, and (fgss) visits that reached the signature test.
 // CNT_QUEUE: candidates the token-major filter appended to its queue;
-// CNT_HEAVY: chunks of list positions it left to the second filter pass.
+// CNT_HEAVY: chunks of visits it left to the second filter pass;
+// CNT_CAND_PROBES: sum of |r| over the distinct probes with candidates (the
+// token-major join, when asked to count it).
 enum {
   CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_CAND_TOKENS = 3, CNT_ITEMS = 4, CNT_ENTRIES = 5,
-  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_WORDS = 16
+  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_CAND_PROBES = 9, CNT_WORDS = 16
 };
 
 struct DevOut {
@@ -144,6 +146,11 @@ struct DevOut {
 void launch_fill_u32(hipStream_t st, uint32_t* p, size_t n, uint32_t v);
 void launch_fill_u64(hipStream_t st, unsigned long long* p, size_t n, unsigned long long v);
 void launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
+// rows r, s, overlap [0, min(counters[CNT_PAIRS], n)) of the result rows
+// (row stride cap) into dst (row stride dst_cap), a device address of
+// mapped host memory
+void launch_rows_to_host(hipStream_t st, const uint32_t* rows, uint64_t cap, const unsigned long long* counters,
+                         uint64_t n, uint32_t* dst, uint64_t dst_cap);
 // XOR-popcount chains: grid x 256 lanes x iters x 4 x 8 steps of 2 VALU ops.
 void launch_popc_peak(hipStream_t st, uint32_t grid, uint32_t iters, uint32_t* out);
 void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, uint32_t* out2, size_t n,
@@ -338,6 +345,10 @@ struct TokArgs {
   uint4* heavy;
   uint32_t heavy_cap;
   uint2* aoff;  // n_ent entries
+  // optional: stamp[r] == epoch marks a probe already counted in
+  // CNT_CAND_PROBES (NULL: not counted)
+  uint32_t* stamp;
+  uint32_t epoch;
   DevOut out;
 };
 // Filter, then verify the queue (a persistent grid of num_cus * 8

@@ -45,6 +45,30 @@ void launch_fill_u64(hipStream_t st, unsigned long long* p, size_t n, unsigned l
   HS_CHECK_HIP(hipGetLastError());
 }
 
+// The first min(count, n) result rows (r, s and overlap: three rows of `cap`
+// words from `rows`; count read from the join's counters) stored straight
+// into mapped host memory laid out as three rows of `dst_cap` words: the
+// pairs reach the host with the join's own completion, without a separate
+// DMA copy.
+__global__ void k_rows_to_host(const uint32_t* __restrict__ rows, uint64_t cap,
+                               const unsigned long long* __restrict__ counters, uint64_t n, uint32_t* dst,
+                               uint64_t dst_cap) {
+  const uint64_t m = min((uint64_t)counters[CNT_PAIRS], n);
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x) {
+    dst[i] = rows[i];
+    dst[dst_cap + i] = rows[cap + i];
+    dst[2 * dst_cap + i] = rows[2 * cap + i];
+  }
+}
+
+void launch_rows_to_host(hipStream_t st, const uint32_t* rows, uint64_t cap, const unsigned long long* counters,
+                         uint64_t n, uint32_t* dst, uint64_t dst_cap) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_rows_to_host, dim3(grid_for(n, 256, 1024)), dim3(256), 0, st, rows, cap, counters, n, dst,
+                     dst_cap);
+  HS_CHECK_HIP(hipGetLastError());
+}
+
 // Plain device copy in 16-byte vectors: the reference rate for HBM traffic.
 __global__ void k_copy_u128(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)

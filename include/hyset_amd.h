@@ -263,7 +263,8 @@ typedef struct {
   double transfer_ms;
   double total_ms;
   double host_filter_ms; /* hybrid: CPU filtering wall time */
-  double kernel_ms;      /* the main filter/verify kernel alone */
+  double kernel_ms;      /* the main filter/verify kernel alone (fgss token-major plan: its filter,
+                            second pass and verification kernels, first start to last end) */
   uint64_t index_entries;
   uint64_t posting_visits;   /* fgss: postings inside length windows; bitmap: pairs tested */
   uint64_t candidates;       /* distinct pairs verified; bitmap: filter survivors */
@@ -278,6 +279,8 @@ typedef struct {
                                 a replayed join is timed as a whole: index_ms and kernel_ms are 0,
                                 filter_ms covers the index build, the launch and the counters */
   uint32_t reserved;
+  uint64_t candidate_probe_tokens; /* fgss token-major plan with HS_TUNE_STAGE_TIMES: sum of |r| over the
+                                      distinct probes with a verified candidate (else 0) */
 } hs_stats;
 
 /* ------------------------------------------------------------- contexts */

@@ -38,7 +38,7 @@ _u32p = C.POINTER(C.c_uint32)
 class Pairs(C.Structure):
     _fields_ = [("count", C.c_uint64), ("r", _u32p), ("s", _u32p), ("overlap", _u32p),
                 ("candidates", C.c_uint64), ("posting_visits", C.c_uint64), ("probe_tokens", C.c_uint64),
-                ("index_entries", C.c_uint64)]
+                ("index_entries", C.c_uint64), ("index_ns", C.c_uint64), ("join_ns", C.c_uint64)]
 
 
 def _sig(name, restype, *argtypes):
@@ -92,6 +92,9 @@ class Result:
         self.posting_visits = int(p.posting_visits)
         self.probe_tokens = int(p.probe_tokens)
         self.index_entries = int(p.index_entries)
+        # steady-clock intervals of the run (cpu_single: index build, then the join proper)
+        self.index_ms = p.index_ns / 1e6
+        self.join_ms = p.join_ns / 1e6
 
     @property
     def count(self) -> int:

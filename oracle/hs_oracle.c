@@ -4,9 +4,11 @@
  * benchmark, never by the product library.  The header lists the reference
  * functions each routine restates.
  */
+#define _POSIX_C_SOURCE 199309L
 #include "hs_oracle.h"
 
 #include <math.h>
+#include <time.h>
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
@@ -428,6 +430,12 @@ static int check_coll(const uint32_t* tokens, const uint32_t* starts, const uint
   return 0;
 }
 
+static uint64_t now_ns(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (uint64_t)ts.tv_sec * 1000000000ull + (uint64_t)ts.tv_nsec;
+}
+
 int or_self_join_blocks(const uint32_t* tokens, const uint32_t* starts, const uint32_t* sizes,
                         uint32_t n, uint32_t universe, int kind, int mode, double t, int algo,
                         uint32_t i_first, uint32_t i_last, uint32_t p_first, uint32_t p_last,
@@ -440,13 +448,17 @@ int or_self_join_blocks(const uint32_t* tokens, const uint32_t* starts, const ui
   if (p_last >= n) p_last = n - 1;
   coll_t C = {tokens, starts, sizes, n};
   index_t idx;
+  const uint64_t t0 = now_ns();
   if (index_build(&idx, tokens, starts, sizes, n, universe, &b, i_first, i_last, 0) != 0) {
     index_free(&idx);
     return -1;
   }
+  const uint64_t t1 = now_ns();
   out->index_entries = idx.entries;
   uint64_t cap = 0;
   int rc = probe_range(&C, &C, &idx, &b, algo, 1, p_first, p_last, out, &cap);
+  out->index_ns = t1 - t0;
+  out->join_ns = now_ns() - t1;
   index_free(&idx);
   return rc;
 }
@@ -539,7 +551,9 @@ static void* worker_run(void* arg) {
   index_t idx;
   memset(&w->out, 0, sizeof(w->out));
   /* per-thread index over [0, end_i] (cpu_multi.cu:143-158) */
+  const uint64_t t0 = now_ns();
   w->rc = index_build(&idx, w->tokens, w->starts, w->sizes, w->n, w->universe, w->b, 0, w->last, 0);
+  w->out.index_ns = now_ns() - t0;
   if (w->rc == 0) {
     uint64_t cap = 0;
     w->out.index_entries = idx.entries;
@@ -568,6 +582,7 @@ int or_cpu_multi(const uint32_t* tokens, const uint32_t* starts, const uint32_t*
     goto done;
   }
   points[0] = 0;
+  const uint64_t t_start = now_ns();
   for (int i = 0; i < threads; ++i) points[i + 1] = (uint32_t)(n * sqrt((double)(i + 1) / (double)threads));
   for (int i = 0; i < threads; ++i) {
     if (points[i + 1] <= points[i]) continue;
@@ -596,8 +611,10 @@ int or_cpu_multi(const uint32_t* tokens, const uint32_t* starts, const uint32_t*
     out->posting_visits += w->out.posting_visits;
     out->probe_tokens += w->out.probe_tokens;
     out->index_entries += w->out.index_entries;
+    if (w->out.index_ns > out->index_ns) out->index_ns = w->out.index_ns;
     or_pairs_free(&w->out);
   }
+  out->join_ns = now_ns() - t_start;
 done:
   free(points);
   free(ws);

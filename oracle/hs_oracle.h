@@ -44,6 +44,13 @@ typedef struct {
   uint64_t posting_visits;  /* posting entries visited inside length windows */
   uint64_t probe_tokens;    /* probe prefix tokens looked up */
   uint64_t index_entries;   /* entries in the inverted index */
+  /* steady-clock intervals (ns) of the reference's cpu_single timers
+     (src/cpu_single.cu:80-144): the index build, then the join proper
+     (probing and verification, src/cpu_single.cu:115-138).  cpu_multi:
+     index_ns is the slowest thread's index build, join_ns the wall time of
+     the whole call. */
+  uint64_t index_ns;
+  uint64_t join_ns;
 } or_pairs;
 
 /* Bounds.  kind: OR_JACCARD / OR_COSINE / OR_DICE.  mode: OR_BOUNDS_REFERENCE

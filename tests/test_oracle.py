@@ -131,3 +131,15 @@ def test_oracle_counters():
     assert res.posting_visits >= res.candidates
     mids = sum(O.bounds("jaccard", 0.6, int(l))[4] for l in c.sizes)
     assert res.index_entries == mids
+
+
+def test_oracle_intervals():
+    """cpu_single reports the reference's two intervals (index build, then
+    the join proper) on a steady clock; cpu_multi the join's wall time and
+    its slowest worker's index build."""
+    c = small_random(3000, 2, 40, 500, 1.0, 3)
+    single = O.self_join(c, 0.6)
+    multi = O.cpu_multi(c, 0.6, 3)
+    assert single.count == multi.count
+    assert single.index_ms > 0 and single.join_ms > 0
+    assert multi.join_ms > 0 and 0 < multi.index_ms <= multi.join_ms
