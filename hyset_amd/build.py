@@ -19,7 +19,7 @@ BUILD = HERE / "_build"
 LIB = HERE / "libhyset_amd.so"
 DRIVER = HERE / "hyset_join"
 
-HIP_SOURCES = ["index.hip", "fgss.hip", "bitmap.hip", "verify.hip", "mlp.hip", "cost.hip", "noalloc.hip"]
+HIP_SOURCES = ["index.hip", "fgss.hip", "bitmap.hip", "verify.hip", "mlp.hip", "cost.hip", "noalloc.hip", "tokindex.hip"]
 CPP_SOURCES = ["api.cpp", "batch.cpp", "serve.cpp", "host_join.cpp", "bounds.cpp", "datagen.cpp", "host_tools.cpp",
                "preprocess.cpp", "coprocess.cpp"]
 ARCH = "gfx950"

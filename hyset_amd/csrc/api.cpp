@@ -527,12 +527,23 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
   P.ir = ir;
   // the token-major plan: a fgss self-join of one range with itself, its
   // index needed by nobody else (co-processing reads the index prefixes on
-  // the host), sizes that fit the 16-bit fields of the records
+  // the host), sizes that fit the 16-bit fields of the filter's metas, and
+  // entries whose fields fit one 64-bit word
   P.token_major = algo == HS_ALGO_FGSS && self && !with_index && !range_empty(ir) && pr.first == ir.first &&
                   pr.last == ir.last && tune.probes_per_tile == 0 &&
                   !(tune.flags & (HS_TUNE_STATIC_TILES | HS_TUNE_TILE_QUEUE | HS_TUNE_BALANCED_TILES |
                                   HS_TUNE_ITEM_MAJOR)) &&
                   indexed->maxlen < 0xFFFFu;
+  if (P.token_major) {
+    const auto& sf = indexed->h_size_first;
+    uint64_t probe_entries = 0;
+    for (uint32_t l = 0; l + 1 < (uint32_t)sf.size(); ++l) {
+      const uint32_t lo = std::max(ir.first, sf[l]);
+      const uint32_t hi = std::min(ir.last + 1, sf[l + 1]);
+      if (hi > lo) probe_entries += (uint64_t)(hi - lo) * ctx->tables.maxprefix[l];
+    }
+    P.token_major = hs::tok_pack_plan(indexed->n, indexed->maxlen, indexed->universe, probe_entries, &P.pk);
+  }
   // ---- parity signatures for the fgss signature test (timed with the index)
   bool sig_in_emit = false;  // the index's emit pass computes them (self-join over the whole collection)
   if (algo == HS_ALGO_FGSS && !(tune.flags & HS_TUNE_NO_SIGNATURE) && !range_empty(ir) && !range_empty(pr)) {
@@ -588,9 +599,10 @@ void hs::prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_r
     }
     if (P.token_major) {
       hs::build_token_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
-                            ctx->d_jcum.as<uint32_t>(), indexed->universe, acc, ctx->ib,
-                            ctx->dev_bounds.midprefix, sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
-                            ctx->counters.as<unsigned long long>(), timed ? ctx->ev[1] : nullptr);
+                            ctx->d_jcum.as<uint32_t>(), acc, P.pk, ctx->ib, ctx->dev_bounds.midprefix,
+                            sig_in_emit ? indexed->sigs.as<uint64_t>() : nullptr,
+                            ctx->counters.as<unsigned long long>(), (uint32_t)ctx->num_cus,
+                            timed ? ctx->ev[1] : nullptr);
       P.counters_ready = true;
     } else {
       hs::build_index(st, indexed->dev(), ir.first, ir.last, prefix_tab, indexed->size_first.as<uint32_t>(),
@@ -622,10 +634,8 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
   } else if (algo == HS_ALGO_FGSS && P.token_major) {
     hs::TokArgs a{};
     a.c = probe->dev();
-    // the last radix pass leaves the sorted keys, ids and metas here
-    a.keys = ctx->ib.keys[1].as<uint32_t>();
-    a.ids = ctx->ib.set_ids.as<uint32_t>();
-    a.metas = ctx->ib.metas.as<uint32_t>();
+    a.ent = ctx->ib.packed_sorted;
+    a.pk = P.pk;
     a.n_ent = (uint32_t)P.index_entries;
     a.bounds = ctx->dev_bounds;
     a.size_first = indexed->size_first.as<uint32_t>();
@@ -996,6 +1006,7 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
       tile_plan = ctx->jgraph_tile_plan;
       speculate();
       read_counters(ctx, cnt, ctx->ev[3]);
+      HS_REQUIRE(cnt[hs::CNT_INDEX_ERR] == 0, HS_E_DEVICE, "token-major index sort did not complete");
       // more pairs (or survivors, or chunks) than the buffers hold: grow,
       // run again eagerly (a full chunk or survivor queue leaves the later
       // counts short, so up to three growths may follow one another)
@@ -1021,6 +1032,8 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
         tile_plan = hs::launch_device_join(ctx, indexed, probe, P, pr, algo, tune);
         speculate();
         read_counters(ctx, cnt, ctx->ev[3]);
+        HS_REQUIRE(attempt > 0 || cnt[hs::CNT_INDEX_ERR] == 0, HS_E_DEVICE,
+                   "token-major index sort did not complete");
         if (fits(P, cnt)) break;
         HS_REQUIRE(attempt < 3, HS_E_DEVICE, "result count changed between runs");
         grow_for(P, cnt);

@@ -92,6 +92,7 @@ struct DevicePlan {
   uint64_t index_side_entries = 0;  // of those, the entries of the index prefixes
   bool counters_ready = false;  // the index build zeroed the work counters
   bool token_major = false;     // fgss self-join of one range: the token-major plan
+  TokPack pk{};                 // the token-major plan's packed entry layout
 };
 }  // namespace hs
 

@@ -970,13 +970,27 @@ struct TokCounts {
   uint32_t visits = 0, items = 0, tests = 0;
 };
 
+// Fields of a packed index word (TokPack): key, set id, meta (size << 16
+// | position), and (key, set id) as one ordered value.
+__device__ __forceinline__ uint32_t pk_key(const TokArgs& A, uint64_t w) { return (uint32_t)(w >> A.pk.kshift); }
+__device__ __forceinline__ uint32_t pk_id(const TokArgs& A, uint64_t w) {
+  return (uint32_t)((w >> (A.pk.pb + A.pk.sb)) & ((1ull << A.pk.ib) - 1));
+}
+__device__ __forceinline__ uint32_t pk_meta(const TokArgs& A, uint64_t w) {
+  const uint32_t pos = (uint32_t)w & ((1u << A.pk.pb) - 1);
+  const uint32_t size = (uint32_t)(w >> A.pk.pb) & ((1u << A.pk.sb) - 1);
+  return (size << 16) | pos;
+}
+__device__ __forceinline__ uint64_t pk_keyid(const TokArgs& A, uint64_t w) { return w >> (A.pk.pb + A.pk.sb); }
+
 // Entry e of the sorted index: its set id, meta and (64-bit signature
 // tests) the set's signature, gathered from the per-set table.
 template <int kW>
 __device__ __forceinline__ void tok_record(const TokArgs& A, uint32_t e, uint32_t& sid, uint32_t& meta,
                                            uint64_t& sig) {
-  sid = A.ids[e];
-  meta = A.metas[e];
+  const uint64_t w = A.ent[e];
+  sid = pk_id(A, w);
+  meta = pk_meta(A, w);
   sig = kW == 1 ? A.sigs[sid] : 0;
 }
 
@@ -1169,11 +1183,11 @@ __device__ __forceinline__ uint32_t tok_lb_lds(const TokLds& L, uint32_t hi, uin
 // The same over the global arrays, galloping back from hi (qualifying):
 // the entries whose lists start before the staged window.
 __device__ __forceinline__ uint32_t tok_lb_back(const TokArgs& A, uint32_t hi, uint32_t k, uint32_t v) {
+  const uint64_t kv = ((uint64_t)k << A.pk.ib) | v;
   uint32_t step = 1, lo;
   for (;;) {
     const uint32_t j = hi >= step ? hi - step : 0u;
-    const uint32_t kj = A.keys[j];
-    if (kj < k || (kj == k && A.ids[j] < v)) {
+    if (pk_keyid(A, A.ent[j]) < kv) {
       lo = j + 1;
       break;
     }
@@ -1183,8 +1197,7 @@ __device__ __forceinline__ uint32_t tok_lb_back(const TokArgs& A, uint32_t hi, u
   }
   while (lo < hi) {
     const uint32_t m = (lo + hi) >> 1;
-    const uint32_t km = A.keys[m];
-    if (km < k || (km == k && A.ids[m] < v)) lo = m + 1;
+    if (pk_keyid(A, A.ent[m]) < kv) lo = m + 1;
     else hi = m;
   }
   return lo;
@@ -1196,12 +1209,12 @@ __device__ __forceinline__ uint32_t tok_lb_back(const TokArgs& A, uint32_t hi, u
 // tile, first entry of that key's index part}.
 __device__ __forceinline__ uint2 tok_header(const TokArgs& A, uint32_t t) {
   const uint32_t i = t * TOK_BLOCK;
-  const uint32_t k0 = A.keys[i] & ~1u;
+  const uint32_t k0 = pk_key(A, A.ent[i]) & ~1u;
   // first position with key >= k0: gallop back from i, then bisect
   uint32_t hi = i, step = 1, lo;
   for (;;) {
     const uint32_t j = hi >= step ? hi - step : 0u;
-    if (A.keys[j] < k0) {
+    if (pk_key(A, A.ent[j]) < k0) {
       lo = j + 1;
       break;
     }
@@ -1211,7 +1224,7 @@ __device__ __forceinline__ uint2 tok_header(const TokArgs& A, uint32_t t) {
   }
   while (lo < hi) {
     const uint32_t m = (lo + hi) >> 1;
-    if (A.keys[m] < k0) lo = m + 1;
+    if (pk_key(A, A.ent[m]) < k0) lo = m + 1;
     else hi = m;
   }
   return make_uint2(k0, lo);
@@ -1241,8 +1254,11 @@ __device__ __forceinline__ void tok_stage(const TokArgs& A, TokLds& L, uint32_t 
     key[q] = sid[q] = meta[q] = 0;
     sig[q] = 0;
     if (w < wn) {
-      tok_record<kW>(A, ws + w, sid[q], meta[q], sig[q]);
-      key[q] = A.keys[ws + w];
+      const uint64_t e = A.ent[ws + w];
+      key[q] = pk_key(A, e);
+      sid[q] = pk_id(A, e);
+      meta[q] = pk_meta(A, e);
+      if (kW == 1) sig[q] = A.sigs[sid[q]];
     }
   }
 #pragma unroll
@@ -1463,7 +1479,9 @@ void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEven
       (void)hipGetLastError();
       nb = 4;
     }
-    per_cu = nb;
+    // at most 3 workgroups per CU: the filter measured 88 us at 3 and
+    // 102 us at 4 per CU on the 1M-set index
+    per_cu = std::min(nb, 3);
   }
   const dim3 gp(std::max<uint32_t>(num_cus, 1) * (uint32_t)per_cu);
   const dim3 g(std::min<uint32_t>(n_tiles, gp.x));

@@ -128,10 +128,12 @@ struct DevIndex {
 // CNT_QUEUE: candidates the token-major filter appended to its queue;
 // CNT_HEAVY: chunks of visits it left to the second filter pass;
 // CNT_CAND_PROBES: sum of |r| over the distinct probes with candidates (the
-// token-major join, when asked to count it).
+// token-major join, when asked to count it); CNT_INDEX_ERR: nonzero when a
+// sort pass of the packed token-major index gave up waiting for an earlier
+// tile (never in a correct run; the join then fails).
 enum {
   CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_CAND_TOKENS = 3, CNT_ITEMS = 4, CNT_ENTRIES = 5,
-  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_CAND_PROBES = 9, CNT_WORDS = 16
+  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_CAND_PROBES = 9, CNT_INDEX_ERR = 10, CNT_WORDS = 16
 };
 
 struct DevOut {
@@ -159,10 +161,26 @@ void launch_exclusive_scan(hipStream_t st, const uint32_t* in, uint32_t* out, ui
 struct IndexBuild {
   DevBuf offsets, set_ids, metas, ent;  // the index
   DevBuf keys[2], vals, pos, hist, scan_scratch;  // radix-sort work space
+  DevBuf packed[2], pk_hist, pk_sync;  // the packed token-major index and its sort's work space
+  const uint64_t* packed_sorted = nullptr;  // the packed index after its last sort pass
   uint32_t ent_words = 2;
   uint32_t universe = 0;
   uint64_t entries = 0;
 };
+
+// Packed token-major index entry: one 64-bit word holding, from the low
+// bits up, the entry's position in its set (pb bits), the set's size (sb
+// bits), the set id (ib bits) and the key (token << 1 | position >=
+// midprefix, key_bits bits from kshift = pb + sb + ib).  Ordering words by
+// their bits above pb + sb orders entries by (key, set id).  The sort runs
+// `passes` stable passes of digit_bits key bits each, low digit first.
+struct TokPack {
+  uint32_t pb, sb, ib, kshift, key_bits, passes, digit_bits;
+};
+// The packing of a self-join index over sets of at most maxlen tokens, ids
+// below n_sets and tokens below universe: false when the fields need more
+// than 64 bits or the index more than 2^30 entries.
+bool tok_pack_plan(uint32_t n_sets, uint32_t maxlen, uint32_t universe, uint64_t entries, TokPack* out);
 // The same buffers as raw device pointers: the index (offsets, set_ids,
 // metas, ent) and the radix sort's work space (keys, vals, pos, the
 // per-tile digit histograms and the digit totals).  A meta is the entry's
@@ -197,24 +215,24 @@ size_t index_region(uint32_t universe, uint64_t entries, uint8_t* base, IndexBuf
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out = nullptr,
-                 unsigned long long* counters = nullptr, hipEvent_t done = nullptr,
-                 const uint32_t* flag_len = nullptr);
-// The token-major index: flagged keys (flag_len = midprefix), sorted with
-// their set ids and metas into (keys[1], set_ids, metas) and nothing more:
-// no offsets, no records (the token-major join reads the sorted arrays).
-// counters (may be NULL) are zeroed; done (may be NULL) takes the end.
+                 unsigned long long* counters = nullptr, hipEvent_t done = nullptr);
+// The packed token-major index (TokPack) over the probe prefixes
+// (prefix_len = maxprefix; flag_len = midprefix) of sets [first, last]: a
+// pass over the sets that writes the words in set-id order and computes
+// the key-digit histograms (and, with sigs_out != NULL, the sets' 64-bit
+// signatures), then one stable scatter per key digit.  cum[l] is the first
+// entry of size class l.  The sorted words are left in ib.packed_sorted.
+// num_cus sizes the emit's grid.  counters are zeroed; done (may be NULL)
+// takes the end.
 void build_token_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
-                       const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                       IndexBuild& ib, const uint32_t* flag_len, uint64_t* sigs_out,
-                       unsigned long long* counters, hipEvent_t done = nullptr);
+                       const uint32_t* size_first, const uint32_t* cum, uint64_t entries, const TokPack& pk,
+                       IndexBuild& ib, const uint32_t* flag_len, uint64_t* sigs_out, unsigned long long* counters,
+                       uint32_t num_cus, hipEvent_t done = nullptr);
 // The same into buffers the caller laid out (index_region); allocates nothing.
-// flag_len (may be NULL; indexed by set size): the token-major index, keys
-// (token << 1) | (position >= flag_len[|s|]) over 2 x universe (pass that
-// key universe here and to index_region).
 void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                       const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                       const IndexBufs& b, const uint64_t* sigs, uint64_t* sigs_out,
-                      unsigned long long* counters, hipEvent_t done = nullptr, const uint32_t* flag_len = nullptr);
+                      unsigned long long* counters, hipEvent_t done = nullptr);
 // The same over every join of a batched collection at once (token ids of
 // different joins are disjoint, so one index serves all joins).
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,
@@ -330,9 +348,8 @@ constexpr uint32_t TOK_WINDOW = 1024;
 constexpr uint32_t TOK_CHUNK = 4096;
 struct TokArgs {
   DevColl c;
-  const uint32_t* keys;   // key of each sorted entry
-  const uint32_t* ids;    // its set id
-  const uint32_t* metas;  // its meta (size << 16 | position, both saturated at 0xFFFF)
+  const uint64_t* ent;  // the packed index, sorted by (key, set id)
+  TokPack pk;           // its word layout
   uint32_t n_ent;
   DevBounds bounds;
   const uint32_t* size_first;  // first id of size >= l, l in [0, maxlen + 1]

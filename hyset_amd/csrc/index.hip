@@ -199,17 +199,12 @@ constexpr int RS_WAVES = RS_TPB / WAVE;
 // pass.
 constexpr uint32_t IX_CHUNK = 16;
 
-// With flag_len != NULL the key is (token << 1) | (position >= flag_len[l]):
-// the token-major index sorts a token's index-prefix entries before its
-// other probe-prefix entries.
 __device__ __forceinline__ void emit_set(const DevColl& c, uint32_t i, uint32_t l, uint32_t pl, uint32_t o,
                                          uint32_t* __restrict__ keys, uint32_t* __restrict__ vals,
-                                         uint32_t* __restrict__ meta, uint64_t* __restrict__ sig,
-                                         const uint32_t* __restrict__ flag_len) {
+                                         uint32_t* __restrict__ meta, uint64_t* __restrict__ sig) {
   const uint32_t* t = c.tokens + c.starts[i];
   const uint32_t len = sig ? l : pl;
   const uint32_t size_bits = min(l, 0xFFFFu) << 16;
-  const uint32_t fl = flag_len ? flag_len[l] : 0xFFFFFFFFu;
   uint64_t w = 0;
   for (uint32_t k0 = 0; k0 < len; k0 += IX_CHUNK) {
     uint32_t v[IX_CHUNK];
@@ -219,7 +214,7 @@ __device__ __forceinline__ void emit_set(const DevColl& c, uint32_t i, uint32_t 
     for (uint32_t q = 0; q < IX_CHUNK; ++q) {
       const uint32_t k = k0 + q;
       if (k < pl) {
-        keys[o + k] = flag_len ? (v[q] << 1) | (k >= fl ? 1u : 0u) : v[q];
+        keys[o + k] = v[q];
         vals[o + k] = i;
         meta[o + k] = size_bits | min(k, 0xFFFFu);
       }
@@ -232,13 +227,13 @@ __device__ __forceinline__ void emit_set(const DevColl& c, uint32_t i, uint32_t 
 __global__ void k_index_emit(DevColl c, uint32_t first, uint32_t last, const uint32_t* __restrict__ prefix_len,
                              const uint32_t* __restrict__ size_first, const uint32_t* __restrict__ cum,
                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* __restrict__ meta,
-                             uint64_t* __restrict__ sig, const uint32_t* __restrict__ flag_len) {
+                             uint64_t* __restrict__ sig) {
   const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
   if (i > last) return;
   const uint32_t l = c.sizes[i];
   const uint32_t pl = prefix_len[l];
   const uint32_t o = cum[l] + (i - max(size_first[l], first)) * pl;
-  emit_set(c, i, l, pl, o, keys, vals, meta, sig, flag_len);
+  emit_set(c, i, l, pl, o, keys, vals, meta, sig);
 }
 
 // Batched collections: set i belongs to join j = last j with join_first[j]
@@ -255,7 +250,7 @@ __global__ void k_index_emit_batch(DevColl c, const uint32_t* __restrict__ join_
   const uint32_t pl = prefix_len[l];
   const uint32_t b = sf_base[j];
   const uint32_t o = cum[b + l] + (i - size_first[b + l]) * pl;
-  emit_set(c, i, l, pl, o, keys, vals, meta, sig, nullptr);
+  emit_set(c, i, l, pl, o, keys, vals, meta, sig);
 }
 
 // Digit histogram of one tile: LDS atomics, then one column of the
@@ -565,7 +560,7 @@ static int emit_buffer(uint32_t passes) { return (passes % 2 == 1) ? 0 : 1; }
 void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                       const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                       const IndexBufs& b, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
-                      hipEvent_t done, const uint32_t* flag_len) {
+                      hipEvent_t done) {
   HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
   const uint32_t n = (uint32_t)entries;
   if (c.n > 0 && first <= last && n > 0) {
@@ -576,7 +571,7 @@ void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t
     uint32_t* pb = cur ? b.metas : b.pos;
     const uint32_t nsets = last - first + 1;
     hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
-                       size_first, cum, kb, vb, pb, sigs_out, flag_len);
+                       size_first, cum, kb, vb, pb, sigs_out);
     sort_and_finish(st, c, n, universe, b, cur, plan, sigs, counters, done);
   } else {
     launch_fill_u32(st, b.offsets, (size_t)universe + 1, 0);
@@ -585,54 +580,14 @@ void build_index_into(hipStream_t st, const DevColl& c, uint32_t first, uint32_t
   HS_CHECK_HIP(hipGetLastError());
 }
 
-__global__ void k_fill_counters(unsigned long long* __restrict__ counters) {
-  if (threadIdx.x < CNT_WORDS) counters[threadIdx.x] = 0;
-}
-
-void build_token_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
-                       const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
-                       IndexBuild& ib, const uint32_t* flag_len, uint64_t* sigs_out,
-                       unsigned long long* counters, hipEvent_t done) {
-  HS_REQUIRE(universe < 0x80000000u, HS_E_UNSUPPORTED, "token-major index limited to 2^31 tokens");
-  const uint32_t key_universe = 2 * universe;
-  const IndexBufs b = prepare(ib, key_universe, entries);
-  ib.ent_words = 0;  // no records
-  HS_REQUIRE(entries < 0xFFFFFFFFull, HS_E_UNSUPPORTED, "index limited to 2^32-1 entries");
-  const uint32_t n = (uint32_t)entries;
-  if (counters) hipLaunchKernelGGL(k_fill_counters, dim3(1), dim3(64), 0, st, counters);
-  if (c.n > 0 && first <= last && n > 0) {
-    const RadixPlan plan = radix_plan(key_universe);
-    int cur = emit_buffer(plan.passes);
-    uint32_t* kb = cur ? b.keys[1] : b.keys[0];
-    uint32_t* vb = cur ? b.set_ids : b.vals;
-    uint32_t* pb = cur ? b.metas : b.pos;
-    const uint32_t nsets = last - first + 1;
-    hipLaunchKernelGGL(k_index_emit, dim3((nsets + 255) / 256), dim3(256), 0, st, c, first, last, prefix_len,
-                       size_first, cum, kb, vb, pb, sigs_out, flag_len);
-    const uint32_t ntiles = radix_tiles(n);
-    uint32_t* const kbuf[2] = {b.keys[0], b.keys[1]};
-    uint32_t* const vbuf[2] = {b.vals, b.set_ids};
-    uint32_t* const pbuf[2] = {b.pos, b.metas};
-    for (uint32_t p = 0; p < plan.passes; ++p) {
-      radix_pass<8>(st, n, plan.bits * p, ntiles, b, kbuf, vbuf, pbuf, cur);
-      cur ^= 1;
-    }
-  }
-  if (done) HS_CHECK_HIP(hipEventRecord(done, st));
-  HS_CHECK_HIP(hipGetLastError());
-}
-
 void build_index(hipStream_t st, const DevColl& c, uint32_t first, uint32_t last, const uint32_t* prefix_len,
                  const uint32_t* size_first, const uint32_t* cum, uint32_t universe, uint64_t entries,
                  IndexBuild& ib, const uint64_t* sigs, uint64_t* sigs_out, unsigned long long* counters,
-                 hipEvent_t done, const uint32_t* flag_len) {
-  // flagged keys run over twice the universe
-  const uint32_t key_universe = flag_len ? 2 * universe : universe;
-  HS_REQUIRE(!flag_len || universe < 0x80000000u, HS_E_UNSUPPORTED, "token-major index limited to 2^31 tokens");
-  const IndexBufs b = prepare(ib, key_universe, entries);
+                 hipEvent_t done) {
+  const IndexBufs b = prepare(ib, universe, entries);
   ib.ent_words = sigs ? 4u : 2u;
-  build_index_into(st, c, first, last, prefix_len, size_first, cum, key_universe, entries, b, sigs, sigs_out,
-                   counters, done, flag_len);
+  build_index_into(st, c, first, last, prefix_len, size_first, cum, universe, entries, b, sigs, sigs_out,
+                   counters, done);
 }
 
 void build_index_batch(hipStream_t st, const DevColl& c, const uint32_t* join_first, uint32_t n_joins,

@@ -331,6 +331,51 @@ def test_sets_longer_than_16_bit_fields(ctx):
         d.free()
 
 
+def _triples_and_longs(n_small, seed):
+    """n_small sets of 3 tokens below 70,000 plus three near-duplicate
+    50,000-token sets, sorted by (size, tokens)."""
+    rng = np.random.default_rng(seed)
+    t0 = rng.integers(1, 23_333, size=n_small)
+    t1 = t0 + rng.integers(1, 23_333, size=n_small)
+    t2 = t1 + rng.integers(1, 23_333, size=n_small)
+    tri = np.stack([t0, t1, t2], axis=1)
+    tri = tri[np.lexsort((tri[:, 2], tri[:, 1], tri[:, 0]))]
+    base = np.sort(rng.choice(np.arange(1, 70_000), size=50_000, replace=False))
+    longs = [base]
+    for k in (1, 2):
+        drop = rng.choice(50_000, size=300 * k, replace=False)
+        add = rng.choice(np.setdiff1d(np.arange(1, 70_000), base), size=300 * k, replace=False)
+        longs.append(np.sort(np.concatenate([np.delete(base, drop), add])))
+    longs.sort(key=lambda a: a.tolist())
+    tokens = np.concatenate([tri.reshape(-1)] + longs).astype(np.uint32)
+    sizes = np.array([3] * n_small + [50_000] * 3, dtype=np.uint32)
+    starts = np.zeros_like(sizes)
+    starts[1:] = np.cumsum(sizes[:-1])
+    return H.SetCollection(tokens, starts, sizes, int(tokens.max()) + 1)
+
+
+@pytest.mark.parametrize("n_small,plan", [(3000, "token"), (1_048_600, "item")])
+def test_token_plan_word_packing(ctx, n_small, plan):
+    """The token-major plan packs an index entry's position, set size, set
+    id and key into one 64-bit word.  With 50,000-token sets (16-bit size
+    and position fields) among 3,003 sets the fields take 62 bits and the
+    plan runs with positions past 10,000; among 1,048,603 sets they would
+    take 71 bits and the join takes the item-major plan.  Both as the
+    oracle, at Jaccard 0.8 and 0.9."""
+    c = _triples_and_longs(n_small, 23)
+    d = ctx.upload(c)
+    try:
+        for t in (0.8, 0.9):
+            ctx.set_threshold(t)
+            want = O.self_join(c, t)
+            assert want.count >= 3
+            _check(ctx.join(d, algo="fgss"), want, f"packing n={c.n} t={t}")
+            st = ctx.stats()
+            assert (st["tile_plan"] == H.PLAN_TOKEN) == (plan == "token"), st["tile_plan"]
+    finally:
+        d.free()
+
+
 @pytest.mark.parametrize("algo", ["fgss", "bitmap", "hybrid"])
 def test_degenerate_inputs(ctx, algo):
     """Empty collection, one set, 400 identical sets (every pair a result),
