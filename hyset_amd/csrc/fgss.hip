@@ -1008,6 +1008,7 @@ struct TokLds {
   uint32_t tmp[TK_TPB / WAVE + 1];
   uint32_t q_r[TK_QCAP], q_s[TK_QCAP], q_j[TK_QCAP];
   uint32_t qn, qbase, ws, wn, trunc;
+  uint32_t next_ws;  // the window start of the workgroup's next tile
   uint32_t stats[3];
 };
 }  // namespace
@@ -1155,33 +1156,9 @@ __device__ __forceinline__ void tok_flush(const TokArgs& A, TokLds& L, const Tok
   }
 }
 
-// First staged position p <= hi with (key, id) >= (k, v), hi qualifying:
-// gallop back from hi, then bisect the last step (runs are short: most take
-// one or two reads).
-__device__ __forceinline__ uint32_t tok_lb_lds(const TokLds& L, uint32_t hi, uint32_t k, uint32_t v) {
-  uint32_t step = 1, lo;
-  for (;;) {
-    const uint32_t j = hi >= step ? hi - step : 0u;
-    const uint32_t kj = L.wk[j];
-    if (kj < k || (kj == k && L.wid[j] < v)) {
-      lo = j + 1;
-      break;
-    }
-    hi = j;
-    if (j == 0) return 0;
-    step <<= 1;
-  }
-  while (lo < hi) {
-    const uint32_t m = (lo + hi) >> 1;
-    const uint32_t km = L.wk[m];
-    if (km < k || (km == k && L.wid[m] < v)) lo = m + 1;
-    else hi = m;
-  }
-  return lo;
-}
-
-// The same over the global arrays, galloping back from hi (qualifying):
-// the entries whose lists start before the staged window.
+// First position p <= hi of the index with (key, id) >= (k, v), hi
+// qualifying: gallop back from hi, then bisect the last step (the entries
+// whose lists start before the staged window).
 __device__ __forceinline__ uint32_t tok_lb_back(const TokArgs& A, uint32_t hi, uint32_t k, uint32_t v) {
   const uint64_t kv = ((uint64_t)k << A.pk.ib) | v;
   uint32_t step = 1, lo;
@@ -1230,20 +1207,57 @@ __device__ __forceinline__ uint2 tok_header(const TokArgs& A, uint32_t t) {
   return make_uint2(k0, lo);
 }
 
+// Tile t's window: from its first token's index part (hdr.y) to the tile's
+// end t1, at most TOK_WINDOW entries back; every window of the tile lies in
+// the span [tok_span_lo(t1), t1).
+__device__ __forceinline__ uint32_t tok_t1(const TokArgs& A, uint32_t t) {
+  return min(t * TOK_BLOCK + TOK_BLOCK, A.n_ent);
+}
+__device__ __forceinline__ uint32_t tok_span_lo(uint32_t t1) { return t1 > TOK_WINDOW ? t1 - TOK_WINDOW : 0u; }
+__device__ __forceinline__ uint32_t tok_ws(const TokArgs& A, uint32_t t, uint2 hdr) {
+  return max(hdr.y, tok_span_lo(tok_t1(A, t)));
+}
+
+// Thread 0 sets the window of tile t from its header; then a barrier.
+__device__ __forceinline__ void tok_window(const TokArgs& A, TokLds& L, uint32_t t, uint2 hdr) {
+  if (threadIdx.x == 0) {
+    const uint32_t ws = tok_ws(A, t, hdr);
+    L.ws = ws;
+    L.wn = tok_t1(A, t) - ws;
+    L.trunc = ws > hdr.y ? hdr.x : TK_NONE;
+  }
+  __syncthreads();
+}
+
+// After the staged keys are in LDS: the first position of every
+// position's key, a running maximum of the positions where the key
+// changes (R consecutive positions per lane); then a barrier.
+__device__ __forceinline__ void tok_key_starts(TokLds& L) {
+  constexpr uint32_t R = TOK_WINDOW / TK_TPB;
+  const uint32_t tid = threadIdx.x, wn = L.wn;
+  uint32_t head[R], m = 0;
+#pragma unroll
+  for (uint32_t q = 0; q < R; ++q) {
+    const uint32_t w = tid * R + q;
+    head[q] = (w < wn && (w == 0 || L.wk[w] != L.wk[w - 1])) ? w : 0u;
+    m = max(m, head[q]);
+  }
+  uint32_t run = block_exclusive_max<TK_TPB>(m, L.tmp);
+#pragma unroll
+  for (uint32_t q = 0; q < R; ++q) {
+    run = max(run, head[q]);
+    L.ks[tid * R + q] = run;
+  }
+  __syncthreads();
+}
+
 // Stages tile t's window from its header; called by every lane between
 // barriers (thread 0's hdr counts).  All loads of a lane are issued before
 // its first LDS store.
 template <int kW>
 __device__ __forceinline__ void tok_stage(const TokArgs& A, TokLds& L, uint32_t t, uint2 hdr) {
   const uint32_t tid = threadIdx.x;
-  const uint32_t t0 = t * TOK_BLOCK, t1 = min(t0 + TOK_BLOCK, A.n_ent);
-  if (tid == 0) {
-    const uint32_t ws = max(hdr.y, t1 > TOK_WINDOW ? t1 - TOK_WINDOW : 0u);
-    L.ws = ws;
-    L.wn = t1 - ws;
-    L.trunc = ws > hdr.y ? hdr.x : TK_NONE;
-  }
-  __syncthreads();
+  tok_window(A, L, t, hdr);
   const uint32_t ws = L.ws, wn = L.wn;
   constexpr uint32_t R = TOK_WINDOW / TK_TPB;
   uint32_t key[R], sid[R], meta[R];
@@ -1272,22 +1286,57 @@ __device__ __forceinline__ void tok_stage(const TokArgs& A, TokLds& L, uint32_t 
     }
   }
   __syncthreads();
-  // the first position of every position's key: a running maximum of the
-  // positions where the key changes (R consecutive positions per lane)
-  uint32_t head[R], m = 0;
+  tok_key_starts(L);
+}
+
+// A lane's share of a tile's span, loaded ahead of the tile's staging:
+// the words, then (once the window start is known) the signatures of
+// those inside the window.
+struct TokAhead {
+  uint64_t w[TOK_WINDOW / TK_TPB];
+  uint64_t sig[TOK_WINDOW / TK_TPB];
+};
+
+__device__ __forceinline__ void tok_ahead_words(const TokArgs& A, uint32_t t, TokAhead& P) {
+  const uint32_t t1 = tok_t1(A, t), lo = tok_span_lo(t1);
 #pragma unroll
-  for (uint32_t q = 0; q < R; ++q) {
-    const uint32_t w = tid * R + q;
-    head[q] = (w < wn && (w == 0 || L.wk[w] != L.wk[w - 1])) ? w : 0u;
-    m = max(m, head[q]);
+  for (uint32_t q = 0; q < TOK_WINDOW / TK_TPB; ++q) {
+    const uint32_t g = lo + threadIdx.x + q * TK_TPB;
+    P.w[q] = g < t1 ? A.ent[g] : 0ull;
   }
-  uint32_t run = block_exclusive_max<TK_TPB>(m, L.tmp);
+}
+
+template <int kW>
+__device__ __forceinline__ void tok_ahead_sigs(const TokArgs& A, uint32_t t, uint32_t ws, TokAhead& P) {
+  if (kW != 1) return;
+  const uint32_t t1 = tok_t1(A, t), lo = tok_span_lo(t1);
 #pragma unroll
-  for (uint32_t q = 0; q < R; ++q) {
-    run = max(run, head[q]);
-    L.ks[tid * R + q] = run;
+  for (uint32_t q = 0; q < TOK_WINDOW / TK_TPB; ++q) {
+    const uint32_t g = lo + threadIdx.x + q * TK_TPB;
+    P.sig[q] = g >= ws && g < t1 ? A.sigs[pk_id(A, P.w[q])] : 0ull;
+  }
+}
+
+// tok_stage from the words (and signatures) loaded ahead.
+template <int kW>
+__device__ __forceinline__ void tok_stage_ahead(const TokArgs& A, TokLds& L, uint32_t t, uint2 hdr,
+                                                const TokAhead& P) {
+  tok_window(A, L, t, hdr);
+  const uint32_t ws = L.ws;
+  const uint32_t t1 = tok_t1(A, t), lo = tok_span_lo(t1);
+#pragma unroll
+  for (uint32_t q = 0; q < TOK_WINDOW / TK_TPB; ++q) {
+    const uint32_t g = lo + threadIdx.x + q * TK_TPB;
+    if (g >= ws && g < t1) {
+      const uint32_t w = g - ws;
+      L.wk[w] = pk_key(A, P.w[q]);
+      L.wid[w] = pk_id(A, P.w[q]);
+      L.wm[w] = pk_meta(A, P.w[q]);
+      if (kW == 1) L.wsig[w] = P.sig[q];
+    }
   }
   __syncthreads();
+  tok_key_starts(L);
 }
 
 // First staged position in [lo, hi) with set id >= v (ids ascend there); hi if none.
@@ -1315,13 +1364,31 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_filter(TokArgs A) {
   const bool lut = tok_lut(A);
   TokCounts C;
   const uint32_t n_tiles = (A.n_ent + TOK_BLOCK - 1) / TOK_BLOCK;
-  // thread 0 loads the next tile's header while the current one runs
+  // The next tile is loaded while the current one runs: its words right
+  // after the current staging, its header by thread 0, and the signatures
+  // of its window once the header's window start is out (after the scan).
   uint2 hdr = make_uint2(0, 0);
-  if (tid == 0 && blockIdx.x < n_tiles) hdr = tok_header(A, blockIdx.x);
+  TokAhead P;
+  if (blockIdx.x < n_tiles) {
+    tok_ahead_words(A, blockIdx.x, P);
+    if (tid == 0) {
+      hdr = tok_header(A, blockIdx.x);
+      L.next_ws = tok_ws(A, blockIdx.x, hdr);
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x < n_tiles) tok_ahead_sigs<kW>(A, blockIdx.x, L.next_ws, P);
   for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
     __syncthreads();  // the previous tile's readers are done with L
-    tok_stage<kW>(A, L, t, hdr);
-    if (tid == 0 && t + gridDim.x < n_tiles) hdr = tok_header(A, t + gridDim.x);
+    tok_stage_ahead<kW>(A, L, t, hdr, P);
+    const uint32_t tn = t + gridDim.x;
+    if (tn < n_tiles) {
+      tok_ahead_words(A, tn, P);
+      if (tid == 0) {
+        hdr = tok_header(A, tn);
+        L.next_ws = tok_ws(A, tn, hdr);
+      }
+    }
     const uint32_t t0 = t * TOK_BLOCK, t1 = min(t0 + TOK_BLOCK, A.n_ent);
     const uint32_t ws = L.ws;
     uint32_t cnt[TK_EPL], sum = 0;
@@ -1362,6 +1429,7 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_filter(TokArgs A) {
     }
     uint32_t total;
     uint32_t run = block_exclusive_scan<TK_TPB>(sum, L.tmp, &total);
+    if (tn < n_tiles) tok_ahead_sigs<kW>(A, tn, L.next_ws, P);  // (the scan's barriers published next_ws)
 #pragma unroll
     for (uint32_t q = 0; q < TK_EPL; ++q) {
       const uint32_t k = tid * TK_EPL + q;
@@ -1396,23 +1464,31 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_heavy(TokArgs A) {
   TokCounts C;
   const unsigned long long queued = A.out.counters[CNT_HEAVY];
   const uint32_t n_items = (uint32_t)min(queued, (unsigned long long)A.heavy_cap);
-  for (uint32_t q = blockIdx.x; q < n_items; q += gridDim.x) {
+  // a contiguous run of items per workgroup: a tile's chunks sit next to
+  // each other in the queue, so its window is staged once per workgroup
+  const uint32_t per = (n_items + gridDim.x - 1) / gridDim.x;
+  const uint32_t q0 = min(blockIdx.x * per, n_items), q1 = min(q0 + per, n_items);
+  uint32_t staged = TK_NONE;
+  for (uint32_t q = q0; q < q1; ++q) {
     __syncthreads();  // the previous item's readers are done with L
     const uint4 it = A.heavy[q];
-    const uint32_t t0 = it.x * TOK_BLOCK;
-    for (uint32_t k = tid; k < TOK_BLOCK; k += TK_TPB) {
-      if (t0 + k < A.n_ent) {
-        const uint2 ao = A.aoff[t0 + k];
-        L.a[k] = ao.x;
-        L.off[k] = ao.y;
-      } else {
-        L.off[k] = it.z;  // past the last entry: no visits
+    if (it.x != staged) {
+      const uint32_t t0 = it.x * TOK_BLOCK;
+      for (uint32_t k = tid; k < TOK_BLOCK; k += TK_TPB) {
+        if (t0 + k < A.n_ent) {
+          const uint2 ao = A.aoff[t0 + k];
+          L.a[k] = ao.x;
+          L.off[k] = ao.y;
+        } else {
+          L.off[k] = it.z;  // past the last entry: no visits
+        }
       }
+      if (tid == 0) L.off[TOK_BLOCK] = it.z;
+      uint2 hdr = make_uint2(0, 0);
+      if (tid == 0) hdr = tok_header(A, it.x);
+      tok_stage<kW>(A, L, it.x, hdr);
+      staged = it.x;
     }
-    if (tid == 0) L.off[TOK_BLOCK] = it.z;
-    uint2 hdr = make_uint2(0, 0);
-    if (tid == 0) hdr = tok_header(A, it.x);
-    tok_stage<kW>(A, L, it.x, hdr);
     tok_visits<kW>(A, L, it.x, it.y * TOK_CHUNK, min(it.z, (it.y + 1) * TOK_CHUNK), C);
     __syncthreads();
     if (L.qn >= TK_QCAP / 2) tok_flush_queue(A, L);
