@@ -213,8 +213,9 @@ __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles
   }
   __syncthreads();
   if (tid == 0) {
-    if (s_stats[0]) atomicAdd(&A.out.counters[CNT_CANDS], s_stats[0]);
-    if (s_stats[1]) atomicAdd(&A.out.counters[CNT_VISITS], s_stats[1]);
+    unsigned long long* slot = stat_slot(A.out.counters);
+    if (s_stats[0]) atomicAdd(&slot[CNT_CANDS], s_stats[0]);
+    if (s_stats[1]) atomicAdd(&slot[CNT_VISITS], s_stats[1]);
   }
 }
 

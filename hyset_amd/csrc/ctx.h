@@ -299,6 +299,7 @@ inline void read_counters(hs_ctx* ctx, unsigned long long* dst, hipEvent_t done 
   if (done) HS_CHECK_HIP(hipEventRecord(done, ctx->stream));
   HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   std::memcpy(dst, ctx->h_counters.ptr, hs::CNT_WORDS * sizeof(unsigned long long));
+  hs::fold_counters(dst);
 }
 
 // Pairs per chunk of the result stream (12 bytes each with the overlap).

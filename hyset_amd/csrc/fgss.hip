@@ -706,11 +706,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4))) void k
   }
   __syncthreads();
   if (tid == 0) {
-    if (L.stats[0]) atomicAdd(&A.out.counters[CNT_VISITS], L.stats[0]);
-    if (L.stats[1]) atomicAdd(&A.out.counters[CNT_CANDS], L.stats[1]);
-    if (L.stats[2]) atomicAdd(&A.out.counters[CNT_CAND_TOKENS], L.stats[2]);
-    if (L.stats[3]) atomicAdd(&A.out.counters[CNT_ITEMS], L.stats[3]);
-    if (L.stats[4]) atomicAdd(&A.out.counters[CNT_SIG_TESTS], L.stats[4]);
+    unsigned long long* slot = stat_slot(A.out.counters);
+    if (L.stats[0]) atomicAdd(&slot[CNT_VISITS], L.stats[0]);
+    if (L.stats[1]) atomicAdd(&slot[CNT_CANDS], L.stats[1]);
+    if (L.stats[2]) atomicAdd(&slot[CNT_CAND_TOKENS], L.stats[2]);
+    if (L.stats[3]) atomicAdd(&slot[CNT_ITEMS], L.stats[3]);
+    if (L.stats[4]) atomicAdd(&slot[CNT_SIG_TESTS], L.stats[4]);
   }
 }
 
@@ -1150,9 +1151,10 @@ __device__ __forceinline__ void tok_flush(const TokArgs& A, TokLds& L, const Tok
   __syncthreads();
   tok_flush_queue(A, L);
   if (threadIdx.x == 0) {
-    if (L.stats[0]) atomicAdd(&A.out.counters[CNT_VISITS], (unsigned long long)L.stats[0]);
-    if (L.stats[1]) atomicAdd(&A.out.counters[CNT_ITEMS], (unsigned long long)L.stats[1]);
-    if (L.stats[2]) atomicAdd(&A.out.counters[CNT_SIG_TESTS], (unsigned long long)L.stats[2]);
+    unsigned long long* slot = stat_slot(A.out.counters);
+    if (L.stats[0]) atomicAdd(&slot[CNT_VISITS], (unsigned long long)L.stats[0]);
+    if (L.stats[1]) atomicAdd(&slot[CNT_ITEMS], (unsigned long long)L.stats[1]);
+    if (L.stats[2]) atomicAdd(&slot[CNT_SIG_TESTS], (unsigned long long)L.stats[2]);
   }
 }
 
@@ -1501,8 +1503,8 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_heavy(TokArgs A) {
 // from the token through which the candidate was reached (a pair reached
 // through several tokens is verified by the visit of its first common one).
 __global__ __launch_bounds__(TK_TPB) void k_tok_verify(TokArgs A) {
-  __shared__ unsigned long long s_stats[2];
-  if (threadIdx.x < 2) s_stats[threadIdx.x] = 0;
+  __shared__ unsigned long long s_stats[3];
+  if (threadIdx.x < 3) s_stats[threadIdx.x] = 0;
   __syncthreads();
   const unsigned long long queued = A.out.counters[CNT_QUEUE];
   const uint32_t total = (uint32_t)min(queued, (unsigned long long)A.queue_cap);
@@ -1529,15 +1531,19 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_verify(TokArgs A) {
   cands = wave_sum(cands);
   cand_tokens = wave_sum(cand_tokens);
   probe_tokens = wave_sum(probe_tokens);
-  if (lane_id() == 0 && probe_tokens) atomicAdd(&A.out.counters[CNT_CAND_PROBES], (unsigned long long)probe_tokens);
+  if (lane_id() == 0 && probe_tokens) atomicAdd(&s_stats[2], (unsigned long long)probe_tokens);
   if (lane_id() == 0 && cands) {
     atomicAdd(&s_stats[0], (unsigned long long)cands);
     atomicAdd(&s_stats[1], (unsigned long long)cand_tokens);
   }
   __syncthreads();
-  if (threadIdx.x == 0 && s_stats[0]) {
-    atomicAdd(&A.out.counters[CNT_CANDS], s_stats[0]);
-    atomicAdd(&A.out.counters[CNT_CAND_TOKENS], s_stats[1]);
+  if (threadIdx.x == 0) {
+    unsigned long long* slot = stat_slot(A.out.counters);
+    if (s_stats[0]) {
+      atomicAdd(&slot[CNT_CANDS], s_stats[0]);
+      atomicAdd(&slot[CNT_CAND_TOKENS], s_stats[1]);
+    }
+    if (s_stats[2]) atomicAdd(&slot[CNT_CAND_PROBES], s_stats[2]);
   }
 }
 

@@ -131,10 +131,29 @@ struct DevIndex {
 // token-major join, when asked to count it); CNT_INDEX_ERR: nonzero when a
 // sort pass of the packed token-major index gave up waiting for an earlier
 // tile (never in a correct run; the join then fails).
+// The first CNT_STATS words hold the counters; CNT_SLOTS slots of
+// CNT_SLOT_WORDS words (a 128-byte line each) follow, into which a
+// workgroup may add its pure statistics (visits, items, signature tests,
+// candidates, their tokens) at the same word index, slot blockIdx.x %
+// CNT_SLOTS: a wide grid's end-of-workgroup atomics then spread over 16
+// lines instead of meeting on one (same-line device atomics serialize,
+// ~8 ns each).  fold_counters adds the slots into the first words.
 enum {
   CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_CAND_TOKENS = 3, CNT_ITEMS = 4, CNT_ENTRIES = 5,
-  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_CAND_PROBES = 9, CNT_INDEX_ERR = 10, CNT_WORDS = 16
+  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_CAND_PROBES = 9, CNT_INDEX_ERR = 10, CNT_STATS = 16,
+  CNT_SLOTS = 16, CNT_SLOT_WORDS = 16, CNT_WORDS = CNT_STATS + CNT_SLOTS * CNT_SLOT_WORDS
 };
+#ifdef __HIPCC__
+// This workgroup's statistics slot of a counters array.
+__device__ __forceinline__ unsigned long long* stat_slot(unsigned long long* counters) {
+  return counters + CNT_STATS + (blockIdx.x % CNT_SLOTS) * CNT_SLOT_WORDS;
+}
+#endif
+// Adds the statistics slots into the first CNT_STATS words (host copy).
+inline void fold_counters(unsigned long long* c) {
+  for (int k = 0; k < CNT_SLOTS; ++k)
+    for (int w = 0; w < CNT_SLOT_WORDS; ++w) c[w] += c[CNT_STATS + k * CNT_SLOT_WORDS + w];
+}
 
 struct DevOut {
   uint32_t* r;

@@ -198,6 +198,7 @@ extern "C" int hs_join_noalloc(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, co
     HS_CHECK_HIP(hipStreamSynchronize(st));
     unsigned long long cnt[CNT_WORDS];
     std::memcpy(cnt, ctx->h_counters.ptr, sizeof(cnt));
+    fold_counters(cnt);
     ctx->stats.total_ms = elapsed(ctx->ev[0], ctx->ev[3]);
     ctx->stats.pairs = cnt[CNT_PAIRS];
     ctx->stats.posting_visits = cnt[CNT_VISITS];

@@ -276,6 +276,7 @@ void run_batch(hs_server* S, Slot& sl, std::vector<Request*>& batch) {
     copy_triples(sl, 0, std::min(guess, sl.out_cap), s);
     HS_CHECK_HIP(hipStreamSynchronize(s));
     std::memcpy(cnt, sl.h_counters.ptr, sizeof(cnt));
+    fold_counters(cnt);
     if (cnt[CNT_PAIRS] <= sl.out_cap) break;
     HS_REQUIRE(attempt == 0, HS_E_DEVICE, "result count changed between runs");
     grow_out(sl, cnt[CNT_PAIRS] + cnt[CNT_PAIRS] / 4 + 1024);
