@@ -1,13 +1,17 @@
-// Inverted-index build over set prefixes, and the device-wide scan it uses.
+// Inverted-index build over set prefixes for the item-major joins (R-S,
+// ranges, batches, serving; the self-join's token-major index is built in
+// tokindex.hip), the fills, copies and scans around it.
 //
 // Reference: include/hyset/index.hpp:217-313 (histogram, thrust::exclusive_scan,
 // create_index with atomic scatter, thrust::sort by (token, setID)).
-// Here: count -> exclusive scan -> atomic scatter of set ids -> per-list sort
-// by set id, so list(t) = set_ids[offsets[t] .. offsets[t+1]) ascending --
-// the order of the reference's sorted host index (index.hpp:278-279).  A
-// parallel 16-bit array keeps each entry's position inside its set (the
-// `position` of the reference's 12-byte entry, structs.hpp:56-63), used by
-// the positional filter.
+// Here: the prefix entries are emitted in set order, sorted by token with a
+// stable LSD radix sort (so each list stays in set-id order), and the
+// offsets written at the token boundaries of the sorted keys: list(t) =
+// set_ids[offsets[t] .. offsets[t+1]) ascending -- the order of the
+// reference's sorted host index (index.hpp:278-279).  Each entry's record
+// carries its set's size, its position inside the set (the `position` of
+// the reference's 12-byte entry, structs.hpp:56-63, used by the positional
+// filter) and, for 64-bit signature tests, the set's signature.
 #include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
