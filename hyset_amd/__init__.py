@@ -29,6 +29,8 @@ BOUNDS_REFERENCE, BOUNDS_EXACT = 0, 1
 FGSS, BITMAP, HYBRID = 0, 1, 2
 SIZE_UNIFORM, SIZE_LOGNORMAL = 0, 1
 ALGOS = {"fgss": FGSS, "bitmap": BITMAP, "hybrid": HYBRID}
+HOST_ALLPAIRS, HOST_PPJOIN = 0, 1
+HOST_ALGOS = {"allpairs": HOST_ALLPAIRS, "ppjoin": HOST_PPJOIN}
 SIMS = {"jaccard": JACCARD, "cosine": COSINE, "dice": DICE}
 
 _u32p = C.POINTER(C.c_uint32)
@@ -128,6 +130,7 @@ _sig("hs_last_error", C.c_char_p, _vp)
 _sig("hs_upload_collection", C.c_int, _vp, _u32p, _u32p, _u32p, C.c_uint32, C.c_uint64, C.POINTER(_vp))
 _sig("hs_free_collection", None, _vp)
 _sig("hs_set_threshold", C.c_int, _vp, C.c_int32, C.c_double, C.c_int32)
+_sig("hs_set_host_algorithm", C.c_int, _vp, C.c_int32)
 _sig("hs_join", C.c_int, _vp, _vp, _vp, C.POINTER(Range), C.POINTER(Range), C.c_int32, C.POINTER(Tune),
      C.POINTER(Result))
 _sig("hs_verify_candidates", C.c_int, _vp, _vp, _vp, _u32p, _u32p, _u32p, C.c_uint32, C.c_int32,
@@ -396,6 +399,13 @@ class Context:
     def set_threshold(self, t: float, sim="jaccard", bounds=BOUNDS_REFERENCE) -> None:
         kind = SIMS[sim] if isinstance(sim, str) else int(sim)
         self._check(_lib.hs_set_threshold(self._h, kind, float(t), int(bounds)), "hs_set_threshold")
+
+    def set_host_algorithm(self, algorithm="allpairs") -> None:
+        """The host cores' filtering (hs_set_host_algorithm): "allpairs" or
+        "ppjoin", for the hybrid technique's filter, join_host and the host
+        side of join_coprocess."""
+        a = HOST_ALGOS[algorithm] if isinstance(algorithm, str) else int(algorithm)
+        self._check(_lib.hs_set_host_algorithm(self._h, a), "hs_set_host_algorithm")
 
     def join(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, algo="fgss",
              indexed_range=None, probe_range=None, tune: Tune | None = None, count_only=False,

@@ -188,11 +188,13 @@ void submit_batch(HybridShared& H, const std::vector<uint32_t>& probes, const st
   sl.used = true;
 }
 
-// CPU candidate generation for probes [first, last] (hybrid::allpairs,
-// include/hyset/hybrid_algorithms.hpp:209-291): every posting of a probe
-// prefix token inside the length window, deduplicated per probe, no
-// verification.  Lists are id-sorted, so the window start is found by
-// binary search instead of the reference's linear skip.
+// CPU candidate generation for probes [first, last] (hybrid::allpairs and
+// hybrid::ppjoin, include/hyset/hybrid_algorithms.hpp:209-387): every
+// posting of a probe prefix token inside the length window, deduplicated
+// per probe, no verification; PPJoin takes a candidate at a common prefix
+// token only if the positional filter passes there (:333-348).  Lists are
+// id-sorted, so the window start is found by binary search instead of the
+// reference's linear skip.
 void host_filter(HybridShared& H, uint32_t first, uint32_t last, uint32_t& next_slot) {
   hs_ctx* ctx = H.ctx;
   const hs_coll& P = *H.probe;
@@ -205,6 +207,7 @@ void host_filter(HybridShared& H, uint32_t first, uint32_t last, uint32_t& next_
   cands.reserve(std::min<uint64_t>(H.batch, 1ull << 26));
   uint64_t n_cand = 0, n_visit = 0;
   const uint32_t last_len = (uint32_t)S.h_size_first.size() - 1;
+  const uint32_t* metas = ctx->host_algo == HS_HOST_PPJOIN ? ctx->h_metas.data() : nullptr;
   for (uint32_t p = first; p <= last && first <= last; ++p) {
     const uint32_t l = P.h_sizes[p];
     const uint32_t* rt = P.h_tokens.data() + P.h_starts[p];
@@ -223,10 +226,16 @@ void host_filter(HybridShared& H, uint32_t first, uint32_t last, uint32_t& next_
       const uint32_t* e = ctx->h_set_ids.data() + ctx->h_offsets[tok + 1];
       for (const uint32_t* it = std::lower_bound(a, e, lo); it != e && *it < hi; ++it) {
         ++n_visit;
-        if (stamp[*it] != p) {
-          stamp[*it] = p;
-          cands.push_back(*it);
+        if (stamp[*it] == p) continue;
+        if (metas) {
+          const uint32_t ip = metas[it - ctx->h_set_ids.data()] & 0xFFFFu;  // saturated: kept
+          if (ip != 0xFFFFu) {
+            const uint32_t ls = S.h_sizes[*it];
+            if (T.minoverlap(l, ls) > std::min(l - j, ls - ip)) continue;
+          }
         }
+        stamp[*it] = p;
+        cands.push_back(*it);
       }
     }
     if (cands.size() > row_begin) {
@@ -274,6 +283,12 @@ void run_hybrid(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const 
   if (entries)
     HS_CHECK_HIP(hipMemcpyAsync(ctx->h_set_ids.data(), ctx->ib.set_ids.ptr, (size_t)entries * 4,
                                 hipMemcpyDeviceToHost, ctx->stream));
+  if (ctx->host_algo == HS_HOST_PPJOIN) {  // positions for the positional filter
+    ctx->h_metas.resize(std::max<uint32_t>(entries, 1));
+    if (entries)
+      HS_CHECK_HIP(hipMemcpyAsync(ctx->h_metas.data(), ctx->ib.metas.ptr, (size_t)entries * 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+  }
   HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   ctx->stats.index_entries = entries;
 
@@ -497,6 +512,15 @@ void hs_free_collection(hs_coll* coll) {
     (void)hipDeviceSynchronize();
   }
   delete coll;
+}
+
+int hs_set_host_algorithm(hs_ctx* ctx, int32_t algorithm) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    HS_REQUIRE(algorithm == HS_HOST_ALLPAIRS || algorithm == HS_HOST_PPJOIN, HS_E_INVALID,
+               "host algorithm must be HS_HOST_ALLPAIRS or HS_HOST_PPJOIN");
+    ctx->host_algo = algorithm;
+  });
 }
 
 int hs_set_threshold(hs_ctx* ctx, int32_t sim, double threshold, int32_t bounds_mode) {

@@ -170,6 +170,13 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     if (entries)
       HS_CHECK_HIP(hipMemcpyAsync(ctx->h_index_ids.ptr, ctx->ib.set_ids.ptr, entries * 4, hipMemcpyDeviceToHost,
                                   ctx->copy_stream));
+    const bool ppjoin = ctx->host_algo == HS_HOST_PPJOIN;  // the host threads filter as PPJoin
+    if (ppjoin) {
+      ctx->h_index_metas.ensure(std::max<uint64_t>(entries, 1) * 4);
+      if (entries)
+        HS_CHECK_HIP(hipMemcpyAsync(ctx->h_index_metas.ptr, ctx->ib.metas.ptr, entries * 4, hipMemcpyDeviceToHost,
+                                    ctx->copy_stream));
+    }
     HS_CHECK_HIP(hipEventRecord(ctx->index_copied, ctx->copy_stream));
 
     // ---- the split of the probe ids
@@ -190,6 +197,7 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     // ---- host consumers
     const uint32_t* h_off = ctx->h_index_offsets.as<uint32_t>();
     const uint32_t* h_ids = ctx->h_index_ids.as<uint32_t>();
+    const uint32_t* h_metas = ppjoin ? ctx->h_index_metas.as<uint32_t>() : nullptr;
     const int device = ctx->device;
     hipEvent_t copied = ctx->index_copied;
     const std::vector<uint32_t>& psf = probe->h_size_first;
@@ -210,7 +218,8 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
           uint32_t want = opt.cpu_block ? opt.cpu_block : 16u, first = 0, n = 0;
           while (S.take_low(want, nthreads, first, n)) {
             const auto a = Clock::now();
-            host_probe_range(*probe, *indexed, h_off, h_ids, T, self, ir, first, first + n - 1, w.out, w.stamp);
+            host_probe_range(*probe, *indexed, h_off, h_ids, h_metas, T, self, ir, first, first + n - 1, w.out,
+                             w.stamp);
             const double dt = ms_between(a, Clock::now());
             host_weight_done += probe_weight(psf, first, first + n);
             w.probes += n;

@@ -111,6 +111,7 @@ struct hs_ctx {
   bool have_threshold = false;
   int kind = HS_JACCARD, mode = HS_BOUNDS_REFERENCE;
   double t = 0.0;
+  int32_t host_algo = HS_HOST_ALLPAIRS;  // hs_set_host_algorithm
   hs::BoundTables tables;
   bool tables_valid = false;
   DevBuf d_tables;
@@ -128,11 +129,11 @@ struct hs_ctx {
   std::vector<hs::Triples> host_tail;  // pairs the host joined (co-processing), after the device's
   HybridSlot slots[2];
   std::mutex submit_mu;
-  std::vector<uint32_t> h_offsets, h_set_ids;  // host copy of the index (hybrid)
+  std::vector<uint32_t> h_offsets, h_set_ids, h_metas;  // host copy of the index (hybrid; metas for PPJoin)
   // CPU/GPU co-processing join: a copy stream for the index, pinned host
   // copies of the index, claim-completion events
   hipStream_t copy_stream = nullptr;
-  PinnedBuf h_index_offsets, h_index_ids;
+  PinnedBuf h_index_offsets, h_index_ids, h_index_metas;
   hipEvent_t index_built = nullptr, index_copied = nullptr;
   hipEvent_t claim_ev[2] = {};
   PinnedBuf stage[2];            // pinned chunks of the result stream
@@ -350,19 +351,31 @@ void prepare_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, hs_range
 uint32_t launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, DevicePlan& P, hs_range pr,
                             int32_t algo, const hs_tune& tune);
 
+// Postings visited inside the length windows and distinct candidates kept
+// by a host probe loop.
+struct HostCounts {
+  uint64_t visits = 0, candidates = 0;
+};
+
 // CPU join of probes [first, last] of P against the prefix index (offsets,
 // ids: id-ascending lists over S's universe) of S's sets in ir, on the
-// calling thread; pairs are appended to out.  stamp is the thread's
-// deduplication array (S.n entries, allocated on first use; keep it across
-// calls of one thread).
+// calling thread; pairs are appended to out.  metas (NULL: AllPairs) are
+// the entries' (size << 16) | min(position, 0xFFFF): PPJoin's positional
+// filter at a candidate's first common prefix token (an entry whose
+// position saturates is kept).  stamp is the thread's deduplication array
+// (S.n entries, allocated on first use; keep it across calls of one
+// thread).  counts (may be NULL) are added to.
 void host_probe_range(const hs_coll& P, const hs_coll& S, const uint32_t* offsets, const uint32_t* ids,
-                      const BoundTables& T, bool self, const hs_range& ir, uint32_t first, uint32_t last,
-                      Triples& out, std::vector<uint32_t>& stamp);
+                      const uint32_t* metas, const BoundTables& T, bool self, const hs_range& ir, uint32_t first,
+                      uint32_t last, Triples& out, std::vector<uint32_t>& stamp, HostCounts* counts = nullptr);
 
-// CPU join (host_join.cpp): pairs of probe range pr against indexed range ir;
-// reads only host copies and the given tables, so it may run on any thread.
+// CPU join (host_join.cpp): pairs of probe range pr against indexed range ir
+// (PPJoin filtering when ppjoin); reads only host copies and the given
+// tables, so it may run on any thread.  counts (may be NULL) receive the
+// visits and candidates summed over the threads.
 uint64_t host_join(const hs_coll& indexed, const hs_coll& probe, bool self, hs_range ir, hs_range pr,
-                   const BoundTables& T, uint32_t threads, std::vector<Triples>* parts_out);
+                   const BoundTables& T, uint32_t threads, std::vector<Triples>* parts_out, bool ppjoin = false,
+                   HostCounts* counts = nullptr);
 
 
 }  // namespace hs

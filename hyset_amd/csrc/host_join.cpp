@@ -1,14 +1,17 @@
 // The CPU partition of the hybrid join: prefix-filter join on host cores.
 //
-// Reference: host::allpairs (include/hyset/host_algorithms.hpp:16-129) and
-// the cpu_multi driver's thread-per-partition layout (src/cpu_multi.cu:
-// 118-207).  Used by the CPU/GPU split to run the units it gives the CPU;
+// Reference: host::allpairs and host::ppjoin (include/hyset/
+// host_algorithms.hpp:16-250) and the cpu_multi driver's thread-per-
+// partition layout (src/cpu_multi.cu:118-207).  Used by the CPU/GPU split to
+// run the units it gives the CPU and by co-processing's host threads;
 // results are the same (r, s, overlap) triples the device techniques emit.
 // Per probe: candidates from the id-sorted posting lists of its prefix
 // tokens inside the length window, deduplicated with a per-thread stamp
-// array; each candidate is verified by a merge with the verify_pair
-// early-termination bound (verification.hpp:9-31), run to the end for
-// passing pairs so the overlap is exact.
+// array (PPJoin: a candidate is taken at its first common prefix token only
+// if the positional filter passes there, host_algorithms.hpp:183-192); each
+// candidate is verified by a merge with the verify_pair early-termination
+// bound (verification.hpp:9-31), run to the end for passing pairs so the
+// overlap is exact.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -22,7 +25,7 @@ using namespace hs;
 namespace {
 
 struct HostIndex {
-  std::vector<uint32_t> offsets, ids;
+  std::vector<uint32_t> offsets, ids, metas;
 };
 
 // Inverted index of the prefix entries of sets [first, last] (counting sort
@@ -36,24 +39,34 @@ void build_host_index(const hs_coll& S, uint32_t first, uint32_t last, const std
   }
   for (uint32_t u = 0; u < S.universe; ++u) X.offsets[u + 1] += X.offsets[u];
   X.ids.resize(X.offsets[S.universe]);
+  X.metas.resize(X.offsets[S.universe]);
   std::vector<uint32_t> cur(X.offsets.begin(), X.offsets.end() - 1);
   for (uint32_t i = first; i <= last; ++i) {
     const uint32_t* t = S.h_tokens.data() + S.h_starts[i];
-    for (uint32_t k = 0; k < plen[S.h_sizes[i]]; ++k) X.ids[cur[t[k]]++] = i;
+    const uint32_t l = S.h_sizes[i];
+    for (uint32_t k = 0; k < plen[l]; ++k) {
+      const uint32_t at = cur[t[k]]++;
+      X.ids[at] = i;
+      X.metas[at] = (std::min(l, 0xFFFFu) << 16) | std::min(k, 0xFFFFu);
+    }
   }
 }
 
 void join_probes(const hs_coll& P, const hs_coll& S, const HostIndex& X, const BoundTables& T, bool self,
-                 const hs_range& ir, uint32_t first, uint32_t last, Triples& out) {
+                 const hs_range& ir, uint32_t first, uint32_t last, bool ppjoin, Triples& out,
+                 hs::HostCounts& counts) {
   std::vector<uint32_t> stamp(S.n, 0xFFFFFFFFu);
-  hs::host_probe_range(P, S, X.offsets.data(), X.ids.data(), T, self, ir, first, last, out, stamp);
+  hs::host_probe_range(P, S, X.offsets.data(), X.ids.data(), ppjoin ? X.metas.data() : nullptr, T, self, ir,
+                       first, last, out, stamp, &counts);
 }
 
 }  // namespace
 
 void hs::host_probe_range(const hs_coll& P, const hs_coll& S, const uint32_t* offsets, const uint32_t* ids,
-                          const BoundTables& T, bool self, const hs_range& ir, uint32_t first, uint32_t last,
-                          Triples& out, std::vector<uint32_t>& stamp) {
+                          const uint32_t* metas, const BoundTables& T, bool self, const hs_range& ir,
+                          uint32_t first, uint32_t last, Triples& out, std::vector<uint32_t>& stamp,
+                          HostCounts* counts) {
+  uint64_t n_visit = 0, n_cand = 0;
   std::vector<uint32_t> cand;
   if (stamp.size() < S.n) stamp.assign(S.n, 0xFFFFFFFFu);
   const uint32_t last_len = (uint32_t)S.h_size_first.size() - 1;
@@ -72,12 +85,23 @@ void hs::host_probe_range(const hs_coll& P, const hs_coll& S, const uint32_t* of
       if (tok >= S.universe) continue;
       const uint32_t* a = ids + offsets[tok];
       const uint32_t* e = ids + offsets[tok + 1];
-      for (const uint32_t* it = std::lower_bound(a, e, lo); it != e && *it < hi; ++it)
-        if (stamp[*it] != p) {
-          stamp[*it] = p;
-          cand.push_back(*it);
+      for (const uint32_t* it = std::lower_bound(a, e, lo); it != e && *it < hi; ++it) {
+        ++n_visit;
+        if (stamp[*it] == p) continue;
+        if (metas) {
+          // PPJoin (host_algorithms.hpp:183-192): a candidate is taken at a
+          // common prefix token only if the positional bound passes there
+          const uint32_t ip = metas[it - ids] & 0xFFFFu;
+          if (ip != 0xFFFFu) {
+            const uint32_t ls = S.h_sizes[*it];
+            if (T.minoverlap(l, ls) > std::min(l - j, ls - ip)) continue;
+          }
         }
+        stamp[*it] = p;
+        cand.push_back(*it);
+      }
     }
+    n_cand += cand.size();
     for (uint32_t sid : cand) {
       const uint32_t ls = S.h_sizes[sid];
       const uint32_t mo = T.minoverlap(l, ls);
@@ -106,17 +130,23 @@ void hs::host_probe_range(const hs_coll& P, const hs_coll& S, const uint32_t* of
       }
     }
   }
+  if (counts) {
+    counts->visits += n_visit;
+    counts->candidates += n_cand;
+  }
 }
 
 namespace hs {
 
 uint64_t host_join(const hs_coll& indexed, const hs_coll& probe, bool self, hs_range ir, hs_range pr,
-                   const BoundTables& T, uint32_t threads, std::vector<Triples>* parts_out) {
+                   const BoundTables& T, uint32_t threads, std::vector<Triples>* parts_out, bool ppjoin,
+                   HostCounts* counts) {
   HostIndex X;
   if (!range_empty(ir)) build_host_index(indexed, ir.first, ir.last, self ? T.midprefix : T.maxprefix, X);
   else X.offsets.assign((size_t)indexed.universe + 1, 0);
   const uint32_t nt = std::max<uint32_t>(1, threads);
   std::vector<Triples> parts(nt);
+  std::vector<HostCounts> pc(nt);
   if (!range_empty(pr) && !range_empty(ir)) {
     // probe partitions: sqrt-spaced for a self-join (src/cpu_multi.cu:118-139)
     std::vector<uint32_t> pts{pr.first};
@@ -125,18 +155,23 @@ uint64_t host_join(const hs_coll& indexed, const hs_coll& probe, bool self, hs_r
       pts.push_back(pr.first + (uint32_t)(n * (self ? std::sqrt((double)i / nt) : (double)i / nt)));
     pts.push_back(pr.last + 1);
     if (nt == 1) {
-      join_probes(probe, indexed, X, T, self, ir, pts[0], pts[1] - 1, parts[0]);
+      join_probes(probe, indexed, X, T, self, ir, pts[0], pts[1] - 1, ppjoin, parts[0], pc[0]);
     } else {
       std::vector<std::thread> pool;
       for (uint32_t i = 0; i < nt; ++i)
         if (pts[i + 1] > pts[i])
           pool.emplace_back(join_probes, std::cref(probe), std::cref(indexed), std::cref(X), std::cref(T), self,
-                            std::cref(ir), pts[i], pts[i + 1] - 1, std::ref(parts[i]));
+                            std::cref(ir), pts[i], pts[i + 1] - 1, ppjoin, std::ref(parts[i]), std::ref(pc[i]));
       for (auto& th : pool) th.join();
     }
   }
   uint64_t count = 0;
   for (auto& p : parts) count += p.r.size();
+  if (counts)
+    for (const auto& c : pc) {
+      counts->visits += c.visits;
+      counts->candidates += c.candidates;
+    }
   if (parts_out) *parts_out = std::move(parts);
   return count;
 }
@@ -159,11 +194,15 @@ int hs_join_host(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* 
     else T.build(ctx->kind, ctx->mode, ctx->t, std::max<uint32_t>(L, 64));
     auto t0 = std::chrono::steady_clock::now();
     std::vector<Triples> parts;
+    HostCounts hc;
     const uint64_t count = host_join(*indexed, *probe, self, clamp_range(indexed_range, indexed->n),
-                                     clamp_range(probe_range, probe->n), T, threads, &parts);
+                                     clamp_range(probe_range, probe->n), T, threads, &parts,
+                                     ctx->host_algo == HS_HOST_PPJOIN, &hc);
     result->count = count;
     ctx->stats = hs_stats{};
     ctx->stats.pairs = count;
+    ctx->stats.posting_visits = hc.visits;
+    ctx->stats.candidates = hc.candidates;
     ctx->stats.host_filter_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->stats.total_ms = ctx->stats.host_filter_ms;

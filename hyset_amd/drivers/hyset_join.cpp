@@ -2,11 +2,19 @@
 // the reference's GPU drivers (src/bitmap.cu, src/fgss.cu, src/hybrid.cu):
 //
 //   hyset_join --input FILE [--threshold 0.95] [--block 10000]
-//              [--algorithm fgss|bitmap|hybrid] [--bitmap 64]
-//              [--scenario 1] [--threads 32] [--device-memory 512M]
+//              [--technique fgss|bitmap|hybrid|cpu] [--algorithm allpairs|ppjoin]
+//              [--bitmap 64] [--scenario 1] [--threads 32] [--device-memory 512M]
 //              [--output FILE] [--overlap]
 //              [--prepare words|qgrams|integers [--q 3] [--lowercase]]
 //              [--binary] [--save-binary FILE]
+//
+// --algorithm allpairs|ppjoin is the CPU filtering algorithm, as in the
+// reference's src/hybrid.cu:36 and src/cpu_single.cu:35: the hybrid
+// technique's host filter (the default technique when only --algorithm
+// allpairs|ppjoin is given, as the reference's hybrid driver) or the CPU
+// technique (cpu_single / cpu_multi: host::allpairs / host::ppjoin over
+// --threads threads).  --algorithm also accepts a technique name
+// (fgss|bitmap|hybrid|cpu), the same as --technique.
 //
 // --prepare reads raw records (one per line) and prepares them with
 // hs_preprocess_file (the step the reference leaves to external scripts,
@@ -19,8 +27,8 @@
 // (src/fgss.cu:108-116).  With --output the pairs are written as "r s" lines
 // (output.hpp:39-76; "r s overlap" with --overlap); otherwise the count is
 // printed.  The hybrid technique filters on the host (candidate generation
-// of hybrid::allpairs, hybrid_algorithms.hpp:209-291) and verifies the
-// batches on the device through the device_handler.
+// of hybrid::allpairs or hybrid::ppjoin, hybrid_algorithms.hpp:209-387) and
+// verifies the batches on the device through the device_handler.
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -36,7 +44,7 @@ namespace ha = hyset_amd;
 namespace {
 
 struct Options {
-  std::string input, output, algorithm = "fgss", deviceMemory = "512M";
+  std::string input, output, technique, filter = "allpairs", deviceMemory = "512M";
   std::string prepare, saveBinary;
   double threshold = 0.95;
   unsigned block = 10000, bitmap = 64, scenario = 1, threads = 32, q = 3;
@@ -45,7 +53,8 @@ struct Options {
 
 [[noreturn]] void usage(const char* msg) {
   std::fprintf(stderr, "error: %s\nusage: hyset_join --input FILE [--threshold T] [--block B] "
-                       "[--algorithm fgss|bitmap|hybrid] [--bitmap 64|128|256] [--scenario 1|2] [--threads N] "
+                       "[--technique fgss|bitmap|hybrid|cpu] [--algorithm allpairs|ppjoin] "
+                       "[--bitmap 64|128|256] [--scenario 1|2] [--threads N] "
                        "[--device-memory 512M] [--output FILE] [--overlap] "
                        "[--prepare words|qgrams|integers [--q N] [--lowercase]] [--binary] [--save-binary FILE]\n",
                msg);
@@ -62,7 +71,16 @@ Options parse(int argc, char** argv) {
     };
     if (a == "--input") o.input = val();
     else if (a == "--output") o.output = val();
-    else if (a == "--algorithm") o.algorithm = val();
+    else if (a == "--technique") o.technique = val();
+    else if (a == "--algorithm") {
+      const std::string v = val();
+      if (v == "allpairs" || v == "ppjoin") {
+        o.filter = v;
+        if (o.technique.empty()) o.technique = "hybrid-default";
+      } else {
+        o.technique = v;
+      }
+    }
     else if (a == "--threshold") o.threshold = std::strtod(val().c_str(), nullptr);
     else if (a == "--block") o.block = (unsigned)std::strtoul(val().c_str(), nullptr, 10);
     else if (a == "--bitmap") o.bitmap = (unsigned)std::strtoul(val().c_str(), nullptr, 10);
@@ -82,6 +100,9 @@ Options parse(int argc, char** argv) {
     usage("--prepare takes words, qgrams or integers");
   if (!o.prepare.empty() && o.binary) usage("--prepare reads raw text, --binary a prepared cache");
   if (o.block == 0) usage("--block must be positive");
+  // a CPU algorithm alone selects the hybrid technique, as the reference's hybrid driver
+  if (o.technique.empty()) o.technique = "fgss";
+  if (o.technique == "hybrid-default") o.technique = "hybrid";
   return o;
 }
 
@@ -94,11 +115,13 @@ unsigned long long candidates_for(const std::string& mem) {
   return (unsigned long long)(bytes / 4.0);
 }
 
-// Host candidate generation for the hybrid driver: AllPairs filtering
-// without verification (hybrid_algorithms.hpp:209-291) over a host index of
-// midprefix entries sorted by (token, set id).
-void hybrid_filter(const hs_host_collection& c, double t, ha::algorithms::hybrid::device_handler& dh) {
-  std::vector<unsigned> offs(c.universe + 1, 0), ids;
+// Host candidate generation for the hybrid driver: AllPairs or PPJoin
+// filtering without verification (hybrid_algorithms.hpp:209-387) over a
+// host index of midprefix entries sorted by (token, set id), each with its
+// position in its set; PPJoin takes a candidate at a common prefix token
+// only if the positional filter passes there (:333-348).
+void hybrid_filter(const hs_host_collection& c, double t, bool ppjoin, ha::algorithms::hybrid::device_handler& dh) {
+  std::vector<unsigned> offs(c.universe + 1, 0), ids, pos;
   std::vector<uint32_t> b(5);
   auto bnd = [&](uint32_t l1, uint32_t l2) {
     ha::check(hs_bounds(HS_JACCARD, t, HS_BOUNDS_REFERENCE, l1, l2, b.data()), nullptr, "hs_bounds");
@@ -117,9 +140,14 @@ void hybrid_filter(const hs_host_collection& c, double t, ha::algorithms::hybrid
     for (uint32_t k = 0; k < midprefix[c.sizes[i]]; ++k) offs[c.tokens[c.starts[i] + k] + 1]++;
   for (uint32_t u = 0; u < c.universe; ++u) offs[u + 1] += offs[u];
   ids.resize(offs[c.universe]);
+  pos.resize(offs[c.universe]);
   std::vector<unsigned> cur(offs.begin(), offs.end() - 1);
   for (uint32_t i = 0; i < c.n; ++i)
-    for (uint32_t k = 0; k < midprefix[c.sizes[i]]; ++k) ids[cur[c.tokens[c.starts[i] + k]]++] = i;
+    for (uint32_t k = 0; k < midprefix[c.sizes[i]]; ++k) {
+      const unsigned at = cur[c.tokens[c.starts[i] + k]]++;
+      ids[at] = i;
+      pos[at] = k;
+    }
   std::vector<unsigned> seen(c.n, 0xFFFFFFFFu), cand;
   for (uint32_t p = 0; p < c.n; ++p) {
     const uint32_t l = c.sizes[p];
@@ -130,10 +158,10 @@ void hybrid_filter(const hs_host_collection& c, double t, ha::algorithms::hybrid
         const unsigned s = ids[it];
         if (c.sizes[s] < minsize[l]) continue;
         if (s >= p || c.sizes[s] > maxsize[l]) break;
-        if (seen[s] != p) {
-          seen[s] = p;
-          cand.push_back(s);
-        }
+        if (seen[s] == p) continue;
+        if (ppjoin && bnd(l, c.sizes[s])[2] > std::min(l - j, c.sizes[s] - pos[it])) continue;
+        seen[s] = p;
+        cand.push_back(s);
       }
     }
     if (!cand.empty()) dh.addCandidates(p, cand);
@@ -191,24 +219,28 @@ int main(int argc, char** argv) {
 
     auto t0 = std::chrono::steady_clock::now();
     if (hc.n > 0) {
-      if (o.algorithm == "fgss") {
+      if (o.technique == "fgss") {
         ha::algorithms::device::fgssjoin::handler h(ctx, dc, o.block, aggregate, o.threshold);
         h.setOutputHandler(out);
         h.join(&part, &part);
-      } else if (o.algorithm == "bitmap") {
+      } else if (o.technique == "bitmap") {
         ha::algorithms::device::bitmap::handler h(ctx, dc, o.bitmap, o.block, aggregate, o.threshold);
         h.setOutputHandler(out);
         h.join(&part, &part);
-      } else if (o.algorithm == "hybrid") {
+      } else if (o.technique == "hybrid") {
         ha::algorithms::hybrid::device_handler dh(ctx, dc, dc, o.threads, o.scenario, aggregate, hc.n,
                                                   (unsigned)std::min<unsigned long long>(candidates_for(o.deviceMemory),
                                                                                          0xFFFFFFFFull),
                                                   o.threshold);
         dh.setMaxSetSize(hc.n ? hc.sizes[hc.n - 1] : 0);
         dh.setOutputHandler(out);
-        hybrid_filter(hc, o.threshold, dh);
+        hybrid_filter(hc, o.threshold, o.filter == "ppjoin", dh);
+      } else if (o.technique == "cpu") {
+        ha::algorithms::host::handler h(ctx, dc, o.threads, o.filter, aggregate, o.threshold);
+        h.setOutputHandler(out);
+        h.join(&part, &part);
       } else {
-        usage("unknown algorithm");
+        usage("unknown technique");
       }
     }
     auto t1 = std::chrono::steady_clock::now();

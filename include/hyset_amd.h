@@ -300,6 +300,20 @@ void hs_free_collection(hs_coll* coll);
 
 int hs_set_threshold(hs_ctx* ctx, int32_t sim, double threshold, int32_t bounds_mode);
 
+/* Candidate generation on the host cores: the reference's CPU filtering
+ * algorithm, --algorithm allpairs|ppjoin of src/hybrid.cu:36 and
+ * src/cpu_single.cu:35.  It applies to the host filter of HS_ALGO_HYBRID
+ * (hybrid::allpairs / hybrid::ppjoin, hybrid_algorithms.hpp:209-387), to
+ * hs_join_host (host::allpairs / host::ppjoin, host_algorithms.hpp:16-250)
+ * and to the host side of hs_join_coprocess.  HS_HOST_PPJOIN keeps a
+ * candidate only if PPJoin's positional filter (host_algorithms.hpp:196-199,
+ * helpers.hpp:46-50) passes at the first prefix token the probe and the
+ * candidate share: minoverlap(|r|, |s|) <= min(|r| - pos_r, |s| - pos_s).
+ * The pairs are the same either way; fewer candidates reach verification.
+ * The default is HS_HOST_ALLPAIRS (as the reference's drivers). */
+enum { HS_HOST_ALLPAIRS = 0, HS_HOST_PPJOIN = 1 };
+int hs_set_host_algorithm(hs_ctx* ctx, int32_t algorithm);
+
 /* Joins probe sets of `probe` (within probe_range) against the sets of
  * `indexed` (within indexed_range).  indexed == probe is a self-join and
  * keeps pairs with s < r only; otherwise every (r, s) is considered (R-S
@@ -379,10 +393,13 @@ int hs_server_wait(hs_server* server, uint64_t request_id, hs_result* result, do
 int hs_server_get_stats(hs_server* server, hs_server_stats* out);
 void hs_server_destroy(hs_server* server);
 
-/* The CPU partition: the same join on host cores (host::allpairs of
- * include/hyset/host_algorithms.hpp:16-129 with the thread layout of
+/* The CPU partition: the same join on host cores (host::allpairs, or
+ * host::ppjoin after hs_set_host_algorithm(HS_HOST_PPJOIN), of
+ * include/hyset/host_algorithms.hpp:16-250 with the thread layout of
  * src/cpu_multi.cu:118-207), with verification and exact overlaps.  Works on
- * device and host-only contexts. */
+ * device and host-only contexts.  hs_stats: pairs, posting_visits (postings
+ * inside the length windows), candidates (distinct candidates verified),
+ * host_filter_ms = total_ms (wall). */
 int hs_join_host(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* indexed_range,
                  const hs_range* probe_range, uint32_t threads, hs_result* result);
 

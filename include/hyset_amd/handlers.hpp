@@ -8,6 +8,8 @@
 //   algorithms::device::bitmap::handler               include/hyset/device_algorithms.hpp:120-260
 //   algorithms::device::fgssjoin::handler             include/hyset/device_algorithms.hpp:305-446
 //   algorithms::hybrid::device_handler                include/hyset/hybrid_algorithms.hpp:16-205
+//   algorithms::host::handler (allpairs / ppjoin)     include/hyset/host_algorithms.hpp:16-250,
+//                                                     src/cpu_single.cu:118-135, src/cpu_multi.cu:118-207
 //
 // Differences from the reference, all deliberate:
 //   * errors throw hyset_amd::error carrying the HS_E_* code and message
@@ -18,6 +20,8 @@
 //     device timer; timings are available through context::stats().
 #pragma once
 
+#include <algorithm>
+#include <future>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -246,8 +250,12 @@ struct handler {
 namespace hybrid {
 // CPU-filtered candidates verified on the device
 // (include/hyset/hybrid_algorithms.hpp:16-205).  Candidates accumulate in a
-// host batch of maxCandidates; a full batch, join() or flush() sends it to
-// hs_verify_candidates.
+// host batch of maxCandidates; a full batch or join() hands it to a worker
+// thread that verifies it (hs_verify_candidates) and delivers its pairs
+// while the caller goes on filtering into the other batch -- the overlap of
+// hybrid_algorithms.hpp:107-120, one batch in flight.  flush() returns once
+// every batch is delivered; an error of a batch surfaces from the next
+// join() or flush().
 struct device_handler {
   device_handler(context& ctx, collection::device_collection& indexed, collection::device_collection& probe,
                  unsigned int threads, unsigned int scenario, bool aggregate, unsigned int maxProbes,
@@ -275,20 +283,29 @@ struct device_handler {
       done += take;
     }
   }
+  ~device_handler() {
+    try {
+      wait();
+    } catch (...) {
+    }
+  }
+  device_handler(const device_handler&) = delete;
+  device_handler& operator=(const device_handler&) = delete;
   void join() {
     if (probes_.empty()) return;
-    check(hs_set_threshold(ctx_.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx_.get(), "hs_set_threshold");
-    hs_result res{};
-    res.count_only = 1;
-    check(hs_verify_candidates(ctx_.get(), indexed_.get(), probe_.get(), probes_.data(), offsets_.data(),
-                               candidates_.data(), (uint32_t)probes_.size(), (int32_t)scenario, &res),
-          ctx_.get(), "hs_verify_candidates");
+    wait();  // one verification at a time on the context; it delivers before the next starts
+    std::swap(probes_, f_probes_);
+    std::swap(offsets_, f_offsets_);
+    std::swap(candidates_, f_candidates_);
     probes_.clear();
     offsets_.clear();
     candidates_.clear();
-    deliver(res.count);
+    inflight_ = std::async(std::launch::async, [this] { verify_batch(); });
   }
-  void flush() { join(); }
+  void flush() {
+    join();
+    wait();
+  }
 
   unsigned int threads, scenario;
   bool aggregate;
@@ -296,6 +313,18 @@ struct device_handler {
   unsigned int maxProbes, maxCandidates, maxSetSize = 0;
 
  private:
+  void wait() {
+    if (inflight_.valid()) inflight_.get();  // rethrows the batch's error
+  }
+  void verify_batch() {
+    check(hs_set_threshold(ctx_.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx_.get(), "hs_set_threshold");
+    hs_result res{};
+    res.count_only = 1;
+    check(hs_verify_candidates(ctx_.get(), indexed_.get(), probe_.get(), f_probes_.data(), f_offsets_.data(),
+                               f_candidates_.data(), (uint32_t)f_probes_.size(), (int32_t)scenario, &res),
+          ctx_.get(), "hs_verify_candidates");
+    deliver(res.count);
+  }
   void deliver(uint64_t count) {
     if (!output_) throw error(HS_E_INVALID, "no output handler set");
     if (aggregate) {
@@ -307,9 +336,77 @@ struct device_handler {
   context& ctx_;
   collection::device_collection& indexed_;
   collection::device_collection& probe_;
-  std::vector<unsigned int> probes_, offsets_, candidates_;
+  std::vector<unsigned int> probes_, offsets_, candidates_;        // the batch being filled
+  std::vector<unsigned int> f_probes_, f_offsets_, f_candidates_;  // the batch in flight
+  std::future<void> inflight_;
   std::shared_ptr<output::handler> output_;
 };
 }  // namespace hybrid
+
+namespace host {
+// The CPU techniques of cpu_single / cpu_multi: host::allpairs or
+// host::ppjoin (include/hyset/host_algorithms.hpp:16-250) over `threads`
+// host threads (src/cpu_multi.cu:118-207), through hs_join_host.
+// algorithm is "allpairs" or "ppjoin" (the reference's --algorithm).
+struct handler {
+  handler(context& ctx, collection::device_collection& coll, unsigned int threads, const std::string& algorithm,
+          bool aggregate, double threshold)
+      : threads(threads), aggregate(aggregate), threshold(threshold), ctx_(ctx), coll_(coll) {
+    if (algorithm == "allpairs") algo_ = HS_HOST_ALLPAIRS;
+    else if (algorithm == "ppjoin") algo_ = HS_HOST_PPJOIN;
+    else throw error(HS_E_INVALID, "CPU algorithm must be allpairs or ppjoin");
+  }
+  void setOutputHandler(std::shared_ptr<output::handler>& h) { output_ = h; }
+  void join(const structs::partition* first, const structs::partition* second) {
+    run(detail::range_of(first), detail::range_of(second));
+  }
+  void join(const structs::block* first, const structs::block* second) {
+    run(detail::range_of(first), detail::range_of(second));
+  }
+  unsigned int threads;
+  bool aggregate;
+  double threshold;
+
+ private:
+  void run(hs_range ir, hs_range pr) {
+    check(hs_set_threshold(ctx_.get(), HS_JACCARD, threshold, HS_BOUNDS_REFERENCE), ctx_.get(), "hs_set_threshold");
+    check(hs_set_host_algorithm(ctx_.get(), algo_), ctx_.get(), "hs_set_host_algorithm");
+    if (!output_) throw error(HS_E_INVALID, "no output handler set");
+    if (aggregate) {
+      auto* c = dynamic_cast<output::count_handler*>(output_.get());
+      if (!c) throw error(HS_E_INVALID, "aggregate mode needs a count_handler");
+      hs_result res{};
+      res.count_only = 1;
+      check(hs_join_host(ctx_.get(), coll_.get(), coll_.get(), &ir, &pr, threads, &res), ctx_.get(), "hs_join_host");
+      c->count += res.count;
+      return;
+    }
+    auto* p = dynamic_cast<output::pairs_handler*>(output_.get());
+    if (!p) throw error(HS_E_INVALID, "pairs mode needs a pairs_handler");
+    // arrays for as many pairs as probes; a larger result runs again at its size
+    uint64_t cap = std::max<uint64_t>(1u << 16, pr.last >= pr.first ? (uint64_t)pr.last - pr.first + 1 : 0);
+    for (;;) {
+      std::vector<uint32_t> r(cap), s(cap), ov(cap);
+      hs_result res{};
+      res.capacity = cap;
+      res.r = r.data();
+      res.s = s.data();
+      res.overlap = ov.data();
+      const int rc = hs_join_host(ctx_.get(), coll_.get(), coll_.get(), &ir, &pr, threads, &res);
+      if (rc == HS_E_OVERFLOW && res.count > cap) {
+        cap = res.count;
+        continue;
+      }
+      check(rc, ctx_.get(), "hs_join_host");
+      for (uint64_t i = 0; i < res.count; ++i) p->pairs.push_back({r[i], s[i], ov[i]});
+      return;
+    }
+  }
+  context& ctx_;
+  collection::device_collection& coll_;
+  int32_t algo_ = HS_HOST_ALLPAIRS;
+  std::shared_ptr<output::handler> output_;
+};
+}  // namespace host
 }  // namespace algorithms
 }  // namespace hyset_amd

@@ -60,6 +60,27 @@ def test_static_split_self_join(ctx, col, share):
     assert info["device_claims"] == (0 if share == 1.0 else 1)
 
 
+@pytest.mark.parametrize("mode,kw", [("static", dict(cpu_share=0.5, cpu_threads=4)),
+                                     ("queue", dict(cpu_threads=3, cpu_block=64, device_min_claim=1000))])
+def test_host_side_ppjoin(ctx, col, mode, kw):
+    """The host threads filter as PPJoin (positions from the device-built
+    index's entries): the same pairs as the oracle, the host side's share
+    still present."""
+    c, d = col
+    try:
+        ctx.set_host_algorithm("ppjoin")
+        for t in (0.6, 0.8):
+            ctx.set_threshold(t)
+            want = O.self_join(c, t, algo=O.PPJOIN)
+            (r, s, ov), info = ctx.join_coprocess(d, mode=mode, **kw)
+            assert _triples(r, s, ov) == _want(want), (mode, t)
+            _check_stats(info, c.n, want.count)
+            if mode == "static":
+                assert info["host_probes"] == round(0.5 * c.n)
+    finally:
+        ctx.set_host_algorithm("allpairs")
+
+
 def test_rs_join_both_modes(ctx):
     R, S = H.generate_rs(H.gen_params(3000, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=3),
                          H.gen_params(2500, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=4),

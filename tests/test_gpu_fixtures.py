@@ -65,6 +65,28 @@ def test_cpp_driver(tmp_path, algo, extra):
         assert res.returncode == 0 and f"Result: {len(got)}" in res.stdout
 
 
+@pytest.mark.parametrize("args", [["--algorithm", "ppjoin"], ["--algorithm", "allpairs", "--scenario", "2"],
+                                  ["--technique", "cpu", "--algorithm", "allpairs", "--threads", "4"],
+                                  ["--technique", "cpu", "--algorithm", "ppjoin", "--threads", "1"]])
+def test_cpp_driver_cpu_algorithms(tmp_path, args):
+    """The reference's --algorithm allpairs|ppjoin: alone it selects the
+    hybrid technique with that host filter (src/hybrid.cu); with --technique
+    cpu the join runs on host threads (src/cpu_single.cu, src/cpu_multi.cu).
+    Pairs and overlaps as the golden fixture."""
+    col = tmp_path / "neardup.txt"
+    H.write_collection_text(col, F.neardup())
+    out = tmp_path / "pairs.txt"
+    for t in (0.7, 0.9):
+        res = subprocess.run([str(DRIVER), "--input", str(col), "--threshold", str(t), "--block", "1000",
+                              "--output", str(out), "--overlap", *args], capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0, res.stderr
+        got = sorted(tuple(int(x) for x in line.split()) for line in out.read_text().splitlines())
+        assert got == F.read_pairs(GOLD / f"neardup_t{t}.pairs"), (args, t)
+    res = subprocess.run([str(DRIVER), "--input", str(col), "--algorithm", "quicksort"], capture_output=True,
+                         text=True, timeout=60)
+    assert res.returncode != 0
+
+
 def _names(n, seed):
     """Strings of 8-20 letters, 30% of them copies of earlier ones with 1-2
     character edits (the string-matching use of the join)."""

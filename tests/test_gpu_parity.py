@@ -151,6 +151,30 @@ def test_hybrid_small(ctx, small, scenario, threads):
     _check(ctx.join(d, algo="hybrid", tune=tune), want, f"hybrid s{scenario} x{threads}")
 
 
+@pytest.mark.parametrize("threads", [1, 4])
+def test_hybrid_cpu_algorithms(ctx, small, threads):
+    """The hybrid technique's host filter as hybrid::allpairs and
+    hybrid::ppjoin (hybrid_algorithms.hpp:209-387): the oracle's pairs
+    either way, and the oracle's counts of postings visited and candidates
+    sent to the device (PPJoin's positional filter sends fewer)."""
+    c, d = small
+    try:
+        for t in (0.6, 0.8):
+            ctx.set_threshold(t)
+            cands = {}
+            for name, algo in (("allpairs", O.ALLPAIRS), ("ppjoin", O.PPJOIN)):
+                want = O.self_join(c, t, algo=algo)
+                ctx.set_host_algorithm(name)
+                tune = H.Tune(host_threads=threads, batch_candidates=5000)
+                _check(ctx.join(d, algo="hybrid", tune=tune), want, f"hybrid {name} t={t}")
+                st = ctx.stats()
+                assert st["candidates"] == want.candidates and st["posting_visits"] == want.posting_visits, name
+                cands[name] = st["candidates"]
+            assert cands["ppjoin"] < cands["allpairs"], t
+    finally:
+        ctx.set_host_algorithm("allpairs")
+
+
 def test_block_pairs(ctx, small):
     """join(block*, block*): index over one block, probes from a later block."""
     c, d = small

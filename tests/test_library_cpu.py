@@ -119,6 +119,39 @@ def test_pairs_text_writer_formatting(tmp_path):
         H.write_pairs_text(tmp_path / "no" / "such" / "dir.txt", [1], [2])
 
 
+@pytest.mark.parametrize("sim", ["jaccard", "cosine", "dice"])
+def test_host_join_ppjoin_matches_oracle(sim):
+    """hs_join_host with each CPU filtering algorithm (host::allpairs /
+    host::ppjoin): the oracle's pairs and overlaps, and the oracle's count of
+    postings visited and of candidates -- PPJoin's positional filter keeps
+    fewer candidates (the oracle restates host_algorithms.hpp:131-250)."""
+    import oracle as O
+    c = H.generate_self(H.gen_params(3000, size_dist=H.SIZE_UNIFORM, lo=2, hi=40, universe=800, zipf=1.0, seed=5))
+    with H.Context(-1) as ctx:
+        d = ctx.upload(c)
+        for t in (0.6, 0.8, 0.9):
+            ctx.set_threshold(t, sim)
+            fewer = False
+            for name, algo in (("allpairs", O.ALLPAIRS), ("ppjoin", O.PPJOIN)):
+                want = O.self_join(c, t, sim, algo=algo)
+                ctx.set_host_algorithm(name)
+                for threads in (1, 3):
+                    r, s, ov = H.sort_pairs(*ctx.join_host(d, threads=threads))
+                    assert np.array_equal(r, want.r) and np.array_equal(s, want.s), (name, t)
+                    assert np.array_equal(ov, want.overlap)
+                    st = ctx.stats()
+                    assert st["posting_visits"] == want.posting_visits, (name, t)
+                    assert st["candidates"] == want.candidates, (name, t)
+                if name == "allpairs":
+                    all_cands = want.candidates
+                else:
+                    fewer = want.candidates < all_cands
+            assert fewer, t
+        ctx.set_host_algorithm("allpairs")
+        with pytest.raises(H.HysetError):
+            ctx.set_host_algorithm(7)
+
+
 def test_host_join_matches_oracle():
     """hs_join_host on a host-only context: the library's own CPU partition."""
     import oracle as O
