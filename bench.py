@@ -85,19 +85,26 @@ def cpu_model() -> str:
 def bench_coprocess(ctx, dI, dP, name, threads, want, reps=3) -> dict:
     """The join shared by the device and `threads` host threads (0: the
     library's default, the usable CPUs less one) through the work-sharing
-    queue, against the device alone; both timed as wall clock around the
-    call, median of `reps`."""
-    alone, shared, infos = [], [], []
+    queue, forced to share (the gate off), against the device alone and
+    against the call with the library's break-even gate on (its default:
+    the device alone unless sharing measured faster); all timed as wall
+    clock around the call, median of `reps`."""
+    alone, shared, gated, infos, ginfos = [], [], [], [], []
     for _ in range(reps + 1):
         a = time.perf_counter()
         assert ctx.join(dI, dP, algo="fgss", count_only=True) == want
         alone.append((time.perf_counter() - a) * 1e3)
         a = time.perf_counter()
-        c, info = ctx.join_coprocess(dI, dP, cpu_threads=threads, count_only=True)
+        c, info = ctx.join_coprocess(dI, dP, cpu_threads=threads, count_only=True, min_device_ms=-1)
         shared.append((time.perf_counter() - a) * 1e3)
         assert c == want
         infos.append(info)
-    alone, shared, infos = alone[1:], shared[1:], infos[1:]
+        a = time.perf_counter()
+        c, ginfo = ctx.join_coprocess(dI, dP, cpu_threads=threads, count_only=True)
+        gated.append((time.perf_counter() - a) * 1e3)
+        assert c == want
+        ginfos.append(ginfo)
+    alone, shared, infos, gated, ginfos = alone[1:], shared[1:], infos[1:], gated[1:], ginfos[1:]
     k = sorted(range(reps), key=lambda i: shared[i])[reps // 2]
     info = infos[k]
     n = info["device_probes"] + info["host_probes"]
@@ -110,10 +117,14 @@ def bench_coprocess(ctx, dI, dP, name, threads, want, reps=3) -> dict:
            f"{name}_coprocess_host_claims": int(info["host_claims"]),
            f"{name}_coprocess_index_copy_ms": round(info["copy_ms"], 3),
            f"{name}_coprocess_device_busy_ms": round(info["device_ms"], 3),
-           f"{name}_coprocess_host_busy_ms": round(info["host_ms"], 3)}
+           f"{name}_coprocess_host_busy_ms": round(info["host_ms"], 3),
+           f"{name}_coprocess_gated_ms": round(statistics.median(gated), 3),
+           f"{name}_coprocess_gated_shared_calls": sum(1 - g["gated"] for g in ginfos)}
     log(f"{name}: device alone {res[f'{name}_device_alone_ms']} ms, shared with {info['host_threads']} host "
         f"threads {res[f'{name}_coprocess_ms']} ms (host took {info['host_probes']} of {n} probes, "
-        f"{info['host_pairs']} pairs; index copy {info['copy_ms']:.3f} ms)")
+        f"{info['host_pairs']} pairs; index copy {info['copy_ms']:.3f} ms); with the break-even gate "
+        f"{res[f'{name}_coprocess_gated_ms']} ms ({res[f'{name}_coprocess_gated_shared_calls']} of {reps} calls "
+        f"shared)")
     return res
 
 
@@ -238,7 +249,8 @@ def bench_split(ctx, H, units, want_counts) -> dict:
     t0 = time.perf_counter()
     train = [ctx.upload(c) for c in W.batch(k=192, seed=1001)]
     Ftr, Ttr, _ = S.measure_units(ctx, train, 0.75)
-    cpu_scale = S.calibrate_cpu_scale(ctx, train, Ttr, threads)
+    cal = S.calibrate_concurrency(ctx, train, Ttr, threads)
+    cpu_scale = cal["cpu"]
     for d in train:
         d.free()
     t_train_mix = time.perf_counter() - t0
@@ -248,15 +260,16 @@ def bench_split(ctx, H, units, want_counts) -> dict:
     F, T, counts = S.measure_units(ctx, units, 0.75)
     assert np.array_equal(counts.astype(np.uint64), want_counts)
     log(f"split: timed {len(train)} training + {len(units)} batch joins x 4 techniques in "
-        f"{time.perf_counter() - t0:.1f}s; CPU slowdown under load {cpu_scale:.2f}; fit {t_fit:.1f}s, "
+        f"{time.perf_counter() - t0:.1f}s; concurrency scales {cal}; fit {t_fit:.1f}s, "
         f"held-out log-MSE {sp.val}")
     pa, pl = sp.analytic.predict(F), sp.learned.predict(F)
     plans = {"all_fgss": np.full(len(units), H.TECH_FGSS, dtype=np.int32),
              "analytic_argmin": S.argmin_plan(pa), "learned_argmin": S.argmin_plan(pl),
-             "analytic_concurrent": S.concurrent_plan(pa, threads, cpu_scale),
-             "learned_concurrent": S.concurrent_plan(pl, threads, cpu_scale),
-             "measured_best_concurrent": S.concurrent_plan(T, threads, cpu_scale)}
-    summary = S.summarize(plans, T, threads, cpu_scale)
+             "analytic_concurrent": S.concurrent_plan(pa, threads, cpu_scale, cal["gpu_loaded"], cal["gpu_alone"]),
+             "learned_concurrent": S.concurrent_plan(pl, threads, cpu_scale, cal["gpu_loaded"], cal["gpu_alone"]),
+             "measured_best_concurrent": S.concurrent_plan(T, threads, cpu_scale, cal["gpu_loaded"],
+                                                           cal["gpu_alone"])}
+    summary = S.summarize(plans, T, threads, cpu_scale, cal["gpu_loaded"], cal["gpu_alone"])
     for name, plan in plans.items():
         got, _, wall = ctx.split_run(units, plan, cpu_threads=threads)
         assert np.array_equal(got, want_counts), name
@@ -266,7 +279,8 @@ def bench_split(ctx, H, units, want_counts) -> dict:
             f"share {summary[name]['share']}")
     judge = sp.judge(F, T)
     log(f"split judge: {judge}")
-    return {"split": {"cpu_threads": threads, "cpu_scale": round(cpu_scale, 3), "training_units": len(train),
+    return {"split": {"cpu_threads": threads, "concurrency_scales": {k: round(v, 3) for k, v in cal.items()},
+                      "training_units": len(train),
                       "units": len(units),
                       "training_mix_s": round(t_train_mix, 1), "fit_s": round(t_fit, 2),
                       "heldout_log_mse": {k: round(v, 4) for k, v in sp.val.items()},

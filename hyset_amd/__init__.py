@@ -140,7 +140,8 @@ _sig("hs_copy_last_result", C.c_int, _vp, C.POINTER(Result))
 
 class Coprocess(C.Structure):
     _fields_ = [("mode", C.c_int32), ("algo", C.c_int32), ("cpu_threads", C.c_uint32), ("cpu_block", C.c_uint32),
-                ("device_min_claim", C.c_uint32), ("cpu_share", C.c_double), ("tune", Tune)]
+                ("device_min_claim", C.c_uint32), ("cpu_share", C.c_double), ("tune", Tune),
+                ("min_device_ms", C.c_double)]
 
 
 class CoprocessStats(C.Structure):
@@ -148,7 +149,8 @@ class CoprocessStats(C.Structure):
                 ("host_probes", C.c_uint32), ("device_claims", C.c_uint32), ("host_claims", C.c_uint32),
                 ("split", C.c_uint32), ("host_threads", C.c_uint32), ("index_ms", C.c_double),
                 ("copy_ms", C.c_double), ("device_ms", C.c_double), ("host_ms", C.c_double),
-                ("wall_ms", C.c_double)]
+                ("wall_ms", C.c_double), ("device_alone_ms", C.c_double), ("shared_ms", C.c_double),
+                ("gated", C.c_uint32), ("reserved", C.c_uint32)]
 
 
 COPROC_QUEUE, COPROC_STATIC = 0, 1
@@ -462,13 +464,16 @@ class Context:
 
     def join_coprocess(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, mode="queue",
                        algo="fgss", cpu_threads=0, cpu_block=0, device_min_claim=0, cpu_share=0.0,
-                       tune: Tune | None = None, count_only=False):
+                       tune: Tune | None = None, count_only=False, min_device_ms=0.0):
         """One join shared by the device and host threads (hs_join_coprocess).
-        Returns ((r, s, overlap) or the pair count, stats dict)."""
+        min_device_ms: the queue mode's break-even gate (0: the library's
+        default, < 0: always share).  Returns ((r, s, overlap) or the pair
+        count, stats dict)."""
         probe = indexed if probe is None else probe
         opt = Coprocess(_COPROC_MODES[mode] if isinstance(mode, str) else int(mode),
                         ALGOS[algo] if isinstance(algo, str) else int(algo), int(cpu_threads), int(cpu_block),
-                        int(device_min_claim), float(cpu_share), tune if tune is not None else Tune())
+                        int(device_min_claim), float(cpu_share), tune if tune is not None else Tune(),
+                        float(min_device_ms))
         res = Result()
         res.count_only = 1
         st = CoprocessStats()

@@ -19,6 +19,8 @@
 // which the result sinks deliver after the device's pairs.
 #include <sched.h>
 
+#include <cstdio>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -106,6 +108,21 @@ struct HostWorker {
 // claim at the thread's last measured rate.
 constexpr double kHostClaimMs = 0.1;
 
+// The break-even gate's default: a join that takes the device alone less
+// than this is not shared.
+constexpr double kGateMs = 50.0;
+
+// The gate's key of a join: the collections, technique, threshold and the
+// knobs that change either side's speed.
+std::string coproc_key(const hs_ctx* ctx, const hs_coll* indexed, const hs_coll* probe, const hs_coprocess& o) {
+  char buf[256];
+  std::snprintf(buf, sizeof(buf), "%p/%u/%llu|%p/%u/%llu|a%d|s%d|m%d|t%.17g|f%u|b%u|c%u|h%d", (const void*)indexed,
+                indexed->n, (unsigned long long)indexed->n_tokens, (const void*)probe, probe->n,
+                (unsigned long long)probe->n_tokens, o.algo, ctx->kind, ctx->mode, ctx->t, o.tune.flags,
+                o.tune.signature_bits, o.cpu_threads, ctx->host_algo);
+  return buf;
+}
+
 }  // namespace
 
 extern "C" {
@@ -143,6 +160,35 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
       cs.wall_ms = ms_between(t_start, Clock::now());
       if (stats) *stats = cs;
       return;
+    }
+
+    // ---- the break-even gate: the device alone unless sharing measured faster
+    hs_ctx::CoprocHistory* hist = nullptr;
+    if (opt.mode == HS_COPROC_QUEUE && opt.min_device_ms >= 0.0) {
+      hist = &ctx->coproc_hist[coproc_key(ctx, indexed, probe, opt)];
+      const double gate_ms = opt.min_device_ms > 0.0 ? opt.min_device_ms : kGateMs;
+      const bool alone = hist->device_ms < 0.0 || hist->device_ms < gate_ms ||
+                         (hist->shared_ms >= 0.0 && hist->shared_ms >= hist->device_ms);
+      if (alone) {
+        const auto t0 = Clock::now();
+        const int rc = hs_join(ctx, indexed, probe, nullptr, nullptr, opt.algo, &tune, result);
+        if (rc != HS_OK) throw Error{rc, ctx->err};
+        const double ms = ms_between(t0, Clock::now());
+        // keep the faster of the measurements (the first call also pays warm-up)
+        hist->device_ms = hist->device_ms < 0.0 ? ms : std::min(hist->device_ms, ms);
+        cs.gated = 1;
+        cs.device_pairs = result->count;
+        cs.device_probes = pr.last - pr.first + 1;
+        cs.device_claims = 1;
+        cs.split = pr.first;
+        cs.device_ms = ms;
+        cs.index_ms = ctx->stats.index_ms;
+        cs.device_alone_ms = hist->device_ms;
+        cs.shared_ms = std::max(hist->shared_ms, 0.0);
+        cs.wall_ms = ms_between(t_start, Clock::now());
+        if (stats) *stats = cs;
+        return;
+      }
     }
     if (!ctx->copy_stream) HS_CHECK_HIP(hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
     if (!ctx->index_built) HS_CHECK_HIP(hipEventCreate(&ctx->index_built));
@@ -368,6 +414,11 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     cs.copy_ms = elapsed(ctx->index_built, ctx->index_copied);
     cs.device_ms = ms_between(t_dev, t_dev_end);
     cs.wall_ms = ms_between(t_start, Clock::now());
+    if (hist) {
+      hist->shared_ms = cs.wall_ms;
+      cs.device_alone_ms = std::max(hist->device_ms, 0.0);
+      cs.shared_ms = cs.wall_ms;
+    }
     ctx->stats.pairs = total;
     ctx->stats.index_ms = cs.index_ms;
     ctx->stats.total_ms = cs.wall_ms;

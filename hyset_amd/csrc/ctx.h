@@ -112,6 +112,12 @@ struct hs_ctx {
   int kind = HS_JACCARD, mode = HS_BOUNDS_REFERENCE;
   double t = 0.0;
   int32_t host_algo = HS_HOST_ALLPAIRS;  // hs_set_host_algorithm
+  // co-processing's break-even gate: per join key, the measured wall times
+  // of the device alone and of the last co-processed run (< 0: none yet)
+  struct CoprocHistory {
+    double device_ms = -1.0, shared_ms = -1.0;
+  };
+  std::map<std::string, CoprocHistory> coproc_hist;
   hs::BoundTables tables;
   bool tables_valid = false;
   DevBuf d_tables;

@@ -162,43 +162,54 @@ def argmin_plan(pred: np.ndarray) -> np.ndarray:
     return np.argmin(pred, axis=1).astype(np.int32)
 
 
-def concurrent_plan(pred: np.ndarray, cpu_threads: int, cpu_scale: float = 1.0) -> np.ndarray:
+def concurrent_plan(pred: np.ndarray, cpu_threads: int, cpu_scale: float = 1.0, gpu_scale: float = 1.0,
+                    gpu_alone_scale: float | None = None, margin: float = 0.05) -> np.ndarray:
     """Longest-first list scheduling of units onto one device queue and
     cpu_threads host threads, by predicted time: each unit goes where it
     would finish first (the device takes its cheapest device technique).
     Runs the split the way hs_split_run executes it: CPU units concurrently
-    with the device units.  cpu_scale stretches the CPU predictions (times
-    measured on one idle core) to what a core achieves while all the others
-    are busy too (calibrate_cpu_scale)."""
+    with the device units.  The scales map the predictions (times of one
+    unit at a time) to the concurrent run (calibrate_concurrency): cpu_scale
+    for a CPU unit while the other host threads are busy, gpu_scale for a
+    device unit while they are.  With gpu_alone_scale (a device unit in a
+    run of device units alone) the plan falls back to every unit on its
+    device technique unless its predicted makespan beats that run by the
+    margin: sharing must pay for itself."""
     pred = np.asarray(pred, dtype=np.float64)
     gtech = np.array(GPU_TECHS)[np.argmin(pred[:, list(GPU_TECHS)], axis=1)]
     g = pred[np.arange(len(pred)), gtech]
+    gl = g * gpu_scale
     c = pred[:, TECH_CPU] * cpu_scale
     plan = gtech.astype(np.int32).copy()
     gpu_load = 0.0
     cpu = [0.0] * max(1, cpu_threads)
-    for i in np.argsort(-np.minimum(g, c), kind="stable"):
-        if cpu_threads > 0 and cpu[0] + c[i] < gpu_load + g[i]:
+    for i in np.argsort(-np.minimum(gl, c), kind="stable"):
+        if cpu_threads > 0 and cpu[0] + c[i] < gpu_load + gl[i]:
             plan[i] = TECH_CPU
             heapq.heapreplace(cpu, cpu[0] + c[i])
         else:
-            gpu_load += g[i]
+            gpu_load += gl[i]
+    if gpu_alone_scale is not None and (plan == TECH_CPU).any():
+        alone = float(g.sum() * gpu_alone_scale)
+        if max(gpu_load, max(cpu)) > alone * (1.0 - margin):
+            return gtech.astype(np.int32)
     return plan
 
 
-def plan_cost(plan: np.ndarray, T: np.ndarray, cpu_threads: int = 0, cpu_scale: float = 1.0) -> float:
+def plan_cost(plan: np.ndarray, T: np.ndarray, cpu_threads: int = 0, cpu_scale: float = 1.0,
+              gpu_scale: float = 1.0, gpu_alone_scale: float = 1.0) -> float:
     """Measured cost of a plan: the sum of its units' times (sequential, as
     cost_model.cu runs them) or, with cpu_threads > 0, the makespan of the
     device queue against the host threads (list-scheduled longest first,
-    CPU times stretched by cpu_scale)."""
+    CPU times stretched by cpu_scale, device times by gpu_scale when the
+    plan has CPU units and by gpu_alone_scale when it has none)."""
     t = T[np.arange(len(plan)), plan]
-    if cpu_threads > 0:
-        t = np.where(plan == TECH_CPU, t * cpu_scale, t)
     if cpu_threads <= 0:
         return float(t.sum())
-    gpu = float(t[plan != TECH_CPU].sum())
+    on_cpu = plan == TECH_CPU
+    gpu = float(t[~on_cpu].sum()) * (gpu_scale if on_cpu.any() else gpu_alone_scale)
     cpu = [0.0] * cpu_threads
-    for x in sorted(t[plan == TECH_CPU], reverse=True):
+    for x in sorted(t[on_cpu] * cpu_scale, reverse=True):
         heapq.heapreplace(cpu, cpu[0] + x)
     return max(gpu, max(cpu))
 
@@ -212,6 +223,32 @@ def calibrate_cpu_scale(ctx: Context, units, T: np.ndarray, cpu_threads: int) ->
     if not np.any(m):
         return 1.0
     return float(max(1.0, np.median(unit_ms[m] / T[m, TECH_CPU])))
+
+
+def calibrate_concurrency(ctx: Context, units, T: np.ndarray, cpu_threads: int) -> dict:
+    """The scale factors of concurrent_plan, measured with hs_split_run on
+    units whose one-at-a-time times T are known: 'gpu_alone', device units
+    (each on its fastest device technique) run back to back with no CPU
+    units; 'gpu_loaded' and 'cpu', the device and the CPU units of a mixed
+    plan (the one concurrent_plan makes of T without interference) run
+    together -- sums of measured over one-at-a-time times for the device
+    queue, the median ratio for CPU units of at least 0.5 ms."""
+    T = np.asarray(T, dtype=np.float64)
+    r = np.arange(len(units))
+    dev = np.array(GPU_TECHS)[np.argmin(T[:, list(GPU_TECHS)], axis=1)].astype(np.int32)
+    _, ms, _ = ctx.split_run(units, dev, cpu_threads=cpu_threads)
+    gpu_alone = float(ms.sum() / max(T[r, dev].sum(), 1e-9))
+    mixed = concurrent_plan(T, cpu_threads, 1.0, gpu_alone)
+    out = {"gpu_alone": gpu_alone, "gpu_loaded": gpu_alone, "cpu": 1.0}
+    on_cpu = mixed == TECH_CPU
+    if on_cpu.any() and (~on_cpu).any():
+        _, ms, _ = ctx.split_run(units, mixed, cpu_threads=cpu_threads)
+        out["gpu_loaded"] = float(max(ms[~on_cpu].sum() / max(T[r[~on_cpu], mixed[~on_cpu]].sum(), 1e-9),
+                                      gpu_alone))
+        m = on_cpu & (T[:, TECH_CPU] >= 0.5)
+        if m.any():
+            out["cpu"] = float(max(1.0, np.median(ms[m] / T[m, TECH_CPU])))
+    return out
 
 
 class Split:
@@ -263,14 +300,16 @@ class Split:
                 "analytic_mispredicted": int((pa != best).sum())}
 
 
-def summarize(plans: dict, T: np.ndarray, cpu_threads: int, cpu_scale: float = 1.0) -> dict:
+def summarize(plans: dict, T: np.ndarray, cpu_threads: int, cpu_scale: float = 1.0, gpu_scale: float = 1.0,
+              gpu_alone_scale: float = 1.0) -> dict:
     """Measured cost of each plan against the best per-unit choice."""
     best = float(T.min(axis=1).sum())
     out = {}
     for name, plan in plans.items():
         seq = plan_cost(plan, T)
         out[name] = {"sum_ms": round(seq, 3), "regret_ms": round(seq - best, 3),
-                     "makespan_ms": round(plan_cost(plan, T, cpu_threads, cpu_scale), 3),
+                     "makespan_ms": round(plan_cost(plan, T, cpu_threads, cpu_scale, gpu_scale, gpu_alone_scale),
+                                          3),
                      "share": {TECHNIQUES[k]: int((plan == k).sum()) for k in range(N_TECH)}}
     return out
 
@@ -281,5 +320,5 @@ def default_threads() -> int:
 
 
 __all__ = ["AnalyticModel", "LearnedModel", "Split", "measure_units", "argmin_plan", "concurrent_plan",
-           "calibrate_cpu_scale",
+           "calibrate_cpu_scale", "calibrate_concurrency",
            "plan_cost", "summarize", "log_error", "reference_terms", "learned_inputs", "default_threads"]

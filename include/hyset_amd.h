@@ -429,7 +429,18 @@ int hs_join_host(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* 
  * host memory, no copy to the device), count-only or copied into result as
  * for hs_join, and hs_copy_last_result, hs_stream_last_result and
  * hs_write_last_result_text deliver it in that order.  Each device stripe chooses its tile
- * plan as hs_join does for a probe range. */
+ * plan as hs_join does for a probe range.
+ *
+ * Break-even gate (queue mode, min_device_ms >= 0): sharing pays only when
+ * the host's share outweighs its fixed cost (the index copy, the threads,
+ * and the device's stripes missing the whole-collection token-major plan),
+ * so the context remembers, per join (collections, technique, threshold,
+ * knobs), the measured wall time of the device alone and of the last
+ * co-processed run, and runs the device alone (hs_join) when (1) the join
+ * has no device-alone time yet (the call measures it), (2) that time is
+ * below min_device_ms (0: 50 ms), or (3) a measured co-processed run was
+ * not faster.  Otherwise it co-processes and records the time.
+ * min_device_ms < 0 always co-processes; static mode is never gated. */
 enum { HS_COPROC_QUEUE = 0, HS_COPROC_STATIC = 1 };
 typedef struct {
   int32_t mode;              /* HS_COPROC_* */
@@ -439,6 +450,7 @@ typedef struct {
   uint32_t device_min_claim; /* queue: smallest device claim in probe sets (0: 16384) */
   double cpu_share;          /* static: fraction of the probe sets the host joins */
   hs_tune tune;              /* device knobs */
+  double min_device_ms;      /* queue: the break-even gate (0: 50 ms; < 0: always co-process) */
 } hs_coprocess;
 
 typedef struct {
@@ -452,6 +464,10 @@ typedef struct {
   double device_ms; /* first device claim to the device's last stripe done */
   double host_ms;   /* host threads: start to the last stripe done */
   double wall_ms;   /* the whole call */
+  double device_alone_ms; /* the gate's measured device-alone wall time of this join (0: none yet) */
+  double shared_ms;       /* the gate's last measured co-processed wall time (0: none yet) */
+  uint32_t gated;         /* 1: the gate ran the device alone */
+  uint32_t reserved;
 } hs_coprocess_stats;
 
 int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_coprocess* options,

@@ -101,6 +101,23 @@ def test_concurrent_plan_never_worse_than_all_device_by_prediction():
             assert S.plan_cost(plan, pred, threads) <= S.plan_cost(gbest, pred, threads) + 1e-9
 
 
+def test_concurrent_plan_falls_back_when_sharing_does_not_pay():
+    """With the measured scale of a device-only run, a mixed plan is kept
+    only when its predicted makespan beats that run by the margin."""
+    # four units: 10 ms on the device, 30 ms on one host thread; 2 threads
+    pred = np.array([[10.0, 50, 50, 30.0]] * 4)
+    mixed = S.concurrent_plan(pred, 2)
+    assert (mixed == H.TECH_CPU).sum() == 1  # makespan 30 against 40 for the device alone
+    # device units 2.5x slower while the host threads run: sharing no longer pays
+    assert (S.concurrent_plan(pred, 2, cpu_scale=1.0, gpu_scale=2.5, gpu_alone_scale=1.0) == H.TECH_FGSS).all()
+    # without that slowdown the mixed plan is kept
+    np.testing.assert_array_equal(S.concurrent_plan(pred, 2, cpu_scale=1.0, gpu_scale=1.0, gpu_alone_scale=1.0),
+                                  mixed)
+    # the measured cost of each plan uses the scale of the run it makes
+    assert S.plan_cost(np.zeros(4, dtype=np.int32), pred, 2, 1.0, 2.5, 1.0) == pytest.approx(40.0)
+    assert S.plan_cost(mixed, pred, 2, 1.0, 2.5, 1.0) == pytest.approx(75.0)
+
+
 def test_plan_cost_makespan_brute_force():
     T = np.array([[2.0, 9, 9, 3], [4.0, 9, 9, 1], [1.0, 9, 9, 5], [3.0, 9, 9, 2]])
     best = min(S.plan_cost(np.array(p), T, 1)

@@ -34,8 +34,9 @@ def _check_stats(info, n, count):
 
 
 @pytest.mark.parametrize("algo", ["fgss", "bitmap"])
-@pytest.mark.parametrize("kw", [dict(cpu_threads=4), dict(cpu_threads=1, cpu_block=1, device_min_claim=256),
-                                dict(cpu_threads=3, cpu_block=64, device_min_claim=1000)])
+@pytest.mark.parametrize("kw", [dict(cpu_threads=4, min_device_ms=-1),
+                                dict(cpu_threads=1, cpu_block=1, device_min_claim=256, min_device_ms=-1),
+                                dict(cpu_threads=3, cpu_block=64, device_min_claim=1000, min_device_ms=-1)])
 def test_queue_self_join(ctx, col, algo, kw):
     c, d = col
     for t in (0.6, 0.8):
@@ -61,7 +62,8 @@ def test_static_split_self_join(ctx, col, share):
 
 
 @pytest.mark.parametrize("mode,kw", [("static", dict(cpu_share=0.5, cpu_threads=4)),
-                                     ("queue", dict(cpu_threads=3, cpu_block=64, device_min_claim=1000))])
+                                     ("queue", dict(cpu_threads=3, cpu_block=64, device_min_claim=1000,
+                                                    min_device_ms=-1))])
 def test_host_side_ppjoin(ctx, col, mode, kw):
     """The host threads filter as PPJoin (positions from the device-built
     index's entries): the same pairs as the oracle, the host side's share
@@ -81,6 +83,30 @@ def test_host_side_ppjoin(ctx, col, mode, kw):
         ctx.set_host_algorithm("allpairs")
 
 
+def test_break_even_gate(ctx, col):
+    """Queue mode with the gate: a join with no measured device-alone time
+    runs on the device alone and records it; one below min_device_ms stays
+    there; above it the next call shares once, and the one after goes by the
+    two measured wall times.  Every answer is the oracle's."""
+    c, d = col
+    ctx.set_threshold(0.7)
+    want = _want(O.self_join(c, 0.7))
+    for _ in range(2):  # the default gate (50 ms): this 20K-set join never shares
+        (r, s, ov), info = ctx.join_coprocess(d, cpu_threads=2)
+        assert _triples(r, s, ov) == want
+        assert info["gated"] == 1 and info["host_probes"] == 0 and info["device_probes"] == c.n
+        assert info["device_alone_ms"] > 0
+    kw = dict(cpu_threads=3, cpu_block=64, min_device_ms=1e-6)
+    (r, s, ov), i1 = ctx.join_coprocess(d, **kw)  # a new join key: measured alone first
+    assert _triples(r, s, ov) == want and i1["gated"] == 1
+    (r, s, ov), i2 = ctx.join_coprocess(d, **kw)  # long enough by this gate: shares once
+    assert _triples(r, s, ov) == want and i2["gated"] == 0 and i2["shared_ms"] > 0
+    _check_stats(i2, c.n, len(want))
+    (r, s, ov), i3 = ctx.join_coprocess(d, **kw)  # by the measured times
+    assert _triples(r, s, ov) == want
+    assert i3["gated"] == int(i2["shared_ms"] >= i2["device_alone_ms"]), (i2, i3)
+
+
 def test_rs_join_both_modes(ctx):
     R, S = H.generate_rs(H.gen_params(3000, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=3),
                          H.gen_params(2500, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=4),
@@ -91,7 +117,7 @@ def test_rs_join_both_modes(ctx):
             ctx.set_threshold(0.7, sim)
             want = O.rs_join(R, S, 0.7, sim)
             assert want.count > 100
-            for kw in (dict(mode="queue", cpu_threads=2, cpu_block=8, device_min_claim=300),
+            for kw in (dict(mode="queue", cpu_threads=2, cpu_block=8, device_min_claim=300, min_device_ms=-1),
                        dict(mode="static", cpu_share=0.3, cpu_threads=2)):
                 (r, s, ov), info = ctx.join_coprocess(dS, dR, **kw)
                 assert _triples(r, s, ov) == _want(want), (sim, kw)

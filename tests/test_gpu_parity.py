@@ -423,7 +423,8 @@ def test_degenerate_inputs(ctx, algo):
                 want = O.self_join(c, t)
                 _check(ctx.join(d, algo=algo), want, f"{algo} {name} t={t}")
                 if algo != "hybrid":
-                    (r, s, ov), _ = ctx.join_coprocess(d, algo=algo, cpu_threads=2, cpu_block=7, device_min_claim=64)
+                    (r, s, ov), _ = ctx.join_coprocess(d, algo=algo, cpu_threads=2, cpu_block=7, device_min_claim=64,
+                                                       min_device_ms=-1)
                     _check((r, s, ov), want, f"coprocess {algo} {name} t={t}")
         finally:
             d.free()

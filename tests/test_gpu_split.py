@@ -149,3 +149,46 @@ def test_measured_split_end_to_end(ctx):
     np.testing.assert_array_equal(got, counts.astype(np.uint64))
     j = sp.judge(F, T)
     assert j["units"] == 8 and j["learned_right"] + j["analytic_right"] <= j["disagree"] <= 8
+
+
+def test_learned_plan_on_measured_timings(ctx):
+    """The split trained on measured unit timings (no synthetic costs): on
+    held-out units the learned model's concurrent plan, run by
+    hs_split_run with the concurrency scales calibrated on the training
+    units, takes no longer than the analytic model's plan or every unit on
+    the device (median of three runs each, within 10% + 1 ms), and every
+    plan's pair counts are exact."""
+    rng = np.random.default_rng(21)
+    specs = [(int(np.exp(rng.uniform(np.log(300), np.log(20_000)))), int(rng.integers(6, 40)),
+              float(rng.uniform(0.8, 1.3))) for _ in range(48)]
+    colls = [_gen(n, 200 + i, median=m, zipf=z) for i, (n, m, z) in enumerate(specs)]
+    units = [ctx.upload(c) for c in colls]
+    try:
+        F, T, counts = S.measure_units(ctx, units, 0.75)
+        te = np.arange(0, 48, 3)  # 16 held-out units
+        tr = np.setdiff1d(np.arange(48), te)  # 32 training units
+        sp = S.Split(ctx, epochs=1500).fit(F[tr], T[tr])
+        threads = 3
+        cal = S.calibrate_concurrency(ctx, [units[i] for i in tr], T[tr], threads)
+        assert cal["gpu_alone"] > 0 and cal["gpu_loaded"] >= cal["gpu_alone"] and cal["cpu"] >= 1.0
+        ev = [units[i] for i in te]
+
+        def plan_of(pred):
+            return S.concurrent_plan(pred, threads, cal["cpu"], cal["gpu_loaded"], cal["gpu_alone"])
+
+        plans = {"learned": plan_of(sp.learned.predict(F[te])), "analytic": plan_of(sp.analytic.predict(F[te])),
+                 "device": np.full(len(te), H.TECH_FGSS, dtype=np.int32)}
+        wall = {}
+        for name, plan in plans.items():
+            runs = []
+            for _ in range(3):
+                got, _, w = ctx.split_run(ev, plan, cpu_threads=threads)
+                np.testing.assert_array_equal(got, counts[te].astype(np.uint64), err_msg=name)
+                runs.append(w)
+            wall[name] = float(np.median(runs))
+        assert wall["learned"] <= 1.1 * wall["analytic"] + 1.0, wall
+        assert wall["learned"] <= 1.1 * wall["device"] + 1.0, wall
+    finally:
+        for u in units:
+            u.free()
+
