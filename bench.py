@@ -289,11 +289,14 @@ def bench_split(ctx, H, units, want_counts) -> dict:
                       "uses_learned": sp.use_learned, "plans": summary, "judge": judge}}
 
 
-def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=2000, offered=None) -> dict:
+def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=2000, offered=(40_000.0, 80_000.0)) -> dict:
     """Serving workload: device-resident index over 2,000,000 sets; requests of
     round(exp(U(0, ln 4096))) query sets at thresholds drawn from {0.6, 0.7,
-    0.8, 0.9}; saturated throughput, then Poisson arrivals at `offered`
-    requests/s (default: half the saturated rate)."""
+    0.8, 0.9}.  Saturated throughput (every request queued at once), then
+    Poisson arrivals at each fixed offered rate of `offered` requests/s and
+    at half the saturated rate; then the saturated run again under a device
+    budget of 1.5x the largest request's bytes (requests wait for bytes),
+    with every request's pair count checked against the unconstrained run."""
     import random
     import threading
 
@@ -319,23 +322,17 @@ def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=2000, offered=
     res = {"serve_index_sets": idx.n, "serve_requests": len(reqs),
            "serve_queries": int(sum(q.n for q, _ in reqs))}
     res.update(serve_slots=slots, serve_max_batch_queries=max_batch_queries)
-    with H.Server(d, slots=slots, hbm_budget_bytes=2 << 30, max_batch_queries=max_batch_queries) as srv:
-        for qc, t in reqs[:20]:
-            srv.wait(srv.submit(qc, t))
-        # saturation: everything queued at once
+
+    def saturated(srv):
         a = time.perf_counter()
         ids = [srv.submit(qc, t) for qc, t in reqs]
-        pairs = sum(srv.wait(i)[0] for i in ids)
-        sat_s = time.perf_counter() - a
-        sat_rps = len(reqs) / sat_s
-        res.update(serve_saturated_req_per_s=round(sat_rps, 1), serve_saturated_ms=round(sat_s * 1e3, 1),
-                   serve_pairs=int(pairs), serve_saturated_pairs_per_s=round(pairs / sat_s, 1))
-        # offered load: Poisson arrivals at half the saturated request rate
-        lam = offered if offered else 0.5 * sat_rps
-        prng = random.Random(3)
+        counts = [srv.wait(i)[0] for i in ids]
+        return counts, time.perf_counter() - a
+
+    def offered_run(srv, lam, seed):
+        prng = random.Random(seed)
         gaps = [prng.expovariate(lam) for _ in reqs]
-        lat, done_pairs = [], [0]
-        ids = []
+        lat, done_pairs, ids = [], [0], []
         lock = threading.Lock()
 
         def waiter():
@@ -356,6 +353,7 @@ def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=2000, offered=
                 done_pairs[0] += c
                 k += 1
 
+        s0 = srv.stats()
         th = threading.Thread(target=waiter)
         th.start()
         a = time.perf_counter()
@@ -370,15 +368,51 @@ def bench_serving(H, slots=4, max_batch_queries=65536, n_requests=2000, offered=
                 ids.append(rid)
         th.join()
         el = time.perf_counter() - a
+        s1 = srv.stats()
+        la = np.array(lat)
+        return {"offered_req_per_s": round(lam, 1), "achieved_req_per_s": round(len(reqs) / el, 1),
+                "p50_ms": round(float(np.percentile(la, 50)), 3), "p99_ms": round(float(np.percentile(la, 99)), 3),
+                "pairs_per_s": round(done_pairs[0] / el, 1), "batches": s1["batches"] - s0["batches"],
+                "max_in_flight": s1["max_in_flight"]}
+
+    with H.Server(d, slots=slots, hbm_budget_bytes=2 << 30, max_batch_queries=max_batch_queries) as srv:
+        for qc, t in reqs[:20]:
+            srv.wait(srv.submit(qc, t))
+        counts, sat_s = saturated(srv)
+        pairs = sum(counts)
+        sat_rps = len(reqs) / sat_s
+        res.update(serve_saturated_req_per_s=round(sat_rps, 1), serve_saturated_ms=round(sat_s * 1e3, 1),
+                   serve_pairs=int(pairs), serve_saturated_pairs_per_s=round(pairs / sat_s, 1))
+        log(f"serving: saturated {sat_rps:.0f} req/s ({pairs / sat_s / 1e6:.1f}M pairs/s)")
+        loads = {}
+        for k, lam in enumerate(list(offered) + [0.5 * sat_rps]):
+            r = offered_run(srv, lam, 3 + k)
+            key = "half_saturated" if k == len(offered) else f"{int(lam / 1000)}k"
+            loads[key] = r
+            log(f"serving at {lam:.0f} req/s offered: achieved {r['achieved_req_per_s']:.0f}, p50 {r['p50_ms']} ms, "
+                f"p99 {r['p99_ms']} ms, {r['batches']} batches, max {r['max_in_flight']} in flight")
+        sig_bytes = srv.stats()["signature_bits"] // 8
         st = srv.stats()
-    lat_arr = np.array(lat)
-    res.update(serve_offered_req_per_s=round(lam, 1), serve_achieved_req_per_s=round(len(reqs) / el, 1),
-               serve_p50_ms=round(float(np.percentile(lat_arr, 50)), 3),
-               serve_p99_ms=round(float(np.percentile(lat_arr, 99)), 3),
-               serve_pairs_per_s=round(done_pairs[0] / el, 1), serve_batches=st["batches"],
-               serve_max_in_flight=st["max_in_flight"], serve_deferred=st["deferred"])
-    log(f"serving: saturated {sat_rps:.0f} req/s; at {lam:.0f} req/s offered: p50 {res['serve_p50_ms']} ms, "
-        f"p99 {res['serve_p99_ms']} ms, {st['batches']} batches, max {st['max_in_flight']} in flight")
+    res["serve_offered_loads"] = loads
+    # compatibility with earlier lines: the half-saturated run's figures
+    h = loads["half_saturated"]
+    res.update(serve_offered_req_per_s=h["offered_req_per_s"], serve_achieved_req_per_s=h["achieved_req_per_s"],
+               serve_p50_ms=h["p50_ms"], serve_p99_ms=h["p99_ms"], serve_pairs_per_s=h["pairs_per_s"],
+               serve_batches=st["batches"], serve_max_in_flight=st["max_in_flight"], serve_deferred=st["deferred"])
+    # the budget run: room for 1.5 of the largest request at a time
+    need = max(4 * int(qc.tokens.shape[0]) + (109 + sig_bytes) * qc.n + 4096 for qc, _ in reqs)
+    budget = need * 3 // 2
+    with H.Server(d, slots=slots, hbm_budget_bytes=budget, max_batch_queries=max_batch_queries) as srv:
+        bcounts, b_s = saturated(srv)
+        bst = srv.stats()
+    same = bcounts == counts
+    res["serve_budget"] = {"hbm_budget_bytes": int(budget), "largest_request_bytes": int(need),
+                           "req_per_s": round(len(reqs) / b_s, 1), "deferred": int(bst["deferred"]),
+                           "rejected": int(bst["rejected"]), "max_bytes_in_flight": int(bst["max_bytes_in_flight"]),
+                           "answers_unchanged": bool(same)}
+    log(f"serving under a {budget / 1e6:.1f} MB budget: {len(reqs) / b_s:.0f} req/s, {bst['deferred']} deferred, "
+        f"max {bst['max_bytes_in_flight'] / 1e6:.1f} MB in flight, answers unchanged: {same}")
+    assert same and bst["max_bytes_in_flight"] <= budget
     ctx.close()
     return res
 

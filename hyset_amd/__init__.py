@@ -65,6 +65,8 @@ TUNE_BALANCED_TILES = 8
 TUNE_STATIC_TILES = 16
 TUNE_STAGE_TIMES = 32
 TUNE_ITEM_MAJOR = 64
+TUNE_NARROW = 128
+TUNE_INDEX_MAXPREFIX = 256
 PLAN_FIXED, PLAN_UPLOAD, PLAN_QUEUE, PLAN_BALANCED, PLAN_TOKEN = 1, 2, 3, 4, 5
 
 

@@ -77,10 +77,12 @@ uint64_t batch_entry_table(const hs_coll* c, const std::vector<uint32_t>& prefix
   return acc;
 }
 
-std::string graph_key(const hs_ctx* ctx, const hs_coll* c, uint32_t workgroups, uint32_t sig_words) {
+std::string graph_key(const hs_ctx* ctx, const hs_coll* c, uint32_t workgroups, uint32_t sig_words,
+                      uint32_t flags) {
   return std::to_string(ctx->kind) + "/" + std::to_string(ctx->mode) + "/" + std::to_string(ctx->t) + "/" +
          std::to_string(ctx->tables.L) + "/" + std::to_string(c->plan_probes) + "x" + std::to_string(c->plan_tokens) +
-         "/" + std::to_string(workgroups) + "/" + std::to_string(sig_words);
+         "/" + std::to_string(workgroups) + "/" + std::to_string(sig_words) + "/" +
+         std::to_string(flags & (HS_TUNE_NARROW | HS_TUNE_INDEX_MAXPREFIX));
 }
 
 void drop_graph(hs_ctx* ctx) {
@@ -95,7 +97,7 @@ void drop_graph(hs_ctx* ctx) {
 // Enqueues the whole batched join (entry table upload, signatures, index
 // build, counter reset, persistent launch) on the context stream.
 void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entries, uint32_t workgroups,
-                   uint32_t sig_words) {
+                   uint32_t sig_words, uint32_t flags) {
   hipStream_t st = ctx->stream;
   // 64-bit signatures come out of the index's emit pass (it reads every set)
   if (sig_words == 1) {
@@ -107,7 +109,9 @@ void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entr
   const uint32_t ntab = (uint32_t)c->h_size_first.size();
   ctx->d_cum.ensure((size_t)ntab * 4);
   HS_CHECK_HIP(hipMemcpyAsync(ctx->d_cum.ptr, h_cum, (size_t)ntab * 4, hipMemcpyHostToDevice, st));
-  build_index_batch(st, c->dev(), c->join_first.as<uint32_t>(), c->n_joins, ctx->dev_bounds.midprefix,
+  const uint32_t* index_prefix =
+      (flags & HS_TUNE_INDEX_MAXPREFIX) ? ctx->dev_bounds.maxprefix : ctx->dev_bounds.midprefix;
+  build_index_batch(st, c->dev(), c->join_first.as<uint32_t>(), c->n_joins, index_prefix,
                     c->size_first.as<uint32_t>(), c->sf_base.as<uint32_t>(), ctx->d_cum.as<uint32_t>(), c->universe,
                     entries, ctx->ib, sig_words == 1 ? c->sigs.as<uint64_t>() : nullptr,
                     sig_words == 1 ? c->sigs.as<uint64_t>() : nullptr);
@@ -126,7 +130,7 @@ void enqueue_batch(hs_ctx* ctx, hs_coll* c, const uint32_t* h_cum, uint64_t entr
   a.out = dev_out(ctx);
   a.sig_words = sig_words;
   a.probe_sigs = a.indexed_sigs = sig_words ? c->sigs.as<uint64_t>() : nullptr;
-  launch_fgss_batch(st, a, (uint32_t)c->h_tile_lo.size(), workgroups);
+  launch_fgss_batch(st, a, (uint32_t)c->h_tile_lo.size(), workgroups, (flags & HS_TUNE_NARROW) != 0);
 }
 
 void check_batch(hs_ctx* ctx, hs_coll* c) {
@@ -239,22 +243,26 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
     HS_REQUIRE(result, HS_E_INVALID, "NULL result");
     hs_tune tune{};
     if (tune_in) tune = *tune_in;
-    HS_REQUIRE((tune.flags & ~(uint32_t)(HS_TUNE_GRAPH | HS_TUNE_NO_SIGNATURE)) == 0, HS_E_INVALID,
-               "unknown tune flags");
+    HS_REQUIRE((tune.flags & ~(uint32_t)(HS_TUNE_GRAPH | HS_TUNE_NO_SIGNATURE | HS_TUNE_NARROW |
+                                         HS_TUNE_INDEX_MAXPREFIX)) == 0,
+               HS_E_INVALID, "unknown tune flags");
+    const bool narrow = (tune.flags & HS_TUNE_NARROW) != 0;
+    const uint32_t max_probes = narrow ? TILE_PROBES / 2 : TILE_PROBES;
+    const uint32_t max_tokens = narrow ? TILE_TOKENS / 2 : TILE_TOKENS;
     ensure_bounds(ctx, batch->maxlen);
     const uint32_t sig_words =
         (tune.flags & HS_TUNE_NO_SIGNATURE) ? 0u : fgss_signature_words(batch, tune.signature_bits);
     // tile plan of the requested granularity
-    const uint32_t probes = tune.probes_per_tile ? std::min<uint32_t>(tune.probes_per_tile, TILE_PROBES) : TILE_PROBES;
-    const uint32_t tokens = tune.tile_tokens ? std::min<uint32_t>(tune.tile_tokens, TILE_TOKENS) : TILE_TOKENS;
+    const uint32_t probes = tune.probes_per_tile ? std::min<uint32_t>(tune.probes_per_tile, max_probes) : max_probes;
+    const uint32_t tokens = tune.tile_tokens ? std::min<uint32_t>(tune.tile_tokens, max_tokens) : max_tokens;
     if (probes != batch->plan_probes || tokens != batch->plan_tokens) plan_batch_tiles(batch, probes, tokens);
-    const uint32_t wpc = tune.workgroups_per_cu ? tune.workgroups_per_cu : 2;
+    const uint32_t wpc = tune.workgroups_per_cu ? tune.workgroups_per_cu : (narrow ? 4 : 2);
     const uint32_t workgroups = (uint32_t)ctx->num_cus * wpc;
     ctx->batch_queue.ensure(8 * 4);
     ctx->stats = hs_stats{};
     hipStream_t st = ctx->stream;
     const bool use_graph = (tune.flags & HS_TUNE_GRAPH) != 0;
-    const std::string key = graph_key(ctx, batch, workgroups, sig_words);
+    const std::string key = graph_key(ctx, batch, workgroups, sig_words, tune.flags);
     HS_CHECK_HIP(hipEventRecord(ctx->ev[0], st));
     unsigned long long cnt[CNT_WORDS];
     for (int attempt = 0; attempt < 2; ++attempt) {
@@ -266,15 +274,17 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
         const uint32_t ntab = (uint32_t)batch->h_size_first.size();
         PinnedBuf& cum = use_graph ? ctx->graph_cum : ctx->h_cum;
         cum.ensure((size_t)ntab * 4);
-        const uint64_t entries = batch_entry_table(batch, ctx->tables.midprefix, cum.as<uint32_t>());
+        const uint64_t entries = batch_entry_table(
+            batch, (tune.flags & HS_TUNE_INDEX_MAXPREFIX) ? ctx->tables.maxprefix : ctx->tables.midprefix,
+            cum.as<uint32_t>());
         if (use_graph) {
           // an eager run first allocates every work buffer; the capture then
           // records a sequence that allocates nothing and replays as is
           drop_graph(ctx);
-          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words);
+          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words, tune.flags);
           HS_CHECK_HIP(hipStreamSynchronize(st));
           HS_CHECK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words);
+          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words, tune.flags);
           HS_CHECK_HIP(hipStreamEndCapture(st, &ctx->graph));
           HS_CHECK_HIP(hipGraphInstantiate(&ctx->graph_exec, ctx->graph, nullptr, nullptr, 0));
           ctx->graph_coll = batch;
@@ -283,7 +293,7 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune_in, hs_result
           ctx->graph_epoch = buffer_epoch();
           HS_CHECK_HIP(hipGraphLaunch(ctx->graph_exec, st));
         } else {
-          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words);
+          enqueue_batch(ctx, batch, cum.as<uint32_t>(), entries, workgroups, sig_words, tune.flags);
         }
         ctx->stats.index_entries = entries;
       }
@@ -341,16 +351,26 @@ int hs_autotune_batch(hs_ctx* ctx, hs_coll* batch, int32_t reps, hs_tune* best, 
         bt = t;
       }
     };
-    // tile granularity and grid at the default signature width, then the width
-    for (uint32_t probes : {64u, 128u, 256u, 512u}) {
-      for (uint32_t wpc : {1u, 2u, 4u}) {
-        hs_tune t{};
-        t.probes_per_tile = probes;
-        t.tile_tokens = probes * 16;
-        t.workgroups_per_cu = wpc;
-        trial(t);
+    // threads per probe (workgroup width over probes per tile) and grid at
+    // the default signature width, then the prefix scheme, then the width
+    for (const bool narrow : {false, true}) {
+      const uint32_t lanes = narrow ? TILE_PROBES / 2 : TILE_PROBES;
+      for (uint32_t probes : {64u, 128u, 256u, 512u}) {
+        if (probes > lanes) continue;
+        for (uint32_t wpc : narrow ? std::initializer_list<uint32_t>{2u, 4u, 8u}
+                                   : std::initializer_list<uint32_t>{1u, 2u, 4u}) {
+          hs_tune t{};
+          t.flags = narrow ? HS_TUNE_NARROW : 0u;
+          t.probes_per_tile = probes;
+          t.tile_tokens = probes * 16;
+          t.workgroups_per_cu = wpc;
+          trial(t);
+        }
       }
     }
+    hs_tune scheme = bt;
+    scheme.flags |= HS_TUNE_INDEX_MAXPREFIX;
+    trial(scheme);
     const hs_tune base = bt;
     for (uint32_t bits : {64u, 128u, 256u}) {
       hs_tune t = base;

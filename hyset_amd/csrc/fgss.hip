@@ -935,11 +935,12 @@ void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t ppt, uint32_t persi
   }
 }
 
-void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups) {
+void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups, bool narrow) {
   if (ntiles == 0) return;
   check_sigs(a);
   launch_fill_u32(st, a.batch.queue, 8, 0);
-  launch_width<FG_TPB, FG_BATCH, false, false>(st, a, dim3(workgroups), 0u, ntiles);
+  if (narrow) launch_width<FG_TPB_NARROW, FG_BATCH, false, false>(st, a, dim3(workgroups), 0u, ntiles);
+  else launch_width<FG_TPB, FG_BATCH, false, false>(st, a, dim3(workgroups), 0u, ntiles);
 }
 
 uint32_t auto_signature_words(double mean) {

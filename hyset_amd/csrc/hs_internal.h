@@ -326,7 +326,10 @@ constexpr uint32_t TILE_PROBES = 512;
 // t0 / t1 (may be NULL): events given the k_fgss dispatch's start and end.
 void launch_fgss(hipStream_t st, const FgssArgs& a, uint32_t probes_per_tile, uint32_t persistent_wgs = 0,
                  hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr);
-void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups);
+// The batched join's persistent launch: `workgroups` workgroups of 512
+// lanes (narrow: 256, for tiles of at most 256 probes) drain the batch's
+// tile queue.
+void launch_fgss_batch(hipStream_t st, const FgssArgs& a, uint32_t ntiles, uint32_t workgroups, bool narrow);
 // Work buffers and the host-made item table of the item-slice pass.
 struct BalancedPlan {
   // device table of 5 * nl entries, per size class l of the probe range:

@@ -236,6 +236,15 @@ enum { HS_TUNE_NO_SIGNATURE = 4 };
  * (total_ms, and transfer_ms when pairs are copied) and the stage times
  * are 0.  The hybrid technique always records its stages. */
 enum { HS_TUNE_STAGE_TIMES = 32 };
+/* hs_join_batch knobs of the autotuner (hs_autotune_batch).
+ * HS_TUNE_NARROW: 256-lane workgroups over tiles of at most 256 probes and
+ * 4096 tokens (default: 512 lanes, 512 probes, 8192 tokens); with
+ * probes_per_tile it sets the threads per probe (lanes / probes per tile).
+ * HS_TUNE_INDEX_MAXPREFIX: the prefix scheme -- the self-joins' index holds
+ * every set's probe prefix (maxprefix) instead of the index prefix
+ * (midprefix) the reference fixes (similarity.hpp:13-19).  Exact either
+ * way: a pair's first common token lies in both prefixes. */
+enum { HS_TUNE_NARROW = 128, HS_TUNE_INDEX_MAXPREFIX = 256 };
 
 /* Result sink.  count is always written.  With count_only == 0 the pairs
  * (r = probe id, s = indexed id, overlap = |r n s|) are copied into the
@@ -345,13 +354,15 @@ int hs_upload_batch(hs_ctx* ctx, uint32_t n_joins, const uint32_t* const* tokens
  * index build over all of them and one persistent prefix-filter launch that
  * drains a device work queue of probe tiles.  tune: probes_per_tile,
  * tile_tokens, workgroups_per_cu, signature_bits, flags (HS_TUNE_GRAPH,
- * HS_TUNE_NO_SIGNATURE). */
+ * HS_TUNE_NO_SIGNATURE, HS_TUNE_NARROW, HS_TUNE_INDEX_MAXPREFIX). */
 int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune, hs_result* result);
 
-/* Times hs_join_batch over a grid of tile sizes and persistent-grid sizes,
- * then over signature widths (64, 128, 256 bits, none) at the best grid
- * point (reps timed runs each after one warm-up) and returns the fastest
- * setting; *best_ms receives its mean device time. */
+/* Times hs_join_batch over a grid of threads per probe (workgroup width
+ * 512 or 256 lanes over 64-512 probes per tile) and persistent-grid sizes,
+ * then over the prefix scheme (midprefix or maxprefix index) and signature
+ * widths (64, 128, 256 bits, none) at the best point so far (reps timed runs
+ * each after one warm-up) and returns the fastest setting; *best_ms receives
+ * its mean device time. */
 int hs_autotune_batch(hs_ctx* ctx, hs_coll* batch, int32_t reps, hs_tune* best, double* best_ms);
 
 /* -------------------------------------------------------------- serving */

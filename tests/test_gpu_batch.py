@@ -67,6 +67,27 @@ def test_batch_tile_sizes_and_grid(ctx, batch):
         assert split(b, *ctx.join_batch(b, tune=tune)) == want, (tune.signature_bits, tune.flags)
 
 
+@pytest.mark.parametrize("t", [0.6, 0.75, 0.9])
+def test_batch_threads_per_probe_and_prefix_scheme(ctx, batch, t):
+    """The autotuner's knobs give the oracle's pairs: 256-lane workgroups
+    (HS_TUNE_NARROW) at 1, 2 and 4 threads per probe, and the index over
+    every set's probe prefix (HS_TUNE_INDEX_MAXPREFIX), also replayed from a
+    graph; the maxprefix index holds more entries."""
+    colls, b = batch
+    ctx.set_threshold(t)
+    want = expected(colls, t)
+    for probes, wpc in ((256, 4), (128, 8), (64, 2)):
+        tune = H.Tune(flags=H.TUNE_NARROW, probes_per_tile=probes, tile_tokens=probes * 16, workgroups_per_cu=wpc)
+        assert split(b, *ctx.join_batch(b, tune=tune)) == want, ("narrow", probes, wpc)
+    mid_entries = (ctx.join_batch(b, count_only=True), ctx.stats()["index_entries"])[1]
+    for flags in (H.TUNE_INDEX_MAXPREFIX, H.TUNE_INDEX_MAXPREFIX | H.TUNE_NARROW,
+                  H.TUNE_INDEX_MAXPREFIX | H.TUNE_GRAPH):
+        for rep in range(2 if flags & H.TUNE_GRAPH else 1):
+            assert split(b, *ctx.join_batch(b, tune=H.Tune(flags=flags))) == want, (flags, rep)
+    ctx.join_batch(b, tune=H.Tune(flags=H.TUNE_INDEX_MAXPREFIX), count_only=True)
+    assert ctx.stats()["index_entries"] > mid_entries
+
+
 def test_batch_graph_replays(ctx, batch):
     """Replays of the captured graph, with eager joins in between, all equal
     the eager result."""
@@ -102,7 +123,8 @@ def test_autotune_batch(ctx, batch):
     colls, b = batch
     ctx.set_threshold(0.75)
     tune, ms = ctx.autotune_batch(b, reps=1)
-    assert tune.probes_per_tile in (64, 128, 256, 512) and tune.workgroups_per_cu in (1, 2, 4) and ms > 0
+    assert tune.probes_per_tile in (64, 128, 256, 512) and tune.workgroups_per_cu in (1, 2, 4, 8) and ms > 0
+    assert tune.probes_per_tile <= (256 if tune.flags & H.TUNE_NARROW else 512)
     assert tune.signature_bits in (0, 64, 128, 256)
     assert split(b, *ctx.join_batch(b, tune=tune)) == expected(colls, 0.75)
     # the tuned configuration as a graph, after the trials moved the buffers
