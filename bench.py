@@ -271,10 +271,16 @@ def bench_split(ctx, H, units, want_counts) -> dict:
                                                            cal["gpu_alone"])}
     summary = S.summarize(plans, T, threads, cpu_scale, cal["gpu_loaded"], cal["gpu_alone"])
     for name, plan in plans.items():
-        got, _, wall = ctx.split_run(units, plan, cpu_threads=threads)
+        got, unit_ms, wall = ctx.split_run(units, plan, cpu_threads=threads)
         assert np.array_equal(got, want_counts), name
         summary[name]["run_wall_ms"] = round(wall, 2)
-        log(f"split {name}: run {wall:.1f} ms, modelled sum {summary[name]['sum_ms']} ms "
+        on_cpu = plan == H.TECH_CPU
+        # the two sides of the run: the device queue's time, and the host
+        # threads' busy time spread over the threads
+        summary[name]["run_device_ms"] = round(float(unit_ms[~on_cpu].sum()), 2)
+        summary[name]["run_cpu_ms_per_thread"] = round(float(unit_ms[on_cpu].sum()) / threads, 2)
+        log(f"split {name}: run {wall:.1f} ms (device side {summary[name]['run_device_ms']} ms, CPU side "
+            f"{summary[name]['run_cpu_ms_per_thread']} ms per thread), modelled sum {summary[name]['sum_ms']} ms "
             f"(regret {summary[name]['regret_ms']}), makespan {summary[name]['makespan_ms']} ms, "
             f"share {summary[name]['share']}")
     judge = sp.judge(F, T)

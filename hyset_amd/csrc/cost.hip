@@ -212,6 +212,10 @@ int hs_split_run(hs_ctx* ctx, uint32_t n_units, hs_coll* const* colls, const int
     const BoundTables T = ctx->tables;
     std::vector<uint32_t> cpu_units, gpu_units;
     for (uint32_t u = 0; u < n_units; ++u) (technique[u] == HS_TECH_CPU ? cpu_units : gpu_units).push_back(u);
+    // host threads claim the largest units first (by token count), the
+    // longest-first order of the planner's list schedule
+    std::stable_sort(cpu_units.begin(), cpu_units.end(),
+                     [&](uint32_t a, uint32_t b) { return colls[a]->n_tokens > colls[b]->n_tokens; });
     auto t0 = std::chrono::steady_clock::now();
     std::atomic<uint32_t> next{0};
     std::vector<std::thread> pool;

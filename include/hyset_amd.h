@@ -532,8 +532,8 @@ int hs_mlp_train(hs_mlp* mlp, const float* X, const float* Y, uint32_t n, uint32
 
 /* Runs a split: every unit (a collection self-join under the context's
  * threshold) with its technique; HS_TECH_CPU units run on cpu_threads host
- * threads while the device units run one after another on the context's
- * stream.  counts[u] = pairs of unit u; unit_ms[u] (may be NULL) its time;
+ * threads (which claim the largest units first, by token count) while the
+ * device units run one after another on the context's stream.  counts[u] = pairs of unit u; unit_ms[u] (may be NULL) its time;
  * *wall_ms (may be NULL) the time of the whole split. */
 int hs_split_run(hs_ctx* ctx, uint32_t n_units, hs_coll* const* colls, const int32_t* technique,
                  uint32_t cpu_threads, uint64_t* counts, double* unit_ms, double* wall_ms);
