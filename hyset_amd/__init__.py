@@ -143,7 +143,7 @@ _sig("hs_copy_last_result", C.c_int, _vp, C.POINTER(Result))
 class Coprocess(C.Structure):
     _fields_ = [("mode", C.c_int32), ("algo", C.c_int32), ("cpu_threads", C.c_uint32), ("cpu_block", C.c_uint32),
                 ("device_min_claim", C.c_uint32), ("cpu_share", C.c_double), ("tune", Tune),
-                ("min_device_ms", C.c_double)]
+                ("min_device_ms", C.c_double), ("policy", C.c_int32), ("reserved", C.c_uint32)]
 
 
 class CoprocessStats(C.Structure):
@@ -157,6 +157,8 @@ class CoprocessStats(C.Structure):
 
 COPROC_QUEUE, COPROC_STATIC = 0, 1
 _COPROC_MODES = {"queue": COPROC_QUEUE, "static": COPROC_STATIC}
+POLICY_CPU_GPU, POLICY_GPU_CPU = 0, 1
+_POLICIES = {"cpu,gpu": POLICY_CPU_GPU, "gpu,cpu": POLICY_GPU_CPU}
 _sig("hs_join_coprocess", C.c_int, _vp, _vp, _vp, C.POINTER(Coprocess), C.POINTER(Result),
      C.POINTER(CoprocessStats))
 PairSink = C.CFUNCTYPE(C.c_int, C.c_void_p, _u32p, _u32p, _u32p, C.c_uint64)
@@ -196,6 +198,60 @@ _sig("hs_mlp_loss_grad", C.c_int, _vp, _f32p, _f32p, C.c_uint32, _f32p, _f32p)
 _sig("hs_mlp_train", C.c_int, _vp, _f32p, _f32p, C.c_uint32, C.c_uint32, C.c_float, _f32p)
 _sig("hs_split_run", C.c_int, _vp, C.c_uint32, C.POINTER(_vp), C.POINTER(C.c_int32), C.c_uint32,
      C.POINTER(C.c_uint64), _f64p, _f64p)
+
+class CostCoefficients(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("intercept", "set_size_std", "average_set_size", "token_freq", "size",
+                                          "size_square", "universe_size", "index_size", "bitmap")] + \
+               [("enabled", C.c_int32), ("reserved", C.c_uint32)]
+
+
+class PartitionPair(C.Structure):
+    _fields_ = [("left", Range), ("right", Range), ("technique", C.c_int32), ("reserved", C.c_uint32),
+                ("pairs", C.c_uint64), ("ms", C.c_double)]
+
+
+COST_TERMS = ("intercept", "set_size_std", "average_set_size", "token_freq", "size", "size_square",
+              "universe_size", "index_size", "bitmap")
+COST_WRAP_SQUARE = 1
+_sig("hs_features_pair", C.c_int, _vp, _vp, C.POINTER(Range), C.POINTER(Range), _f32p)
+_sig("hs_partition_even", C.c_int, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Range), C.POINTER(C.c_uint32))
+_sig("hs_partition_dichotomize", C.c_int, C.c_uint32, C.c_uint32, C.c_double, C.POINTER(Range))
+_sig("hs_cost_run", C.c_int, _vp, _vp, C.POINTER(Range), C.c_uint32, C.POINTER(CostCoefficients), C.c_uint32,
+     C.c_uint32, C.POINTER(C.c_int32), _f32p, _f64p)
+_sig("hs_run_partition_pairs", C.c_int, _vp, _vp, C.POINTER(PartitionPair), C.c_uint32, C.c_uint32,
+     C.POINTER(Tune), C.POINTER(Result), _f64p)
+
+
+def partition_even(n, block, parts):
+    """partitioner::even (hs_partition_even): set-id ranges [(first, last)]."""
+    out = (Range * max(1, int(parts)))()
+    k = C.c_uint32()
+    rc = _lib.hs_partition_even(int(n), int(block), int(parts), out, C.byref(k))
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_partition_even: invalid arguments")
+    return [(out[i].first, out[i].last) for i in range(k.value)]
+
+
+def partition_dichotomize(n, block, split_pct):
+    """partitioner::dichotomize (hs_partition_dichotomize): (whole, Rs, Rl), None for an empty part."""
+    out = (Range * 3)()
+    rc = _lib.hs_partition_dichotomize(int(n), int(block), float(split_pct), out)
+    if rc != HS_OK:
+        raise HysetError(rc, "hs_partition_dichotomize: invalid arguments")
+    return tuple((r.first, r.last) if r.first <= r.last else None for r in out)
+
+
+def cost_coefficients(coefs, enabled=(1, 1, 1, 1)):
+    """A (4, 9) array (rows HS_TECH_*, columns COST_TERMS) as the C array of 4 hs_cost_coefficients."""
+    coefs = np.asarray(coefs, dtype=np.float64)
+    assert coefs.shape == (4, len(COST_TERMS))
+    arr = (CostCoefficients * 4)()
+    for k in range(4):
+        for j, name in enumerate(COST_TERMS):
+            setattr(arr[k], name, float(coefs[k, j]))
+        arr[k].enabled = int(bool(enabled[k]))
+    return arr
+
 
 NFEATURES = 7
 FEATURES = ("size", "averageSetSize", "setSizeStd", "tokenFreqDistribution", "universeSize", "indexSize",
@@ -466,16 +522,19 @@ class Context:
 
     def join_coprocess(self, indexed: "DeviceCollection", probe: "DeviceCollection" = None, *, mode="queue",
                        algo="fgss", cpu_threads=0, cpu_block=0, device_min_claim=0, cpu_share=0.0,
-                       tune: Tune | None = None, count_only=False, min_device_ms=0.0):
+                       tune: Tune | None = None, count_only=False, min_device_ms=0.0, policy="cpu,gpu"):
         """One join shared by the device and host threads (hs_join_coprocess).
         min_device_ms: the queue mode's break-even gate (0: the library's
-        default, < 0: always share).  Returns ((r, s, overlap) or the pair
-        count, stats dict)."""
+        default, < 0: always share).  policy (static mode): "cpu,gpu" or
+        "gpu,cpu", the sides of the small part Rs (cpu_share of the probe
+        sets) and of the large part, as the reference's --policy pair; algo
+        "hybrid" makes the device side the cpu-gpu technique.  Returns
+        ((r, s, overlap) or the pair count, stats dict)."""
         probe = indexed if probe is None else probe
         opt = Coprocess(_COPROC_MODES[mode] if isinstance(mode, str) else int(mode),
                         ALGOS[algo] if isinstance(algo, str) else int(algo), int(cpu_threads), int(cpu_block),
                         int(device_min_claim), float(cpu_share), tune if tune is not None else Tune(),
-                        float(min_device_ms))
+                        float(min_device_ms), _POLICIES[policy] if isinstance(policy, str) else int(policy), 0)
         res = Result()
         res.count_only = 1
         st = CoprocessStats()
@@ -586,6 +645,55 @@ class Context:
         self._check(_lib.hs_features(self._h, coll._h, C.byref(r) if r else None,
                                      out.ctypes.data_as(_f32p)), "hs_features")
         return out
+
+    def features_pair(self, coll: "DeviceCollection", left, right) -> np.ndarray:
+        """statistics::extract(left, right) of two set ranges (hs_features_pair)."""
+        out = np.zeros(NFEATURES, dtype=np.float32)
+        self._check(_lib.hs_features_pair(self._h, coll._h, C.byref(Range(*left)), C.byref(Range(*right)),
+                                          out.ctypes.data_as(_f32p)), "hs_features_pair")
+        return out
+
+    def cost_run(self, coll: "DeviceCollection", parts, coefs, enabled=(1, 1, 1, 1), bitmap_bits=64,
+                 wrap_square=False):
+        """cost::run over the partition pairs (i, j), i <= j (hs_cost_run).
+        coefs: (4, 9) per HS_TECH_* in COST_TERMS order.  Returns (pairs
+        [(i, j)], techniques, features (P, 7), costs (P, 4))."""
+        n = len(parts)
+        rng = (Range * n)(*[Range(*p) for p in parts])
+        npairs = n * (n + 1) // 2
+        tech = np.zeros(npairs, dtype=np.int32)
+        feats = np.zeros((npairs, NFEATURES), dtype=np.float32)
+        costs = np.zeros((npairs, 4), dtype=np.float64)
+        self._check(_lib.hs_cost_run(self._h, coll._h, rng, n, cost_coefficients(coefs, enabled), int(bitmap_bits),
+                                     COST_WRAP_SQUARE if wrap_square else 0,
+                                     tech.ctypes.data_as(C.POINTER(C.c_int32)), feats.ctypes.data_as(_f32p),
+                                     costs.ctypes.data_as(_f64p)), "hs_cost_run")
+        ij = [(i, j) for i in range(n) for j in range(i, n)]
+        return ij, tech, feats, costs
+
+    def run_partition_pairs(self, coll: "DeviceCollection", plan, *, cpu_threads=1, tune: Tune | None = None,
+                            count_only=False):
+        """Runs partition pairs [(left, right, technique)] of coll's self-join
+        (hs_run_partition_pairs).  Returns (triples or the pair count,
+        per-pair pairs, per-pair ms, wall ms)."""
+        n = len(plan)
+        pp = (PartitionPair * max(n, 1))()
+        for k, (left, right, tech) in enumerate(plan):
+            pp[k].left = Range(*left)
+            pp[k].right = Range(*right)
+            pp[k].technique = int(tech)
+        wall = C.c_double()
+        res = Result()
+        res.count_only = int(bool(count_only))
+        rc = _lib.hs_run_partition_pairs(self._h, coll._h, pp, n, int(cpu_threads), C.byref(tune) if tune else None,
+                                         C.byref(res), C.byref(wall))
+        if rc != HS_E_OVERFLOW:
+            self._check(rc, "hs_run_partition_pairs")
+        counts = np.array([pp[k].pairs for k in range(n)], dtype=np.uint64)
+        ms = np.array([pp[k].ms for k in range(n)], dtype=np.float64)
+        if count_only:
+            return int(res.count), counts, ms, wall.value
+        return self.last_result(int(res.count)), counts, ms, wall.value
 
     def split_run(self, colls, techniques, cpu_threads=1):
         """Runs every collection's self-join with its technique (hs_split_run);

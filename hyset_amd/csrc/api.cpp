@@ -152,6 +152,11 @@ struct HybridShared {
   hs_range ir, pr;
   int scenario;
   uint64_t batch;
+  // the host copy of the index: offsets over the universe, id-ascending
+  // lists, and (PPJoin filtering, else NULL) the entries' metas
+  const uint32_t* off = nullptr;
+  const uint32_t* ids = nullptr;
+  const uint32_t* metas = nullptr;
   std::atomic<uint64_t> candidates{0}, visits{0};
 };
 
@@ -207,7 +212,7 @@ void host_filter(HybridShared& H, uint32_t first, uint32_t last, uint32_t& next_
   cands.reserve(std::min<uint64_t>(H.batch, 1ull << 26));
   uint64_t n_cand = 0, n_visit = 0;
   const uint32_t last_len = (uint32_t)S.h_size_first.size() - 1;
-  const uint32_t* metas = ctx->host_algo == HS_HOST_PPJOIN ? ctx->h_metas.data() : nullptr;
+  const uint32_t* metas = H.metas;
   for (uint32_t p = first; p <= last && first <= last; ++p) {
     const uint32_t l = P.h_sizes[p];
     const uint32_t* rt = P.h_tokens.data() + P.h_starts[p];
@@ -222,13 +227,13 @@ void host_filter(HybridShared& H, uint32_t first, uint32_t last, uint32_t& next_
     for (uint32_t j = 0; j < pre; ++j) {
       const uint32_t tok = rt[j];
       if (tok >= S.universe) continue;
-      const uint32_t* a = ctx->h_set_ids.data() + ctx->h_offsets[tok];
-      const uint32_t* e = ctx->h_set_ids.data() + ctx->h_offsets[tok + 1];
+      const uint32_t* a = H.ids + H.off[tok];
+      const uint32_t* e = H.ids + H.off[tok + 1];
       for (const uint32_t* it = std::lower_bound(a, e, lo); it != e && *it < hi; ++it) {
         ++n_visit;
         if (stamp[*it] == p) continue;
         if (metas) {
-          const uint32_t ip = metas[it - ctx->h_set_ids.data()] & 0xFFFFu;  // saturated: kept
+          const uint32_t ip = metas[it - H.ids] & 0xFFFFu;  // saturated: kept
           if (ip != 0xFFFFu) {
             const uint32_t ls = S.h_sizes[*it];
             if (T.minoverlap(l, ls) > std::min(l - j, ls - ip)) continue;
@@ -269,29 +274,11 @@ std::vector<uint32_t> probe_splits(const hs_range& pr, uint32_t parts, bool self
   return pts;
 }
 
-void run_hybrid(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const hs_range& ir, const hs_range& pr,
-                const hs_tune& tune) {
-  // host copy of the device-built index (the reference builds its CPU index
-  // on the device and copies it back, index.hpp:250-282)
-  const uint32_t U = indexed->universe;
-  ctx->h_offsets.resize((size_t)U + 1);
-  HS_CHECK_HIP(hipMemcpyAsync(ctx->h_offsets.data(), ctx->ib.offsets.ptr, ((size_t)U + 1) * 4, hipMemcpyDeviceToHost,
-                              ctx->stream));
-  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  const uint32_t entries = ctx->h_offsets[U];
-  ctx->h_set_ids.resize(std::max<uint32_t>(entries, 1));
-  if (entries)
-    HS_CHECK_HIP(hipMemcpyAsync(ctx->h_set_ids.data(), ctx->ib.set_ids.ptr, (size_t)entries * 4,
-                                hipMemcpyDeviceToHost, ctx->stream));
-  if (ctx->host_algo == HS_HOST_PPJOIN) {  // positions for the positional filter
-    ctx->h_metas.resize(std::max<uint32_t>(entries, 1));
-    if (entries)
-      HS_CHECK_HIP(hipMemcpyAsync(ctx->h_metas.data(), ctx->ib.metas.ptr, (size_t)entries * 4, hipMemcpyDeviceToHost,
-                                  ctx->stream));
-  }
-  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-  ctx->stats.index_entries = entries;
+}  // namespace
 
+void hs::hybrid_filter_verify(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const hs_range& ir,
+                              const hs_range& pr, const hs_tune& tune, const uint32_t* off, const uint32_t* ids,
+                              const uint32_t* metas, HostCounts* counts) {
   HybridShared H;
   H.ctx = ctx;
   H.indexed = indexed;
@@ -299,14 +286,15 @@ void run_hybrid(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const 
   H.self = self;
   H.ir = ir;
   H.pr = pr;
+  H.off = off;
+  H.ids = ids;
+  H.metas = metas;
   const double avg = probe->n ? (double)probe->n_tokens / probe->n : 0.0;
   H.scenario = tune.verify_scenario ? (int)tune.verify_scenario : (avg > 64.0 ? 2 : 1);
   H.batch = tune.batch_candidates ? tune.batch_candidates : (1ull << 22);
   for (auto& sl : ctx->slots)
     if (!sl.done) HS_CHECK_HIP(hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
-
   const uint32_t threads = std::max<uint32_t>(1, tune.host_threads);
-  auto t0 = std::chrono::steady_clock::now();
   if (!range_empty(pr)) {
     if (threads == 1) {
       uint32_t slot = 0;
@@ -335,11 +323,47 @@ void run_hybrid(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const 
       if (!errs.empty()) throw errs.front();
     }
   }
+  if (counts) {
+    counts->candidates += H.candidates.load();
+    counts->visits += H.visits.load();
+  }
+}
+
+namespace {
+
+void run_hybrid(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const hs_range& ir, const hs_range& pr,
+                const hs_tune& tune) {
+  // host copy of the device-built index (the reference builds its CPU index
+  // on the device and copies it back, index.hpp:250-282)
+  const uint32_t U = indexed->universe;
+  ctx->h_offsets.resize((size_t)U + 1);
+  HS_CHECK_HIP(hipMemcpyAsync(ctx->h_offsets.data(), ctx->ib.offsets.ptr, ((size_t)U + 1) * 4, hipMemcpyDeviceToHost,
+                              ctx->stream));
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  const uint32_t entries = ctx->h_offsets[U];
+  ctx->h_set_ids.resize(std::max<uint32_t>(entries, 1));
+  if (entries)
+    HS_CHECK_HIP(hipMemcpyAsync(ctx->h_set_ids.data(), ctx->ib.set_ids.ptr, (size_t)entries * 4,
+                                hipMemcpyDeviceToHost, ctx->stream));
+  const bool ppjoin = ctx->host_algo == HS_HOST_PPJOIN;
+  if (ppjoin) {  // positions for the positional filter
+    ctx->h_metas.resize(std::max<uint32_t>(entries, 1));
+    if (entries)
+      HS_CHECK_HIP(hipMemcpyAsync(ctx->h_metas.data(), ctx->ib.metas.ptr, (size_t)entries * 4, hipMemcpyDeviceToHost,
+                                  ctx->stream));
+  }
+  HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+  ctx->stats.index_entries = entries;
+
+  HostCounts hc;
+  auto t0 = std::chrono::steady_clock::now();
+  hs::hybrid_filter_verify(ctx, indexed, probe, self, ir, pr, tune, ctx->h_offsets.data(), ctx->h_set_ids.data(),
+                           ppjoin ? ctx->h_metas.data() : nullptr, &hc);
   HS_CHECK_HIP(hipStreamSynchronize(ctx->stream));
   auto t1 = std::chrono::steady_clock::now();
   ctx->stats.host_filter_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-  ctx->stats.candidates = H.candidates.load();
-  ctx->stats.posting_visits = H.visits.load();
+  ctx->stats.candidates = hc.candidates;
+  ctx->stats.posting_visits = hc.visits;
 }
 
 void check_pair(hs_ctx* ctx, hs_coll* a, hs_coll* b) {

@@ -1,9 +1,12 @@
 // One join shared by the device and host cores running concurrently.
 //
 // Reference: the work-sharing driver src/queue.cu:142-279 (a queue of block
-// pairs drained by a device thread and by the host thread) and the static
-// split of src/dichotomy.cu:143-285 (the host joins the small-set partition
-// Rs with itself, the device R against the rest Rl).  Here the unit of work
+// pairs drained by a device thread and by the host thread; the device
+// thread runs fgss, bitmap or, --cooperative, the hybrid technique) and the
+// static split of src/dichotomy.cu:143-285 (the small-set partition Rs
+// joined with itself by one side, R against the rest Rl by the other, the
+// sides named by the --policy pair, the device side fgss, bitmap or the
+// hybrid cpu-gpu technique).  Here the unit of work
 // is a stripe of consecutive probe ids joined against the whole indexed
 // range, so any set of disjoint stripes covering the probe ids gives exactly
 // the join.  Sets are sorted by size, so the work per probe grows with its
@@ -139,8 +142,12 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
                "collection lives on another device");
     const hs_coprocess opt = *options;
     HS_REQUIRE(opt.mode == HS_COPROC_QUEUE || opt.mode == HS_COPROC_STATIC, HS_E_INVALID, "unknown mode");
-    HS_REQUIRE(opt.algo == HS_ALGO_FGSS || opt.algo == HS_ALGO_BITMAP, HS_E_INVALID,
-               "the device technique must be fgss or bitmap");
+    HS_REQUIRE(opt.algo == HS_ALGO_FGSS || opt.algo == HS_ALGO_BITMAP || opt.algo == HS_ALGO_HYBRID, HS_E_INVALID,
+               "the device technique must be fgss, bitmap or hybrid");
+    HS_REQUIRE(opt.policy == HS_POLICY_CPU_GPU || opt.policy == HS_POLICY_GPU_CPU, HS_E_INVALID, "unknown policy");
+    HS_REQUIRE(opt.mode == HS_COPROC_STATIC || opt.policy == HS_POLICY_CPU_GPU, HS_E_INVALID,
+               "the queue mode's device claims from the large sets (policy cpu,gpu)");
+    const bool hybrid = opt.algo == HS_ALGO_HYBRID;
     HS_REQUIRE(opt.mode != HS_COPROC_STATIC || (opt.cpu_share >= 0.0 && opt.cpu_share <= 1.0), HS_E_INVALID,
                "cpu_share must lie in [0, 1]");
     const hs_tune tune = opt.tune;
@@ -231,10 +238,17 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     S.hi = pr.last + 1;
     S.host_limit = S.hi;
     const uint32_t np = pr.last - pr.first + 1;
+    // static: [pr.first, cut) is the small part Rs, [cut, n) the large part Rl
+    const uint32_t cut = opt.mode == HS_COPROC_STATIC
+                             ? pr.first + (uint32_t)std::llround(opt.cpu_share * (double)np) : pr.last + 1;
+    const bool dev_small = opt.mode == HS_COPROC_STATIC && opt.policy == HS_POLICY_GPU_CPU;
     if (opt.mode == HS_COPROC_STATIC) {
-      const uint32_t cut = pr.first + (uint32_t)std::llround(opt.cpu_share * (double)np);
-      S.host_limit = cut;
-      S.hi = cut;  // the device's share [cut, n) is claimed below, in one launch
+      if (dev_small) {
+        S.lo = cut;  // the host's share is Rl; the device's Rs is launched below
+      } else {
+        S.host_limit = cut;
+        S.hi = cut;  // the device's share Rl is launched below, in one claim
+      }
     }
     const uint32_t nthreads = opt.cpu_threads ? opt.cpu_threads : std::max(1u, usable_cpus() - 1);
     cs.host_threads = nthreads;
@@ -289,8 +303,11 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     // ---- device consumer (this thread): up to two stripes in flight.  Until
     // both sides have a measured rate the device claims half of what is
     // left; then it claims the top part of the unclaimed ids sized so that
-    // the host, joining the rest, ends when the device does.
-    uint32_t dev_first = pr.last + 1;  // the device's share is [dev_first, n)
+    // the host, joining the rest, ends when the device does.  With the
+    // hybrid technique a claim is filtered on this thread (and
+    // tune.host_threads - 1 more) before its candidates are verified.
+    uint32_t dev_lo = pr.last + 1, dev_hi = pr.first;  // the device's share is [dev_lo, dev_hi)
+    HostCounts dev_hc;
     uint32_t tile_plan = 0;
     std::string dev_err;
     int dev_rc = HS_OK;
@@ -307,11 +324,19 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
       double dev_busy_ms = 0.0;
       Clock::time_point prev_done = t_dev;
       auto launch = [&](uint32_t first, uint32_t n) {
-        tile_plan = launch_device_join(ctx, indexed, probe, P, hs_range{first, first + n - 1}, opt.algo, tune);
+        const auto t_launch = Clock::now();
+        const hs_range cr{first, first + n - 1};
+        if (hybrid) {
+          HS_CHECK_HIP(hipEventSynchronize(copied));
+          hybrid_filter_verify(ctx, indexed, probe, self, ir, cr, tune, h_off, h_ids, h_metas, &dev_hc);
+        } else {
+          tile_plan = launch_device_join(ctx, indexed, probe, P, cr, opt.algo, tune);
+        }
         HS_CHECK_HIP(hipEventRecord(ctx->claim_ev[claims & 1], st));
-        inflight.push_back(Claim{probe_weight(psf, first, first + n), Clock::now(), claims & 1u});
+        inflight.push_back(Claim{probe_weight(psf, first, first + n), t_launch, claims & 1u});
         ++claims;
-        dev_first = first;
+        dev_lo = std::min(dev_lo, first);
+        dev_hi = std::max(dev_hi, first + n);
         cs.device_probes += n;
       };
       // the next device claim [x, hi) under the lock, or false
@@ -343,7 +368,11 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
         return true;
       };
       if (opt.mode == HS_COPROC_STATIC) {
-        if (S.host_limit <= pr.last) launch(S.host_limit, pr.last + 1 - S.host_limit);
+        if (dev_small) {
+          if (cut > pr.first) launch(pr.first, cut - pr.first);
+        } else if (cut <= pr.last) {
+          launch(cut, pr.last + 1 - cut);
+        }
       }
       for (;;) {
         uint32_t first = 0, n = 0;
@@ -374,7 +403,7 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     for (auto& w : workers)
       if (w.failed) throw w.err;
     HS_REQUIRE(S.lo == S.hi, HS_E_DEVICE, "probe stripes left unclaimed");
-    cs.split = dev_first;
+    cs.split = dev_small ? cut : std::max(dev_lo, S.hi);
 
     // ---- the device's pairs; a second pass over its share if they overflowed
     unsigned long long cnt[CNT_WORDS];
@@ -382,8 +411,13 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     if (cnt[CNT_PAIRS] > ctx->out_cap) {
       ensure_output(ctx, cnt[CNT_PAIRS] + cnt[CNT_PAIRS] / 4 + 1024);
       reset_counters(ctx);
-      if (dev_first <= pr.last)
-        launch_device_join(ctx, indexed, probe, P, hs_range{dev_first, pr.last}, opt.algo, tune);
+      if (dev_lo < dev_hi) {
+        const hs_range dr{dev_lo, dev_hi - 1};
+        if (hybrid)
+          hybrid_filter_verify(ctx, indexed, probe, self, ir, dr, tune, h_off, h_ids, h_metas, nullptr);
+        else
+          launch_device_join(ctx, indexed, probe, P, dr, opt.algo, tune);
+      }
       unsigned long long again[CNT_WORDS];
       read_counters(ctx, again);
       HS_REQUIRE(again[CNT_PAIRS] == cnt[CNT_PAIRS], HS_E_DEVICE, "result count changed between runs");
@@ -422,8 +456,9 @@ int hs_join_coprocess(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_co
     ctx->stats.pairs = total;
     ctx->stats.index_ms = cs.index_ms;
     ctx->stats.total_ms = cs.wall_ms;
-    ctx->stats.posting_visits = cnt[CNT_VISITS];
-    ctx->stats.candidates = cnt[CNT_CANDS];
+    // the hybrid device side counts its visits and candidates on the host
+    ctx->stats.posting_visits = hybrid ? dev_hc.visits : cnt[CNT_VISITS];
+    ctx->stats.candidates = hybrid ? dev_hc.candidates : cnt[CNT_CANDS];
     ctx->stats.candidate_tokens = cnt[CNT_CAND_TOKENS];
     ctx->stats.probe_prefix_tokens = cnt[CNT_ITEMS];
     ctx->stats.index_entries = entries;

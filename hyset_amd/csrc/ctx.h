@@ -147,6 +147,7 @@ struct hs_ctx {
   int num_cus = 256;
   DevBuf batch_queue;  // per-XCD tile counters of the batched launch
   DevBuf feat;         // partition-feature accumulators (hs_features)
+  DevBuf feat_hist;    // per-token prefix counts of two partitions (hs_features_pair)
   DevBuf bal_cost, bal_tiles, bal_ntiles;  // work-balanced fgss tile plan
   DevBuf bal_slice_a, bal_slice_n, d_icum;  // item slices and the item table
   DevBuf bal_rec, bal_slot;                 // per-probe records of the plan
@@ -374,6 +375,18 @@ struct HostCounts {
 void host_probe_range(const hs_coll& P, const hs_coll& S, const uint32_t* offsets, const uint32_t* ids,
                       const uint32_t* metas, const BoundTables& T, bool self, const hs_range& ir, uint32_t first,
                       uint32_t last, Triples& out, std::vector<uint32_t>& stamp, HostCounts* counts = nullptr);
+
+// The hybrid technique over probe range pr against the index of ir
+// (hybrid::allpairs / hybrid::ppjoin feeding device_handler,
+// hybrid_algorithms.hpp:16-387): tune.host_threads host threads filter the
+// probes against the host copy of the index (off, ids; metas for PPJoin's
+// positional filter, else NULL) and queue candidate batches, double-buffered
+// in pinned slots, for verification on the context stream; pairs and
+// counters accumulate in the context's result buffers.  Returns when every
+// batch is queued (not when verified).  counts (may be NULL) are added to.
+void hybrid_filter_verify(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, bool self, const hs_range& ir,
+                          const hs_range& pr, const hs_tune& tune, const uint32_t* off, const uint32_t* ids,
+                          const uint32_t* metas, HostCounts* counts);
 
 // CPU join (host_join.cpp): pairs of probe range pr against indexed range ir
 // (PPJoin filtering when ppjoin); reads only host copies and the given

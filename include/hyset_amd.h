@@ -432,9 +432,19 @@ int hs_join_host(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* 
  *     thread to about 0.1 ms at its measured rate when cpu_block is 0, and
  *     never more than an equal share of what is left.  Rates are taken per
  *     unit of a work proxy: set size + 8 per probe.
- *   HS_COPROC_STATIC: the split of src/dichotomy.cu:143-285: the host joins
- *     the probe sets below cpu_share * n (Rs, the small sets), the device
- *     the rest (Rl) against everything.
+ *   HS_COPROC_STATIC: the split of src/dichotomy.cu:143-285 into the small
+ *     sets Rs (the probe sets below cpu_share * n; cpu_share is the
+ *     reference's --split / 100) and the rest Rl.  policy names the sides
+ *     as the reference's --policy pair (small part, large part):
+ *     HS_POLICY_CPU_GPU (cpu,gpu): the host joins Rs with itself, the
+ *     device R against Rl; HS_POLICY_GPU_CPU (gpu,cpu): the device joins
+ *     Rs, the host R against Rl.
+ * The device technique (algo) is fgss, bitmap or hybrid.  With hybrid the
+ * device side is the reference's cpu-gpu side (dichotomy.cu:204-258, and in
+ * queue mode the --cooperative consumer of queue.cu:189-241): its stripes
+ * are filtered on the host, by tune.host_threads threads (default 1) beside
+ * the cpu_threads that join, against the same host copy of the index, and
+ * their candidates verified on the device.
  * The result is the same set of pairs as hs_join: afterwards the context's
  * last result is the device's pairs followed by the host's (those stay in
  * host memory, no copy to the device), count-only or copied into result as
@@ -453,22 +463,26 @@ int hs_join_host(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* 
  * not faster.  Otherwise it co-processes and records the time.
  * min_device_ms < 0 always co-processes; static mode is never gated. */
 enum { HS_COPROC_QUEUE = 0, HS_COPROC_STATIC = 1 };
+enum { HS_POLICY_CPU_GPU = 0, HS_POLICY_GPU_CPU = 1 };
 typedef struct {
   int32_t mode;              /* HS_COPROC_* */
-  int32_t algo;              /* device technique: HS_ALGO_FGSS or HS_ALGO_BITMAP */
+  int32_t algo;              /* device technique: HS_ALGO_FGSS, HS_ALGO_BITMAP or HS_ALGO_HYBRID */
   uint32_t cpu_threads;      /* host threads (0: the usable CPUs less one, for the device thread) */
   uint32_t cpu_block;        /* queue: probe sets per host claim (0: about 0.1 ms of work) */
   uint32_t device_min_claim; /* queue: smallest device claim in probe sets (0: 16384) */
-  double cpu_share;          /* static: fraction of the probe sets the host joins */
+  double cpu_share;          /* static: fraction of the probe sets in the small part Rs */
   hs_tune tune;              /* device knobs */
   double min_device_ms;      /* queue: the break-even gate (0: 50 ms; < 0: always co-process) */
+  int32_t policy;            /* static: HS_POLICY_* (queue: HS_POLICY_CPU_GPU only) */
+  uint32_t reserved;
 } hs_coprocess;
 
 typedef struct {
   uint64_t device_pairs, host_pairs;
   uint32_t device_probes, host_probes; /* probe sets joined by each side */
   uint32_t device_claims, host_claims; /* stripes each side claimed */
-  uint32_t split;                      /* first probe id of the device's share */
+  uint32_t split;                      /* first probe id of the large part: the device's share (queue,
+                                          HS_POLICY_CPU_GPU) or the host's (HS_POLICY_GPU_CPU) */
   uint32_t host_threads;
   double index_ms;  /* device index build and signatures */
   double copy_ms;   /* index copy to the host, from the end of the build */
@@ -537,6 +551,96 @@ int hs_mlp_train(hs_mlp* mlp, const float* X, const float* Y, uint32_t n, uint32
  * *wall_ms (may be NULL) the time of the whole split. */
 int hs_split_run(hs_ctx* ctx, uint32_t n_units, hs_coll* const* colls, const int32_t* technique,
                  uint32_t cpu_threads, uint64_t* counts, double* unit_ms, double* wall_ms);
+
+/* ------------------------------------- partition pairs of one self-join */
+
+/* Partition features of two set-id ranges joined with each other
+ * (statistics::extract(left, right), include/hyset/statistics.hpp:216-283),
+ * laid out as hs_features' (the threshold in slot 6):
+ *   0 size = |left| + |right|;
+ *   1, 2 mean and standard deviation (population) of |left| + |right|
+ *     values: the sizes of left's sets but its last, a zero, the sizes of
+ *     right's sets but its last, a zero (the reference copies [startID,
+ *     endID) of each range into a zero-filled vector of |left| + |right|
+ *     slots, endID being inclusive);
+ *   3 standard deviation over the universe of the per-token counts of the
+ *     midprefix entries of both ranges' sets;
+ *   4 universe; 5 indexSize = left's midprefix entries.
+ * Computed on the device (size moments, a token histogram). */
+int hs_features_pair(hs_ctx* ctx, hs_coll* coll, const hs_range* left, const hs_range* right,
+                     float out[HS_NFEATURES]);
+
+/* Partitioners over blocks of block_size consecutive sets, the last block
+ * possibly short (blocks as include/hyset/collection.hpp:147-172 cuts them).
+ * hs_partition_even (partitioner::even, include/hyset/partitioner.hpp:11-30):
+ * partitions of ceil(blocks / parts) consecutive blocks; out (room for
+ * `parts` ranges) receives their set-id ranges, *n_out their number (which
+ * may be below parts).  hs_partition_dichotomize (partitioner::dichotomize,
+ * :32-68): out[0] the whole collection, out[1] the first
+ * ceil(split_pct / 100 * blocks) blocks (Rs), out[2] the rest (Rl); an empty
+ * range has first > last. */
+int hs_partition_even(uint32_t n_sets, uint32_t block_size, uint32_t parts, hs_range* out, uint32_t* n_out);
+int hs_partition_dichotomize(uint32_t n_sets, uint32_t block_size, double split_pct, hs_range out[3]);
+
+/* One technique's linear cost (statistics::coefficients,
+ * include/hyset/statistics.hpp:132-161; the per-technique, per-threshold
+ * table of src/cost_model.cu:296-323): cost = |intercept + set_size_std *
+ * setSizeStd + average_set_size * averageSetSize + token_freq *
+ * tokenFreqDistribution + size * size + size_square * size^2 +
+ * universe_size * universeSize + index_size * indexSize + bitmap * bitmap|,
+ * evaluated in long double.  enabled = 0 leaves the technique out of the
+ * choice (cost_model.cu:133-135 drops bitmap when --bitmap is 0). */
+typedef struct {
+  double intercept, set_size_std, average_set_size, token_freq, size, size_square, universe_size, index_size,
+      bitmap;
+  int32_t enabled;
+  uint32_t reserved;
+} hs_cost_coefficients;
+
+/* hs_cost_run flags: HS_COST_WRAP_SQUARE evaluates size^2 as the reference
+ * does, an unsigned 32-bit product that wraps above 65,535 sets
+ * (statistics.hpp:155); by default the square is exact. */
+#define HS_COST_WRAP_SQUARE 1u
+
+/* cost::run (include/hyset/cost.hpp:28-58) on the device collection under
+ * the context's threshold: for every partition pair (i, j), i <= j, in the
+ * order (0,0), (0,1), ..., (0,n-1), (1,1), ..., the features (hs_features
+ * of parts[i] when i == j, hs_features_pair(parts[i], parts[j]) otherwise),
+ * bitmap = bitmap_bits, and the enabled technique of least cost, ties to
+ * the first in the reference's map order (bitmap, cpu, fgssjoin, hybrid).
+ * coef[HS_TECH_*].  technique receives n_parts * (n_parts + 1) / 2 HS_TECH_*
+ * values; features (may be NULL) HS_NFEATURES floats per pair; costs (may
+ * be NULL) 4 doubles per pair, indexed by HS_TECH_* (disabled: -1). */
+int hs_cost_run(hs_ctx* ctx, hs_coll* coll, const hs_range* parts, uint32_t n_parts,
+                const hs_cost_coefficients coef[4], uint32_t bitmap_bits, uint32_t flags, int32_t* technique,
+                float* features, double* costs);
+
+/* One partition pair of a plan: the probe sets of `right` joined with the
+ * sets of `left` (pairs s < r, s in left, r in right) by `technique`. */
+typedef struct {
+  hs_range left;
+  hs_range right;
+  int32_t technique; /* HS_TECH_* */
+  uint32_t reserved;
+  uint64_t pairs;    /* out: pairs found */
+  double ms;         /* out: wall time of this pair's join */
+} hs_partition_pair;
+
+/* Runs a plan of partition pairs of one self-join, the execution half of
+ * src/cost_model.cu:150-262: the device pairs on this thread one after
+ * another (hybrid first, then bitmap, then fgss, as the reference orders
+ * them; tune as for hs_join: bitmap_bits, host_threads for the hybrid
+ * filter, ...), the HS_TECH_CPU pairs (host::allpairs, or host::ppjoin
+ * after hs_set_host_algorithm) on cpu_threads host threads meanwhile,
+ * largest probe side first.  The pairs (i, j), i <= j, of a partition of the
+ * collection's ids give exactly the join.  result->count = all pairs.
+ * Unless count_only, the pairs are gathered in host memory (each device
+ * pair's after its join, then the CPU pairs) and become the context's last
+ * result, copied into result when it has room (else HS_E_OVERFLOW, the
+ * result kept for hs_copy_last_result); count-only runs keep none.
+ * *wall_ms (may be NULL) = the time of the whole plan. */
+int hs_run_partition_pairs(hs_ctx* ctx, hs_coll* coll, hs_partition_pair* pairs, uint32_t n_pairs,
+                           uint32_t cpu_threads, const hs_tune* tune, hs_result* result, double* wall_ms);
 
 /* Copies the pairs of the last join into result (count_only ignored).  The
  * pairs travel in chunks through pinned host buffers, the next chunk's copy

@@ -153,12 +153,85 @@ def test_count_only_then_copy_and_dense_host_share(ctx):
         d.free()
 
 
+@pytest.mark.parametrize("policy", ["cpu,gpu", "gpu,cpu"])
+@pytest.mark.parametrize("algo", ["fgss", "bitmap", "hybrid"])
+def test_static_policies(ctx, col, policy, algo):
+    """The --policy pairs of dichotomy.cu:179-258 with every device technique
+    (hybrid: the cpu-gpu side, its stripes filtered on the host and verified
+    on the device): Rs = the probe sets below share * n goes to the first
+    side, Rl to the second; the pairs are the oracle's for every share."""
+    c, d = col
+    tune = H.Tune(host_threads=2) if algo == "hybrid" else None
+    for t, share in ((0.6, 0.3), (0.8, 0.0), (0.8, 1.0), (0.7, 0.75)):
+        ctx.set_threshold(t)
+        want = O.self_join(c, t)
+        (r, s, ov), info = ctx.join_coprocess(d, mode="static", policy=policy, algo=algo, cpu_share=share,
+                                              cpu_threads=3, tune=tune)
+        assert _triples(r, s, ov) == _want(want), (t, share)
+        cut = round(share * c.n)
+        assert info["split"] == cut
+        assert info["device_probes"] + info["host_probes"] == c.n
+        assert info["device_pairs"] + info["host_pairs"] == want.count
+        dev_part = cut if policy == "gpu,cpu" else c.n - cut
+        assert info["device_probes"] == dev_part
+        assert info["device_claims"] == (1 if dev_part else 0)
+        if policy == "gpu,cpu":
+            # the device's pairs have their larger id in Rs, the host's in Rl
+            g = info["device_pairs"]
+            assert np.all(r[:g] < cut) and np.all(r[g:] >= cut)
+
+
+@pytest.mark.parametrize("host_algo", ["allpairs", "ppjoin"])
+@pytest.mark.parametrize("filter_threads", [1, 3])
+def test_queue_cooperative_hybrid(ctx, col, host_algo, filter_threads):
+    """queue.cu's --cooperative device consumer: the device side's claims
+    are filtered on the host (AllPairs or PPJoin, 1 or 3 filter threads) and
+    verified on the device while the host threads join from the bottom."""
+    c, d = col
+    try:
+        ctx.set_host_algorithm(host_algo)
+        for t in (0.6, 0.8):
+            ctx.set_threshold(t)
+            want = O.self_join(c, t)
+            (r, s, ov), info = ctx.join_coprocess(d, algo="hybrid", cpu_threads=2, cpu_block=32,
+                                                  device_min_claim=500, min_device_ms=-1,
+                                                  tune=H.Tune(host_threads=filter_threads))
+            assert _triples(r, s, ov) == _want(want), (t, host_algo)
+            _check_stats(info, c.n, want.count)
+            assert info["device_claims"] >= 1
+    finally:
+        ctx.set_host_algorithm("allpairs")
+
+
+def test_rs_join_policies_hybrid(ctx):
+    R, S = H.generate_rs(H.gen_params(3000, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=3),
+                         H.gen_params(2500, size_dist=H.SIZE_UNIFORM, lo=5, hi=60, universe=2000, zipf=0.8, seed=4),
+                         dup_frac=0.3)
+    dR, dS = ctx.upload(R), ctx.upload(S)
+    try:
+        ctx.set_threshold(0.7, "cosine")
+        want = _want(O.rs_join(R, S, 0.7, "cosine"))
+        for kw in (dict(mode="static", policy="gpu,cpu", algo="hybrid", cpu_share=0.4),
+                   dict(mode="static", policy="gpu,cpu", algo="fgss", cpu_share=0.4),
+                   dict(mode="queue", algo="hybrid", cpu_block=16, device_min_claim=300, min_device_ms=-1)):
+            (r, s, ov), info = ctx.join_coprocess(dS, dR, cpu_threads=2, **kw)
+            assert _triples(r, s, ov) == want, kw
+            assert info["device_probes"] + info["host_probes"] == R.n
+    finally:
+        dR.free()
+        dS.free()
+
+
 def test_bad_options(ctx, col):
     _, d = col
     ctx.set_threshold(0.8)
     with pytest.raises(H.HysetError):
-        ctx.join_coprocess(d, algo="hybrid")
+        ctx.join_coprocess(d, algo=7)
     with pytest.raises(H.HysetError):
         ctx.join_coprocess(d, mode="static", cpu_share=1.5)
     with pytest.raises(H.HysetError):
         ctx.join_coprocess(d, mode=7)
+    with pytest.raises(H.HysetError):
+        ctx.join_coprocess(d, mode="static", policy=5)
+    with pytest.raises(H.HysetError):  # the queue's device claims from the large sets
+        ctx.join_coprocess(d, mode="queue", policy="gpu,cpu")
