@@ -21,7 +21,7 @@ DRIVER = HERE / "hyset_join"
 
 HIP_SOURCES = ["index.hip", "fgss.hip", "bitmap.hip", "verify.hip", "mlp.hip", "cost.hip", "noalloc.hip", "tokindex.hip"]
 CPP_SOURCES = ["api.cpp", "batch.cpp", "serve.cpp", "host_join.cpp", "bounds.cpp", "datagen.cpp", "host_tools.cpp",
-               "preprocess.cpp", "coprocess.cpp"]
+               "preprocess.cpp", "coprocess.cpp", "partition.cpp"]
 ARCH = "gfx950"
 
 

@@ -78,6 +78,59 @@ __global__ void k_zero_f64(double* p, uint32_t n) {
   if (threadIdx.x < n) p[threadIdx.x] = 0.0;
 }
 
+// hist[t] += 1 for every midprefix token t of the sets [a0, a0 + na) and
+// [b0, b0 + nb), one thread per set (the histogram kernel of
+// include/hyset/index.hpp:217-231 over the prefixes of two partitions)
+__global__ void k_prefix_hist(DevColl c, const uint32_t* __restrict__ midprefix, uint32_t L, uint32_t a0,
+                              uint32_t na, uint32_t b0, uint32_t nb, uint32_t universe, uint32_t* hist) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < na + nb; k += gridDim.x * blockDim.x) {
+    const uint32_t i = k < na ? a0 + k : b0 + (k - na);
+    const uint32_t l = c.sizes[i];
+    const uint32_t p = midprefix[l < L ? l : L];
+    const uint32_t* t = c.tokens + c.starts[i];
+    for (uint32_t j = 0; j < p; ++j) {
+      const uint32_t tok = t[j];
+      if (tok < universe) atomicAdd(&hist[tok], 1u);
+    }
+  }
+}
+
+// acc[2] += sum c, acc[3] += sum c^2 over the counts hist[0 .. universe)
+__global__ void k_count_moments(const uint32_t* __restrict__ hist, uint32_t universe, double* acc) {
+  __shared__ double s0[256], s1[256];
+  double a = 0, b = 0;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < universe; t += gridDim.x * blockDim.x) {
+    const double c = hist[t];
+    a += c;
+    b += c * c;
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (uint32_t s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s0[threadIdx.x] += s0[threadIdx.x + s];
+      s1[threadIdx.x] += s1[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&acc[2], s0[0]);
+    atomicAdd(&acc[3], s1[0]);
+  }
+}
+
+// Midprefix entries of the sets of range r (sets are sorted by size).
+uint64_t midprefix_entries(const hs_ctx* ctx, const hs_coll* coll, const hs_range& r) {
+  const auto& sf = coll->h_size_first;
+  uint64_t acc = 0;
+  for (uint32_t l = 0; l + 1 < (uint32_t)sf.size(); ++l) {
+    const uint32_t lo = std::max(r.first, sf[l]), hi = std::min(r.last + 1, sf[l + 1]);
+    if (hi > lo) acc += (uint64_t)(hi - lo) * ctx->tables.midprefix[std::min(l, ctx->tables.L)];
+  }
+  return acc;
+}
+
 // Cholesky solve of the (k x k) symmetric positive-definite system A x = b.
 bool cholesky_solve(std::vector<double> A, std::vector<double>& b, uint32_t k) {
   for (uint32_t j = 0; j < k; ++j) {
@@ -161,6 +214,56 @@ int hs_features(hs_ctx* ctx, hs_coll* coll, const hs_range* range, float out[HS_
     out[4] = (float)U;                // universeSize
     out[5] = (float)acc;              // indexSize
     out[6] = (float)ctx->t;           // threshold
+  });
+}
+
+int hs_features_pair(hs_ctx* ctx, hs_coll* coll, const hs_range* left, const hs_range* right,
+                     float out[HS_NFEATURES]) {
+  if (!ctx) return HS_E_INVALID;
+  return guarded(ctx, [&] {
+    HS_REQUIRE(coll && left && right && out, HS_E_INVALID, "NULL argument");
+    HS_REQUIRE(ctx->device >= 0 && coll->n_joins == 0, HS_E_INVALID, "features need a device collection");
+    const hs_range a = clamp_range(left, coll->n), b = clamp_range(right, coll->n);
+    HS_REQUIRE(!range_empty(a) && !range_empty(b) && left->last < coll->n && right->last < coll->n, HS_E_INVALID,
+               "ranges must be non-empty and inside the collection");
+    ensure_bounds(ctx, coll->maxlen);
+    hipStream_t st = ctx->stream;
+    const uint32_t U = coll->universe;
+    ctx->feat.ensure(4 * sizeof(double));
+    ctx->feat_hist.ensure((size_t)std::max<uint32_t>(U, 1) * 4);
+    double* dacc = ctx->feat.as<double>();
+    uint32_t* hist = ctx->feat_hist.as<uint32_t>();
+    hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, st, dacc, 4u);
+    launch_fill_u32(st, hist, U, 0u);
+    // sizes of [first, last) of each range (statistics.hpp:236-237); the
+    // slots of the two last sets stay zero
+    const uint32_t na = a.last - a.first + 1, nb = b.last - b.first + 1;
+    if (na > 1)
+      hipLaunchKernelGGL(k_size_moments, dim3(std::min<uint32_t>((na + 254) / 256, 1024)), dim3(256), 0, st,
+                         coll->sizes.as<uint32_t>(), a.first, a.last - 1, dacc);
+    if (nb > 1)
+      hipLaunchKernelGGL(k_size_moments, dim3(std::min<uint32_t>((nb + 254) / 256, 1024)), dim3(256), 0, st,
+                         coll->sizes.as<uint32_t>(), b.first, b.last - 1, dacc);
+    const uint32_t nsets = na + nb;
+    hipLaunchKernelGGL(k_prefix_hist, dim3(std::min<uint32_t>((nsets + 255) / 256, 4096)), dim3(256), 0, st,
+                       coll->dev(), ctx->dev_bounds.midprefix, ctx->dev_bounds.L, a.first, na, b.first, nb, U, hist);
+    if (U)
+      hipLaunchKernelGGL(k_count_moments, dim3(std::min<uint32_t>((U + 255) / 256, 1024)), dim3(256), 0, st, hist,
+                         U, dacc);
+    HS_CHECK_HIP(hipGetLastError());
+    double h[4];
+    HS_CHECK_HIP(hipMemcpyAsync(h, dacc, sizeof(h), hipMemcpyDeviceToHost, st));
+    HS_CHECK_HIP(hipStreamSynchronize(st));
+    const double m = (double)na + nb;
+    const double mean = h[0] / m, var = std::max(0.0, h[1] / m - mean * mean);
+    const double Ud = std::max<uint32_t>(U, 1), fmean = h[2] / Ud, fvar = std::max(0.0, h[3] / Ud - fmean * fmean);
+    out[0] = (float)m;                               // size
+    out[1] = (float)mean;                            // averageSetSize
+    out[2] = (float)std::sqrt(var);                  // setSizeStd
+    out[3] = (float)std::sqrt(fvar);                 // tokenFreqDistribution
+    out[4] = (float)U;                               // universeSize
+    out[5] = (float)midprefix_entries(ctx, coll, a);  // indexSize: the left partition's entries
+    out[6] = (float)ctx->t;                          // threshold
   });
 }
 

@@ -10,6 +10,9 @@
 //   algorithms::hybrid::device_handler                include/hyset/hybrid_algorithms.hpp:16-205
 //   algorithms::host::handler (allpairs / ppjoin)     include/hyset/host_algorithms.hpp:16-250,
 //                                                     src/cpu_single.cu:118-135, src/cpu_multi.cu:118-207
+//   partitioner::even / dichotomize                   include/hyset/partitioner.hpp:11-68
+//   cost::run, cost::read_coefficients               include/hyset/cost.hpp:28-58,
+//                                                     src/cost_model.cu:296-323
 //
 // Differences from the reference, all deliberate:
 //   * errors throw hyset_amd::error carrying the HS_E_* code and message
@@ -21,8 +24,16 @@
 #pragma once
 
 #include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
 #include <future>
+#include <map>
 #include <memory>
+#include <sstream>
+#include <utility>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -409,4 +420,188 @@ struct handler {
 };
 }  // namespace host
 }  // namespace algorithms
+
+// ------------------------------------------------ partitioner, cost model
+
+namespace partitioner {
+// partitioner::even (partitioner.hpp:11-30): ceil(blocks / n) consecutive
+// blocks per partition.
+inline std::vector<structs::partition> even(std::vector<structs::block>& blocks, unsigned int numOfPartitions) {
+  std::vector<structs::partition> out;
+  if (numOfPartitions == 0) throw error(HS_E_INVALID, "no partitions");
+  const unsigned int per = (unsigned int)std::ceil((double)blocks.size() / (double)numOfPartitions);
+  unsigned int id = 0;
+  for (size_t i = 0; i < blocks.size(); i += per) {
+    structs::partition p(id++);
+    for (size_t b = i; b < std::min(blocks.size(), i + per); ++b) p.blocks.push_back(&blocks[b]);
+    out.push_back(p);
+  }
+  return out;
+}
+
+// partitioner::dichotomize (partitioner.hpp:32-68): {complete, the first
+// ceil(split / 100 * blocks) blocks, the rest}; an empty part has no blocks.
+inline std::vector<structs::partition> dichotomize(std::vector<structs::block>& blocks, double split) {
+  const size_t portion = std::min(blocks.size(), (size_t)std::ceil((split / 100.0) * blocks.size()));
+  std::vector<structs::partition> out{structs::partition(0), structs::partition(1), structs::partition(2)};
+  for (size_t b = 0; b < blocks.size(); ++b) {
+    out[0].blocks.push_back(&blocks[b]);
+    out[b < portion ? 1 : 2].blocks.push_back(&blocks[b]);
+  }
+  return out;
+}
+}  // namespace partitioner
+
+namespace cost {
+// The reference's technique names (cost_model.cu:304) and HS_TECH_* values.
+inline const std::vector<std::pair<std::string, int32_t>>& techniques() {
+  static const std::vector<std::pair<std::string, int32_t>> t{
+      {"bitmap", HS_TECH_BITMAP}, {"cpu", HS_TECH_CPU}, {"fgssjoin", HS_TECH_FGSS}, {"hybrid", HS_TECH_HYBRID}};
+  return t;
+}
+
+namespace detail {
+// A JSON value reduced to what a coefficient file holds: objects and numbers.
+struct json {
+  std::map<std::string, json> obj;
+  double num = 0.0;
+  bool is_num = false;
+};
+struct json_reader {
+  const std::string& s;
+  size_t i = 0;
+  void ws() {
+    while (i < s.size() && std::isspace((unsigned char)s[i])) ++i;
+  }
+  [[noreturn]] void fail(const char* what) const {
+    throw error(HS_E_INVALID, std::string("coefficient file: ") + what + " at offset " + std::to_string(i));
+  }
+  std::string str() {
+    if (s[i] != '"') fail("expected a string");
+    std::string out;
+    for (++i; i < s.size() && s[i] != '"'; ++i) {
+      if (s[i] == '\\' && i + 1 < s.size()) ++i;
+      out += s[i];
+    }
+    if (i >= s.size()) fail("unterminated string");
+    ++i;
+    return out;
+  }
+  json value() {
+    ws();
+    if (i >= s.size()) fail("unexpected end");
+    json v;
+    if (s[i] == '{') {
+      ++i;
+      ws();
+      if (s[i] == '}') {
+        ++i;
+        return v;
+      }
+      for (;;) {
+        ws();
+        std::string k = str();
+        ws();
+        if (s[i] != ':') fail("expected ':'");
+        ++i;
+        v.obj[k] = value();
+        ws();
+        if (s[i] == ',') {
+          ++i;
+          continue;
+        }
+        if (s[i] == '}') {
+          ++i;
+          return v;
+        }
+        fail("expected ',' or '}'");
+      }
+    }
+    char* end = nullptr;
+    v.num = std::strtod(s.c_str() + i, &end);
+    if (end == s.c_str() + i) fail("expected a number or an object");
+    i = (size_t)(end - s.c_str());
+    v.is_num = true;
+    return v;
+  }
+};
+inline double number(const json& j, const std::string& key, const char* technique) {
+  auto it = j.obj.find(key);
+  if (it == j.obj.end() || !it->second.is_num)
+    throw error(HS_E_INVALID, std::string("coefficient file: no number ") + key + " for " + technique);
+  return it->second.num;
+}
+}  // namespace detail
+
+// read_coefficients (src/cost_model.cu:296-323): per technique the entry of
+// the threshold printed with two significant digits ("0.8", "0.85"), terms
+// Intercept, set_size_std, avg_set_size, token_freq, dataset_size,
+// dataset_size_square, universe_size, and index_size (not bitmap) or bitmap
+// (bitmap only).
+inline std::map<std::string, hs_cost_coefficients> read_coefficients(const std::string& path, double threshold) {
+  std::ifstream f(path);
+  if (!f) throw error(HS_E_INVALID, "cannot read " + path);
+  std::stringstream buf;
+  buf << f.rdbuf();
+  const std::string text = buf.str();
+  detail::json_reader rd{text};
+  const detail::json root = rd.value();
+  char key[32];
+  std::snprintf(key, sizeof(key), "%.2g", threshold);
+  std::map<std::string, hs_cost_coefficients> out;
+  for (const auto& [name, tech] : techniques()) {
+    (void)tech;
+    auto t = root.obj.find(name);
+    if (t == root.obj.end()) throw error(HS_E_INVALID, "coefficient file: no technique " + name);
+    auto e = t->second.obj.find(key);
+    if (e == t->second.obj.end()) throw error(HS_E_INVALID, "coefficient file: no threshold " + std::string(key));
+    const detail::json& j = e->second;
+    hs_cost_coefficients c{};
+    c.intercept = detail::number(j, "Intercept", name.c_str());
+    c.set_size_std = detail::number(j, "set_size_std", name.c_str());
+    c.average_set_size = detail::number(j, "avg_set_size", name.c_str());
+    c.token_freq = detail::number(j, "token_freq", name.c_str());
+    c.size = detail::number(j, "dataset_size", name.c_str());
+    c.size_square = detail::number(j, "dataset_size_square", name.c_str());
+    c.universe_size = detail::number(j, "universe_size", name.c_str());
+    if (name == "bitmap") c.bitmap = detail::number(j, "bitmap", name.c_str());
+    else c.index_size = detail::number(j, "index_size", name.c_str());
+    c.enabled = 1;
+    out[name] = c;
+  }
+  return out;
+}
+
+using partition_pair = std::pair<structs::partition*, structs::partition*>;
+
+// cost::run (cost.hpp:28-58): every partition pair (left, right), right.id
+// >= left.id, under the technique of least modelled cost; techniques
+// missing from `coefficients` (as bitmap when --bitmap is 0) are no choice.
+inline std::map<std::string, std::vector<partition_pair>> run(std::map<std::string, hs_cost_coefficients>& coefficients,
+                                                              std::vector<structs::partition>& partitions,
+                                                              context& ctx, collection::device_collection& coll,
+                                                              unsigned int bitmapSize) {
+  hs_cost_coefficients coef[4] = {};
+  std::map<int32_t, std::string> name_of;
+  for (const auto& [name, tech] : techniques()) {
+    name_of[tech] = name;
+    auto it = coefficients.find(name);
+    if (it != coefficients.end()) coef[tech] = it->second;
+    else coef[tech].enabled = 0;
+  }
+  std::vector<hs_range> ranges;
+  for (auto& p : partitions) ranges.push_back(hs_range{p.start_id(), p.end_id()});
+  const size_t n = partitions.size();
+  std::vector<int32_t> tech(n * (n + 1) / 2);
+  check(hs_cost_run(ctx.get(), coll.get(), ranges.data(), (uint32_t)n, coef, bitmapSize, 0, tech.data(), nullptr,
+                    nullptr),
+        ctx.get(), "hs_cost_run");
+  std::map<std::string, std::vector<partition_pair>> out;
+  size_t q = 0;
+  for (size_t i = 0; i < n; ++i)
+    for (size_t j = i; j < n; ++j, ++q) out[name_of[tech[q]]].push_back({&partitions[i], &partitions[j]});
+  return out;
+}
+}  // namespace cost
+
 }  // namespace hyset_amd

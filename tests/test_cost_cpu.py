@@ -125,3 +125,76 @@ def test_plan_cost_makespan_brute_force():
     plan = S.concurrent_plan(T, 1)
     assert S.plan_cost(plan, T, 1) == pytest.approx(best)
     assert S.plan_cost(np.array([0, 0, 0, 0]), T) == 10.0
+
+
+# ------------------------------------------------- partition pairs (cost::run)
+
+def test_partitioners_hand_cut_and_library_agree():
+    # 25 sets in blocks of 10: [0-9] [10-19] [20-24]
+    assert OC.partition_even(25, 10, 2) == [(0, 19), (20, 24)]  # ceil(3/2) = 2 blocks each
+    assert OC.partition_even(25, 10, 3) == [(0, 9), (10, 19), (20, 24)]
+    assert OC.partition_even(25, 10, 7) == [(0, 9), (10, 19), (20, 24)]  # fewer partitions than asked
+    assert OC.partition_even(30, 10, 3) == [(0, 9), (10, 19), (20, 29)]
+    assert OC.partition_dichotomize(25, 10, 50) == ((0, 24), (0, 19), (20, 24))  # ceil(1.5) = 2 blocks
+    assert OC.partition_dichotomize(25, 10, 0) == ((0, 24), None, (0, 24))
+    assert OC.partition_dichotomize(25, 10, 100) == ((0, 24), (0, 24), None)
+    assert OC.partition_dichotomize(25, 10, 10) == ((0, 24), (0, 9), (10, 24))  # ceil(0.3) = 1 block
+    rng = np.random.default_rng(5)
+    for _ in range(300):
+        n, block = int(rng.integers(1, 5000)), int(rng.integers(1, 700))
+        parts, split = int(rng.integers(1, 40)), float(rng.uniform(0, 100))
+        ev = H.partition_even(n, block, parts)
+        assert ev == OC.partition_even(n, block, parts)
+        # a cover of [0, n) by consecutive ranges, each a whole number of blocks but the last
+        assert ev[0][0] == 0 and ev[-1][1] == n - 1 and len(ev) <= parts
+        assert all(a[1] + 1 == b[0] and b[0] % block == 0 for a, b in zip(ev, ev[1:]))
+        d = H.partition_dichotomize(n, block, split)
+        assert d == OC.partition_dichotomize(n, block, split)
+        small, large = d[1], d[2]
+        assert (small[1] + 1 if small else 0) == (large[0] if large else n)
+    with pytest.raises(H.HysetError):
+        H.partition_even(10, 0, 2)
+    with pytest.raises(H.HysetError):
+        H.partition_dichotomize(10, 5, 101)
+
+
+def test_oracle_features_pair_hand_computed():
+    # sizes 2, 3, 3, 5, 5; Jaccard 0.8 -> midprefix 1 for every size here
+    sets = [[0, 1], [0, 2, 3], [1, 2, 4], [0, 1, 2, 3, 5], [2, 3, 4, 5, 6]]
+    c = _coll(sets)
+    assert [or_bounds("jaccard", 0.8, l)[4] for l in (2, 3, 5)] == [1, 1, 1]
+    f = OC.features_pair(c, 0.8, (0, 1), (2, 4))
+    # the reference's vector: sizes[0:1] + [0] + sizes[2:4] + [0] = [2, 0, 3, 5, 0]
+    v = np.array([2, 0, 3, 5, 0], dtype=float)
+    assert f[0] == 5
+    assert f[1] == pytest.approx(v.mean())
+    assert f[2] == pytest.approx(np.sqrt(((v - v.mean()) ** 2).mean()))
+    # midprefix tokens of all five sets: 0, 0, 1, 0, 2 over universe 7 -> counts [3, 1, 1, 0, 0, 0, 0]
+    cnt = np.array([3, 1, 1, 0, 0, 0, 0], dtype=float)
+    assert f[3] == pytest.approx(np.sqrt(((cnt - cnt.mean()) ** 2).mean()))
+    assert f[4] == 7 and f[5] == 2 and f[6] == pytest.approx(0.8)  # indexSize: the left range only
+    # one-set ranges: both slots zero
+    g = OC.features_pair(c, 0.8, (3, 3), (4, 4))
+    assert g[0] == 2 and g[1] == 0 and g[2] == 0 and g[5] == 1
+
+
+def test_oracle_cost_run_hand_computed():
+    f = [np.array([4, 10.0, 2.0, 3.0, 100, 50, 0.8])]  # size 4, avg 10, std 2, tok 3, U 100, index 50
+    coefs = np.zeros((4, 9))
+    coefs[H.TECH_FGSS] = [1, 0, 0, 0, 0, 0, 0, 0.5, 0]           # |1 + 0.5*50| = 26
+    coefs[H.TECH_BITMAP] = [0, 0, 0, 0, 0, 0, 0, 0, 0.25]        # |0.25 * 64| = 16
+    coefs[H.TECH_HYBRID] = [-30, 1, 1, 1, 0, 0, 0, 0, 0]         # |-30 + 2 + 10 + 3| = 15
+    coefs[H.TECH_CPU] = [0, 0, 0, 0, 1, 1, 0, 0, 0]              # |4 + 16| = 20
+    assert OC.linear_cost(coefs[H.TECH_HYBRID], f[0], 64) == pytest.approx(15)
+    assert OC.cost_run(f, coefs, (1, 1, 1, 1), 64) == [H.TECH_HYBRID]
+    assert OC.cost_run(f, coefs, (1, 1, 0, 1), 64) == [H.TECH_BITMAP]
+    # a tie between cpu and fgssjoin goes to cpu (map order bitmap, cpu, fgssjoin, hybrid)
+    coefs[H.TECH_CPU] = [26, 0, 0, 0, 0, 0, 0, 0, 0]
+    assert OC.cost_run(f, coefs, (1, 0, 0, 1), 64) == [H.TECH_CPU]
+    assert OC.cost_run(f, coefs, (1, 0, 0, 0), 64) == [H.TECH_FGSS]
+    # size^2 as the reference's unsigned product: 70000^2 mod 2^32
+    big = np.array([70000, 0, 0, 0, 0, 0, 0.8])
+    sq = np.zeros(9)
+    sq[5] = 1.0
+    assert OC.linear_cost(sq, big, 0) == 70000 ** 2
+    assert OC.linear_cost(sq, big, 0, wrap_square=True) == 70000 ** 2 - (1 << 32)

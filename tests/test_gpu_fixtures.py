@@ -87,6 +87,57 @@ def test_cpp_driver_cpu_algorithms(tmp_path, args):
     assert res.returncode != 0
 
 
+def _coefficient_file(path, t):
+    """A coefficient table in the reference's format (src/cost_model.cu:296-323):
+    the CPU cheapest for small partition pairs, fgss for large ones, bitmap
+    and hybrid never (their intercepts are large)."""
+    import json
+    key = f"{t:.2g}"
+    base = dict(Intercept=0.0, set_size_std=0.0, avg_set_size=0.0, token_freq=0.0, dataset_size=0.0,
+                dataset_size_square=0.0, universe_size=0.0)
+    tab = {"cpu": dict(base, dataset_size=1e-3, index_size=0.0),
+           "fgssjoin": dict(base, Intercept=1.5, index_size=0.0),
+           "hybrid": dict(base, Intercept=1e6, index_size=0.0),
+           "bitmap": dict(base, Intercept=1e6, bitmap=0.0)}
+    path.write_text(json.dumps({k: {key: v, "0.5": v} for k, v in tab.items()}))
+
+
+@pytest.mark.parametrize("args", [["--technique", "queue", "--cpu-threads", "2"],
+                                  ["--technique", "queue", "--cooperative", "--cpu-threads", "2"],
+                                  ["--technique", "queue", "--bitmap", "128", "--cpu-threads", "2"],
+                                  ["--technique", "dichotomy", "--policy", "cpu,gpu", "--split", "40"],
+                                  ["--technique", "dichotomy", "--policy", "gpu,cpu", "--split", "40"],
+                                  ["--technique", "dichotomy", "--policy", "cpu,cpu-gpu", "--split", "70"],
+                                  ["--technique", "dichotomy", "--policy", "cpu-gpu,cpu", "--split", "20",
+                                   "--algorithm", "ppjoin"],
+                                  ["--technique", "cost", "--partitions", "3", "--cpu-threads", "2"]])
+def test_cpp_driver_cpu_and_device_together(tmp_path, args):
+    """The reference's queue, dichotomy and cost_model drivers: CPU threads
+    and the device on one join, pairs and overlaps as the golden fixture."""
+    col = tmp_path / "neardup.txt"
+    H.write_collection_text(col, F.neardup())
+    out = tmp_path / "pairs.txt"
+    coef = tmp_path / "coefficients.json"
+    extra = ["--coefficients", str(coef)] if "cost" in args else []
+    for t in (0.7, 0.9):
+        _coefficient_file(coef, t)
+        res = subprocess.run([str(DRIVER), "--input", str(col), "--threshold", str(t), "--block", "100",
+                              "--output", str(out), "--overlap", *args, *extra], capture_output=True, text=True,
+                             timeout=120)
+        assert res.returncode == 0, res.stderr
+        got = sorted(tuple(int(x) for x in line.split()) for line in out.read_text().splitlines())
+        assert got == F.read_pairs(GOLD / f"neardup_t{t}.pairs"), (args, t)
+        res = subprocess.run([str(DRIVER), "--input", str(col), "--threshold", str(t), "--block", "100", *args,
+                              *extra], capture_output=True, text=True, timeout=120)
+        assert res.returncode == 0 and f"Result: {len(got)}" in res.stdout, res.stdout + res.stderr
+        if "cost" in args:
+            assert "cpu:" in res.stdout and "fgssjoin:" in res.stdout  # both chosen for some pairs
+    for bad in (["--technique", "dichotomy", "--policy", "gpu,cpu-gpu"], ["--technique", "dichotomy", "--policy",
+                "cpu,cpu"], ["--technique", "dichotomy", "--split", "120"], ["--technique", "cost"]):
+        res = subprocess.run([str(DRIVER), "--input", str(col), *bad], capture_output=True, text=True, timeout=60)
+        assert res.returncode != 0, bad
+
+
 def _names(n, seed):
     """Strings of 8-20 letters, 30% of them copies of earlier ones with 1-2
     character edits (the string-matching use of the join)."""
