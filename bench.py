@@ -128,6 +128,76 @@ def bench_coprocess(ctx, dI, dP, name, threads, want, reps=3) -> dict:
     return res
 
 
+def bench_policies(ctx, d, want, threads) -> dict:
+    """The dichotomy's policy pairs (src/dichotomy.cu) at the reference's
+    default --split 50 and the queue's cooperative (hybrid) device consumer
+    on the headline join: wall time of one call each, after a warm-up."""
+    res = {}
+    runs = (("cpu_gpu", dict(mode="static", policy="cpu,gpu", algo="fgss", cpu_share=0.5)),
+            ("gpu_cpu", dict(mode="static", policy="gpu,cpu", algo="fgss", cpu_share=0.5)),
+            ("cpu_hybrid", dict(mode="static", policy="cpu,gpu", algo="hybrid", cpu_share=0.5)),
+            ("hybrid_cpu", dict(mode="static", policy="gpu,cpu", algo="hybrid", cpu_share=0.5)),
+            ("queue_cooperative", dict(mode="queue", algo="hybrid", min_device_ms=-1)))
+    for name, kw in runs:
+        tune = None
+        if kw["algo"] == "hybrid":
+            import hyset_amd as H
+            tune = H.Tune(host_threads=max(1, threads // 2))
+        ms = []
+        for _ in range(2):
+            a = time.perf_counter()
+            c, info = ctx.join_coprocess(d, cpu_threads=max(1, threads - (threads // 2 if tune else 0)), tune=tune,
+                                         count_only=True, **kw)
+            ms.append((time.perf_counter() - a) * 1e3)
+            assert c == want
+        res[f"policy_{name}_ms"] = round(ms[-1], 2)
+        res[f"policy_{name}_host_probes"] = int(info["host_probes"])
+        log(f"policy {name}: {ms[-1]:.2f} ms (host {info['host_probes']} probes, {info['host_pairs']} pairs; "
+            f"device {info['device_probes']} probes, {info['device_pairs']} pairs)")
+    return res
+
+
+def bench_partition_pairs(ctx, H, d, coll, t, want, threads) -> dict:
+    """cost::run over partition pairs of the headline collection: the
+    analytic model fitted to measured pair timings (every technique, bitmap
+    at 64/128/256 bits) of 2, 3 and 6 even partitions (blocks of 10,000
+    sets, the reference's default), then the plan of 4 partitions run by
+    hs_run_partition_pairs (CPU pairs on host threads), against each
+    technique alone over the same pairs and the whole join in one call."""
+    from hyset_amd import split as S
+    trains = [H.partition_even(coll.n, 10_000, p) for p in (2, 3, 6)]
+    a = time.perf_counter()
+    F, T, bt, counts, _ = S.measure_partition_pairs(ctx, d, trains, t, cpu_threads=1)
+    meas_s = time.perf_counter() - a
+    model = S.AnalyticModel().fit(F, T, bitmap_times=bt)
+    parts = H.partition_even(coll.n, 10_000, 4)
+    plan = S.partition_plan(ctx, d, parts, model, t)
+
+    def wall(p):
+        ms = []
+        for _ in range(3):
+            n, _, _, w = ctx.run_partition_pairs(d, p, cpu_threads=threads, count_only=True)
+            assert n == want
+            ms.append(w)
+        return round(statistics.median(ms), 3)
+
+    res = {"partition_pairs": {
+        "train_pairs": int(F.shape[0]), "measure_s": round(meas_s, 1),
+        "plan": {S.TECH_NAMES[k]: sum(1 for *_, x in plan if x == k) for k in range(4)},
+        "plan_ms": wall(plan),
+        "fixed_ms": {S.TECH_NAMES[k]: wall([(l, r, k) for l, r, _ in plan]) for k in range(4)},
+        "whole_join_ms": None,
+        "pairs_4_partitions": len(plan)}}
+    ms = []
+    for _ in range(3):
+        a = time.perf_counter()
+        assert ctx.join(d, algo="fgss", count_only=True) == want
+        ms.append((time.perf_counter() - a) * 1e3)
+    res["partition_pairs"]["whole_join_ms"] = round(statistics.median(ms), 3)
+    log(f"partition pairs: {res['partition_pairs']}")
+    return res
+
+
 def bench_long_rs(ctx, H) -> dict:
     """The long-set R-S workload: R = S = 200,000 sets of 100-700 tokens over
     a 50,000-token Zipf(0.8) universe, 20% of S edited copies of R sets,
@@ -681,6 +751,8 @@ def main() -> int:
         assert c == pairs
         log(f"hybrid ({min(16, ncores)} host threads): {extra['hybrid_ms_wall']} ms")
         extra.update(bench_coprocess(ctx, d, d, "t0.8", 0, pairs))
+        extra.update(bench_policies(ctx, d, pairs, max(2, ncores - 1)))
+        extra.update(bench_partition_pairs(ctx, H, d, coll, t, pairs, max(2, ncores - 1)))
         ctx.set_threshold(0.6)
         extra.update(bench_coprocess(ctx, d, d, "t0.6", 0, extra["fgss_t0.6_pairs"]))
         ctx.set_threshold(t)

@@ -10,9 +10,11 @@ no coefficients, so:
 
 * AnalyticModel fits those linear costs by ridge least squares (the
   library's hs_linear_fit) to timings this library measures, one table per
-  technique and threshold, on the reference's eight terms (setSizeStd,
-  averageSetSize, tokenFreqDistribution, size, size^2, universeSize,
-  indexSize, bitmap);
+  technique and threshold, on each technique's terms of the reference's
+  coefficient file (setSizeStd, averageSetSize, tokenFreqDistribution,
+  size, size^2, universeSize, and indexSize -- or, for bitmap, the
+  signature width); it reads and writes that file's JSON layout, and its
+  tables drive hs_cost_run (cost::run over partition pairs) as they are;
 * LearnedModel trains the library's MLP (hs_mlp_train: device forward and
   backward kernels, Adam) on the same timings to predict log-time per
   technique from log-features and the threshold, and predicts with the
@@ -30,18 +32,10 @@ import time
 
 import numpy as np
 
-from . import TECH_BITMAP, TECH_CPU, TECH_FGSS, TECH_HYBRID, TECHNIQUES, Context, linear_fit, linear_predict
+from . import TECH_BITMAP, TECH_CPU, TECH_FGSS, TECH_HYBRID, TECHNIQUES, Context, linear_fit
 
 N_TECH = len(TECHNIQUES)
 GPU_TECHS = (TECH_FGSS, TECH_BITMAP, TECH_HYBRID)
-
-
-def reference_terms(F: np.ndarray, bitmap_bits: int = 64) -> np.ndarray:
-    """The eight terms of statistics.hpp:150-160 from hs_features rows."""
-    F = np.atleast_2d(np.asarray(F, dtype=np.float64))
-    size = F[:, 0]
-    return np.stack([F[:, 2], F[:, 1], F[:, 3], size, size * size, F[:, 4], F[:, 5],
-                     np.full_like(size, float(bitmap_bits))], axis=1)
 
 
 def learned_inputs(F: np.ndarray) -> np.ndarray:
@@ -88,41 +82,124 @@ def measure_units(ctx: Context, units, t: float, reps: int = 1, cpu_threads: int
 
 # ---------------------------------------------------------------- models
 
+TECH_NAMES = {TECH_BITMAP: "bitmap", TECH_CPU: "cpu", TECH_FGSS: "fgssjoin", TECH_HYBRID: "hybrid"}
+# the reference's JSON keys per coefficient (src/cost_model.cu:306-317), in COST_TERMS order
+JSON_KEYS = ("Intercept", "set_size_std", "avg_set_size", "token_freq", "dataset_size", "dataset_size_square",
+             "universe_size", "index_size", "bitmap")
+
+
+def technique_terms(F: np.ndarray, tech: int, bitmap_bits) -> np.ndarray:
+    """The terms of one technique's linear cost, in COST_TERMS order after
+    the intercept: setSizeStd, averageSetSize, tokenFreqDistribution, size,
+    size^2, universeSize, and indexSize for every technique but bitmap,
+    bitmap (the signature width) for bitmap -- read_coefficients gives
+    bitmap no index_size and the others no bitmap (cost_model.cu:314-315)."""
+    F = np.atleast_2d(np.asarray(F, dtype=np.float64))
+    size = F[:, 0]
+    last = np.broadcast_to(np.asarray(bitmap_bits, dtype=np.float64), size.shape) if tech == TECH_BITMAP else F[:, 5]
+    return np.stack([F[:, 2], F[:, 1], F[:, 3], size, size * size, F[:, 4], last], axis=1)
+
+
 class AnalyticModel:
-    """Linear cost per technique and threshold (statistics.hpp:132-161)."""
+    """The reference's linear cost per technique and threshold
+    (statistics.hpp:132-161; one table per threshold key as
+    cost_model.cu:296-323), fitted by ridge least squares (hs_linear_fit)
+    on column-scaled terms and stored unscaled in the reference's layout,
+    so it evaluates as cost::run does and exports to the reference's
+    coefficient file."""
 
     def __init__(self, ridge: float = 1e-6, bitmap_bits: int = 64):
         self.ridge = ridge
         self.bitmap_bits = bitmap_bits
-        self.coef: dict[str, np.ndarray] = {}
-        self.scale: np.ndarray | None = None
+        self.table: dict[str, np.ndarray] = {}  # threshold key -> (4, 9) in COST_TERMS order
 
-    def _terms(self, F):
-        X = reference_terms(F, self.bitmap_bits)
-        # the bitmap term is constant for one width: it folds into the intercept
-        return X[:, :7] / self.scale
+    @staticmethod
+    def _fit_one(X, y, ridge):
+        # a term constant over the samples (the universe of one collection's
+        # partitions, the width of one bitmap) is the intercept's: weight 0
+        live = np.ptp(X, axis=0) > 1e-12 * np.maximum(np.abs(X).max(axis=0), 1e-300)
+        w = np.zeros(X.shape[1])
+        if not live.any():
+            return w, float(np.mean(y))
+        Xl = X[:, live]
+        scale = np.maximum(np.abs(Xl).max(axis=0), 1e-12)
+        coef = linear_fit(Xl / scale, np.asarray(y, dtype=np.float64), ridge)  # (d + 1, 1): terms, intercept
+        w[live] = coef[:-1, 0] / scale
+        return w, float(coef[-1, 0])
 
-    def fit(self, F, T) -> "AnalyticModel":
+    def fit(self, F, T, bitmap_times: dict | None = None) -> "AnalyticModel":
+        """F (n, 7) features, T (n, 4) times per HS_TECH_*.  bitmap_times
+        {bits: (n,) times} gives the bitmap technique at several signature
+        widths (its bitmap term); without it T[:, bitmap] is at
+        self.bitmap_bits and the term is constant."""
         F = np.atleast_2d(F)
-        X = reference_terms(F, self.bitmap_bits)[:, :7]
-        self.scale = np.maximum(np.abs(X).max(axis=0), 1e-12)
+        T = np.asarray(T, dtype=np.float64)
         keys = np.array([threshold_key(t) for t in F[:, 6]])
+        bt = bitmap_times if bitmap_times else {self.bitmap_bits: T[:, TECH_BITMAP]}
         for k in np.unique(keys):
             m = keys == k
-            self.coef[k] = linear_fit(self._terms(F[m]), np.asarray(T, dtype=np.float64)[m], self.ridge)
+            tab = np.zeros((4, 9))
+            for tech in range(N_TECH):
+                if tech == TECH_BITMAP:
+                    X = np.concatenate([technique_terms(F[m], tech, b) for b in bt])
+                    y = np.concatenate([np.asarray(v, dtype=np.float64)[m] for v in bt.values()])
+                else:
+                    X, y = technique_terms(F[m], tech, 0), T[m, tech]
+                w, b0 = self._fit_one(X, y, self.ridge)
+                tab[tech, 0] = b0
+                tab[tech, 1:7] = w[:6]
+                tab[tech, 8 if tech == TECH_BITMAP else 7] = w[6]
+            self.table[k] = tab
         return self
 
-    def predict(self, F) -> np.ndarray:
+    def coefficients(self, t: float) -> np.ndarray:
+        """(4, 9) table of threshold t (the nearest fitted one), rows HS_TECH_*."""
+        k = threshold_key(t)
+        if k not in self.table:
+            k = min(self.table, key=lambda c: abs(float(c) - float(k)))
+        return self.table[k]
+
+    def predict(self, F, bitmap_bits: int | None = None) -> np.ndarray:
         F = np.atleast_2d(F)
+        bits = self.bitmap_bits if bitmap_bits is None else bitmap_bits
         out = np.empty((F.shape[0], N_TECH))
-        keys = np.array([threshold_key(t) for t in F[:, 6]])
-        for k in np.unique(keys):
-            m = keys == k
-            coef = self.coef.get(k)
-            if coef is None:  # nearest fitted threshold
-                coef = self.coef[min(self.coef, key=lambda c: abs(float(c) - float(k)))]
-            out[m] = linear_predict(coef, self._terms(F[m]))
+        for i, f in enumerate(F):
+            tab = self.coefficients(float(f[6]))
+            for tech in range(N_TECH):
+                x = technique_terms(f[None, :], tech, bits)[0]
+                w = np.concatenate([tab[tech, 1:7], tab[tech, 8:9] if tech == TECH_BITMAP else tab[tech, 7:8]])
+                out[i, tech] = abs(tab[tech, 0] + float(w @ x))  # statistics.hpp:150-160 takes |.|
         return out
+
+    def to_reference_json(self) -> dict:
+        """The coefficient file of src/cost_model.cu:296-323:
+        {technique: {threshold key: {Intercept, set_size_std, ...}}}."""
+        out = {TECH_NAMES[t]: {} for t in range(N_TECH)}
+        for k, tab in self.table.items():
+            for tech in range(N_TECH):
+                row = {JSON_KEYS[j]: float(tab[tech, j]) for j in range(7)}
+                if tech == TECH_BITMAP:
+                    row["bitmap"] = float(tab[tech, 8])
+                else:
+                    row["index_size"] = float(tab[tech, 7])
+                out[TECH_NAMES[tech]][k] = row
+        return out
+
+    @classmethod
+    def from_reference_json(cls, d: dict, bitmap_bits: int = 64) -> "AnalyticModel":
+        m = cls(bitmap_bits=bitmap_bits)
+        keys = set()
+        for tech in range(N_TECH):
+            keys |= set(d[TECH_NAMES[tech]])
+        for k in keys:
+            tab = np.zeros((4, 9))
+            for tech in range(N_TECH):
+                row = d[TECH_NAMES[tech]][k]
+                for j in range(7):
+                    tab[tech, j] = row[JSON_KEYS[j]]
+                tab[tech, 8 if tech == TECH_BITMAP else 7] = row["bitmap" if tech == TECH_BITMAP else "index_size"]
+            m.table[k] = tab
+        return m
 
 
 class LearnedModel:
@@ -148,6 +225,58 @@ class LearnedModel:
 
     def predict(self, F) -> np.ndarray:
         return np.exp(self.net.predict(learned_inputs(F).astype(np.float32)).astype(np.float64))
+
+
+# ------------------------------------------------------- partition pairs
+
+def measure_partition_pairs(ctx: Context, coll, partitionings, t: float, cpu_threads: int = 1,
+                            bitmap_widths=(64, 128, 256), host_threads: int = 1):
+    """Every partition pair (i <= j) of each partitioning (a list of set-id
+    ranges of coll) at threshold t: its features (hs_cost_run's
+    hs_features / hs_features_pair) and its time with every technique
+    (hs_run_partition_pairs, all pairs on one technique: device pairs one
+    after another, CPU pairs on cpu_threads threads -- 1 gives one-at-a-time
+    times).  Returns (features (P, 7), times (P, 4) with bitmap at
+    bitmap_widths[0], {width: (P,) bitmap times}, counts (P,), pairs
+    [(left, right)]) and asserts every technique found each pair's count."""
+    from . import Tune
+    ctx.set_threshold(t)
+    feats, times, bt, counts, pairs = [], [], {b: [] for b in bitmap_widths}, [], []
+    zero = np.zeros((4, 9))
+    for parts in partitionings:
+        ij, _, f, _ = ctx.cost_run(coll, parts, zero)
+        plan_pairs = [(parts[i], parts[j]) for i, j in ij]
+        tt = np.zeros((len(ij), N_TECH))
+        seen = None
+        for tech in range(N_TECH):
+            widths = bitmap_widths if tech == TECH_BITMAP else (0,)
+            for b in widths:
+                tune = Tune(bitmap_bits=b or 64, host_threads=host_threads)
+                n, c, ms, _ = ctx.run_partition_pairs(coll, [(l, r, tech) for l, r in plan_pairs],
+                                                      cpu_threads=cpu_threads, tune=tune, count_only=True)
+                assert seen is None or np.array_equal(c, seen), f"technique {tech} disagrees on pair counts"
+                seen = c
+                if tech == TECH_BITMAP:
+                    bt[b].append(ms)
+                    if b == bitmap_widths[0]:
+                        tt[:, tech] = ms
+                else:
+                    tt[:, tech] = ms
+        feats.append(f.astype(np.float64))
+        times.append(tt)
+        counts.append(seen)
+        pairs.extend(plan_pairs)
+    return (np.concatenate(feats), np.concatenate(times), {b: np.concatenate(v) for b, v in bt.items()},
+            np.concatenate(counts), pairs)
+
+
+def partition_plan(ctx: Context, coll, parts, model: AnalyticModel, t: float, bitmap_bits: int = 64,
+                   enabled=(1, 1, 1, 1)):
+    """cost::run with the model's table for t (hs_cost_run): the plan
+    [(left, right, technique)] over the partition pairs of parts."""
+    ctx.set_threshold(t)
+    ij, tech, _, _ = ctx.cost_run(coll, parts, model.coefficients(t), enabled=enabled, bitmap_bits=bitmap_bits)
+    return [(parts[i], parts[j], int(k)) for (i, j), k in zip(ij, tech)]
 
 
 def log_error(pred: np.ndarray, T: np.ndarray) -> float:
@@ -321,4 +450,5 @@ def default_threads() -> int:
 
 __all__ = ["AnalyticModel", "LearnedModel", "Split", "measure_units", "argmin_plan", "concurrent_plan",
            "calibrate_cpu_scale", "calibrate_concurrency",
-           "plan_cost", "summarize", "log_error", "reference_terms", "learned_inputs", "default_threads"]
+           "plan_cost", "summarize", "log_error", "technique_terms", "learned_inputs", "default_threads",
+           "TECH_NAMES", "JSON_KEYS", "measure_partition_pairs", "partition_plan"]

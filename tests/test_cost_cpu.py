@@ -198,3 +198,45 @@ def test_oracle_cost_run_hand_computed():
     sq[5] = 1.0
     assert OC.linear_cost(sq, big, 0) == 70000 ** 2
     assert OC.linear_cost(sq, big, 0, wrap_square=True) == 70000 ** 2 - (1 << 32)
+
+
+def test_analytic_model_recovers_reference_tables_and_round_trips():
+    """Costs generated from a known coefficient table (rows HS_TECH_*,
+    columns COST_TERMS) are fitted back to that table, the bitmap term from
+    timings at three signature widths; predictions are the oracle's
+    cost::run costs of the fitted table; the reference's JSON layout round
+    trips."""
+    rng = np.random.default_rng(11)
+    n = 80
+    F = np.stack([rng.uniform(1e3, 5e4, n), rng.uniform(5, 40, n), rng.uniform(1, 10, n), rng.uniform(0.5, 20, n),
+                  rng.choice([2e4, 5e4, 1e5], n), rng.uniform(1e3, 1e5, n), np.full(n, 0.8)], axis=1)
+    F = np.round(F[:, :6]).astype(np.float32).astype(np.float64).tolist()  # hs_features rows are float32
+    F = np.concatenate([np.asarray(F), np.full((n, 1), 0.8)], axis=1)
+    true = np.zeros((4, 9))
+    true[H.TECH_FGSS] = [0.3, 0.01, 0.02, 0.05, 1e-5, 1e-10, 2e-7, 3e-6, 0]
+    true[H.TECH_BITMAP] = [1.0, 0.0, 0.01, 0.0, 2e-5, 4e-9, 0.0, 0, 0.004]
+    true[H.TECH_HYBRID] = [2.0, 0.03, 0.01, 0.1, 5e-5, 1e-9, 1e-7, 1e-5, 0]
+    true[H.TECH_CPU] = [0.1, 0.02, 0.05, 0.02, 8e-5, 2e-9, 0.0, 2e-5, 0]
+
+    def costs(bits):
+        return np.array([[OC.linear_cost(true[k], f, bits) for k in range(4)] for f in F])
+
+    T = costs(64)
+    bt = {b: costs(b)[:, H.TECH_BITMAP] for b in (64, 128, 256)}
+    m = S.AnalyticModel(ridge=1e-12).fit(F, T, bitmap_times=bt)
+    tab = m.coefficients(0.8)
+    np.testing.assert_allclose(tab, true, rtol=2e-3, atol=1e-5)  # normal equations on scaled terms
+    for bits in (64, 256):
+        want = np.array([[OC.linear_cost(tab[k], f, bits) for k in range(4)] for f in F])
+        np.testing.assert_allclose(m.predict(F, bitmap_bits=bits), want, rtol=1e-9)
+        np.testing.assert_allclose(m.predict(F, bitmap_bits=bits), costs(bits), rtol=1e-4)
+    d = m.to_reference_json()
+    assert set(d) == {"bitmap", "cpu", "fgssjoin", "hybrid"} and set(d["cpu"]) == {"0.8"}
+    assert "bitmap" in d["bitmap"]["0.8"] and "index_size" not in d["bitmap"]["0.8"]
+    assert "index_size" in d["fgssjoin"]["0.8"] and "bitmap" not in d["fgssjoin"]["0.8"]
+    assert set(d["cpu"]["0.8"]) == {"Intercept", "set_size_std", "avg_set_size", "token_freq", "dataset_size",
+                                    "dataset_size_square", "universe_size", "index_size"}
+    np.testing.assert_array_equal(S.AnalyticModel.from_reference_json(d).coefficients(0.8), tab)
+    # without widths the bitmap term is constant: it folds into the intercept, predictions still fit
+    m1 = S.AnalyticModel(ridge=1e-12).fit(F, T)
+    np.testing.assert_allclose(m1.predict(F), T, rtol=1e-4)

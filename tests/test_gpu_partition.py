@@ -140,3 +140,43 @@ def test_bad_plans(ctx, col):
         ctx.cost_run(d, [(0, 10)], np.zeros((4, 9)), enabled=(0, 0, 0, 0))
     with pytest.raises(H.HysetError):
         ctx.features_pair(d, (5, 4), (6, 7))
+
+
+def test_analytic_partition_plan_on_measured_timings(ctx):
+    """The analytic model fitted to measured partition-pair timings (every
+    technique, bitmap at three widths) over 2, 3 and 5 even partitions of
+    a 60K-set collection plans the 4-partition join through cost::run; the
+    plan's pairs are the oracle's, and its measured time (median of three)
+    is within 10% + 1 ms of the best single-technique plan's."""
+    from hyset_amd import split as S
+    c = H.generate_self(H.gen_params(60000, median=12, sigma=0.6, lo=2, hi=200, universe=60000, zipf=1.1, seed=8))
+    d = ctx.upload(c)
+    try:
+        t = 0.75
+        trains = [H.partition_even(c.n, 2500, p) for p in (2, 3, 5)]
+        F, T, bt, counts, _ = S.measure_partition_pairs(ctx, d, trains, t, cpu_threads=1)
+        assert F.shape == (3 + 6 + 15, 7) and T.shape == (24, 4) and set(bt) == {64, 128, 256}
+        for k, parts in enumerate(trains):  # each partitioning's pair counts add up to the join
+            lo = sum(len(p) * (len(p) + 1) // 2 for p in trains[:k])
+            n = len(parts) * (len(parts) + 1) // 2
+            assert int(counts[lo:lo + n].sum()) == O.self_join(c, t).count
+        model = S.AnalyticModel().fit(F, T, bitmap_times=bt)
+        parts = H.partition_even(c.n, 2500, 4)
+        plan = S.partition_plan(ctx, d, parts, model, t)
+        want = _want(O.self_join(c, t))
+        (r, s, ov), _, _, _ = ctx.run_partition_pairs(d, plan, cpu_threads=3)
+        assert _rows(r, s, ov) == want
+
+        def wall(p):
+            runs = []
+            for _ in range(3):
+                n, _, _, w = ctx.run_partition_pairs(d, p, cpu_threads=3, count_only=True)
+                assert n == len(want)
+                runs.append(w)
+            return float(np.median(runs))
+
+        fixed = {k: wall([(a, b, k) for a, b, _ in plan]) for k in range(4)}
+        got = wall(plan)
+        assert got <= 1.1 * min(fixed.values()) + 1.0, (got, fixed, [k for _, _, k in plan])
+    finally:
+        d.free()
