@@ -711,6 +711,9 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     a.heavy_cap = ctx->tok_heavy_cap;
     ctx->tok_aoff.ensure(std::max<uint64_t>(P.index_entries, 1) * sizeof(uint2));
     a.aoff = ctx->tok_aoff.as<uint2>();
+    ctx->tok_hdrs.ensure(std::max<uint64_t>((P.index_entries + hs::TOK_BLOCK - 1) / hs::TOK_BLOCK, 1) *
+                         sizeof(uint2));
+    a.hdrs = ctx->tok_hdrs.as<uint2>();
     if (ctx->stage_times && !ctx->capturing_join) {
       // stage-timed joins also count |r| over the distinct probes with candidates
       if (ctx->tok_stamp_n < probe->n || ctx->tok_epoch == 0xFFFFFFFFu) {

@@ -188,6 +188,7 @@ struct hs_ctx {
   DevBuf tok_queue;          // token-major join: survivors of the filters, verified next
   uint32_t tok_queue_cap = 0;
   DevBuf tok_heavy, tok_aoff;  // token-major join: chunks of heavy blocks and the blocks' layouts
+  DevBuf tok_hdrs;             // token-major join: each tile's header (first key and its list start)
   uint32_t tok_heavy_cap = 0;
   DevBuf tok_stamp;            // token-major join with stage times: probes already counted
   uint32_t tok_epoch = 0;

@@ -965,6 +965,7 @@ constexpr int TK_TPB = 512;
 constexpr uint32_t TK_EPL = TOK_BLOCK / TK_TPB;  // entries per lane
 constexpr uint32_t TK_LUT = 512;  // set sizes whose window start and minoverlap sit in LDS
 constexpr uint32_t TK_QCAP = 512;  // survivors held in LDS before one flush to the global queue
+constexpr uint32_t TK_HDR = 64;    // tile headers a filter workgroup computes at a time
 constexpr uint32_t TK_NONE = 0xFFFFFFFFu;
 static_assert(TOK_BLOCK % TK_TPB == 0 && TOK_WINDOW >= TOK_BLOCK, "tile geometry");
 
@@ -1012,6 +1013,7 @@ struct TokLds {
   uint32_t qn, qbase, ws, wn, trunc;
   uint32_t next_ws;  // the window start of the workgroup's next tile
   uint32_t stats[3];
+  uint2 hdr[TK_HDR];  // headers of the workgroup's tiles k0 .. k0 + TK_HDR - 1 (its k-th tile: g + k * grid)
 };
 }  // namespace
 
@@ -1359,6 +1361,21 @@ __device__ __forceinline__ uint32_t tok_lb_ids(const TokLds& L, uint32_t lo, uin
 // first token when its list starts before the staged window).  A scan
 // numbers the tile's visits, the workgroup tests the first TOK_CHUNK of
 // them, and a heavier tile queues its other chunks for k_tok_heavy.
+// Headers of the workgroup's tiles k0 .. k0 + TK_HDR - 1 (tile g + k * grid),
+// one lane each, into L.hdr and A.hdrs (for k_tok_heavy); then a barrier.
+// Their searches run side by side instead of one per tile on the tile's
+// critical path.
+__device__ __forceinline__ void tok_headers(const TokArgs& A, TokLds& L, uint32_t k0, uint32_t n_tiles) {
+  const uint32_t k = k0 + threadIdx.x;
+  const uint32_t t = blockIdx.x + k * gridDim.x;
+  if (threadIdx.x < TK_HDR && t < n_tiles) {
+    const uint2 h = tok_header(A, t);
+    L.hdr[threadIdx.x] = h;
+    A.hdrs[t] = h;
+  }
+  __syncthreads();
+}
+
 template <int kW>
 __global__ __launch_bounds__(TK_TPB) void k_tok_filter(TokArgs A) {
   __shared__ TokLds L;
@@ -1368,29 +1385,23 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_filter(TokArgs A) {
   TokCounts C;
   const uint32_t n_tiles = (A.n_ent + TOK_BLOCK - 1) / TOK_BLOCK;
   // The next tile is loaded while the current one runs: its words right
-  // after the current staging, its header by thread 0, and the signatures
-  // of its window once the header's window start is out (after the scan).
-  uint2 hdr = make_uint2(0, 0);
+  // after the current staging, and the signatures of its window once its
+  // window start is out (after the scan).  Headers come in batches of TK_HDR.
   TokAhead P;
-  if (blockIdx.x < n_tiles) {
-    tok_ahead_words(A, blockIdx.x, P);
-    if (tid == 0) {
-      hdr = tok_header(A, blockIdx.x);
-      L.next_ws = tok_ws(A, blockIdx.x, hdr);
-    }
-  }
+  if (blockIdx.x < n_tiles) tok_ahead_words(A, blockIdx.x, P);
+  tok_headers(A, L, 0, n_tiles);
+  if (tid == 0 && blockIdx.x < n_tiles) L.next_ws = tok_ws(A, blockIdx.x, L.hdr[0]);
   __syncthreads();
   if (blockIdx.x < n_tiles) tok_ahead_sigs<kW>(A, blockIdx.x, L.next_ws, P);
-  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+  uint32_t k = 0;  // the workgroup's tile count so far
+  for (uint32_t t = blockIdx.x; t < n_tiles; t += gridDim.x, ++k) {
     __syncthreads();  // the previous tile's readers are done with L
-    tok_stage_ahead<kW>(A, L, t, hdr, P);
+    tok_stage_ahead<kW>(A, L, t, L.hdr[k % TK_HDR], P);
     const uint32_t tn = t + gridDim.x;
     if (tn < n_tiles) {
+      if ((k + 1) % TK_HDR == 0) tok_headers(A, L, k + 1, n_tiles);  // (block-uniform; L.hdr no longer read)
       tok_ahead_words(A, tn, P);
-      if (tid == 0) {
-        hdr = tok_header(A, tn);
-        L.next_ws = tok_ws(A, tn, hdr);
-      }
+      if (tid == 0) L.next_ws = tok_ws(A, tn, L.hdr[(k + 1) % TK_HDR]);
     }
     const uint32_t t0 = t * TOK_BLOCK, t1 = min(t0 + TOK_BLOCK, A.n_ent);
     const uint32_t ws = L.ws;
@@ -1488,7 +1499,7 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_heavy(TokArgs A) {
       }
       if (tid == 0) L.off[TOK_BLOCK] = it.z;
       uint2 hdr = make_uint2(0, 0);
-      if (tid == 0) hdr = tok_header(A, it.x);
+      if (tid == 0) hdr = A.hdrs[it.x];  // written by k_tok_filter
       tok_stage<kW>(A, L, it.x, hdr);
       staged = it.x;
     }

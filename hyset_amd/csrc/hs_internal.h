@@ -386,6 +386,7 @@ struct TokArgs {
   uint4* heavy;
   uint32_t heavy_cap;
   uint2* aoff;  // n_ent entries
+  uint2* hdrs;  // one per tile of TOK_BLOCK entries: {first key, first entry of its index part}
   // optional: stamp[r] == epoch marks a probe already counted in
   // CNT_CAND_PROBES (NULL: not counted)
   uint32_t* stamp;
