@@ -114,22 +114,45 @@ __device__ __forceinline__ bool verify_merge(const uint32_t* r, uint32_t lr, con
 }
 
 constexpr int BM_TPB = 256;
-constexpr uint32_t BM_CHUNK = 1024;
 
+// The indexed signatures through the constant address space: every lane
+// of a wave reads the same words, so they arrive by scalar loads into SGPRs
+// (the kernel never writes them).
+typedef const __attribute__((address_space(4))) uint64_t ConstU64;
+
+// XOR-popcount of words [K0, K1) of the lane's signature against a
+// wave-uniform one, added to pop: one popcount-with-accumulate per 32-bit
+// half, chained.
+template <int K0, int K1, int W>
+__device__ __forceinline__ uint32_t bm_pop(const uint64_t (&sig)[W], const uint64_t (&ss)[W], uint32_t pop = 0) {
+#pragma unroll
+  for (int k = K0; k < K1; ++k) {
+    const uint64_t x = sig[k] ^ ss[k];
+    pop = (uint32_t)__popc((uint32_t)x) + pop;
+    pop = (uint32_t)__popc((uint32_t)(x >> 32)) + pop;
+  }
+  return pop;
+}
+
+// Tests per loop round (the round's signatures sit in SGPRs).
+constexpr uint32_t BM_UNROLL = 8;
+
+// One wave per 64 consecutive probes (sets are sorted by size, so their
+// length windows nearly coincide): the wave walks the union of its lanes'
+// windows once, in segments over which the indexed sets' size and the set of
+// lanes whose window holds the id are both constant, so a pair test is the
+// XOR-popcount of the lane's signature against a wave-uniform one and one
+// compare with a per-lane slack fixed for the segment (-1 for lanes outside
+// their window).  Survivors (rare) are merged by their lanes.
 template <int W>
 __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles) {
-  __shared__ uint64_t s_sig[BM_CHUNK * W];
-  __shared__ uint32_t s_size[BM_CHUNK];
-  __shared__ uint32_t s_red[2 * (BM_TPB / WAVE)];
   __shared__ unsigned long long s_stats[2];
-
   const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
   if (tile == 0xFFFFFFFFu) return;
   const uint32_t tid = threadIdx.x;
+  if (tid < 2) s_stats[tid] = 0;
   const uint32_t p = A.probe_first + tile * BM_TPB + tid;
   const bool valid = p <= A.probe_last;
-  if (tid < 2) s_stats[tid] = 0;
-
   uint64_t sig[W];
   uint32_t lr = 0, lo = 0xFFFFFFFFu, hi = 0;
   if (valid) {
@@ -150,59 +173,83 @@ __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles
 #pragma unroll
     for (int k = 0; k < W; ++k) sig[k] = 0;
   }
-  // union window of the tile
-  uint32_t wlo = wave_min(lo), whi = wave_max(hi);
-  if (lane_id() == 0) {
-    s_red[tid / WAVE] = wlo;
-    s_red[BM_TPB / WAVE + tid / WAVE] = whi;
-  }
-  __syncthreads();
-  uint32_t ulo = 0xFFFFFFFFu, uhi = 0;
-#pragma unroll
-  for (int w = 0; w < BM_TPB / WAVE; ++w) {
-    ulo = min(ulo, s_red[w]);
-    uhi = max(uhi, s_red[BM_TPB / WAVE + w]);
-  }
-
+  __syncthreads();  // s_stats zeroed
   const uint32_t* rtok = valid ? A.probe.tokens + A.probe.starts[p] : nullptr;
-  uint32_t cur_ls = 0xFFFFFFFFu;
-  int32_t slack = -1;  // |r| + |s| - 2*minoverlap for the current |s|
-  uint32_t mo_cur = 0;
+  const uint32_t wlo = __builtin_amdgcn_readfirstlane(wave_min(lo));
+  const uint32_t whi = __builtin_amdgcn_readfirstlane(wave_max(hi));
+  const uint32_t sf_last = A.size_first_len - 1;
   uint32_t survivors = 0, tests = 0;
-
-  for (uint32_t c0 = ulo; c0 < uhi; c0 += BM_CHUNK) {
-    const uint32_t cn = min(BM_CHUNK, uhi - c0);
-    __syncthreads();
-    for (uint32_t i = tid; i < cn * W; i += BM_TPB) s_sig[i] = A.indexed_sigs[(size_t)c0 * W + i];
-    for (uint32_t i = tid; i < cn; i += BM_TPB) s_size[i] = A.indexed.sizes[c0 + i];
-    __syncthreads();
-    const uint32_t b = max(lo, c0), e = min(hi, c0 + cn);
-    tests += e > b ? e - b : 0u;
-    for (uint32_t sidx = b; sidx < e; ++sidx) {
-      const uint32_t q = sidx - c0;
-      const uint32_t ls = s_size[q];
-      if (ls != cur_ls) {
-        cur_ls = ls;
-        mo_cur = dev_minoverlap(A.bounds, lr, ls);
-        slack = (int32_t)(lr + ls) - 2 * (int32_t)mo_cur;
-      }
-      uint32_t pop = 0;
-#pragma unroll
-      for (int k = 0; k < W; ++k) pop += __popcll(sig[k] ^ s_sig[q * W + k]);
-      if ((int32_t)pop <= slack) {
-        ++survivors;
-        uint32_t ov = 0;
-        const uint32_t* s = A.indexed.tokens + A.indexed.starts[sidx];
-        if (verify_merge(rtok, lr, s, ls, mo_cur, &ov)) {
-          // single-lane append (survivors are rare and lanes diverge here)
-          unsigned long long idx = atomicAdd(&A.out.counters[CNT_PAIRS], 1ull);
-          if (idx < A.out.capacity) {
-            A.out.r[idx] = p;
-            A.out.s[idx] = sidx;
-            A.out.ov[idx] = ov;
-          }
+  for (uint32_t sidx = wlo; sidx < whi;) {
+    // the size class of sidx and where it ends
+    const uint32_t ls = __builtin_amdgcn_readfirstlane(A.indexed.sizes[sidx]);
+    const uint32_t cend = min(whi, A.size_first[min(ls + 1, sf_last)]);
+    // the next id at which some lane's window opens or closes
+    const uint32_t nb = lo > sidx ? lo : hi > sidx ? hi : 0xFFFFFFFFu;
+    const uint32_t send = min(cend, __builtin_amdgcn_readfirstlane(wave_min(nb)));
+    const bool in = lo <= sidx && sidx < hi;  // constant over [sidx, send)
+    int32_t slack = -1;
+    uint32_t mo = 0;
+    if (in) {
+      mo = dev_minoverlap(A.bounds, lr, ls);
+      slack = (int32_t)(lr + ls) - 2 * (int32_t)mo;
+      tests += send - sidx;
+    }
+    ConstU64* cs = (ConstU64*)(A.indexed_sigs) + (size_t)sidx * W;
+    // a survivor of id q: its lane merges the two sets
+    auto survive = [&](uint32_t q) {
+      ++survivors;
+      uint32_t ov = 0;
+      const uint32_t* s = A.indexed.tokens + A.indexed.starts[q];
+      if (verify_merge(rtok, lr, s, ls, mo, &ov)) {
+        unsigned long long idx = atomicAdd(&A.out.counters[CNT_PAIRS], 1ull);
+        if (idx < A.out.capacity) {
+          A.out.r[idx] = p;
+          A.out.s[idx] = q;
+          A.out.ov[idx] = ov;
         }
       }
+    };
+    // a round tests the first H words of BM_UNROLL signatures (a lower
+    // bound of the whole popcount, so a valid filter), and only a round
+    // where some lane passes that reads and tests the rest
+    constexpr int H = W > 1 ? W / 2 : 1;
+    for (; sidx + BM_UNROLL <= send; sidx += BM_UNROLL, cs += BM_UNROLL * W) {
+      uint64_t t[BM_UNROLL][W];
+#pragma unroll
+      for (uint32_t u = 0; u < BM_UNROLL; ++u)
+#pragma unroll
+        for (int k = 0; k < H; ++k) t[u][k] = cs[u * W + k];
+      uint64_t any = 0;  // lanes with a passing test among the round's
+      uint32_t part[BM_UNROLL];
+#pragma unroll
+      for (uint32_t u = 0; u < BM_UNROLL; ++u) {
+        part[u] = bm_pop<0, H>(sig, t[u]);
+        any |= __ballot((int32_t)part[u] <= slack);
+      }
+      if (any == 0) continue;  // wave-uniform; survivors are rare
+      if (H < W) {  // the rest of the round's signatures, straight-line
+#pragma unroll
+        for (uint32_t u = 0; u < BM_UNROLL; ++u)
+#pragma unroll
+          for (int k = H; k < W; ++k) t[u][k] = cs[u * W + k];
+        any = 0;
+#pragma unroll
+        for (uint32_t u = 0; u < BM_UNROLL; ++u) {
+          part[u] = bm_pop<H, W>(sig, t[u], part[u]);
+          any |= __ballot((int32_t)part[u] <= slack);
+        }
+        if (any == 0) continue;
+      }
+      for (uint32_t u = 0; u < BM_UNROLL; ++u)
+        if ((int32_t)part[u] <= slack) survive(sidx + u);
+    }
+    for (; sidx < send; ++sidx, cs += W) {
+      uint64_t t[W];
+#pragma unroll
+      for (int k = 0; k < W; ++k) t[k] = cs[k];
+      const bool pass = (int32_t)bm_pop<0, W>(sig, t) <= slack;
+      if (__ballot(pass) == 0) continue;
+      if (pass) survive(sidx);
     }
   }
   survivors = wave_sum(survivors);
