@@ -855,7 +855,8 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     hs::launch_fgss(st, a, ppt, wgs, timed ? ctx->ev[5] : nullptr, timed ? ctx->ev[2] : nullptr);
   } else if (algo == HS_ALGO_BITMAP) {
     const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
-    HS_REQUIRE(bits == 64 || bits == 128 || bits == 256, HS_E_INVALID, "bitmap width must be 64, 128 or 256");
+    HS_REQUIRE(bits == 64 || bits == 128 || bits == 256 || bits == 512 || bits == 1024, HS_E_INVALID,
+               "bitmap width must be 64, 128, 256, 512 or 1024");
     const uint32_t words = bits / 64;
     if (P.bitmap_words != words) {  // once per join, whatever the number of launches
       hs::compute_signatures(st, indexed, words);
@@ -877,6 +878,9 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     b.idx_last = ir.last;
     b.self = self;
     b.out = dev_out(ctx);
+    // about 16 waves per SIMD: fewer probe tiles share their windows out
+    const uint64_t waves = ((uint64_t)pr.last - pr.first + 1 + 63) / 64;
+    b.parts = (uint32_t)std::clamp<uint64_t>((uint64_t)ctx->num_cus * 64 / std::max<uint64_t>(waves, 1), 1, 64);
     if (timed) HS_CHECK_HIP(hipEventRecord(ctx->ev[5], st));
     hs::launch_bitmap(st, b);
     if (timed) HS_CHECK_HIP(hipEventRecord(ctx->ev[2], st));

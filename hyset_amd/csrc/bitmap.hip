@@ -134,8 +134,13 @@ __device__ __forceinline__ uint32_t bm_pop(const uint64_t (&sig)[W], const uint6
   return pop;
 }
 
-// Tests per loop round (the round's signatures sit in SGPRs).
-constexpr uint32_t BM_UNROLL = 8;
+// Tests per loop round: the first halves of the round's signatures sit in
+// SGPRs, at most 32 of them (8 tests of up to 256 bits, 4 of 512, 2 of 1024).
+template <int W>
+constexpr uint32_t bm_unroll() {
+  constexpr int H = W > 1 ? W / 2 : 1;
+  return H >= 8 ? 2u : H >= 4 ? 4u : 8u;
+}
 
 // One wave per 64 consecutive probes (sets are sorted by size, so their
 // length windows nearly coincide): the wave walks the union of its lanes'
@@ -143,12 +148,16 @@ constexpr uint32_t BM_UNROLL = 8;
 // lanes whose window holds the id are both constant, so a pair test is the
 // XOR-popcount of the lane's signature against a wave-uniform one and one
 // compare with a per-lane slack fixed for the segment (-1 for lanes outside
-// their window).  Survivors (rare) are merged by their lanes.
+// their window).  Survivors (rare) are merged by their lanes.  With few
+// probes, A.parts waves share one group of 64 probes, each walking an equal
+// share of the union window, so the device still holds enough waves.
 template <int W>
 __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles) {
   __shared__ unsigned long long s_stats[2];
-  const uint32_t tile = xcd_tile(blockIdx.x, gridDim.x, ntiles);
-  if (tile == 0xFFFFFFFFu) return;
+  const uint32_t parts = A.parts ? A.parts : 1u;
+  const uint32_t job = xcd_tile(blockIdx.x, gridDim.x, ntiles * parts);
+  if (job == 0xFFFFFFFFu) return;
+  const uint32_t tile = job / parts, part = job % parts;
   const uint32_t tid = threadIdx.x;
   if (tid < 2) s_stats[tid] = 0;
   const uint32_t p = A.probe_first + tile * BM_TPB + tid;
@@ -175,8 +184,12 @@ __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles
   }
   __syncthreads();  // s_stats zeroed
   const uint32_t* rtok = valid ? A.probe.tokens + A.probe.starts[p] : nullptr;
-  const uint32_t wlo = __builtin_amdgcn_readfirstlane(wave_min(lo));
-  const uint32_t whi = __builtin_amdgcn_readfirstlane(wave_max(hi));
+  // this wave's share of its lanes' union window
+  const uint32_t ulo = __builtin_amdgcn_readfirstlane(wave_min(lo));
+  const uint32_t uhi = __builtin_amdgcn_readfirstlane(wave_max(hi));
+  const uint32_t span = uhi > ulo ? uhi - ulo : 0u;
+  const uint32_t wlo = ulo + (uint32_t)((uint64_t)span * part / parts);
+  const uint32_t whi = ulo + (uint32_t)((uint64_t)span * (part + 1) / parts);
   const uint32_t sf_last = A.size_first_len - 1;
   uint32_t survivors = 0, tests = 0;
   for (uint32_t sidx = wlo; sidx < whi;) {
@@ -213,6 +226,7 @@ __global__ __launch_bounds__(BM_TPB) void k_bitmap(BitmapArgs A, uint32_t ntiles
     // bound of the whole popcount, so a valid filter), and only a round
     // where some lane passes that reads and tests the rest
     constexpr int H = W > 1 ? W / 2 : 1;
+    constexpr uint32_t BM_UNROLL = bm_unroll<W>();
     for (; sidx + BM_UNROLL <= send; sidx += BM_UNROLL, cs += BM_UNROLL * W) {
       uint64_t t[BM_UNROLL][W];
 #pragma unroll
@@ -270,12 +284,16 @@ void launch_bitmap(hipStream_t st, const BitmapArgs& a) {
   if (a.probe.n == 0 || a.probe_first > a.probe_last) return;
   const uint32_t nprobes = a.probe_last - a.probe_first + 1;
   const uint32_t ntiles = (nprobes + BM_TPB - 1) / BM_TPB;
-  const uint32_t grid = (ntiles + 7) / 8 * 8;
+  const uint64_t jobs = (uint64_t)ntiles * (a.parts ? a.parts : 1u);
+  HS_REQUIRE(jobs < 0x7FFFFFF0ull, HS_E_UNSUPPORTED, "bitmap launch too large");
+  const uint32_t grid = (uint32_t)((jobs + 7) / 8 * 8);
   switch (a.words) {
     case 1: hipLaunchKernelGGL(k_bitmap<1>, dim3(grid), dim3(BM_TPB), 0, st, a, ntiles); break;
     case 2: hipLaunchKernelGGL(k_bitmap<2>, dim3(grid), dim3(BM_TPB), 0, st, a, ntiles); break;
     case 4: hipLaunchKernelGGL(k_bitmap<4>, dim3(grid), dim3(BM_TPB), 0, st, a, ntiles); break;
-    default: throw Error{HS_E_INVALID, "bitmap width must be 64, 128 or 256 bits"};
+    case 8: hipLaunchKernelGGL(k_bitmap<8>, dim3(grid), dim3(BM_TPB), 0, st, a, ntiles); break;
+    case 16: hipLaunchKernelGGL(k_bitmap<16>, dim3(grid), dim3(BM_TPB), 0, st, a, ntiles); break;
+    default: throw Error{HS_E_INVALID, "bitmap width must be 64, 128, 256, 512 or 1024 bits"};
   }
   HS_CHECK_HIP(hipGetLastError());
 }

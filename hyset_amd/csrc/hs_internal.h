@@ -407,11 +407,12 @@ struct BitmapArgs {
   const uint64_t* indexed_sigs;
   const uint32_t* size_first;
   uint32_t size_first_len;
-  uint32_t words;  // 1, 2 or 4 (64, 128, 256 bits)
+  uint32_t words;  // 1, 2, 4, 8 or 16 (64 to 1024 bits)
   uint32_t probe_first, probe_last;
   uint32_t idx_first, idx_last;
   int self;
   DevOut out;
+  uint32_t parts;  // waves per 64 probes, each taking a share of their window (0: 1)
 };
 // words: 1, 2, 4, 8 or 16 (64 ... 1024 bits)
 void launch_signatures(hipStream_t st, const DevColl& c, uint32_t words, uint64_t* sigs);

@@ -179,7 +179,7 @@ typedef struct {
 
 /* Tuning knobs; zero selects the default. */
 typedef struct {
-  uint32_t bitmap_bits;       /* bitmap: signature width 64, 128 or 256 (default 64) */
+  uint32_t bitmap_bits;       /* bitmap: signature width 64, 128, 256, 512 or 1024 (default 64) */
   uint32_t probes_per_tile;   /* fgss: fixed tiles of this many probes, 1..512 (default 0: the
                                  collection's tile plan); batched fgss: probes per tile (default 512) */
   uint32_t verify_scenario;   /* hybrid: 1 lane per candidate, 2 wave merge-path per

@@ -55,10 +55,11 @@ def full(ctx):
     dS.free()
 
 
-def test_long_rs_fgss_matches_bitmap(ctx, full):
+@pytest.mark.parametrize("bits", [64, 1024])
+def test_long_rs_fgss_matches_bitmap(ctx, full, bits):
     R, S, dR, dS, got = full
     ctx.set_threshold(T, "cosine")
-    other = H.sort_pairs(*ctx.join(dS, dR, algo="bitmap", tune=H.Tune(bitmap_bits=64)))
+    other = H.sort_pairs(*ctx.join(dS, dR, algo="bitmap", tune=H.Tune(bitmap_bits=bits)))
     for a, b in zip(got, other):
         assert np.array_equal(a, b)
     # the planted near-duplicates are found (about 20% of S)

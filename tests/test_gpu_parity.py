@@ -132,7 +132,7 @@ def test_fgss_signature_prunes_candidates(ctx):
         d.free()
 
 
-@pytest.mark.parametrize("bits", [64, 128, 256])
+@pytest.mark.parametrize("bits", [64, 128, 256, 512, 1024])
 @pytest.mark.parametrize("t", [0.6, 0.8, 0.9])
 def test_bitmap_small(ctx, small, t, bits):
     c, d = small
@@ -281,6 +281,8 @@ def test_rs_long_sets_cosine(ctx):
         # ~58 index entries per token: balanced tiles in 256-lane workgroups
         assert ctx.stats()["tile_plan"] == H.PLAN_BALANCED
         _check(ctx.join(dS, dR, algo="fgss", tune=H.Tune(flags=H.TUNE_TILE_QUEUE)), want, "fgss long cosine queue")
+        for bits in (512, 1024):
+            _check(ctx.join(dS, dR, algo="bitmap", tune=H.Tune(bitmap_bits=bits)), want, f"bitmap{bits} long cosine")
     finally:
         dR.free()
         dS.free()
@@ -350,7 +352,8 @@ def test_sets_longer_than_16_bit_fields(ctx):
             want = O.self_join(c, t)
             assert int((np.asarray(want.overlap) > 65_535).sum()) > 0
             _check(ctx.join(d, algo="fgss"), want, f"long sets fgss t={t}")
-            _check(ctx.join(d, algo="bitmap", tune=H.Tune(bitmap_bits=256)), want, f"long sets bitmap t={t}")
+            for bits in (256, 1024):
+                _check(ctx.join(d, algo="bitmap", tune=H.Tune(bitmap_bits=bits)), want, f"long sets bitmap{bits} t={t}")
     finally:
         d.free()
 
