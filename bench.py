@@ -201,7 +201,8 @@ def bench_partition_pairs(ctx, H, d, coll, t, want, threads) -> dict:
 def bench_long_rs(ctx, H) -> dict:
     """The long-set R-S workload: R = S = 200,000 sets of 100-700 tokens over
     a 50,000-token Zipf(0.8) universe, 20% of S edited copies of R sets,
-    cosine 0.8, fgss (index over S, probes R)."""
+    cosine 0.8: fgss (index over S, probes R) and the bitmap technique with
+    1024-bit signatures."""
     from hyset_amd import workloads as W
     a = time.perf_counter()
     R, S = W.long_rs()
@@ -224,6 +225,18 @@ def bench_long_rs(ctx, H) -> dict:
     assert ctx.join(dS, dR, algo="fgss", tune=H.Tune(flags=H.TUNE_NO_SIGNATURE), count_only=True) == c
     res["long_rs_fgss_no_signature_ms"] = round(ctx.stats()["total_ms"], 1)
     log(f"long R-S without the signature test: {res['long_rs_fgss_no_signature_ms']} ms")
+    # the bitmap technique with 1024-bit signatures: every pair of the length
+    # windows tested, the first 512 bits per round, survivors merged
+    bt = H.Tune(bitmap_bits=1024, flags=H.TUNE_STAGE_TIMES)
+    ms = []
+    for _ in range(3):
+        assert ctx.join(dS, dR, algo="bitmap", tune=bt, count_only=True) == c
+        ms.append(ctx.stats()["total_ms"])
+    st = ctx.stats()
+    res.update({"long_rs_bitmap1024_ms": round(min(ms), 2), "long_rs_bitmap1024_pair_tests": int(st["posting_visits"]),
+                "long_rs_bitmap1024_survivors": int(st["candidates"])})
+    log(f"long R-S bitmap 1024 bits: {res['long_rs_bitmap1024_ms']} ms, {st['posting_visits']} pair tests, "
+        f"{st['candidates']} survivors")
     res.update(bench_coprocess(ctx, dS, dR, "long_rs", 0, c))
     dR.free()
     dS.free()
