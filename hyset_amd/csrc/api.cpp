@@ -854,7 +854,9 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     // kernel_ms: the k_fgss dispatch alone, timed by the dispatch itself
     hs::launch_fgss(st, a, ppt, wgs, timed ? ctx->ev[5] : nullptr, timed ? ctx->ev[2] : nullptr);
   } else if (algo == HS_ALGO_BITMAP) {
-    const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64;
+    // the width the fgss signature test would take: 64 bits for sets of a
+    // few dozen tokens, up to 1024 for sets of hundreds
+    const uint32_t bits = tune.bitmap_bits ? tune.bitmap_bits : 64 * hs::fgss_signature_words(indexed, 0);
     HS_REQUIRE(bits == 64 || bits == 128 || bits == 256 || bits == 512 || bits == 1024, HS_E_INVALID,
                "bitmap width must be 64, 128, 256, 512 or 1024");
     const uint32_t words = bits / 64;

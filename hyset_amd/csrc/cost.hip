@@ -342,6 +342,7 @@ int hs_split_run(hs_ctx* ctx, uint32_t n_units, hs_coll* const* colls, const int
       hs_result res{};
       res.count_only = 1;
       hs_tune tune{};
+      tune.bitmap_bits = 64;  // the width the split's models are measured and fitted at
       const int32_t algo = technique[u] == HS_TECH_FGSS ? HS_ALGO_FGSS
                            : technique[u] == HS_TECH_BITMAP ? HS_ALGO_BITMAP : HS_ALGO_HYBRID;
       auto a = std::chrono::steady_clock::now();

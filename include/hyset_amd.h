@@ -179,7 +179,8 @@ typedef struct {
 
 /* Tuning knobs; zero selects the default. */
 typedef struct {
-  uint32_t bitmap_bits;       /* bitmap: signature width 64, 128, 256, 512 or 1024 (default 64) */
+  uint32_t bitmap_bits;       /* bitmap: signature width 64, 128, 256, 512 or 1024 (default: the smallest
+                                 power of two >= 2 x the mean indexed set size, within 64..1024) */
   uint32_t probes_per_tile;   /* fgss: fixed tiles of this many probes, 1..512 (default 0: the
                                  collection's tile plan); batched fgss: probes per tile (default 512) */
   uint32_t verify_scenario;   /* hybrid: 1 lane per candidate, 2 wave merge-path per
@@ -545,7 +546,7 @@ int hs_mlp_train(hs_mlp* mlp, const float* X, const float* Y, uint32_t n, uint32
 #define HS_TECH_CPU 3
 
 /* Runs a split: every unit (a collection self-join under the context's
- * threshold) with its technique; HS_TECH_CPU units run on cpu_threads host
+ * threshold) with its technique (bitmap at 64 bits); HS_TECH_CPU units run on cpu_threads host
  * threads (which claim the largest units first, by token count) while the
  * device units run one after another on the context's stream.  counts[u] = pairs of unit u; unit_ms[u] (may be NULL) its time;
  * *wall_ms (may be NULL) the time of the whole split. */

@@ -281,7 +281,7 @@ def test_rs_long_sets_cosine(ctx):
         # ~58 index entries per token: balanced tiles in 256-lane workgroups
         assert ctx.stats()["tile_plan"] == H.PLAN_BALANCED
         _check(ctx.join(dS, dR, algo="fgss", tune=H.Tune(flags=H.TUNE_TILE_QUEUE)), want, "fgss long cosine queue")
-        for bits in (512, 1024):
+        for bits in (512, 1024, 0):  # 0: the width from the mean set size (1024 bits here)
             _check(ctx.join(dS, dR, algo="bitmap", tune=H.Tune(bitmap_bits=bits)), want, f"bitmap{bits} long cosine")
     finally:
         dR.free()
