@@ -1523,8 +1523,11 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_verify(TokArgs A) {
   const uint32_t stride = gridDim.x * TK_TPB;
   uint32_t cands = 0, cand_tokens = 0, probe_tokens = 0;
   // wave-uniform trip count: emit_pairs needs every lane of the wave
-  for (uint32_t base = blockIdx.x * TK_TPB; base < total; base += stride) {
-    const uint32_t q = base + threadIdx.x;
+  // wave k of the grid (waves numbered across workgroups first, so a short
+  // queue spreads over every CU) takes candidates 64 k .. 64 k + 63
+  const uint32_t wave0 = (blockIdx.x + (threadIdx.x / WAVE) * gridDim.x) * WAVE;
+  for (uint32_t base = wave0; base < total; base += stride) {
+    const uint32_t q = base + lane_id();
     bool pass = false;
     uint32_t ov = 0, r = 0, s = 0;
     if (q < total) {
