@@ -361,15 +361,22 @@ void plan_balanced_tiles(hipStream_t st, FgssArgs& a, const BalancedPlan& bp);
 // tile's visits -- positional filter, signature test -- from LDS; the
 // survivors {r, s, position of the token in r} go to `queue`, verified by a
 // last kernel.  A tile's first TOK_CHUNK visits are tested by its own
-// workgroup, the others through `heavy` ({tile, chunk, visits of the tile})
-// by a persistent second pass, `aoff` ({run start, first visit} per entry)
-// carrying the tile's layout there.
+// workgroup, the others in items of TOK_HCHUNK through `heavy` ({tile,
+// item >= 1, visits of the tile}) by a persistent second pass, `aoff`
+// ({run start, first visit} per entry) carrying the tile's layout there.
+// Items smaller than the first chunk spread a few heavy tiles over the
+// whole grid (on the 1M-set index at Jaccard 0.8, 240 items of 8192
+// visits kept 272 of 512 workgroups idle, and a tile that passes nearly
+// every visit held its workgroup four times as long as the others; the
+// second pass took 25.6 us with 8192-visit items, 21.9 / 19.9 / 23.0 us
+// with 4096 / 2048 / 1024).
 // 1024-entry tiles: the token filter measured 78 us on the 1M-set index
 // at Jaccard 0.8 (512-entry tiles: 86 us; either with 8192-visit chunks:
 // 93 / 84 us at 512 / 1024 threads)
 constexpr uint32_t TOK_BLOCK = 1024;
 constexpr uint32_t TOK_WINDOW = 2048;
 constexpr uint32_t TOK_CHUNK = 8192;
+constexpr uint32_t TOK_HCHUNK = 2048;
 struct TokArgs {
   DevColl c;
   const uint64_t* ent;  // the packed index, sorted by (key, set id)
