@@ -1185,31 +1185,36 @@ __device__ __forceinline__ uint32_t tok_lb_back(const TokArgs& A, uint32_t hi, u
   return lo;
 }
 
+
 // The staged window of tile t: from the start of its first token's list
 // (its index part) to the tile's end, at most TOK_WINDOW entries; every
 // other token of the tile lies inside it whole.  hdr = {first key of the
 // tile, first entry of that key's index part}.
-__device__ __forceinline__ uint2 tok_header(const TokArgs& A, uint32_t t) {
+// Found by a whole wave (every lane calls): 64 probes at i, i - 1, i - 3,
+// ..., i - (2^l - 1) bracket the first position with key >= k0, then each
+// round 64 evenly spaced probes narrow the bracket 64-fold -- a handful of
+// memory latencies where one lane's gallop and bisection take two per
+// halving (a tile inside a list of 40K entries: ~32 dependent loads).
+__device__ __forceinline__ uint2 tok_header_wave(const TokArgs& A, uint32_t t) {
+  const uint32_t lane = lane_id();
   const uint32_t i = t * TOK_BLOCK;
   const uint32_t k0 = pk_key(A, A.ent[i]) & ~1u;
-  // first position with key >= k0: gallop back from i, then bisect
-  uint32_t hi = i, step = 1, lo;
-  for (;;) {
-    const uint32_t j = hi >= step ? hi - step : 0u;
-    if (pk_key(A, A.ent[j]) < k0) {
-      lo = j + 1;
-      break;
-    }
-    hi = j;
-    if (j == 0) return make_uint2(k0, 0u);
-    step <<= 1;
+  const uint32_t d = lane < 32 ? (1u << lane) - 1u : 0xFFFFFFFFu;
+  const uint32_t pos = d >= i ? 0u : i - d;
+  const uint64_t m = __ballot(pk_key(A, A.ent[pos]) < k0);  // a suffix of the lanes (positions descend)
+  if (m == 0) return make_uint2(k0, 0u);
+  // key(lo) < k0 <= key(hi); lane 0 probed i itself, whose key is >= k0
+  const uint32_t l = (uint32_t)__ffsll((unsigned long long)m) - 1u;
+  uint32_t lo = __builtin_amdgcn_readlane(pos, l), hi = __builtin_amdgcn_readlane(pos, l - 1u);
+  while (hi - lo > 1) {
+    const uint32_t step = (hi - lo + WAVE - 1) / WAVE;
+    const uint32_t p = min(lo + (lane + 1) * step, hi);  // lane 63's probe is hi
+    const uint32_t nb = (uint32_t)__popcll(__ballot(pk_key(A, A.ent[p]) < k0));  // a prefix of the lanes
+    const uint32_t lo0 = lo;
+    lo = lo0 + nb * step;  // the last probe below k0 (lo itself when none)
+    hi = min(lo0 + (nb + 1) * step, hi);
   }
-  while (lo < hi) {
-    const uint32_t m = (lo + hi) >> 1;
-    if (pk_key(A, A.ent[m]) < k0) lo = m + 1;
-    else hi = m;
-  }
-  return make_uint2(k0, lo);
+  return make_uint2(k0, hi);
 }
 
 // Tile t's window: from its first token's index part (hdr.y) to the tile's
@@ -1383,11 +1388,14 @@ __device__ __forceinline__ uint32_t tok_tile_of(uint32_t b, uint32_t k, uint32_t
 // Their searches run side by side instead of one per tile on the tile's
 // critical path.
 __device__ __forceinline__ void tok_headers(const TokArgs& A, TokLds& L, uint32_t k0, uint32_t n_tiles) {
-  const uint32_t t = tok_tile_of(blockIdx.x, k0 + threadIdx.x, gridDim.x, n_tiles);
-  if (threadIdx.x < TK_HDR && t != TK_NONE) {
-    const uint2 h = tok_header(A, t);
-    L.hdr[threadIdx.x] = h;
-    A.hdrs[t] = h;
+  for (uint32_t q = threadIdx.x / WAVE; q < TK_HDR; q += TK_TPB / WAVE) {
+    const uint32_t t = tok_tile_of(blockIdx.x, k0 + q, gridDim.x, n_tiles);
+    if (t == TK_NONE) break;  // (wave-uniform; the later turns have no tile either)
+    const uint2 h = tok_header_wave(A, t);
+    if (lane_id() == 0) {
+      L.hdr[q] = h;
+      A.hdrs[t] = h;
+    }
   }
   __syncthreads();
 }
