@@ -1075,61 +1075,95 @@ __device__ __forceinline__ void tok_entry(const TokArgs& A, const TokLds& L, uin
 // Visits [p0, p1) of tile t (entries t TOK_BLOCK + k, their runs L.a and
 // first visits L.off): each lane takes a contiguous range of them, finds
 // its first entry once and walks on through the runs, reading entries from
-// the staged window.  Survivors go to the LDS queue (to the global queue
-// directly when it is full); call tok_flush_queue between tiles.
+// the staged window.  Survivors go to the LDS queue; those past its room
+// to the global queue, a wave at a time (below); call tok_flush_queue
+// between tiles.
 template <int kW, bool kMixed>
 __device__ __forceinline__ void tok_visits_(const TokArgs& A, TokLds& L, uint32_t t, uint32_t p0, uint32_t p1,
                                             TokCounts& C) {
+  static_assert(TOK_CHUNK <= 32 * TK_TPB, "a lane's visits fit one 32-bit mask");
   if (p1 <= p0) return;
   const bool lut = tok_lut(A);
   const uint32_t per = (p1 - p0 + TK_TPB - 1) / TK_TPB;
-  uint32_t p = p0 + threadIdx.x * per;
-  const uint32_t pe = min(p + per, p1);
-  if (p >= pe) return;
-  uint32_t k = segment_of(L.off, TOK_BLOCK, p);
-  uint32_t r, mr;
-  uint64_t sig_r;
-  tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
-  for (; p < pe; ++p) {
-    if (p >= L.off[k + 1]) {  // the next entry with visits
-      do ++k;
-      while (p >= L.off[k + 1]);
-      tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
-    }
-    const uint32_t j = L.a[k] + (p - L.off[k]);
-    uint32_t s, ms;
-    uint64_t sig_s;
-    tok_entry<kW, kMixed>(A, L, j, s, ms, sig_s);
-    const uint32_t lr = mr >> 16, jr = mr & 0xFFFFu, ls = ms >> 16, ps = ms & 0xFFFFu;
-    ++C.visits;
-    const uint32_t mo = lut ? min((uint32_t)L.mo[lr + ls], min(lr, ls)) : dev_minoverlap(A.bounds, lr, ls);
-    // positional filter (helpers.hpp:46-50) at this token, which only the
-    // first common token of r and s may be
-    bool keep = 1u + min(lr - 1u - jr, ls - 1u - ps) >= mo;
-    if (keep && kW == 1) {
-      ++C.tests;
-      keep = lr + ls >= 2u * mo + (uint32_t)__popcll(sig_r ^ sig_s);
-    } else if (keep && kW > 1) {
-      ++C.tests;
-      const uint64_t* sr = A.sigs + (size_t)r * kW;
-      const uint64_t* ss = A.sigs + (size_t)s * kW;
-      const uint32_t slack = lr + ls - 2u * mo;
-      uint32_t pc = 0;
-      for (int q = 0; q < kW && pc <= slack; q += 2)
-        pc += (uint32_t)__popcll(sr[q] ^ ss[q]) + (uint32_t)__popcll(sr[q + 1] ^ ss[q + 1]);
-      keep = pc <= slack;
-    }
-    if (keep) {  // rare: one LDS atomic per survivor
-      const uint32_t slot = atomicAdd(&L.qn, 1u);
-      if (slot < TK_QCAP) {
-        L.q_r[slot] = r;
-        L.q_s[slot] = s;
-        L.q_j[slot] = jr;
-      } else {
-        const unsigned long long g = atomicAdd(&A.out.counters[CNT_QUEUE], 1ull);
-        if (g < A.queue_cap) A.queue[g] = make_uint4(r, s, jr, 0u);
+  const uint32_t pb = p0 + threadIdx.x * per, pe = min(pb + per, p1);
+  uint32_t spill = 0, k0 = 0;  // bit i: visit pb + i survives past the LDS queue's room; k0: its first entry
+  if (pb < pe) {
+    uint32_t k = k0 = segment_of(L.off, TOK_BLOCK, pb);
+    uint32_t r, mr;
+    uint64_t sig_r;
+    tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
+    for (uint32_t p = pb; p < pe; ++p) {
+      if (p >= L.off[k + 1]) {  // the next entry with visits
+        do ++k;
+        while (p >= L.off[k + 1]);
+        tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
+      }
+      const uint32_t j = L.a[k] + (p - L.off[k]);
+      uint32_t s, ms;
+      uint64_t sig_s;
+      tok_entry<kW, kMixed>(A, L, j, s, ms, sig_s);
+      const uint32_t lr = mr >> 16, jr = mr & 0xFFFFu, ls = ms >> 16, ps = ms & 0xFFFFu;
+      ++C.visits;
+      const uint32_t mo = lut ? min((uint32_t)L.mo[lr + ls], min(lr, ls)) : dev_minoverlap(A.bounds, lr, ls);
+      // positional filter (helpers.hpp:46-50) at this token, which only the
+      // first common token of r and s may be
+      bool keep = 1u + min(lr - 1u - jr, ls - 1u - ps) >= mo;
+      if (keep && kW == 1) {
+        ++C.tests;
+        keep = lr + ls >= 2u * mo + (uint32_t)__popcll(sig_r ^ sig_s);
+      } else if (keep && kW > 1) {
+        ++C.tests;
+        const uint64_t* sr = A.sigs + (size_t)r * kW;
+        const uint64_t* ss = A.sigs + (size_t)s * kW;
+        const uint32_t slack = lr + ls - 2u * mo;
+        uint32_t pc = 0;
+        for (int q = 0; q < kW && pc <= slack; q += 2)
+          pc += (uint32_t)__popcll(sr[q] ^ ss[q]) + (uint32_t)__popcll(sr[q + 1] ^ ss[q + 1]);
+        keep = pc <= slack;
+      }
+      if (keep) {  // rare: one LDS atomic per survivor
+        const uint32_t slot = atomicAdd(&L.qn, 1u);
+        if (slot < TK_QCAP) {
+          L.q_r[slot] = r;
+          L.q_s[slot] = s;
+          L.q_j[slot] = jr;
+        } else {
+          spill |= 1u << (p - pb);
+        }
       }
     }
+  }
+  // Survivors past the LDS queue's room (a dense chunk passes nearly all
+  // of its 8192 visits): the wave reserves global queue slots for all of
+  // its own with one atomic -- an atomic per survivor on the one queue
+  // counter, from every dense tile at once, serialized -- and walks its
+  // ranges again to write them, one step of every lane at a time.
+  if (__ballot(spill != 0) == 0) return;
+  const uint32_t n = wave_sum((uint32_t)__popc(spill));
+  uint32_t g = 0;
+  if (lane_id() == 0)
+    g = (uint32_t)min(atomicAdd(&A.out.counters[CNT_QUEUE], (unsigned long long)n), (unsigned long long)0xFFFFFFFFu);
+  g = __builtin_amdgcn_readfirstlane(g);
+  uint32_t k = k0, r = 0, mr = 0;
+  uint64_t sig_r;
+  if (spill) tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
+  for (uint32_t b = 0; b < per; ++b) {
+    const bool bit = (spill >> b) & 1u;
+    const uint64_t mb = __ballot(bit);
+    if (bit) {
+      const uint32_t p = pb + b;
+      if (p >= L.off[k + 1]) {
+        do ++k;
+        while (p >= L.off[k + 1]);
+        tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
+      }
+      uint32_t s, ms;
+      uint64_t sig_s;
+      tok_entry<kW, kMixed>(A, L, L.a[k] + (p - L.off[k]), s, ms, sig_s);
+      const uint64_t slot = (uint64_t)g + lanes_below(mb);
+      if (slot < A.queue_cap) A.queue[slot] = make_uint4(r, s, mr & 0xFFFFu, 0u);
+    }
+    g += (uint32_t)__popcll(mb);
   }
 }
 
