@@ -966,6 +966,7 @@ constexpr uint32_t TK_EPL = TOK_BLOCK / TK_TPB;  // entries per lane
 constexpr uint32_t TK_LUT = 512;  // set sizes whose window start and minoverlap sit in LDS
 constexpr uint32_t TK_QCAP = 512;  // survivors held in LDS before one flush to the global queue
 constexpr uint32_t TK_HDR = 64;    // tile headers a filter workgroup computes at a time
+constexpr uint32_t TK_HMERGE = 8;  // second-pass items of one tile visited in one pass
 constexpr uint32_t TK_NONE = 0xFFFFFFFFu;
 static_assert(TOK_BLOCK % TK_TPB == 0 && TOK_WINDOW >= TOK_BLOCK, "tile geometry");
 
@@ -1072,6 +1073,35 @@ __device__ __forceinline__ void tok_entry(const TokArgs& A, const TokLds& L, uin
   }
 }
 
+// A visit's test: the probe entry (r, meta mr = |r| << 16 | position,
+// 64-bit signature) against the list entry (s, ms, sig_s).
+template <int kW>
+__device__ __forceinline__ bool tok_keep(const TokArgs& A, const TokLds& L, bool lut, uint32_t r, uint32_t mr,
+                                         uint64_t sig_r, uint32_t s, uint32_t ms, uint64_t sig_s, TokCounts& C) {
+  const uint32_t lr = mr >> 16, jr = mr & 0xFFFFu, ls = ms >> 16, ps = ms & 0xFFFFu;
+  ++C.visits;
+  const uint32_t mo = lut ? min((uint32_t)L.mo[lr + ls], min(lr, ls)) : dev_minoverlap(A.bounds, lr, ls);
+  // positional filter (helpers.hpp:46-50) at this token, which only the
+  // first common token of r and s may be
+  bool keep = 1u + min(lr - 1u - jr, ls - 1u - ps) >= mo;
+  if (keep && kW == 1) {
+    ++C.tests;
+    keep = lr + ls >= 2u * mo + (uint32_t)__popcll(sig_r ^ sig_s);
+  } else if (keep && kW > 1) {
+    ++C.tests;
+    const uint64_t* sr = A.sigs + (size_t)r * kW;
+    const uint64_t* ss = A.sigs + (size_t)s * kW;
+    const uint32_t slack = lr + ls - 2u * mo;
+    uint32_t pc = 0;
+    for (int q = 0; q < kW && pc <= slack; q += 2)
+      pc += (uint32_t)__popcll(sr[q] ^ ss[q]) + (uint32_t)__popcll(sr[q + 1] ^ ss[q + 1]);
+    keep = pc <= slack;
+  }
+  return keep;
+}
+
+constexpr uint32_t TK_G = 4;  // visits a lane loads at once outside the staged window
+
 // Visits [p0, p1) of tile t (entries t TOK_BLOCK + k, their runs L.a and
 // first visits L.off): each lane takes a contiguous range of them, finds
 // its first entry once and walks on through the runs, reading entries from
@@ -1081,55 +1111,77 @@ __device__ __forceinline__ void tok_entry(const TokArgs& A, const TokLds& L, uin
 template <int kW, bool kMixed>
 __device__ __forceinline__ void tok_visits_(const TokArgs& A, TokLds& L, uint32_t t, uint32_t p0, uint32_t p1,
                                             TokCounts& C) {
-  static_assert(TOK_CHUNK <= 32 * TK_TPB, "a lane's visits fit one 32-bit mask");
+  static_assert(TOK_CHUNK <= 32 * TK_TPB && TK_HMERGE * TOK_HCHUNK <= 32 * TK_TPB,
+                "a lane's visits fit one 32-bit mask");
   if (p1 <= p0) return;
   const bool lut = tok_lut(A);
   const uint32_t per = (p1 - p0 + TK_TPB - 1) / TK_TPB;
   const uint32_t pb = p0 + threadIdx.x * per, pe = min(pb + per, p1);
   uint32_t spill = 0, k0 = 0;  // bit i: visit pb + i survives past the LDS queue's room; k0: its first entry
-  if (pb < pe) {
+  // a survivor of visit p: one LDS atomic (rare), or past the queue's room a spill bit
+  auto enqueue = [&](uint32_t p, uint32_t r, uint32_t s, uint32_t jr) {
+    const uint32_t slot = atomicAdd(&L.qn, 1u);
+    if (slot < TK_QCAP) {
+      L.q_r[slot] = r;
+      L.q_s[slot] = s;
+      L.q_j[slot] = jr;
+    } else {
+      spill |= 1u << (p - pb);
+    }
+  };
+  if (pb < pe && !kMixed) {
     uint32_t k = k0 = segment_of(L.off, TOK_BLOCK, pb);
     uint32_t r, mr;
     uint64_t sig_r;
-    tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
+    tok_entry<kW, false>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
     for (uint32_t p = pb; p < pe; ++p) {
       if (p >= L.off[k + 1]) {  // the next entry with visits
         do ++k;
         while (p >= L.off[k + 1]);
-        tok_entry<kW, kMixed>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
+        tok_entry<kW, false>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);
       }
       const uint32_t j = L.a[k] + (p - L.off[k]);
       uint32_t s, ms;
       uint64_t sig_s;
-      tok_entry<kW, kMixed>(A, L, j, s, ms, sig_s);
-      const uint32_t lr = mr >> 16, jr = mr & 0xFFFFu, ls = ms >> 16, ps = ms & 0xFFFFu;
-      ++C.visits;
-      const uint32_t mo = lut ? min((uint32_t)L.mo[lr + ls], min(lr, ls)) : dev_minoverlap(A.bounds, lr, ls);
-      // positional filter (helpers.hpp:46-50) at this token, which only the
-      // first common token of r and s may be
-      bool keep = 1u + min(lr - 1u - jr, ls - 1u - ps) >= mo;
-      if (keep && kW == 1) {
-        ++C.tests;
-        keep = lr + ls >= 2u * mo + (uint32_t)__popcll(sig_r ^ sig_s);
-      } else if (keep && kW > 1) {
-        ++C.tests;
-        const uint64_t* sr = A.sigs + (size_t)r * kW;
-        const uint64_t* ss = A.sigs + (size_t)s * kW;
-        const uint32_t slack = lr + ls - 2u * mo;
-        uint32_t pc = 0;
-        for (int q = 0; q < kW && pc <= slack; q += 2)
-          pc += (uint32_t)__popcll(sr[q] ^ ss[q]) + (uint32_t)__popcll(sr[q + 1] ^ ss[q + 1]);
-        keep = pc <= slack;
+      tok_entry<kW, false>(A, L, j, s, ms, sig_s);
+      if (tok_keep<kW>(A, L, lut, r, mr, sig_r, s, ms, sig_s, C)) enqueue(p, r, s, mr & 0xFFFFu);
+    }
+  } else if (pb < pe) {
+    // Visits of a tile whose window misses the start of its first token's
+    // list: those before the window read the index words and signatures
+    // from global memory.  TK_G visits a lane at a time, with every load of
+    // the group issued unconditionally (inside the window as well, where
+    // the data is the same): two memory latencies a group instead of two a
+    // visit; loads under per-visit branches each waited for the one before.
+    uint32_t k = k0 = segment_of(L.off, TOK_BLOCK, pb), kr = k;
+    uint32_t r, mr;
+    uint64_t sig_r;
+    tok_entry<kW, false>(A, L, t * TOK_BLOCK + k, r, mr, sig_r);  // (a tile's own entries lie in its window)
+    for (uint32_t g0 = pb; g0 < pe; g0 += TK_G) {
+      uint32_t kq[TK_G], jq[TK_G];
+#pragma unroll
+      for (uint32_t q = 0; q < TK_G; ++q) {
+        const uint32_t p = g0 + q;
+        if (p < pe)
+          while (p >= L.off[k + 1]) ++k;
+        kq[q] = k;
+        jq[q] = min(L.a[k] + (p < pe ? p - L.off[k] : 0u), A.n_ent - 1u);
       }
-      if (keep) {  // rare: one LDS atomic per survivor
-        const uint32_t slot = atomicAdd(&L.qn, 1u);
-        if (slot < TK_QCAP) {
-          L.q_r[slot] = r;
-          L.q_s[slot] = s;
-          L.q_j[slot] = jr;
-        } else {
-          spill |= 1u << (p - pb);
+      uint64_t w[TK_G], sg[TK_G];
+#pragma unroll
+      for (uint32_t q = 0; q < TK_G; ++q) w[q] = A.ent[jq[q]];
+#pragma unroll
+      for (uint32_t q = 0; q < TK_G; ++q) sg[q] = kW == 1 ? A.sigs[pk_id(A, w[q])] : 0ull;
+#pragma unroll
+      for (uint32_t q = 0; q < TK_G; ++q) {
+        const uint32_t p = g0 + q;
+        if (p >= pe) break;
+        if (kq[q] != kr) {
+          kr = kq[q];
+          tok_entry<kW, false>(A, L, t * TOK_BLOCK + kr, r, mr, sig_r);
         }
+        const uint32_t s = pk_id(A, w[q]), ms = pk_meta(A, w[q]);
+        if (tok_keep<kW>(A, L, lut, r, mr, sig_r, s, ms, sg[q], C)) enqueue(p, r, s, mr & 0xFFFFu);
       }
     }
   }
@@ -1542,9 +1594,16 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_heavy(TokArgs A) {
   const uint32_t per = (n_items + gridDim.x - 1) / gridDim.x;
   const uint32_t q0 = min(blockIdx.x * per, n_items), q1 = min(q0 + per, n_items);
   uint32_t staged = TK_NONE;
-  for (uint32_t q = q0; q < q1; ++q) {
+  for (uint32_t q = q0; q < q1;) {
     __syncthreads();  // the previous item's readers are done with L
     const uint4 it = A.heavy[q];
+    // the following items of the same tile (a tile's items sit together,
+    // in order) join this one, up to TK_HMERGE: one visit pass over their
+    // contiguous visits instead of one each -- a pass's fixed cost (the
+    // lanes' bisection into the runs, barriers) outweighed 4 visits a lane
+    // at Jaccard 0.6, where the second pass takes ~116K items
+    const uint32_t n_items_t = (it.z - TOK_CHUNK + TOK_HCHUNK - 1) / TOK_HCHUNK;
+    const uint32_t q_end = min(q1, q + min(n_items_t - it.y + 1, TK_HMERGE));
     if (it.x != staged) {
       const uint32_t t0 = it.x * TOK_BLOCK;
       for (uint32_t k = tid; k < TOK_BLOCK; k += TK_TPB) {
@@ -1562,9 +1621,11 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_heavy(TokArgs A) {
       tok_stage<kW>(A, L, it.x, hdr);
       staged = it.x;
     }
-    tok_visits<kW>(A, L, it.x, TOK_CHUNK + (it.y - 1) * TOK_HCHUNK, min(it.z, TOK_CHUNK + it.y * TOK_HCHUNK), C);
+    const uint32_t y_last = it.y + (q_end - q) - 1;
+    tok_visits<kW>(A, L, it.x, TOK_CHUNK + (it.y - 1) * TOK_HCHUNK, min(it.z, TOK_CHUNK + y_last * TOK_HCHUNK), C);
     __syncthreads();
     if (L.qn >= TK_QCAP / 2) tok_flush_queue(A, L);
+    q = q_end;
   }
   __syncthreads();
   tok_flush(A, L, C);
