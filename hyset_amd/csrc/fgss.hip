@@ -1742,7 +1742,10 @@ void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEven
   }
 #undef HS_TOK_FILTER
   HS_CHECK_HIP(hipGetLastError());
-  hipExtLaunchKernelGGL(k_tok_verify, gp, b, 0, st, nullptr, t1, 0, a);
+  // verification: no LDS to speak of, four workgroups a CU; each lane's
+  // candidates are a chain of dependent loads, so more lanes, fewer rounds
+  const dim3 gv(std::max<uint32_t>(num_cus, 1) * 4);
+  hipExtLaunchKernelGGL(k_tok_verify, gv, b, 0, st, nullptr, t1, 0, a);
   HS_CHECK_HIP(hipGetLastError());
 }
 
