@@ -130,7 +130,8 @@ struct DevIndex {
 // CNT_CAND_PROBES: sum of |r| over the distinct probes with candidates (the
 // token-major join, when asked to count it); CNT_INDEX_ERR: nonzero when a
 // sort pass of the packed token-major index gave up waiting for an earlier
-// tile (never in a correct run; the join then fails).
+// tile (never in a correct run; the join then fails); CNT_HCLAIM: runs of
+// second-pass items claimed so far.
 // The first CNT_STATS words hold the counters; CNT_SLOTS slots of
 // CNT_SLOT_WORDS words (a 128-byte line each) follow, into which a
 // workgroup may add its pure statistics (visits, items, signature tests,
@@ -140,7 +141,8 @@ struct DevIndex {
 // ~8 ns each).  fold_counters adds the slots into the first words.
 enum {
   CNT_PAIRS = 0, CNT_VISITS = 1, CNT_CANDS = 2, CNT_CAND_TOKENS = 3, CNT_ITEMS = 4, CNT_ENTRIES = 5,
-  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_CAND_PROBES = 9, CNT_INDEX_ERR = 10, CNT_STATS = 16,
+  CNT_SIG_TESTS = 6, CNT_QUEUE = 7, CNT_HEAVY = 8, CNT_CAND_PROBES = 9, CNT_INDEX_ERR = 10, CNT_HCLAIM = 11,
+  CNT_STATS = 16,
   CNT_SLOTS = 16, CNT_SLOT_WORDS = 16, CNT_WORDS = CNT_STATS + CNT_SLOTS * CNT_SLOT_WORDS
 };
 #ifdef __HIPCC__
