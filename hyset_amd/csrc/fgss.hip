@@ -1013,6 +1013,7 @@ struct TokLds {
   uint32_t tmp[TK_TPB / WAVE + 1];
   uint32_t q_r[TK_QCAP], q_s[TK_QCAP], q_j[TK_QCAP];
   uint32_t qn, qbase, ws, wn, trunc;
+  uint32_t hq;  // the tile's first second-pass item
   uint32_t next_ws;  // the window start of the workgroup's next tile
   uint32_t stats[3];
   uint2 hdr[TK_HDR];  // headers of the workgroup's tiles k0 .. k0 + TK_HDR - 1 (its k-th tile: g + k * grid)
@@ -1562,16 +1563,22 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_filter(TokArgs A) {
       if (total > TOK_CHUNK && t0 + k < t1) A.aoff[t0 + k] = make_uint2(L.a[k], run);
       run += cnt[q];
     }
+    // the tile's other visits, for the second pass: items 1 .. n of
+    // TOK_HCHUNK visits, reserved by one lane and written by all (a tile
+    // of a long list at Jaccard 0.6 has over a hundred)
+    const uint32_t n_heavy = total > TOK_CHUNK ? (total - TOK_CHUNK + TOK_HCHUNK - 1) / TOK_HCHUNK : 0u;
     if (tid == 0) {
       L.off[TOK_BLOCK] = total;
-      if (total > TOK_CHUNK) {  // the tile's other visits, for the second pass
-        const uint32_t n = (total - TOK_CHUNK + TOK_HCHUNK - 1) / TOK_HCHUNK;  // items 1 .. n
-        const unsigned long long q = atomicAdd(&A.out.counters[CNT_HEAVY], (unsigned long long)n);
-        for (uint32_t k = 0; k < n; ++k)
-          if (q + k < A.heavy_cap) A.heavy[q + k] = make_uint4(t, k + 1, total, 0u);
-      }
+      if (n_heavy)
+        L.hq = (uint32_t)min(atomicAdd(&A.out.counters[CNT_HEAVY], (unsigned long long)n_heavy),
+                             (unsigned long long)0xFFFFFFFFu);
     }
     __syncthreads();
+    if (n_heavy) {
+      const uint64_t hq = L.hq;
+      for (uint32_t k = tid; k < n_heavy; k += TK_TPB)
+        if (hq + k < A.heavy_cap) A.heavy[hq + k] = make_uint4(t, k + 1, total, 0u);
+    }
     tok_visits<kW>(A, L, t, 0, min(total, TOK_CHUNK), C);
     __syncthreads();
     if (L.qn >= TK_QCAP / 2) tok_flush_queue(A, L);  // (a fuller queue spills to the global one)
