@@ -286,13 +286,16 @@ def bench_batched(ctx, H, split=True) -> dict:
     c, ms_g = timed(H.Tune(flags=H.TUNE_GRAPH))
     assert c == total
     res["batch_persistent_graph_ms"] = round(ms_g, 2)
-    tune, _ = ctx.autotune_batch(b, reps=1)
-    tune.flags = H.TUNE_GRAPH
+    tune, _ = ctx.autotune_batch(b, reps=3)
+    tune.flags |= H.TUNE_GRAPH  # (the tuned flags kept: lane width, prefix scheme, signature test)
     c, ms_t = timed(tune)
     assert c == total
     res["batch_autotuned_ms"] = round(ms_t, 2)
     res["batch_autotuned_config"] = {"probes_per_tile": tune.probes_per_tile, "tile_tokens": tune.tile_tokens,
-                                     "workgroups_per_cu": tune.workgroups_per_cu}
+                                     "workgroups_per_cu": tune.workgroups_per_cu, "signature_bits": tune.signature_bits,
+                                     "narrow": bool(tune.flags & H.TUNE_NARROW),
+                                     "index_maxprefix": bool(tune.flags & H.TUNE_INDEX_MAXPREFIX),
+                                     "no_signature": bool(tune.flags & H.TUNE_NO_SIGNATURE)}
     best = min(ms, ms_g, ms_t)
     res["batch_speedup_vs_plain"] = round(plain / best, 2)
     # per-join result parity of the batched form against the plain form on a sample of joins

@@ -624,7 +624,7 @@ class Context:
             return int(res.count)
         return self.last_result(int(res.count))
 
-    def autotune_batch(self, batch: "DeviceCollection", reps=2):
+    def autotune_batch(self, batch: "DeviceCollection", reps=3):
         t = Tune()
         ms = C.c_double()
         self._check(_lib.hs_autotune_batch(self._h, batch._h, reps, C.byref(t), C.byref(ms)), "hs_autotune_batch")

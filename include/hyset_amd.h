@@ -361,9 +361,13 @@ int hs_join_batch(hs_ctx* ctx, hs_coll* batch, const hs_tune* tune, hs_result* r
 /* Times hs_join_batch over a grid of threads per probe (workgroup width
  * 512 or 256 lanes over 64-512 probes per tile) and persistent-grid sizes,
  * then over the prefix scheme (midprefix or maxprefix index) and signature
- * widths (64, 128, 256 bits, none) at the best point so far (reps timed runs
- * each after one warm-up) and returns the fastest setting; *best_ms receives
- * its mean device time. */
+ * widths (64, 128, 256 bits, none) at the best point so far, timing each
+ * setting as the median of reps runs after one warm-up (reps <= 0: 3).  The
+ * default settings (all fields 0) start as the best; a setting replaces the
+ * best only when 2% faster, and the final winner is timed against the
+ * defaults again with twice the runs, so noise cannot return a setting
+ * slower than the defaults.  *best_ms receives the winner's median device
+ * time. */
 int hs_autotune_batch(hs_ctx* ctx, hs_coll* batch, int32_t reps, hs_tune* best, double* best_ms);
 
 /* -------------------------------------------------------------- serving */
