@@ -1756,8 +1756,9 @@ void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEven
       (void)hipGetLastError();
       nb = 4;
     }
-    // at most 3 workgroups per CU: the filter measured 88 us at 3 and
-    // 102 us at 4 per CU on the 1M-set index
+    // at most 3 workgroups per CU (an earlier, smaller layout measured 88 us
+    // at 3 and 102 us at 4 on the 1M-set index); the 68 KB layout fits 2,
+    // and a 51 KB one launched 3 a CU took 76 against 56 us
     per_cu = std::min(nb, 3);
   }
   const dim3 gp(std::max<uint32_t>(num_cus, 1) * (uint32_t)per_cu);
