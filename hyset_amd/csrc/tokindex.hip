@@ -73,7 +73,7 @@ __global__ __launch_bounds__(PK_EMIT_TPB) void k_pack_emit(
   for (uint32_t k = threadIdx.x; k < PK_MAX_PASSES * NB; k += PK_EMIT_TPB) h[k] = 0;
   const uint32_t gtid = blockIdx.x * PK_EMIT_TPB + threadIdx.x, stride = gridDim.x * PK_EMIT_TPB;
   for (uint32_t k = gtid; k < n_zero; k += stride) zero[k] = 0;
-  if (gtid < CNT_WORDS) counters[gtid] = 0;
+  for (uint32_t k = gtid; k < CNT_WORDS; k += stride) counters[k] = 0;  // (a grid of one workgroup included)
   __syncthreads();
   for (uint32_t q = gtid; q < nsets; q += stride) {
     const uint32_t i = first + q;
