@@ -481,3 +481,23 @@ def test_stage_times(ctx, small, algo):
         st = ctx.stats()
         assert st["graph"] == 2
         assert st["index_ms"] == 0 and st["kernel_ms"] == 0 and 0 < st["filter_ms"] <= st["total_ms"]
+
+
+def test_token_join_stats_repeat_on_a_tiny_collection(ctx):
+    """The work counters of a repeated token-major join over 250 sets (an
+    index build of one workgroup) come out the same each time: the build
+    clears every counter word, the last statistics slot included, whatever
+    its grid.  The sets are equal, so every pair is a candidate and the
+    verification's workgroups reach that slot."""
+    c = H.SetCollection.from_sets([list(range(1, 9))] * 250)
+    d = ctx.upload(c)
+    ctx.set_threshold(0.8)
+    want = O.self_join(c, 0.8)
+    assert want.count == 250 * 249 // 2
+    seen = []
+    for _ in range(3):
+        _check(ctx.join(d, algo="fgss"), want, "fgss equal sets")
+        st = ctx.stats()
+        seen.append((st["posting_visits"], st["candidates"], st["signature_tests"], st["tile_plan"]))
+    assert seen[0][3] == H.PLAN_TOKEN and seen[0][1] >= want.count, seen[0]
+    assert seen[1] == seen[0] and seen[2] == seen[0], seen
