@@ -1639,6 +1639,13 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_heavy(TokArgs A) {
     const uint32_t n_items_t = (it.z - TOK_CHUNK + TOK_HCHUNK - 1) / TOK_HCHUNK;
     const uint32_t q_end = min(q1, q + min(n_items_t - it.y + 1, TK_HMERGE));
     if (it.x != staged) {
+      // the span's words, the header and the layout load together (none
+      // waits on another), then the signatures of the window's words: two
+      // memory latencies a staging instead of four
+      TokAhead P;
+      tok_ahead_words(A, it.x, P);
+      uint2 hdr = make_uint2(0, 0);
+      if (tid == 0) hdr = A.hdrs[it.x];  // written by k_tok_filter
       const uint32_t t0 = it.x * TOK_BLOCK;
       for (uint32_t k = tid; k < TOK_BLOCK; k += TK_TPB) {
         if (t0 + k < A.n_ent) {
@@ -1649,10 +1656,13 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_heavy(TokArgs A) {
           L.off[k] = it.z;  // past the last entry: no visits
         }
       }
-      if (tid == 0) L.off[TOK_BLOCK] = it.z;
-      uint2 hdr = make_uint2(0, 0);
-      if (tid == 0) hdr = A.hdrs[it.x];  // written by k_tok_filter
-      tok_stage<kW>(A, L, it.x, hdr);
+      if (tid == 0) {
+        L.off[TOK_BLOCK] = it.z;
+        L.next_ws = tok_ws(A, it.x, hdr);
+      }
+      __syncthreads();
+      tok_ahead_sigs<kW>(A, it.x, L.next_ws, P);
+      tok_stage_ahead<kW>(A, L, it.x, hdr, P);
       staged = it.x;
     }
     const uint32_t y_last = it.y + (q_end - q) - 1;
