@@ -727,7 +727,8 @@ uint32_t hs::launch_device_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, D
     }
     a.out = dev_out(ctx);
     tile_plan = HS_PLAN_TOKEN;
-    hs::launch_tok_join(st, a, (uint32_t)ctx->num_cus, timed ? ctx->ev[5] : nullptr, timed ? ctx->ev[2] : nullptr);
+    hs::launch_tok_join(st, a, (uint32_t)ctx->num_cus, timed ? ctx->ev[5] : nullptr, timed ? ctx->ev[2] : nullptr,
+                        timed ? &ctx->ev[6] : nullptr);
   } else if (algo == HS_ALGO_FGSS) {
     hs::FgssArgs a{};
     a.probe = probe->dev();
@@ -1114,7 +1115,13 @@ int hs_join(hs_ctx* ctx, hs_coll* indexed, hs_coll* probe, const hs_range* index
     } else if (ctx->stage_times) {
       ctx->stats.index_ms = elapsed(ctx->ev[0], ctx->ev[1]);
       ctx->stats.filter_ms = elapsed(ctx->ev[1], ctx->ev[2]);
-      ctx->stats.kernel_ms = elapsed(ctx->ev[5], ctx->ev[2]);
+      // the token-major plan's three kernels: the sum of their dispatches'
+      // own durations (the dispatches with events wait ~12 us between
+      // them, which plain joins do not)
+      ctx->stats.kernel_ms = tile_plan == HS_PLAN_TOKEN ? elapsed(ctx->ev[5], ctx->ev[6]) +
+                                                              elapsed(ctx->ev[7], ctx->ev[8]) +
+                                                              elapsed(ctx->ev[9], ctx->ev[2])
+                                                        : elapsed(ctx->ev[5], ctx->ev[2]);
     }
     ctx->stage_times = false;
     ctx->stats.pairs = count;

@@ -129,7 +129,10 @@ struct hs_ctx {
   uint64_t out_cap = 0;
   PinnedBuf h_counters, h_cum;
   DevBuf d_cum;
-  hipEvent_t ev[6] = {};
+  // ev[0..5]: join start, index end, kernel end, counters back, -, kernel
+  // start; ev[6..9]: the token-major plan's filter end, second pass start
+  // and end, verification start (HS_TUNE_STAGE_TIMES)
+  hipEvent_t ev[10] = {};
   hs_stats stats{};
   uint64_t last_count = 0;  // pairs of the last result: on the device, then host_tail
   std::vector<hs::Triples> host_tail;  // pairs the host joined (co-processing), after the device's

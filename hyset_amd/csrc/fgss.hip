@@ -1752,7 +1752,8 @@ __global__ __launch_bounds__(TK_TPB) void k_tok_verify(TokArgs A) {
   }
 }
 
-void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEvent_t t0, hipEvent_t t1) {
+void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEvent_t t0, hipEvent_t t1,
+                     const hipEvent_t* mid) {
   HS_REQUIRE(a.sig_words == 0 || a.sigs, HS_E_INVALID, "token-major join: signature table missing");
   const dim3 b(TK_TPB);
   const uint32_t n_tiles = std::max<uint32_t>((a.n_ent + TOK_BLOCK - 1) / TOK_BLOCK, 1);
@@ -1773,9 +1774,11 @@ void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEven
   }
   const dim3 gp(std::max<uint32_t>(num_cus, 1) * (uint32_t)per_cu);
   const dim3 g(std::min<uint32_t>(n_tiles, gp.x));
-#define HS_TOK_FILTER(W)                                                           \
-  hipExtLaunchKernelGGL(k_tok_filter<W>, g, b, 0, st, t0, nullptr, 0, a);          \
-  hipExtLaunchKernelGGL(k_tok_heavy<W>, gp, b, 0, st, nullptr, nullptr, 0, a);
+  const hipEvent_t f1 = mid ? mid[0] : nullptr, h0 = mid ? mid[1] : nullptr, h1 = mid ? mid[2] : nullptr,
+                   v0 = mid ? mid[3] : nullptr;
+#define HS_TOK_FILTER(W)                                                  \
+  hipExtLaunchKernelGGL(k_tok_filter<W>, g, b, 0, st, t0, f1, 0, a);      \
+  hipExtLaunchKernelGGL(k_tok_heavy<W>, gp, b, 0, st, h0, h1, 0, a);
   switch (a.sig_words) {
     case 0: HS_TOK_FILTER(0) break;
     case 1: HS_TOK_FILTER(1) break;
@@ -1790,7 +1793,7 @@ void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEven
   // verification: no LDS to speak of, four workgroups a CU; each lane's
   // candidates are a chain of dependent loads, so more lanes, fewer rounds
   const dim3 gv(std::max<uint32_t>(num_cus, 1) * 4);
-  hipExtLaunchKernelGGL(k_tok_verify, gv, b, 0, st, nullptr, t1, 0, a);
+  hipExtLaunchKernelGGL(k_tok_verify, gv, b, 0, st, v0, t1, 0, a);
   HS_CHECK_HIP(hipGetLastError());
 }
 

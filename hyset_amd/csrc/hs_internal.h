@@ -405,8 +405,11 @@ struct TokArgs {
 // Filter, then verify the queue (a persistent grid of num_cus * 8
 // workgroups); t0 / t1 (may be NULL) take the filter's start and the
 // verify's end.
+// t0 / t1 (may be NULL) take the filter's start and the verification's
+// end; mid (may be NULL) four events: the filter's end, the second pass's
+// start and end, the verification's start, each taken by its dispatch.
 void launch_tok_join(hipStream_t st, const TokArgs& a, uint32_t num_cus, hipEvent_t t0 = nullptr,
-                     hipEvent_t t1 = nullptr);
+                     hipEvent_t t1 = nullptr, const hipEvent_t* mid = nullptr);
 
 struct BitmapArgs {
   DevColl probe;

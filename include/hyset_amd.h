@@ -274,7 +274,8 @@ typedef struct {
   double total_ms;
   double host_filter_ms; /* hybrid: CPU filtering wall time */
   double kernel_ms;      /* the main filter/verify kernel alone (fgss token-major plan: its filter,
-                            second pass and verification kernels, first start to last end) */
+                            second pass and verification kernels, the sum of the three
+                            dispatches' durations) */
   uint64_t index_entries;
   uint64_t posting_visits;   /* fgss: postings inside length windows; bitmap: pairs tested */
   uint64_t candidates;       /* distinct pairs verified; bitmap: filter survivors */
